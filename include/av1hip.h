@@ -1,0 +1,408 @@
+/*
+ * av1hip.h - the C ABI of the MI355X (gfx950) AV1 reconstruction backend.
+ *
+ * This is the drop-in boundary. A caller hands over the decoded *syntax* of one
+ * frame - prediction blocks, transform blocks, dequantised coefficients,
+ * deblocking edges, CDEF blocks, loop restoration units, film grain
+ * parameters - as flat arrays of plain records, and the library reconstructs
+ * the picture on the GPU and keeps it resident in HBM as a reference slot.
+ *
+ * The records mirror the reference's public syntax types one field for one
+ * field (include/av1gpu/av1gpu.hpp:187-537 in the reference), with the nested
+ * std::vectors of BlockSyntax (transforms, OBMC strips) turned into index
+ * ranges into frame-wide arrays, so any FFI (ctypes, cgo, JNI, N-API) can fill
+ * them without C++.
+ *
+ * The entry points mirror av1gpu::Decoder (reference
+ * include/av1gpu/av1gpu.hpp:615-787): decode_reference / decode /
+ * decode_reference_reserved -> av1h_decode, release_reference ->
+ * av1h_release_reference, reserve_reference -> av1h_reserve_reference, to_nv12
+ * -> av1h_to_nv12, to_rgb -> av1h_to_rgb, reset -> av1h_reset, stats ->
+ * av1h_get_stats. Every call returns an int status (AV1H_OK or a negative
+ * code) and leaves a message in av1h_last_error(); nothing throws.
+ *
+ * All integers are little-endian two's complement; every struct is made of
+ * int32_t fields only, so its layout is the same for every compiler.
+ */
+#ifndef AV1HIP_H
+#define AV1HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <string.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+
+#define AV1H_OK 0
+#define AV1H_ERR_INVALID -1      /* bad argument or malformed syntax          */
+#define AV1H_ERR_UNSUPPORTED -2  /* a tool this build does not reconstruct    */
+#define AV1H_ERR_NO_DEVICE -3    /* no usable gfx950 device                    */
+#define AV1H_ERR_DEVICE -4       /* a HIP call failed                          */
+#define AV1H_ERR_SLOTS -5        /* the reference pool is exhausted            */
+#define AV1H_ERR_MEMORY -6       /* an allocation (host or HBM) failed         */
+
+/* ------------------------------------------------------------------ syntax */
+/* Numbering follows the reference enums (av1gpu.hpp:117-183), which are the
+ * specification's own numbers. */
+
+enum {
+  AV1H_PRED_DC = 0, AV1H_PRED_V = 1, AV1H_PRED_H = 2, AV1H_PRED_D45 = 3,
+  AV1H_PRED_D135 = 4, AV1H_PRED_D113 = 5, AV1H_PRED_D157 = 6, AV1H_PRED_D203 = 7,
+  AV1H_PRED_D67 = 8, AV1H_PRED_SMOOTH = 9, AV1H_PRED_SMOOTH_V = 10,
+  AV1H_PRED_SMOOTH_H = 11, AV1H_PRED_PAETH = 12, AV1H_PRED_CFL = 13
+};
+enum {
+  AV1H_FILT_REGULAR = 0, AV1H_FILT_SMOOTH = 1, AV1H_FILT_SHARP = 2, AV1H_FILT_BILINEAR = 3
+};
+enum {
+  AV1H_COMPOUND_NONE = 0, AV1H_COMPOUND_AVERAGE = 1, AV1H_COMPOUND_DISTANCE = 2,
+  AV1H_COMPOUND_WEDGE = 3, AV1H_COMPOUND_DIFFWTD = 4
+};
+enum { AV1H_RESTORE_NONE = 0, AV1H_RESTORE_SWITCHABLE = 1, AV1H_RESTORE_WIENER = 2,
+       AV1H_RESTORE_SGRPROJ = 3 };
+
+/* TransformBlock (reference av1gpu.hpp:187-208). */
+typedef struct av1h_transform {
+  int32_t x, y, w, h;        /* position inside the prediction block, size      */
+  int32_t coef_offset;       /* raster start in the coefficient (or residual) pool */
+  int32_t eob;               /* 0: the residual is zero                          */
+  int32_t tx_type;           /* 0..15, Tx_Type                                   */
+  int32_t level_offset, qindex, dc_step, ac_step, denominator, qm_off;
+} av1h_transform;
+
+/* ObmcCandidate (reference av1gpu.hpp:215-225). */
+typedef struct av1h_obmc {
+  int32_t pass, x, y, w, h;
+  int32_t mv_x, mv_y;
+  int32_t filter, filter_v;
+  int32_t ref;
+} av1h_obmc;
+
+/* BlockSyntax (reference av1gpu.hpp:229-342); `transforms` and `obmc` are the
+ * ranges [tx_first, tx_first + tx_count) and [obmc_first, obmc_first +
+ * obmc_count) of the frame's transform and OBMC arrays. */
+typedef struct av1h_block {
+  int32_t x, y, w, h, plane, bit_depth;
+  int32_t is_intra, intrabc;
+  int32_t mode, have_left, have_above, have_above_right, have_below_left;
+  int32_t angle_delta, filter_type, use_filter_intra, filter_intra_mode, cfl_alpha;
+  int32_t lossless;
+  int32_t use_palette, palette_size, palette[8], palette_off, palette_stride;
+  int32_t ref[2];
+  int32_t mv[2][2];
+  int32_t filter, filter_v, motion_mode, compound, blend[2], wedge_index, mask_sign;
+  int32_t interintra, interintra_mode, interintra_wedge;
+  int32_t warp[2];
+  int32_t warp_params[2][6];
+  int32_t mask_warp[2];
+  int32_t residual_provided;
+  int32_t tx_first, tx_count, obmc_first, obmc_count;
+} av1h_block;
+
+/* DeblockEdge (reference av1gpu.hpp:347-358). */
+typedef struct av1h_deblock_edge {
+  int32_t plane, x, y, vertical, length, level, filter_size;
+} av1h_deblock_edge;
+
+/* CdefBlock (reference av1gpu.hpp:361-369). */
+typedef struct av1h_cdef_block {
+  int32_t plane, x, y, pri_strength, sec_strength, damping, dir;
+} av1h_cdef_block;
+
+/* RestorationUnit (reference av1gpu.hpp:418-426). `type` is a RestorationType
+ * value, the way the reference planner reads it (engine.cpp:1425-1448):
+ * AV1H_RESTORE_NONE skips the unit, AV1H_RESTORE_WIENER is the Wiener filter and
+ * any other value the self guided filter. */
+typedef struct av1h_lr_unit {
+  int32_t type;
+  int32_t wiener[2][3];
+  int32_t sgr_set;
+  int32_t sgr_xqd[2];
+} av1h_lr_unit;
+
+/* FilmGrainParams (reference av1gpu.hpp:434-468). */
+typedef struct av1h_grain {
+  int32_t apply, seed;
+  int32_t num_y_points, point_y_value[16], point_y_scaling[16];
+  int32_t chroma_scaling_from_luma;
+  int32_t num_cb_points, point_cb_value[16], point_cb_scaling[16];
+  int32_t num_cr_points, point_cr_value[16], point_cr_scaling[16];
+  int32_t grain_scaling_minus_8, ar_coeff_lag;
+  int32_t ar_coeffs_y[24], ar_coeffs_cb[25], ar_coeffs_cr[25];
+  int32_t ar_coeff_shift_minus_6, grain_scale_shift;
+  int32_t cb_mult, cb_luma_mult, cb_offset, cr_mult, cr_luma_mult, cr_offset;
+  int32_t overlap_flag, clip_to_restricted_range;
+} av1h_grain;
+
+#define AV1H_MAX_REFS 8
+
+/* The scalar part of FrameSyntax (reference av1gpu.hpp:476-537). */
+typedef struct av1h_frame_header {
+  int32_t width, coded_width, height, bit_depth, ss_x, ss_y;
+  int32_t base_q_idx, sharpness, enable_intra_edge_filter;
+  int32_t num_reference_slots;
+  int32_t reference_slots[AV1H_MAX_REFS];
+  int32_t reference_width[7], reference_height[7];
+  int32_t color_matrix, color_range;
+  int32_t restoration_type[3], restoration_unit_log2[3], restoration_unit_cols[3];
+  av1h_grain grain;
+} av1h_frame_header;
+
+/* One frame's syntax: the header plus borrowed pointers to the pools. The
+ * library copies what it needs before av1h_decode returns. */
+typedef struct av1h_frame_desc {
+  av1h_frame_header hdr;
+  const av1h_block* blocks;
+  const av1h_transform* transforms;
+  const av1h_obmc* obmc;
+  const int32_t* coefficients;
+  const int32_t* residual;
+  const int16_t* levels;
+  const uint8_t* qm_pool;
+  const uint8_t* palette_map;
+  const av1h_deblock_edge* deblock;
+  const av1h_cdef_block* cdef;
+  const av1h_lr_unit* restoration[3];
+  int64_t num_blocks, num_transforms, num_obmc, num_coefficients, num_residual;
+  int64_t num_levels, num_qm, num_palette, num_deblock, num_cdef;
+  int64_t num_restoration[3];
+} av1h_frame_desc;
+
+/* ------------------------------------------------------------- wire format */
+/*
+ * A frame's syntax as one byte string, for fixtures and replay: the magic
+ * "AV1WSYN1", a uint32 version (1), a uint32 byte size of av1h_frame_header, the
+ * header, then thirteen int64 counts (blocks, transforms, obmc, coefficients,
+ * residual, levels, qm, palette, deblock, cdef, lr0, lr1, lr2) and the thirteen
+ * arrays in that order, each starting on an 8 byte boundary. Parsing only
+ * points into the buffer; it copies nothing.
+ */
+#define AV1H_WIRE_MAGIC "AV1WSYN1"
+#define AV1H_WIRE_VERSION 1u
+
+static inline size_t av1h__pad8(size_t v) { return (v + 7u) & ~(size_t)7u; }
+
+static inline size_t av1h__wire_elem(int i) {
+  static const size_t sizes[13] = {sizeof(av1h_block), sizeof(av1h_transform),
+                                   sizeof(av1h_obmc), 4, 4, 2, 1, 1,
+                                   sizeof(av1h_deblock_edge), sizeof(av1h_cdef_block),
+                                   sizeof(av1h_lr_unit), sizeof(av1h_lr_unit),
+                                   sizeof(av1h_lr_unit)};
+  return sizes[i];
+}
+
+static inline void av1h__wire_counts(const av1h_frame_desc* d, int64_t n[13]) {
+  n[0] = d->num_blocks; n[1] = d->num_transforms; n[2] = d->num_obmc;
+  n[3] = d->num_coefficients; n[4] = d->num_residual; n[5] = d->num_levels;
+  n[6] = d->num_qm; n[7] = d->num_palette; n[8] = d->num_deblock; n[9] = d->num_cdef;
+  n[10] = d->num_restoration[0]; n[11] = d->num_restoration[1];
+  n[12] = d->num_restoration[2];
+}
+
+static inline size_t av1h_wire_size(const av1h_frame_desc* d) {
+  int64_t n[13];
+  size_t size = av1h__pad8(16 + sizeof(av1h_frame_header)) + 13 * 8;
+  int i;
+  av1h__wire_counts(d, n);
+  for (i = 0; i < 13; ++i) size += av1h__pad8((size_t)n[i] * av1h__wire_elem(i));
+  return size;
+}
+
+/* Writes the wire form of `d` into `out` (av1h_wire_size bytes); returns the
+ * number of bytes written. */
+static inline size_t av1h_wire_write(const av1h_frame_desc* d, uint8_t* out) {
+  const void* arrays[13];
+  int64_t n[13];
+  const uint32_t version = AV1H_WIRE_VERSION, hsize = (uint32_t)sizeof(av1h_frame_header);
+  size_t pos = 0;
+  int i;
+  av1h__wire_counts(d, n);
+  arrays[0] = d->blocks; arrays[1] = d->transforms; arrays[2] = d->obmc;
+  arrays[3] = d->coefficients; arrays[4] = d->residual; arrays[5] = d->levels;
+  arrays[6] = d->qm_pool; arrays[7] = d->palette_map; arrays[8] = d->deblock;
+  arrays[9] = d->cdef; arrays[10] = d->restoration[0]; arrays[11] = d->restoration[1];
+  arrays[12] = d->restoration[2];
+  memcpy(out, AV1H_WIRE_MAGIC, 8);
+  memcpy(out + 8, &version, 4);
+  memcpy(out + 12, &hsize, 4);
+  memcpy(out + 16, &d->hdr, sizeof(av1h_frame_header));
+  pos = av1h__pad8(16 + sizeof(av1h_frame_header));
+  for (i = 0; i < 13; ++i) {
+    memcpy(out + pos, &n[i], 8);
+    pos += 8;
+  }
+  for (i = 0; i < 13; ++i) {
+    const size_t bytes = (size_t)n[i] * av1h__wire_elem(i);
+    if (bytes) memcpy(out + pos, arrays[i], bytes);
+    if (av1h__pad8(bytes) > bytes) memset(out + pos + bytes, 0, av1h__pad8(bytes) - bytes);
+    pos += av1h__pad8(bytes);
+  }
+  return pos;
+}
+
+/* Points `d` into `buf` (which must stay alive and be 8 byte aligned). Returns
+ * AV1H_OK, or AV1H_ERR_INVALID when the buffer is not a complete version 1
+ * wire frame. */
+static inline int av1h_wire_parse(const uint8_t* buf, size_t len, av1h_frame_desc* d) {
+  uint32_t version = 0, hsize = 0;
+  int64_t n[13];
+  const void* arrays[13];
+  size_t pos;
+  int i;
+  if (!buf || !d || len < 16 || memcmp(buf, AV1H_WIRE_MAGIC, 8) != 0) return AV1H_ERR_INVALID;
+  memcpy(&version, buf + 8, 4);
+  memcpy(&hsize, buf + 12, 4);
+  if (version != AV1H_WIRE_VERSION || hsize != sizeof(av1h_frame_header)) return AV1H_ERR_INVALID;
+  pos = av1h__pad8(16 + sizeof(av1h_frame_header));
+  if (len < pos + 13 * 8) return AV1H_ERR_INVALID;
+  memset(d, 0, sizeof(*d));
+  memcpy(&d->hdr, buf + 16, sizeof(av1h_frame_header));
+  for (i = 0; i < 13; ++i) {
+    memcpy(&n[i], buf + pos, 8);
+    if (n[i] < 0 || n[i] > ((int64_t)1 << 40)) return AV1H_ERR_INVALID;
+    pos += 8;
+  }
+  for (i = 0; i < 13; ++i) {
+    const size_t bytes = (size_t)n[i] * av1h__wire_elem(i);
+    if (len < pos + bytes) return AV1H_ERR_INVALID;
+    arrays[i] = bytes ? (const void*)(buf + pos) : (const void*)0;
+    pos += av1h__pad8(bytes);
+  }
+  d->blocks = (const av1h_block*)arrays[0];
+  d->transforms = (const av1h_transform*)arrays[1];
+  d->obmc = (const av1h_obmc*)arrays[2];
+  d->coefficients = (const int32_t*)arrays[3];
+  d->residual = (const int32_t*)arrays[4];
+  d->levels = (const int16_t*)arrays[5];
+  d->qm_pool = (const uint8_t*)arrays[6];
+  d->palette_map = (const uint8_t*)arrays[7];
+  d->deblock = (const av1h_deblock_edge*)arrays[8];
+  d->cdef = (const av1h_cdef_block*)arrays[9];
+  d->restoration[0] = (const av1h_lr_unit*)arrays[10];
+  d->restoration[1] = (const av1h_lr_unit*)arrays[11];
+  d->restoration[2] = (const av1h_lr_unit*)arrays[12];
+  d->num_blocks = n[0]; d->num_transforms = n[1]; d->num_obmc = n[2];
+  d->num_coefficients = n[3]; d->num_residual = n[4]; d->num_levels = n[5];
+  d->num_qm = n[6]; d->num_palette = n[7]; d->num_deblock = n[8]; d->num_cdef = n[9];
+  d->num_restoration[0] = n[10]; d->num_restoration[1] = n[11];
+  d->num_restoration[2] = n[12];
+  return AV1H_OK;
+}
+
+/* Byte offset of hdr.reference_slots inside a wire buffer: a driver that
+ * replays a clip patches the slots a frame names there, because slot numbers
+ * are handed out by the decoder at run time. */
+#define AV1H_WIRE_REFSLOTS_OFFSET (16 + offsetof(av1h_frame_header, reference_slots))
+#define AV1H_WIRE_NUMREFS_OFFSET (16 + offsetof(av1h_frame_header, num_reference_slots))
+
+/* ---------------------------------------------------------------- decoding */
+
+typedef struct av1h_ctx av1h_ctx;
+
+/* DeviceInfo (reference av1gpu.hpp:556-566). */
+typedef struct av1h_device_info {
+  char name[128];
+  char arch[64];
+  int32_t compute_units;
+  int32_t l2_bytes;
+  int64_t global_mem;
+  int32_t clock_khz;
+  int32_t mem_clock_khz;
+  int32_t mem_bus_width;
+} av1h_device_info;
+
+/* DecodeStats (reference av1gpu.hpp:544-552), plus the device time of the two
+ * halves measured with HIP events on the decode stream. */
+typedef struct av1h_stats {
+  int32_t blocks, waves, kernel_launches;
+  double reconstruct_ms, filter_ms, readback_ms, total_ms;
+  double device_recon_ms, device_filter_ms;
+} av1h_stats;
+
+/* Geometry of a decoded picture (what Frame::allocate needs). */
+typedef struct av1h_frame_info {
+  int32_t width, height, bit_depth, ss_x, ss_y;
+} av1h_frame_info;
+
+/* The output planes of a read back: three planes of 16 bit samples, row pitch
+ * in samples. Rows of the display picture are written, nothing else. */
+typedef struct av1h_planes {
+  uint16_t* plane[3];
+  int32_t stride[3];
+} av1h_planes;
+
+/* Creates a decoder on HIP device `device_index` (its reference pool, streams
+ * and work buffers). Replaces Decoder::create with BackendKind::HIP
+ * (reference src/core/engine.cpp:1509-1558). */
+int av1h_create(int device_index, av1h_ctx** out);
+void av1h_destroy(av1h_ctx* ctx);
+const char* av1h_last_error(const av1h_ctx* ctx);
+int av1h_get_device_info(av1h_ctx* ctx, av1h_device_info* out);
+
+/* Decodes one frame (reference Decoder::decode_reference, av1gpu.hpp:635;
+ * Decoder::decode, 733; Decoder::decode_reference_reserved, 645).
+ *   keep_reference 0: decode and keep no slot (`*slot_out` = -1);
+ *   keep_reference 1: store the pre-grain picture in a reference slot - the
+ *                     reserved one when `reserved_slot` >= 0, else the lowest
+ *                     free one - and return it in `*slot_out`.
+ * `out` (nullable) receives the post-grain display picture; a null `out`
+ * leaves it on the device. `info` (nullable) receives the geometry. The call
+ * returns once the picture is complete on the device. */
+int av1h_decode(av1h_ctx* ctx, const av1h_frame_desc* frame, int keep_reference,
+                int reserved_slot, const av1h_planes* out, av1h_frame_info* info,
+                int* slot_out);
+
+/* Reserves a reference slot (reference Decoder::reserve_reference,
+ * av1gpu.hpp:661). Returns the slot or a negative status. */
+int av1h_reserve_reference(av1h_ctx* ctx);
+
+/* Releases a slot (reference Decoder::release_reference, av1gpu.hpp:736).
+ * Unknown and already released slots are ignored. */
+void av1h_release_reference(av1h_ctx* ctx, int slot);
+
+/* Bytes an NV12 picture of this geometry and pitch takes, or 0 when it cannot
+ * be packed (not 4:2:0, odd pitch, pitch narrower than a chroma row): the rule
+ * of the reference's nv12_buffer_size (src/core/factory.cpp:236-243). */
+size_t av1h_nv12_size(int width, int height, int ss_x, int ss_y, size_t pitch);
+
+/* Packs a slot's picture into NV12 on the device and copies it to `out`
+ * (reference Decoder::to_nv12, av1gpu.hpp:745). The most recently decoded
+ * slot packs the post-grain picture, an older slot its pre-grain reference
+ * (reference src/backends/cpu/cpu_backend.cpp:557-587). `slot` -1 packs the
+ * last decoded picture. */
+int av1h_to_nv12(av1h_ctx* ctx, int slot, size_t pitch, uint8_t* out, size_t out_bytes);
+
+/* YUV -> RGB24 of a slot (reference Decoder::to_rgb, av1gpu.hpp:739): `matrix`
+ * 0 BT.601, 1 BT.709, 2 BT.2020, 3 identity; `range` 0 limited, 1 full. */
+int av1h_to_rgb(av1h_ctx* ctx, int slot, int matrix, int range, uint8_t* out,
+                size_t out_bytes);
+
+/* Copies a slot's picture (same slot rule as av1h_to_nv12) to the host. */
+int av1h_read_slot(av1h_ctx* ctx, int slot, const av1h_planes* out, av1h_frame_info* info);
+
+/* Drops every slot and clears the pool (reference Decoder::reset). */
+void av1h_reset(av1h_ctx* ctx);
+
+int av1h_get_stats(const av1h_ctx* ctx, av1h_stats* out);
+
+/* --------------------------------------------------------------- batching */
+/*
+ * Multi-stream batching: frames of several independent streams (each with its
+ * own context) decoded together, one launch per stage group for the whole
+ * batch. `frames[i]` is decoded by `ctxs[i]` exactly as av1h_decode would
+ * (keep_reference 1, no read back); slots land in `slots_out[i]`. All contexts
+ * must live on the same device. Each stream's result is bit identical to
+ * decoding it alone.
+ */
+int av1h_decode_batch(av1h_ctx* const* ctxs, const av1h_frame_desc* const* frames, int count,
+                      int* slots_out);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* AV1HIP_H */
