@@ -1,0 +1,1174 @@
+/*
+ * libav1hip.so host side: decoder contexts, frame plans and the C ABI of
+ * include/av1hip.h.
+ *
+ * A context owns, in HBM: the frame being decoded, the snapshots the filter
+ * stages read (the CDEF source, the pre-CDEF picture and its upscaled copy,
+ * the CDEF output), the 32 slot reference pool, and one epoch word per
+ * dependency cell. A frame plan is the syntax flattened into compact
+ * descriptors in one device buffer (the reference's FramePlan,
+ * src/core/backend.hpp:140-206, without the host wave schedule: dependencies
+ * resolve on the device). Decoding runs the stage order of the reference CPU
+ * backend (src/backends/cpu/cpu_backend.cpp:149-553) on the context's stream.
+ */
+#include <algorithm>
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#define AV1T_TABLE_QUAL static const
+#include "av1hip_internal.hpp"
+#include "spec_tables.h"
+
+using namespace av1hip;
+
+namespace av1hip {
+
+void make_layout(Layout* g, int width, int height, int bd, int ss_x, int ss_y, int coded_width) {
+  std::memset(g, 0, sizeof(*g));
+  if (coded_width <= 0 || coded_width > width) coded_width = width;
+  g->width = width;
+  g->height = height;
+  g->bit_depth = bd;
+  g->ss_x = ss_x;
+  g->ss_y = ss_y;
+  g->out_w[0] = width;
+  g->out_h[0] = height;
+  g->out_w[1] = g->out_w[2] = (width + (1 << ss_x) - 1) >> ss_x;
+  g->out_h[1] = g->out_h[2] = (height + (1 << ss_y) - 1) >> ss_y;
+  g->plane_w[0] = coded_width;
+  g->plane_h[0] = height;
+  g->plane_w[1] = g->plane_w[2] = (coded_width + (1 << ss_x) - 1) >> ss_x;
+  g->plane_h[1] = g->plane_h[2] = g->out_h[1];
+  int64_t size = 0, cells = 0;
+  for (int p = 0; p < 3; ++p) {
+    const int sx = p ? ss_x : 0, sy = p ? ss_y : 0;
+    g->grid_w[p] = ((coded_width + 7) & ~7) >> sx;
+    g->grid_h[p] = ((height + 7) & ~7) >> sy;
+    const int pad = p == 0 ? 64 : (64 >> (ss_x ? 1 : 0));
+    g->stride[p] = (g->out_w[p] + pad + 127) & ~127;
+    g->plane_off[p] = size;
+    size += (int64_t)g->stride[p] * (g->out_h[p] + pad);
+    /* dependency cells: 4 luma samples, 2 for a subsampled chroma axis (the
+     * reference's owner grid, engine.cpp:739-750) */
+    g->cell_shift[p] = p == 0 ? 2 : (ss_x ? 1 : 2);
+    g->cells_w[p] = (g->grid_w[p] + (1 << g->cell_shift[p]) - 1) >> g->cell_shift[p];
+    g->cells_h[p] = (g->grid_h[p] + (1 << g->cell_shift[p]) - 1) >> g->cell_shift[p];
+    g->cell_off[p] = cells;
+    cells += (int64_t)g->cells_w[p] * g->cells_h[p];
+  }
+  g->size = size;
+  g->cells = cells;
+}
+
+}  // namespace av1hip
+
+namespace {
+
+double now_ms() {
+  using clk = std::chrono::steady_clock;
+  static const clk::time_point t0 = clk::now();
+  return std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+}
+
+size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+struct Plan {
+  Layout lay;
+  av1h_frame_header hdr;
+  FramePlanDev dev;
+  DevGrain grain;
+  void* dbuf = nullptr;
+  size_t dbytes = 0;
+  std::vector<uint8_t> hbuf;
+  bool uses_lr = false, upscaled = false, has_grain = false;
+  int blocks = 0;
+  std::vector<int> slots; /* pool slots the frame reads */
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipDeviceProp_t prop;
+  int cus = 0;
+  std::string err;
+  std::mutex mu;
+  Layout lay;
+  bool have_layout = false;
+  size_t cap = 0; /* samples per buffer */
+  uint16_t *frame = nullptr, *scratch = nullptr, *lr_curr = nullptr, *lr_cdef = nullptr, *lr_prev_up = nullptr;
+  uint16_t* pool = nullptr;
+  uint32_t* cells = nullptr;
+  int64_t cells_cap = 0;
+  uint32_t epoch = 0;
+  uint32_t* counters = nullptr; /* [0] queue counter, [1] error word */
+  int16_t* grain_tmpl = nullptr;
+  uint8_t* grain_offs = nullptr;
+  size_t grain_offs_cap = 0;
+  uint8_t* pack = nullptr;
+  size_t pack_cap = 0;
+  void* host_pin = nullptr;
+  size_t host_pin_cap = 0;
+  bool used[kSlots] = {};
+  Layout slot_lay[kSlots];
+  int last_slot = -1;
+  bool have_frame = false;
+  Plan scratch_plan;
+  av1h_stats stats;
+  hipEvent_t ev[4];
+};
+
+int set_err(Ctx* c, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  c->err = buf;
+  return code;
+}
+
+#define HIP_OK(c, call)                                                                     \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return set_err((c), AV1H_ERR_DEVICE, "%s: %s", #call, hipGetErrorString(e_));         \
+  } while (0)
+
+int tx_row_shift(int w, int h) {
+  /* Transform_Row_Shift (reference src/core/tables.cpp:357-369) */
+  static const int t[19][3] = {{4, 4, 0},   {8, 8, 1},   {16, 16, 2}, {32, 32, 2}, {64, 64, 2},
+                               {4, 8, 0},   {8, 4, 0},   {8, 16, 1},  {16, 8, 1},  {16, 32, 1},
+                               {32, 16, 1}, {32, 64, 1}, {64, 32, 1}, {4, 16, 1},  {16, 4, 1},
+                               {8, 32, 2},  {32, 8, 2},  {16, 64, 2}, {64, 16, 2}};
+  for (auto& e : t)
+    if (e[0] == w && e[1] == h) return e[2];
+  return -1;
+}
+
+bool pow2_in(int v, int lo, int hi) { return v >= lo && v <= hi && (v & (v - 1)) == 0; }
+
+int kind_ok(int kind, int n) { return kind == 0 ? n <= 64 : (kind == 1 ? n <= 16 : n <= 32); }
+int row_kind(int t) { return (0x66295450u >> (2 * t)) & 3; }
+int col_kind(int t) { return (0x99895144u >> (2 * t)) & 3; }
+
+const int kModeAngle[13] = {0, 90, 180, 45, 135, 113, 157, 203, 67, 0, 0, 0, 0};
+
+/* scaling_lookup_initialization (reference av1gpu_kernels.h:3157-3188) */
+void scaling_lut(const av1h_grain& p, int plane, int* lut) {
+  const int *val, *scl;
+  int n;
+  if (plane == 0 || p.chroma_scaling_from_luma) {
+    n = p.num_y_points;
+    val = p.point_y_value;
+    scl = p.point_y_scaling;
+  } else if (plane == 1) {
+    n = p.num_cb_points;
+    val = p.point_cb_value;
+    scl = p.point_cb_scaling;
+  } else {
+    n = p.num_cr_points;
+    val = p.point_cr_value;
+    scl = p.point_cr_scaling;
+  }
+  if (n <= 0) {
+    std::memset(lut, 0, 256 * sizeof(int));
+    return;
+  }
+  for (int x = 0; x < val[0]; ++x) lut[x] = scl[0];
+  for (int i = 0; i + 1 < n; ++i) {
+    const int dy = scl[i + 1] - scl[i], dx = val[i + 1] - val[i];
+    const int delta = dy * ((65536 + (dx >> 1)) / dx);
+    for (int x = 0; x < dx; ++x) lut[val[i] + x] = scl[i] + ((x * delta + 32768) >> 16);
+  }
+  for (int x = val[n - 1]; x < 256; ++x) lut[x] = scl[n - 1];
+}
+
+/* ---- flattening ----------------------------------------------------------- */
+
+struct Host {
+  std::vector<DevBlock> blocks;
+  std::vector<DevTx> tx;
+  std::vector<int32_t> inter_list, queue_list;
+  std::vector<int32_t> coef, residual;
+  std::vector<uint8_t> palette;
+  std::vector<DevDeblock> deblock;
+  std::vector<DevCdef> cdef;
+  std::vector<DevLr> lr;
+  DevGrain grain;
+};
+
+/* BlockDecoded availability for blocks whose syntax asks to derive it
+ * (reference engine.cpp:779-832): every in-grid cell of the rectangle belongs
+ * to a block earlier in coding order. */
+struct Owner {
+  const Layout* L;
+  std::vector<int> grid[3];
+  int shift[3], cw[3], ch[3];
+  void build(const Layout& lay, const av1h_frame_desc& f) {
+    L = &lay;
+    for (int p = 0; p < 3; ++p) {
+      shift[p] = lay.cell_shift[p];
+      cw[p] = lay.cells_w[p];
+      ch[p] = lay.cells_h[p];
+      grid[p].assign((size_t)cw[p] * ch[p], -1);
+    }
+    for (int64_t i = 0; i < f.num_blocks; ++i) {
+      const av1h_block& b = f.blocks[i];
+      const int p = b.plane, oc = 1 << shift[p];
+      const int cx1 = std::min((b.x + b.w + oc - 1) / oc, cw[p]);
+      const int cy1 = std::min((b.y + b.h + oc - 1) / oc, ch[p]);
+      for (int cy = std::max(b.y, 0) / oc; cy < cy1; ++cy)
+        for (int cx = std::max(b.x, 0) / oc; cx < cx1; ++cx) grid[p][(size_t)cy * cw[p] + cx] = (int)i;
+    }
+  }
+  int at(int p, int x, int y) const {
+    if (x < 0 || y < 0 || x >= L->grid_w[p] || y >= L->grid_h[p]) return -1;
+    return grid[p][(size_t)(y >> shift[p]) * cw[p] + (x >> shift[p])];
+  }
+  bool decoded(int p, int x0, int x1, int y0, int y1, int bi) const {
+    const int oc = 1 << shift[p];
+    for (int y = y0 & ~(oc - 1); y <= y1; y += oc)
+      for (int x = x0 & ~(oc - 1); x <= x1; x += oc) {
+        const int o = at(p, x, y);
+        if (o >= 0 && o >= bi) return false;
+      }
+    return true;
+  }
+};
+
+int resolve_slot(const av1h_frame_header& h, int ref) {
+  if (ref < 0 || ref >= h.num_reference_slots) return -1;
+  return h.reference_slots[ref];
+}
+
+int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
+  const av1h_frame_header& hdr = f.hdr;
+  if (hdr.width <= 0 || hdr.height <= 0 || hdr.width > 16384 || hdr.height > 16384)
+    return set_err(c, AV1H_ERR_INVALID, "frame has no size");
+  if (hdr.bit_depth != 8 && hdr.bit_depth != 10 && hdr.bit_depth != 12)
+    return set_err(c, AV1H_ERR_INVALID, "bit depth %d", hdr.bit_depth);
+  if (hdr.ss_x < 0 || hdr.ss_x > 1 || hdr.ss_y < 0 || hdr.ss_y > 1 || (hdr.ss_y && !hdr.ss_x))
+    return set_err(c, AV1H_ERR_UNSUPPORTED, "chroma subsampling %d,%d", hdr.ss_x, hdr.ss_y);
+  if (hdr.num_reference_slots < 0 || hdr.num_reference_slots > AV1H_MAX_REFS)
+    return set_err(c, AV1H_ERR_INVALID, "bad reference slot count");
+  for (int i = 0; i < 7; ++i) {
+    const int rw = hdr.reference_width[i] > 0 ? hdr.reference_width[i] : hdr.width;
+    const int rh = hdr.reference_height[i] > 0 ? hdr.reference_height[i] : hdr.height;
+    if (rw != hdr.width || rh != hdr.height)
+      return set_err(c, AV1H_ERR_UNSUPPORTED,
+                     "reference scaling between pictures of different sizes is not supported");
+  }
+  Layout& L = plan->lay;
+  make_layout(&L, hdr.width, hdr.height, hdr.bit_depth, hdr.ss_x, hdr.ss_y,
+              hdr.coded_width > 0 ? hdr.coded_width : hdr.width);
+  plan->hdr = hdr;
+  plan->upscaled = L.out_w[0] != L.plane_w[0];
+  const int fw = L.plane_w[0], fh = L.plane_h[0];
+  int scale_x = 1 << 14, scale_y = 1 << 14;
+  if (plan->upscaled) {
+    scale_x = ((hdr.width << 14) + fw / 2) / fw;
+    scale_y = ((hdr.height << 14) + fh / 2) / fh;
+    if (scale_x > 2 << 14 || scale_y > 2 << 14)
+      return set_err(c, AV1H_ERR_UNSUPPORTED, "superres beyond 2:1");
+  }
+  const int nb = (int)f.num_blocks;
+  H->blocks.clear();
+  H->tx.clear();
+  H->inter_list.clear();
+  H->queue_list.clear();
+  H->blocks.reserve(nb);
+  plan->slots.clear();
+  bool derive = false;
+  for (int i = 0; i < nb; ++i)
+    if (f.blocks[i].is_intra && (f.blocks[i].have_above_right < 0 || f.blocks[i].have_below_left < 0))
+      derive = true;
+  Owner own;
+  if (derive) own.build(L, f);
+  size_t cend = 0, rend = 0;
+  for (int i = 0; i < nb; ++i) {
+    const av1h_block& b = f.blocks[i];
+    const int p = b.plane;
+    if (p < 0 || p > 2) return set_err(c, AV1H_ERR_INVALID, "block %d: plane %d", i, p);
+    if (!pow2_in(b.w, 4, 64) || !pow2_in(b.h, 4, 64) || b.x < 0 || b.y < 0 || (b.x & 3) || (b.y & 3) ||
+        b.x + b.w > L.grid_w[p] || b.y + b.h > L.grid_h[p])
+      return set_err(c, AV1H_ERR_UNSUPPORTED, "block %d: %dx%d at %d,%d outside the coded grid", i, b.w,
+                     b.h, b.x, b.y);
+    if (b.bit_depth != hdr.bit_depth)
+      return set_err(c, AV1H_ERR_UNSUPPORTED, "block %d: bit depth differs from the frame", i);
+    if (b.tx_first < 0 || b.tx_count < 0 || b.tx_first + (int64_t)b.tx_count > f.num_transforms)
+      return set_err(c, AV1H_ERR_INVALID, "block %d: transform range", i);
+    DevBlock d;
+    std::memset(&d, 0, sizeof(d));
+    d.x = b.x;
+    d.y = b.y;
+    d.w = b.w;
+    d.h = b.h;
+    d.plane = p;
+    d.slot[0] = d.slot[1] = -1;
+    if (b.have_left && b.x > 0) d.flags |= kHaveLeft;
+    if (b.have_above && b.y > 0) d.flags |= kHaveAbove;
+    if (b.lossless) d.flags |= kLossless;
+    if (b.residual_provided) d.flags |= kResidualRaw;
+    if (b.lossless && !b.residual_provided)
+      for (int t = 0; t < b.tx_count; ++t) {
+        const av1h_transform& tx = f.transforms[b.tx_first + t];
+        if (tx.w != 4 || tx.h != 4) return set_err(c, AV1H_ERR_UNSUPPORTED, "lossless transform %dx%d", tx.w, tx.h);
+      }
+    if (b.is_intra) {
+      d.kind = kIntra;
+      d.mode = b.mode;
+      if (b.mode < 0 || b.mode > 13) return set_err(c, AV1H_ERR_INVALID, "block %d: intra mode %d", i, b.mode);
+      if (b.mode == AV1H_PRED_CFL && p == 0)
+        return set_err(c, AV1H_ERR_UNSUPPORTED, "chroma from luma on the luma plane");
+      d.angle = b.angle_delta;
+      d.filter_type = hdr.enable_intra_edge_filter ? b.filter_type : -1;
+      d.fi_mode = b.filter_intra_mode;
+      d.cfl_alpha = b.cfl_alpha;
+      const int ar = b.have_above_right >= 0
+                         ? b.have_above_right != 0
+                         : (b.y > 0 && own.decoded(p, b.x + b.w, b.x + 2 * b.w - 1, b.y - 1, b.y - 1, i));
+      const int bl = b.have_below_left >= 0
+                         ? b.have_below_left != 0
+                         : (b.x > 0 && own.decoded(p, b.x - 1, b.x - 1, b.y + b.h, b.y + 2 * b.h - 1, i));
+      if (ar) d.flags |= kHaveAR;
+      if (bl) d.flags |= kHaveBL;
+      if (b.use_filter_intra && p == 0) {
+        d.flags |= kFilterIntra;
+        if (b.filter_intra_mode < 0 || b.filter_intra_mode > 4)
+          return set_err(c, AV1H_ERR_INVALID, "filter intra mode %d", b.filter_intra_mode);
+      }
+      /* the read set the reference scheduler waits for (engine.cpp:811-875) */
+      int ly1 = b.y + b.h - 1, ax1 = b.x + b.w - 1;
+      if (b.use_palette) {
+        d.flags |= kPalette;
+        if (b.palette_off < 0 || b.palette_stride < b.w ||
+            (int64_t)b.palette_off + (int64_t)(b.h - 1) * b.palette_stride + b.w > f.num_palette)
+          return set_err(c, AV1H_ERR_INVALID, "block %d: palette map out of bounds", i);
+        for (int k = 0; k < 8; ++k) d.palette[k] = b.palette[k];
+        d.palette_off = b.palette_off;
+        d.palette_stride = b.palette_stride;
+        ax1 = b.x - 1;
+        ly1 = b.y - 1;
+      } else if (b.mode >= 1 && b.mode <= 8) {
+        const int pa = kModeAngle[b.mode] + 3 * b.angle_delta;
+        if (pa < 90 && ar) ax1 = b.x + 2 * b.w - 1;
+        if (pa > 180 && bl) ly1 = b.y + 2 * b.h - 1;
+      }
+      d.dep_x1 = ax1;
+      d.dep_y1 = ly1;
+      d.dep_corner = (ax1 >= b.x || ly1 >= b.y) ? 1 : 0;
+      H->queue_list.push_back(i);
+    } else {
+      d.kind = kInter;
+      if (b.intrabc) return set_err(c, AV1H_ERR_UNSUPPORTED, "intra block copy is not supported on the GPU yet");
+      if (b.interintra) return set_err(c, AV1H_ERR_UNSUPPORTED, "inter/intra compound is not supported on the GPU yet");
+      if (b.obmc_count > 0) return set_err(c, AV1H_ERR_UNSUPPORTED, "OBMC is not supported on the GPU yet");
+      if (b.warp[0] || b.warp[1] || b.mask_warp[0] || b.mask_warp[1])
+        return set_err(c, AV1H_ERR_UNSUPPORTED, "warped motion is not supported on the GPU yet");
+      if (b.compound == AV1H_COMPOUND_WEDGE || b.compound == AV1H_COMPOUND_DIFFWTD)
+        return set_err(c, AV1H_ERR_UNSUPPORTED, "masked compound is not supported on the GPU yet");
+      if (b.compound < 0 || b.compound > 2) return set_err(c, AV1H_ERR_INVALID, "compound %d", b.compound);
+      const int nref = b.compound ? 2 : 1;
+      for (int r = 0; r < nref; ++r) {
+        const int s = resolve_slot(hdr, b.ref[r]);
+        if (s < 0) return set_err(c, AV1H_ERR_UNSUPPORTED, "block %d: inter prediction without a reference", i);
+        if (s >= kSlots) return set_err(c, AV1H_ERR_INVALID, "block %d: reference slot %d", i, s);
+        d.slot[r] = s;
+        if (std::find(plan->slots.begin(), plan->slots.end(), s) == plan->slots.end()) plan->slots.push_back(s);
+      }
+      for (int r = 0; r < 2; ++r) {
+        d.mv[r][0] = b.mv[r][0];
+        d.mv[r][1] = b.mv[r][1];
+      }
+      if (b.filter < 0 || b.filter > 3 || b.filter_v < 0 || b.filter_v > 3)
+        return set_err(c, AV1H_ERR_INVALID, "block %d: filter", i);
+      d.filt = b.filter;
+      d.filt_v = b.filter_v;
+      d.compound = b.compound;
+      d.w0 = b.blend[0];
+      d.w1 = b.blend[1];
+      d.ref_w = L.out_w[p];
+      d.ref_h = L.out_h[p];
+      d.scale_x = scale_x;
+      d.scale_y = scale_y;
+      H->inter_list.push_back(i);
+    }
+    d.tx_first = (int)H->tx.size();
+    for (int t = 0; t < b.tx_count; ++t) {
+      const av1h_transform& tx = f.transforms[b.tx_first + t];
+      if (tx.eob <= 0) continue;
+      if (!pow2_in(tx.w, 4, 64) || !pow2_in(tx.h, 4, 64) || tx.x < 0 || tx.y < 0 || tx.x + tx.w > b.w ||
+          tx.y + tx.h > b.h || tx.coef_offset < 0 || tx.tx_type < 0 || tx.tx_type > 15)
+        return set_err(c, AV1H_ERR_INVALID, "block %d: transform %d out of range", i, t);
+      const int rs = b.lossless ? 0 : tx_row_shift(tx.w, tx.h);
+      if (rs < 0) return set_err(c, AV1H_ERR_UNSUPPORTED, "transform size %dx%d", tx.w, tx.h);
+      if (!b.lossless && !b.residual_provided &&
+          (!kind_ok(row_kind(tx.tx_type), tx.w) || !kind_ok(col_kind(tx.tx_type), tx.h)))
+        return set_err(c, AV1H_ERR_UNSUPPORTED, "transform type %d is not defined for %dx%d", tx.tx_type, tx.w, tx.h);
+      DevTx dt;
+      dt.x = b.x + tx.x;
+      dt.y = b.y + tx.y;
+      dt.w = tx.w;
+      dt.h = tx.h;
+      dt.coef = tx.coef_offset;
+      dt.tx_type = tx.tx_type;
+      dt.row_shift = rs;
+      dt.eob = tx.eob;
+      const size_t end = (size_t)tx.coef_offset + (size_t)tx.w * tx.h;
+      if (b.residual_provided)
+        rend = std::max(rend, end);
+      else
+        cend = std::max(cend, end);
+      H->tx.push_back(dt);
+    }
+    d.tx_count = (int)H->tx.size() - d.tx_first;
+    H->blocks.push_back(d);
+  }
+  /* pools padded like the reference planner's (engine.cpp:595-660) */
+  H->coef.assign(std::max((size_t)f.num_coefficients, cend + 64), 0);
+  if (f.num_coefficients) std::memcpy(H->coef.data(), f.coefficients, (size_t)f.num_coefficients * 4);
+  H->residual.assign(std::max((size_t)f.num_residual, rend + 64), 0);
+  if (f.num_residual) std::memcpy(H->residual.data(), f.residual, (size_t)f.num_residual * 4);
+  H->palette.assign(f.palette_map, f.palette_map + f.num_palette);
+  H->palette.resize(H->palette.size() + 8, 0);
+
+  /* deblocking: vertical edges then horizontal, with a line prefix each */
+  H->deblock.clear();
+  int lines[2] = {0, 0}, nedge[2] = {0, 0};
+  for (int pass = 0; pass < 2; ++pass)
+    for (int64_t i = 0; i < f.num_deblock; ++i) {
+      const av1h_deblock_edge& e = f.deblock[i];
+      if ((e.vertical != 0) != (pass == 0)) continue;
+      if (e.plane < 0 || e.plane > 2 || e.length < 0)
+        return set_err(c, AV1H_ERR_INVALID, "deblock edge %lld", (long long)i);
+      const int reach = e.filter_size >= 16 ? 7 : (e.filter_size >= 8 ? 4 : 2);
+      const int rows = L.out_h[e.plane] + (e.plane ? 32 : 64);
+      const int across = e.vertical ? e.x : e.y, along = e.vertical ? e.y : e.x;
+      const int across_lim = e.vertical ? L.stride[e.plane] : rows;
+      const int along_lim = e.vertical ? rows : L.stride[e.plane];
+      if (across < reach || across + reach > across_lim || along < 0 || along + e.length > along_lim)
+        return set_err(c, AV1H_ERR_INVALID, "deblock edge %lld out of range", (long long)i);
+      if (e.length == 0) continue;
+      DevDeblock d;
+      d.plane = e.plane;
+      d.x = e.x;
+      d.y = e.y;
+      d.vertical = e.vertical ? 1 : 0;
+      d.len = e.length;
+      d.level = e.level;
+      d.filter_size = e.filter_size;
+      d.line0 = lines[pass];
+      lines[pass] += e.length;
+      ++nedge[pass];
+      H->deblock.push_back(d);
+    }
+  /* CDEF */
+  H->cdef.clear();
+  for (int64_t i = 0; i < f.num_cdef; ++i) {
+    const av1h_cdef_block& cb = f.cdef[i];
+    if (cb.plane < 0 || cb.plane > 2 || cb.x < 0 || cb.y < 0 || cb.x >= L.grid_w[cb.plane] ||
+        cb.y >= L.grid_h[cb.plane] || cb.damping < 0 || cb.damping > 15 || cb.dir > 7)
+      return set_err(c, AV1H_ERR_INVALID, "cdef block %lld", (long long)i);
+    DevCdef d{cb.plane, cb.x, cb.y, cb.pri_strength, cb.sec_strength, cb.damping, cb.dir, 0};
+    H->cdef.push_back(d);
+  }
+  /* loop restoration jobs: one per unit per stripe (engine.cpp:1382-1462) */
+  H->lr.clear();
+  int lr_px = 0;
+  for (int p = 0; p < 3; ++p) {
+    if (hdr.restoration_type[p] == AV1H_RESTORE_NONE) continue;
+    const int sy = p > 0 ? hdr.ss_y : 0;
+    if (hdr.restoration_unit_log2[p] < 0 || hdr.restoration_unit_log2[p] > 10)
+      return set_err(c, AV1H_ERR_INVALID, "restoration unit size");
+    const int usz = 1 << hdr.restoration_unit_log2[p];
+    const int ucols = std::max(hdr.restoration_unit_cols[p], 1);
+    const int pw = L.out_w[p], ph = L.out_h[p];
+    const int fh2 = (hdr.height + (sy ? 1 << (sy - 1) : 0)) >> sy;
+    const int urows = std::max((fh2 + (usz >> 1)) / usz, 1);
+    for (int y = 0; y < ph;) {
+      const int stripe = ((y << sy) + 8) / 64;
+      const int s0 = (-8 + stripe * 64) >> sy;
+      const int s1 = s0 + (64 >> sy) - 1;
+      const int y1 = std::min(ph, s1 + 1);
+      const int urow = std::min(urows - 1, (((y << sy) + 8) >> sy) / usz);
+      for (int uc = 0; uc < ucols; ++uc) {
+        const int64_t idx = (int64_t)urow * ucols + uc;
+        if (idx >= f.num_restoration[p]) continue;
+        const av1h_lr_unit& u = f.restoration[p][idx];
+        if (u.type == AV1H_RESTORE_NONE) continue;
+        const int ux0 = uc * usz;
+        const int ux1 = uc + 1 == ucols ? pw : std::min(ux0 + usz, pw);
+        if (ux1 <= ux0) continue;
+        DevLr j;
+        std::memset(&j, 0, sizeof(j));
+        j.plane = p;
+        j.x = ux0;
+        j.y = y;
+        j.w = ux1 - ux0;
+        j.h = y1 - y;
+        j.stripe0 = s0;
+        j.stripe1 = s1;
+        j.type = u.type == AV1H_RESTORE_WIENER ? 1 : 2;
+        if (j.type == 1) {
+          for (int k = 0; k < 3; ++k) {
+            j.vf[k] = u.wiener[0][k];
+            j.hf[k] = u.wiener[1][k];
+          }
+        } else {
+          const int* sp = av1t_spec_table + AV1T_OFF_SGR_PARAMS + 4 * (u.sgr_set & 15);
+          for (int k = 0; k < 2; ++k) {
+            j.r[k] = sp[2 * k];
+            j.eps[k] = sp[2 * k + 1];
+            j.xqd[k] = u.sgr_xqd[k];
+          }
+        }
+        j.px0 = lr_px;
+        lr_px += j.w * j.h;
+        H->lr.push_back(j);
+      }
+      y = y1;
+    }
+  }
+  plan->uses_lr = !H->lr.empty();
+  /* film grain parameters */
+  std::memset(&H->grain, 0, sizeof(H->grain));
+  plan->has_grain = hdr.grain.apply != 0;
+  if (plan->has_grain) {
+    const av1h_grain& g = hdr.grain;
+    DevGrain& d = H->grain;
+    if (g.seed < 0 || g.seed > 65535) return set_err(c, AV1H_ERR_INVALID, "grain seed");
+    if (g.ar_coeff_lag < 0 || g.ar_coeff_lag > 3) return set_err(c, AV1H_ERR_INVALID, "grain lag");
+    if (g.num_y_points < 0 || g.num_y_points > 16 || g.num_cb_points < 0 || g.num_cb_points > 16 ||
+        g.num_cr_points < 0 || g.num_cr_points > 16)
+      return set_err(c, AV1H_ERR_INVALID, "grain points");
+    d.seed = g.seed;
+    d.bit_depth = hdr.bit_depth;
+    d.num_y = g.num_y_points;
+    d.num_cb = g.num_cb_points;
+    d.num_cr = g.num_cr_points;
+    d.cfl = g.chroma_scaling_from_luma;
+    d.lag = g.ar_coeff_lag;
+    d.shift = g.ar_coeff_shift_minus_6 + 6;
+    d.scale_shift = 12 - hdr.bit_depth + g.grain_scale_shift;
+    d.scaling_shift = g.grain_scaling_minus_8 + 8;
+    d.overlap = g.overlap_flag;
+    d.cb_mult = g.cb_mult;
+    d.cb_luma_mult = g.cb_luma_mult;
+    d.cb_offset = g.cb_offset;
+    d.cr_mult = g.cr_mult;
+    d.cr_luma_mult = g.cr_luma_mult;
+    d.cr_offset = g.cr_offset;
+    const int maxv = (1 << hdr.bit_depth) - 1;
+    d.min_value = 0;
+    d.max_luma = d.max_chroma = maxv;
+    if (g.clip_to_restricted_range) {
+      d.min_value = 16 << (hdr.bit_depth - 8);
+      d.max_luma = 235 << (hdr.bit_depth - 8);
+      d.max_chroma = hdr.color_matrix == 3 ? d.max_luma : (240 << (hdr.bit_depth - 8));
+    }
+    for (int k = 0; k < 24; ++k) d.ar_y[k] = g.ar_coeffs_y[k];
+    for (int k = 0; k < 25; ++k) {
+      d.ar_cb[k] = g.ar_coeffs_cb[k];
+      d.ar_cr[k] = g.ar_coeffs_cr[k];
+    }
+    const bool chroma = g.chroma_scaling_from_luma || g.num_cb_points > 0 || g.num_cr_points > 0;
+    for (int pl = 0; pl < 3; ++pl) {
+      if (pl > 0 && !chroma)
+        std::memset(d.lut + 256 * pl, 0, 256 * sizeof(int));
+      else
+        scaling_lut(g, pl, d.lut + 256 * pl);
+    }
+    const int sy = hdr.ss_y;
+    const int ph = (L.out_h[0] + sy) >> sy;
+    d.stripes = std::max((L.out_h[0] + 31) / 32, (ph + (32 >> sy) - 1) / (32 >> sy));
+    d.blocks = ((L.out_w[0] + 1) / 2 + 15) / 16;
+  }
+  plan->blocks = nb;
+  plan->grain = H->grain;
+
+  /* one device buffer: offsets, then the arrays */
+  struct Part {
+    const void* src;
+    size_t bytes;
+    size_t off;
+  };
+  Part parts[11] = {
+      {H->blocks.data(), H->blocks.size() * sizeof(DevBlock), 0},
+      {H->tx.data(), H->tx.size() * sizeof(DevTx), 0},
+      {nullptr, 0, 0},
+      {H->inter_list.data(), H->inter_list.size() * 4, 0},
+      {H->queue_list.data(), H->queue_list.size() * 4, 0},
+      {H->coef.data(), H->coef.size() * 4, 0},
+      {H->residual.data(), H->residual.size() * 4, 0},
+      {H->palette.data(), H->palette.size(), 0},
+      {H->deblock.data(), H->deblock.size() * sizeof(DevDeblock), 0},
+      {H->cdef.data(), H->cdef.size() * sizeof(DevCdef), 0},
+      {H->lr.data(), H->lr.size() * sizeof(DevLr), 0},
+  };
+  size_t total = 0;
+  for (Part& pt : parts) {
+    pt.off = total;
+    total += align256(pt.bytes);
+  }
+  const size_t grain_off = total;
+  total += align256(sizeof(DevGrain));
+  if (plan->dbytes < total) {
+    if (plan->dbuf) (void)hipFree(plan->dbuf);
+    plan->dbuf = nullptr;
+    plan->dbytes = 0;
+    HIP_OK(c, hipMalloc(&plan->dbuf, total));
+    plan->dbytes = total;
+  }
+  plan->hbuf.resize(total);
+  for (Part& pt : parts)
+    if (pt.bytes) std::memcpy(plan->hbuf.data() + pt.off, pt.src, pt.bytes);
+  std::memcpy(plan->hbuf.data() + grain_off, &H->grain, sizeof(DevGrain));
+  uint8_t* db = static_cast<uint8_t*>(plan->dbuf);
+  FramePlanDev& P = plan->dev;
+  P.blocks = reinterpret_cast<const DevBlock*>(db + parts[0].off);
+  P.tx = reinterpret_cast<const DevTx*>(db + parts[1].off);
+  P.obmc = nullptr;
+  P.inter_list = reinterpret_cast<const int32_t*>(db + parts[3].off);
+  P.queue_list = reinterpret_cast<const int32_t*>(db + parts[4].off);
+  P.coef = reinterpret_cast<const int32_t*>(db + parts[5].off);
+  P.residual = reinterpret_cast<const int32_t*>(db + parts[6].off);
+  P.palette = db + parts[7].off;
+  P.deblock = reinterpret_cast<const DevDeblock*>(db + parts[8].off);
+  P.cdef = reinterpret_cast<const DevCdef*>(db + parts[9].off);
+  P.lr = reinterpret_cast<const DevLr*>(db + parts[10].off);
+  P.grain = reinterpret_cast<const DevGrain*>(db + grain_off);
+  P.n_inter = (int)H->inter_list.size();
+  P.n_queue = (int)H->queue_list.size();
+  P.n_deblock_v = nedge[0];
+  P.n_deblock_h = nedge[1];
+  P.lines_v = lines[0];
+  P.lines_h = lines[1];
+  P.n_cdef = (int)H->cdef.size();
+  P.n_lr = (int)H->lr.size();
+  P.lr_px = lr_px;
+  return AV1H_OK;
+}
+
+int upload(Ctx* c, Plan* plan) {
+  /* staged through a pinned buffer so the copy is a DMA */
+  const size_t n = plan->hbuf.size();
+  if (c->host_pin_cap < n) {
+    if (c->host_pin) (void)hipHostFree(c->host_pin);
+    c->host_pin = nullptr;
+    c->host_pin_cap = 0;
+    HIP_OK(c, hipHostMalloc(&c->host_pin, n, hipHostMallocDefault));
+    c->host_pin_cap = n;
+  }
+  HIP_OK(c, hipStreamSynchronize(c->stream));
+  std::memcpy(c->host_pin, plan->hbuf.data(), n);
+  HIP_OK(c, hipMemcpyAsync(plan->dbuf, c->host_pin, n, hipMemcpyHostToDevice, c->stream));
+  return AV1H_OK;
+}
+
+void free_plan(Plan* p) {
+  if (p->dbuf) (void)hipFree(p->dbuf);
+  p->dbuf = nullptr;
+  p->dbytes = 0;
+  p->hbuf.clear();
+  p->hbuf.shrink_to_fit();
+}
+
+/* grows the per geometry buffers; the pool is re-created when it must grow
+ * (as the reference CPU backend does, cpu_backend.cpp:77-91) */
+int prepare(Ctx* c, const Layout& L) {
+  if (c->cap < (size_t)L.size) {
+    for (uint16_t** b : {&c->frame, &c->scratch, &c->lr_curr, &c->lr_cdef, &c->lr_prev_up, &c->pool})
+      if (*b) {
+        (void)hipFree(*b);
+        *b = nullptr;
+      }
+    c->cap = 0;
+    const size_t bytes = (size_t)L.size * 2;
+    HIP_OK(c, hipMalloc(&c->frame, bytes));
+    HIP_OK(c, hipMalloc(&c->scratch, bytes));
+    HIP_OK(c, hipMalloc(&c->lr_curr, bytes));
+    HIP_OK(c, hipMalloc(&c->lr_cdef, bytes));
+    HIP_OK(c, hipMalloc(&c->lr_prev_up, bytes));
+    HIP_OK(c, hipMalloc(&c->pool, bytes * kSlots));
+    HIP_OK(c, hipMemsetAsync(c->frame, 0, bytes, c->stream));
+    HIP_OK(c, hipMemsetAsync(c->pool, 0, bytes * kSlots, c->stream));
+    c->cap = (size_t)L.size;
+    std::memset(c->used, 0, sizeof(c->used));
+    c->last_slot = -1;
+  }
+  if (c->cells_cap < L.cells) {
+    if (c->cells) (void)hipFree(c->cells);
+    c->cells = nullptr;
+    HIP_OK(c, hipMalloc(&c->cells, (size_t)L.cells * 4));
+    HIP_OK(c, hipMemsetAsync(c->cells, 0, (size_t)L.cells * 4, c->stream));
+    c->cells_cap = L.cells;
+    c->epoch = 0;
+  }
+  c->lay = L;
+  c->have_layout = true;
+  return AV1H_OK;
+}
+
+int alloc_slot(Ctx* c) {
+  for (int i = 0; i < kSlots; ++i)
+    if (!c->used[i]) return i;
+  return -1;
+}
+
+int run_plan(Ctx* c, Plan* plan, int keep, int reserved, const av1h_planes* out, av1h_frame_info* info,
+             int* slot_out, double t0) {
+  if (slot_out) *slot_out = -1;
+  const Layout& L = plan->lay;
+  int rc = prepare(c, L);
+  if (rc != AV1H_OK) return rc;
+  for (int s : plan->slots)
+    if (!c->used[s]) return set_err(c, AV1H_ERR_INVALID, "a block names reference slot %d, which holds no picture", s);
+  int slot = -1;
+  if (keep) {
+    if (reserved >= 0) {
+      if (reserved >= kSlots) return set_err(c, AV1H_ERR_INVALID, "invalid reference slot");
+      slot = reserved;
+    } else {
+      slot = alloc_slot(c);
+      if (slot < 0) return set_err(c, AV1H_ERR_SLOTS, "reference pool exhausted");
+    }
+  }
+  hipStream_t s = c->stream;
+  int launches = 0;
+  ++c->epoch;
+  const size_t bytes = (size_t)L.size * 2;
+  HIP_OK(c, hipEventRecord(c->ev[0], s));
+  ReconArgs ra;
+  ra.plan = plan->dev;
+  ra.lay = L;
+  ra.frame = c->frame;
+  ra.pool = c->pool;
+  ra.slot_size = L.size;
+  ra.cells = c->cells;
+  ra.epoch = c->epoch;
+  ra.queue_counter = c->counters;
+  ra.error = c->counters + 1;
+  ra.maxv = (1 << L.bit_depth) - 1;
+  ra.edge_filter = plan->hdr.enable_intra_edge_filter;
+  HIP_OK(c, launch_recon(ra, c->cus, s, &launches));
+  HIP_OK(c, hipEventRecord(c->ev[1], s));
+  FilterArgs fa;
+  fa.plan = plan->dev;
+  fa.lay = L;
+  fa.frame = c->frame;
+  fa.scratch = c->scratch;
+  fa.lr_curr = c->lr_curr;
+  fa.lr_cdef = c->lr_cdef;
+  fa.lr_prev_up = c->lr_prev_up;
+  fa.sharpness = plan->hdr.sharpness;
+  fa.maxv = ra.maxv;
+  HIP_OK(c, launch_deblock(fa, s, &launches));
+  if (plan->uses_lr) HIP_OK(c, hipMemcpyAsync(c->lr_curr, c->frame, bytes, hipMemcpyDeviceToDevice, s));
+  if (plan->dev.n_cdef > 0) {
+    HIP_OK(c, hipMemcpyAsync(c->scratch, c->frame, bytes, hipMemcpyDeviceToDevice, s));
+    HIP_OK(c, launch_cdef(fa, s, &launches));
+  }
+  if (plan->upscaled) {
+    HIP_OK(c, hipMemcpyAsync(c->scratch, c->frame, bytes, hipMemcpyDeviceToDevice, s));
+    HIP_OK(c, launch_superres(fa, c->scratch, c->frame, s, &launches));
+    if (plan->uses_lr) HIP_OK(c, launch_superres(fa, c->lr_curr, c->lr_prev_up, s, &launches));
+  }
+  if (plan->uses_lr) {
+    HIP_OK(c, hipMemcpyAsync(c->lr_cdef, c->frame, bytes, hipMemcpyDeviceToDevice, s));
+    HIP_OK(c, launch_lr(fa, c->lr_cdef, plan->upscaled ? c->lr_prev_up : c->lr_curr, s, &launches));
+  }
+  if (slot >= 0)
+    HIP_OK(c, hipMemcpyAsync(c->pool + (size_t)slot * L.size, c->frame, bytes, hipMemcpyDeviceToDevice, s));
+  if (plan->has_grain) {
+    const size_t need = (size_t)plan->grain.stripes * plan->grain.blocks + 64;
+    if (c->grain_offs_cap < need) {
+      HIP_OK(c, hipStreamSynchronize(s));
+      if (c->grain_offs) (void)hipFree(c->grain_offs);
+      c->grain_offs = nullptr;
+      HIP_OK(c, hipMalloc(&c->grain_offs, need));
+      c->grain_offs_cap = need;
+    }
+    HIP_OK(c, launch_grain(L, plan->dev.grain, plan->grain, c->frame, c->grain_tmpl, c->grain_offs, s, &launches));
+  }
+  HIP_OK(c, hipEventRecord(c->ev[2], s));
+  const double t_sub = now_ms();
+  if (slot >= 0) {
+    c->used[slot] = true;
+    c->slot_lay[slot] = L;
+  }
+  c->last_slot = slot;
+  c->have_frame = true;
+  /* read back */
+  if (out) {
+    size_t host_need = (size_t)L.size * 2;
+    if (c->host_pin_cap < host_need) {
+      HIP_OK(c, hipStreamSynchronize(s));
+      if (c->host_pin) (void)hipHostFree(c->host_pin);
+      c->host_pin = nullptr;
+      c->host_pin_cap = 0;
+      HIP_OK(c, hipHostMalloc(&c->host_pin, host_need, hipHostMallocDefault));
+      c->host_pin_cap = host_need;
+    }
+    HIP_OK(c, hipMemcpyAsync(c->host_pin, c->frame, host_need, hipMemcpyDeviceToHost, s));
+  }
+  uint32_t err_word = 0;
+  HIP_OK(c, hipMemcpyAsync(&err_word, &c->counters[1], 4, hipMemcpyDeviceToHost, s));
+  HIP_OK(c, hipStreamSynchronize(s));
+  const double t_done = now_ms();
+  if (err_word) {
+    HIP_OK(c, hipMemsetAsync(c->counters + 1, 0, 4, s));
+    HIP_OK(c, hipStreamSynchronize(s));
+    if (slot >= 0) c->used[slot] = false;
+    return set_err(c, AV1H_ERR_INVALID,
+                   "a block waited too long for a neighbour: the availability flags name blocks "
+                   "that come later in coding order");
+  }
+  if (out) {
+    const uint16_t* h = static_cast<const uint16_t*>(c->host_pin);
+    for (int p = 0; p < 3; ++p)
+      for (int y = 0; y < L.out_h[p]; ++y)
+        std::memcpy(out->plane[p] + (size_t)y * out->stride[p], h + L.plane_off[p] + (size_t)y * L.stride[p],
+                    (size_t)L.out_w[p] * 2);
+  }
+  if (info) {
+    info->width = L.width;
+    info->height = L.height;
+    info->bit_depth = L.bit_depth;
+    info->ss_x = L.ss_x;
+    info->ss_y = L.ss_y;
+  }
+  float recon = 0, filt = 0;
+  (void)hipEventElapsedTime(&recon, c->ev[0], c->ev[1]);
+  (void)hipEventElapsedTime(&filt, c->ev[1], c->ev[2]);
+  c->stats.blocks = plan->blocks;
+  c->stats.waves = 0;
+  c->stats.kernel_launches = launches;
+  c->stats.device_recon_ms = recon;
+  c->stats.device_filter_ms = filt;
+  c->stats.reconstruct_ms = recon;
+  c->stats.filter_ms = filt;
+  c->stats.readback_ms = t_done - t_sub;
+  c->stats.total_ms = t_done - t0;
+  if (slot_out) *slot_out = slot;
+  return AV1H_OK;
+}
+
+/* the picture a slot names (the reference's gather rule,
+ * cpu_backend.cpp:557-587) */
+const uint16_t* gather(Ctx* c, int slot, Layout* L) {
+  if (slot >= 0) {
+    if (slot >= kSlots || !c->used[slot]) {
+      set_err(c, AV1H_ERR_INVALID, "invalid reference slot");
+      return nullptr;
+    }
+    if (slot != c->last_slot) {
+      *L = c->slot_lay[slot];
+      return c->pool + (size_t)slot * c->lay.size;
+    }
+  }
+  if (!c->have_frame) {
+    set_err(c, AV1H_ERR_INVALID, "no frame to convert");
+    return nullptr;
+  }
+  *L = c->lay;
+  return c->frame;
+}
+
+int ensure_pack(Ctx* c, size_t bytes) {
+  if (c->pack_cap < bytes) {
+    if (c->pack) (void)hipFree(c->pack);
+    c->pack = nullptr;
+    c->pack_cap = 0;
+    HIP_OK(c, hipMalloc(&c->pack, bytes));
+    c->pack_cap = bytes;
+  }
+  return AV1H_OK;
+}
+
+}  // namespace
+
+/* ============================================================== C ABI === */
+
+struct av1h_ctx {
+  Ctx c;
+};
+
+extern "C" {
+
+int av1h_create(int device_index, av1h_ctx** out) {
+  if (!out) return AV1H_ERR_INVALID;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= device_index || device_index < 0) return AV1H_ERR_NO_DEVICE;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device_index) != hipSuccess) return AV1H_ERR_NO_DEVICE;
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return AV1H_ERR_NO_DEVICE;
+  av1h_ctx* h = new av1h_ctx;
+  Ctx* c = &h->c;
+  c->device = device_index;
+  c->prop = prop;
+  c->cus = prop.multiProcessorCount;
+  std::memset(&c->stats, 0, sizeof(c->stats));
+  if (hipSetDevice(device_index) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(&c->counters, 64) != hipSuccess || hipMemset(c->counters, 0, 64) != hipSuccess ||
+      hipMalloc(&c->grain_tmpl, 3 * 82 * 73 * 2) != hipSuccess) {
+    delete h;
+    return AV1H_ERR_DEVICE;
+  }
+  for (auto& e : c->ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      delete h;
+      return AV1H_ERR_DEVICE;
+    }
+  *out = h;
+  return AV1H_OK;
+}
+
+void av1h_destroy(av1h_ctx* h) {
+  if (!h) return;
+  Ctx* c = &h->c;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (void* p : {(void*)c->frame, (void*)c->scratch, (void*)c->lr_curr, (void*)c->lr_cdef, (void*)c->lr_prev_up,
+                  (void*)c->pool, (void*)c->cells, (void*)c->counters, (void*)c->grain_tmpl, (void*)c->grain_offs,
+                  (void*)c->pack})
+    if (p) (void)hipFree(p);
+  if (c->host_pin) (void)hipHostFree(c->host_pin);
+  free_plan(&c->scratch_plan);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete h;
+}
+
+const char* av1h_last_error(const av1h_ctx* h) { return h ? h->c.err.c_str() : "no context"; }
+
+int av1h_get_device_info(av1h_ctx* h, av1h_device_info* out) {
+  if (!h || !out) return AV1H_ERR_INVALID;
+  const hipDeviceProp_t& p = h->c.prop;
+  std::memset(out, 0, sizeof(*out));
+  std::snprintf(out->name, sizeof(out->name), "%s", p.name);
+  std::snprintf(out->arch, sizeof(out->arch), "%s", p.gcnArchName);
+  out->compute_units = p.multiProcessorCount;
+  out->l2_bytes = p.l2CacheSize;
+  out->global_mem = (int64_t)p.totalGlobalMem;
+  out->clock_khz = p.clockRate;
+  out->mem_clock_khz = p.memoryClockRate;
+  out->mem_bus_width = p.memoryBusWidth;
+  return AV1H_OK;
+}
+
+int av1h_decode(av1h_ctx* h, const av1h_frame_desc* f, int keep, int reserved, const av1h_planes* out,
+                av1h_frame_info* info, int* slot_out) {
+  if (!h || !f) return AV1H_ERR_INVALID;
+  Ctx* c = &h->c;
+  std::lock_guard<std::mutex> lk(c->mu);
+  (void)hipSetDevice(c->device);
+  const double t0 = now_ms();
+  Host host;
+  int rc = flatten(c, *f, &c->scratch_plan, &host);
+  if (rc != AV1H_OK) return rc;
+  rc = upload(c, &c->scratch_plan);
+  if (rc != AV1H_OK) return rc;
+  return run_plan(c, &c->scratch_plan, keep, reserved, out, info, slot_out, t0);
+}
+
+int av1h_decode_wire(av1h_ctx* h, const uint8_t* wire, size_t len, int keep, int reserved, const av1h_planes* out,
+                     av1h_frame_info* info, int* slot_out) {
+  av1h_frame_desc d;
+  if (!h) return AV1H_ERR_INVALID;
+  if (av1h_wire_parse(wire, len, &d) != AV1H_OK) {
+    h->c.err = "not a wire frame";
+    return AV1H_ERR_INVALID;
+  }
+  return av1h_decode(h, &d, keep, reserved, out, info, slot_out);
+}
+
+/* Pre-planning (reference Decoder::plan_frame / decode_reference_planned,
+ * av1gpu.hpp:670-712): the syntax is flattened and uploaded into device
+ * memory owned by the plan, so the decode itself starts from HBM. */
+int av1h_plan_frame(av1h_ctx* h, const av1h_frame_desc* f, void** plan_out) {
+  if (!h || !f || !plan_out) return AV1H_ERR_INVALID;
+  Ctx* c = &h->c;
+  std::lock_guard<std::mutex> lk(c->mu);
+  (void)hipSetDevice(c->device);
+  Plan* p = new Plan;
+  Host host;
+  int rc = flatten(c, *f, p, &host);
+  if (rc == AV1H_OK) rc = upload(c, p);
+  if (rc == AV1H_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = AV1H_ERR_DEVICE;
+  if (rc != AV1H_OK) {
+    free_plan(p);
+    delete p;
+    return rc;
+  }
+  p->hbuf.clear();
+  p->hbuf.shrink_to_fit();
+  *plan_out = p;
+  return AV1H_OK;
+}
+
+int av1h_plan_frame_wire(av1h_ctx* h, const uint8_t* wire, size_t len, void** plan_out) {
+  av1h_frame_desc d;
+  if (!h) return AV1H_ERR_INVALID;
+  if (av1h_wire_parse(wire, len, &d) != AV1H_OK) {
+    h->c.err = "not a wire frame";
+    return AV1H_ERR_INVALID;
+  }
+  return av1h_plan_frame(h, &d, plan_out);
+}
+
+int av1h_decode_planned(av1h_ctx* h, void* plan, int keep, int reserved, const av1h_planes* out,
+                        av1h_frame_info* info, int* slot_out) {
+  if (!h || !plan) return AV1H_ERR_INVALID;
+  Ctx* c = &h->c;
+  std::lock_guard<std::mutex> lk(c->mu);
+  (void)hipSetDevice(c->device);
+  return run_plan(c, static_cast<Plan*>(plan), keep, reserved, out, info, slot_out, now_ms());
+}
+
+void av1h_discard_plan(av1h_ctx* h, void* plan) {
+  if (!plan) return;
+  if (h) (void)hipSetDevice(h->c.device);
+  Plan* p = static_cast<Plan*>(plan);
+  free_plan(p);
+  delete p;
+}
+
+int av1h_reserve_reference(av1h_ctx* h) {
+  if (!h) return AV1H_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(h->c.mu);
+  const int s = alloc_slot(&h->c);
+  if (s < 0) return set_err(&h->c, AV1H_ERR_SLOTS, "reference pool exhausted");
+  h->c.used[s] = true;
+  return s;
+}
+
+void av1h_release_reference(av1h_ctx* h, int slot) {
+  if (!h) return;
+  std::lock_guard<std::mutex> lk(h->c.mu);
+  if (slot >= 0 && slot < kSlots) h->c.used[slot] = false;
+}
+
+size_t av1h_nv12_size(int width, int height, int ss_x, int ss_y, size_t pitch) {
+  /* reference nv12_buffer_size, src/core/factory.cpp:236-243 */
+  if (width <= 0 || height <= 0 || ss_x != 1 || ss_y != 1) return 0;
+  const size_t crow = 2 * (((size_t)width + 1) / 2);
+  if ((pitch & 1) || pitch < crow) return 0;
+  return pitch * ((size_t)height + ((size_t)height + 1) / 2);
+}
+
+int av1h_to_nv12(av1h_ctx* h, int slot, size_t pitch, uint8_t* out, size_t out_bytes) {
+  if (!h || !out) return AV1H_ERR_INVALID;
+  Ctx* c = &h->c;
+  std::lock_guard<std::mutex> lk(c->mu);
+  (void)hipSetDevice(c->device);
+  Layout L;
+  const uint16_t* src = gather(c, slot, &L);
+  if (!src) return AV1H_ERR_INVALID;
+  const size_t need = av1h_nv12_size(L.width, L.height, L.ss_x, L.ss_y, pitch);
+  if (need == 0)
+    return set_err(c, AV1H_ERR_INVALID, "NV12 output needs a 4:2:0 frame and an even pitch at least as wide");
+  if (out_bytes < need) return set_err(c, AV1H_ERR_INVALID, "output buffer too small");
+  int rc = ensure_pack(c, need);
+  if (rc != AV1H_OK) return rc;
+  HIP_OK(c, launch_nv12(L, src, c->pack, (int)pitch, c->stream));
+  HIP_OK(c, hipMemcpyAsync(out, c->pack, need, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(c, hipStreamSynchronize(c->stream));
+  return AV1H_OK;
+}
+
+int av1h_to_rgb(av1h_ctx* h, int slot, int matrix, int range, uint8_t* out, size_t out_bytes) {
+  if (!h || !out) return AV1H_ERR_INVALID;
+  Ctx* c = &h->c;
+  std::lock_guard<std::mutex> lk(c->mu);
+  (void)hipSetDevice(c->device);
+  Layout L;
+  const uint16_t* src = gather(c, slot, &L);
+  if (!src) return AV1H_ERR_INVALID;
+  const size_t need = (size_t)L.width * L.height * 3;
+  if (out_bytes < need) return set_err(c, AV1H_ERR_INVALID, "output buffer too small");
+  /* Q16 coefficients (reference src/core/tables.cpp:481-533) */
+  int m[8] = {0}, o[3] = {0};
+  const int sh = L.bit_depth - 8;
+  const int yo = range == 0 ? (16 << sh) : 0;
+  const int co = range == 0 ? (128 << sh) : (1 << (L.bit_depth - 1));
+  if (matrix == 3) {
+    m[0] = m[2] = m[5] = 65536;
+  } else {
+    if (matrix == 2) {
+      const int v[7] = {76293, 109969, 76293, -12255, -42598, 76293, 121634};
+      std::memcpy(m, v, sizeof(v));
+    } else {
+      const int v[7] = {76293, 117504, 76293, -13926, -34937, 76293, 138412};
+      std::memcpy(m, v, sizeof(v));
+    }
+    o[0] = (1 << 15) - m[0] * yo - m[1] * co;
+    o[1] = (1 << 15) - m[2] * yo - m[3] * co - m[4] * co;
+    o[2] = (1 << 15) - m[5] * yo - m[6] * co;
+  }
+  int rc = ensure_pack(c, need);
+  if (rc != AV1H_OK) return rc;
+  HIP_OK(c, launch_rgb(L, src, c->pack, m, o, c->stream));
+  HIP_OK(c, hipMemcpyAsync(out, c->pack, need, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(c, hipStreamSynchronize(c->stream));
+  return AV1H_OK;
+}
+
+int av1h_read_slot(av1h_ctx* h, int slot, const av1h_planes* out, av1h_frame_info* info) {
+  if (!h || !out) return AV1H_ERR_INVALID;
+  Ctx* c = &h->c;
+  std::lock_guard<std::mutex> lk(c->mu);
+  (void)hipSetDevice(c->device);
+  Layout L;
+  const uint16_t* src = gather(c, slot, &L);
+  if (!src) return AV1H_ERR_INVALID;
+  for (int p = 0; p < 3; ++p)
+    HIP_OK(c, hipMemcpy2DAsync(out->plane[p], (size_t)out->stride[p] * 2, src + L.plane_off[p], (size_t)L.stride[p] * 2,
+                               (size_t)L.out_w[p] * 2, L.out_h[p], hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(c, hipStreamSynchronize(c->stream));
+  if (info) {
+    info->width = L.width;
+    info->height = L.height;
+    info->bit_depth = L.bit_depth;
+    info->ss_x = L.ss_x;
+    info->ss_y = L.ss_y;
+  }
+  return AV1H_OK;
+}
+
+void av1h_reset(av1h_ctx* h) {
+  if (!h) return;
+  Ctx* c = &h->c;
+  std::lock_guard<std::mutex> lk(c->mu);
+  (void)hipSetDevice(c->device);
+  std::memset(c->used, 0, sizeof(c->used));
+  c->last_slot = -1;
+  c->have_frame = false;
+  if (c->frame) (void)hipMemsetAsync(c->frame, 0, c->cap * 2, c->stream);
+  if (c->pool) (void)hipMemsetAsync(c->pool, 0, c->cap * 2 * kSlots, c->stream);
+  (void)hipStreamSynchronize(c->stream);
+  std::memset(&c->stats, 0, sizeof(c->stats));
+}
+
+int av1h_get_stats(const av1h_ctx* h, av1h_stats* out) {
+  if (!h || !out) return AV1H_ERR_INVALID;
+  *out = h->c.stats;
+  return AV1H_OK;
+}
+
+int av1h_decode_batch(av1h_ctx* const* ctxs, const av1h_frame_desc* const* frames, int count, int* slots_out) {
+  if (!ctxs || !frames || count < 0) return AV1H_ERR_INVALID;
+  for (int i = 0; i < count; ++i) {
+    const int rc = av1h_decode(ctxs[i], frames[i], 1, -1, nullptr, nullptr, slots_out ? &slots_out[i] : nullptr);
+    if (rc != AV1H_OK) return rc;
+  }
+  return AV1H_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,294 @@
+/*
+ * Film grain synthesis (specification 7.18.3) on gfx950.
+ *
+ * The reference synthesises the grain on the host every frame and uploads a
+ * full-frame noise image (src/core/grain.cpp:70-139, 6.2 MB at 1080p). Here:
+ *   k_grain_template - one workgroup builds the three grain templates
+ *     (82x73 luma, 44x38 chroma at 4:2:0): the Gaussian white noise with the
+ *     16 bit LFSR jumped ahead per row (so every row starts in parallel), then
+ *     the auto-regressive filter as a wavefront over rows (a row lags its
+ *     predecessor by four columns, which is all the lag-3 filter reads), and
+ *     the per stripe block offsets;
+ *   k_grain_apply - the noise sample of every picture sample is derived on
+ *     the fly from the templates (the block and stripe overlap blends of
+ *     av1gpu_kernels.h:3218-3291 evaluated per sample) and blended in, chroma
+ *     before luma inside each tile because chroma reads the luma before grain.
+ * The arithmetic is the reference's (av1gpu_kernels.h:3039-3365).
+ */
+#define AV1T_TABLE_QUAL static __constant__ const
+#include "av1hip_internal.hpp"
+#include "spec_tables.h"
+
+namespace av1hip {
+namespace {
+
+#define SPEC(off) (av1t_spec_table + (off))
+
+constexpr int kLW = 82, kLH = 73;
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+__device__ __forceinline__ int grnd(int v, int n) { return n <= 0 ? v : (v + (1 << (n - 1))) >> n; }
+
+__device__ __forceinline__ unsigned lfsr_step(unsigned r) {
+  const unsigned bit = (r ^ (r >> 1) ^ (r >> 3) ^ (r >> 12)) & 1u;
+  return (r >> 1) | (bit << 15);
+}
+/* apply a 16x16 GF(2) matrix given by its columns */
+__device__ __forceinline__ unsigned mat_apply(const uint16_t* cols, unsigned v) {
+  unsigned r = 0;
+  for (int j = 0; j < 16; ++j)
+    if (v & (1u << j)) r ^= cols[j];
+  return r;
+}
+
+struct Jump {
+  uint16_t m82[16], m44[16];
+};
+
+/* white noise row `y` of a w wide template (av1gpu_kernels.h:3070-3079) */
+__device__ void noise_row(int16_t* t, int w, int y, unsigned seed, const uint16_t* jump, int shift) {
+  unsigned st = seed;
+  for (int k = 0; k < y; ++k) st = mat_apply(jump, st);
+  const int* gs = SPEC(AV1T_OFF_GAUSSIAN);
+  for (int x = 0; x < w; ++x) {
+    st = lfsr_step(st);
+    t[y * w + x] = (int16_t)grnd(gs[(st >> 5) & 2047], shift);
+  }
+}
+
+/* one step of the auto-regressive filter for sample (x, y)
+ * (av1gpu_kernels.h:3087-3122) */
+__device__ void ar_sample(const DevGrain& g, int16_t* pl, int w, int x, int y, const int* coeffs,
+                          const int16_t* luma, int ss_x, int ss_y, int lo, int hi) {
+  const int lag = g.lag;
+  int sum = 0, pos = 0;
+  for (int dr = -lag; dr <= 0; ++dr)
+    for (int dc = -lag; dc <= lag; ++dc) {
+      if (dr == 0 && dc == 0) {
+        if (luma && g.num_y > 0) {
+          const int lx = ((x - 3) << ss_x) + 3, ly = ((y - 3) << ss_y) + 3;
+          int acc = 0;
+          for (int i = 0; i <= ss_y; ++i)
+            for (int k = 0; k <= ss_x; ++k) acc += luma[(ly + i) * kLW + lx + k];
+          sum += grnd(acc, ss_x + ss_y) * coeffs[pos];
+        }
+        dr = 1;
+        break;
+      }
+      sum += pl[(y + dr) * w + x + dc] * coeffs[pos];
+      ++pos;
+    }
+  pl[y * w + x] = (int16_t)clampi(pl[y * w + x] + grnd(sum, g.shift), lo, hi);
+}
+
+/* Templates and block offsets. `tmpl` receives luma (82x73) then cb and cr
+ * (cw x ch each); `offs` the 8 bit offset draw of every (stripe, block). */
+__global__ __launch_bounds__(256) void k_grain_template(const DevGrain* gp, Jump jmp, int ss_x, int ss_y,
+                                                         int16_t* tmpl, uint8_t* offs) {
+  __shared__ int16_t s_l[kLW * kLH];
+  __shared__ int16_t s_c[2][kLW * kLH];
+  const DevGrain& g = *gp;
+  const int tid = threadIdx.x;
+  const int bd = g.bit_depth;
+  const int lo = -(128 << (bd - 8)), hi = (256 << (bd - 8)) - 1 - (128 << (bd - 8));
+  const int cw = ss_x ? 44 : kLW, ch = ss_y ? 38 : kLH;
+  const int chroma = g.cfl || g.num_cb > 0 || g.num_cr > 0;
+  const unsigned seed = (unsigned)g.seed;
+  /* white noise, one row per thread */
+  if (tid < kLH) {
+    if (g.num_y > 0)
+      noise_row(s_l, kLW, tid, seed, jmp.m82, g.scale_shift);
+    else
+      for (int x = 0; x < kLW; ++x) s_l[tid * kLW + x] = 0;
+  }
+  if (chroma) {
+    const uint16_t* jc = ss_x ? jmp.m44 : jmp.m82;
+    if (tid >= 96 && tid < 96 + ch) {
+      const int y = tid - 96;
+      if (g.num_cb > 0 || g.cfl)
+        noise_row(s_c[0], cw, y, seed ^ 0xb524u, jc, g.scale_shift);
+      else
+        for (int x = 0; x < cw; ++x) s_c[0][y * cw + x] = 0;
+    }
+    if (tid >= 176 && tid < 176 + ch) {
+      const int y = tid - 176;
+      if (g.num_cr > 0 || g.cfl)
+        noise_row(s_c[1], cw, y, seed ^ 0x49d8u, jc, g.scale_shift);
+      else
+        for (int x = 0; x < cw; ++x) s_c[1][y * cw + x] = 0;
+    }
+  }
+  __syncthreads();
+  /* luma auto-regression: row y handles column x at step (x - 3) + 4 (y - 3) */
+  if (g.num_y > 0) {
+    const int steps = (kLW - 7) + 4 * (kLH - 4) + 1;
+    for (int s = 0; s < steps; ++s) {
+      if (tid < kLH - 3) {
+        const int y = 3 + tid, x = s - 4 * tid + 3;
+        if (x >= 3 && x < kLW - 3) ar_sample(g, s_l, kLW, x, y, g.ar_y, nullptr, ss_x, ss_y, lo, hi);
+      }
+      __syncthreads();
+    }
+  }
+  if (chroma) {
+    const int steps = (cw - 7) + 4 * (ch - 4) + 1;
+    for (int s = 0; s < steps; ++s) {
+      const int pl = tid >> 7, r = tid & 127;
+      if (r < ch - 3) {
+        const int y = 3 + r, x = s - 4 * r + 3;
+        if (x >= 3 && x < cw - 3)
+          ar_sample(g, s_c[pl], cw, x, y, pl ? g.ar_cr : g.ar_cb, s_l, ss_x, ss_y, lo, hi);
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = tid; i < kLW * kLH; i += 256) tmpl[i] = s_l[i];
+  if (chroma)
+    for (int i = tid; i < cw * ch; i += 256) {
+      tmpl[kLW * kLH + i] = s_c[0][i];
+      tmpl[kLW * kLH + cw * ch + i] = s_c[1][i];
+    }
+  /* per stripe block offsets (av1gpu_kernels.h:3236-3247) */
+  for (int s = tid; s < g.stripes; s += 256) {
+    unsigned st = seed;
+    st ^= (unsigned)(((s * 37 + 178) & 255) << 8);
+    st ^= (unsigned)((s * 173 + 105) & 255);
+    for (int b = 0; b < g.blocks; ++b) {
+      st = lfsr_step(st);
+      offs[s * g.blocks + b] = (uint8_t)((st >> 8) & 255);
+    }
+  }
+}
+
+struct NoiseCtx {
+  const int16_t* t;   /* template of this plane */
+  const uint8_t* offs;
+  int tw, blocks, sx, sy, overlap, lo, hi;
+  __device__ __forceinline__ int G(int s, int b, int i, int j) const {
+    const int r = offs[s * blocks + b];
+    const int ox = r >> 4, oy = r & 15;
+    const int gx = sx ? 6 + ox : 9 + 2 * ox, gy = sy ? 6 + oy : 9 + 2 * oy;
+    return t[(gy + i) * tw + gx + j];
+  }
+  /* the stripe buffer after the horizontal overlap blends */
+  __device__ __forceinline__ int H(int s, int i, int x) const {
+    const int bw = 32 >> sx;
+    const int b = x / bw, j = x - b * bw;
+    int g = G(s, b, i, j);
+    if (overlap && b > 0 && j < (sx ? 1 : 2)) {
+      const int old = G(s, b - 1, i, j + bw);
+      g = clampi(grnd(sx ? old * 23 + g * 22 : (j == 0 ? old * 27 + g * 17 : old * 17 + g * 27), 5), lo, hi);
+    }
+    return g;
+  }
+  /* the noise image sample after the vertical blends */
+  __device__ __forceinline__ int N(int x, int y) const {
+    const int bh = 32 >> sy;
+    const int s = y / bh, row = y - s * bh;
+    int v = H(s, row, x);
+    if (overlap && s > 0 && row < (sy ? 1 : 2)) {
+      const int old = H(s - 1, row + bh, x);
+      v = clampi(grnd(sy ? old * 23 + v * 22 : (row == 0 ? old * 27 + v * 17 : old * 17 + v * 27), 5), lo, hi);
+    }
+    return v;
+  }
+};
+
+__device__ __forceinline__ int scale_lut(const int* lut, int index, int bd) {
+  const int sh = bd - 8;
+  const int x = sh > 0 ? index >> sh : index;
+  if (bd == 8 || x >= 255) return lut[x > 255 ? 255 : x];
+  const int rem = index - (x << sh);
+  return lut[x] + grnd((lut[x + 1] - lut[x]) * rem, sh);
+}
+
+/* 64x32 luma tiles (32x16 chroma at 4:2:0), 256 threads */
+__global__ __launch_bounds__(256) void k_grain_apply(Layout L, const DevGrain* gp, const int16_t* tmpl,
+                                                      const uint8_t* offs, uint16_t* frame) {
+  const DevGrain& g = *gp;
+  const int bd = L.bit_depth;
+  const int maxv = (1 << bd) - 1;
+  const int lo = -(128 << (bd - 8)), hi = (256 << (bd - 8)) - 1 - (128 << (bd - 8));
+  const int tx0 = blockIdx.x * 64, ty0 = blockIdx.y * 32;
+  const int cw = L.ss_x ? 44 : kLW, ch = L.ss_y ? 38 : kLH;
+  const int chroma = g.cfl || g.num_cb > 0 || g.num_cr > 0;
+  const uint16_t* luma = frame + L.plane_off[0];
+  const int ls = L.stride[0];
+  for (int pl = 1; pl < 3 && chroma; ++pl) {
+    if (!g.cfl && (pl == 1 ? g.num_cb : g.num_cr) <= 0) continue;
+    NoiseCtx nc{tmpl + kLW * kLH + (pl - 1) * cw * ch, offs, cw, g.blocks, L.ss_x, L.ss_y, g.overlap, lo, hi};
+    const int pw = L.out_w[pl], ph = L.out_h[pl];
+    const int x0 = tx0 >> L.ss_x, y0 = ty0 >> L.ss_y;
+    const int tw = 64 >> L.ss_x, th = 32 >> L.ss_y;
+    const int* lut = g.lut + 256 * pl;
+    const int mult = pl == 1 ? g.cb_mult : g.cr_mult;
+    const int lmult = pl == 1 ? g.cb_luma_mult : g.cr_luma_mult;
+    const int offset = pl == 1 ? g.cb_offset : g.cr_offset;
+    uint16_t* plane = frame + L.plane_off[pl];
+    const int st = L.stride[pl];
+    const int lw = L.out_w[0];
+    for (int i = threadIdx.x; i < tw * th; i += 256) {
+      const int x = x0 + (i % tw), y = y0 + (i / tw);
+      if (x >= pw || y >= ph) continue;
+      const int orig = plane[(size_t)y * st + x];
+      const int lx = x << L.ss_x, ly = y << L.ss_y;
+      const int nx = lx + 1 < lw ? lx + 1 : lw - 1;
+      int avg = luma[(size_t)ly * ls + lx];
+      if (L.ss_x) avg = grnd(avg + luma[(size_t)ly * ls + nx], 1);
+      int index;
+      if (g.cfl) {
+        index = avg;
+      } else {
+        const int comb = avg * (lmult - 128) + orig * (mult - 128);
+        index = clampi((comb >> 6) + (offset - 256) * (1 << (bd - 8)), 0, maxv);
+      }
+      const int n = nc.N(x, y);
+      const int gv = grnd(scale_lut(lut, index, bd) * n, g.scaling_shift);
+      plane[(size_t)y * st + x] = (uint16_t)clampi(orig + gv, g.min_value, g.max_chroma);
+    }
+  }
+  __syncthreads();
+  if (g.num_y > 0) {
+    NoiseCtx nc{tmpl, offs, kLW, g.blocks, 0, 0, g.overlap, lo, hi};
+    uint16_t* plane = frame + L.plane_off[0];
+    for (int i = threadIdx.x; i < 64 * 32; i += 256) {
+      const int x = tx0 + (i & 63), y = ty0 + (i >> 6);
+      if (x >= L.out_w[0] || y >= L.out_h[0]) continue;
+      const int orig = plane[(size_t)y * ls + x];
+      const int n = nc.N(x, y);
+      const int gv = grnd(scale_lut(g.lut, orig, bd) * n, g.scaling_shift);
+      plane[(size_t)y * ls + x] = (uint16_t)clampi(orig + gv, g.min_value, g.max_luma);
+    }
+  }
+}
+
+unsigned host_step(unsigned r) {
+  const unsigned bit = (r ^ (r >> 1) ^ (r >> 3) ^ (r >> 12)) & 1u;
+  return (r >> 1) | (bit << 15);
+}
+
+}  // namespace
+
+void grain_jump_matrices(uint16_t m82[16], uint16_t m44[16]) {
+  for (int j = 0; j < 16; ++j) {
+    unsigned a = 1u << j, b = 1u << j;
+    for (int k = 0; k < 82; ++k) a = host_step(a);
+    for (int k = 0; k < 44; ++k) b = host_step(b);
+    m82[j] = (uint16_t)a;
+    m44[j] = (uint16_t)b;
+  }
+}
+
+hipError_t launch_grain(const Layout& lay, const DevGrain* g, const DevGrain& hg, uint16_t* frame,
+                        int16_t* templates, uint8_t* offsets, hipStream_t s, int* launches) {
+  Jump j;
+  grain_jump_matrices(j.m82, j.m44);
+  hipLaunchKernelGGL(k_grain_template, dim3(1), dim3(256), 0, s, g, j, lay.ss_x, lay.ss_y, templates, offsets);
+  hipLaunchKernelGGL(k_grain_apply, dim3((lay.out_w[0] + 63) / 64, (lay.out_h[0] + 31) / 32), dim3(256), 0, s,
+                     lay, g, templates, offsets, frame);
+  *launches += 2;
+  (void)hg;
+  return hipGetLastError();
+}
+
+}  // namespace av1hip

@@ -1,0 +1,182 @@
+/*
+ * Internal definitions shared by the host side and the gfx950 kernels of
+ * libav1hip.so: the device frame layout, the compact per frame work
+ * descriptors the host flattens FrameSyntax into, and the kernel launchers.
+ */
+#ifndef AV1HIP_INTERNAL_HPP
+#define AV1HIP_INTERNAL_HPP
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "av1hip.h"
+
+namespace av1hip {
+
+constexpr int kSlots = 32;          /* reference pictures per context          */
+constexpr int kWavesPerGroup = 2;   /* recon kernels: independent waves per WG  */
+
+/* ---- frame layout --------------------------------------------------------
+ * One flat buffer of 16 bit samples, the three planes back to back. The stride
+ * follows the display width plus a border of 64 luma / 32 chroma samples, and
+ * is rounded up to 128 samples (256 bytes) so rows start on cache lines. */
+struct Layout {
+  int width, height, bit_depth, ss_x, ss_y;
+  int plane_w[3], plane_h[3]; /* coded visible samples            */
+  int out_w[3], out_h[3];     /* display samples                  */
+  int grid_w[3], grid_h[3];   /* coded size rounded up to 8 luma  */
+  int stride[3];
+  int64_t plane_off[3];
+  int64_t size;               /* samples                          */
+  int cell_shift[3];          /* dependency cell size per plane   */
+  int cells_w[3], cells_h[3];
+  int64_t cell_off[3];
+  int64_t cells;
+};
+
+void make_layout(Layout* g, int width, int height, int bit_depth, int ss_x, int ss_y,
+                 int coded_width);
+
+/* ---- per frame descriptors --------------------------------------------------- */
+
+enum BlockKind { kInter = 0, kIntra = 1 };
+
+/* flag bits of DevBlock::flags */
+enum {
+  kHaveLeft = 1, kHaveAbove = 2, kHaveAR = 4, kHaveBL = 8, kFilterIntra = 16,
+  kPalette = 32, kLossless = 64, kResidualRaw = 128, kInterIntra = 256,
+  kIIWedge = 512, kWarp0 = 1024, kWarp1 = 2048, kMaskWarp0 = 4096, kMaskWarp1 = 8192,
+  kIntraBC = 16384
+};
+
+struct DevBlock {
+  int32_t x, y, w, h;
+  int32_t plane, kind, flags, mode;
+  int32_t angle, filter_type, fi_mode, cfl_alpha;
+  int32_t tx_first, tx_count, obmc_first, obmc_count;
+  int32_t slot[2];       /* pool slots, -1: the frame being decoded              */
+  int32_t mv[2][2];
+  int32_t filt, filt_v, compound, w0, w1;
+  int32_t ref_w, ref_h;  /* visible reference size, for clamping                */
+  int32_t scale_x, scale_y;
+  int32_t mask_kind, mask_sign, wedge_dir, wedge_xoff, wedge_yoff, wedge_flip;
+  int32_t ii_mode;
+  int32_t palette_off, palette_stride;
+  int32_t palette[8];
+  int32_t warp_params[2][6];
+  int32_t warp_abcd[2][4];
+  /* the cells this block must wait for (the reference scheduler's read set):
+   * left column rows [dep_y0, dep_y1], above row columns [dep_x0, dep_x1],
+   * the corner when dep_corner; dep_x1 < x and dep_y1 < y disable a side */
+  int32_t dep_x1, dep_y1, dep_corner;
+  int32_t pad;
+};
+
+struct DevTx {
+  int32_t x, y, w, h;   /* in the plane                      */
+  int32_t coef;         /* raster offset in the coefficient (or residual) pool */
+  int32_t tx_type;
+  int32_t row_shift;
+  int32_t eob;
+};
+
+struct DevObmc {
+  int32_t pass, rx, ry, w, h, mvx, mvy, filt, filt_v, slot;
+};
+
+/* one line of one deblocking edge: the four samples p1..q1 plus the taps the
+ * filter may read, at `p1` + k * cross */
+struct DevDeblock {
+  int32_t plane, x, y;   /* q0 of the first line                */
+  int32_t vertical, len, level, filter_size;
+  int32_t line0;         /* prefix: first global line index   */
+};
+
+struct DevCdef {
+  int32_t plane, x, y, pri, sec, damping, dir, pad;
+};
+
+struct DevLr {
+  int32_t plane, x, y, w, h;
+  int32_t stripe0, stripe1, type;
+  int32_t vf[3], hf[3];
+  int32_t r[2], eps[2], xqd[2];
+  int32_t px0;           /* prefix: first output sample index */
+};
+
+/* grain parameters the kernels read */
+struct DevGrain {
+  int32_t seed, bit_depth, num_y, num_cb, num_cr, cfl;
+  int32_t lag, shift, scale_shift, scaling_shift, overlap;
+  int32_t cb_mult, cb_luma_mult, cb_offset, cr_mult, cr_luma_mult, cr_offset;
+  int32_t min_value, max_luma, max_chroma;
+  int32_t ar_y[24], ar_cb[25], ar_cr[25];
+  int32_t lut[768];
+  int32_t stripes, blocks; /* offsets table size                 */
+};
+
+/* The frame plan as the kernels see it: pointers into one device buffer. */
+struct FramePlanDev {
+  const DevBlock* blocks;
+  const DevTx* tx;
+  const DevObmc* obmc;
+  const int32_t* inter_list;
+  const int32_t* queue_list;
+  const int32_t* coef;
+  const int32_t* residual;
+  const uint8_t* palette;
+  const DevDeblock* deblock;  /* vertical edges first, then horizontal */
+  const DevCdef* cdef;
+  const DevLr* lr;
+  const DevGrain* grain;
+  int32_t n_inter, n_queue, n_deblock_v, n_deblock_h, lines_v, lines_h, n_cdef, n_lr, lr_px;
+};
+
+/* launchers (one per translation unit, all on the context's stream) */
+struct ReconArgs {
+  FramePlanDev plan;
+  Layout lay;
+  uint16_t* frame;
+  const uint16_t* pool;
+  int64_t slot_size;
+  uint32_t* cells;      /* epoch per dependency cell          */
+  uint32_t epoch;
+  uint32_t* queue_counter;
+  uint32_t* error;      /* bit 0: a dependency wait timed out  */
+  int maxv;
+  int edge_filter;      /* enable_intra_edge_filter           */
+};
+
+hipError_t launch_recon(const ReconArgs& a, int cus, hipStream_t s, int* launches);
+
+struct FilterArgs {
+  FramePlanDev plan;
+  Layout lay;
+  uint16_t* frame;
+  uint16_t* scratch;
+  uint16_t* lr_curr;
+  uint16_t* lr_cdef;
+  uint16_t* lr_prev_up;
+  int sharpness;
+  int maxv;
+};
+
+hipError_t launch_deblock(const FilterArgs& a, hipStream_t s, int* launches);
+hipError_t launch_cdef(const FilterArgs& a, hipStream_t s, int* launches);
+hipError_t launch_superres(const FilterArgs& a, const uint16_t* src, uint16_t* dst, hipStream_t s,
+                           int* launches);
+hipError_t launch_lr(const FilterArgs& a, const uint16_t* cdef, const uint16_t* curr, hipStream_t s,
+                     int* launches);
+hipError_t launch_grain(const Layout& lay, const DevGrain* g, const DevGrain& host_g,
+                        uint16_t* frame, int16_t* templates, uint8_t* offsets, hipStream_t s,
+                        int* launches);
+hipError_t launch_nv12(const Layout& lay, const uint16_t* src, uint8_t* out, int pitch, hipStream_t s);
+hipError_t launch_rgb(const Layout& lay, const uint16_t* src, uint8_t* out, const int m[8],
+                      const int o[3], hipStream_t s);
+
+/* film grain: the LFSR jump matrices the template kernel uses */
+void grain_jump_matrices(uint16_t m82[16], uint16_t m44[16]);
+
+}  // namespace av1hip
+
+#endif

@@ -356,6 +356,24 @@ int av1h_decode(av1h_ctx* ctx, const av1h_frame_desc* frame, int keep_reference,
                 int reserved_slot, const av1h_planes* out, av1h_frame_info* info,
                 int* slot_out);
 
+/* av1h_decode of a wire frame (see the wire format above). */
+int av1h_decode_wire(av1h_ctx* ctx, const uint8_t* wire, size_t len, int keep_reference,
+                     int reserved_slot, const av1h_planes* out, av1h_frame_info* info,
+                     int* slot_out);
+
+/* Pre-planning (reference Decoder::plan_frame, av1gpu.hpp:670-689, and
+ * Decoder::decode_reference_planned, 700-709): av1h_plan_frame flattens the
+ * syntax and uploads it into device memory owned by the returned plan, so a
+ * pipelined caller can prepare frame N+1 while frame N decodes and the decode
+ * itself starts from HBM. A plan is consumed by nothing: decode it (any number
+ * of times) with av1h_decode_planned, and free it with av1h_discard_plan. The
+ * reference slots a plan names are resolved when it is decoded. */
+int av1h_plan_frame(av1h_ctx* ctx, const av1h_frame_desc* frame, void** plan_out);
+int av1h_plan_frame_wire(av1h_ctx* ctx, const uint8_t* wire, size_t len, void** plan_out);
+int av1h_decode_planned(av1h_ctx* ctx, void* plan, int keep_reference, int reserved_slot,
+                        const av1h_planes* out, av1h_frame_info* info, int* slot_out);
+void av1h_discard_plan(av1h_ctx* ctx, void* plan);
+
 /* Reserves a reference slot (reference Decoder::reserve_reference,
  * av1gpu.hpp:661). Returns the slot or a negative status. */
 int av1h_reserve_reference(av1h_ctx* ctx);
