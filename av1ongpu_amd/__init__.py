@@ -216,8 +216,20 @@ def hip_lib():
         lib.av1h_get_stats.argtypes = [P, ctypes.POINTER(Stats)]
         lib.av1h_decode_batch.argtypes = [ctypes.POINTER(P), ctypes.POINTER(P), ctypes.c_int,
                                           ctypes.POINTER(ctypes.c_int)]
+        lib.av1h_plan_frame_wire.argtypes = [P, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(P)]
+        lib.av1h_decode_planned.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Planes),
+                                            ctypes.POINTER(FrameInfo), ctypes.POINTER(ctypes.c_int)]
+        lib.av1h_discard_plan.argtypes = [P, P]
+        lib.av1h_set_stage_timing.argtypes = [P, ctypes.c_int]
+        lib.av1h_get_stage_times.argtypes = [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64),
+                                             ctypes.POINTER(ctypes.c_int64)]
+        lib.av1h_nv12_device_buffer.restype = ctypes.c_void_p
+        lib.av1h_nv12_device_buffer.argtypes = [P]
         _hip = lib
     return _hip
+
+
+STAGES = ("inter", "queue", "deblock", "cdef", "superres", "lr", "store", "grain")
 
 
 class HipError(RuntimeError):
@@ -260,6 +272,43 @@ class HipDecoder:
         if rc != AV1H_OK:
             raise HipError(f"decode failed ({rc}): {self.error()}")
         return slot.value, arrs
+
+    def plan_wire(self, wire):
+        """Flatten and upload a wire frame into a device-resident plan."""
+        p = ctypes.c_void_p()
+        rc = self._lib.av1h_plan_frame_wire(self._h, wire, len(wire), ctypes.byref(p))
+        if rc != AV1H_OK:
+            raise HipError(f"plan_frame failed ({rc}): {self.error()}")
+        return p
+
+    def decode_planned(self, plan, keep=True, reserved=-1):
+        """Decode a plan, picture left on the device; returns the slot."""
+        slot = ctypes.c_int(-1)
+        rc = self._lib.av1h_decode_planned(self._h, plan, 1 if keep else 0, reserved, None, None,
+                                           ctypes.byref(slot))
+        if rc != AV1H_OK:
+            raise HipError(f"decode_planned failed ({rc}): {self.error()}")
+        return slot.value
+
+    def discard_plan(self, plan):
+        self._lib.av1h_discard_plan(self._h, plan)
+
+    def set_stage_timing(self, enable=True):
+        self._lib.av1h_set_stage_timing(self._h, 1 if enable else 0)
+
+    def stage_times(self):
+        """{stage name: (accumulated device ms, decodes counted, compulsory bytes)}"""
+        ms = (ctypes.c_double * len(STAGES))()
+        n = (ctypes.c_int64 * len(STAGES))()
+        b = (ctypes.c_int64 * len(STAGES))()
+        self._lib.av1h_get_stage_times(self._h, ms, n, b)
+        return {name: (ms[i], n[i], b[i]) for i, name in enumerate(STAGES)}
+
+    def pack_nv12_device(self, slot, pitch):
+        """NV12 pack left on the device (no host copy)."""
+        rc = self._lib.av1h_to_nv12(self._h, slot, pitch, None, 0)
+        if rc != AV1H_OK:
+            raise HipError(f"to_nv12 failed ({rc}): {self.error()}")
 
     def reserve(self):
         return self._lib.av1h_reserve_reference(self._h)

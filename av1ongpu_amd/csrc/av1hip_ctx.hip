@@ -88,6 +88,7 @@ struct Plan {
   bool uses_lr = false, upscaled = false, has_grain = false;
   int blocks = 0;
   std::vector<int> slots; /* pool slots the frame reads */
+  int64_t alg_bytes[AV1H_STAGES] = {}; /* compulsory HBM bytes per stage */
 };
 
 struct Ctx {
@@ -120,6 +121,12 @@ struct Ctx {
   Plan scratch_plan;
   av1h_stats stats;
   hipEvent_t ev[4];
+  bool timing = false;
+  hipEvent_t sev[2 * AV1H_STAGES] = {};
+  bool stage_hit[AV1H_STAGES] = {};
+  double stage_ms[AV1H_STAGES] = {};
+  int64_t stage_n[AV1H_STAGES] = {};
+  int64_t stage_bytes[AV1H_STAGES] = {};
 };
 
 int set_err(Ctx* c, int code, const char* fmt, ...) {
@@ -591,6 +598,46 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
   plan->blocks = nb;
   plan->grain = H->grain;
 
+  /* compulsory HBM bytes of each stage at 16 bit samples: what a stage must
+   * read and write at least, independent of how the kernels tile it */
+  for (int64_t& v : plan->alg_bytes) v = 0;
+  {
+    auto recon_bytes = [&](const std::vector<int32_t>& list) {
+      int64_t sum = 0;
+      for (int32_t i : list) {
+        const DevBlock& d = H->blocks[i];
+        const int refs = d.kind == kInter ? (d.compound ? 2 : 1) : 0;
+        sum += (int64_t)d.w * d.h * (2 + 2 * refs + (d.tx_count > 0 ? 2 : 0));
+      }
+      return sum;
+    };
+    plan->alg_bytes[AV1H_STAGE_INTER] = recon_bytes(H->inter_list);
+    plan->alg_bytes[AV1H_STAGE_QUEUE] = recon_bytes(H->queue_list);
+    bool has_edges[3] = {false, false, false};
+    for (const DevDeblock& d : H->deblock) has_edges[d.plane] = true;
+    int64_t disp = 0;
+    for (int p = 0; p < 3; ++p) {
+      if (has_edges[p]) plan->alg_bytes[AV1H_STAGE_DEBLOCK] += 4 * (int64_t)L.plane_w[p] * L.plane_h[p];
+      if (plan->upscaled)
+        plan->alg_bytes[AV1H_STAGE_SUPERRES] +=
+            (plan->uses_lr ? 2 : 1) * 2 * (int64_t)(L.plane_w[p] + L.out_w[p]) * L.out_h[p];
+      disp += (int64_t)L.out_w[p] * L.out_h[p];
+    }
+    for (const DevCdef& d : H->cdef) {
+      const int sx = d.plane ? hdr.ss_x : 0, sy = d.plane ? hdr.ss_y : 0;
+      plan->alg_bytes[AV1H_STAGE_CDEF] += 4 * (int64_t)(8 >> sx) * (8 >> sy);
+    }
+    for (const DevLr& d : H->lr) plan->alg_bytes[AV1H_STAGE_LR] += 4 * (int64_t)d.w * d.h + 8 * (int64_t)d.w;
+    plan->alg_bytes[AV1H_STAGE_STORE] = 4 * disp;
+    if (plan->has_grain) {
+      const DevGrain& g = H->grain;
+      const bool chroma = g.cfl || g.num_cb > 0 || g.num_cr > 0;
+      plan->alg_bytes[AV1H_STAGE_GRAIN] =
+          (g.num_y > 0 ? 4 * (int64_t)L.out_w[0] * L.out_h[0] : 0) +
+          (chroma ? 8 * (int64_t)L.out_w[1] * L.out_h[1] : 0);
+    }
+  }
+
   /* one device buffer: offsets, then the arrays */
   struct Part {
     const void* src;
@@ -755,7 +802,26 @@ int run_plan(Ctx* c, Plan* plan, int keep, int reserved, const av1h_planes* out,
   ra.error = c->counters + 1;
   ra.maxv = (1 << L.bit_depth) - 1;
   ra.edge_filter = plan->hdr.enable_intra_edge_filter;
-  HIP_OK(c, launch_recon(ra, c->cus, s, &launches));
+  for (bool& h : c->stage_hit) h = false;
+  auto begin = [&](int st) {
+    if (c->timing) (void)hipEventRecord(c->sev[2 * st], s);
+  };
+  auto end = [&](int st) {
+    if (c->timing) {
+      (void)hipEventRecord(c->sev[2 * st + 1], s);
+      c->stage_hit[st] = true;
+    }
+  };
+  if (plan->dev.n_inter > 0) {
+    begin(AV1H_STAGE_INTER);
+    HIP_OK(c, launch_recon(ra, 0, c->cus, s, &launches));
+    end(AV1H_STAGE_INTER);
+  }
+  if (plan->dev.n_queue > 0) {
+    begin(AV1H_STAGE_QUEUE);
+    HIP_OK(c, launch_recon(ra, 1, c->cus, s, &launches));
+    end(AV1H_STAGE_QUEUE);
+  }
   HIP_OK(c, hipEventRecord(c->ev[1], s));
   FilterArgs fa;
   fa.plan = plan->dev;
@@ -767,23 +833,36 @@ int run_plan(Ctx* c, Plan* plan, int keep, int reserved, const av1h_planes* out,
   fa.lr_prev_up = c->lr_prev_up;
   fa.sharpness = plan->hdr.sharpness;
   fa.maxv = ra.maxv;
-  HIP_OK(c, launch_deblock(fa, s, &launches));
+  if (plan->dev.lines_v + plan->dev.lines_h > 0) {
+    begin(AV1H_STAGE_DEBLOCK);
+    HIP_OK(c, launch_deblock(fa, s, &launches));
+    end(AV1H_STAGE_DEBLOCK);
+  }
   if (plan->uses_lr) HIP_OK(c, hipMemcpyAsync(c->lr_curr, c->frame, bytes, hipMemcpyDeviceToDevice, s));
   if (plan->dev.n_cdef > 0) {
     HIP_OK(c, hipMemcpyAsync(c->scratch, c->frame, bytes, hipMemcpyDeviceToDevice, s));
+    begin(AV1H_STAGE_CDEF);
     HIP_OK(c, launch_cdef(fa, s, &launches));
+    end(AV1H_STAGE_CDEF);
   }
   if (plan->upscaled) {
     HIP_OK(c, hipMemcpyAsync(c->scratch, c->frame, bytes, hipMemcpyDeviceToDevice, s));
+    begin(AV1H_STAGE_SUPERRES);
     HIP_OK(c, launch_superres(fa, c->scratch, c->frame, s, &launches));
     if (plan->uses_lr) HIP_OK(c, launch_superres(fa, c->lr_curr, c->lr_prev_up, s, &launches));
+    end(AV1H_STAGE_SUPERRES);
   }
   if (plan->uses_lr) {
     HIP_OK(c, hipMemcpyAsync(c->lr_cdef, c->frame, bytes, hipMemcpyDeviceToDevice, s));
+    begin(AV1H_STAGE_LR);
     HIP_OK(c, launch_lr(fa, c->lr_cdef, plan->upscaled ? c->lr_prev_up : c->lr_curr, s, &launches));
+    end(AV1H_STAGE_LR);
   }
-  if (slot >= 0)
+  if (slot >= 0) {
+    begin(AV1H_STAGE_STORE);
     HIP_OK(c, hipMemcpyAsync(c->pool + (size_t)slot * L.size, c->frame, bytes, hipMemcpyDeviceToDevice, s));
+    end(AV1H_STAGE_STORE);
+  }
   if (plan->has_grain) {
     const size_t need = (size_t)plan->grain.stripes * plan->grain.blocks + 64;
     if (c->grain_offs_cap < need) {
@@ -793,7 +872,9 @@ int run_plan(Ctx* c, Plan* plan, int keep, int reserved, const av1h_planes* out,
       HIP_OK(c, hipMalloc(&c->grain_offs, need));
       c->grain_offs_cap = need;
     }
+    begin(AV1H_STAGE_GRAIN);
     HIP_OK(c, launch_grain(L, plan->dev.grain, plan->grain, c->frame, c->grain_tmpl, c->grain_offs, s, &launches));
+    end(AV1H_STAGE_GRAIN);
   }
   HIP_OK(c, hipEventRecord(c->ev[2], s));
   const double t_sub = now_ms();
@@ -842,6 +923,16 @@ int run_plan(Ctx* c, Plan* plan, int keep, int reserved, const av1h_planes* out,
     info->ss_x = L.ss_x;
     info->ss_y = L.ss_y;
   }
+  if (c->timing)
+    for (int st = 0; st < AV1H_STAGES; ++st)
+      if (c->stage_hit[st]) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, c->sev[2 * st], c->sev[2 * st + 1]) == hipSuccess) {
+          c->stage_ms[st] += ms;
+          ++c->stage_n[st];
+          c->stage_bytes[st] += plan->alg_bytes[st];
+        }
+      }
   float recon = 0, filt = 0;
   (void)hipEventElapsedTime(&recon, c->ev[0], c->ev[1]);
   (void)hipEventElapsedTime(&filt, c->ev[1], c->ev[2]);
@@ -942,15 +1033,16 @@ void av1h_destroy(av1h_ctx* h) {
   free_plan(&c->scratch_plan);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto& e : c->sev)
+    if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete h;
 }
 
 const char* av1h_last_error(const av1h_ctx* h) { return h ? h->c.err.c_str() : "no context"; }
 
-int av1h_get_device_info(av1h_ctx* h, av1h_device_info* out) {
-  if (!h || !out) return AV1H_ERR_INVALID;
-  const hipDeviceProp_t& p = h->c.prop;
+namespace {
+void describe(const hipDeviceProp_t& p, av1h_device_info* out) {
   std::memset(out, 0, sizeof(*out));
   std::snprintf(out->name, sizeof(out->name), "%s", p.name);
   std::snprintf(out->arch, sizeof(out->arch), "%s", p.gcnArchName);
@@ -960,7 +1052,23 @@ int av1h_get_device_info(av1h_ctx* h, av1h_device_info* out) {
   out->clock_khz = p.clockRate;
   out->mem_clock_khz = p.memoryClockRate;
   out->mem_bus_width = p.memoryBusWidth;
+}
+}  // namespace
+
+int av1h_get_device_info(av1h_ctx* h, av1h_device_info* out) {
+  if (!h || !out) return AV1H_ERR_INVALID;
+  describe(h->c.prop, out);
   return AV1H_OK;
+}
+
+int av1h_describe_device(int device_index, av1h_device_info* out) {
+  if (!out) return AV1H_ERR_INVALID;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device_index < 0 || device_index >= n) return AV1H_ERR_NO_DEVICE;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device_index) != hipSuccess) return AV1H_ERR_NO_DEVICE;
+  describe(prop, out);
+  return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0 ? AV1H_OK : AV1H_ERR_UNSUPPORTED;
 }
 
 int av1h_decode(av1h_ctx* h, const av1h_frame_desc* f, int keep, int reserved, const av1h_planes* out,
@@ -1064,7 +1172,7 @@ size_t av1h_nv12_size(int width, int height, int ss_x, int ss_y, size_t pitch) {
 }
 
 int av1h_to_nv12(av1h_ctx* h, int slot, size_t pitch, uint8_t* out, size_t out_bytes) {
-  if (!h || !out) return AV1H_ERR_INVALID;
+  if (!h) return AV1H_ERR_INVALID;
   Ctx* c = &h->c;
   std::lock_guard<std::mutex> lk(c->mu);
   (void)hipSetDevice(c->device);
@@ -1074,14 +1182,16 @@ int av1h_to_nv12(av1h_ctx* h, int slot, size_t pitch, uint8_t* out, size_t out_b
   const size_t need = av1h_nv12_size(L.width, L.height, L.ss_x, L.ss_y, pitch);
   if (need == 0)
     return set_err(c, AV1H_ERR_INVALID, "NV12 output needs a 4:2:0 frame and an even pitch at least as wide");
-  if (out_bytes < need) return set_err(c, AV1H_ERR_INVALID, "output buffer too small");
+  if (out && out_bytes < need) return set_err(c, AV1H_ERR_INVALID, "output buffer too small");
   int rc = ensure_pack(c, need);
   if (rc != AV1H_OK) return rc;
   HIP_OK(c, launch_nv12(L, src, c->pack, (int)pitch, c->stream));
-  HIP_OK(c, hipMemcpyAsync(out, c->pack, need, hipMemcpyDeviceToHost, c->stream));
+  if (out) HIP_OK(c, hipMemcpyAsync(out, c->pack, need, hipMemcpyDeviceToHost, c->stream));
   HIP_OK(c, hipStreamSynchronize(c->stream));
   return AV1H_OK;
 }
+
+const uint8_t* av1h_nv12_device_buffer(const av1h_ctx* h) { return h ? h->c.pack : nullptr; }
 
 int av1h_to_rgb(av1h_ctx* h, int slot, int matrix, int range, uint8_t* out, size_t out_bytes) {
   if (!h || !out) return AV1H_ERR_INVALID;
@@ -1121,17 +1231,20 @@ int av1h_to_rgb(av1h_ctx* h, int slot, int matrix, int range, uint8_t* out, size
 }
 
 int av1h_read_slot(av1h_ctx* h, int slot, const av1h_planes* out, av1h_frame_info* info) {
-  if (!h || !out) return AV1H_ERR_INVALID;
+  if (!h) return AV1H_ERR_INVALID;
   Ctx* c = &h->c;
   std::lock_guard<std::mutex> lk(c->mu);
   (void)hipSetDevice(c->device);
   Layout L;
   const uint16_t* src = gather(c, slot, &L);
   if (!src) return AV1H_ERR_INVALID;
-  for (int p = 0; p < 3; ++p)
-    HIP_OK(c, hipMemcpy2DAsync(out->plane[p], (size_t)out->stride[p] * 2, src + L.plane_off[p], (size_t)L.stride[p] * 2,
-                               (size_t)L.out_w[p] * 2, L.out_h[p], hipMemcpyDeviceToHost, c->stream));
-  HIP_OK(c, hipStreamSynchronize(c->stream));
+  if (out) {
+    for (int p = 0; p < 3; ++p)
+      HIP_OK(c, hipMemcpy2DAsync(out->plane[p], (size_t)out->stride[p] * 2, src + L.plane_off[p],
+                                 (size_t)L.stride[p] * 2, (size_t)L.out_w[p] * 2, L.out_h[p],
+                                 hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+  }
   if (info) {
     info->width = L.width;
     info->height = L.height;
@@ -1159,6 +1272,36 @@ void av1h_reset(av1h_ctx* h) {
 int av1h_get_stats(const av1h_ctx* h, av1h_stats* out) {
   if (!h || !out) return AV1H_ERR_INVALID;
   *out = h->c.stats;
+  return AV1H_OK;
+}
+
+void av1h_set_stage_timing(av1h_ctx* h, int enable) {
+  if (!h) return;
+  Ctx* c = &h->c;
+  std::lock_guard<std::mutex> lk(c->mu);
+  (void)hipSetDevice(c->device);
+  if (enable && !c->sev[0])
+    for (auto& e : c->sev)
+      if (hipEventCreate(&e) != hipSuccess) {
+        set_err(c, AV1H_ERR_DEVICE, "hipEventCreate failed");
+        return;
+      }
+  c->timing = enable != 0 && c->sev[2 * AV1H_STAGES - 1] != nullptr;
+  for (int st = 0; st < AV1H_STAGES; ++st) {
+    c->stage_ms[st] = 0;
+    c->stage_n[st] = 0;
+    c->stage_bytes[st] = 0;
+  }
+}
+
+int av1h_get_stage_times(const av1h_ctx* h, double ms[AV1H_STAGES], int64_t count[AV1H_STAGES],
+                         int64_t bytes[AV1H_STAGES]) {
+  if (!h) return AV1H_ERR_INVALID;
+  for (int st = 0; st < AV1H_STAGES; ++st) {
+    if (ms) ms[st] = h->c.stage_ms[st];
+    if (count) count[st] = h->c.stage_n[st];
+    if (bytes) bytes[st] = h->c.stage_bytes[st];
+  }
   return AV1H_OK;
 }
 

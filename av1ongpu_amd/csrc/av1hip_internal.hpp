@@ -147,7 +147,8 @@ struct ReconArgs {
   int edge_filter;      /* enable_intra_edge_filter           */
 };
 
-hipError_t launch_recon(const ReconArgs& a, int cus, hipStream_t s, int* launches);
+/* part 0: the blocks that read only references; part 1: the dependency queue */
+hipError_t launch_recon(const ReconArgs& a, int part, int cus, hipStream_t s, int* launches);
 
 struct FilterArgs {
   FramePlanDev plan;

@@ -731,12 +731,12 @@ __global__ void k_zero(uint32_t* p, int n) {
 
 }  // namespace
 
-hipError_t launch_recon(const ReconArgs& a, int cus, hipStream_t s, int* launches) {
-  if (a.plan.n_inter > 0) {
+hipError_t launch_recon(const ReconArgs& a, int part, int cus, hipStream_t s, int* launches) {
+  if (part == 0 && a.plan.n_inter > 0) {
     hipLaunchKernelGGL(k_inter, dim3(a.plan.n_inter), dim3(64), 0, s, a);
     ++*launches;
   }
-  if (a.plan.n_queue > 0) {
+  if (part == 1 && a.plan.n_queue > 0) {
     hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, a.queue_counter, 1);
     const int grid = min(a.plan.n_queue, cus * 6);
     hipLaunchKernelGGL(k_queue, dim3(grid), dim3(64), 0, s, a);

@@ -1,0 +1,262 @@
+#!/usr/bin/env python3
+"""Decode throughput of the gfx950 AV1 reconstruction backend on one MI355X.
+
+Workload (default): 1920x1080 10-bit 4:2:0, 60 frames in a hierarchical
+16-frame group with 3 references, compound prediction (distance weighted and
+average), transforms up to 64x64 with identity and flip-ADST, Gaussian motion
+vectors with all three subpel filter families, deblocking + CDEF + Wiener and
+self-guided restoration, superres 3/4 (coded width 1440), film grain with 24
+luma AR coefficients, every frame packed to NV12 at a 2048 byte pitch. The
+syntax is synthetic and seeded (av1ongpu_amd/csrc/synth.cpp).
+
+A step decodes the whole clip in order: each frame's syntax has been flattened
+and uploaded to HBM beforehand (av1h_plan_frame, the reference's plan_frame
+hook), the decode runs every stage on the device, keeps the reference slots on
+the device, and packs NV12 on the device. `value` is display luma Mpixel/s over
+the timed steps. Frame latency (submit to NV12 packed, each decode call
+returns once its frame is complete on the device) is reported as p50/p99.
+
+Also reported: the same clip through av1h_decode_wire, whose time includes
+flattening and the upload of the syntax (`ingest_inclusive`); the dominant
+kernel's roofline from HIP events around each stage, on the decode stream,
+during the timed steps; the CPU oracle on a bounded sample of the same clip,
+timed on this host, and the bit-exactness of the GPU output on that sample.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = "decoded Mpixel/s on 1 MI355X, batched vs plain, bit-exact; p99 frame latency"
+HBM_PEAK_GBS = 8000.0
+NV12_PITCH = 2048
+STAGE_KERNEL = {"inter": "k_inter", "queue": "k_queue", "deblock": "k_deblock", "cdef": "k_cdef",
+                "superres": "k_upscale", "lr": "k_lr", "store": "reference store copy",
+                "grain": "k_grain_apply"}
+WORKLOADS = {
+    "c2": "1920x1080 10-bit 4:2:0, 60 frames, hierarchical GOP 16 with 3 references, compound "
+          "(distance weighted + average), transforms 4x4-64x64 incl. identity and flip-ADST, "
+          "Gaussian MVs sigma 24 px with regular/smooth/sharp filters, deblock + CDEF + Wiener + "
+          "self-guided restoration, superres 3/4 (coded width 1440), film grain AR lag 3, "
+          "NV12 pitch 2048",
+    "c1": "1280x720 8-bit 4:2:0, 30 frames (1 key + 29 inter, single reference), deblock + CDEF + "
+          "Wiener restoration, NV12 pitch 2048",
+}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def slot_schedule(clip):
+    """The slots a decoder that always takes the lowest free slot would use:
+    (slot per frame or -1, slots released after each frame, reference slots per
+    frame)."""
+    used = set()
+    slot_of = {}
+    keep_slot, releases, ref_slots = [], [], []
+    for f in range(clip.frames):
+        ref_slots.append([slot_of[r] for r in clip.refs[f]])
+        if clip.last_use[f] > f:
+            s = min(i for i in range(32) if i not in used)
+            used.add(s)
+            slot_of[f] = s
+            keep_slot.append(s)
+        else:
+            keep_slot.append(-1)
+        rel = []
+        for r in clip.refs[f]:
+            if clip.last_use[r] == f:
+                s = slot_of.pop(r)
+                used.discard(s)
+                rel.append(s)
+        releases.append(rel)
+    return keep_slot, releases, ref_slots
+
+
+def display_pixels(clip):
+    w, _cw, h, _bd, _sx, _sy = __import__("av1ongpu_amd").wire_header(clip.wire(0))
+    return w * h
+
+
+def pct(values, q):
+    v = sorted(values)
+    if not v:
+        return None
+    k = min(len(v) - 1, max(0, int(round(q / 100.0 * (len(v) - 1)))))
+    return v[k]
+
+
+def load_traffic(workload, kernel):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 counter
+    summary (FETCH_SIZE and WRITE_SIZE, scaled by a copy kernel of known size),
+    or None."""
+    path = os.path.join(REPO, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as fh:
+            t = json.load(fh)
+        if t.get("workload") != workload:
+            return None
+        k = t["kernels"].get(kernel)
+        return None if k is None else int(k["bytes_per_launch"])
+    except (OSError, ValueError, KeyError):
+        return None
+
+
+def cpu_baseline(clip, seconds, gpu_hashes):
+    """The CPU oracle on the first frames of the clip, in order, until
+    `seconds` have passed; the GPU's per frame hashes are compared on the same
+    frames."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import av1ongpu_amd as pkg
+    from oracle_lib import OracleDecoder, oracle_lib
+
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, int(os.environ.get("OMP_NUM_THREADS", cores))))
+    oracle_lib().av1o_set_threads(cores)
+    px = display_pixels(clip)
+    n = 0
+    mism = []
+    t0 = time.perf_counter()
+    for f, _slot, planes in pkg.decode_clip(OracleDecoder(), clip):
+        h = pkg.fnv1a64(pkg.yuv16_bytes(planes))
+        n += 1
+        if f < len(gpu_hashes) and gpu_hashes[f] != h:
+            mism.append(f)
+        if time.perf_counter() - t0 >= seconds:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": round(n * px / dt / 1e6, 3), "unit": "Mpixel/s", "cores": cores, "kind": "port",
+            "sample": f"first {n} of {clip.frames} frames of the same clip, decoded in order by the CPU "
+                      f"oracle (OpenMP, {cores} threads) with syntax ingest and read back, {dt:.1f} s"}, n, mism
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline and the parity sample")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this backend decodes each stream on one device and has no "
+              f"multi-GPU path (asked for --gpus {args.gpus})", file=sys.stderr)
+        sys.exit(2)
+    if int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs as a single process on one GPU", file=sys.stderr)
+        sys.exit(2)
+
+    import av1ongpu_amd as pkg
+
+    dec = pkg.HipDecoder(0)
+    log(f"device: {dec.device_info().name.decode(errors='replace')}")
+    t = time.perf_counter()
+    clip = pkg.Clip(args.workload, args.seed)
+    n = clip.frames
+    keep_slot, releases, ref_slots = slot_schedule(clip)
+    wires = [pkg.wire_set_slots(clip.wire(f), ref_slots[f]) for f in range(n)]
+    px = display_pixels(clip)
+    log(f"clip {args.workload} seed {args.seed}: {n} frames, {px} display px/frame, "
+        f"{sum(len(w) for w in wires) / 1e6:.0f} MB of syntax, generated in {time.perf_counter() - t:.1f} s")
+
+    # --- ingest-inclusive pass: syntax flattened and uploaded inside the timing
+    t0 = time.perf_counter()
+    for f in range(n):
+        slot, _ = dec.decode_wire(wires[f], keep=keep_slot[f] >= 0, reserved=keep_slot[f], read_back=False)
+        dec.pack_nv12_device(-1, NV12_PITCH)
+        for s in releases[f]:
+            dec.release(s)
+    ingest_s = time.perf_counter() - t0
+    log(f"ingest-inclusive pass: {ingest_s * 1e3 / n:.2f} ms/frame")
+
+    # --- plans resident in HBM
+    t = time.perf_counter()
+    plans = [dec.plan_wire(w) for w in wires]
+    log(f"planned {n} frames in {time.perf_counter() - t:.1f} s")
+
+    def step(lat):
+        for f in range(n):
+            t1 = time.perf_counter()
+            dec.decode_planned(plans[f], keep=keep_slot[f] >= 0, reserved=keep_slot[f])
+            dec.pack_nv12_device(-1, NV12_PITCH)
+            lat.append(time.perf_counter() - t1)
+            for s in releases[f]:
+                dec.release(s)
+
+    for i in range(args.warmup):
+        step([])
+        log(f"warmup step {i + 1}/{args.warmup} done")
+    lat = []
+    dec.set_stage_timing(True)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step(lat)
+        log(f"step {i + 1}/{args.steps} done")
+    elapsed = time.perf_counter() - t0
+    stages = dec.stage_times()
+    dec.set_stage_timing(False)
+    for p in plans:
+        dec.discard_plan(p)
+
+    value = args.steps * n * px / elapsed / 1e6
+    ms_step = elapsed * 1e3 / max(args.steps, 1)
+    # dominant kernel: the stage with the most device time in the timed steps
+    dom = max((s for s in stages if stages[s][1] > 0), key=lambda s: stages[s][0])
+    ms, cnt, byts = stages[dom]
+    avg_ms = ms / cnt
+    achieved = byts / cnt / (avg_ms * 1e-3) / 1e9
+    traffic = load_traffic(args.workload, STAGE_KERNEL[dom])
+    roofline = {"bound": "hbm", "kernel": STAGE_KERNEL[dom], "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic, "algorithmic_bytes_per_launch": byts // cnt,
+                "avg_launch_ms": round(avg_ms, 4)}
+    stage_report = {s: {"avg_ms": round(v[0] / v[1], 4), "frames": v[1],
+                        "gbs": round(v[2] / (v[0] * 1e-3) / 1e9, 1) if v[0] > 0 else None}
+                    for s, v in stages.items() if v[1] > 0}
+
+    result = {
+        "metric": METRIC, "value": round(value, 2), "unit": "Mpixel/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+        "data": "synthetic (seeded FrameSyntax of the named shape)",
+        "config": {"workload": f"{args.workload}: {WORKLOADS[args.workload]}", "frames": n, "seed": args.seed,
+                   "path": "plain per-frame decode, syntax pre-planned into HBM, NV12 packed on the device"},
+        "frame_latency_ms": {"p50": round(pct(lat, 50) * 1e3, 3), "p99": round(pct(lat, 99) * 1e3, 3)},
+        "ingest_inclusive": {"value": round(n * px / ingest_s / 1e6, 2), "unit": "Mpixel/s"},
+        "stages": stage_report,
+        "roofline": roofline,
+        "cpu_baseline": None,
+    }
+
+    if not args.no_cpu:
+        # GPU hashes of the frames the CPU sample will cover (read back, untimed)
+        import av1ongpu_amd as pkg2
+        gpu_hashes = []
+        t = time.perf_counter()
+        for f, _slot, planes in pkg2.decode_clip(pkg2.HipDecoder(0), clip):
+            gpu_hashes.append(pkg2.fnv1a64(pkg2.yuv16_bytes(planes)))
+        log(f"GPU read back of {len(gpu_hashes)} frames for parity in {time.perf_counter() - t:.1f} s")
+        base, checked, mism = cpu_baseline(clip, args.cpu_seconds, gpu_hashes)
+        log(f"CPU oracle: {base['value']} Mpixel/s on {checked} frames, {len(mism)} mismatched")
+        result["cpu_baseline"] = base
+        result["bit_exact"] = {"frames_checked": checked, "mismatched_frames": len(mism),
+                               "method": "FNV-1a-64 of every sample of each decoded frame, GPU vs CPU oracle"}
+        if mism:
+            log(f"MISMATCH on frames {mism[:10]}")
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

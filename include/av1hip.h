@@ -342,6 +342,11 @@ int av1h_create(int device_index, av1h_ctx** out);
 void av1h_destroy(av1h_ctx* ctx);
 const char* av1h_last_error(const av1h_ctx* ctx);
 int av1h_get_device_info(av1h_ctx* ctx, av1h_device_info* out);
+/* Describes HIP device `device_index` without creating a context (for
+ * enumerate_devices, reference src/core/factory.cpp:40-54): AV1H_OK for a
+ * gfx950 device, AV1H_ERR_UNSUPPORTED (info filled) for another HIP device,
+ * AV1H_ERR_NO_DEVICE past the last one. */
+int av1h_describe_device(int device_index, av1h_device_info* out);
 
 /* Decodes one frame (reference Decoder::decode_reference, av1gpu.hpp:635;
  * Decoder::decode, 733; Decoder::decode_reference_reserved, 645).
@@ -393,19 +398,54 @@ size_t av1h_nv12_size(int width, int height, int ss_x, int ss_y, size_t pitch);
  * (reference src/backends/cpu/cpu_backend.cpp:557-587). `slot` -1 packs the
  * last decoded picture. */
 int av1h_to_nv12(av1h_ctx* ctx, int slot, size_t pitch, uint8_t* out, size_t out_bytes);
+/* With `out` null, av1h_to_nv12 packs on the device only: the picture stays in
+ * the context's pack buffer, returned here (valid until the next pack on this
+ * context), for consumers that take the device picture (no counterpart in the
+ * reference, whose to_nv12 always copies to the host). */
+const uint8_t* av1h_nv12_device_buffer(const av1h_ctx* ctx);
 
 /* YUV -> RGB24 of a slot (reference Decoder::to_rgb, av1gpu.hpp:739): `matrix`
  * 0 BT.601, 1 BT.709, 2 BT.2020, 3 identity; `range` 0 limited, 1 full. */
 int av1h_to_rgb(av1h_ctx* ctx, int slot, int matrix, int range, uint8_t* out,
                 size_t out_bytes);
 
-/* Copies a slot's picture (same slot rule as av1h_to_nv12) to the host. */
+/* Copies a slot's picture (same slot rule as av1h_to_nv12) to the host; a
+ * null `out` returns only the geometry in `info`. */
 int av1h_read_slot(av1h_ctx* ctx, int slot, const av1h_planes* out, av1h_frame_info* info);
 
 /* Drops every slot and clears the pool (reference Decoder::reset). */
 void av1h_reset(av1h_ctx* ctx);
 
 int av1h_get_stats(const av1h_ctx* ctx, av1h_stats* out);
+
+/* Stage timing: with timing enabled, every decode records HIP events on the
+ * decode stream between its stages and accumulates the device time and the
+ * count of each stage, kernels only, the snapshot copies between stages
+ * excluded (the reference's env-gated timers,
+ * src/backends/cuda/cuda_backend.cpp:1048-1056, as an API). */
+enum {
+  AV1H_STAGE_INTER = 0,   /* k_inter: blocks that read only references       */
+  AV1H_STAGE_QUEUE = 1,   /* k_queue: the dependency queue (intra)           */
+  AV1H_STAGE_DEBLOCK = 2, /* both deblocking passes                          */
+  AV1H_STAGE_CDEF = 3,    /* CDEF                                            */
+  AV1H_STAGE_SUPERRES = 4,/* upscaling (the frame and the pre-CDEF copy)     */
+  AV1H_STAGE_LR = 5,      /* loop restoration                                */
+  AV1H_STAGE_STORE = 6,   /* reference store                                 */
+  AV1H_STAGE_GRAIN = 7,   /* template synthesis + blend                      */
+  AV1H_STAGES = 8
+};
+void av1h_set_stage_timing(av1h_ctx* ctx, int enable);
+/* Accumulated device milliseconds, decodes, and compulsory HBM bytes per stage
+ * since timing was enabled (a stage a frame skips is not counted for it).
+ * Compulsory bytes, at 16 bit samples: reconstruction 2 B written per sample,
+ * plus 2 B per reference read and 2 B of coefficients where transforms cover
+ * it; deblocking, CDEF, grain and the reference store 4 B (one read, one
+ * write) per sample they cover; restoration 4 B per sample plus the two
+ * boundary rows above and below each stripe; upscaling 2 B per coded and 2 B
+ * per upscaled sample, for each picture it upscales. Any argument may be
+ * null. */
+int av1h_get_stage_times(const av1h_ctx* ctx, double ms[AV1H_STAGES], int64_t count[AV1H_STAGES],
+                         int64_t bytes[AV1H_STAGES]);
 
 /* --------------------------------------------------------------- batching */
 /*
