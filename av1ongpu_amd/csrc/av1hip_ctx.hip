@@ -121,6 +121,10 @@ struct Ctx {
   Plan scratch_plan;
   av1h_stats stats;
   hipEvent_t ev[4];
+  /* grain templates depend only on the grain parameters: they are built on a
+   * second stream while the frame reconstructs */
+  hipStream_t side = nullptr;
+  hipEvent_t ev_side_go = nullptr, ev_side_done = nullptr;
   bool timing = false;
   hipEvent_t sev[2 * AV1H_STAGES] = {};
   bool stage_hit[AV1H_STAGES] = {};
@@ -201,6 +205,8 @@ struct Host {
   std::vector<DevBlock> blocks;
   std::vector<DevTx> tx;
   std::vector<int32_t> inter_list, queue_list;
+  std::vector<int32_t> mc_px0, mc_chunk;
+  std::vector<int32_t> itx[kTxClasses];
   std::vector<int32_t> coef, residual;
   std::vector<uint8_t> palette;
   std::vector<DevDeblock> deblock;
@@ -332,6 +338,8 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
       if (b.mode < 0 || b.mode > 13) return set_err(c, AV1H_ERR_INVALID, "block %d: intra mode %d", i, b.mode);
       if (b.mode == AV1H_PRED_CFL && p == 0)
         return set_err(c, AV1H_ERR_UNSUPPORTED, "chroma from luma on the luma plane");
+      if (b.mode == AV1H_PRED_CFL && (b.w > 32 || b.h > 32))
+        return set_err(c, AV1H_ERR_INVALID, "block %d: chroma from luma on a %dx%d block", i, b.w, b.h);
       d.angle = b.angle_delta;
       d.filter_type = hdr.enable_intra_edge_filter ? b.filter_type : -1;
       d.fi_mode = b.filter_intra_mode;
@@ -426,6 +434,8 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
       dt.tx_type = tx.tx_type;
       dt.row_shift = rs;
       dt.eob = tx.eob;
+      dt.plane = p;
+      dt.flags = (b.lossless ? kTxLossless : 0) | (b.residual_provided ? kTxRaw : 0) | (b.is_intra ? kTxToResidual : 0);
       const size_t end = (size_t)tx.coef_offset + (size_t)tx.w * tx.h;
       if (b.residual_provided)
         rend = std::max(rend, end);
@@ -443,6 +453,34 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
   if (f.num_residual) std::memcpy(H->residual.data(), f.residual, (size_t)f.num_residual * 4);
   H->palette.assign(f.palette_map, f.palette_map + f.num_palette);
   H->palette.resize(H->palette.size() + 8, 0);
+
+  /* motion compensation numbering: the samples of the inter blocks one block
+   * after another, and the block holding the first sample of every chunk */
+  H->mc_px0.assign(1, 0);
+  for (int32_t bi : H->inter_list) {
+    const int64_t next = (int64_t)H->mc_px0.back() + (int64_t)H->blocks[bi].w * H->blocks[bi].h;
+    if (next > INT32_MAX) return set_err(c, AV1H_ERR_UNSUPPORTED, "frame too large");
+    H->mc_px0.push_back((int32_t)next);
+  }
+  const int32_t mc_total = H->mc_px0.back();
+  const int32_t mc_chunks = (mc_total + kMcChunkSamples - 1) / kMcChunkSamples;
+  H->mc_chunk.resize(mc_chunks);
+  for (int32_t k = 0, bi = 0; k < mc_chunks; ++k) {
+    while (H->mc_px0[bi + 1] <= k * kMcChunkSamples) ++bi;
+    H->mc_chunk[k] = bi;
+  }
+  /* transform classes by the larger side, each sorted by size so a wave's
+   * transforms mostly share one */
+  for (auto& v : H->itx) v.clear();
+  for (int t = 0; t < (int)H->tx.size(); ++t) {
+    const int m = std::max(H->tx[t].w, H->tx[t].h);
+    H->itx[m <= 4 ? 0 : m <= 8 ? 1 : m <= 16 ? 2 : m <= 32 ? 3 : 4].push_back(t);
+  }
+  for (auto& v : H->itx)
+    std::stable_sort(v.begin(), v.end(), [&](int32_t a, int32_t b) {
+      const DevTx &ta = H->tx[a], &tb = H->tx[b];
+      return ta.w != tb.w ? ta.w < tb.w : ta.h < tb.h;
+    });
 
   /* deblocking: vertical edges then horizontal, with a line prefix each */
   H->deblock.clear();
@@ -602,17 +640,16 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
    * read and write at least, independent of how the kernels tile it */
   for (int64_t& v : plan->alg_bytes) v = 0;
   {
-    auto recon_bytes = [&](const std::vector<int32_t>& list) {
-      int64_t sum = 0;
-      for (int32_t i : list) {
-        const DevBlock& d = H->blocks[i];
-        const int refs = d.kind == kInter ? (d.compound ? 2 : 1) : 0;
-        sum += (int64_t)d.w * d.h * (2 + 2 * refs + (d.tx_count > 0 ? 2 : 0));
-      }
-      return sum;
-    };
-    plan->alg_bytes[AV1H_STAGE_INTER] = recon_bytes(H->inter_list);
-    plan->alg_bytes[AV1H_STAGE_QUEUE] = recon_bytes(H->queue_list);
+    /* part 0: inter samples written once and read from each reference, the
+     * coefficients of every transform (int16 each); the queue: intra samples
+     * written once */
+    for (int32_t i : H->inter_list) {
+      const DevBlock& d = H->blocks[i];
+      plan->alg_bytes[AV1H_STAGE_INTER] += (int64_t)d.w * d.h * (2 + 2 * (d.compound ? 2 : 1));
+    }
+    for (const DevTx& t : H->tx) plan->alg_bytes[AV1H_STAGE_INTER] += 2 * (int64_t)t.w * t.h;
+    for (int32_t i : H->queue_list)
+      plan->alg_bytes[AV1H_STAGE_QUEUE] += 2 * (int64_t)H->blocks[i].w * H->blocks[i].h;
     bool has_edges[3] = {false, false, false};
     for (const DevDeblock& d : H->deblock) has_edges[d.plane] = true;
     int64_t disp = 0;
@@ -644,7 +681,7 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
     size_t bytes;
     size_t off;
   };
-  Part parts[11] = {
+  Part parts[11 + 2 + kTxClasses] = {
       {H->blocks.data(), H->blocks.size() * sizeof(DevBlock), 0},
       {H->tx.data(), H->tx.size() * sizeof(DevTx), 0},
       {nullptr, 0, 0},
@@ -656,6 +693,13 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
       {H->deblock.data(), H->deblock.size() * sizeof(DevDeblock), 0},
       {H->cdef.data(), H->cdef.size() * sizeof(DevCdef), 0},
       {H->lr.data(), H->lr.size() * sizeof(DevLr), 0},
+      {H->mc_px0.data(), H->mc_px0.size() * 4, 0},
+      {H->mc_chunk.data(), H->mc_chunk.size() * 4, 0},
+      {H->itx[0].data(), H->itx[0].size() * 4, 0},
+      {H->itx[1].data(), H->itx[1].size() * 4, 0},
+      {H->itx[2].data(), H->itx[2].size() * 4, 0},
+      {H->itx[3].data(), H->itx[3].size() * 4, 0},
+      {H->itx[4].data(), H->itx[4].size() * 4, 0},
   };
   size_t total = 0;
   for (Part& pt : parts) {
@@ -689,6 +733,14 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
   P.cdef = reinterpret_cast<const DevCdef*>(db + parts[9].off);
   P.lr = reinterpret_cast<const DevLr*>(db + parts[10].off);
   P.grain = reinterpret_cast<const DevGrain*>(db + grain_off);
+  P.mc_px0 = reinterpret_cast<const int32_t*>(db + parts[11].off);
+  P.mc_chunk = reinterpret_cast<const int32_t*>(db + parts[12].off);
+  for (int k = 0; k < kTxClasses; ++k) {
+    P.itx_list[k] = reinterpret_cast<const int32_t*>(db + parts[13 + k].off);
+    P.itx_n[k] = (int32_t)H->itx[k].size();
+  }
+  P.mc_total = mc_total;
+  P.mc_chunks = mc_chunks;
   P.n_inter = (int)H->inter_list.size();
   P.n_queue = (int)H->queue_list.size();
   P.n_deblock_v = nedge[0];
@@ -789,6 +841,22 @@ int run_plan(Ctx* c, Plan* plan, int keep, int reserved, const av1h_planes* out,
   int launches = 0;
   ++c->epoch;
   const size_t bytes = (size_t)L.size * 2;
+  if (plan->has_grain) {
+    const size_t need = (size_t)plan->grain.stripes * plan->grain.blocks + 64;
+    if (c->grain_offs_cap < need) {
+      HIP_OK(c, hipStreamSynchronize(s));
+      if (c->grain_offs) (void)hipFree(c->grain_offs);
+      c->grain_offs = nullptr;
+      HIP_OK(c, hipMalloc(&c->grain_offs, need));
+      c->grain_offs_cap = need;
+    }
+    /* after everything queued before this frame (the plan upload, the last
+     * frame's grain) */
+    HIP_OK(c, hipEventRecord(c->ev_side_go, s));
+    HIP_OK(c, hipStreamWaitEvent(c->side, c->ev_side_go, 0));
+    HIP_OK(c, launch_grain_template(L, plan->dev.grain, c->grain_tmpl, c->grain_offs, c->side, &launches));
+    HIP_OK(c, hipEventRecord(c->ev_side_done, c->side));
+  }
   HIP_OK(c, hipEventRecord(c->ev[0], s));
   ReconArgs ra;
   ra.plan = plan->dev;
@@ -800,6 +868,7 @@ int run_plan(Ctx* c, Plan* plan, int keep, int reserved, const av1h_planes* out,
   ra.epoch = c->epoch;
   ra.queue_counter = c->counters;
   ra.error = c->counters + 1;
+  ra.res = reinterpret_cast<int16_t*>(c->scratch); /* free until the CDEF snapshot */
   ra.maxv = (1 << L.bit_depth) - 1;
   ra.edge_filter = plan->hdr.enable_intra_edge_filter;
   for (bool& h : c->stage_hit) h = false;
@@ -812,7 +881,9 @@ int run_plan(Ctx* c, Plan* plan, int keep, int reserved, const av1h_planes* out,
       c->stage_hit[st] = true;
     }
   };
-  if (plan->dev.n_inter > 0) {
+  bool any_tx = false;
+  for (int k = 0; k < kTxClasses; ++k) any_tx = any_tx || plan->dev.itx_n[k] > 0;
+  if (plan->dev.mc_total > 0 || any_tx) {
     begin(AV1H_STAGE_INTER);
     HIP_OK(c, launch_recon(ra, 0, c->cus, s, &launches));
     end(AV1H_STAGE_INTER);
@@ -864,16 +935,9 @@ int run_plan(Ctx* c, Plan* plan, int keep, int reserved, const av1h_planes* out,
     end(AV1H_STAGE_STORE);
   }
   if (plan->has_grain) {
-    const size_t need = (size_t)plan->grain.stripes * plan->grain.blocks + 64;
-    if (c->grain_offs_cap < need) {
-      HIP_OK(c, hipStreamSynchronize(s));
-      if (c->grain_offs) (void)hipFree(c->grain_offs);
-      c->grain_offs = nullptr;
-      HIP_OK(c, hipMalloc(&c->grain_offs, need));
-      c->grain_offs_cap = need;
-    }
+    HIP_OK(c, hipStreamWaitEvent(s, c->ev_side_done, 0));
     begin(AV1H_STAGE_GRAIN);
-    HIP_OK(c, launch_grain(L, plan->dev.grain, plan->grain, c->frame, c->grain_tmpl, c->grain_offs, s, &launches));
+    HIP_OK(c, launch_grain_apply(L, plan->dev.grain, c->frame, c->grain_tmpl, c->grain_offs, s, &launches));
     end(AV1H_STAGE_GRAIN);
   }
   HIP_OK(c, hipEventRecord(c->ev[2], s));
@@ -1016,6 +1080,12 @@ int av1h_create(int device_index, av1h_ctx** out) {
       delete h;
       return AV1H_ERR_DEVICE;
     }
+  if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_side_go, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_side_done, hipEventDisableTiming) != hipSuccess) {
+    delete h;
+    return AV1H_ERR_DEVICE;
+  }
   *out = h;
   return AV1H_OK;
 }
@@ -1035,6 +1105,10 @@ void av1h_destroy(av1h_ctx* h) {
     if (e) (void)hipEventDestroy(e);
   for (auto& e : c->sev)
     if (e) (void)hipEventDestroy(e);
+  if (c->side) (void)hipStreamSynchronize(c->side);
+  if (c->ev_side_go) (void)hipEventDestroy(c->ev_side_go);
+  if (c->ev_side_done) (void)hipEventDestroy(c->ev_side_done);
+  if (c->side) (void)hipStreamDestroy(c->side);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete h;
 }

@@ -56,107 +56,122 @@ __device__ void noise_row(int16_t* t, int w, int y, unsigned seed, const uint16_
   }
 }
 
-/* one step of the auto-regressive filter for sample (x, y)
- * (av1gpu_kernels.h:3087-3122) */
-__device__ void ar_sample(const DevGrain& g, int16_t* pl, int w, int x, int y, const int* coeffs,
-                          const int16_t* luma, int ss_x, int ss_y, int lo, int hi) {
-  const int lag = g.lag;
-  int sum = 0, pos = 0;
-  for (int dr = -lag; dr <= 0; ++dr)
-    for (int dc = -lag; dc <= lag; ++dc) {
-      if (dr == 0 && dc == 0) {
-        if (luma && g.num_y > 0) {
+/* LDS accesses of one wave complete in issue order; this keeps the compiler
+ * from moving a later read above an earlier write and waits for the writes. */
+__device__ __forceinline__ void wave_lds_sync() { __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
+
+/* The auto-regressive filter of one template (av1gpu_kernels.h:3087-3122) run
+ * by ONE wave as a wavefront: lane l filters row 3 + l (and row 67 + l when the
+ * template is taller than 67 rows) and lags the lane above by four columns,
+ * which is exactly the lag-3 filter's reach into the previous row, so no two
+ * waves ever synchronise. The coefficients stay in registers. */
+template <int LAG>
+__device__ void ar_plane(int16_t* pl, int w, int h, const int* coeffs, const int16_t* luma, int ss_x, int ss_y,
+                         int shift, int lo, int hi) {
+  constexpr int kTaps = (2 * LAG + 1) * LAG + LAG;
+  int c[kTaps + 1];
+#pragma unroll
+  for (int i = 0; i < kTaps; ++i) c[i] = coeffs[i];
+  c[kTaps] = luma ? coeffs[kTaps] : 0; /* chroma only: the luma tap */
+  const int lane = threadIdx.x & 63;
+  const int rows = h - 3, cols = w - 6;
+  const int steps = (cols - 1) + 4 * (rows - 1) + 1;
+  for (int s = 0; s < steps; ++s) {
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const int r = lane + 64 * p;
+      const int x = s - 4 * r + 3, y = 3 + r;
+      if (r < rows && x >= 3 && x < w - 3) {
+        int sum = 0, pos = 0;
+#pragma unroll
+        for (int dr = -LAG; dr <= 0; ++dr)
+#pragma unroll
+          for (int dc = -LAG; dc <= LAG; ++dc) {
+            if (dr == 0 && dc >= 0) break;
+            sum += pl[(y + dr) * w + x + dc] * c[pos];
+            ++pos;
+          }
+        if (luma) {
           const int lx = ((x - 3) << ss_x) + 3, ly = ((y - 3) << ss_y) + 3;
           int acc = 0;
           for (int i = 0; i <= ss_y; ++i)
             for (int k = 0; k <= ss_x; ++k) acc += luma[(ly + i) * kLW + lx + k];
-          sum += grnd(acc, ss_x + ss_y) * coeffs[pos];
+          sum += grnd(acc, ss_x + ss_y) * c[kTaps];
         }
-        dr = 1;
-        break;
+        pl[y * w + x] = (int16_t)clampi(pl[y * w + x] + grnd(sum, shift), lo, hi);
       }
-      sum += pl[(y + dr) * w + x + dc] * coeffs[pos];
-      ++pos;
     }
-  pl[y * w + x] = (int16_t)clampi(pl[y * w + x] + grnd(sum, g.shift), lo, hi);
+    wave_lds_sync();
+  }
 }
 
+__device__ void ar_dispatch(int lag, int16_t* pl, int w, int h, const int* coeffs, const int16_t* luma, int ss_x,
+                            int ss_y, int shift, int lo, int hi) {
+  switch (lag) {
+    case 0: ar_plane<0>(pl, w, h, coeffs, luma, ss_x, ss_y, shift, lo, hi); break;
+    case 1: ar_plane<1>(pl, w, h, coeffs, luma, ss_x, ss_y, shift, lo, hi); break;
+    case 2: ar_plane<2>(pl, w, h, coeffs, luma, ss_x, ss_y, shift, lo, hi); break;
+    default: ar_plane<3>(pl, w, h, coeffs, luma, ss_x, ss_y, shift, lo, hi); break;
+  }
+}
+
+constexpr int kTmplThreads = 192; /* three waves: luma, cb, cr */
+
 /* Templates and block offsets. `tmpl` receives luma (82x73) then cb and cr
- * (cw x ch each); `offs` the 8 bit offset draw of every (stripe, block). */
-__global__ __launch_bounds__(256) void k_grain_template(const DevGrain* gp, Jump jmp, int ss_x, int ss_y,
-                                                         int16_t* tmpl, uint8_t* offs) {
+ * (cw x ch each); `offs` the 8 bit offset draw of every (stripe, block).
+ * Phases: white noise (one row per thread); luma filter on wave 0 while
+ * waves 1-2 draw the offsets; cb on wave 1 and cr on wave 2 (both read the
+ * finished luma template) while wave 0 stores luma. */
+__global__ __launch_bounds__(kTmplThreads) void k_grain_template(const DevGrain* gp, Jump jmp, int ss_x, int ss_y,
+                                                                 int16_t* tmpl, uint8_t* offs) {
   __shared__ int16_t s_l[kLW * kLH];
   __shared__ int16_t s_c[2][kLW * kLH];
   const DevGrain& g = *gp;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, wave = tid >> 6;
   const int bd = g.bit_depth;
   const int lo = -(128 << (bd - 8)), hi = (256 << (bd - 8)) - 1 - (128 << (bd - 8));
   const int cw = ss_x ? 44 : kLW, ch = ss_y ? 38 : kLH;
   const int chroma = g.cfl || g.num_cb > 0 || g.num_cr > 0;
   const unsigned seed = (unsigned)g.seed;
-  /* white noise, one row per thread */
-  if (tid < kLH) {
-    if (g.num_y > 0)
-      noise_row(s_l, kLW, tid, seed, jmp.m82, g.scale_shift);
-    else
-      for (int x = 0; x < kLW; ++x) s_l[tid * kLW + x] = 0;
-  }
-  if (chroma) {
-    const uint16_t* jc = ss_x ? jmp.m44 : jmp.m82;
-    if (tid >= 96 && tid < 96 + ch) {
-      const int y = tid - 96;
-      if (g.num_cb > 0 || g.cfl)
-        noise_row(s_c[0], cw, y, seed ^ 0xb524u, jc, g.scale_shift);
+  /* white noise (av1gpu_kernels.h:3070-3079), one row per thread */
+  const uint16_t* jc = ss_x ? jmp.m44 : jmp.m82;
+  for (int r = tid; r < kLH + 2 * ch; r += kTmplThreads) {
+    if (r < kLH) {
+      if (g.num_y > 0)
+        noise_row(s_l, kLW, r, seed, jmp.m82, g.scale_shift);
       else
-        for (int x = 0; x < cw; ++x) s_c[0][y * cw + x] = 0;
-    }
-    if (tid >= 176 && tid < 176 + ch) {
-      const int y = tid - 176;
-      if (g.num_cr > 0 || g.cfl)
-        noise_row(s_c[1], cw, y, seed ^ 0x49d8u, jc, g.scale_shift);
+        for (int x = 0; x < kLW; ++x) s_l[r * kLW + x] = 0;
+    } else if (chroma) {
+      const int pl = (r - kLH) / ch, y = (r - kLH) - pl * ch;
+      if (pl == 0 ? (g.num_cb > 0 || g.cfl) : (g.num_cr > 0 || g.cfl))
+        noise_row(s_c[pl], cw, y, seed ^ (pl == 0 ? 0xb524u : 0x49d8u), jc, g.scale_shift);
       else
-        for (int x = 0; x < cw; ++x) s_c[1][y * cw + x] = 0;
+        for (int x = 0; x < cw; ++x) s_c[pl][y * cw + x] = 0;
     }
   }
   __syncthreads();
-  /* luma auto-regression: row y handles column x at step (x - 3) + 4 (y - 3) */
-  if (g.num_y > 0) {
-    const int steps = (kLW - 7) + 4 * (kLH - 4) + 1;
-    for (int s = 0; s < steps; ++s) {
-      if (tid < kLH - 3) {
-        const int y = 3 + tid, x = s - 4 * tid + 3;
-        if (x >= 3 && x < kLW - 3) ar_sample(g, s_l, kLW, x, y, g.ar_y, nullptr, ss_x, ss_y, lo, hi);
+  if (wave == 0) {
+    if (g.num_y > 0) ar_dispatch(g.lag, s_l, kLW, kLH, g.ar_y, nullptr, 0, 0, g.shift, lo, hi);
+  } else {
+    /* per stripe block offsets (av1gpu_kernels.h:3236-3247) */
+    for (int st = tid - 64; st < g.stripes; st += kTmplThreads - 64) {
+      unsigned r = seed;
+      r ^= (unsigned)(((st * 37 + 178) & 255) << 8);
+      r ^= (unsigned)((st * 173 + 105) & 255);
+      for (int b = 0; b < g.blocks; ++b) {
+        r = lfsr_step(r);
+        offs[st * g.blocks + b] = (uint8_t)((r >> 8) & 255);
       }
-      __syncthreads();
     }
   }
-  if (chroma) {
-    const int steps = (cw - 7) + 4 * (ch - 4) + 1;
-    for (int s = 0; s < steps; ++s) {
-      const int pl = tid >> 7, r = tid & 127;
-      if (r < ch - 3) {
-        const int y = 3 + r, x = s - 4 * r + 3;
-        if (x >= 3 && x < cw - 3)
-          ar_sample(g, s_c[pl], cw, x, y, pl ? g.ar_cr : g.ar_cb, s_l, ss_x, ss_y, lo, hi);
-      }
-      __syncthreads();
-    }
-  }
-  for (int i = tid; i < kLW * kLH; i += 256) tmpl[i] = s_l[i];
-  if (chroma)
-    for (int i = tid; i < cw * ch; i += 256) {
-      tmpl[kLW * kLH + i] = s_c[0][i];
-      tmpl[kLW * kLH + cw * ch + i] = s_c[1][i];
-    }
-  /* per stripe block offsets (av1gpu_kernels.h:3236-3247) */
-  for (int s = tid; s < g.stripes; s += 256) {
-    unsigned st = seed;
-    st ^= (unsigned)(((s * 37 + 178) & 255) << 8);
-    st ^= (unsigned)((s * 173 + 105) & 255);
-    for (int b = 0; b < g.blocks; ++b) {
-      st = lfsr_step(st);
-      offs[s * g.blocks + b] = (uint8_t)((st >> 8) & 255);
-    }
+  __syncthreads();
+  if (wave == 0) {
+    for (int i = tid; i < kLW * kLH; i += 64) tmpl[i] = s_l[i];
+  } else if (chroma) {
+    const int pl = wave - 1;
+    ar_dispatch(g.lag, s_c[pl], cw, ch, pl ? g.ar_cr : g.ar_cb, g.num_y > 0 ? s_l : nullptr, ss_x, ss_y, g.shift,
+                lo, hi);
+    for (int i = tid & 63; i < cw * ch; i += 64) tmpl[kLW * kLH + pl * cw * ch + i] = s_c[pl][i];
   }
 }
 
@@ -279,15 +294,21 @@ void grain_jump_matrices(uint16_t m82[16], uint16_t m44[16]) {
   }
 }
 
-hipError_t launch_grain(const Layout& lay, const DevGrain* g, const DevGrain& hg, uint16_t* frame,
-                        int16_t* templates, uint8_t* offsets, hipStream_t s, int* launches) {
+hipError_t launch_grain_template(const Layout& lay, const DevGrain* g, int16_t* templates, uint8_t* offsets,
+                                 hipStream_t s, int* launches) {
   Jump j;
   grain_jump_matrices(j.m82, j.m44);
-  hipLaunchKernelGGL(k_grain_template, dim3(1), dim3(256), 0, s, g, j, lay.ss_x, lay.ss_y, templates, offsets);
+  hipLaunchKernelGGL(k_grain_template, dim3(1), dim3(kTmplThreads), 0, s, g, j, lay.ss_x, lay.ss_y, templates,
+                     offsets);
+  ++*launches;
+  return hipGetLastError();
+}
+
+hipError_t launch_grain_apply(const Layout& lay, const DevGrain* g, uint16_t* frame, const int16_t* templates,
+                              const uint8_t* offsets, hipStream_t s, int* launches) {
   hipLaunchKernelGGL(k_grain_apply, dim3((lay.out_w[0] + 63) / 64, (lay.out_h[0] + 31) / 32), dim3(256), 0, s,
                      lay, g, templates, offsets, frame);
-  *launches += 2;
-  (void)hg;
+  ++*launches;
   return hipGetLastError();
 }
 

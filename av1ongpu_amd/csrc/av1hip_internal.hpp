@@ -72,13 +72,22 @@ struct DevBlock {
   int32_t pad;
 };
 
+/* flag bits of DevTx::flags */
+enum { kTxLossless = 1, kTxRaw = 2, kTxToResidual = 4 };
+
 struct DevTx {
   int32_t x, y, w, h;   /* in the plane                      */
   int32_t coef;         /* raster offset in the coefficient (or residual) pool */
   int32_t tx_type;
   int32_t row_shift;
   int32_t eob;
+  int32_t plane;
+  int32_t flags;        /* kTx*: lossless (WHT), raw residual, intra block  */
 };
+
+/* transform classes: the larger side, 4 8 16 32 64 */
+constexpr int kTxClasses = 5;
+constexpr int kMcChunkSamples = 256; /* motion compensation samples per workgroup */
 
 struct DevObmc {
   int32_t pass, rx, ry, w, h, mvx, mvy, filt, filt_v, slot;
@@ -129,7 +138,15 @@ struct FramePlanDev {
   const DevCdef* cdef;
   const DevLr* lr;
   const DevGrain* grain;
+  /* motion compensation: sample prefix over inter_list (n_inter + 1 entries),
+   * and the inter_list entry holding the first sample of each 256 sample chunk */
+  const int32_t* mc_px0;
+  const int32_t* mc_chunk;
+  /* transform blocks by class, sorted by size inside a class */
+  const int32_t* itx_list[kTxClasses];
+  int32_t itx_n[kTxClasses];
   int32_t n_inter, n_queue, n_deblock_v, n_deblock_h, lines_v, lines_h, n_cdef, n_lr, lr_px;
+  int32_t mc_total, mc_chunks;
 };
 
 /* launchers (one per translation unit, all on the context's stream) */
@@ -143,11 +160,14 @@ struct ReconArgs {
   uint32_t epoch;
   uint32_t* queue_counter;
   uint32_t* error;      /* bit 0: a dependency wait timed out  */
+  int16_t* res;         /* residual of intra blocks, frame layout */
   int maxv;
   int edge_filter;      /* enable_intra_edge_filter           */
 };
 
-/* part 0: the blocks that read only references; part 1: the dependency queue */
+/* part 0: motion compensation of every inter block, then every transform
+ * (inter residuals added into the frame, intra residuals into `res`);
+ * part 1: the dependency queue over the intra blocks */
 hipError_t launch_recon(const ReconArgs& a, int part, int cus, hipStream_t s, int* launches);
 
 struct FilterArgs {
@@ -168,9 +188,12 @@ hipError_t launch_superres(const FilterArgs& a, const uint16_t* src, uint16_t* d
                            int* launches);
 hipError_t launch_lr(const FilterArgs& a, const uint16_t* cdef, const uint16_t* curr, hipStream_t s,
                      int* launches);
-hipError_t launch_grain(const Layout& lay, const DevGrain* g, const DevGrain& host_g,
-                        uint16_t* frame, int16_t* templates, uint8_t* offsets, hipStream_t s,
-                        int* launches);
+/* the three grain templates and the block offsets (one workgroup; depends only
+ * on the grain parameters), then the blend into the picture */
+hipError_t launch_grain_template(const Layout& lay, const DevGrain* g, int16_t* templates, uint8_t* offsets,
+                                 hipStream_t s, int* launches);
+hipError_t launch_grain_apply(const Layout& lay, const DevGrain* g, uint16_t* frame, const int16_t* templates,
+                              const uint8_t* offsets, hipStream_t s, int* launches);
 hipError_t launch_nv12(const Layout& lay, const uint16_t* src, uint8_t* out, int pitch, hipStream_t s);
 hipError_t launch_rgb(const Layout& lay, const uint16_t* src, uint8_t* out, const int m[8],
                       const int o[3], hipStream_t s);

@@ -1,22 +1,26 @@
 /*
  * Reconstruction kernels for gfx950: prediction + inverse transform + residual.
  *
- * Two launches per frame:
- *   k_inter  - every block that reads nothing of the frame being decoded (plain
- *              inter prediction): one wave64 per block, all in parallel.
- *   k_queue  - a persistent work queue over the blocks that do read the frame
- *              (intra, inter/intra): waves take blocks in coding order, spin on
- *              per cell epoch flags until the cells the block reads are final,
- *              reconstruct the block and publish its own cells. This replaces
- *              the reference's host wavefront schedule (src/core/engine.cpp:
+ * Per frame, in stream order:
+ *   k_mc     - motion compensation of every inter block, one thread per
+ *              output sample, straight into the frame;
+ *   k_itx<G> - every transform block, grouped by its larger side G: G lanes
+ *              per transform. Inter residuals are added into the frame, intra
+ *              residuals are kept in a residual plane;
+ *   k_queue  - a persistent work queue over the intra blocks: waves take
+ *              blocks in coding order, spin on per cell epoch flags until the
+ *              cells the block reads are final, predict the block, add its
+ *              residual and publish its own cells. This replaces the
+ *              reference's host wavefront schedule (src/core/engine.cpp:
  *              695-926): the dependencies are the same cells that scheduler
  *              waits for, but they resolve on the device as soon as each
  *              neighbour is done instead of one global wave at a time.
  *
- * A block is reconstructed in LDS: the prediction is written into s_pred, its
- * transforms add their residual there (row pass then column pass through an
- * LDS intermediate), and the finished block is stored once. The arithmetic is
- * the reference's (kernels/av1gpu_kernels.h), cited per function.
+ * This is the reference CPU backend's stage order inside a wave (inter, intra,
+ * transforms, residual add; src/backends/cpu/cpu_backend.cpp:213-328) with
+ * the transforms of intra blocks hoisted out of the dependency chain: their
+ * residual does not depend on the prediction. The arithmetic is the
+ * reference's (kernels/av1gpu_kernels.h), cited per function.
  */
 #define AV1T_TABLE_QUAL static __constant__ const
 #include "av1hip_internal.hpp"
@@ -27,9 +31,7 @@ namespace {
 
 #define SPEC(off) (av1t_spec_table + (off))
 
-constexpr int kMidRows = 72;           /* MC intermediate rows per strip        */
-constexpr int kWork = kMidRows * 64;   /* int32 work area (>= 64 x 65 itx tmp)  */
-constexpr int kStrip = 32;             /* output rows per MC strip              */
+constexpr int kWork = 2048;            /* int32 intra work area (edges, CFL)    */
 constexpr int kSpinLimit = 1 << 22;    /* dependency wait bound (then an error) */
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
@@ -56,9 +58,9 @@ __device__ int wave_sum(int v) {
 
 /* row pass (av1gpu_kernels.h:869-904): lane = row */
 template <int N>
-__device__ void row_pass(int* tmp, int w, int h, int tx_type, int lossless, int row_shift, int bd,
+__device__ void row_pass(int* tmp, int lane, int w, int h, int tx_type, int lossless, int row_shift, int bd,
                          int rect) {
-  const int y = threadIdx.x;
+  const int y = lane;
   if (y >= h) return;
   const int cs = w < 32 ? w : 32;
   const int st = w + 1;
@@ -88,11 +90,11 @@ __device__ void row_pass(int* tmp, int w, int h, int tx_type, int lossless, int 
   }
 }
 
-/* column pass + add (av1gpu_kernels.h:907-930): lane = column */
-template <int M>
-__device__ void col_pass(const int* tmp, int w, int tx_type, int lossless, int bd, uint16_t* pred,
-                         int pstride, int maxv) {
-  const int x = threadIdx.x;
+/* column pass (av1gpu_kernels.h:907-930): lane = column; hands each residual
+ * sample to `out(x, y, value)` */
+template <int M, typename Out>
+__device__ void col_pass(const int* tmp, int lane, int w, int tx_type, int lossless, int bd, Out&& out) {
+  const int x = lane;
   if (x >= w) return;
   const int st = w + 1;
   int t[M];
@@ -107,55 +109,86 @@ __device__ void col_pass(const int* tmp, int w, int tx_type, int lossless, int b
   const int ud = itx::flip_ud(tx_type), lr = itx::flip_lr(tx_type);
   const int xx = lr ? w - 1 - x : x;
 #pragma unroll
-  for (int i = 0; i < M; ++i) {
-    const int v = lossless ? t[i] : rnd2(t[i], 4);
-    const int yy = ud ? M - 1 - i : i;
-    uint16_t* p = pred + yy * pstride + xx;
-    *p = (uint16_t)clampi((int)*p + v, 0, maxv);
+  for (int i = 0; i < M; ++i) out(xx, ud ? M - 1 - i : i, lossless ? t[i] : rnd2(t[i], 4));
+}
+
+/* the row / column pass of the size `l2` (log2 points), instantiated only up
+ * to the class size G */
+template <int G>
+__device__ void row_pass_n(int l2, int* tmp, int lane, int w, int h, int tx_type, int lossless, int row_shift,
+                           int bd, int rect) {
+  switch (l2) {
+    case 2: row_pass<4>(tmp, lane, w, h, tx_type, lossless, row_shift, bd, rect); break;
+    case 3: if constexpr (G >= 8) row_pass<8>(tmp, lane, w, h, tx_type, lossless, row_shift, bd, rect); break;
+    case 4: if constexpr (G >= 16) row_pass<16>(tmp, lane, w, h, tx_type, lossless, row_shift, bd, rect); break;
+    case 5: if constexpr (G >= 32) row_pass<32>(tmp, lane, w, h, tx_type, lossless, row_shift, bd, rect); break;
+    default: if constexpr (G >= 64) row_pass<64>(tmp, lane, w, h, tx_type, lossless, row_shift, bd, rect); break;
+  }
+}
+template <int G, typename Out>
+__device__ void col_pass_n(int l2, const int* tmp, int lane, int w, int tx_type, int lossless, int bd, Out&& out) {
+  switch (l2) {
+    case 2: col_pass<4>(tmp, lane, w, tx_type, lossless, bd, out); break;
+    case 3: if constexpr (G >= 8) col_pass<8>(tmp, lane, w, tx_type, lossless, bd, out); break;
+    case 4: if constexpr (G >= 16) col_pass<16>(tmp, lane, w, tx_type, lossless, bd, out); break;
+    case 5: if constexpr (G >= 32) col_pass<32>(tmp, lane, w, tx_type, lossless, bd, out); break;
+    default: if constexpr (G >= 64) col_pass<64>(tmp, lane, w, tx_type, lossless, bd, out); break;
   }
 }
 
-__device__ void apply_tx(const ReconArgs& a, const DevBlock& b, const DevTx& tx, int* tmp,
-                         uint16_t* pred) {
-  const int lane = threadIdx.x;
+/* Every transform block of class G (larger side G): G lanes per transform,
+ * 64 / G transforms per wave, coefficients staged in LDS between the row and
+ * the column pass. The residual of an inter block is added into the frame
+ * (its prediction is already there); that of an intra block goes to `res`,
+ * where the dependency queue adds it after the prediction. */
+template <int G>
+__global__ __launch_bounds__(64) void k_itx(ReconArgs a, int cls) {
+  constexpr int kPer = 64 / G;
+  __shared__ int s_tmp[kPer * G * (G + 1)];
+  const int grp = threadIdx.x / G, l = threadIdx.x - grp * G;
+  const int idx = blockIdx.x * kPer + grp;
+  const bool live = idx < a.plan.itx_n[cls];
+  DevTx tx;
+  if (live) {
+    tx = a.plan.tx[a.plan.itx_list[cls][idx]];
+  } else {
+    tx.w = tx.h = 4;
+    tx.flags = kTxRaw;
+    tx.plane = 0;
+    tx.x = tx.y = tx.coef = tx.tx_type = tx.row_shift = tx.eob = 0;
+  }
+  int* tmp = s_tmp + grp * G * (G + 1);
   const int w = tx.w, h = tx.h;
-  const int ox = tx.x - b.x, oy = tx.y - b.y;
-  uint16_t* p0 = pred + oy * b.w + ox;
-  const int bd = a.lay.bit_depth;
-  if (b.flags & kResidualRaw) {
+  const int stride = a.lay.stride[tx.plane];
+  const int64_t off = a.lay.plane_off[tx.plane] + (int64_t)tx.y * stride + tx.x;
+  const bool to_res = (tx.flags & kTxToResidual) != 0;
+  const int maxv = a.maxv;
+  auto out = [&](int xx, int yy, int v) {
+    const int64_t o = off + (int64_t)yy * stride + xx;
+    if (to_res)
+      a.res[o] = (int16_t)clampi(v, -32768, 32767);
+    else
+      a.frame[o] = (uint16_t)clampi((int)a.frame[o] + v, 0, maxv);
+  };
+  const bool coded = live && !(tx.flags & kTxRaw);
+  if (live && !coded) {
     /* raw residual (av1gpu_kernels.h:933-937) */
-    for (int s = lane; s < w * h; s += 64) {
-      const int y = s / w, x = s - y * w;
-      uint16_t* p = p0 + y * b.w + x;
-      *p = (uint16_t)clampi((int)*p + a.plan.residual[tx.coef + s], 0, a.maxv);
-    }
-    wsync();
-    return;
+    const int lw = ilog2(w);
+    for (int i = l; i < w * h; i += G) out(i & (w - 1), i >> lw, a.plan.residual[tx.coef + i]);
   }
-  const int cs = w < 32 ? w : 32, cr = h < 32 ? h : 32;
-  const int lcs = ilog2(cs);
-  const int st = w + 1;
-  for (int i = lane; i < cs * cr; i += 64) tmp[(i >> lcs) * st + (i & (cs - 1))] = a.plan.coef[tx.coef + i];
-  wsync();
-  const int lossless = (b.flags & kLossless) ? 1 : 0;
   const int lw = ilog2(w), lh = ilog2(h);
-  const int rect = (lw - lh == 1 || lh - lw == 1) ? 1 : 0;
-  switch (lw) {
-    case 2: row_pass<4>(tmp, w, h, tx.tx_type, lossless, tx.row_shift, bd, rect); break;
-    case 3: row_pass<8>(tmp, w, h, tx.tx_type, lossless, tx.row_shift, bd, rect); break;
-    case 4: row_pass<16>(tmp, w, h, tx.tx_type, lossless, tx.row_shift, bd, rect); break;
-    case 5: row_pass<32>(tmp, w, h, tx.tx_type, lossless, tx.row_shift, bd, rect); break;
-    default: row_pass<64>(tmp, w, h, tx.tx_type, lossless, tx.row_shift, bd, rect); break;
+  if (coded) {
+    const int cs = w < 32 ? w : 32, cr = h < 32 ? h : 32;
+    const int lcs = ilog2(cs), st = w + 1;
+    for (int i = l; i < cs * cr; i += G) tmp[(i >> lcs) * st + (i & (cs - 1))] = a.plan.coef[tx.coef + i];
   }
-  wsync();
-  switch (lh) {
-    case 2: col_pass<4>(tmp, w, tx.tx_type, lossless, bd, p0, b.w, a.maxv); break;
-    case 3: col_pass<8>(tmp, w, tx.tx_type, lossless, bd, p0, b.w, a.maxv); break;
-    case 4: col_pass<16>(tmp, w, tx.tx_type, lossless, bd, p0, b.w, a.maxv); break;
-    case 5: col_pass<32>(tmp, w, tx.tx_type, lossless, bd, p0, b.w, a.maxv); break;
-    default: col_pass<64>(tmp, w, tx.tx_type, lossless, bd, p0, b.w, a.maxv); break;
-  }
-  wsync();
+  __syncthreads();
+  const int lossless = (tx.flags & kTxLossless) ? 1 : 0;
+  if (coded)
+    row_pass_n<G>(lw, tmp, l, w, h, tx.tx_type, lossless, tx.row_shift, a.lay.bit_depth,
+                  (lw - lh == 1 || lh - lw == 1) ? 1 : 0);
+  __syncthreads();
+  if (coded) col_pass_n<G>(lh, tmp, l, w, tx.tx_type, lossless, a.lay.bit_depth, out);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -219,100 +252,35 @@ __device__ McPos mc_pos(int bx, int by, int mvx, int mvy, int sub_x, int sub_y, 
   return p;
 }
 
-/* One reference's prediction of the strip rows [y0, y0 + rows) of a w wide
- * region, two stage filtered through the LDS intermediate `mid`
- * (av1gpu_kernels.h:1756-1857). Results land in v[k] for sample
- * s = lane + 64 k of the strip. */
-__device__ void mc_strip(const uint16_t* ref, int stride, int ref_w, int ref_h, const McPos& P,
-                         int w, int y0, int rows, int ew, int eh, int filt, int filt_v,
-                         int compound, int bd, int* mid, int (&v)[kStrip]) {
-  const int lane = threadIdx.x;
-  const int lw = ilog2(w);
-  const int round0 = bd >= 12 ? 5 : 3;
-  int round1 = compound ? 7 : 11;
-  if (bd >= 12 && !compound) round1 -= 2;
-  const int fh = filt_row(filt, ew), fv = filt_row(filt_v, eh);
-  int cy0, fy0, cy1, fy1;
-  P.row(y0, &cy0, &fy0);
-  P.row(y0 + rows - 1, &cy1, &fy1);
-  const int r0 = cy0 - 3;
-  const int nr = cy1 + 4 - r0 + 1;
-  const int* hk0 = SPEC(AV1T_OFF_SUBPEL) + fh * 128;
-  const int* vk0 = SPEC(AV1T_OFF_SUBPEL) + fv * 128;
-  for (int i = lane; i < nr * w; i += 64) {
-    const int rr = i >> lw, x = i & (w - 1);
-    int cx, fx;
-    P.col(x, &cx, &fx);
-    const uint16_t* row = ref + (size_t)clampi(r0 + rr, 0, ref_h - 1) * stride;
-    const int* hk = hk0 + fx * 8;
-    int s = 0;
-#pragma unroll
-    for (int t = 0; t < 8; ++t) s += (int)row[clampi(cx - 3 + t, 0, ref_w - 1)] * hk[t];
-    mid[rr * 64 + x] = (s + (1 << (round0 - 1))) >> round0;
-  }
-  wsync();
-#pragma unroll
-  for (int k = 0; k < kStrip; ++k) {
-    const int s = lane + 64 * k;
-    if (s < rows * w) {
-      const int y = y0 + (s >> lw), x = s & (w - 1);
-      int cy, fy;
-      P.row(y, &cy, &fy);
-      const int* vk = vk0 + fy * 8;
-      const int* m = mid + (cy - 3 - r0) * 64 + x;
-      int acc = 0;
-#pragma unroll
-      for (int t = 0; t < 8; ++t) acc += m[t * 64] * vk[t];
-      v[k] = (acc + (1 << (round1 - 1))) >> round1;
-    }
-  }
-  wsync();
-}
-
 __device__ const uint16_t* ref_plane(const ReconArgs& a, int slot, int plane) {
   const uint16_t* base = slot >= 0 ? a.pool + (size_t)slot * a.slot_size : a.frame;
   return base + a.lay.plane_off[plane];
 }
 
-/* inter prediction of a whole block into s_pred (av1gpu_kernels.h:2207-2293) */
-__device__ void predict_inter(const ReconArgs& a, const DevBlock& b, int* mid, uint16_t* pred) {
-  const int lane = threadIdx.x;
-  const int p = b.plane;
-  const int sub_x = p > 0 ? a.lay.ss_x : 0, sub_y = p > 0 ? a.lay.ss_y : 0;
-  const int stride = a.lay.stride[p];
-  const int bd = a.lay.bit_depth;
-  const int w = b.w, h = b.h, lw = ilog2(w);
-  const int compound = b.compound != 0;
-  const McPos P0 = mc_pos(b.x, b.y, b.mv[0][0], b.mv[0][1], sub_x, sub_y, b.scale_x, b.scale_y);
-  const McPos P1 = mc_pos(b.x, b.y, b.mv[1][0], b.mv[1][1], sub_x, sub_y, b.scale_x, b.scale_y);
-  const uint16_t* r0 = ref_plane(a, b.slot[0], p);
-  const uint16_t* r1 = ref_plane(a, b.slot[1], p);
-  const int round0 = bd >= 12 ? 5 : 3;
-  const int post = 2 * 7 - (round0 + 7);
-  for (int y0 = 0; y0 < h; y0 += kStrip) {
-    const int rows = min(kStrip, h - y0);
-    int v0[kStrip], v1[kStrip];
-    mc_strip(r0, stride, b.ref_w, b.ref_h, P0, w, y0, rows, w, h, b.filt, b.filt_v, compound, bd,
-             mid, v0);
-    if (compound)
-      mc_strip(r1, stride, b.ref_w, b.ref_h, P1, w, y0, rows, w, h, b.filt, b.filt_v, 1, bd, mid,
-               v1);
+/* One reference's prediction of sample (x, y) of a block: eight horizontally
+ * filtered rows, rounded, then the vertical filter (av1gpu_kernels.h:
+ * 1756-1857). Every thread filters its own eight rows; the reference samples
+ * its neighbours share come from the cache. */
+__device__ __forceinline__ int mc_sample(const uint16_t* ref, int stride, int ref_w, int ref_h, const McPos& P,
+                                         int x, int y, const int* hk0, const int* vk0, int round0, int round1) {
+  int cx, fx, cy, fy;
+  P.col(x, &cx, &fx);
+  P.row(y, &cy, &fy);
+  const int* hk = hk0 + fx * 8;
+  const int* vk = vk0 + fy * 8;
+  int cols[8];
 #pragma unroll
-    for (int k = 0; k < kStrip; ++k) {
-      const int s = lane + 64 * k;
-      if (s < rows * w) {
-        int out = v0[k];
-        if (compound) {
-          if (b.compound == AV1H_COMPOUND_AVERAGE)
-            out = (v0[k] + v1[k] + (1 << post)) >> (1 + post);
-          else
-            out = (b.w0 * v0[k] + b.w1 * v1[k] + (1 << (3 + post))) >> (4 + post);
-        }
-        pred[(y0 << lw) + s] = (uint16_t)clampi(out, 0, a.maxv);
-      }
-    }
+  for (int t = 0; t < 8; ++t) cols[t] = clampi(cx - 3 + t, 0, ref_w - 1);
+  int acc = 0;
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const uint16_t* row = ref + (size_t)clampi(cy - 3 + r, 0, ref_h - 1) * stride;
+    int s = 0;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) s += (int)row[cols[t]] * hk[t];
+    acc += ((s + (1 << (round0 - 1))) >> round0) * vk[r];
   }
-  wsync();
+  return (acc + (1 << (round1 - 1))) >> round1;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -685,25 +653,77 @@ __device__ void wait_cells(const ReconArgs& a, const DevBlock& b) {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
-__device__ void reconstruct(const ReconArgs& a, const DevBlock& b, int* work, uint16_t* pred) {
-  if (b.kind == kIntra)
-    predict_intra(a, b, work, pred);
-  else
-    predict_inter(a, b, work, pred);
+/* Intra block: the prediction, then the residuals of its transforms, which
+ * k_itx computed before the queue started (av1gpu_kernels.h:933-937 for the
+ * add), then one store. */
+__device__ void reconstruct_intra(const ReconArgs& a, const DevBlock& b, int* work, uint16_t* pred) {
+  predict_intra(a, b, work, pred);
+  const int p = b.plane, stride = a.lay.stride[p];
   for (int t = 0; t < b.tx_count; ++t) {
     const DevTx& tx = a.plan.tx[b.tx_first + t];
-    if (tx.eob > 0) apply_tx(a, b, tx, work, pred);
+    const int ox = tx.x - b.x, oy = tx.y - b.y, ltw = ilog2(tx.w);
+    const int16_t* r = a.res + a.lay.plane_off[p] + (int64_t)tx.y * stride + tx.x;
+    for (int s = threadIdx.x; s < tx.w * tx.h; s += 64) {
+      const int yy = s >> ltw, xx = s & (tx.w - 1);
+      uint16_t* q = pred + (oy + yy) * b.w + ox + xx;
+      *q = (uint16_t)clampi((int)*q + r[(int64_t)yy * stride + xx], 0, a.maxv);
+    }
   }
+  wsync();
   store_block(a, b, pred);
 }
 
-__global__ __launch_bounds__(64) void k_inter(ReconArgs a) {
-  __shared__ int s_work[kWork];
-  __shared__ uint16_t s_pred[64 * 64];
-  const int bi = __builtin_amdgcn_readfirstlane(a.plan.inter_list[blockIdx.x]);
-  const DevBlock& b = a.plan.blocks[bi];
-  reconstruct(a, b, s_work, s_pred);
-  mark_cells(a, b, false);
+/* Motion compensation of every inter block, one thread per output sample:
+ * the samples of the inter blocks are numbered block after block, and a
+ * workgroup takes 256 consecutive ones. */
+constexpr int kMcChunk = kMcChunkSamples;
+
+__global__ __launch_bounds__(kMcChunk) void k_mc(ReconArgs a) {
+  __shared__ int s_filt[768];
+  for (int i = threadIdx.x; i < 768; i += kMcChunk) s_filt[i] = av1t_spec_table[AV1T_OFF_SUBPEL + i];
+  __syncthreads();
+  const int g = blockIdx.x * kMcChunk + threadIdx.x;
+  if (g >= a.plan.mc_total) return;
+  int i = a.plan.mc_chunk[blockIdx.x];
+  while (a.plan.mc_px0[i + 1] <= g) ++i;
+  const DevBlock& b = a.plan.blocks[a.plan.inter_list[i]];
+  const int s = g - a.plan.mc_px0[i];
+  const int w = b.w, lw = ilog2(w), x = s & (w - 1), y = s >> lw;
+  const int p = b.plane;
+  const int sub_x = p > 0 ? a.lay.ss_x : 0, sub_y = p > 0 ? a.lay.ss_y : 0;
+  const int stride = a.lay.stride[p];
+  const int bd = a.lay.bit_depth;
+  const int compound = b.compound != 0;
+  const int round0 = bd >= 12 ? 5 : 3;
+  int round1 = compound ? 7 : 11;
+  if (bd >= 12 && !compound) round1 -= 2;
+  const int* hk0 = s_filt + filt_row(b.filt, w) * 128;
+  const int* vk0 = s_filt + filt_row(b.filt_v, b.h) * 128;
+  const McPos P0 = mc_pos(b.x, b.y, b.mv[0][0], b.mv[0][1], sub_x, sub_y, b.scale_x, b.scale_y);
+  int out = mc_sample(ref_plane(a, b.slot[0], p), stride, b.ref_w, b.ref_h, P0, x, y, hk0, vk0, round0, round1);
+  if (compound) {
+    /* av1gpu_kernels.h:2207-2293 */
+    const McPos P1 = mc_pos(b.x, b.y, b.mv[1][0], b.mv[1][1], sub_x, sub_y, b.scale_x, b.scale_y);
+    const int v1 =
+        mc_sample(ref_plane(a, b.slot[1], p), stride, b.ref_w, b.ref_h, P1, x, y, hk0, vk0, round0, round1);
+    const int post = 2 * 7 - (round0 + 7);
+    if (b.compound == AV1H_COMPOUND_AVERAGE)
+      out = (out + v1 + (1 << post)) >> (1 + post);
+    else
+      out = (b.w0 * out + b.w1 * v1 + (1 << (3 + post))) >> (4 + post);
+  }
+  a.frame[a.lay.plane_off[p] + (int64_t)(b.y + y) * stride + b.x + x] = (uint16_t)clampi(out, 0, a.maxv);
+  if (s == 0) {
+    /* the block's dependency cells: final once this launch and the
+     * transforms after it have run, which the queue launch follows */
+    const int sh = a.lay.cell_shift[p];
+    const int cw = max(b.w >> sh, 1), ch = max(b.h >> sh, 1);
+    for (int k = 0; k < cw * ch; ++k) {
+      const int cx = (b.x >> sh) + k % cw, cy = (b.y >> sh) + k / cw;
+      if (cx < a.lay.cells_w[p] && cy < a.lay.cells_h[p])
+        a.cells[a.lay.cell_off[p] + (int64_t)cy * a.lay.cells_w[p] + cx] = a.epoch;
+    }
+  }
 }
 
 __global__ __launch_bounds__(64) void k_queue(ReconArgs a) {
@@ -717,7 +737,7 @@ __global__ __launch_bounds__(64) void k_queue(ReconArgs a) {
     const int bi = __builtin_amdgcn_readfirstlane(a.plan.queue_list[idx]);
     const DevBlock& b = a.plan.blocks[bi];
     wait_cells(a, b);
-    reconstruct(a, b, s_work, s_pred);
+    reconstruct_intra(a, b, s_work, s_pred);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     mark_cells(a, b, true);
     wsync();
@@ -732,9 +752,25 @@ __global__ void k_zero(uint32_t* p, int n) {
 }  // namespace
 
 hipError_t launch_recon(const ReconArgs& a, int part, int cus, hipStream_t s, int* launches) {
-  if (part == 0 && a.plan.n_inter > 0) {
-    hipLaunchKernelGGL(k_inter, dim3(a.plan.n_inter), dim3(64), 0, s, a);
-    ++*launches;
+  if (part == 0) {
+    if (a.plan.mc_total > 0) {
+      hipLaunchKernelGGL(k_mc, dim3(a.plan.mc_chunks), dim3(kMcChunk), 0, s, a);
+      ++*launches;
+    }
+    const int per[kTxClasses] = {16, 8, 4, 2, 1};
+    for (int c = 0; c < kTxClasses; ++c) {
+      const int n = a.plan.itx_n[c];
+      if (n <= 0) continue;
+      const dim3 grid((n + per[c] - 1) / per[c]);
+      switch (c) {
+        case 0: hipLaunchKernelGGL(k_itx<4>, grid, dim3(64), 0, s, a, c); break;
+        case 1: hipLaunchKernelGGL(k_itx<8>, grid, dim3(64), 0, s, a, c); break;
+        case 2: hipLaunchKernelGGL(k_itx<16>, grid, dim3(64), 0, s, a, c); break;
+        case 3: hipLaunchKernelGGL(k_itx<32>, grid, dim3(64), 0, s, a, c); break;
+        default: hipLaunchKernelGGL(k_itx<64>, grid, dim3(64), 0, s, a, c); break;
+      }
+      ++*launches;
+    }
   }
   if (part == 1 && a.plan.n_queue > 0) {
     hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, a.queue_counter, 1);

@@ -424,8 +424,8 @@ int av1h_get_stats(const av1h_ctx* ctx, av1h_stats* out);
  * excluded (the reference's env-gated timers,
  * src/backends/cuda/cuda_backend.cpp:1048-1056, as an API). */
 enum {
-  AV1H_STAGE_INTER = 0,   /* k_inter: blocks that read only references       */
-  AV1H_STAGE_QUEUE = 1,   /* k_queue: the dependency queue (intra)           */
+  AV1H_STAGE_INTER = 0,   /* motion compensation and every inverse transform */
+  AV1H_STAGE_QUEUE = 1,   /* intra prediction: the dependency queue          */
   AV1H_STAGE_DEBLOCK = 2, /* both deblocking passes                          */
   AV1H_STAGE_CDEF = 3,    /* CDEF                                            */
   AV1H_STAGE_SUPERRES = 4,/* upscaling (the frame and the pre-CDEF copy)     */
@@ -437,9 +437,9 @@ enum {
 void av1h_set_stage_timing(av1h_ctx* ctx, int enable);
 /* Accumulated device milliseconds, decodes, and compulsory HBM bytes per stage
  * since timing was enabled (a stage a frame skips is not counted for it).
- * Compulsory bytes, at 16 bit samples: reconstruction 2 B written per sample,
- * plus 2 B per reference read and 2 B of coefficients where transforms cover
- * it; deblocking, CDEF, grain and the reference store 4 B (one read, one
+ * Compulsory bytes, at 16 bit samples: motion compensation and transforms 2 B
+ * written per inter sample, 2 B per reference read, 2 B of coefficients per
+ * transform sample; intra prediction 2 B written per sample; deblocking, CDEF, grain and the reference store 4 B (one read, one
  * write) per sample they cover; restoration 4 B per sample plus the two
  * boundary rows above and below each stripe; upscaling 2 B per coded and 2 B
  * per upscaled sample, for each picture it upscales. Any argument may be

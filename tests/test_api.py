@@ -1,0 +1,83 @@
+"""The library boundary: every entry point include/av1hip.h declares is
+exported by libav1hip.so, the C++ Decoder API's host contracts hold, and the
+C++ API decodes the golden clips bit exact on the GPU (directly and through
+plan_frame / decode_reference_planned)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+import av1ongpu_amd as pkg
+from golden_cases import case_id, load_cases
+
+REPO = pkg.REPO_DIR
+API_CHECK = os.path.join(REPO, "tests", "cpp", "api_check")
+
+
+def declared_functions():
+    with open(os.path.join(REPO, "include", "av1hip.h")) as fh:
+        text = re.sub(r"/\*.*?\*/", "", fh.read(), flags=re.S)
+    names = set()
+    for m in re.finditer(r"^(?!static)[A-Za-z_][\w\s\*]*?\b(av1h_\w+)\s*\(", text, flags=re.M):
+        names.add(m.group(1))
+    return sorted(names)
+
+
+def test_header_declares_the_entry_points():
+    names = declared_functions()
+    for want in ("av1h_create", "av1h_decode", "av1h_decode_wire", "av1h_release_reference", "av1h_to_nv12",
+                 "av1h_to_rgb", "av1h_plan_frame", "av1h_decode_planned", "av1h_reserve_reference"):
+        assert want in names
+    assert "av1h_wire_parse" not in names  # static inline helpers are not exports
+
+
+def test_libav1hip_exports_every_declared_symbol():
+    lib = ctypes.CDLL(os.path.join(REPO, "av1ongpu_amd", "libav1hip.so"))
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_libraries_load():
+    for so in ("libav1syn.so", "libav1gpu.so"):
+        ctypes.CDLL(os.path.join(REPO, "av1ongpu_amd", so))
+    ctypes.CDLL(os.path.join(REPO, "oracle", "liboracle.so"))
+
+
+def test_cpp_host_contracts():
+    r = subprocess.run([API_CHECK, "host"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+
+def test_wire_roundtrip():
+    lib = pkg.syn_lib()
+    for wl, seed in (("c1_mini", 1), ("c2_mini", 2), ("contract_mini", 1)):
+        clip = pkg.Clip(wl, seed)
+        for f in range(clip.frames):
+            w = clip.wire(f)
+            assert lib.av1s_wire_roundtrip(w, len(w)) == 1, (wl, seed, f)
+
+
+GPU_CASES = [c for c in load_cases() if (c["workload"], c["seed"]) in
+             {("c1_mini", 1), ("c2_mini", 1), ("contract_mini", 2), ("c1_mini", 11)}]
+
+
+def _run_decode(case, planned):
+    args = [API_CHECK, "decode", case["workload"], str(case["seed"]), str(case["nv12_pitch"])]
+    if planned:
+        args.append("planned")
+    args += [f"{k}={v}" for k, v in case["overrides"].items()]
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    rows = [line.split() for line in r.stdout.strip().splitlines()]
+    assert len(rows) == len(case["frames"])
+    for f, (row, exp) in enumerate(zip(rows, case["frames"])):
+        assert (row[2], row[3], row[4]) == (exp["yuv16"], exp["nv12"], exp["rgb"]), f"{case_id(case)} frame {f}"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", GPU_CASES, ids=[case_id(c) for c in GPU_CASES])
+@pytest.mark.parametrize("planned", [False, True], ids=["direct", "planned"])
+def test_cpp_api_decodes_golden(case, planned):
+    _run_decode(case, planned)
