@@ -107,6 +107,8 @@ struct Ctx {
   int64_t cells_cap = 0;
   uint32_t epoch = 0;
   uint32_t* counters = nullptr; /* [0] queue counter, [1] error word */
+  int32_t* cdef_dir = nullptr;   /* CDEF direction map                */
+  size_t cdef_dir_cap = 0;
   int16_t* grain_tmpl = nullptr;
   uint8_t* grain_offs = nullptr;
   size_t grain_offs_cap = 0;
@@ -206,6 +208,8 @@ struct Host {
   std::vector<DevTx> tx;
   std::vector<int32_t> inter_list, queue_list;
   std::vector<int32_t> mc_px0, mc_chunk;
+  std::vector<int32_t> db_chunk, cdef_dir_list;
+  std::vector<int32_t> lr_tiles; /* pairs */
   std::vector<int32_t> itx[kTxClasses];
   std::vector<int32_t> coef, residual;
   std::vector<uint8_t> palette;
@@ -512,6 +516,19 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
       ++nedge[pass];
       H->deblock.push_back(d);
     }
+  /* the first edge of every 256 line chunk of each pass */
+  H->db_chunk.clear();
+  {
+    size_t e0 = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+      const size_t n = (size_t)nedge[pass];
+      for (int64_t k = 0, e = 0; k < (lines[pass] + 255) / 256; ++k) {
+        while ((size_t)e + 1 < n && H->deblock[e0 + e + 1].line0 <= k * 256) ++e;
+        H->db_chunk.push_back((int32_t)e);
+      }
+      e0 += n;
+    }
+  }
   /* CDEF */
   H->cdef.clear();
   for (int64_t i = 0; i < f.num_cdef; ++i) {
@@ -519,8 +536,30 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
     if (cb.plane < 0 || cb.plane > 2 || cb.x < 0 || cb.y < 0 || cb.x >= L.grid_w[cb.plane] ||
         cb.y >= L.grid_h[cb.plane] || cb.damping < 0 || cb.damping > 15 || cb.dir > 7)
       return set_err(c, AV1H_ERR_INVALID, "cdef block %lld", (long long)i);
+    const int bw = cb.plane ? 8 >> hdr.ss_x : 8, bh = cb.plane ? 8 >> hdr.ss_y : 8;
+    if (cb.x % bw || cb.y % bh)
+      return set_err(c, AV1H_ERR_UNSUPPORTED, "cdef block %lld is not on the %dx%d grid", (long long)i, bw, bh);
     DevCdef d{cb.plane, cb.x, cb.y, cb.pri_strength, cb.sec_strength, cb.damping, cb.dir, 0};
     H->cdef.push_back(d);
+  }
+  /* luma blocks first (one launch per block size), and the luma 8x8 blocks
+   * whose direction is searched */
+  std::stable_partition(H->cdef.begin(), H->cdef.end(), [](const DevCdef& d) { return d.plane == 0; });
+  int32_t n_cdef_luma = 0;
+  for (const DevCdef& d : H->cdef) n_cdef_luma += d.plane == 0;
+  H->cdef_dir_list.clear();
+  {
+    const int bw8 = L.grid_w[0] >> 3, bh8 = L.grid_h[0] >> 3;
+    std::vector<uint8_t> need((size_t)bw8 * bh8, 0);
+    for (const DevCdef& d : H->cdef) {
+      if (d.dir >= 0) continue;
+      const int lx = d.plane ? d.x << hdr.ss_x : d.x, ly = d.plane ? d.y << hdr.ss_y : d.y;
+      const int bx = lx >> 3, by = ly >> 3;
+      if (bx < bw8 && by < bh8 && !need[(size_t)by * bw8 + bx]) {
+        need[(size_t)by * bw8 + bx] = 1;
+        H->cdef_dir_list.push_back(bx | (by << 16));
+      }
+    }
   }
   /* loop restoration jobs: one per unit per stripe (engine.cpp:1382-1462) */
   H->lr.clear();
@@ -580,6 +619,13 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
     }
   }
   plan->uses_lr = !H->lr.empty();
+  H->lr_tiles.clear();
+  for (int jn = 0; jn < (int)H->lr.size(); ++jn)
+    for (int ty = 0; ty < H->lr[jn].h; ty += kLrTileRows)
+      for (int tx = 0; tx < H->lr[jn].w; tx += kLrTileCols) {
+        H->lr_tiles.push_back(jn);
+        H->lr_tiles.push_back(tx | (ty << 16));
+      }
   /* film grain parameters */
   std::memset(&H->grain, 0, sizeof(H->grain));
   plan->has_grain = hdr.grain.apply != 0;
@@ -681,7 +727,7 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
     size_t bytes;
     size_t off;
   };
-  Part parts[11 + 2 + kTxClasses] = {
+  Part parts[11 + 2 + kTxClasses + 3] = {
       {H->blocks.data(), H->blocks.size() * sizeof(DevBlock), 0},
       {H->tx.data(), H->tx.size() * sizeof(DevTx), 0},
       {nullptr, 0, 0},
@@ -700,6 +746,9 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
       {H->itx[2].data(), H->itx[2].size() * 4, 0},
       {H->itx[3].data(), H->itx[3].size() * 4, 0},
       {H->itx[4].data(), H->itx[4].size() * 4, 0},
+      {H->db_chunk.data(), H->db_chunk.size() * 4, 0},
+      {H->cdef_dir_list.data(), H->cdef_dir_list.size() * 4, 0},
+      {H->lr_tiles.data(), H->lr_tiles.size() * 4, 0},
   };
   size_t total = 0;
   for (Part& pt : parts) {
@@ -741,6 +790,12 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
   }
   P.mc_total = mc_total;
   P.mc_chunks = mc_chunks;
+  P.db_chunk = reinterpret_cast<const int32_t*>(db + parts[13 + kTxClasses].off);
+  P.cdef_dir_list = reinterpret_cast<const int32_t*>(db + parts[14 + kTxClasses].off);
+  P.n_cdef_luma = n_cdef_luma;
+  P.n_cdef_dir = (int32_t)H->cdef_dir_list.size();
+  P.lr_tiles = reinterpret_cast<const int2*>(db + parts[15 + kTxClasses].off);
+  P.n_lr_tiles = (int32_t)(H->lr_tiles.size() / 2);
   P.n_inter = (int)H->inter_list.size();
   P.n_queue = (int)H->queue_list.size();
   P.n_deblock_v = nedge[0];
@@ -799,6 +854,14 @@ int prepare(Ctx* c, const Layout& L) {
     c->cap = (size_t)L.size;
     std::memset(c->used, 0, sizeof(c->used));
     c->last_slot = -1;
+  }
+  const size_t ndir = (size_t)(L.grid_w[0] >> 3) * (L.grid_h[0] >> 3) * 2;
+  if (c->cdef_dir_cap < ndir) {
+    if (c->cdef_dir) (void)hipFree(c->cdef_dir);
+    c->cdef_dir = nullptr;
+    c->cdef_dir_cap = 0;
+    HIP_OK(c, hipMalloc(&c->cdef_dir, ndir * 4));
+    c->cdef_dir_cap = ndir;
   }
   if (c->cells_cap < L.cells) {
     if (c->cells) (void)hipFree(c->cells);
@@ -902,6 +965,8 @@ int run_plan(Ctx* c, Plan* plan, int keep, int reserved, const av1h_planes* out,
   fa.lr_curr = c->lr_curr;
   fa.lr_cdef = c->lr_cdef;
   fa.lr_prev_up = c->lr_prev_up;
+  fa.cdef_dir = c->cdef_dir;
+  fa.cdef_dir_w = L.grid_w[0] >> 3;
   fa.sharpness = plan->hdr.sharpness;
   fa.maxv = ra.maxv;
   if (plan->dev.lines_v + plan->dev.lines_h > 0) {
@@ -1097,7 +1162,7 @@ void av1h_destroy(av1h_ctx* h) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (void* p : {(void*)c->frame, (void*)c->scratch, (void*)c->lr_curr, (void*)c->lr_cdef, (void*)c->lr_prev_up,
                   (void*)c->pool, (void*)c->cells, (void*)c->counters, (void*)c->grain_tmpl, (void*)c->grain_offs,
-                  (void*)c->pack})
+                  (void*)c->pack, (void*)c->cdef_dir})
     if (p) (void)hipFree(p);
   if (c->host_pin) (void)hipHostFree(c->host_pin);
   free_plan(&c->scratch_plan);

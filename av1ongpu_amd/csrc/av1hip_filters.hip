@@ -7,6 +7,8 @@
 #include "av1hip_internal.hpp"
 #include "spec_tables.h"
 
+#include <climits>
+
 namespace av1hip {
 namespace {
 
@@ -19,6 +21,29 @@ __device__ __forceinline__ int rsr64(int64_t v, int n) { return (int)((v + ((int
 /* ---- deblocking (av1gpu_kernels.h:2422-2572) --------------------------- */
 
 __device__ __forceinline__ int f4c(int v, int bd) { return clampi(v, -(1 << (bd - 1)), (1 << (bd - 1)) - 1); }
+
+/* The flat filters: outputs p(N-1)..q(N-1) from taps p(N)..q(N), the centre
+ * N2 taps weighted twice, all indices fixed at compile time
+ * (av1gpu_kernels.h:2540-2568). */
+template <int N, int N2, int LG>
+__device__ __forceinline__ void wide_filter(uint16_t* b, int s, int maxv) {
+  int v[2 * N + 2];
+#pragma unroll
+  for (int i = -(N + 1); i <= N; ++i) v[i + N + 1] = b[(2 + i) * s];
+  int res[2 * N];
+#pragma unroll
+  for (int i = -N; i < N; ++i) {
+    int t = 0;
+#pragma unroll
+    for (int k = -N; k <= N; ++k) {
+      const int idx = i + k < -(N + 1) ? -(N + 1) : (i + k > N ? N : i + k);
+      t += v[idx + N + 1] * ((k <= N2 && k >= -N2) ? 2 : 1);
+    }
+    res[i + N] = rnd2(t, LG);
+  }
+#pragma unroll
+  for (int i = -N; i < N; ++i) b[(2 + i) * s] = (uint16_t)clampi(res[i + N], 0, maxv);
+}
 
 /* one line across one edge: b points at p1, s is the step across the edge */
 __device__ void deblock_line(uint16_t* b, int s, int level, int sharp, int fs, int chroma, int bd,
@@ -70,37 +95,27 @@ __device__ void deblock_line(uint16_t* b, int s, int level, int sharp, int fs, i
     }
     return;
   }
-  const int lg = (fs == 8 || !flat2) ? 3 : 4;
-  const int n = lg == 4 ? 6 : (chroma ? 2 : 3);
-  const int n2 = (lg == 3 && !chroma) ? 0 : 1;
-  int v[14];
-  for (int i = -(n + 1); i <= n; ++i) v[i + 7] = b[(2 + i) * s];
-  int res[12];
-  for (int i = -n; i < n; ++i) {
-    int t = 0;
-    for (int k = -n; k <= n; ++k) t += v[clampi(i + k, -(n + 1), n) + 7] * (abs(k) <= n2 ? 2 : 1);
-    res[i + n] = rnd2(t, lg);
-  }
-  for (int i = -n; i < n; ++i) b[(2 + i) * s] = (uint16_t)clampi(res[i + n], 0, maxv);
+  if (fs >= 16 && flat2)
+    wide_filter<6, 1, 4>(b, s, maxv);
+  else if (chroma)
+    wide_filter<2, 1, 3>(b, s, maxv);
+  else
+    wide_filter<3, 0, 3>(b, s, maxv);
 }
 
-/* one thread per line; edges of one direction never share samples */
-__global__ void k_deblock(FilterArgs a, int first, int count, int lines) {
+/* One thread per line; edges of one direction never share samples. The edge
+ * owning line g is found from the first edge of the thread's 256 line chunk
+ * (a table the host builds) and a short forward scan. */
+__global__ __launch_bounds__(256) void k_deblock(FilterArgs a, int first, int count, int lines,
+                                                 const int32_t* chunk) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= lines) return;
-  /* binary search of the edge owning line g (prefix in line0) */
-  int lo = first, hi = first + count - 1;
-  const int base = a.plan.deblock[first].line0;
-  while (lo < hi) {
-    const int m = (lo + hi + 1) >> 1;
-    if (a.plan.deblock[m].line0 - base <= g)
-      lo = m;
-    else
-      hi = m - 1;
-  }
-  const DevDeblock& e = a.plan.deblock[lo];
+  const DevDeblock* edges = a.plan.deblock + first;
+  int i = chunk[blockIdx.x];
+  while (i + 1 < count && edges[i + 1].line0 <= g) ++i;
+  const DevDeblock& e = edges[i];
   if (e.level == 0) return;
-  const int k = g - (e.line0 - base);
+  const int k = g - e.line0;
   const int p = e.plane, st = a.lay.stride[p];
   uint16_t* plane = a.frame + a.lay.plane_off[p];
   uint16_t* b;
@@ -117,52 +132,6 @@ __global__ void k_deblock(FilterArgs a, int first, int count, int lines) {
 
 /* ---- CDEF (av1gpu_kernels.h:2601-2775) ---------------------------------- */
 
-__device__ void cdef_direction(const uint16_t* l, int stride, int bd, int* dir_out, int* var_out) {
-  const int div[9] = {0, 840, 420, 280, 210, 168, 140, 120, 105};
-  int part[8][15];
-  int cost[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (int i = 0; i < 8; ++i)
-    for (int j = 0; j < 15; ++j) part[i][j] = 0;
-  for (int i = 0; i < 8; ++i)
-    for (int k = 0; k < 8; ++k) {
-      const int x = (l[i * stride + k] >> (bd - 8)) - 128;
-      part[0][i + k] += x;
-      part[1][i + k / 2] += x;
-      part[2][i] += x;
-      part[3][3 + i - k / 2] += x;
-      part[4][7 + i - k] += x;
-      part[5][3 - i / 2 + k] += x;
-      part[6][k] += x;
-      part[7][i / 2 + k] += x;
-    }
-  for (int i = 0; i < 8; ++i) {
-    cost[2] += part[2][i] * part[2][i];
-    cost[6] += part[6][i] * part[6][i];
-  }
-  cost[2] *= div[8];
-  cost[6] *= div[8];
-  for (int i = 0; i < 7; ++i) {
-    cost[0] += (part[0][i] * part[0][i] + part[0][14 - i] * part[0][14 - i]) * div[i + 1];
-    cost[4] += (part[4][i] * part[4][i] + part[4][14 - i] * part[4][14 - i]) * div[i + 1];
-  }
-  cost[0] += part[0][7] * part[0][7] * div[8];
-  cost[4] += part[4][7] * part[4][7] * div[8];
-  for (int i = 1; i < 8; i += 2) {
-    for (int k = 0; k < 5; ++k) cost[i] += part[i][3 + k] * part[i][3 + k];
-    cost[i] *= div[8];
-    for (int k = 0; k < 3; ++k)
-      cost[i] += (part[i][k] * part[i][k] + part[i][10 - k] * part[i][10 - k]) * div[2 * k + 2];
-  }
-  int best = 0, d = 0;
-  for (int i = 0; i < 8; ++i)
-    if (cost[i] > best) {
-      best = cost[i];
-      d = i;
-    }
-  *dir_out = d;
-  *var_out = (best - cost[(d + 4) & 7]) >> 10;
-}
-
 __device__ __forceinline__ int flog2(int v) { return 31 - __builtin_clz(v); }
 
 __device__ __forceinline__ int constrain(int diff, int thr, int damping) {
@@ -173,77 +142,150 @@ __device__ __forceinline__ int constrain(int diff, int thr, int damping) {
   return diff < 0 ? -v : v;
 }
 
-/* one thread per CDEF block, reading the deblocked snapshot */
-__global__ void k_cdef(FilterArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.plan.n_cdef) return;
-  const DevCdef c = a.plan.cdef[i];
-  const int dirs[8][2][2] = {{{-1, 1}, {-2, 2}}, {{0, 1}, {-1, 2}}, {{0, 1}, {0, 2}}, {{0, 1}, {1, 2}},
-                             {{1, 1}, {2, 2}},   {{1, 0}, {2, 1}},  {{1, 0}, {2, 0}}, {{1, 0}, {2, -1}}};
-  const int uvdir[2][2][8] = {{{0, 1, 2, 3, 4, 5, 6, 7}, {1, 2, 2, 2, 3, 4, 6, 0}},
-                              {{7, 0, 2, 4, 5, 6, 6, 6}, {0, 1, 2, 3, 4, 5, 6, 7}}};
-  const int pri_taps[2][2] = {{4, 2}, {3, 3}};
-  const int sec_taps[2][2] = {{2, 1}, {2, 1}};
+__constant__ const int kCdefDirs[8][2][2] = {{{-1, 1}, {-2, 2}}, {{0, 1}, {-1, 2}}, {{0, 1}, {0, 2}},
+                                             {{0, 1}, {1, 2}},   {{1, 1}, {2, 2}},   {{1, 0}, {2, 1}},
+                                             {{1, 0}, {2, 0}},   {{1, 0}, {2, -1}}};
+__constant__ const int kCdefUvDir[2][2][8] = {{{0, 1, 2, 3, 4, 5, 6, 7}, {1, 2, 2, 2, 3, 4, 6, 0}},
+                                              {{7, 0, 2, 4, 5, 6, 6, 6}, {0, 1, 2, 3, 4, 5, 6, 7}}};
+
+/* Direction search of luma 8x8 blocks (av1gpu_kernels.h:2601-2660), one wave
+ * per block, one lane per sample: the eight directional line sums are
+ * accumulated with LDS atomics, lanes 0-7 weigh one direction each. Writes
+ * (direction, variance) for the block into `dirmap`. */
+__global__ __launch_bounds__(256) void k_cdef_dir(FilterArgs a) {
+  __shared__ int s_part[4][8][16];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int bi = blockIdx.x * 4 + wave;
+  const bool live = bi < a.plan.n_cdef_dir;
+  int* part = &s_part[wave][0][0];
+  for (int i = lane; i < 128; i += 64) part[i] = 0;
+  __syncthreads();
+  int bx = 0, by = 0;
+  if (live) {
+    const int pos = a.plan.cdef_dir_list[bi];
+    bx = pos & 0xffff;
+    by = pos >> 16;
+    const int i = lane >> 3, k = lane & 7;
+    const int bd = a.lay.bit_depth;
+    const int x = (a.scratch[a.lay.plane_off[0] + (size_t)(by * 8 + i) * a.lay.stride[0] + bx * 8 + k] >> (bd - 8)) -
+                  128;
+    atomicAdd(&part[0 * 16 + i + k], x);
+    atomicAdd(&part[1 * 16 + i + k / 2], x);
+    atomicAdd(&part[2 * 16 + i], x);
+    atomicAdd(&part[3 * 16 + 3 + i - k / 2], x);
+    atomicAdd(&part[4 * 16 + 7 + i - k], x);
+    atomicAdd(&part[5 * 16 + 3 - i / 2 + k], x);
+    atomicAdd(&part[6 * 16 + k], x);
+    atomicAdd(&part[7 * 16 + i / 2 + k], x);
+  }
+  __syncthreads();
+  if (!live) return;
+  const int div[9] = {0, 840, 420, 280, 210, 168, 140, 120, 105};
+  int cost = 0;
+  if (lane < 8) {
+    const int d = lane;
+    const int* pd = part + d * 16;
+    if (d == 2 || d == 6) {
+      for (int i = 0; i < 8; ++i) cost += pd[i] * pd[i];
+      cost *= div[8];
+    } else if (d == 0 || d == 4) {
+      for (int i = 0; i < 7; ++i) cost += (pd[i] * pd[i] + pd[14 - i] * pd[14 - i]) * div[i + 1];
+      cost += pd[7] * pd[7] * div[8];
+    } else {
+      for (int k = 0; k < 5; ++k) cost += pd[3 + k] * pd[3 + k];
+      cost *= div[8];
+      for (int k = 0; k < 3; ++k) cost += (pd[k] * pd[k] + pd[10 - k] * pd[10 - k]) * div[2 * k + 2];
+    }
+  }
+  /* the first direction with the largest cost (a strict > scan from 0) */
+  int best = lane < 8 ? cost : INT_MIN, bdir = lane;
+#pragma unroll
+  for (int o = 1; o < 8; o <<= 1) {
+    const int oc = __shfl_xor(best, o), od = __shfl_xor(bdir, o);
+    if (oc > best || (oc == best && od < bdir)) {
+      best = oc;
+      bdir = od;
+    }
+  }
+  const int bcost = __shfl(best, 0);
+  /* a block whose costs are all zero keeps direction 0, as the scan does */
+  const int dd = bcost > 0 ? __shfl(bdir, 0) : 0;
+  const int opp = __shfl(cost, (dd + 4) & 7);
+  if (lane == 0) {
+    const int var = ((bcost > 0 ? bcost : 0) - opp) >> 10;
+    int* m = a.cdef_dir + 2 * (by * a.cdef_dir_w + bx);
+    m[0] = dd;
+    m[1] = var;
+  }
+}
+
+/* CDEF of the listed blocks (av1gpu_kernels.h:2662-2775), one thread per
+ * sample, reading the deblocked snapshot; blocks of `S` samples, 256 / S per
+ * workgroup. Samples of unlisted blocks are already final in the frame. */
+template <int S>
+__global__ __launch_bounds__(256) void k_cdef(FilterArgs a, int first, int count) {
+  const int bi = blockIdx.x * (256 / S) + threadIdx.x / S;
+  if (bi >= count) return;
+  const DevCdef c = a.plan.cdef[first + bi];
   const Layout& L = a.lay;
   const int p = c.plane, uv = p > 0;
   const int bd = L.bit_depth, cs = bd - 8;
   const int st = L.stride[p];
-  const int w = uv ? (8 >> L.ss_x) : 8, h = uv ? (8 >> L.ss_y) : 8;
+  const int w = uv ? (8 >> L.ss_x) : 8;
+  const int r = (threadIdx.x % S) / w, k = (threadIdx.x % S) % w;
   const int pw = L.grid_w[p], ph = L.grid_h[p];
-  const uint16_t* src = a.scratch + L.plane_off[p] + (size_t)c.y * st + c.x;
-  uint16_t* dst = a.frame + L.plane_off[p] + (size_t)c.y * st + c.x;
-  int pri = c.pri << cs, sec = c.sec << cs;
+  if (c.x + k >= pw || c.y + r >= ph) return;
+  int pri = c.pri << cs;
+  const int sec = c.sec << cs;
   const int damping = c.damping + cs - uv;
   int dir = c.dir;
   if (dir < 0) {
-    int yd = 0, var = 0;
     const int lx = uv ? c.x << L.ss_x : c.x, ly = uv ? c.y << L.ss_y : c.y;
-    cdef_direction(a.scratch + L.plane_off[0] + (size_t)ly * L.stride[0] + lx, L.stride[0], bd, &yd, &var);
+    const int* m = a.cdef_dir + 2 * ((ly >> 3) * a.cdef_dir_w + (lx >> 3));
+    const int yd = m[0], var = m[1];
     if (uv) {
-      dir = pri == 0 ? 0 : uvdir[L.ss_x][L.ss_y][yd];
+      dir = pri == 0 ? 0 : kCdefUvDir[L.ss_x][L.ss_y][yd];
     } else {
       const int vs = (var >> 6) ? min(flog2(var >> 6), 12) : 0;
       dir = pri == 0 ? 0 : yd;
       pri = var ? ((pri * (4 + vs) + 8) >> 4) : 0;
     }
   }
-  if (pri == 0 && sec == 0) {
-    for (int r = 0; r < h && c.y + r < ph; ++r)
-      for (int k = 0; k < w && c.x + k < pw; ++k) dst[r * st + k] = src[r * st + k];
-    return;
-  }
+  if (pri == 0 && sec == 0) return; /* the frame already holds the snapshot's value */
+  const uint16_t* src = a.scratch + L.plane_off[p] + (size_t)(c.y + r) * st + c.x + k;
   const int ts = (pri >> cs) & 1;
-  for (int r = 0; r < h; ++r)
-    for (int k = 0; k < w; ++k) {
-      if (c.x + k >= pw || c.y + r >= ph) continue;
-      const int x = src[r * st + k];
-      int sum = 0, lo = x, hi = x;
-      for (int t = 0; t < 2; ++t)
-        for (int sg = -1; sg <= 1; sg += 2) {
-          int dy = sg * dirs[dir][t][0], dx = sg * dirs[dir][t][1];
-          int px = c.x + k + dx, py = c.y + r + dy;
-          if (px >= 0 && px < pw && py >= 0 && py < ph) {
-            const int v = src[(r + dy) * st + k + dx];
-            sum += pri_taps[ts][t] * constrain(v - x, pri, damping);
-            lo = min(lo, v);
-            hi = max(hi, v);
-          }
-          for (int o = -2; o <= 2; o += 4) {
-            const int sd = (dir + o) & 7;
-            dy = sg * dirs[sd][t][0];
-            dx = sg * dirs[sd][t][1];
-            px = c.x + k + dx;
-            py = c.y + r + dy;
-            if (px >= 0 && px < pw && py >= 0 && py < ph) {
-              const int v = src[(r + dy) * st + k + dx];
-              sum += sec_taps[ts][t] * constrain(v - x, sec, damping);
-              lo = min(lo, v);
-              hi = max(hi, v);
-            }
-          }
+  const int pt0 = ts ? 3 : 4, pt1 = ts ? 3 : 2;
+  const int x = src[0];
+  int sum = 0, lo = x, hi = x;
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int sg = -1; sg <= 1; sg += 2) {
+      int dy = sg * kCdefDirs[dir][t][0], dx = sg * kCdefDirs[dir][t][1];
+      int px = c.x + k + dx, py = c.y + r + dy;
+      if (px >= 0 && px < pw && py >= 0 && py < ph) {
+        const int v = src[dy * st + dx];
+        sum += (t ? pt1 : pt0) * constrain(v - x, pri, damping);
+        lo = min(lo, v);
+        hi = max(hi, v);
+      }
+#pragma unroll
+      for (int o = -2; o <= 2; o += 4) {
+        const int sd = (dir + o) & 7;
+        dy = sg * kCdefDirs[sd][t][0];
+        dx = sg * kCdefDirs[sd][t][1];
+        px = c.x + k + dx;
+        py = c.y + r + dy;
+        if (px >= 0 && px < pw && py >= 0 && py < ph) {
+          const int v = src[dy * st + dx];
+          sum += (t ? 1 : 2) * constrain(v - x, sec, damping);
+          lo = min(lo, v);
+          hi = max(hi, v);
         }
-      dst[r * st + k] = (uint16_t)clampi(clampi(x + ((8 + sum - (sum < 0)) >> 4), lo, hi), 0, a.maxv);
+      }
     }
+  a.frame[L.plane_off[p] + (size_t)(c.y + r) * st + c.x + k] =
+      (uint16_t)clampi(clampi(x + ((8 + sum - (sum < 0)) >> 4), lo, hi), 0, a.maxv);
 }
 
 /* ---- superres (av1gpu_kernels.h:2387-2402) ------------------------------ */
@@ -285,95 +327,29 @@ struct LrCtx {
   }
 };
 
-__device__ int wiener_px(const LrCtx& c, const DevLr& j, int bd, int x, int y) {
-  const int r0 = bd == 12 ? 5 : 3, r1 = bd == 12 ? 9 : 11;
-  const int off = 1 << (bd + 7 - r0 - 1);
-  const int lim = (1 << (bd + 1 + 7 - r0)) - 1;
-  int vt[7], ht[7];
-  for (int t = 0; t < 3; ++t) {
-    vt[t] = vt[6 - t] = j.vf[t];
-    ht[t] = ht[6 - t] = j.hf[t];
-  }
-  vt[3] = 128 - 2 * (j.vf[0] + j.vf[1] + j.vf[2]);
-  ht[3] = 128 - 2 * (j.hf[0] + j.hf[1] + j.hf[2]);
-  int sum = 0;
-  for (int t = 0; t < 7; ++t) {
-    int s = 0;
-    for (int k = 0; k < 7; ++k) s += ht[k] * c.src(x + k - 3, y + t - 3);
-    sum += vt[t] * clampi(rsr64(s, r0), -off, lim - off);
-  }
-  return rsr64(sum, r1);
-}
+/* Loop restoration of one tile (at most kLrTileW x kLrTileH samples of one
+ * restoration job, i.e. one unit inside one 64 row stripe) per workgroup:
+ * the source window with its 3 sample halo is staged in LDS through the
+ * stripe rule (LrCtx::src), then
+ *   Wiener - the horizontal 7 tap pass into an LDS intermediate, rounded and
+ *            clamped, then the vertical pass (av1gpu_kernels.h:2832-2870);
+ *   SGR    - the box sums, A and B of both passes at every position the 3x3
+ *            neighbourhoods of the tile touch, into LDS, then the weighted
+ *            neighbourhood sums and the projection (av1gpu_kernels.h:2872-2982).
+ * Each intermediate is computed once instead of once per output sample. */
+constexpr int kLrTileW = kLrTileCols, kLrTileH = kLrTileRows;
+constexpr int kLrSrcW = kLrTileW + 6, kLrSrcH = kLrTileH + 6;
+constexpr int kLrAbW = kLrTileW + 2, kLrAbH = kLrTileH + 2;
 
-__device__ void sgr_ab(const LrCtx& c, int bd, int x, int y, int d, int eps, int* ao, int* bo) {
-  const int r = d >> 1, n = d * d;
-  const int64_t n2e = (int64_t)n * n * eps;
-  const int64_t mt = (((int64_t)1 << 20) + n2e / 2) / n2e;
-  int64_t sa = 0, sb = 0;
-  for (int dy = -r; dy <= r; ++dy)
-    for (int dx = -r; dx <= r; ++dx) {
-      const int v = c.src(x + dx, y + dy);
-      sa += (int64_t)v * v;
-      sb += v;
-    }
-  const int64_t braw = sb;
-  if (bd > 8) {
-    sa = rsr64(sa, 2 * (bd - 8));
-    sb = rsr64(sb, bd - 8);
-  }
-  int64_t p = sa * n - sb * sb;
-  if (p < 0) p = 0;
-  const int z = rsr64(p * mt, 20);
-  const int a2 = z >= 255 ? 256 : (z == 0 ? 1 : ((z << 8) + z / 2) / (z + 1));
-  const int inv = ((1 << 12) + n / 2) / n;
-  *ao = a2;
-  *bo = rsr64((int64_t)(256 - a2) * braw * inv, 12);
-}
-
-__device__ int sgr_px(const LrCtx& c, const DevLr& j, int bd, int x, int y) {
-  const int u = c.src(x, y);
-  const int w0 = j.xqd[0], w1 = j.xqd[1], w2 = (1 << 7) - w0 - w1;
-  int flt[2] = {0, 0};
-  for (int pass = 0; pass < 2; ++pass) {
-    const int r = j.r[pass];
-    if (r == 0) continue;
-    int a = 0, b = 0;
-    for (int dy = -1; dy <= 1; ++dy)
-      for (int dx = -1; dx <= 1; ++dx) {
-        const int wgt = pass == 0 ? (((y + dy) & 1) ? (dx == 0 ? 6 : 5) : 0) : ((dx == 0 || dy == 0) ? 4 : 3);
-        if (!wgt) continue;
-        int a2, b2;
-        sgr_ab(c, bd, x + dx, y + dy, 2 * r + 1, j.eps[pass], &a2, &b2);
-        a += wgt * a2;
-        b += wgt * b2;
-      }
-    const int sh = (pass == 0 && (y & 1)) ? 4 : 5;
-    flt[pass] = rsr64((int64_t)a * u + b, 8 + sh - 4);
-  }
-  const int us = u << 4;
-  int64_t v = (int64_t)w1 * us;
-  v += (int64_t)w0 * (j.r[0] ? flt[0] : us);
-  v += (int64_t)w2 * (j.r[1] ? flt[1] : us);
-  return rsr64(v, 4 + 7);
-}
-
-/* one thread per restored sample; job found by binary search on px0 */
-__global__ void k_lr(FilterArgs a, const uint16_t* cdef, const uint16_t* curr) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= a.plan.lr_px) return;
-  int lo = 0, hi = a.plan.n_lr - 1;
-  while (lo < hi) {
-    const int m = (lo + hi + 1) >> 1;
-    if (a.plan.lr[m].px0 <= g)
-      lo = m;
-    else
-      hi = m - 1;
-  }
-  const DevLr& j = a.plan.lr[lo];
-  const int k = g - j.px0;
-  const int y = j.y + k / j.w, x = j.x + k % j.w;
+__global__ __launch_bounds__(256) void k_lr(FilterArgs a, const uint16_t* cdef, const uint16_t* curr) {
+  __shared__ int s_src[kLrSrcH * kLrSrcW];
+  __shared__ int s_mid[4 * kLrAbH * kLrAbW]; /* Wiener rows, or A0 B0 A1 B1 */
+  const int2 tile = a.plan.lr_tiles[blockIdx.x];
+  const DevLr& j = a.plan.lr[tile.x];
+  const int x0 = j.x + (tile.y & 0xffff), y0 = j.y + (tile.y >> 16);
+  const int tw = min(kLrTileW, j.x + j.w - x0), th = min(kLrTileH, j.y + j.h - y0);
   const Layout& L = a.lay;
-  const int p = j.plane;
+  const int p = j.plane, bd = L.bit_depth;
   LrCtx c;
   c.cdef = cdef + L.plane_off[p];
   c.curr = curr + L.plane_off[p];
@@ -382,8 +358,113 @@ __global__ void k_lr(FilterArgs a, const uint16_t* cdef, const uint16_t* curr) {
   c.ph = L.out_h[p];
   c.s0 = j.stripe0;
   c.s1 = j.stripe1;
-  const int v = j.type == 1 ? wiener_px(c, j, L.bit_depth, x, y) : sgr_px(c, j, L.bit_depth, x, y);
-  a.frame[L.plane_off[p] + (size_t)y * L.stride[p] + x] = (uint16_t)clampi(v, 0, a.maxv);
+  for (int i = threadIdx.x; i < (th + 6) * (tw + 6); i += 256) {
+    const int r = i / (tw + 6), q = i - r * (tw + 6);
+    s_src[r * kLrSrcW + q] = c.src(x0 - 3 + q, y0 - 3 + r);
+  }
+  __syncthreads();
+  uint16_t* dst = a.frame + L.plane_off[p] + (size_t)y0 * c.stride + x0;
+  if (j.type == 1) {
+    const int r0 = bd == 12 ? 5 : 3, r1 = bd == 12 ? 9 : 11;
+    const int off = 1 << (bd + 7 - r0 - 1);
+    const int lim = (1 << (bd + 1 + 7 - r0)) - 1;
+    int ht[7], vt[7];
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+      vt[t] = vt[6 - t] = j.vf[t];
+      ht[t] = ht[6 - t] = j.hf[t];
+    }
+    vt[3] = 128 - 2 * (j.vf[0] + j.vf[1] + j.vf[2]);
+    ht[3] = 128 - 2 * (j.hf[0] + j.hf[1] + j.hf[2]);
+    for (int i = threadIdx.x; i < (th + 6) * tw; i += 256) {
+      const int r = i / tw, q = i - r * tw;
+      const int* row = s_src + r * kLrSrcW + q;
+      int sum = 0;
+#pragma unroll
+      for (int k = 0; k < 7; ++k) sum += ht[k] * row[k];
+      s_mid[r * kLrTileW + q] = clampi(rsr64(sum, r0), -off, lim - off);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < th * tw; i += 256) {
+      const int r = i / tw, q = i - r * tw;
+      int sum = 0;
+#pragma unroll
+      for (int t = 0; t < 7; ++t) sum += vt[t] * s_mid[(r + t) * kLrTileW + q];
+      dst[(size_t)r * c.stride + q] = (uint16_t)clampi(rsr64(sum, r1), 0, a.maxv);
+    }
+    return;
+  }
+  /* self guided: A and B of each pass on the (th + 2) x (tw + 2) positions
+   * around the tile */
+  const int npos = (th + 2) * (tw + 2);
+  for (int pass = 0; pass < 2; ++pass) {
+    const int rr = j.r[pass];
+    if (rr == 0) continue;
+    const int d = 2 * rr + 1, n = d * d;
+    const int64_t n2e = (int64_t)n * n * j.eps[pass];
+    const int64_t mt = (((int64_t)1 << 20) + n2e / 2) / n2e;
+    const int inv = ((1 << 12) + n / 2) / n;
+    int* A = s_mid + (2 * pass) * kLrAbH * kLrAbW;
+    int* B = A + kLrAbH * kLrAbW;
+    for (int i = threadIdx.x; i < npos; i += 256) {
+      const int r = i / (tw + 2), q = i - r * (tw + 2);
+      /* position (x0 - 1 + q, y0 - 1 + r); its window starts at src row
+       * r + 2 - rr, column q + 2 - rr of the staged window */
+      int64_t sa = 0, sb = 0;
+      for (int dy = 0; dy < d; ++dy) {
+        const int* row = s_src + (r + 2 - rr + dy) * kLrSrcW + q + 2 - rr;
+        for (int dx = 0; dx < d; ++dx) {
+          const int v = row[dx];
+          sa += (int64_t)v * v;
+          sb += v;
+        }
+      }
+      const int64_t braw = sb;
+      if (bd > 8) {
+        sa = rsr64(sa, 2 * (bd - 8));
+        sb = rsr64(sb, bd - 8);
+      }
+      int64_t pp = sa * n - sb * sb;
+      if (pp < 0) pp = 0;
+      const int z = rsr64(pp * mt, 20);
+      const int a2 = z >= 255 ? 256 : (z == 0 ? 1 : ((z << 8) + z / 2) / (z + 1));
+      A[r * kLrAbW + q] = a2;
+      B[r * kLrAbW + q] = rsr64((int64_t)(256 - a2) * braw * inv, 12);
+    }
+  }
+  __syncthreads();
+  const int w0 = j.xqd[0], w1 = j.xqd[1], w2 = (1 << 7) - w0 - w1;
+  for (int i = threadIdx.x; i < th * tw; i += 256) {
+    const int r = i / tw, q = i - r * tw;
+    const int y = y0 + r;
+    const int u = s_src[(r + 3) * kLrSrcW + q + 3];
+    int flt[2] = {0, 0};
+#pragma unroll
+    for (int pass = 0; pass < 2; ++pass) {
+      if (j.r[pass] == 0) continue;
+      const int* A = s_mid + (2 * pass) * kLrAbH * kLrAbW;
+      const int* B = A + kLrAbH * kLrAbW;
+      int sa = 0, sb = 0;
+#pragma unroll
+      for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int wgt =
+              pass == 0 ? (((y + dy) & 1) ? (dx == 0 ? 6 : 5) : 0) : ((dx == 0 || dy == 0) ? 4 : 3);
+          if (!wgt) continue;
+          const int o = (r + 1 + dy) * kLrAbW + q + 1 + dx;
+          sa += wgt * A[o];
+          sb += wgt * B[o];
+        }
+      const int sh = (pass == 0 && (y & 1)) ? 4 : 5;
+      flt[pass] = rsr64((int64_t)sa * u + sb, 8 + sh - 4);
+    }
+    const int us = u << 4;
+    int64_t v = (int64_t)w1 * us;
+    v += (int64_t)w0 * (j.r[0] ? flt[0] : us);
+    v += (int64_t)w2 * (j.r[1] ? flt[1] : us);
+    dst[(size_t)r * c.stride + q] = (uint16_t)clampi(rsr64(v, 4 + 7), 0, a.maxv);
+  }
 }
 
 /* ---- NV12 and RGB packs (av1gpu_kernels.h:3379-3472) -------------------- */
@@ -440,12 +521,13 @@ __global__ void k_rgb(Layout L, const uint16_t* src, uint8_t* out, ColorM cm) {
 hipError_t launch_deblock(const FilterArgs& a, hipStream_t s, int* launches) {
   if (a.plan.n_deblock_v > 0 && a.plan.lines_v > 0) {
     hipLaunchKernelGGL(k_deblock, dim3((a.plan.lines_v + 255) / 256), dim3(256), 0, s, a, 0,
-                       a.plan.n_deblock_v, a.plan.lines_v);
+                       a.plan.n_deblock_v, a.plan.lines_v, a.plan.db_chunk);
     ++*launches;
   }
   if (a.plan.n_deblock_h > 0 && a.plan.lines_h > 0) {
     hipLaunchKernelGGL(k_deblock, dim3((a.plan.lines_h + 255) / 256), dim3(256), 0, s, a,
-                       a.plan.n_deblock_v, a.plan.n_deblock_h, a.plan.lines_h);
+                       a.plan.n_deblock_v, a.plan.n_deblock_h, a.plan.lines_h,
+                       a.plan.db_chunk + (a.plan.lines_v + 255) / 256);
     ++*launches;
   }
   return hipGetLastError();
@@ -453,8 +535,24 @@ hipError_t launch_deblock(const FilterArgs& a, hipStream_t s, int* launches) {
 
 hipError_t launch_cdef(const FilterArgs& a, hipStream_t s, int* launches) {
   if (a.plan.n_cdef <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_cdef, dim3((a.plan.n_cdef + 127) / 128), dim3(128), 0, s, a);
-  ++*launches;
+  if (a.plan.n_cdef_dir > 0) {
+    hipLaunchKernelGGL(k_cdef_dir, dim3((a.plan.n_cdef_dir + 3) / 4), dim3(256), 0, s, a);
+    ++*launches;
+  }
+  const int nl = a.plan.n_cdef_luma, nc = a.plan.n_cdef - nl;
+  if (nl > 0) {
+    hipLaunchKernelGGL(k_cdef<64>, dim3((nl + 3) / 4), dim3(256), 0, s, a, 0, nl);
+    ++*launches;
+  }
+  if (nc > 0) {
+    const int cs = (8 >> a.lay.ss_x) * (8 >> a.lay.ss_y);
+    switch (cs) {
+      case 16: hipLaunchKernelGGL(k_cdef<16>, dim3((nc + 15) / 16), dim3(256), 0, s, a, nl, nc); break;
+      case 32: hipLaunchKernelGGL(k_cdef<32>, dim3((nc + 7) / 8), dim3(256), 0, s, a, nl, nc); break;
+      default: hipLaunchKernelGGL(k_cdef<64>, dim3((nc + 3) / 4), dim3(256), 0, s, a, nl, nc); break;
+    }
+    ++*launches;
+  }
   return hipGetLastError();
 }
 
@@ -479,8 +577,8 @@ hipError_t launch_superres(const FilterArgs& a, const uint16_t* src, uint16_t* d
 
 hipError_t launch_lr(const FilterArgs& a, const uint16_t* cdef, const uint16_t* curr, hipStream_t s,
                      int* launches) {
-  if (a.plan.lr_px <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_lr, dim3((a.plan.lr_px + 255) / 256), dim3(256), 0, s, a, cdef, curr);
+  if (a.plan.n_lr_tiles <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_lr, dim3(a.plan.n_lr_tiles), dim3(256), 0, s, a, cdef, curr);
   ++*launches;
   return hipGetLastError();
 }

@@ -88,6 +88,7 @@ struct DevTx {
 /* transform classes: the larger side, 4 8 16 32 64 */
 constexpr int kTxClasses = 5;
 constexpr int kMcChunkSamples = 256; /* motion compensation samples per workgroup */
+constexpr int kLrTileCols = 64, kLrTileRows = 16; /* loop restoration tile per workgroup */
 
 struct DevObmc {
   int32_t pass, rx, ry, w, h, mvx, mvy, filt, filt_v, slot;
@@ -147,6 +148,16 @@ struct FramePlanDev {
   int32_t itx_n[kTxClasses];
   int32_t n_inter, n_queue, n_deblock_v, n_deblock_h, lines_v, lines_h, n_cdef, n_lr, lr_px;
   int32_t mc_total, mc_chunks;
+  /* deblocking: the first edge of every 256 line chunk, vertical pass chunks
+   * then horizontal */
+  const int32_t* db_chunk;
+  /* CDEF: luma blocks first; the luma 8x8 blocks (x8 | y8 << 16) whose
+   * direction the listed blocks need */
+  const int32_t* cdef_dir_list;
+  int32_t n_cdef_luma, n_cdef_dir;
+  /* loop restoration tiles: (job, x offset | y offset << 16) */
+  const int2* lr_tiles;
+  int32_t n_lr_tiles;
 };
 
 /* launchers (one per translation unit, all on the context's stream) */
@@ -178,6 +189,8 @@ struct FilterArgs {
   uint16_t* lr_curr;
   uint16_t* lr_cdef;
   uint16_t* lr_prev_up;
+  int32_t* cdef_dir;    /* (direction, variance) per luma 8x8 block */
+  int cdef_dir_w;       /* luma 8x8 blocks per row                  */
   int sharpness;
   int maxv;
 };
