@@ -223,6 +223,9 @@ def hip_lib():
         lib.av1h_set_stage_timing.argtypes = [P, ctypes.c_int]
         lib.av1h_get_stage_times.argtypes = [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64),
                                              ctypes.POINTER(ctypes.c_int64)]
+        lib.av1h_set_queue_trace.argtypes = [P, ctypes.c_int]
+        lib.av1h_get_queue_trace.restype = ctypes.c_int64
+        lib.av1h_get_queue_trace.argtypes = [P, ctypes.c_void_p, ctypes.c_int64]
         lib.av1h_nv12_device_buffer.restype = ctypes.c_void_p
         lib.av1h_nv12_device_buffer.argtypes = [P]
         _hip = lib
@@ -303,6 +306,17 @@ class HipDecoder:
         b = (ctypes.c_int64 * len(STAGES))()
         self._lib.av1h_get_stage_times(self._h, ms, n, b)
         return {name: (ms[i], n[i], b[i]) for i, name in enumerate(STAGES)}
+
+    def set_queue_trace(self, enable=True):
+        self._lib.av1h_set_queue_trace(self._h, 1 if enable else 0)
+
+    def queue_trace(self):
+        """(n, 4) uint64: taken, ready, published clock, workgroup per intra block."""
+        n = self._lib.av1h_get_queue_trace(self._h, None, 0)
+        out = np.zeros((max(n, 0), 4), dtype=np.uint64)
+        if n > 0:
+            self._lib.av1h_get_queue_trace(self._h, out.ctypes.data, out.size)
+        return out
 
     def pack_nv12_device(self, slot, pitch):
         """NV12 pack left on the device (no host copy)."""

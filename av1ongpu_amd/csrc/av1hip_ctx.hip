@@ -108,6 +108,10 @@ struct Ctx {
   uint32_t epoch = 0;
   uint32_t* counters = nullptr; /* [0] queue counter, [1] error word */
   int32_t* cdef_dir = nullptr;   /* CDEF direction map                */
+  bool trace_on = false;         /* queue timestamps (diagnostics)    */
+  uint64_t* trace = nullptr;
+  size_t trace_cap = 0;
+  int trace_n = 0;
   size_t cdef_dir_cap = 0;
   int16_t* grain_tmpl = nullptr;
   uint8_t* grain_offs = nullptr;
@@ -126,7 +130,7 @@ struct Ctx {
   /* grain templates depend only on the grain parameters: they are built on a
    * second stream while the frame reconstructs */
   hipStream_t side = nullptr;
-  hipEvent_t ev_side_go = nullptr, ev_side_done = nullptr;
+  hipEvent_t ev_side_go = nullptr, ev_side_done = nullptr, ev_itx_done = nullptr;
   bool timing = false;
   hipEvent_t sev[2 * AV1H_STAGES] = {};
   bool stage_hit[AV1H_STAGES] = {};
@@ -201,13 +205,86 @@ void scaling_lut(const av1h_grain& p, int plane, int* lut) {
   for (int x = val[n - 1]; x < 256; ++x) lut[x] = scl[n - 1];
 }
 
+/* wedge master masks and the codebook entry of a block (tables.cpp:616-635;
+ * the masks of av1gpu_kernels.h:1968-2014) */
+int wedge_1d(int which, int j) {
+  static const int even[8] = {1, 4, 11, 27, 46, 58, 62, 63};
+  static const int odd[8] = {1, 2, 6, 18, 37, 53, 60, 63};
+  static const int vert[6] = {2, 7, 21, 43, 57, 62};
+  if (which == 2) return j < 29 ? 0 : (j >= 35 ? 64 : vert[j - 29]);
+  if (j < 28) return 0;
+  if (j >= 36) return 64;
+  return which == 0 ? even[j - 28] : odd[j - 28];
+}
+int wedge_ob63(int i, int j) {
+  const int shift = (i & 1) ? 15 - (i >> 1) : 16 - (i >> 1);
+  return wedge_1d(i & 1, std::min(std::max(j - shift, 0), 63));
+}
+int wedge_master(int dir, int i, int j) {
+  switch (dir) {
+    case 0: return wedge_1d(2, i);
+    case 1: return wedge_1d(2, j);
+    case 2: return wedge_ob63(j, i);
+    case 4: return 64 - wedge_ob63(i, 63 - j);
+    case 5: return 64 - wedge_ob63(j, 63 - i);
+    default: return wedge_ob63(i, j);
+  }
+}
+bool wedge_entry(int bw, int bh, int index, int* dir, int* xoff, int* yoff, int* flip) {
+  if (index < 0 || index >= 16 || bw < 8 || bh < 8 || bw > 32 || bh > 32) return false;
+  const int shape = bh > bw ? 0 : (bh < bw ? 1 : 2);
+  const int* e = av1t_spec_table + AV1T_OFF_WEDGE_CODEBOOK + (shape * 16 + index) * 3;
+  *dir = e[0];
+  *xoff = 32 - ((e[1] * bw) >> 3);
+  *yoff = 32 - ((e[2] * bh) >> 3);
+  int sum = 0;
+  for (int i = 0; i < bw; ++i) sum += wedge_master(e[0], *yoff, *xoff + i);
+  for (int i = 1; i < bh; ++i) sum += wedge_master(e[0], *yoff + i, *xoff);
+  const int n = bw + bh - 1;
+  *flip = ((sum + n / 2) / n) < 32 ? 1 : 0;
+  return true;
+}
+
+/* warp shear parameters and validity (tables.cpp:444-475) */
+int64_t round2s64(int64_t v, int n) {
+  if (n <= 0) return v;
+  const int64_t h = (int64_t)1 << (n - 1);
+  return v >= 0 ? (v + h) >> n : -((-v + h) >> n);
+}
+int round2s(int v, int n) { return v >= 0 ? (v + (1 << (n - 1))) >> n : -((-v + (1 << (n - 1))) >> n); }
+bool setup_shear(const int32_t* p, int* alpha, int* beta, int* gamma, int* delta) {
+  const int* lut = av1t_spec_table + AV1T_OFF_DIV_LUT;
+  const int a0 = std::min(32767, std::max(-32768, p[2] - (1 << 16)));
+  const int b0 = std::min(32767, std::max(-32768, p[3]));
+  const int ad = p[2] < 0 ? -p[2] : p[2];
+  int n = 0;
+  for (int v = ad; v > 1; v >>= 1) ++n;
+  const int e = ad - (1 << n);
+  const int f = n > 8 ? (e + (1 << (n - 8 - 1))) >> (n - 8) : e << (8 - n);
+  const int dshift = n + 14;
+  const int dfac = p[2] < 0 ? -lut[f] : lut[f];
+  const int64_t v = (int64_t)p[4] * ((int64_t)1 << 16);
+  const int g0 = (int)std::min<int64_t>(32767, std::max<int64_t>(-32768, round2s64(v * dfac, dshift)));
+  const int64_t w = (int64_t)p[3] * p[4];
+  const int d0 = (int)std::min<int64_t>(32767, std::max<int64_t>(-32768, p[5] - round2s64(w * dfac, dshift) - (1 << 16)));
+  *alpha = round2s(a0, 6) * 64;
+  *beta = round2s(b0, 6) * 64;
+  *gamma = round2s(g0, 6) * 64;
+  *delta = round2s(d0, 6) * 64;
+  if (4 * std::abs(*alpha) + 7 * std::abs(*beta) >= (1 << 16)) return false;
+  if (4 * std::abs(*gamma) + 4 * std::abs(*delta) >= (1 << 16)) return false;
+  return true;
+}
+
 /* ---- flattening ----------------------------------------------------------- */
 
 struct Host {
   std::vector<DevBlock> blocks;
   std::vector<DevTx> tx;
-  std::vector<int32_t> inter_list, queue_list;
-  std::vector<int32_t> mc_px0, mc_chunk;
+  std::vector<int32_t> inter_list, queue_list, special_list;
+  std::vector<DevObmc> obmc;
+  std::vector<int32_t> itx_all;
+  std::vector<int32_t> mc_items, mc_task;
   std::vector<int32_t> db_chunk, cdef_dir_list;
   std::vector<int32_t> lr_tiles; /* pairs */
   std::vector<int32_t> itx[kTxClasses];
@@ -298,6 +375,8 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
   H->tx.clear();
   H->inter_list.clear();
   H->queue_list.clear();
+  H->special_list.clear();
+  H->obmc.clear();
   H->blocks.reserve(nb);
   plan->slots.clear();
   bool derive = false;
@@ -381,24 +460,21 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
       d.dep_x1 = ax1;
       d.dep_y1 = ly1;
       d.dep_corner = (ax1 >= b.x || ly1 >= b.y) ? 1 : 0;
+      d.dep_rx1 = d.dep_rx0 - 1;
       H->queue_list.push_back(i);
     } else {
       d.kind = kInter;
-      if (b.intrabc) return set_err(c, AV1H_ERR_UNSUPPORTED, "intra block copy is not supported on the GPU yet");
-      if (b.interintra) return set_err(c, AV1H_ERR_UNSUPPORTED, "inter/intra compound is not supported on the GPU yet");
-      if (b.obmc_count > 0) return set_err(c, AV1H_ERR_UNSUPPORTED, "OBMC is not supported on the GPU yet");
-      if (b.warp[0] || b.warp[1] || b.mask_warp[0] || b.mask_warp[1])
-        return set_err(c, AV1H_ERR_UNSUPPORTED, "warped motion is not supported on the GPU yet");
-      if (b.compound == AV1H_COMPOUND_WEDGE || b.compound == AV1H_COMPOUND_DIFFWTD)
-        return set_err(c, AV1H_ERR_UNSUPPORTED, "masked compound is not supported on the GPU yet");
-      if (b.compound < 0 || b.compound > 2) return set_err(c, AV1H_ERR_INVALID, "compound %d", b.compound);
+      if (b.compound < 0 || b.compound > 4) return set_err(c, AV1H_ERR_INVALID, "compound %d", b.compound);
       const int nref = b.compound ? 2 : 1;
+      const int sx = p > 0 ? hdr.ss_x : 0, sy = p > 0 ? hdr.ss_y : 0;
       for (int r = 0; r < nref; ++r) {
-        const int s = resolve_slot(hdr, b.ref[r]);
-        if (s < 0) return set_err(c, AV1H_ERR_UNSUPPORTED, "block %d: inter prediction without a reference", i);
+        const int s = b.intrabc ? -1 : resolve_slot(hdr, b.ref[r]);
+        if (s < 0 && !b.intrabc)
+          return set_err(c, AV1H_ERR_UNSUPPORTED, "block %d: inter prediction without a reference", i);
         if (s >= kSlots) return set_err(c, AV1H_ERR_INVALID, "block %d: reference slot %d", i, s);
         d.slot[r] = s;
-        if (std::find(plan->slots.begin(), plan->slots.end(), s) == plan->slots.end()) plan->slots.push_back(s);
+        if (s >= 0 && std::find(plan->slots.begin(), plan->slots.end(), s) == plan->slots.end())
+          plan->slots.push_back(s);
       }
       for (int r = 0; r < 2; ++r) {
         d.mv[r][0] = b.mv[r][0];
@@ -415,7 +491,101 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
       d.ref_h = L.out_h[p];
       d.scale_x = scale_x;
       d.scale_y = scale_y;
-      H->inter_list.push_back(i);
+      bool special = false;
+      if (b.intrabc) {
+        /* the frame being decoded, 4-rounded coded size, unscaled
+         * (engine.cpp job fields; av1gpu_kernels.h intrabc) */
+        if (b.compound) return set_err(c, AV1H_ERR_INVALID, "block %d: compound intra block copy", i);
+        d.flags |= kIntraBC;
+        d.ref_w = ((fw + 3) & ~3) >> sx;
+        d.ref_h = ((fh + 3) & ~3) >> sy;
+        d.scale_x = d.scale_y = 1 << 14;
+        const int msx = 3 + sx, msy = 3 + sy;
+        const int x0 = b.x + (b.mv[0][0] >> msx), y0 = b.y + (b.mv[0][1] >> msy);
+        const int x1 = std::min(x0 + b.w, L.grid_w[p]) - 1, y1 = std::min(y0 + b.h, L.grid_h[p]) - 1;
+        const int sh = L.cell_shift[p];
+        d.dep_rx0 = std::max(x0, 0) >> sh;
+        d.dep_ry0 = std::max(y0, 0) >> sh;
+        d.dep_rx1 = x1 < 0 ? -1 : x1 >> sh;
+        d.dep_ry1 = y1 < 0 ? -1 : y1 >> sh;
+        if (x1 < std::max(x0, 0) || y1 < std::max(y0, 0)) d.dep_rx1 = d.dep_rx0 - 1; /* nothing to wait for */
+      }
+      if (b.compound == AV1H_COMPOUND_WEDGE || b.compound == AV1H_COMPOUND_DIFFWTD) {
+        d.mask_kind = b.compound == AV1H_COMPOUND_WEDGE ? 1 : 2;
+        d.mask_sign = b.mask_sign;
+        if (d.mask_kind == 1 &&
+            !wedge_entry(b.w << sx, b.h << sy, b.wedge_index, &d.wedge_dir, &d.wedge_xoff, &d.wedge_yoff,
+                         &d.wedge_flip))
+          return set_err(c, AV1H_ERR_INVALID, "a wedge on a block size that has none");
+        special = true;
+      }
+      if (b.interintra) {
+        d.flags |= kInterIntra;
+        d.ii_mode = b.interintra_mode;
+        if (b.interintra_mode < 0 || b.interintra_mode > 3)
+          return set_err(c, AV1H_ERR_INVALID, "block %d: inter/intra mode", i);
+        if (b.interintra_wedge) {
+          if (!wedge_entry(b.w << sx, b.h << sy, b.wedge_index, &d.wedge_dir, &d.wedge_xoff, &d.wedge_yoff,
+                           &d.wedge_flip))
+            return set_err(c, AV1H_ERR_INVALID, "an inter/intra wedge on a block size that has none");
+          d.flags |= kIIWedge;
+        }
+      }
+      if (b.obmc_count > 0) {
+        if (b.obmc_first < 0 || b.obmc_first + (int64_t)b.obmc_count > f.num_obmc)
+          return set_err(c, AV1H_ERR_INVALID, "block %d: OBMC range", i);
+        d.obmc_first = (int)H->obmc.size();
+        for (int k = 0; k < b.obmc_count && k < 8; ++k) {
+          const av1h_obmc& o = f.obmc[b.obmc_first + k];
+          const int s = resolve_slot(hdr, o.ref);
+          if (s < 0 || s >= kSlots) return set_err(c, AV1H_ERR_UNSUPPORTED, "block %d: OBMC without a reference", i);
+          if (o.w != 2 && o.w != 4 && o.w != 8 && o.w != 16 && o.w != 32 && o.pass != 0)
+            return set_err(c, AV1H_ERR_INVALID, "block %d: OBMC width %d", i, o.w);
+          if (o.h != 2 && o.h != 4 && o.h != 8 && o.h != 16 && o.h != 32 && o.pass == 0)
+            return set_err(c, AV1H_ERR_INVALID, "block %d: OBMC height %d", i, o.h);
+          if (o.filter < 0 || o.filter > 3 || o.filter_v < 0 || o.filter_v > 3)
+            return set_err(c, AV1H_ERR_INVALID, "block %d: OBMC filter", i);
+          if (std::find(plan->slots.begin(), plan->slots.end(), s) == plan->slots.end()) plan->slots.push_back(s);
+          H->obmc.push_back(DevObmc{o.pass, o.x, o.y, o.w, o.h, o.mv_x, o.mv_y, o.filter, o.filter_v, s});
+        }
+        d.obmc_count = (int)H->obmc.size() - d.obmc_first;
+        special = true;
+      }
+      for (int r = 0; r < 2; ++r) {
+        if (!b.warp[r] && !b.mask_warp[r]) continue;
+        if (b.warp[r]) d.flags |= r ? kWarp1 : kWarp0;
+        if (b.mask_warp[r]) d.flags |= r ? kMaskWarp1 : kMaskWarp0;
+        special = true;
+        int al, be, ga, de;
+        /* parameters kept only when the shear is valid (engine.cpp job fields) */
+        if (!setup_shear(b.warp_params[r], &al, &be, &ga, &de)) continue;
+        for (int k = 0; k < 6; ++k) d.warp_params[r][k] = b.warp_params[r][k];
+        d.warp_abcd[r][0] = al;
+        d.warp_abcd[r][1] = be;
+        d.warp_abcd[r][2] = ga;
+        d.warp_abcd[r][3] = de;
+      }
+      if (b.intrabc || b.interintra) {
+        /* reads the frame being decoded: the dependency queue. Inter/intra
+         * waits for its left column, above row and corner (engine.cpp:
+         * 811-875); intra block copy for its source rectangle */
+        if (b.interintra) {
+          d.flags |= (b.have_left && b.x > 0 ? kHaveLeft : 0) | (b.have_above && b.y > 0 ? kHaveAbove : 0);
+          d.dep_x1 = b.x + b.w - 1;
+          d.dep_y1 = b.y + b.h - 1;
+          d.dep_corner = 1;
+        } else {
+          d.dep_x1 = b.x - 1;
+          d.dep_y1 = b.y - 1;
+          d.dep_corner = 0;
+        }
+        if (!b.intrabc) d.dep_rx1 = d.dep_rx0 - 1;
+        H->queue_list.push_back(i);
+      } else if (special) {
+        H->special_list.push_back(i);
+      } else {
+        H->inter_list.push_back(i);
+      }
     }
     d.tx_first = (int)H->tx.size();
     for (int t = 0; t < b.tx_count; ++t) {
@@ -439,7 +609,8 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
       dt.row_shift = rs;
       dt.eob = tx.eob;
       dt.plane = p;
-      dt.flags = (b.lossless ? kTxLossless : 0) | (b.residual_provided ? kTxRaw : 0) | (b.is_intra ? kTxToResidual : 0);
+      dt.flags = (b.lossless ? kTxLossless : 0) | (b.residual_provided ? kTxRaw : 0) |
+                 ((b.is_intra || b.intrabc || b.interintra) ? kTxToResidual : 0);
       const size_t end = (size_t)tx.coef_offset + (size_t)tx.w * tx.h;
       if (b.residual_provided)
         rend = std::max(rend, end);
@@ -458,33 +629,85 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
   H->palette.assign(f.palette_map, f.palette_map + f.num_palette);
   H->palette.resize(H->palette.size() + 8, 0);
 
-  /* motion compensation numbering: the samples of the inter blocks one block
-   * after another, and the block holding the first sample of every chunk */
-  H->mc_px0.assign(1, 0);
-  for (int32_t bi : H->inter_list) {
-    const int64_t next = (int64_t)H->mc_px0.back() + (int64_t)H->blocks[bi].w * H->blocks[bi].h;
-    if (next > INT32_MAX) return set_err(c, AV1H_ERR_UNSUPPORTED, "frame too large");
-    H->mc_px0.push_back((int32_t)next);
+  /* The order the queue hands blocks out: superblock anti-diagonals (a 64x64
+   * luma superblock at (sx, sy) has key sx + 2 sy, the wavefront of the
+   * left, above and above-right dependencies), coding order inside a key.
+   * Every block a block waits for has a key no larger and comes earlier in
+   * coding order, so the order is a topological one and the queue always
+   * progresses; an intra block copy takes the largest key of its source. */
+  {
+    std::vector<int64_t> key(H->queue_list.size());
+    for (size_t k = 0; k < H->queue_list.size(); ++k) {
+      const DevBlock& d = H->blocks[H->queue_list[k]];
+      const int sx = d.plane ? hdr.ss_x : 0, sy = d.plane ? hdr.ss_y : 0;
+      int64_t kk = ((d.x << sx) >> 6) + 2 * ((d.y << sy) >> 6);
+      if (d.flags & kIntraBC) {
+        const int sh = L.cell_shift[d.plane];
+        if (d.dep_rx1 >= d.dep_rx0 && d.dep_ry1 >= d.dep_ry0) {
+          const int64_t rk = ((((d.dep_rx1 << sh) << sx)) >> 6) + 2 * (((d.dep_ry1 << sh) << sy) >> 6);
+          kk = std::max(kk, rk);
+        }
+      }
+      key[k] = (kk << 32) | (int64_t)k;
+    }
+    std::vector<size_t> ord(H->queue_list.size());
+    for (size_t k = 0; k < ord.size(); ++k) ord[k] = k;
+    std::sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return key[a] < key[b]; });
+    std::vector<int32_t> q(ord.size());
+    for (size_t k = 0; k < ord.size(); ++k) q[k] = H->queue_list[ord[k]];
+    H->queue_list.swap(q);
   }
-  const int32_t mc_total = H->mc_px0.back();
-  const int32_t mc_chunks = (mc_total + kMcChunkSamples - 1) / kMcChunkSamples;
-  H->mc_chunk.resize(mc_chunks);
-  for (int32_t k = 0, bi = 0; k < mc_chunks; ++k) {
-    while (H->mc_px0[bi + 1] <= k * kMcChunkSamples) ++bi;
-    H->mc_chunk[k] = bi;
+
+  /* motion compensation tasks: items (whole blocks, or row strips of blocks
+   * over kMcSamples samples) in coding order, packed into tasks within the
+   * per task budgets of the kernel */
+  H->mc_items.clear();
+  H->mc_task.clear();
+  {
+    int tn = 0, ts = 0, tw = 0, tm = 0;
+    for (int32_t bi : H->inter_list) {
+      const DevBlock& d = H->blocks[bi];
+      const int per = d.w * d.h <= kMcSamples ? d.h : kMcSamples / d.w;
+      for (int y0 = 0; y0 < d.h; y0 += per) {
+        const int rows = std::min(per, d.h - y0);
+        const int sb = rows * d.w, wb = (2 * d.w + 9) * (2 * rows + 9), mb = (2 * rows + 9) * d.w;
+        if (H->mc_task.empty() || tn + 1 > kMcItems || ts + sb > kMcSamples || tw + wb > kMcWinPerRef ||
+            tm + mb > kMcMidPerRef) {
+          H->mc_task.push_back((int32_t)(H->mc_items.size() / 2));
+          tn = ts = tw = tm = 0;
+        }
+        H->mc_items.push_back(bi);
+        H->mc_items.push_back(y0 | (rows << 16));
+        ++tn;
+        ts += sb;
+        tw += wb;
+        tm += mb;
+      }
+    }
+    H->mc_task.push_back((int32_t)(H->mc_items.size() / 2));
   }
-  /* transform classes by the larger side, each sorted by size so a wave's
-   * transforms mostly share one */
-  for (auto& v : H->itx) v.clear();
+  const int32_t mc_tasks = (int32_t)H->mc_task.size() - 1;
+  /* transforms: inter set then intra set, each by class (larger side), each
+   * class sorted by size so a wave's transforms mostly share one */
+  std::vector<int32_t> itx_sets[2][kTxClasses];
   for (int t = 0; t < (int)H->tx.size(); ++t) {
     const int m = std::max(H->tx[t].w, H->tx[t].h);
-    H->itx[m <= 4 ? 0 : m <= 8 ? 1 : m <= 16 ? 2 : m <= 32 ? 3 : 4].push_back(t);
+    const int set = (H->tx[t].flags & kTxToResidual) ? 1 : 0;
+    itx_sets[set][m <= 4 ? 0 : m <= 8 ? 1 : m <= 16 ? 2 : m <= 32 ? 3 : 4].push_back(t);
   }
-  for (auto& v : H->itx)
-    std::stable_sort(v.begin(), v.end(), [&](int32_t a, int32_t b) {
-      const DevTx &ta = H->tx[a], &tb = H->tx[b];
-      return ta.w != tb.w ? ta.w < tb.w : ta.h < tb.h;
-    });
+  H->itx_all.clear();
+  int32_t itx_off[2][kTxClasses + 1];
+  for (int set = 0; set < 2; ++set)
+    for (int k = 0; k < kTxClasses; ++k) {
+      auto& v = itx_sets[set][k];
+      std::stable_sort(v.begin(), v.end(), [&](int32_t a, int32_t b) {
+        const DevTx &ta = H->tx[a], &tb = H->tx[b];
+        return ta.w != tb.w ? ta.w < tb.w : ta.h < tb.h;
+      });
+      itx_off[set][k] = (int32_t)H->itx_all.size();
+      H->itx_all.insert(H->itx_all.end(), v.begin(), v.end());
+      itx_off[set][k + 1] = (int32_t)H->itx_all.size();
+    }
 
   /* deblocking: vertical edges then horizontal, with a line prefix each */
   H->deblock.clear();
@@ -693,7 +916,8 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
       const DevBlock& d = H->blocks[i];
       plan->alg_bytes[AV1H_STAGE_INTER] += (int64_t)d.w * d.h * (2 + 2 * (d.compound ? 2 : 1));
     }
-    for (const DevTx& t : H->tx) plan->alg_bytes[AV1H_STAGE_INTER] += 2 * (int64_t)t.w * t.h;
+    for (const DevTx& t : H->tx)
+      if (!(t.flags & kTxToResidual)) plan->alg_bytes[AV1H_STAGE_INTER] += 2 * (int64_t)t.w * t.h;
     for (int32_t i : H->queue_list)
       plan->alg_bytes[AV1H_STAGE_QUEUE] += 2 * (int64_t)H->blocks[i].w * H->blocks[i].h;
     bool has_edges[3] = {false, false, false};
@@ -721,42 +945,42 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
     }
   }
 
-  /* one device buffer: offsets, then the arrays */
+  /* one device buffer: the arrays back to back, 256 byte aligned */
   struct Part {
     const void* src;
     size_t bytes;
     size_t off;
   };
-  Part parts[11 + 2 + kTxClasses + 3] = {
-      {H->blocks.data(), H->blocks.size() * sizeof(DevBlock), 0},
-      {H->tx.data(), H->tx.size() * sizeof(DevTx), 0},
-      {nullptr, 0, 0},
-      {H->inter_list.data(), H->inter_list.size() * 4, 0},
-      {H->queue_list.data(), H->queue_list.size() * 4, 0},
-      {H->coef.data(), H->coef.size() * 4, 0},
-      {H->residual.data(), H->residual.size() * 4, 0},
-      {H->palette.data(), H->palette.size(), 0},
-      {H->deblock.data(), H->deblock.size() * sizeof(DevDeblock), 0},
-      {H->cdef.data(), H->cdef.size() * sizeof(DevCdef), 0},
-      {H->lr.data(), H->lr.size() * sizeof(DevLr), 0},
-      {H->mc_px0.data(), H->mc_px0.size() * 4, 0},
-      {H->mc_chunk.data(), H->mc_chunk.size() * 4, 0},
-      {H->itx[0].data(), H->itx[0].size() * 4, 0},
-      {H->itx[1].data(), H->itx[1].size() * 4, 0},
-      {H->itx[2].data(), H->itx[2].size() * 4, 0},
-      {H->itx[3].data(), H->itx[3].size() * 4, 0},
-      {H->itx[4].data(), H->itx[4].size() * 4, 0},
-      {H->db_chunk.data(), H->db_chunk.size() * 4, 0},
-      {H->cdef_dir_list.data(), H->cdef_dir_list.size() * 4, 0},
-      {H->lr_tiles.data(), H->lr_tiles.size() * 4, 0},
+  std::vector<Part> parts;
+  auto add = [&](const void* src, size_t bytes) {
+    parts.push_back(Part{src, bytes, 0});
+    return parts.size() - 1;
   };
+  const size_t p_blocks = add(H->blocks.data(), H->blocks.size() * sizeof(DevBlock));
+  const size_t p_tx = add(H->tx.data(), H->tx.size() * sizeof(DevTx));
+  const size_t p_inter = add(H->inter_list.data(), H->inter_list.size() * 4);
+  const size_t p_queue = add(H->queue_list.data(), H->queue_list.size() * 4);
+  const size_t p_spec = add(H->special_list.data(), H->special_list.size() * 4);
+  const size_t p_obmc = add(H->obmc.data(), H->obmc.size() * sizeof(DevObmc));
+  const size_t p_coef = add(H->coef.data(), H->coef.size() * 4);
+  const size_t p_res = add(H->residual.data(), H->residual.size() * 4);
+  const size_t p_pal = add(H->palette.data(), H->palette.size());
+  const size_t p_db = add(H->deblock.data(), H->deblock.size() * sizeof(DevDeblock));
+  const size_t p_cdef = add(H->cdef.data(), H->cdef.size() * sizeof(DevCdef));
+  const size_t p_lr = add(H->lr.data(), H->lr.size() * sizeof(DevLr));
+  const size_t p_mcit = add(H->mc_items.data(), H->mc_items.size() * 4);
+  const size_t p_mcta = add(H->mc_task.data(), H->mc_task.size() * 4);
+  const size_t p_itx = add(H->itx_all.data(), H->itx_all.size() * 4);
+  const size_t p_itxo = add(itx_off, sizeof(itx_off));
+  const size_t p_dbc = add(H->db_chunk.data(), H->db_chunk.size() * 4);
+  const size_t p_cdir = add(H->cdef_dir_list.data(), H->cdef_dir_list.size() * 4);
+  const size_t p_lrt = add(H->lr_tiles.data(), H->lr_tiles.size() * 4);
+  const size_t p_grain = add(&H->grain, sizeof(DevGrain));
   size_t total = 0;
   for (Part& pt : parts) {
     pt.off = total;
     total += align256(pt.bytes);
   }
-  const size_t grain_off = total;
-  total += align256(sizeof(DevGrain));
   if (plan->dbytes < total) {
     if (plan->dbuf) (void)hipFree(plan->dbuf);
     plan->dbuf = nullptr;
@@ -767,34 +991,34 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
   plan->hbuf.resize(total);
   for (Part& pt : parts)
     if (pt.bytes) std::memcpy(plan->hbuf.data() + pt.off, pt.src, pt.bytes);
-  std::memcpy(plan->hbuf.data() + grain_off, &H->grain, sizeof(DevGrain));
   uint8_t* db = static_cast<uint8_t*>(plan->dbuf);
+  auto at = [&](size_t k) { return db + parts[k].off; };
   FramePlanDev& P = plan->dev;
-  P.blocks = reinterpret_cast<const DevBlock*>(db + parts[0].off);
-  P.tx = reinterpret_cast<const DevTx*>(db + parts[1].off);
-  P.obmc = nullptr;
-  P.inter_list = reinterpret_cast<const int32_t*>(db + parts[3].off);
-  P.queue_list = reinterpret_cast<const int32_t*>(db + parts[4].off);
-  P.coef = reinterpret_cast<const int32_t*>(db + parts[5].off);
-  P.residual = reinterpret_cast<const int32_t*>(db + parts[6].off);
-  P.palette = db + parts[7].off;
-  P.deblock = reinterpret_cast<const DevDeblock*>(db + parts[8].off);
-  P.cdef = reinterpret_cast<const DevCdef*>(db + parts[9].off);
-  P.lr = reinterpret_cast<const DevLr*>(db + parts[10].off);
-  P.grain = reinterpret_cast<const DevGrain*>(db + grain_off);
-  P.mc_px0 = reinterpret_cast<const int32_t*>(db + parts[11].off);
-  P.mc_chunk = reinterpret_cast<const int32_t*>(db + parts[12].off);
-  for (int k = 0; k < kTxClasses; ++k) {
-    P.itx_list[k] = reinterpret_cast<const int32_t*>(db + parts[13 + k].off);
-    P.itx_n[k] = (int32_t)H->itx[k].size();
-  }
-  P.mc_total = mc_total;
-  P.mc_chunks = mc_chunks;
-  P.db_chunk = reinterpret_cast<const int32_t*>(db + parts[13 + kTxClasses].off);
-  P.cdef_dir_list = reinterpret_cast<const int32_t*>(db + parts[14 + kTxClasses].off);
+  P.blocks = reinterpret_cast<const DevBlock*>(at(p_blocks));
+  P.tx = reinterpret_cast<const DevTx*>(at(p_tx));
+  P.obmc = reinterpret_cast<const DevObmc*>(at(p_obmc));
+  P.special_list = reinterpret_cast<const int32_t*>(at(p_spec));
+  P.n_special = (int32_t)H->special_list.size();
+  P.inter_list = reinterpret_cast<const int32_t*>(at(p_inter));
+  P.queue_list = reinterpret_cast<const int32_t*>(at(p_queue));
+  P.coef = reinterpret_cast<const int32_t*>(at(p_coef));
+  P.residual = reinterpret_cast<const int32_t*>(at(p_res));
+  P.palette = at(p_pal);
+  P.deblock = reinterpret_cast<const DevDeblock*>(at(p_db));
+  P.cdef = reinterpret_cast<const DevCdef*>(at(p_cdef));
+  P.lr = reinterpret_cast<const DevLr*>(at(p_lr));
+  P.grain = reinterpret_cast<const DevGrain*>(at(p_grain));
+  P.itx_list = reinterpret_cast<const int32_t*>(at(p_itx));
+  std::memcpy(P.itx_off, itx_off, sizeof(itx_off));
+  P.itx_offs = reinterpret_cast<const int32_t*>(at(p_itxo));
+  P.mc_items = reinterpret_cast<const int2*>(at(p_mcit));
+  P.mc_task = reinterpret_cast<const int32_t*>(at(p_mcta));
+  P.mc_tasks = mc_tasks;
+  P.db_chunk = reinterpret_cast<const int32_t*>(at(p_dbc));
+  P.cdef_dir_list = reinterpret_cast<const int32_t*>(at(p_cdir));
   P.n_cdef_luma = n_cdef_luma;
   P.n_cdef_dir = (int32_t)H->cdef_dir_list.size();
-  P.lr_tiles = reinterpret_cast<const int2*>(db + parts[15 + kTxClasses].off);
+  P.lr_tiles = reinterpret_cast<const int2*>(at(p_lrt));
   P.n_lr_tiles = (int32_t)(H->lr_tiles.size() / 2);
   P.n_inter = (int)H->inter_list.size();
   P.n_queue = (int)H->queue_list.size();
@@ -904,23 +1128,6 @@ int run_plan(Ctx* c, Plan* plan, int keep, int reserved, const av1h_planes* out,
   int launches = 0;
   ++c->epoch;
   const size_t bytes = (size_t)L.size * 2;
-  if (plan->has_grain) {
-    const size_t need = (size_t)plan->grain.stripes * plan->grain.blocks + 64;
-    if (c->grain_offs_cap < need) {
-      HIP_OK(c, hipStreamSynchronize(s));
-      if (c->grain_offs) (void)hipFree(c->grain_offs);
-      c->grain_offs = nullptr;
-      HIP_OK(c, hipMalloc(&c->grain_offs, need));
-      c->grain_offs_cap = need;
-    }
-    /* after everything queued before this frame (the plan upload, the last
-     * frame's grain) */
-    HIP_OK(c, hipEventRecord(c->ev_side_go, s));
-    HIP_OK(c, hipStreamWaitEvent(c->side, c->ev_side_go, 0));
-    HIP_OK(c, launch_grain_template(L, plan->dev.grain, c->grain_tmpl, c->grain_offs, c->side, &launches));
-    HIP_OK(c, hipEventRecord(c->ev_side_done, c->side));
-  }
-  HIP_OK(c, hipEventRecord(c->ev[0], s));
   ReconArgs ra;
   ra.plan = plan->dev;
   ra.lay = L;
@@ -932,8 +1139,50 @@ int run_plan(Ctx* c, Plan* plan, int keep, int reserved, const av1h_planes* out,
   ra.queue_counter = c->counters;
   ra.error = c->counters + 1;
   ra.res = reinterpret_cast<int16_t*>(c->scratch); /* free until the CDEF snapshot */
+  ra.trace = nullptr;
+  if (c->trace_on && plan->dev.n_queue > 0) {
+    const size_t need = (size_t)plan->dev.n_queue * 4;
+    if (c->trace_cap < need) {
+      HIP_OK(c, hipStreamSynchronize(s));
+      if (c->trace) (void)hipFree(c->trace);
+      c->trace = nullptr;
+      c->trace_cap = 0;
+      HIP_OK(c, hipMalloc(&c->trace, need * 8));
+      c->trace_cap = need;
+    }
+    ra.trace = c->trace;
+  }
+  c->trace_n = c->trace_on ? plan->dev.n_queue : 0;
   ra.maxv = (1 << L.bit_depth) - 1;
   ra.edge_filter = plan->hdr.enable_intra_edge_filter;
+  if (plan->has_grain) {
+    const size_t need = (size_t)plan->grain.stripes * plan->grain.blocks + 64;
+    if (c->grain_offs_cap < need) {
+      HIP_OK(c, hipStreamSynchronize(s));
+      if (c->grain_offs) (void)hipFree(c->grain_offs);
+      c->grain_offs = nullptr;
+      HIP_OK(c, hipMalloc(&c->grain_offs, need));
+      c->grain_offs_cap = need;
+    }
+  }
+  /* The second stream takes what does not depend on the prediction: the
+   * transforms of intra blocks (the queue waits for them) and the grain
+   * templates (the grain blend waits for them). It starts after everything
+   * queued before this frame: the plan upload, the last frame's grain. */
+  const bool side_itx = itx_groups(plan->dev, 1) > 0;
+  if (side_itx || plan->has_grain) {
+    HIP_OK(c, hipEventRecord(c->ev_side_go, s));
+    HIP_OK(c, hipStreamWaitEvent(c->side, c->ev_side_go, 0));
+  }
+  if (side_itx) {
+    HIP_OK(c, launch_itx(ra, 1, c->side, &launches));
+    HIP_OK(c, hipEventRecord(c->ev_itx_done, c->side));
+  }
+  if (plan->has_grain) {
+    HIP_OK(c, launch_grain_template(L, plan->dev.grain, c->grain_tmpl, c->grain_offs, c->side, &launches));
+    HIP_OK(c, hipEventRecord(c->ev_side_done, c->side));
+  }
+  HIP_OK(c, hipEventRecord(c->ev[0], s));
   for (bool& h : c->stage_hit) h = false;
   auto begin = [&](int st) {
     if (c->timing) (void)hipEventRecord(c->sev[2 * st], s);
@@ -944,16 +1193,17 @@ int run_plan(Ctx* c, Plan* plan, int keep, int reserved, const av1h_planes* out,
       c->stage_hit[st] = true;
     }
   };
-  bool any_tx = false;
-  for (int k = 0; k < kTxClasses; ++k) any_tx = any_tx || plan->dev.itx_n[k] > 0;
-  if (plan->dev.mc_total > 0 || any_tx) {
+  if (plan->dev.mc_tasks > 0 || plan->dev.n_special > 0 || itx_groups(plan->dev, 0) > 0) {
     begin(AV1H_STAGE_INTER);
-    HIP_OK(c, launch_recon(ra, 0, c->cus, s, &launches));
+    HIP_OK(c, launch_mc(ra, c->cus, s, &launches));
+    HIP_OK(c, launch_special(ra, s, &launches));
+    HIP_OK(c, launch_itx(ra, 0, s, &launches));
     end(AV1H_STAGE_INTER);
   }
+  if (side_itx) HIP_OK(c, hipStreamWaitEvent(s, c->ev_itx_done, 0));
   if (plan->dev.n_queue > 0) {
     begin(AV1H_STAGE_QUEUE);
-    HIP_OK(c, launch_recon(ra, 1, c->cus, s, &launches));
+    HIP_OK(c, launch_queue(ra, c->cus, s, &launches));
     end(AV1H_STAGE_QUEUE);
   }
   HIP_OK(c, hipEventRecord(c->ev[1], s));
@@ -984,8 +1234,8 @@ int run_plan(Ctx* c, Plan* plan, int keep, int reserved, const av1h_planes* out,
   if (plan->upscaled) {
     HIP_OK(c, hipMemcpyAsync(c->scratch, c->frame, bytes, hipMemcpyDeviceToDevice, s));
     begin(AV1H_STAGE_SUPERRES);
-    HIP_OK(c, launch_superres(fa, c->scratch, c->frame, s, &launches));
-    if (plan->uses_lr) HIP_OK(c, launch_superres(fa, c->lr_curr, c->lr_prev_up, s, &launches));
+    HIP_OK(c, launch_superres(fa, c->scratch, c->frame, plan->uses_lr ? c->lr_curr : nullptr,
+                              plan->uses_lr ? c->lr_prev_up : nullptr, s, &launches));
     end(AV1H_STAGE_SUPERRES);
   }
   if (plan->uses_lr) {
@@ -1147,7 +1397,8 @@ int av1h_create(int device_index, av1h_ctx** out) {
     }
   if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_side_go, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_side_done, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&c->ev_side_done, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_itx_done, hipEventDisableTiming) != hipSuccess) {
     delete h;
     return AV1H_ERR_DEVICE;
   }
@@ -1162,7 +1413,7 @@ void av1h_destroy(av1h_ctx* h) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (void* p : {(void*)c->frame, (void*)c->scratch, (void*)c->lr_curr, (void*)c->lr_cdef, (void*)c->lr_prev_up,
                   (void*)c->pool, (void*)c->cells, (void*)c->counters, (void*)c->grain_tmpl, (void*)c->grain_offs,
-                  (void*)c->pack, (void*)c->cdef_dir})
+                  (void*)c->pack, (void*)c->cdef_dir, (void*)c->trace})
     if (p) (void)hipFree(p);
   if (c->host_pin) (void)hipHostFree(c->host_pin);
   free_plan(&c->scratch_plan);
@@ -1173,6 +1424,7 @@ void av1h_destroy(av1h_ctx* h) {
   if (c->side) (void)hipStreamSynchronize(c->side);
   if (c->ev_side_go) (void)hipEventDestroy(c->ev_side_go);
   if (c->ev_side_done) (void)hipEventDestroy(c->ev_side_done);
+  if (c->ev_itx_done) (void)hipEventDestroy(c->ev_itx_done);
   if (c->side) (void)hipStreamDestroy(c->side);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete h;
@@ -1442,6 +1694,22 @@ int av1h_get_stage_times(const av1h_ctx* h, double ms[AV1H_STAGES], int64_t coun
     if (bytes) bytes[st] = h->c.stage_bytes[st];
   }
   return AV1H_OK;
+}
+
+void av1h_set_queue_trace(av1h_ctx* h, int enable) {
+  if (h) h->c.trace_on = enable != 0;
+}
+
+int64_t av1h_get_queue_trace(av1h_ctx* h, uint64_t* out, int64_t cap) {
+  if (!h) return -1;
+  Ctx* c = &h->c;
+  std::lock_guard<std::mutex> lk(c->mu);
+  const int64_t n = c->trace_on && c->trace ? c->trace_n : 0;
+  if (out && cap >= 4 * n && n > 0) {
+    (void)hipSetDevice(c->device);
+    HIP_OK(c, hipMemcpy(out, c->trace, (size_t)n * 32, hipMemcpyDeviceToHost));
+  }
+  return n;
 }
 
 int av1h_decode_batch(av1h_ctx* const* ctxs, const av1h_frame_desc* const* frames, int count, int* slots_out) {

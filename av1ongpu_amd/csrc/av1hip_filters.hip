@@ -294,22 +294,48 @@ struct UpscalePlane {
   int step[3], initial[3];
 };
 
-__global__ void k_upscale(FilterArgs a, UpscalePlane u, const uint16_t* src, uint16_t* dst) {
-  const int p = blockIdx.z;
+/* 256 x 4 output samples of one plane of one picture per workgroup: the
+ * source span of the 256 columns (at most 264 samples when upscaling) is
+ * staged in LDS per row, clamped to the coded width as each tap is
+ * (av1gpu_kernels.h:2387-2402). blockIdx.z: picture * 3 + plane. */
+constexpr int kUpW = 256, kUpRows = 4, kUpSpan = 272;
+
+__global__ __launch_bounds__(256) void k_upscale(FilterArgs a, UpscalePlane u, const uint16_t* src0,
+                                                 uint16_t* dst0, const uint16_t* src1, uint16_t* dst1) {
+  __shared__ int s_k[512];
+  __shared__ uint16_t s_row[kUpRows][kUpSpan];
+  const int pic = blockIdx.z / 3, p = blockIdx.z - 3 * pic;
+  const uint16_t* src = pic ? src1 : src0;
+  uint16_t* dst = pic ? dst1 : dst0;
   const Layout& L = a.lay;
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y = blockIdx.y;
-  if (x >= L.out_w[p] || y >= L.out_h[p]) return;
-  const int* uf = SPEC(AV1T_OFF_UPSCALE);
-  const uint16_t* row = src + L.plane_off[p] + (size_t)y * L.stride[p];
-  const int sx = -(1 << 14) + u.initial[p] + x * u.step[p];
-  const int px = sx >> 14;
-  const int* k = uf + ((sx & ((1 << 14) - 1)) >> 8) * 8;
-  const int maxx = L.plane_w[p] - 1;
-  int sum = 0;
+  const int ow = p == 0 ? L.out_w[0] : L.out_w[1], oh = p == 0 ? L.out_h[0] : L.out_h[1];
+  const int x0 = blockIdx.x * kUpW, y0 = blockIdx.y * kUpRows;
+  if (x0 >= ow || y0 >= oh) return;
+  for (int i = threadIdx.x; i < 512; i += 256) s_k[i] = av1t_spec_table[AV1T_OFF_UPSCALE + i];
+  const int step = u.step[p], init = u.initial[p];
+  const int maxx = (p == 0 ? L.plane_w[0] : L.plane_w[1]) - 1;
+  const int stride = p == 0 ? L.stride[0] : L.stride[1];
+  const int64_t poff = p == 0 ? L.plane_off[0] : (p == 1 ? L.plane_off[1] : L.plane_off[2]);
+  const int xl = min(x0 + kUpW, ow) - 1;
+  const int ws = ((-(1 << 14) + init + x0 * step) >> 14) - 3;
+  const int wn = ((-(1 << 14) + init + xl * step) >> 14) + 4 - ws + 1;
+  const int rows = min(kUpRows, oh - y0);
+  for (int i = threadIdx.x; i < rows * wn; i += 256) {
+    const int r = i / wn, c = i - r * wn;
+    s_row[r][c] = src[poff + (size_t)(y0 + r) * stride + clampi(ws + c, 0, maxx)];
+  }
+  __syncthreads();
+  const int x = x0 + threadIdx.x;
+  if (x > xl) return;
+  const int sx = -(1 << 14) + init + x * step;
+  const int o = (sx >> 14) - 3 - ws;
+  const int* k = s_k + ((sx & ((1 << 14) - 1)) >> 8) * 8;
+  for (int r = 0; r < rows; ++r) {
+    int sum = 0;
 #pragma unroll
-  for (int t = 0; t < 8; ++t) sum += row[clampi(px + t - 3, 0, maxx)] * k[t];
-  dst[L.plane_off[p] + (size_t)y * L.stride[p] + x] = (uint16_t)clampi((sum + 64) >> 7, 0, a.maxv);
+    for (int t = 0; t < 8; ++t) sum += s_row[r][o + t] * k[t];
+    dst[poff + (size_t)(y0 + r) * stride + x] = (uint16_t)clampi((sum + 64) >> 7, 0, a.maxv);
+  }
 }
 
 /* ---- loop restoration (av1gpu_kernels.h:2832-2982) ---------------------- */
@@ -565,12 +591,14 @@ void superres_params(int dw, int uw, int* step, int* initial) {
   *initial = sub & ((1 << 14) - 1);
 }
 
-hipError_t launch_superres(const FilterArgs& a, const uint16_t* src, uint16_t* dst, hipStream_t s,
-                           int* launches) {
+hipError_t launch_superres(const FilterArgs& a, const uint16_t* src, uint16_t* dst, const uint16_t* src2,
+                           uint16_t* dst2, hipStream_t s, int* launches) {
   UpscalePlane u;
   for (int p = 0; p < 3; ++p) superres_params(a.lay.plane_w[p], a.lay.out_w[p], &u.step[p], &u.initial[p]);
-  hipLaunchKernelGGL(k_upscale, dim3((a.lay.out_w[0] + 127) / 128, a.lay.out_h[0], 3), dim3(128), 0, s,
-                     a, u, src, dst);
+  const int pics = src2 ? 2 : 1;
+  hipLaunchKernelGGL(k_upscale, dim3((a.lay.out_w[0] + kUpW - 1) / kUpW, (a.lay.out_h[0] + kUpRows - 1) / kUpRows,
+                                     3 * pics),
+                     dim3(256), 0, s, a, u, src, dst, src2 ? src2 : src, dst2 ? dst2 : dst);
   ++*launches;
   return hipGetLastError();
 }

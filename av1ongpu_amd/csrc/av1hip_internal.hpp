@@ -69,7 +69,7 @@ struct DevBlock {
    * left column rows [dep_y0, dep_y1], above row columns [dep_x0, dep_x1],
    * the corner when dep_corner; dep_x1 < x and dep_y1 < y disable a side */
   int32_t dep_x1, dep_y1, dep_corner;
-  int32_t pad;
+  int32_t dep_rx0, dep_ry0, dep_rx1, dep_ry1; /* intra block copy: source cells */
 };
 
 /* flag bits of DevTx::flags */
@@ -87,7 +87,13 @@ struct DevTx {
 
 /* transform classes: the larger side, 4 8 16 32 64 */
 constexpr int kTxClasses = 5;
-constexpr int kMcChunkSamples = 256; /* motion compensation samples per workgroup */
+/* motion compensation tasks: at most 256 samples and kMcItems items, and
+ * within the LDS budget of the window bounds (2w + 9) x (2h + 9) and of the
+ * intermediate bounds (2h + 9) x w, per reference */
+constexpr int kMcSamples = 256;
+constexpr int kMcItems = 16;
+constexpr int kMcWinPerRef = kMcItems * 17 * 17;
+constexpr int kMcMidPerRef = kMcItems * 17 * 4;
 constexpr int kLrTileCols = 64, kLrTileRows = 16; /* loop restoration tile per workgroup */
 
 struct DevObmc {
@@ -132,6 +138,7 @@ struct FramePlanDev {
   const DevObmc* obmc;
   const int32_t* inter_list;
   const int32_t* queue_list;
+  const int32_t* special_list; /* inter blocks with warp, masks or OBMC */
   const int32_t* coef;
   const int32_t* residual;
   const uint8_t* palette;
@@ -139,15 +146,21 @@ struct FramePlanDev {
   const DevCdef* cdef;
   const DevLr* lr;
   const DevGrain* grain;
-  /* motion compensation: sample prefix over inter_list (n_inter + 1 entries),
-   * and the inter_list entry holding the first sample of each 256 sample chunk */
-  const int32_t* mc_px0;
-  const int32_t* mc_chunk;
-  /* transform blocks by class, sorted by size inside a class */
-  const int32_t* itx_list[kTxClasses];
-  int32_t itx_n[kTxClasses];
+  /* motion compensation: items (block index, first row | rows << 16) in
+   * coding order, grouped into tasks; task t is items [mc_task[t],
+   * mc_task[t + 1]) */
+  const int2* mc_items;
+  const int32_t* mc_task;
+  int32_t mc_tasks;
+  /* transform blocks: set 0 those of inter blocks, set 1 those of intra
+   * blocks; inside a set by class (larger side 4 8 16 32 64), inside a class
+   * by size; set s class c is itx_list[itx_off[s][c] .. itx_off[s][c + 1]).
+   * itx_offs is the same table in device memory. */
+  const int32_t* itx_list;
+  const int32_t* itx_offs;
+  int32_t itx_off[2][kTxClasses + 1];
+  int32_t n_special;
   int32_t n_inter, n_queue, n_deblock_v, n_deblock_h, lines_v, lines_h, n_cdef, n_lr, lr_px;
-  int32_t mc_total, mc_chunks;
   /* deblocking: the first edge of every 256 line chunk, vertical pass chunks
    * then horizontal */
   const int32_t* db_chunk;
@@ -172,14 +185,22 @@ struct ReconArgs {
   uint32_t* queue_counter;
   uint32_t* error;      /* bit 0: a dependency wait timed out  */
   int16_t* res;         /* residual of intra blocks, frame layout */
+  uint64_t* trace;      /* null, or 4 timestamps per queue entry (diagnostics) */
   int maxv;
   int edge_filter;      /* enable_intra_edge_filter           */
 };
 
-/* part 0: motion compensation of every inter block, then every transform
- * (inter residuals added into the frame, intra residuals into `res`);
- * part 1: the dependency queue over the intra blocks */
-hipError_t launch_recon(const ReconArgs& a, int part, int cus, hipStream_t s, int* launches);
+/* motion compensation of every inter block into the frame */
+hipError_t launch_mc(const ReconArgs& a, int cus, hipStream_t s, int* launches);
+/* the inter blocks with warp, masks or OBMC, after motion compensation */
+hipError_t launch_special(const ReconArgs& a, hipStream_t s, int* launches);
+/* the transforms of one set: 0 inter blocks (residual added into the frame,
+ * after motion compensation), 1 intra blocks (residual into `res`) */
+hipError_t launch_itx(const ReconArgs& a, int set, hipStream_t s, int* launches);
+/* the dependency queue over the intra blocks */
+hipError_t launch_queue(const ReconArgs& a, int cus, hipStream_t s, int* launches);
+/* workgroups a transform set launches (0: nothing to do) */
+int itx_groups(const FramePlanDev& p, int set);
 
 struct FilterArgs {
   FramePlanDev plan;
@@ -197,8 +218,9 @@ struct FilterArgs {
 
 hipError_t launch_deblock(const FilterArgs& a, hipStream_t s, int* launches);
 hipError_t launch_cdef(const FilterArgs& a, hipStream_t s, int* launches);
-hipError_t launch_superres(const FilterArgs& a, const uint16_t* src, uint16_t* dst, hipStream_t s,
-                           int* launches);
+/* upscales src into dst, and src2 into dst2 when src2 is not null, in one launch */
+hipError_t launch_superres(const FilterArgs& a, const uint16_t* src, uint16_t* dst, const uint16_t* src2,
+                           uint16_t* dst2, hipStream_t s, int* launches);
 hipError_t launch_lr(const FilterArgs& a, const uint16_t* cdef, const uint16_t* curr, hipStream_t s,
                      int* launches);
 /* the three grain templates and the block offsets (one workgroup; depends only

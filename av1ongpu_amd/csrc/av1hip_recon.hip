@@ -43,6 +43,17 @@ __device__ __forceinline__ int rnd2s(int v, int n) {
 __device__ __forceinline__ int rnd2s_add(int v, int n) { return (v + (1 << (n - 1)) + (v < 0 ? -1 : 0)) >> n; }
 __device__ __forceinline__ int ilog2(int v) { return 31 - __builtin_clz(v); }
 
+/* Samples other waves wrote in this launch are exchanged at device scope:
+ * the stores and loads go to the level every XCD sees (the queue publishes a
+ * block's cells only after its stores completed), so no cache writeback or
+ * invalidate is needed around the dependency flags. */
+__device__ __forceinline__ int ld_dev(const uint16_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_dev(uint16_t* p, uint16_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 /* one wave per workgroup: the workgroup barrier is the wave barrier */
 __device__ __forceinline__ void wsync() { __syncthreads(); }
 
@@ -58,7 +69,7 @@ __device__ int wave_sum(int v) {
 
 /* row pass (av1gpu_kernels.h:869-904): lane = row */
 template <int N>
-__device__ void row_pass(int* tmp, int lane, int w, int h, int tx_type, int lossless, int row_shift, int bd,
+__device__ __forceinline__ void row_pass(int* tmp, int lane, int w, int h, int tx_type, int lossless, int row_shift, int bd,
                          int rect) {
   const int y = lane;
   if (y >= h) return;
@@ -93,7 +104,7 @@ __device__ void row_pass(int* tmp, int lane, int w, int h, int tx_type, int loss
 /* column pass (av1gpu_kernels.h:907-930): lane = column; hands each residual
  * sample to `out(x, y, value)` */
 template <int M, typename Out>
-__device__ void col_pass(const int* tmp, int lane, int w, int tx_type, int lossless, int bd, Out&& out) {
+__device__ __forceinline__ void col_pass(const int* tmp, int lane, int w, int tx_type, int lossless, int bd, Out&& out) {
   const int x = lane;
   if (x >= w) return;
   const int st = w + 1;
@@ -115,7 +126,7 @@ __device__ void col_pass(const int* tmp, int lane, int w, int tx_type, int lossl
 /* the row / column pass of the size `l2` (log2 points), instantiated only up
  * to the class size G */
 template <int G>
-__device__ void row_pass_n(int l2, int* tmp, int lane, int w, int h, int tx_type, int lossless, int row_shift,
+__device__ __forceinline__ void row_pass_n(int l2, int* tmp, int lane, int w, int h, int tx_type, int lossless, int row_shift,
                            int bd, int rect) {
   switch (l2) {
     case 2: row_pass<4>(tmp, lane, w, h, tx_type, lossless, row_shift, bd, rect); break;
@@ -126,7 +137,7 @@ __device__ void row_pass_n(int l2, int* tmp, int lane, int w, int h, int tx_type
   }
 }
 template <int G, typename Out>
-__device__ void col_pass_n(int l2, const int* tmp, int lane, int w, int tx_type, int lossless, int bd, Out&& out) {
+__device__ __forceinline__ void col_pass_n(int l2, const int* tmp, int lane, int w, int tx_type, int lossless, int bd, Out&& out) {
   switch (l2) {
     case 2: col_pass<4>(tmp, lane, w, tx_type, lossless, bd, out); break;
     case 3: if constexpr (G >= 8) col_pass<8>(tmp, lane, w, tx_type, lossless, bd, out); break;
@@ -136,21 +147,20 @@ __device__ void col_pass_n(int l2, const int* tmp, int lane, int w, int tx_type,
   }
 }
 
-/* Every transform block of class G (larger side G): G lanes per transform,
+/* The transform blocks of class G (larger side G): G lanes per transform,
  * 64 / G transforms per wave, coefficients staged in LDS between the row and
  * the column pass. The residual of an inter block is added into the frame
  * (its prediction is already there); that of an intra block goes to `res`,
  * where the dependency queue adds it after the prediction. */
 template <int G>
-__global__ __launch_bounds__(64) void k_itx(ReconArgs a, int cls) {
+__device__ __forceinline__ void itx_group(const ReconArgs& a, const int32_t* list, int n, int wg, int* s_tmp) {
   constexpr int kPer = 64 / G;
-  __shared__ int s_tmp[kPer * G * (G + 1)];
   const int grp = threadIdx.x / G, l = threadIdx.x - grp * G;
-  const int idx = blockIdx.x * kPer + grp;
-  const bool live = idx < a.plan.itx_n[cls];
+  const int idx = wg * kPer + grp;
+  const bool live = idx < n;
   DevTx tx;
   if (live) {
-    tx = a.plan.tx[a.plan.itx_list[cls][idx]];
+    tx = a.plan.tx[list[idx]];
   } else {
     tx.w = tx.h = 4;
     tx.flags = kTxRaw;
@@ -159,8 +169,10 @@ __global__ __launch_bounds__(64) void k_itx(ReconArgs a, int cls) {
   }
   int* tmp = s_tmp + grp * G * (G + 1);
   const int w = tx.w, h = tx.h;
-  const int stride = a.lay.stride[tx.plane];
-  const int64_t off = a.lay.plane_off[tx.plane] + (int64_t)tx.y * stride + tx.x;
+  const int stride = tx.plane == 0 ? a.lay.stride[0] : (tx.plane == 1 ? a.lay.stride[1] : a.lay.stride[2]);
+  const int64_t poff =
+      tx.plane == 0 ? a.lay.plane_off[0] : (tx.plane == 1 ? a.lay.plane_off[1] : a.lay.plane_off[2]);
+  const int64_t off = poff + (int64_t)tx.y * stride + tx.x;
   const bool to_res = (tx.flags & kTxToResidual) != 0;
   const int maxv = a.maxv;
   auto out = [&](int xx, int yy, int v) {
@@ -171,12 +183,11 @@ __global__ __launch_bounds__(64) void k_itx(ReconArgs a, int cls) {
       a.frame[o] = (uint16_t)clampi((int)a.frame[o] + v, 0, maxv);
   };
   const bool coded = live && !(tx.flags & kTxRaw);
+  const int lw = ilog2(w), lh = ilog2(h);
   if (live && !coded) {
     /* raw residual (av1gpu_kernels.h:933-937) */
-    const int lw = ilog2(w);
     for (int i = l; i < w * h; i += G) out(i & (w - 1), i >> lw, a.plan.residual[tx.coef + i]);
   }
-  const int lw = ilog2(w), lh = ilog2(h);
   if (coded) {
     const int cs = w < 32 ? w : 32, cr = h < 32 ? h : 32;
     const int lcs = ilog2(cs), st = w + 1;
@@ -189,6 +200,30 @@ __global__ __launch_bounds__(64) void k_itx(ReconArgs a, int cls) {
                   (lw - lh == 1 || lh - lw == 1) ? 1 : 0);
   __syncthreads();
   if (coded) col_pass_n<G>(lh, tmp, l, w, tx.tx_type, lossless, a.lay.bit_depth, out);
+}
+
+/* One launch for every class of a set: workgroups of the 64 class first (the
+ * longest), then 32, 16, 8, 4. */
+__global__ __launch_bounds__(64) void k_itx(ReconArgs a, const int32_t* offs) {
+  __shared__ int s_tmp[64 * 65];
+  int b = blockIdx.x;
+  for (int c = kTxClasses - 1; c >= 0; --c) {
+    const int o0 = offs[c], n = offs[c + 1] - o0;
+    const int per = 16 >> c;
+    const int wgs = (n + per - 1) / per;
+    if (b < wgs) {
+      const int32_t* list = a.plan.itx_list + o0;
+      switch (c) {
+        case 0: itx_group<4>(a, list, n, b, s_tmp); break;
+        case 1: itx_group<8>(a, list, n, b, s_tmp); break;
+        case 2: itx_group<16>(a, list, n, b, s_tmp); break;
+        case 3: itx_group<32>(a, list, n, b, s_tmp); break;
+        default: itx_group<64>(a, list, n, b, s_tmp); break;
+      }
+      return;
+    }
+    b -= wgs;
+  }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -257,30 +292,277 @@ __device__ const uint16_t* ref_plane(const ReconArgs& a, int slot, int plane) {
   return base + a.lay.plane_off[plane];
 }
 
-/* One reference's prediction of sample (x, y) of a block: eight horizontally
- * filtered rows, rounded, then the vertical filter (av1gpu_kernels.h:
- * 1756-1857). Every thread filters its own eight rows; the reference samples
- * its neighbours share come from the cache. */
-__device__ __forceinline__ int mc_sample(const uint16_t* ref, int stride, int ref_w, int ref_h, const McPos& P,
-                                         int x, int y, const int* hk0, const int* vk0, int round0, int round1) {
+/* ------------------------------------------------------------------------ */
+/* the remaining inter tools, one sample at a time (av1gpu_kernels.h:        */
+/* 1880-2326): global warp, wedge and difference weighted masks, OBMC,       */
+/* inter/intra, intra block copy. They are rare, so they are computed the    */
+/* direct way; samples of the frame being decoded are read at device scope.  */
+/* ------------------------------------------------------------------------ */
+
+__device__ __forceinline__ int fetch_px(const uint16_t* p, bool cur, int stride, int x, int y, int x1, int y1) {
+  x = clampi(x, 0, x1);
+  y = clampi(y, 0, y1);
+  const uint16_t* q = p + (size_t)y * stride + x;
+  return cur ? ld_dev(q) : (int)*q;
+}
+
+/* one sample of one reference through the two stage filter, unscaled or
+ * scaled (av1gpu_kernels.h:1756-1857) */
+__device__ int mc_tap(const uint16_t* ref, bool cur, int stride, int ref_w, int ref_h, int sub_x, int sub_y,
+                      int scx, int scy, int bx, int by, int ew, int eh, int mvx, int mvy, int filt, int filt_v,
+                      int x, int y, int compound, int bd) {
+  const int round0 = bd >= 12 ? 5 : 3;
+  int round1 = compound ? 7 : 11;
+  if (bd >= 12 && !compound) round1 -= 2;
+  const McPos P = mc_pos(bx, by, mvx, mvy, sub_x, sub_y, scx, scy);
   int cx, fx, cy, fy;
   P.col(x, &cx, &fx);
   P.row(y, &cy, &fy);
-  const int* hk = hk0 + fx * 8;
-  const int* vk = vk0 + fy * 8;
-  int cols[8];
-#pragma unroll
-  for (int t = 0; t < 8; ++t) cols[t] = clampi(cx - 3 + t, 0, ref_w - 1);
+  const int* hk = SPEC(AV1T_OFF_SUBPEL) + (filt_row(filt, ew) * 16 + fx) * 8;
+  const int* vk = SPEC(AV1T_OFF_SUBPEL) + (filt_row(filt_v, eh) * 16 + fy) * 8;
   int acc = 0;
-#pragma unroll
   for (int r = 0; r < 8; ++r) {
-    const uint16_t* row = ref + (size_t)clampi(cy - 3 + r, 0, ref_h - 1) * stride;
     int s = 0;
-#pragma unroll
-    for (int t = 0; t < 8; ++t) s += (int)row[cols[t]] * hk[t];
+    for (int t = 0; t < 8; ++t) s += fetch_px(ref, cur, stride, cx - 3 + t, cy - 3 + r, ref_w - 1, ref_h - 1) * hk[t];
     acc += ((s + (1 << (round0 - 1))) >> round0) * vk[r];
   }
   return (acc + (1 << (round1 - 1))) >> round1;
+}
+
+__device__ __forceinline__ int warp_offs(int v) { return clampi(((v + 512) >> 10) + 64, 0, 192); }
+
+/* one sample of a warped reference (av1gpu_kernels.h:1919-1958) */
+__device__ int warp_tap(int sub_x, int sub_y, int ref_w, int ref_h, const int* wp, const int* abcd,
+                        const uint16_t* ref, bool cur, int stride, int bx, int by, int x, int y, int bd,
+                        int compound) {
+  const int* wf = SPEC(AV1T_OFF_WARP);
+  const int i1 = (y & 7) - 4, i2 = (x & 7) - 4;
+  const int64_t srcx = (int64_t)((bx + (x >> 3) * 8 + 4) << sub_x);
+  const int64_t srcy = (int64_t)((by + (y >> 3) * 8 + 4) << sub_y);
+  const int64_t dstx = (int64_t)wp[2] * srcx + (int64_t)wp[3] * srcy + wp[0];
+  const int64_t dsty = (int64_t)wp[4] * srcx + (int64_t)wp[5] * srcy + wp[1];
+  const int64_t x4 = dstx >> sub_x, y4 = dsty >> sub_y;
+  const int ix4 = (int)(x4 >> 16), sx4 = (int)(x4 & 0xffff);
+  const int iy4 = (int)(y4 >> 16), sy4 = (int)(y4 & 0xffff);
+  const int round0 = bd >= 12 ? 5 : 3;
+  int round1 = compound ? 7 : 11;
+  if (bd >= 12 && !compound) round1 -= 2;
+  const int oy = warp_offs(sy4 + abcd[2] * i2 + abcd[3] * i1);
+  int acc = 0;
+  for (int k = 0; k < 8; ++k) {
+    const int row = i1 - 3 + k;
+    const int ox = warp_offs(sx4 + abcd[0] * i2 + abcd[1] * row);
+    int s = 0;
+    for (int t = 0; t < 8; ++t)
+      s += wf[ox * 8 + t] * fetch_px(ref, cur, stride, ix4 + i2 - 3 + t, iy4 + row, ref_w - 1, ref_h - 1);
+    acc += ((s + (1 << (round0 - 1))) >> round0) * wf[oy * 8 + k];
+  }
+  return (acc + (1 << (round1 - 1))) >> round1;
+}
+
+/* the wedge master masks (av1gpu_kernels.h:1968-2014) */
+__constant__ const int kWedgeEven[8] = {1, 4, 11, 27, 46, 58, 62, 63};
+__constant__ const int kWedgeOdd[8] = {1, 2, 6, 18, 37, 53, 60, 63};
+__constant__ const int kWedgeVert[6] = {2, 7, 21, 43, 57, 62};
+
+__device__ int wedge_1d(int which, int j) {
+  if (which == 2) return j < 29 ? 0 : (j >= 35 ? 64 : kWedgeVert[j - 29]);
+  if (j < 28) return 0;
+  if (j >= 36) return 64;
+  return which == 0 ? kWedgeEven[j - 28] : kWedgeOdd[j - 28];
+}
+__device__ int wedge_ob63(int i, int j) {
+  const int shift = (i & 1) ? 15 - (i >> 1) : 16 - (i >> 1);
+  return wedge_1d(i & 1, clampi(j - shift, 0, 63));
+}
+__device__ int wedge_master(int dir, int i, int j) {
+  switch (dir) {
+    case 0: return wedge_1d(2, i);
+    case 1: return wedge_1d(2, j);
+    case 2: return wedge_ob63(j, i);
+    case 4: return 64 - wedge_ob63(i, 63 - j);
+    case 5: return 64 - wedge_ob63(j, 63 - i);
+    default: return wedge_ob63(i, j);
+  }
+}
+__device__ int wedge_px(const DevBlock& b, int i, int k) {
+  const int m = wedge_master(b.wedge_dir, clampi(b.wedge_yoff + i, 0, 63), clampi(b.wedge_xoff + k, 0, 63));
+  return b.wedge_flip ? 64 - m : m;
+}
+
+/* av1gpu_kernels.h:2025-2029 */
+__device__ __forceinline__ int diffwtd(int d, int sh, int sign) {
+  d = (d + (1 << (sh - 1))) >> sh;
+  d = clampi(38 + (d >> 4), 0, 64);
+  return sign ? 64 - d : d;
+}
+
+/* av1gpu_kernels.h:2076-2100 */
+__constant__ const int kObmcMask[62] = {45, 64,
+                                        39, 50, 59, 64,
+                                        36, 42, 48, 53, 57, 61, 64, 64,
+                                        34, 37, 40, 43, 46, 49, 52, 54, 56, 58, 60, 61, 64, 64, 64, 64,
+                                        33, 35, 36, 38, 40, 41, 43, 44, 45, 47, 48, 50, 51, 52, 53, 55,
+                                        56, 57, 58, 59, 60, 60, 61, 62, 64, 64, 64, 64, 64, 64, 64, 64};
+__device__ __forceinline__ int obmc_mask(int len, int i) {
+  const int base = len == 2 ? 0 : (len == 4 ? 2 : (len == 8 ? 6 : (len == 16 ? 14 : 30)));
+  return kObmcMask[base + i];
+}
+
+/* inter/intra edges and prediction (av1gpu_kernels.h:2120-2205) */
+struct IiCtx {
+  const uint16_t* p; /* the block's top left sample in the frame being decoded */
+  int stride, x, y, w, h, max_x, max_y, have_left, have_above, bd;
+  __device__ int above(int i) const {
+    const int mid = 1 << (bd - 1);
+    if (!have_above) return have_left ? ld_dev(p - 1) : mid - 1;
+    if (x + i > max_x) i = max_x - x;
+    return ld_dev(p - stride + i);
+  }
+  __device__ int left(int i) const {
+    const int mid = 1 << (bd - 1);
+    if (!have_left) return have_above ? ld_dev(p - stride) : mid + 1;
+    if (y + i > max_y) i = max_y - y;
+    return ld_dev(p + (ptrdiff_t)i * stride - 1);
+  }
+};
+
+__device__ int ii_sample(const IiCtx& c, int mode, int xs, int ys) {
+  if (mode == 1) return c.above(xs);
+  if (mode == 2) return c.left(ys);
+  const int* sw = SPEC(AV1T_OFF_SM_WEIGHTS);
+  if (mode == 3) {
+    const int wy = sw[c.h - 4 + ys], wx = sw[c.w - 4 + xs];
+    return rnd2(wy * c.above(xs) + (256 - wy) * c.left(c.h - 1) + wx * c.left(ys) + (256 - wx) * c.above(c.w - 1), 9);
+  }
+  int sum = 0;
+  if (c.have_above && c.have_left) {
+    for (int k = 0; k < c.w; ++k) sum += c.above(k);
+    for (int k = 0; k < c.h; ++k) sum += c.left(k);
+    return (sum + ((c.w + c.h) >> 1)) / (c.w + c.h);
+  }
+  if (c.have_left) {
+    for (int k = 0; k < c.h; ++k) sum += c.left(k);
+    return (sum + (c.h >> 1)) >> ilog2(c.h);
+  }
+  if (c.have_above) {
+    for (int k = 0; k < c.w; ++k) sum += c.above(k);
+    return (sum + (c.w >> 1)) >> ilog2(c.w);
+  }
+  return 1 << (c.bd - 1);
+}
+
+__device__ int ii_mask(const DevBlock& b, int sub_x, int sub_y, int xs, int ys) {
+  if (b.flags & kIIWedge) {
+    const int ny = sub_y + 1, nx = sub_x + 1, n = nx * ny;
+    int sum = 0;
+    for (int i = 0; i < ny; ++i)
+      for (int k = 0; k < nx; ++k) sum += wedge_px(b, (ys << sub_y) + i, (xs << sub_x) + k);
+    return n == 1 ? sum : (n == 2 ? (sum + 1) >> 1 : (sum + 2) >> 2);
+  }
+  if (b.ii_mode == 0) return 32;
+  const int scale = 128 / max(b.w, b.h);
+  int k = 0;
+  if (b.ii_mode == 1)
+    k = ys * scale;
+  else if (b.ii_mode == 2)
+    k = xs * scale;
+  else if (b.ii_mode == 3)
+    k = min(xs, ys) * scale;
+  return SPEC(AV1T_OFF_II_WEIGHTS)[min(k, 127)];
+}
+
+/* Sample (xs, ys) of a block using any inter tool (av1gpu_kernels.h:2207-2326).
+ * References of slot -1 (intra block copy) are the frame being decoded. */
+__device__ int inter_sample(const ReconArgs& a, const DevBlock& b, int xs, int ys) {
+  const Layout& L = a.lay;
+  const int p = b.plane, bd = L.bit_depth;
+  const int sub_x = p > 0 ? L.ss_x : 0, sub_y = p > 0 ? L.ss_y : 0;
+  const int stride = L.stride[p];
+  const int compound = b.compound != 0;
+  int v[2] = {0, 0};
+  for (int r = 0; r < (compound ? 2 : 1); ++r) {
+    const uint16_t* base = ref_plane(a, b.slot[r], p);
+    const bool cur = b.slot[r] < 0;
+    const int cmp = r ? 1 : compound;
+    v[r] = (b.flags & (r ? kWarp1 : kWarp0))
+               ? warp_tap(sub_x, sub_y, b.ref_w, b.ref_h, b.warp_params[r], b.warp_abcd[r], base, cur, stride,
+                          b.x, b.y, xs, ys, bd, cmp)
+               : mc_tap(base, cur, stride, b.ref_w, b.ref_h, sub_x, sub_y, b.scale_x, b.scale_y, b.x, b.y, b.w,
+                        b.h, b.mv[r][0], b.mv[r][1], b.filt, b.filt_v, xs, ys, cmp, bd);
+  }
+  int out = v[0];
+  if (compound) {
+    const int round0 = bd >= 12 ? 5 : 3;
+    const int post = 2 * 7 - (round0 + 7);
+    if (b.mask_kind != 0) {
+      const int sh = post + (bd - 8);
+      const int ny = sub_y + 1, nx = sub_x + 1;
+      int sum = 0, n = 0;
+      for (int i = 0; i < ny; ++i)
+        for (int k = 0; k < nx; ++k) {
+          const int lx = (xs << sub_x) + k, ly = (ys << sub_y) + i;
+          int m;
+          if (b.mask_kind == 1) {
+            m = wedge_px(b, ly, lx);
+            if (b.mask_sign) m = 64 - m;
+          } else if (p == 0) {
+            m = diffwtd(abs(v[0] - v[1]), sh, b.mask_sign);
+          } else {
+            /* the luma difference at the co-located luma sample */
+            int lv[2];
+            for (int r = 0; r < 2; ++r) {
+              const uint16_t* lb = ref_plane(a, b.slot[r], 0);
+              const bool cur = b.slot[r] < 0;
+              lv[r] = (b.flags & (r ? kMaskWarp1 : kMaskWarp0))
+                          ? warp_tap(0, 0, L.out_w[0], L.out_h[0], b.warp_params[r], b.warp_abcd[r], lb, cur,
+                                     L.stride[0], b.x << L.ss_x, b.y << L.ss_y, lx, ly, bd, 1)
+                          : mc_tap(lb, cur, L.stride[0], L.out_w[0], L.out_h[0], 0, 0, b.scale_x, b.scale_y,
+                                   b.x << L.ss_x, b.y << L.ss_y, b.w << sub_x, b.h << sub_y, b.mv[r][0],
+                                   b.mv[r][1], b.filt, b.filt_v, lx, ly, 1, bd);
+            }
+            m = diffwtd(abs(lv[0] - lv[1]), sh, b.mask_sign);
+          }
+          sum += m;
+          ++n;
+        }
+      const int m = n == 1 ? sum : (n == 2 ? (sum + 1) >> 1 : (sum + 2) >> 2);
+      out = (m * v[0] + (64 - m) * v[1] + (1 << (5 + post))) >> (6 + post);
+    } else if (b.compound == AV1H_COMPOUND_AVERAGE) {
+      out = (v[0] + v[1] + (1 << post)) >> (1 + post);
+    } else {
+      out = (b.w0 * v[0] + b.w1 * v[1] + (1 << (3 + post))) >> (4 + post);
+    }
+  }
+  out = clampi(out, 0, a.maxv);
+  if (b.flags & kInterIntra) {
+    IiCtx c;
+    c.p = a.frame + L.plane_off[p] + (size_t)b.y * stride + b.x;
+    c.stride = stride;
+    c.x = b.x;
+    c.y = b.y;
+    c.w = b.w;
+    c.h = b.h;
+    c.max_x = L.grid_w[p] - 1;
+    c.max_y = L.grid_h[p] - 1;
+    c.have_left = (b.flags & kHaveLeft) != 0;
+    c.have_above = (b.flags & kHaveAbove) != 0;
+    c.bd = bd;
+    const int m = ii_mask(b, sub_x, sub_y, xs, ys);
+    out = (m * ii_sample(c, b.ii_mode, xs, ys) + (64 - m) * out + 32) >> 6;
+  }
+  for (int k = 0; k < b.obmc_count; ++k) {
+    const DevObmc& o = a.plan.obmc[b.obmc_first + k];
+    if (xs < o.rx || xs >= o.rx + o.w || ys < o.ry || ys >= o.ry + o.h) continue;
+    const int cj = xs - o.rx, ci = ys - o.ry;
+    const int m = o.pass == 0 ? obmc_mask(o.h, ci) : obmc_mask(o.w, cj);
+    const uint16_t* base = ref_plane(a, o.slot, p);
+    int pred = mc_tap(base, o.slot < 0, stride, b.ref_w, b.ref_h, sub_x, sub_y, b.scale_x, b.scale_y, b.x + o.rx,
+                      b.y + o.ry, o.w, o.h, o.mvx, o.mvy, o.filt, o.filt_v, cj, ci, 0, bd);
+    pred = clampi(pred, 0, a.maxv);
+    out = (m * out + (64 - m) * pred + 32) >> 6;
+  }
+  return out;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -373,19 +655,19 @@ __device__ void predict_intra(const ReconArgs& a, const DevBlock& b, int* work, 
     const int alast = min(max_x, b.x + ((b.flags & kHaveAR) ? 2 * w : w) - 1);
     const int llast = min(max_y, b.y + ((b.flags & kHaveBL) ? 2 * h : h) - 1);
     for (int i = lane; i < n; i += 64) {
-      above[i] = have_above ? (int)src[-stride + (min(b.x + i, alast) - b.x)]
-                            : (have_left ? (int)src[-1] : mid - 1);
-      left[i] = have_left ? (int)src[(ptrdiff_t)(min(b.y + i, llast) - b.y) * stride - 1]
-                          : (have_above ? (int)src[-stride] : mid + 1);
+      above[i] = have_above ? ld_dev(src - stride + (min(b.x + i, alast) - b.x))
+                            : (have_left ? ld_dev(src - 1) : mid - 1);
+      left[i] = have_left ? ld_dev(src + (ptrdiff_t)(min(b.y + i, llast) - b.y) * stride - 1)
+                          : (have_above ? ld_dev(src - stride) : mid + 1);
     }
     if (lane == 0) {
       int c;
       if (have_above && have_left)
-        c = src[-stride - 1];
+        c = ld_dev(src - stride - 1);
       else if (have_above)
-        c = src[-stride];
+        c = ld_dev(src - stride);
       else if (have_left)
-        c = src[-1];
+        c = ld_dev(src - 1);
       else
         c = mid;
       above[-1] = c;
@@ -421,7 +703,7 @@ __device__ void predict_intra(const ReconArgs& a, const DevBlock& b, int* work, 
       const int ly = min((b.y + y) << sy, ylim), lx = min((b.x + x) << sx, xlim);
       int t = 0;
       for (int dy = 0; dy <= sy; ++dy)
-        for (int dx = 0; dx <= sx; ++dx) t += luma[(size_t)(ly + dy) * ls + lx + dx];
+        for (int dx = 0; dx <= sx; ++dx) t += ld_dev(luma + (size_t)(ly + dy) * ls + lx + dx);
       const int v = t << (3 - sx - sy);
       tmp[s] = v;
       part += v;
@@ -578,7 +860,7 @@ __device__ void store_block(const ReconArgs& a, const DevBlock& b, const uint16_
   const int p = b.plane, lw = ilog2(b.w);
   const int stride = a.lay.stride[p];
   uint16_t* dst = a.frame + a.lay.plane_off[p] + (size_t)b.y * stride + b.x;
-  for (int s = threadIdx.x; s < b.w * b.h; s += 64) dst[(size_t)(s >> lw) * stride + (s & (b.w - 1))] = pred[s];
+  for (int s = threadIdx.x; s < b.w * b.h; s += 64) st_dev(dst + (size_t)(s >> lw) * stride + (s & (b.w - 1)), pred[s]);
 }
 
 __device__ __forceinline__ int64_t cell_index(const Layout& L, int p, int x, int y) {
@@ -625,7 +907,14 @@ __device__ void wait_cells(const ReconArgs& a, const DevBlock& b) {
     lcw = lx1 - lx0 + 1;
     ncfl = lcw * (ly1 - ly0l + 1);
   }
-  const int total = nl + na + nc + ncfl;
+  int nr = 0, rcw = 0;
+  if (b.flags & kIntraBC) {
+    /* intra block copy: every cell of the source rectangle */
+    rcw = b.dep_rx1 - b.dep_rx0 + 1;
+    nr = rcw * (b.dep_ry1 - b.dep_ry0 + 1);
+    if (rcw <= 0 || nr < 0) nr = 0;
+  }
+  const int total = nl + na + nc + ncfl + nr;
   if (total == 0) return;
   for (int iter = 0;; ++iter) {
     int ok = 1;
@@ -637,9 +926,12 @@ __device__ void wait_cells(const ReconArgs& a, const DevBlock& b) {
         idx = cell_index(L, p, ax0 + (i - nl) * cell, b.y - 1);
       else if (i < nl + na + nc)
         idx = cell_index(L, p, b.x - 1, b.y - 1);
-      else {
+      else if (i < nl + na + nc + ncfl) {
         const int k = i - nl - na - nc;
         idx = L.cell_off[0] + (int64_t)(ly0l + k / lcw) * L.cells_w[0] + lx0 + k % lcw;
+      } else {
+        const int k = i - nl - na - nc - ncfl;
+        idx = L.cell_off[p] + (int64_t)(b.dep_ry0 + k / rcw) * L.cells_w[p] + b.dep_rx0 + k % rcw;
       }
       if (__hip_atomic_load(a.cells + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != a.epoch) ok = 0;
     }
@@ -650,14 +942,24 @@ __device__ void wait_cells(const ReconArgs& a, const DevBlock& b) {
     }
     __builtin_amdgcn_s_sleep(1);
   }
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  /* the neighbours' samples are read with device scope loads issued after
+   * the flag loads returned */
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-/* Intra block: the prediction, then the residuals of its transforms, which
+/* A block of the queue: the prediction, then the residuals of its transforms, which
  * k_itx computed before the queue started (av1gpu_kernels.h:933-937 for the
  * add), then one store. */
-__device__ void reconstruct_intra(const ReconArgs& a, const DevBlock& b, int* work, uint16_t* pred) {
-  predict_intra(a, b, work, pred);
+__device__ void reconstruct_queued(const ReconArgs& a, const DevBlock& b, int* work, uint16_t* pred) {
+  if (b.kind == kIntra) {
+    predict_intra(a, b, work, pred);
+  } else {
+    /* inter/intra or intra block copy: reads the frame being decoded */
+    const int lw = ilog2(b.w);
+    for (int s = threadIdx.x; s < b.w * b.h; s += 64) pred[s] = (uint16_t)inter_sample(a, b, s & (b.w - 1), s >> lw);
+    wsync();
+  }
   const int p = b.plane, stride = a.lay.stride[p];
   for (int t = 0; t < b.tx_count; ++t) {
     const DevTx& tx = a.plan.tx[b.tx_first + t];
@@ -673,49 +975,197 @@ __device__ void reconstruct_intra(const ReconArgs& a, const DevBlock& b, int* wo
   store_block(a, b, pred);
 }
 
-/* Motion compensation of every inter block, one thread per output sample:
- * the samples of the inter blocks are numbered block after block, and a
- * workgroup takes 256 consecutive ones. */
-constexpr int kMcChunk = kMcChunkSamples;
+/* Motion compensation through LDS. The inter blocks are cut into items (a
+ * whole block, or a strip of rows of a large one) of at most 256 samples, and
+ * consecutive items in coding order are packed into tasks of at most 256
+ * samples and kMcItems items; one 256 thread workgroup runs one task:
+ *   A  one thread per (item, reference): the block's position arithmetic and
+ *      the reference window its samples read (8 taps around the first and
+ *      last position of each axis);
+ *   B  the windows are loaded into LDS, clamped to the visible reference the
+ *      way each tap is (av1gpu_kernels.h:1704-1711), coalesced along rows;
+ *   C  the horizontal 8 tap pass over every window row, rounded by
+ *      InterRound0, into an LDS intermediate;
+ *   D  one thread per output sample: the vertical pass, InterRound1, the
+ *      compound blend (av1gpu_kernels.h:2207-2293), the store.
+ * With at most 2:1 scaling a window needs at most (2w + 9) x (2h + 9) samples,
+ * so a task of 256 samples never needs more than kMcWin of them. */
+constexpr int kMcPairs = 2 * kMcItems;
+constexpr int kMcWin = 2 * kMcWinPerRef;
+constexpr int kMcMid = 2 * kMcMidPerRef;
 
-__global__ __launch_bounds__(kMcChunk) void k_mc(ReconArgs a) {
+struct McPair {
+  McPos P;
+  const uint16_t* ref;
+  int stride, ref_w, ref_h;
+  int wx0, wy0, ww, wh;
+  int hk, vk; /* filter rows (offsets into the staged subpel table) */
+  int item;
+};
+
+__device__ __forceinline__ int find_prefix(const int* pre, int n, int v) {
+  /* the last k < n with pre[k] <= v */
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int m = (lo + hi + 1) >> 1;
+    if (pre[m] <= v)
+      lo = m;
+    else
+      hi = m - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void k_mc(ReconArgs a) {
   __shared__ int s_filt[768];
-  for (int i = threadIdx.x; i < 768; i += kMcChunk) s_filt[i] = av1t_spec_table[AV1T_OFF_SUBPEL + i];
-  __syncthreads();
-  const int g = blockIdx.x * kMcChunk + threadIdx.x;
-  if (g >= a.plan.mc_total) return;
-  int i = a.plan.mc_chunk[blockIdx.x];
-  while (a.plan.mc_px0[i + 1] <= g) ++i;
-  const DevBlock& b = a.plan.blocks[a.plan.inter_list[i]];
-  const int s = g - a.plan.mc_px0[i];
-  const int w = b.w, lw = ilog2(w), x = s & (w - 1), y = s >> lw;
-  const int p = b.plane;
-  const int sub_x = p > 0 ? a.lay.ss_x : 0, sub_y = p > 0 ? a.lay.ss_y : 0;
-  const int stride = a.lay.stride[p];
+  __shared__ uint16_t s_win[kMcWin];
+  __shared__ int s_mid[kMcMid];
+  __shared__ McPair s_pair[kMcPairs];
+  __shared__ int s_woff[kMcPairs + 1], s_moff[kMcPairs + 1], s_px[kMcItems + 1];
+  __shared__ int s_blk[kMcItems], s_y0[kMcItems], s_rows[kMcItems];
+  const int tid = threadIdx.x;
+  const int it0 = a.plan.mc_task[blockIdx.x], nit = a.plan.mc_task[blockIdx.x + 1] - it0;
+  for (int i = tid; i < 768; i += 256) s_filt[i] = av1t_spec_table[AV1T_OFF_SUBPEL + i];
   const int bd = a.lay.bit_depth;
-  const int compound = b.compound != 0;
   const int round0 = bd >= 12 ? 5 : 3;
+  int wsz = 0, msz = 0;
+  if (tid < 2 * nit) {
+    const int item = tid >> 1, r = tid & 1;
+    const int2 it = a.plan.mc_items[it0 + item];
+    const DevBlock& b = a.plan.blocks[it.x];
+    const int y0 = it.y & 0xffff, rows = it.y >> 16;
+    if (r == 0) {
+      s_blk[item] = it.x;
+      s_y0[item] = y0;
+      s_rows[item] = rows;
+    }
+    McPair& q = s_pair[tid];
+    q.item = item;
+    if (r == 0 || b.compound) {
+      const int p = b.plane;
+      const int sub_x = p > 0 ? a.lay.ss_x : 0, sub_y = p > 0 ? a.lay.ss_y : 0;
+      q.P = mc_pos(b.x, b.y, b.mv[r][0], b.mv[r][1], sub_x, sub_y, b.scale_x, b.scale_y);
+      q.ref = ref_plane(a, b.slot[r], p);
+      q.stride = p == 0 ? a.lay.stride[0] : (p == 1 ? a.lay.stride[1] : a.lay.stride[2]);
+      q.ref_w = b.ref_w;
+      q.ref_h = b.ref_h;
+      int c0, c1, c2, c3, f;
+      q.P.col(0, &c0, &f);
+      q.P.col(b.w - 1, &c1, &f);
+      q.P.row(y0, &c2, &f);
+      q.P.row(y0 + rows - 1, &c3, &f);
+      q.wx0 = c0 - 3;
+      q.wy0 = c2 - 3;
+      q.ww = c1 - c0 + 8;
+      q.wh = c3 - c2 + 8;
+      q.hk = filt_row(b.filt, b.w) * 128;
+      q.vk = filt_row(b.filt_v, b.h) * 128;
+      wsz = q.ww * q.wh;
+      msz = q.wh * b.w;
+    }
+  }
+  /* prefix sums of the window and intermediate sizes (32 pairs), and of the
+   * item samples */
+  if (tid < 64) {
+    int ws = tid < 2 * nit ? wsz : 0, ms = tid < 2 * nit ? msz : 0;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int w2 = __shfl_up(ws, o), m2 = __shfl_up(ms, o);
+      if ((tid & 63) >= o) {
+        ws += w2;
+        ms += m2;
+      }
+    }
+    if (tid < kMcPairs) {
+      s_woff[tid + 1] = ws;
+      s_moff[tid + 1] = ms;
+    }
+    if (tid == 0) {
+      s_woff[0] = 0;
+      s_moff[0] = 0;
+    }
+  }
+  __syncthreads();
+  if (tid < 64) {
+    int n = 0;
+    if (tid < nit) n = s_rows[tid] * a.plan.blocks[s_blk[tid]].w;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int n2 = __shfl_up(n, o);
+      if (tid >= o) n += n2;
+    }
+    if (tid < kMcItems) s_px[tid + 1] = n;
+    if (tid == 0) s_px[0] = 0;
+  }
+  const int npairs = 2 * nit;
+  /* B: the windows */
+  const int wtotal = s_woff[npairs];
+  for (int e = tid; e < wtotal; e += 256) {
+    const int k = find_prefix(s_woff, npairs, e);
+    const McPair& q = s_pair[k];
+    const int o = e - s_woff[k];
+    const int rr = o / q.ww, cc = o - rr * q.ww;
+    const int y = clampi(q.wy0 + rr, 0, q.ref_h - 1), x = clampi(q.wx0 + cc, 0, q.ref_w - 1);
+    s_win[e] = q.ref[(size_t)y * q.stride + x];
+  }
+  __syncthreads();
+  /* C: horizontal pass over every window row */
+  const int mtotal = s_moff[npairs];
+  for (int e = tid; e < mtotal; e += 256) {
+    const int k = find_prefix(s_moff, npairs, e);
+    const McPair& q = s_pair[k];
+    const int w = q.ww, o = e - s_moff[k];
+    const int bw = (s_moff[k + 1] - s_moff[k]) / q.wh;
+    const int rr = o / bw, x = o - rr * bw;
+    int cx, fx;
+    q.P.col(x, &cx, &fx);
+    const uint16_t* row = s_win + s_woff[k] + rr * w + (cx - 3 - q.wx0);
+    const int* hk = s_filt + q.hk + fx * 8;
+    int sum = 0;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) sum += (int)row[t] * hk[t];
+    s_mid[e] = (sum + (1 << (round0 - 1))) >> round0;
+  }
+  __syncthreads();
+  /* D: vertical pass and blend, one thread per output sample */
+  if (tid >= s_px[nit]) return;
+  const int item = find_prefix(s_px, nit, tid);
+  const DevBlock& b = a.plan.blocks[s_blk[item]];
+  const int s = tid - s_px[item];
+  const int lw = ilog2(b.w), x = s & (b.w - 1), y = s_y0[item] + (s >> lw);
+  const int compound = b.compound != 0;
   int round1 = compound ? 7 : 11;
   if (bd >= 12 && !compound) round1 -= 2;
-  const int* hk0 = s_filt + filt_row(b.filt, w) * 128;
-  const int* vk0 = s_filt + filt_row(b.filt_v, b.h) * 128;
-  const McPos P0 = mc_pos(b.x, b.y, b.mv[0][0], b.mv[0][1], sub_x, sub_y, b.scale_x, b.scale_y);
-  int out = mc_sample(ref_plane(a, b.slot[0], p), stride, b.ref_w, b.ref_h, P0, x, y, hk0, vk0, round0, round1);
+  int v[2] = {0, 0};
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    if (r == 1 && !compound) break;
+    const int k = 2 * item + r;
+    const McPair& q = s_pair[k];
+    int cy, fy;
+    q.P.row(y, &cy, &fy);
+    const int* m = s_mid + s_moff[k] + (cy - 3 - q.wy0) * b.w + x;
+    const int* vk = s_filt + q.vk + fy * 8;
+    int acc = 0;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) acc += m[t * b.w] * vk[t];
+    v[r] = (acc + (1 << (round1 - 1))) >> round1;
+  }
+  int out = v[0];
   if (compound) {
-    /* av1gpu_kernels.h:2207-2293 */
-    const McPos P1 = mc_pos(b.x, b.y, b.mv[1][0], b.mv[1][1], sub_x, sub_y, b.scale_x, b.scale_y);
-    const int v1 =
-        mc_sample(ref_plane(a, b.slot[1], p), stride, b.ref_w, b.ref_h, P1, x, y, hk0, vk0, round0, round1);
     const int post = 2 * 7 - (round0 + 7);
     if (b.compound == AV1H_COMPOUND_AVERAGE)
-      out = (out + v1 + (1 << post)) >> (1 + post);
+      out = (v[0] + v[1] + (1 << post)) >> (1 + post);
     else
-      out = (b.w0 * out + b.w1 * v1 + (1 << (3 + post))) >> (4 + post);
+      out = (b.w0 * v[0] + b.w1 * v[1] + (1 << (3 + post))) >> (4 + post);
   }
-  a.frame[a.lay.plane_off[p] + (int64_t)(b.y + y) * stride + b.x + x] = (uint16_t)clampi(out, 0, a.maxv);
-  if (s == 0) {
-    /* the block's dependency cells: final once this launch and the
-     * transforms after it have run, which the queue launch follows */
+  const int p = b.plane;
+  const int stride = p == 0 ? a.lay.stride[0] : (p == 1 ? a.lay.stride[1] : a.lay.stride[2]);
+  const int64_t poff = p == 0 ? a.lay.plane_off[0] : (p == 1 ? a.lay.plane_off[1] : a.lay.plane_off[2]);
+  a.frame[poff + (int64_t)(b.y + y) * stride + b.x + x] = (uint16_t)clampi(out, 0, a.maxv);
+  if (s == 0 && s_y0[item] == 0) {
+    /* the block's dependency cells: final once the inter transforms after
+     * this launch have run, which the queue launch follows */
     const int sh = a.lay.cell_shift[p];
     const int cw = max(b.w >> sh, 1), ch = max(b.h >> sh, 1);
     for (int k = 0; k < cw * ch; ++k) {
@@ -726,6 +1176,18 @@ __global__ __launch_bounds__(kMcChunk) void k_mc(ReconArgs a) {
   }
 }
 
+/* Inter blocks with warp, masks or OBMC (they read only references): one
+ * workgroup per block, the direct per sample computation. */
+__global__ __launch_bounds__(64) void k_inter_special(ReconArgs a) {
+  const int bi = __builtin_amdgcn_readfirstlane(a.plan.special_list[blockIdx.x]);
+  const DevBlock& b = a.plan.blocks[bi];
+  const int p = b.plane, stride = a.lay.stride[p], lw = ilog2(b.w);
+  uint16_t* dst = a.frame + a.lay.plane_off[p] + (size_t)b.y * stride + b.x;
+  for (int s = threadIdx.x; s < b.w * b.h; s += 64)
+    dst[(size_t)(s >> lw) * stride + (s & (b.w - 1))] = (uint16_t)inter_sample(a, b, s & (b.w - 1), s >> lw);
+  mark_cells(a, b, false);
+}
+
 __global__ __launch_bounds__(64) void k_queue(ReconArgs a) {
   __shared__ int s_work[kWork];
   __shared__ uint16_t s_pred[64 * 64];
@@ -734,13 +1196,25 @@ __global__ __launch_bounds__(64) void k_queue(ReconArgs a) {
     if (threadIdx.x == 0) idx = (int)atomicAdd(a.queue_counter, 1u);
     idx = __builtin_amdgcn_readfirstlane(__shfl(idx, 0));
     if (idx >= a.plan.n_queue) return;
+    const uint64_t t0 = a.trace ? (uint64_t)wall_clock64() : 0;
     const int bi = __builtin_amdgcn_readfirstlane(a.plan.queue_list[idx]);
     const DevBlock& b = a.plan.blocks[bi];
     wait_cells(a, b);
-    reconstruct_intra(a, b, s_work, s_pred);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    const uint64_t t1 = a.trace ? (uint64_t)wall_clock64() : 0;
+    reconstruct_queued(a, b, s_work, s_pred);
+    /* every sample store of the block has completed before its cells flip */
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     mark_cells(a, b, true);
     wsync();
+    if (a.trace && threadIdx.x == 0) {
+      /* diagnostics: when the block was taken, found ready, published */
+      uint64_t* t = a.trace + 4 * (size_t)idx;
+      t[0] = t0;
+      t[1] = t1;
+      t[2] = (uint64_t)wall_clock64();
+      t[3] = (uint64_t)blockIdx.x;
+    }
   }
 }
 
@@ -751,33 +1225,43 @@ __global__ void k_zero(uint32_t* p, int n) {
 
 }  // namespace
 
-hipError_t launch_recon(const ReconArgs& a, int part, int cus, hipStream_t s, int* launches) {
-  if (part == 0) {
-    if (a.plan.mc_total > 0) {
-      hipLaunchKernelGGL(k_mc, dim3(a.plan.mc_chunks), dim3(kMcChunk), 0, s, a);
-      ++*launches;
-    }
-    const int per[kTxClasses] = {16, 8, 4, 2, 1};
-    for (int c = 0; c < kTxClasses; ++c) {
-      const int n = a.plan.itx_n[c];
-      if (n <= 0) continue;
-      const dim3 grid((n + per[c] - 1) / per[c]);
-      switch (c) {
-        case 0: hipLaunchKernelGGL(k_itx<4>, grid, dim3(64), 0, s, a, c); break;
-        case 1: hipLaunchKernelGGL(k_itx<8>, grid, dim3(64), 0, s, a, c); break;
-        case 2: hipLaunchKernelGGL(k_itx<16>, grid, dim3(64), 0, s, a, c); break;
-        case 3: hipLaunchKernelGGL(k_itx<32>, grid, dim3(64), 0, s, a, c); break;
-        default: hipLaunchKernelGGL(k_itx<64>, grid, dim3(64), 0, s, a, c); break;
-      }
-      ++*launches;
-    }
+int itx_groups(const FramePlanDev& p, int set) {
+  int g = 0;
+  for (int c = 0; c < kTxClasses; ++c) {
+    const int n = p.itx_off[set][c + 1] - p.itx_off[set][c], per = 16 >> c;
+    g += (n + per - 1) / per;
   }
-  if (part == 1 && a.plan.n_queue > 0) {
-    hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, a.queue_counter, 1);
-    const int grid = min(a.plan.n_queue, cus * 6);
-    hipLaunchKernelGGL(k_queue, dim3(grid), dim3(64), 0, s, a);
-    *launches += 2;
-  }
+  return g;
+}
+
+hipError_t launch_mc(const ReconArgs& a, int cus, hipStream_t s, int* launches) {
+  if (a.plan.mc_tasks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_mc, dim3(a.plan.mc_tasks), dim3(256), 0, s, a);
+  ++*launches;
+  return hipGetLastError();
+}
+
+hipError_t launch_special(const ReconArgs& a, hipStream_t s, int* launches) {
+  if (a.plan.n_special <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_inter_special, dim3(a.plan.n_special), dim3(64), 0, s, a);
+  ++*launches;
+  return hipGetLastError();
+}
+
+hipError_t launch_itx(const ReconArgs& a, int set, hipStream_t s, int* launches) {
+  const int groups = itx_groups(a.plan, set);
+  if (groups <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_itx, dim3(groups), dim3(64), 0, s, a, a.plan.itx_offs + set * (kTxClasses + 1));
+  ++*launches;
+  return hipGetLastError();
+}
+
+hipError_t launch_queue(const ReconArgs& a, int cus, hipStream_t s, int* launches) {
+  if (a.plan.n_queue <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, a.queue_counter, 1);
+  const int grid = min(a.plan.n_queue, cus * 6);
+  hipLaunchKernelGGL(k_queue, dim3(grid), dim3(64), 0, s, a);
+  *launches += 2;
   return hipGetLastError();
 }
 

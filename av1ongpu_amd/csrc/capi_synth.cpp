@@ -57,6 +57,8 @@ int av1s_config_set(void* h, const char* key, double v) {
   else if (k == "lr") c->lr = i;
   else if (k == "grain") c->grain = i != 0;
   else if (k == "contract_tools") c->contract_tools = i != 0;
+  else if (k == "inter_tools") c->inter_tools = v;
+  else if (k == "intrabc") c->intrabc = v;
   else if (k == "seed") c->seed = static_cast<uint64_t>(v);
   else return -1;
   return 0;

@@ -152,6 +152,20 @@ class FrameBuilder {
   }
 
  private:
+  struct InterParams {
+    int ref[2] = {0, -1};
+    int mv[2][2] = {{0, 0}, {0, 0}};
+    int filt = 0, filt_v = 0;
+    int compound = 0;
+    int blend[2] = {8, 8};
+    int wedge_index = 0, mask_sign = 0;
+    bool interintra = false, interintra_wedge = false;
+    int interintra_mode = 0;
+    bool warp = false;
+    int32_t warp_params[6] = {0, 0, 0, 0, 0, 0};
+    bool intrabc = false;
+    std::vector<ObmcCandidate> obmc; /* luma geometry */
+  };
   const Config& cfg_;
   Rng rng_;
   bool key_;
@@ -281,7 +295,9 @@ class FrameBuilder {
     const bool intra = key_ || nrefs_ == 0 || rng_.chance(cfg_.intra_frac);
     const int level = rng_.range(cfg_.deblock_lo, cfg_.deblock_hi);
     const bool lossless = cfg_.contract_tools && intra && w == 8 && h == 8 && rng_.chance(0.05);
-    if (intra) {
+    if (intra && key_ && cfg_.intrabc > 0 && y >= 64 && rng_.chance(cfg_.intrabc)) {
+      intrabc_block(x, y, w, h, level);
+    } else if (intra) {
       const int ymode = rng_.range(0, 12);
       const int yangle = (ymode >= 1 && ymode <= 8) ? rng_.range(-3, 3) : 0;
       intra_block(0, x, y, w, h, ymode, yangle, level, lossless);
@@ -294,10 +310,94 @@ class FrameBuilder {
                     cfl_alpha);
     } else {
       InterParams ip = draw_inter();
+      if (cfg_.inter_tools > 0 && w >= 8 && h >= 8 && w <= 32 && h <= 32 && rng_.chance(cfg_.inter_tools))
+        draw_tool(&ip, w, h);
       for (int p = 0; p < 3; ++p) {
         const int s = p ? 1 : 0;
         inter_block(p, x >> s, y >> s, w >> s, h >> s, ip, level);
       }
+    }
+  }
+
+  /* One of the contract inter tools for a coding block (luma w x h). */
+  void draw_tool(InterParams* ip, int w, int h) {
+    const int tool = rng_.range(0, 4);
+    auto second_ref = [&]() {
+      if (nrefs_ >= 2) {
+        int r1 = rng_.range(0, nrefs_ - 2);
+        if (r1 >= ip->ref[0]) ++r1;
+        ip->ref[1] = r1;
+      } else {
+        ip->ref[1] = ip->ref[0];
+      }
+    };
+    if (tool == 0 || tool == 1) {
+      /* wedge or difference weighted compound */
+      ip->compound = tool == 0 ? 3 : 4;
+      second_ref();
+      ip->wedge_index = rng_.range(0, 15);
+      ip->mask_sign = rng_.range(0, 1);
+    } else if (tool == 2) {
+      /* inter/intra: single reference */
+      ip->compound = 0;
+      ip->ref[1] = -1;
+      ip->interintra = true;
+      ip->interintra_mode = rng_.range(0, 3);
+      ip->interintra_wedge = rng_.chance(0.5);
+      ip->wedge_index = rng_.range(0, 15);
+    } else if (tool == 3) {
+      /* OBMC: single reference; the above neighbours in 8 wide strips of
+       * half the block height, the left ones in 8 tall strips of half its width */
+      ip->compound = 0;
+      ip->ref[1] = -1;
+      for (int rx = 0; rx < w; rx += 8) ip->obmc.push_back(obmc_candidate(0, rx, 0, 8, h / 2));
+      for (int ry = 0; ry < h; ry += 8) ip->obmc.push_back(obmc_candidate(1, 0, ry, w / 2, 8));
+    } else {
+      /* global warp of the first reference, close to the identity */
+      ip->warp = true;
+      int32_t* p = ip->warp_params;
+      p[0] = rng_.range(-8 << 16, 8 << 16);
+      p[1] = rng_.range(-8 << 16, 8 << 16);
+      p[2] = (1 << 16) + rng_.range(-3000, 3000);
+      p[3] = rng_.range(-2000, 2000);
+      p[4] = rng_.range(-2000, 2000);
+      p[5] = (1 << 16) + rng_.range(-3000, 3000);
+      if (ip->compound > 2) ip->compound = 1;
+    }
+  }
+
+  ObmcCandidate obmc_candidate(int pass, int x, int y, int w, int h) {
+    ObmcCandidate c;
+    c.pass = pass;
+    c.x = x;
+    c.y = y;
+    c.w = w;
+    c.h = h;
+    c.mv[0] = draw_mv_component();
+    c.mv[1] = draw_mv_component();
+    if (cfg_.all_filters) {
+      c.filter = static_cast<FilterType>(rng_.range(0, 2));
+      c.filter_v = static_cast<FilterType>(rng_.range(0, 2));
+    }
+    c.ref = rng_.range(0, nrefs_ - 1);
+    return c;
+  }
+
+  /* Intra block copy from the superblock row above (key frames): every plane
+   * copies the co-located block 64 luma rows up, shifted by a whole number of
+   * chroma samples sideways. */
+  void intrabc_block(int x, int y, int w, int h, int level) {
+    const int span = grid_w_ - w;
+    const int sx = (rng_.range(0, span) / 16) * 16;
+    InterParams ip;
+    ip.ref[0] = -1;
+    ip.ref[1] = -1;
+    ip.mv[0][0] = (sx - x) * 8;
+    ip.mv[0][1] = -64 * 8;
+    ip.intrabc = true;
+    for (int p = 0; p < 3; ++p) {
+      const int s = p ? 1 : 0;
+      inter_block(p, x >> s, y >> s, w >> s, h >> s, ip, level);
     }
   }
 
@@ -368,13 +468,7 @@ class FrameBuilder {
     }
   }
 
-  struct InterParams {
-    int ref[2] = {0, -1};
-    int mv[2][2] = {{0, 0}, {0, 0}};
-    int filt = 0, filt_v = 0;
-    int compound = 0;
-    int blend[2] = {8, 8};
-  };
+
 
   int draw_mv_component() {
     if (cfg_.mv_gaussian) {
@@ -440,6 +534,25 @@ class FrameBuilder {
     b.compound = static_cast<CompoundType>(ip.compound);
     b.blend[0] = ip.blend[0];
     b.blend[1] = ip.blend[1];
+    b.wedge_index = ip.wedge_index;
+    b.mask_sign = ip.mask_sign;
+    b.interintra = ip.interintra;
+    b.interintra_mode = ip.interintra_mode;
+    b.interintra_wedge = ip.interintra_wedge;
+    b.intrabc = ip.intrabc;
+    if (ip.warp) {
+      b.warp[0] = true;
+      for (int k = 0; k < 6; ++k) b.warp_params[0][k] = ip.warp_params[k];
+    }
+    const int sh = plane ? 1 : 0;
+    for (const ObmcCandidate& c : ip.obmc) {
+      ObmcCandidate o = c;
+      o.x >>= sh;
+      o.y >>= sh;
+      o.w >>= sh;
+      o.h >>= sh;
+      b.obmc.push_back(o);
+    }
     for (int ty = 0; ty < h; ty += th)
       for (int tx = 0; tx < w; tx += tw) {
         b.transforms.push_back(make_tx(tx, ty, tw, th, false));
@@ -727,6 +840,27 @@ Config workload(const std::string& name, uint64_t seed) {
       c.gop = 4;
       c.mv_sigma = 48.0;
     }
+  } else if (name == "tools_mini") {
+    /* every inter tool of the contract, intra block copy, 10 bit */
+    c.width = 192;
+    c.height = 144;
+    c.bit_depth = 10;
+    c.frames = 4;
+    c.num_refs = 3;
+    c.gop = 4;
+    c.max_tx = 32;
+    c.rect_tx = true;
+    c.tx_ext = true;
+    c.all_filters = true;
+    c.mv_gaussian = true;
+    c.mv_sigma = 48.0;
+    c.compound_frac = 0.3;
+    c.intra_frac = 0.1;
+    c.inter_tools = 0.6;
+    c.intrabc = 0.25;
+    c.lr = 2;
+    c.grain = true;
+    c.contract_tools = true;
   } else if (name == "contract_mini") {
     c.width = 136;
     c.height = 72;

@@ -54,6 +54,10 @@ struct Config {
   bool cdef = true;
   int lr = 1;                /* 0 none, 1 Wiener, 2 switchable Wiener / self guided */
   bool grain = false;
+  double inter_tools = 0.0;  /* inter coding blocks 8x8..32x32 using wedge / difference
+                                weighted compound, inter/intra, OBMC or global warp */
+  double intrabc = 0.0;      /* key frame coding blocks (below the first superblock
+                                row) copied from the superblock row above */
   bool contract_tools = false; /* CFL, filter intra, palette, lossless, OBMC, masks,
                                   inter-intra, global warp, edge filter off        */
   uint64_t seed = 1;

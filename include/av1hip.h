@@ -447,6 +447,16 @@ void av1h_set_stage_timing(av1h_ctx* ctx, int enable);
 int av1h_get_stage_times(const av1h_ctx* ctx, double ms[AV1H_STAGES], int64_t count[AV1H_STAGES],
                          int64_t bytes[AV1H_STAGES]);
 
+/* Diagnostics of the dependency queue: with tracing on, each decode records
+ * for every intra block (in queue order) four u64 values: the device clock
+ * (wall_clock64, 100 MHz on gfx950) when a wave took the block, when its
+ * neighbours were final, when it was published, and the workgroup that ran it.
+ * av1h_get_queue_trace copies the last decode's records (4 per block) and
+ * returns the block count, or that count with nothing copied when `cap` is
+ * too small. No counterpart in the reference (its schedule is on the host). */
+void av1h_set_queue_trace(av1h_ctx* ctx, int enable);
+int64_t av1h_get_queue_trace(av1h_ctx* ctx, uint64_t* out, int64_t cap);
+
 /* --------------------------------------------------------------- batching */
 /*
  * Multi-stream batching: frames of several independent streams (each with its

@@ -37,6 +37,7 @@ CASES = [
     ("c2_mini", 1, {}), ("c2_mini", 2, {}),
     ("c2_mini", 7, {"cdef": 0, "lr": 0, "deblock_lo": 0, "deblock_hi": 0, "grain": 0}),
     ("contract_mini", 1, {}), ("contract_mini", 2, {}),
+    ("tools_mini", 1, {}), ("tools_mini", 2, {}),
 ]
 
 

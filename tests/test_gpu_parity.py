@@ -4,7 +4,8 @@ same seeded syntax, bit exact on every sample of every frame.
 Cases cover the tools of the measured workloads at fixture size: 720p-class
 8-bit single reference (c1_mini), 1080p-class 10-bit full tool with superres,
 film grain, compound and self guided restoration (c2_mini), and the contract
-tools (CFL, filter intra, palette, lossless, edge filter off: contract_mini).
+tools (CFL, filter intra, palette, lossless, edge filter off: contract_mini),
+and the remaining inter tools (tools_mini).
 """
 import numpy as np
 import pytest
@@ -42,6 +43,13 @@ def test_c2_mini(seed):
 @pytest.mark.parametrize("seed", [1, 2])
 def test_contract_tools(seed):
     _compare(pkg.Clip("contract_mini", seed))
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_inter_tools(seed):
+    """Wedge and difference weighted compound, inter/intra (with wedges), OBMC,
+    global warp, and intra block copy in the key frame."""
+    _compare(pkg.Clip("tools_mini", seed))
 
 
 @pytest.mark.parametrize("stage", ["recon", "deblock", "cdef", "lr", "grain"])
