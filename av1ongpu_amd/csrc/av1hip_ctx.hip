@@ -276,6 +276,34 @@ bool setup_shear(const int32_t* p, int* alpha, int* beta, int* gamma, int* delta
   return true;
 }
 
+/* motion compensation positions on the host: the arithmetic of
+ * av1gpu_kernels.h:1761-1763 (unscaled) and 1819-1840 (scaled, 7.11.3.3),
+ * which the kernel then evaluates from these fields */
+int rnd2s_add(int v, int n) { return (v + (1 << (n - 1)) + (v < 0 ? -1 : 0)) >> n; }
+void mc_position(int bx, int by, int mvx, int mvy, int sub_x, int sub_y, int scx, int scy, McRefDev* q) {
+  q->scaled = !(scx == (1 << 14) && scy == (1 << 14));
+  const int mx = mvx * 2, my = mvy * 2;
+  q->ix = mx >> (4 + sub_x);
+  q->fx = (mx >> sub_x) & 15;
+  q->iy = my >> (4 + sub_y);
+  q->fy = (my >> sub_y) & 15;
+  const int ox = (bx << 4) + ((2 * mvx) >> sub_x) + 8;
+  const int oy = (by << 4) + ((2 * mvy) >> sub_y) + 8;
+  q->sx0 = rnd2s_add(ox * scx - (8 << 14), 8) + 32;
+  q->sy0 = rnd2s_add(oy * scy - (8 << 14), 8) + 32;
+  q->stx = rnd2s_add(scx, 4);
+  q->sty = rnd2s_add(scy, 4);
+}
+int mc_col(int bx, const McRefDev& q, int x) { return q.scaled ? (q.sx0 + q.stx * x) >> 10 : bx + x + q.ix; }
+int mc_row(int by, const McRefDev& q, int y) { return q.scaled ? (q.sy0 + q.sty * y) >> 10 : by + y + q.iy; }
+/* the 4 tap rows for extents of 4 or less (av1gpu_kernels.h:1696-1701) */
+int mc_filter_row(int cls, int extent) {
+  if (extent > 4) return cls;
+  if (cls == 0 || cls == 2) return 4;
+  if (cls == 1) return 5;
+  return cls;
+}
+
 /* ---- flattening ----------------------------------------------------------- */
 
 struct Host {
@@ -284,7 +312,8 @@ struct Host {
   std::vector<int32_t> inter_list, queue_list, special_list;
   std::vector<DevObmc> obmc;
   std::vector<int32_t> itx_all;
-  std::vector<int32_t> mc_items, mc_task;
+  std::vector<McItemDev> mc_items;
+  std::vector<McTaskDev> mc_task;
   std::vector<int32_t> db_chunk, cdef_dir_list;
   std::vector<int32_t> lr_tiles; /* pairs */
   std::vector<int32_t> itx[kTxClasses];
@@ -659,34 +688,77 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
   }
 
   /* motion compensation tasks: items (whole blocks, or row strips of blocks
-   * over kMcSamples samples) in coding order, packed into tasks within the
-   * per task budgets of the kernel */
+   * over kMcSamples samples) in coding order, with their positions, windows
+   * and LDS offsets, packed into tasks within the kernel's budgets */
   H->mc_items.clear();
   H->mc_task.clear();
   {
-    int tn = 0, ts = 0, tw = 0, tm = 0;
+    McTaskDev cur{};
+    auto close_task = [&]() {
+      if (cur.count > 0) H->mc_task.push_back(cur);
+      cur = McTaskDev{};
+      cur.first = (int32_t)H->mc_items.size();
+    };
+    close_task();
     for (int32_t bi : H->inter_list) {
       const DevBlock& d = H->blocks[bi];
+      const int sx = d.plane ? hdr.ss_x : 0, sy = d.plane ? hdr.ss_y : 0;
       const int per = d.w * d.h <= kMcSamples ? d.h : kMcSamples / d.w;
       for (int y0 = 0; y0 < d.h; y0 += per) {
-        const int rows = std::min(per, d.h - y0);
-        const int sb = rows * d.w, wb = (2 * d.w + 9) * (2 * rows + 9), mb = (2 * rows + 9) * d.w;
-        if (H->mc_task.empty() || tn + 1 > kMcItems || ts + sb > kMcSamples || tw + wb > kMcWinPerRef ||
-            tm + mb > kMcMidPerRef) {
-          H->mc_task.push_back((int32_t)(H->mc_items.size() / 2));
-          tn = ts = tw = tm = 0;
+        McItemDev it;
+        std::memset(&it, 0, sizeof(it));
+        it.x = d.x;
+        it.y = d.y;
+        it.w = d.w;
+        it.h = d.h;
+        it.plane = d.plane;
+        it.y0 = y0;
+        it.rows = std::min(per, d.h - y0);
+        it.compound = d.compound;
+        it.w0 = d.w0;
+        it.w1 = d.w1;
+        it.hk = mc_filter_row(d.filt, d.w) * 128;
+        it.vk = mc_filter_row(d.filt_v, d.h) * 128;
+        it.ref_w = d.ref_w;
+        it.ref_h = d.ref_h;
+        it.nref = d.compound ? 2 : 1;
+        int wsz = 0, msz = 0;
+        for (int r = 0; r < it.nref; ++r) {
+          McRefDev& q = it.r[r];
+          q.slot = d.slot[r];
+          mc_position(d.x, d.y, d.mv[r][0], d.mv[r][1], sx, sy, d.scale_x, d.scale_y, &q);
+          const int c0 = mc_col(d.x, q, 0), c1 = mc_col(d.x, q, d.w - 1);
+          const int c2 = mc_row(d.y, q, y0), c3 = mc_row(d.y, q, y0 + it.rows - 1);
+          q.wx0 = c0 - 3;
+          q.wy0 = c2 - 3;
+          q.ww = c1 - c0 + 8;
+          q.wh = c3 - c2 + 8;
+          wsz += q.ww * q.wh;
+          msz += q.wh * d.w;
         }
-        H->mc_items.push_back(bi);
-        H->mc_items.push_back(y0 | (rows << 16));
-        ++tn;
-        ts += sb;
-        tw += wb;
-        tm += mb;
+        const int sb = it.rows * d.w;
+        if (cur.count + 1 > kMcItems || cur.samples + sb > kMcSamples || cur.wtotal + wsz > kMcWin ||
+            cur.mtotal + msz > kMcMid)
+          close_task();
+        if (wsz > kMcWin || msz > kMcMid) return set_err(c, AV1H_ERR_UNSUPPORTED, "motion compensation window");
+        it.px0 = cur.samples;
+        for (int r = 0; r < 2; ++r) {
+          McRefDev& q = it.r[r];
+          q.win_off = cur.wtotal;
+          q.mid_off = cur.mtotal;
+          if (r < it.nref) {
+            cur.wtotal += q.ww * q.wh;
+            cur.mtotal += q.wh * d.w;
+          }
+        }
+        cur.samples += sb;
+        ++cur.count;
+        H->mc_items.push_back(it);
       }
     }
-    H->mc_task.push_back((int32_t)(H->mc_items.size() / 2));
+    close_task();
   }
-  const int32_t mc_tasks = (int32_t)H->mc_task.size() - 1;
+  const int32_t mc_tasks = (int32_t)H->mc_task.size();
   /* transforms: inter set then intra set, each by class (larger side), each
    * class sorted by size so a wave's transforms mostly share one */
   std::vector<int32_t> itx_sets[2][kTxClasses];
@@ -968,8 +1040,8 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
   const size_t p_db = add(H->deblock.data(), H->deblock.size() * sizeof(DevDeblock));
   const size_t p_cdef = add(H->cdef.data(), H->cdef.size() * sizeof(DevCdef));
   const size_t p_lr = add(H->lr.data(), H->lr.size() * sizeof(DevLr));
-  const size_t p_mcit = add(H->mc_items.data(), H->mc_items.size() * 4);
-  const size_t p_mcta = add(H->mc_task.data(), H->mc_task.size() * 4);
+  const size_t p_mcit = add(H->mc_items.data(), H->mc_items.size() * sizeof(McItemDev));
+  const size_t p_mcta = add(H->mc_task.data(), H->mc_task.size() * sizeof(McTaskDev));
   const size_t p_itx = add(H->itx_all.data(), H->itx_all.size() * 4);
   const size_t p_itxo = add(itx_off, sizeof(itx_off));
   const size_t p_dbc = add(H->db_chunk.data(), H->db_chunk.size() * 4);
@@ -1011,8 +1083,8 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
   P.itx_list = reinterpret_cast<const int32_t*>(at(p_itx));
   std::memcpy(P.itx_off, itx_off, sizeof(itx_off));
   P.itx_offs = reinterpret_cast<const int32_t*>(at(p_itxo));
-  P.mc_items = reinterpret_cast<const int2*>(at(p_mcit));
-  P.mc_task = reinterpret_cast<const int32_t*>(at(p_mcta));
+  P.mc_items = reinterpret_cast<const McItemDev*>(at(p_mcit));
+  P.mc_task = reinterpret_cast<const McTaskDev*>(at(p_mcta));
   P.mc_tasks = mc_tasks;
   P.db_chunk = reinterpret_cast<const int32_t*>(at(p_dbc));
   P.cdef_dir_list = reinterpret_cast<const int32_t*>(at(p_cdir));

@@ -87,13 +87,34 @@ struct DevTx {
 
 /* transform classes: the larger side, 4 8 16 32 64 */
 constexpr int kTxClasses = 5;
-/* motion compensation tasks: at most 256 samples and kMcItems items, and
- * within the LDS budget of the window bounds (2w + 9) x (2h + 9) and of the
- * intermediate bounds (2h + 9) x w, per reference */
+/* Motion compensation tasks: at most kMcSamples samples and kMcItems items
+ * (blocks, or row strips of large blocks) whose reference windows fit
+ * kMcWin samples and whose horizontal pass fits kMcMid values. The host
+ * computes each item's positions and windows (the position arithmetic of the
+ * reference, av1gpu_kernels.h:1761-1763 and 1819-1840) and its place in the
+ * task's LDS, so a workgroup reads one descriptor array and nothing else. */
 constexpr int kMcSamples = 256;
 constexpr int kMcItems = 16;
-constexpr int kMcWinPerRef = kMcItems * 17 * 17;
-constexpr int kMcMidPerRef = kMcItems * 17 * 4;
+constexpr int kMcWin = 6144;
+constexpr int kMcMid = 3072;
+
+struct McRefDev {
+  int32_t slot;
+  int32_t scaled, ix, iy, fx, fy, sx0, sy0, stx, sty; /* positions */
+  int32_t wx0, wy0, ww, wh;                           /* reference window */
+  int32_t win_off, mid_off;                           /* in the task's LDS */
+};
+
+struct McItemDev {
+  int32_t x, y, w, h, plane, y0, rows, px0; /* block, strip, first sample in the task */
+  int32_t compound, w0, w1, hk, vk, ref_w, ref_h, nref;
+  McRefDev r[2];
+};
+
+struct McTaskDev {
+  int32_t first, count, wtotal, mtotal, samples, pad[3];
+};
+
 constexpr int kLrTileCols = 64, kLrTileRows = 16; /* loop restoration tile per workgroup */
 
 struct DevObmc {
@@ -146,11 +167,9 @@ struct FramePlanDev {
   const DevCdef* cdef;
   const DevLr* lr;
   const DevGrain* grain;
-  /* motion compensation: items (block index, first row | rows << 16) in
-   * coding order, grouped into tasks; task t is items [mc_task[t],
-   * mc_task[t + 1]) */
-  const int2* mc_items;
-  const int32_t* mc_task;
+  /* motion compensation: item descriptors in coding order, grouped into tasks */
+  const McItemDev* mc_items;
+  const McTaskDev* mc_task;
   int32_t mc_tasks;
   /* transform blocks: set 0 those of inter blocks, set 1 those of intra
    * blocks; inside a set by class (larger side 4 8 16 32 64), inside a class

@@ -975,33 +975,19 @@ __device__ void reconstruct_queued(const ReconArgs& a, const DevBlock& b, int* w
   store_block(a, b, pred);
 }
 
-/* Motion compensation through LDS. The inter blocks are cut into items (a
- * whole block, or a strip of rows of a large one) of at most 256 samples, and
- * consecutive items in coding order are packed into tasks of at most 256
- * samples and kMcItems items; one 256 thread workgroup runs one task:
- *   A  one thread per (item, reference): the block's position arithmetic and
- *      the reference window its samples read (8 taps around the first and
- *      last position of each axis);
- *   B  the windows are loaded into LDS, clamped to the visible reference the
- *      way each tap is (av1gpu_kernels.h:1704-1711), coalesced along rows;
+/* Motion compensation through LDS, one 256 thread workgroup per task (at
+ * most 256 samples of consecutive items in coding order, an item being a
+ * block or a row strip of a large one):
+ *   A  the task's item descriptors (positions, reference windows, LDS
+ *      offsets, all computed by the host) are copied into LDS;
+ *   B  the reference windows are loaded into LDS, clamped to the visible
+ *      reference the way each tap is (av1gpu_kernels.h:1704-1711), with the
+ *      loads of a thread issued in batches;
  *   C  the horizontal 8 tap pass over every window row, rounded by
- *      InterRound0, into an LDS intermediate;
+ *      InterRound0, into an LDS intermediate (16 bit: at most 200 * 4095 >> 5);
  *   D  one thread per output sample: the vertical pass, InterRound1, the
- *      compound blend (av1gpu_kernels.h:2207-2293), the store.
- * With at most 2:1 scaling a window needs at most (2w + 9) x (2h + 9) samples,
- * so a task of 256 samples never needs more than kMcWin of them. */
+ *      compound blend (av1gpu_kernels.h:2207-2293), the store. */
 constexpr int kMcPairs = 2 * kMcItems;
-constexpr int kMcWin = 2 * kMcWinPerRef;
-constexpr int kMcMid = 2 * kMcMidPerRef;
-
-struct McPair {
-  McPos P;
-  const uint16_t* ref;
-  int stride, ref_w, ref_h;
-  int wx0, wy0, ww, wh;
-  int hk, vk; /* filter rows (offsets into the staged subpel table) */
-  int item;
-};
 
 __device__ __forceinline__ int find_prefix(const int* pre, int n, int v) {
   /* the last k < n with pre[k] <= v */
@@ -1016,160 +1002,140 @@ __device__ __forceinline__ int find_prefix(const int* pre, int n, int v) {
   return lo;
 }
 
+__device__ __forceinline__ McPos mc_pos_of(const McItemDev& it, const McRefDev& r, int sub_x, int sub_y) {
+  McPos P;
+  P.scaled = r.scaled;
+  P.bx = it.x;
+  P.by = it.y;
+  P.sub_x = sub_x;
+  P.sub_y = sub_y;
+  P.ix = r.ix;
+  P.iy = r.iy;
+  P.fx = r.fx;
+  P.fy = r.fy;
+  P.sx0 = r.sx0;
+  P.sy0 = r.sy0;
+  P.stx = r.stx;
+  P.sty = r.sty;
+  return P;
+}
+
 __global__ __launch_bounds__(256) void k_mc(ReconArgs a) {
   __shared__ int s_filt[768];
   __shared__ uint16_t s_win[kMcWin];
-  __shared__ int s_mid[kMcMid];
-  __shared__ McPair s_pair[kMcPairs];
+  __shared__ int16_t s_mid[kMcMid];
+  __shared__ McItemDev s_it[kMcItems];
   __shared__ int s_woff[kMcPairs + 1], s_moff[kMcPairs + 1], s_px[kMcItems + 1];
-  __shared__ int s_blk[kMcItems], s_y0[kMcItems], s_rows[kMcItems];
   const int tid = threadIdx.x;
-  const int it0 = a.plan.mc_task[blockIdx.x], nit = a.plan.mc_task[blockIdx.x + 1] - it0;
+  const McTaskDev task = a.plan.mc_task[blockIdx.x];
+  const int nit = task.count, npairs = 2 * nit;
   for (int i = tid; i < 768; i += 256) s_filt[i] = av1t_spec_table[AV1T_OFF_SUBPEL + i];
-  const int bd = a.lay.bit_depth;
-  const int round0 = bd >= 12 ? 5 : 3;
-  int wsz = 0, msz = 0;
-  if (tid < 2 * nit) {
-    const int item = tid >> 1, r = tid & 1;
-    const int2 it = a.plan.mc_items[it0 + item];
-    const DevBlock& b = a.plan.blocks[it.x];
-    const int y0 = it.y & 0xffff, rows = it.y >> 16;
-    if (r == 0) {
-      s_blk[item] = it.x;
-      s_y0[item] = y0;
-      s_rows[item] = rows;
-    }
-    McPair& q = s_pair[tid];
-    q.item = item;
-    if (r == 0 || b.compound) {
-      const int p = b.plane;
-      const int sub_x = p > 0 ? a.lay.ss_x : 0, sub_y = p > 0 ? a.lay.ss_y : 0;
-      q.P = mc_pos(b.x, b.y, b.mv[r][0], b.mv[r][1], sub_x, sub_y, b.scale_x, b.scale_y);
-      q.ref = ref_plane(a, b.slot[r], p);
-      q.stride = p == 0 ? a.lay.stride[0] : (p == 1 ? a.lay.stride[1] : a.lay.stride[2]);
-      q.ref_w = b.ref_w;
-      q.ref_h = b.ref_h;
-      int c0, c1, c2, c3, f;
-      q.P.col(0, &c0, &f);
-      q.P.col(b.w - 1, &c1, &f);
-      q.P.row(y0, &c2, &f);
-      q.P.row(y0 + rows - 1, &c3, &f);
-      q.wx0 = c0 - 3;
-      q.wy0 = c2 - 3;
-      q.ww = c1 - c0 + 8;
-      q.wh = c3 - c2 + 8;
-      q.hk = filt_row(b.filt, b.w) * 128;
-      q.vk = filt_row(b.filt_v, b.h) * 128;
-      wsz = q.ww * q.wh;
-      msz = q.wh * b.w;
-    }
-  }
-  /* prefix sums of the window and intermediate sizes (32 pairs), and of the
-   * item samples */
-  if (tid < 64) {
-    int ws = tid < 2 * nit ? wsz : 0, ms = tid < 2 * nit ? msz : 0;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const int w2 = __shfl_up(ws, o), m2 = __shfl_up(ms, o);
-      if ((tid & 63) >= o) {
-        ws += w2;
-        ms += m2;
-      }
-    }
-    if (tid < kMcPairs) {
-      s_woff[tid + 1] = ws;
-      s_moff[tid + 1] = ms;
-    }
-    if (tid == 0) {
-      s_woff[0] = 0;
-      s_moff[0] = 0;
-    }
+  {
+    const int32_t* src = reinterpret_cast<const int32_t*>(a.plan.mc_items + task.first);
+    int32_t* dst = reinterpret_cast<int32_t*>(s_it);
+    const int n = nit * (int)(sizeof(McItemDev) / 4);
+    for (int i = tid; i < n; i += 256) dst[i] = src[i];
   }
   __syncthreads();
-  if (tid < 64) {
-    int n = 0;
-    if (tid < nit) n = s_rows[tid] * a.plan.blocks[s_blk[tid]].w;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const int n2 = __shfl_up(n, o);
-      if (tid >= o) n += n2;
-    }
-    if (tid < kMcItems) s_px[tid + 1] = n;
-    if (tid == 0) s_px[0] = 0;
+  if (tid < npairs) {
+    s_woff[tid] = s_it[tid >> 1].r[tid & 1].win_off;
+    s_moff[tid] = s_it[tid >> 1].r[tid & 1].mid_off;
   }
-  const int npairs = 2 * nit;
+  if (tid < nit) s_px[tid] = s_it[tid].px0;
+  if (tid == 0) {
+    s_woff[npairs] = task.wtotal;
+    s_moff[npairs] = task.mtotal;
+    s_px[nit] = task.samples;
+  }
+  __syncthreads();
+  const int bd = a.lay.bit_depth;
+  const int round0 = bd >= 12 ? 5 : 3;
   /* B: the windows */
-  const int wtotal = s_woff[npairs];
-  for (int e = tid; e < wtotal; e += 256) {
-    const int k = find_prefix(s_woff, npairs, e);
-    const McPair& q = s_pair[k];
-    const int o = e - s_woff[k];
-    const int rr = o / q.ww, cc = o - rr * q.ww;
-    const int y = clampi(q.wy0 + rr, 0, q.ref_h - 1), x = clampi(q.wx0 + cc, 0, q.ref_w - 1);
-    s_win[e] = q.ref[(size_t)y * q.stride + x];
+  constexpr int kBatch = 8;
+  for (int e0 = tid; e0 < task.wtotal; e0 += 256 * kBatch) {
+    uint16_t v[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const int e = e0 + u * 256;
+      if (e < task.wtotal) {
+        const int k = find_prefix(s_woff, npairs, e);
+        const McItemDev& it = s_it[k >> 1];
+        const McRefDev& q = it.r[k & 1];
+        const int o = e - q.win_off;
+        const int rr = o / q.ww, cc = o - rr * q.ww;
+        const int y = clampi(q.wy0 + rr, 0, it.ref_h - 1), x = clampi(q.wx0 + cc, 0, it.ref_w - 1);
+        const uint16_t* ref = a.pool + (size_t)q.slot * a.slot_size + a.lay.plane_off[it.plane];
+        v[u] = ref[(size_t)y * a.lay.stride[it.plane] + x];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u)
+      if (e0 + u * 256 < task.wtotal) s_win[e0 + u * 256] = v[u];
   }
   __syncthreads();
   /* C: horizontal pass over every window row */
-  const int mtotal = s_moff[npairs];
-  for (int e = tid; e < mtotal; e += 256) {
+  for (int e = tid; e < task.mtotal; e += 256) {
     const int k = find_prefix(s_moff, npairs, e);
-    const McPair& q = s_pair[k];
-    const int w = q.ww, o = e - s_moff[k];
-    const int bw = (s_moff[k + 1] - s_moff[k]) / q.wh;
-    const int rr = o / bw, x = o - rr * bw;
+    const McItemDev& it = s_it[k >> 1];
+    const McRefDev& q = it.r[k & 1];
+    const int o = e - q.mid_off;
+    const int rr = o / it.w, x = o - rr * it.w;
+    const int sub_x = it.plane > 0 ? a.lay.ss_x : 0;
+    const McPos P = mc_pos_of(it, q, sub_x, 0);
     int cx, fx;
-    q.P.col(x, &cx, &fx);
-    const uint16_t* row = s_win + s_woff[k] + rr * w + (cx - 3 - q.wx0);
-    const int* hk = s_filt + q.hk + fx * 8;
+    P.col(x, &cx, &fx);
+    const uint16_t* row = s_win + q.win_off + rr * q.ww + (cx - 3 - q.wx0);
+    const int* hk = s_filt + it.hk + fx * 8;
     int sum = 0;
 #pragma unroll
     for (int t = 0; t < 8; ++t) sum += (int)row[t] * hk[t];
-    s_mid[e] = (sum + (1 << (round0 - 1))) >> round0;
+    s_mid[e] = (int16_t)((sum + (1 << (round0 - 1))) >> round0);
   }
   __syncthreads();
   /* D: vertical pass and blend, one thread per output sample */
-  if (tid >= s_px[nit]) return;
+  if (tid >= task.samples) return;
   const int item = find_prefix(s_px, nit, tid);
-  const DevBlock& b = a.plan.blocks[s_blk[item]];
-  const int s = tid - s_px[item];
-  const int lw = ilog2(b.w), x = s & (b.w - 1), y = s_y0[item] + (s >> lw);
-  const int compound = b.compound != 0;
+  const McItemDev& it = s_it[item];
+  const int s = tid - it.px0;
+  const int lw = ilog2(it.w), x = s & (it.w - 1), y = it.y0 + (s >> lw);
+  const int compound = it.compound != 0;
   int round1 = compound ? 7 : 11;
   if (bd >= 12 && !compound) round1 -= 2;
+  const int sub_y = it.plane > 0 ? a.lay.ss_y : 0;
   int v[2] = {0, 0};
 #pragma unroll
   for (int r = 0; r < 2; ++r) {
     if (r == 1 && !compound) break;
-    const int k = 2 * item + r;
-    const McPair& q = s_pair[k];
+    const McRefDev& q = it.r[r];
+    const McPos P = mc_pos_of(it, q, 0, sub_y);
     int cy, fy;
-    q.P.row(y, &cy, &fy);
-    const int* m = s_mid + s_moff[k] + (cy - 3 - q.wy0) * b.w + x;
-    const int* vk = s_filt + q.vk + fy * 8;
+    P.row(y, &cy, &fy);
+    const int16_t* m = s_mid + q.mid_off + (cy - 3 - q.wy0) * it.w + x;
+    const int* vk = s_filt + it.vk + fy * 8;
     int acc = 0;
 #pragma unroll
-    for (int t = 0; t < 8; ++t) acc += m[t * b.w] * vk[t];
+    for (int t = 0; t < 8; ++t) acc += (int)m[t * it.w] * vk[t];
     v[r] = (acc + (1 << (round1 - 1))) >> round1;
   }
   int out = v[0];
   if (compound) {
     const int post = 2 * 7 - (round0 + 7);
-    if (b.compound == AV1H_COMPOUND_AVERAGE)
+    if (it.compound == AV1H_COMPOUND_AVERAGE)
       out = (v[0] + v[1] + (1 << post)) >> (1 + post);
     else
-      out = (b.w0 * v[0] + b.w1 * v[1] + (1 << (3 + post))) >> (4 + post);
+      out = (it.w0 * v[0] + it.w1 * v[1] + (1 << (3 + post))) >> (4 + post);
   }
-  const int p = b.plane;
+  const int p = it.plane;
   const int stride = p == 0 ? a.lay.stride[0] : (p == 1 ? a.lay.stride[1] : a.lay.stride[2]);
   const int64_t poff = p == 0 ? a.lay.plane_off[0] : (p == 1 ? a.lay.plane_off[1] : a.lay.plane_off[2]);
-  a.frame[poff + (int64_t)(b.y + y) * stride + b.x + x] = (uint16_t)clampi(out, 0, a.maxv);
-  if (s == 0 && s_y0[item] == 0) {
+  a.frame[poff + (int64_t)(it.y + y) * stride + it.x + x] = (uint16_t)clampi(out, 0, a.maxv);
+  if (s == 0 && it.y0 == 0) {
     /* the block's dependency cells: final once the inter transforms after
      * this launch have run, which the queue launch follows */
     const int sh = a.lay.cell_shift[p];
-    const int cw = max(b.w >> sh, 1), ch = max(b.h >> sh, 1);
+    const int cw = max(it.w >> sh, 1), ch = max(it.h >> sh, 1);
     for (int k = 0; k < cw * ch; ++k) {
-      const int cx = (b.x >> sh) + k % cw, cy = (b.y >> sh) + k / cw;
+      const int cx = (it.x >> sh) + k % cw, cy = (it.y >> sh) + k / cw;
       if (cx < a.lay.cells_w[p] && cy < a.lay.cells_h[p])
         a.cells[a.lay.cell_off[p] + (int64_t)cy * a.lay.cells_w[p] + cx] = a.epoch;
     }
