@@ -216,6 +216,9 @@ def hip_lib():
         lib.av1h_get_stats.argtypes = [P, ctypes.POINTER(Stats)]
         lib.av1h_decode_batch.argtypes = [ctypes.POINTER(P), ctypes.POINTER(P), ctypes.c_int,
                                           ctypes.POINTER(ctypes.c_int)]
+        lib.av1h_decode_planned_batch.argtypes = [ctypes.POINTER(P), ctypes.POINTER(P), ctypes.POINTER(ctypes.c_int),
+                                                  ctypes.POINTER(ctypes.c_int), ctypes.c_int,
+                                                  ctypes.POINTER(ctypes.c_int)]
         lib.av1h_plan_frame_wire.argtypes = [P, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(P)]
         lib.av1h_decode_planned.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Planes),
                                             ctypes.POINTER(FrameInfo), ctypes.POINTER(ctypes.c_int)]
@@ -226,10 +229,157 @@ def hip_lib():
         lib.av1h_set_queue_trace.argtypes = [P, ctypes.c_int]
         lib.av1h_get_queue_trace.restype = ctypes.c_int64
         lib.av1h_get_queue_trace.argtypes = [P, ctypes.c_void_p, ctypes.c_int64]
+        lib.av1h_set_decode_nv12.argtypes = [P, ctypes.c_size_t]
         lib.av1h_nv12_device_buffer.restype = ctypes.c_void_p
         lib.av1h_nv12_device_buffer.argtypes = [P]
+        _bind_server(lib)
+        _bind_costmodel(lib)
         _hip = lib
     return _hip
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("max_batch", ctypes.c_int32), ("planner_threads", ctypes.c_int32),
+                ("capture", ctypes.c_int32), ("hbm_budget", ctypes.c_int64), ("nv12_pitch", ctypes.c_size_t)]
+
+
+class Completion(ctypes.Structure):
+    _fields_ = [("tag", ctypes.c_uint64), ("stream", ctypes.c_int32), ("slot", ctypes.c_int32),
+                ("status", ctypes.c_int32), ("batch_frames", ctypes.c_int32), ("arrival_ms", ctypes.c_double),
+                ("start_ms", ctypes.c_double), ("done_ms", ctypes.c_double),
+                ("picture", ctypes.POINTER(ctypes.c_uint16)), ("picture_samples", ctypes.c_int64)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("frames", ctypes.c_int64), ("batches", ctypes.c_int64), ("admitted", ctypes.c_int64),
+                ("waiting", ctypes.c_int64), ("active", ctypes.c_int64), ("budget_used", ctypes.c_int64),
+                ("busy_ms", ctypes.c_double), ("max_batch_frames", ctypes.c_int32)]
+
+
+def _bind_server(lib):
+    P = ctypes.c_void_p
+    lib.av1h_server_now_ms.restype = ctypes.c_double
+    lib.av1h_server_now_ms.argtypes = []
+    lib.av1h_server_create.argtypes = [ctypes.POINTER(ServerConfig), ctypes.POINTER(P)]
+    lib.av1h_server_destroy.argtypes = [P]
+    lib.av1h_server_open_stream.argtypes = [P] + [ctypes.c_int] * 6
+    lib.av1h_server_submit_wire.argtypes = [P, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t, ctypes.c_int,
+                                            ctypes.POINTER(ctypes.c_int32), ctypes.c_int, ctypes.c_double,
+                                            ctypes.c_uint64]
+    lib.av1h_server_submit_plan.argtypes = [P, ctypes.c_int, P, ctypes.c_int, ctypes.POINTER(ctypes.c_int32),
+                                            ctypes.c_int, ctypes.c_double, ctypes.c_uint64]
+    lib.av1h_server_close_stream.argtypes = [P, ctypes.c_int]
+    lib.av1h_server_poll.argtypes = [P, ctypes.POINTER(Completion), ctypes.c_int, ctypes.c_int]
+    lib.av1h_server_free_picture.argtypes = [ctypes.POINTER(ctypes.c_uint16)]
+    lib.av1h_server_get_stats.argtypes = [P, ctypes.POINTER(ServerStats)]
+    lib.av1h_server_last_error.restype = ctypes.c_char_p
+    lib.av1h_server_last_error.argtypes = [P]
+    lib.av1h_stream_bytes.restype = ctypes.c_int64
+    lib.av1h_stream_bytes.argtypes = [ctypes.c_int] * 6 + [ctypes.c_size_t]
+    lib.av1h_set_pool_slots.argtypes = [P, ctypes.c_int]
+
+
+def _bind_costmodel(lib):
+    P = ctypes.c_void_p
+    F = ctypes.POINTER(ctypes.c_float)
+    lib.av1h_plan_sb_features.argtypes = [P, F]
+    lib.av1h_plan_sb_times.argtypes = [P, P, F]
+    lib.av1h_plan_schedule.argtypes = [P, P, F]
+    lib.av1h_plan_schedule_model.argtypes = [P, P, P]
+    lib.av1h_cm_create.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(P)]
+    lib.av1h_cm_destroy.argtypes = [P]
+    lib.av1h_cm_set_params.argtypes = [P, F]
+    lib.av1h_cm_get_params.argtypes = [P, F]
+    lib.av1h_cm_set_norm.argtypes = [P, F, F, ctypes.c_float, ctypes.c_float]
+    lib.av1h_cm_get_norm.argtypes = [P, F, F, F, F]
+    lib.av1h_cm_predict.argtypes = [P, F, ctypes.c_int64, F]
+    lib.av1h_cm_gradients.argtypes = [P, F, F, ctypes.c_int64, F, F]
+    lib.av1h_cm_train.argtypes = [P, F, F, ctypes.c_int64, ctypes.c_int, ctypes.c_float, ctypes.c_uint64,
+                                  ctypes.c_int, F]
+
+
+def _fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+CM_FEATURES = 24
+CM_PARAMS = 5825
+
+
+class CostModel:
+    """The superblock cost model (av1h_cm_*): a 24-64-64-1 FP32 perceptron
+    trained and evaluated by HIP kernels."""
+
+    def __init__(self, device=0, seed=1):
+        lib = hip_lib()
+        self._lib = lib
+        h = ctypes.c_void_p()
+        rc = lib.av1h_cm_create(device, seed, ctypes.byref(h))
+        if rc != AV1H_OK:
+            raise HipError(f"av1h_cm_create failed ({rc})")
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def params(self):
+        out = np.zeros(CM_PARAMS, np.float32)
+        self._lib.av1h_cm_get_params(self._h, _fptr(out))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == CM_PARAMS
+        if self._lib.av1h_cm_set_params(self._h, _fptr(p)) != AV1H_OK:
+            raise HipError("set_params failed")
+
+    def norm(self):
+        mean = np.zeros(CM_FEATURES, np.float32)
+        dev = np.zeros(CM_FEATURES, np.float32)
+        ym, yd = ctypes.c_float(), ctypes.c_float()
+        self._lib.av1h_cm_get_norm(self._h, _fptr(mean), _fptr(dev), ctypes.byref(ym), ctypes.byref(yd))
+        return mean, dev, ym.value, yd.value
+
+    def set_norm(self, mean, dev, y_mean, y_dev):
+        mean = np.ascontiguousarray(mean, np.float32)
+        dev = np.ascontiguousarray(dev, np.float32)
+        if self._lib.av1h_cm_set_norm(self._h, _fptr(mean), _fptr(dev), float(y_mean), float(y_dev)) != AV1H_OK:
+            raise HipError("set_norm failed")
+
+    def predict(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        out = np.zeros(X.shape[0], np.float32)
+        if self._lib.av1h_cm_predict(self._h, _fptr(X), X.shape[0], _fptr(out)) != AV1H_OK:
+            raise HipError("predict failed")
+        return out
+
+    def gradients(self, X, y):
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        g = np.zeros(CM_PARAMS, np.float32)
+        loss = ctypes.c_float()
+        if self._lib.av1h_cm_gradients(self._h, _fptr(X), _fptr(y), X.shape[0], _fptr(g),
+                                       ctypes.byref(loss)) != AV1H_OK:
+            raise HipError("gradients failed")
+        return g, loss.value
+
+    def train(self, X, y, epochs=10, lr=1e-3, seed=1, set_norm=True):
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        losses = np.zeros(epochs, np.float32)
+        if self._lib.av1h_cm_train(self._h, _fptr(X), _fptr(y), X.shape[0], epochs, lr, seed,
+                                   1 if set_norm else 0, _fptr(losses)) != AV1H_OK:
+            raise HipError("train failed")
+        return losses
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.av1h_cm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
 
 
 STAGES = ("inter", "queue", "deblock", "cdef", "superres", "lr", "store", "grain")
@@ -324,6 +474,40 @@ class HipDecoder:
         if rc != AV1H_OK:
             raise HipError(f"to_nv12 failed ({rc}): {self.error()}")
 
+    def sb_features(self, plan):
+        """(superblocks x 24) features of a plan."""
+        n = self._lib.av1h_plan_sb_features(plan, None)
+        out = np.zeros((max(n, 0), CM_FEATURES), np.float32)
+        if n > 0:
+            self._lib.av1h_plan_sb_features(plan, _fptr(out))
+        return out
+
+    def sb_times(self, plan):
+        """Queue work per superblock (us) of the last traced decode of `plan` alone."""
+        n = self._lib.av1h_plan_sb_features(plan, None)
+        out = np.zeros(max(n, 0), np.float32)
+        rc = self._lib.av1h_plan_sb_times(self._h, plan, _fptr(out))
+        if rc < 0:
+            raise HipError(f"sb_times failed ({rc}): {self.error()}")
+        return out
+
+    def schedule(self, plan, costs=None):
+        """Order the plan's queue by per-superblock costs (None: static)."""
+        c = None if costs is None else np.ascontiguousarray(costs, np.float32)
+        rc = self._lib.av1h_plan_schedule(self._h, plan, None if c is None else _fptr(c))
+        if rc != AV1H_OK:
+            raise HipError(f"schedule failed ({rc}): {self.error()}")
+
+    def schedule_model(self, plan, model):
+        rc = self._lib.av1h_plan_schedule_model(self._h, plan, model.handle)
+        if rc != AV1H_OK:
+            raise HipError(f"schedule_model failed ({rc}): {self.error()}")
+
+    def set_decode_nv12(self, pitch):
+        """Pack NV12 at `pitch` on the device as part of every decode (0: off)."""
+        if self._lib.av1h_set_decode_nv12(self._h, pitch) != AV1H_OK:
+            raise HipError(f"set_decode_nv12({pitch}) failed")
+
     def reserve(self):
         return self._lib.av1h_reserve_reference(self._h)
 
@@ -367,6 +551,169 @@ class HipDecoder:
 
     def __del__(self):
         self.close()
+
+
+def decode_planned_batch(decoders, plans, keep=None, reserved=None):
+    """Decode one planned frame per decoder as one batch (av1h_decode_planned_batch);
+    returns the slots. The launches go to the first decoder's stream."""
+    n = len(decoders)
+    if n != len(plans):
+        raise ValueError("one plan per decoder")
+    lib = hip_lib()
+    P = ctypes.c_void_p
+    hs = (P * n)(*[d.handle for d in decoders])
+    ps = (P * n)(*plans)
+    ks = (ctypes.c_int * n)(*[1 if k else 0 for k in (keep if keep is not None else [True] * n)])
+    rs = (ctypes.c_int * n)(*(reserved if reserved is not None else [-1] * n))
+    slots = (ctypes.c_int * n)()
+    rc = lib.av1h_decode_planned_batch(hs, ps, ks, rs, n, slots)
+    if rc != AV1H_OK:
+        raise HipError(f"decode_planned_batch failed ({rc}): {decoders[0].error()}")
+    return list(slots)
+
+
+def decode_clips_batched(decoders, clips, frames=None, read_back=True):
+    """Decode several clips, one per decoder, in lockstep: round r decodes
+    frame r of every clip that has one as one batch. Yields (round, [(clip
+    index, slot, planes or None)])."""
+    state = [dict() for _ in clips]
+    n = max(c.frames for c in clips) if frames is None else frames
+    for f in range(n):
+        live = [i for i, c in enumerate(clips) if f < c.frames]
+        plans, keeps = [], []
+        for i in live:
+            c = clips[i]
+            wire = wire_set_slots(c.wire(f), [state[i][r] for r in c.refs[f]])
+            plans.append(decoders[i].plan_wire(wire))
+            keeps.append(c.last_use[f] > f)
+        try:
+            slots = decode_planned_batch([decoders[i] for i in live], plans, keeps)
+        finally:
+            for i, p in zip(live, plans):
+                decoders[i].discard_plan(p)
+        out = []
+        for i, slot, keep in zip(live, slots, keeps):
+            c = clips[i]
+            planes = None
+            if read_back:
+                w, _cw, h, _bd, sx, sy = wire_header(c.wire(f))
+                planes = decoders[i].read_slot(-1, w, h, sx, sy)
+            if keep:
+                state[i][f] = slot
+            for r in c.refs[f]:
+                if c.last_use[r] == f:
+                    decoders[i].release(state[i].pop(r))
+            out.append((i, slot, planes))
+        yield f, out
+
+
+class Server:
+    """The serving loop (av1h_server_*): streams admitted against a device
+    memory budget, frames planned by host threads and decoded in continuously
+    formed batches."""
+
+    def __init__(self, device=0, max_batch=16, planner_threads=4, capture=False, hbm_budget=200 << 30,
+                 nv12_pitch=0):
+        lib = hip_lib()
+        self._lib = lib
+        cfg = ServerConfig(device, max_batch, planner_threads, 1 if capture else 0, hbm_budget, nv12_pitch)
+        h = ctypes.c_void_p()
+        rc = lib.av1h_server_create(ctypes.byref(cfg), ctypes.byref(h))
+        if rc != AV1H_OK:
+            raise HipError(f"av1h_server_create failed ({rc})")
+        self._h = h
+
+    @staticmethod
+    def now_ms():
+        return hip_lib().av1h_server_now_ms()
+
+    def open_stream(self, width, height, bit_depth, ss_x=1, ss_y=1, pool_slots=8):
+        sid = self._lib.av1h_server_open_stream(self._h, width, height, bit_depth, ss_x, ss_y, pool_slots)
+        if sid < 0:
+            raise HipError(f"open_stream failed ({sid})")
+        return sid
+
+    def _rel(self, release):
+        release = list(release or [])
+        return (ctypes.c_int32 * max(1, len(release)))(*release), len(release)
+
+    def submit_wire(self, stream, wire, reserved=-1, release=(), arrival_ms=None, tag=0):
+        arr, n = self._rel(release)
+        t = self.now_ms() if arrival_ms is None else arrival_ms
+        rc = self._lib.av1h_server_submit_wire(self._h, stream, wire, len(wire), reserved, arr, n, t, tag)
+        if rc != AV1H_OK:
+            raise HipError(f"submit_wire failed ({rc})")
+
+    def submit_plan(self, stream, plan, reserved=-1, release=(), arrival_ms=None, tag=0):
+        arr, n = self._rel(release)
+        t = self.now_ms() if arrival_ms is None else arrival_ms
+        rc = self._lib.av1h_server_submit_plan(self._h, stream, plan, reserved, arr, n, t, tag)
+        if rc != AV1H_OK:
+            raise HipError(f"submit_plan failed ({rc})")
+
+    def close_stream(self, stream):
+        self._lib.av1h_server_close_stream(self._h, stream)
+
+    def poll(self, max_items=64, wait_ms=10):
+        """Completions as dicts; a captured picture comes as a flat uint16 array."""
+        buf = (Completion * max_items)()
+        n = self._lib.av1h_server_poll(self._h, buf, max_items, wait_ms)
+        out = []
+        for i in range(max(n, 0)):
+            c = buf[i]
+            d = {k: getattr(c, k) for k, _ in Completion._fields_ if k != "picture"}
+            d["picture"] = None
+            if c.picture:
+                d["picture"] = np.ctypeslib.as_array(c.picture, shape=(c.picture_samples,)).copy()
+                self._lib.av1h_server_free_picture(c.picture)
+            out.append(d)
+        return out
+
+    def stats(self):
+        st = ServerStats()
+        self._lib.av1h_server_get_stats(self._h, ctypes.byref(st))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def error(self):
+        return self._lib.av1h_server_last_error(self._h).decode(errors="replace")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.av1h_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+def stream_bytes(width, height, bit_depth, ss_x=1, ss_y=1, pool_slots=8, nv12_pitch=0):
+    return hip_lib().av1h_stream_bytes(width, height, bit_depth, ss_x, ss_y, pool_slots, nv12_pitch)
+
+
+def slot_schedule(clip, pool_slots=32):
+    """The slots a decoder that always takes the lowest free slot uses for a
+    clip: (slot per frame or -1, slots released after each frame, reference
+    slots per frame)."""
+    used = set()
+    slot_of = {}
+    keep_slot, releases, ref_slots = [], [], []
+    for f in range(clip.frames):
+        ref_slots.append([slot_of[r] for r in clip.refs[f]])
+        if clip.last_use[f] > f:
+            s = min(i for i in range(pool_slots) if i not in used)
+            used.add(s)
+            slot_of[f] = s
+            keep_slot.append(s)
+        else:
+            keep_slot.append(-1)
+        rel = []
+        for r in clip.refs[f]:
+            if clip.last_use[r] == f:
+                s = slot_of.pop(r)
+                used.discard(s)
+                rel.append(s)
+        releases.append(rel)
+    return keep_slot, releases, ref_slots
 
 
 def decode_clip(decoder_factory, clip, frames=None, read_back=True):

@@ -89,6 +89,16 @@ struct Plan {
   int blocks = 0;
   std::vector<int> slots; /* pool slots the frame reads */
   int64_t alg_bytes[AV1H_STAGES] = {}; /* compulsory HBM bytes per stage */
+  /* the queue list in hand-out order, and where each superblock diagonal key
+   * starts in it (key_off[k] .. key_off[k + 1]), for merging the queues of a
+   * batch */
+  std::vector<int32_t> queue_host, key_off;
+  /* superblocks (64x64 luma): features, the superblock of every queue entry
+   * (static order), the static order itself, intra block copy edges
+   * (source superblock, superblock) */
+  int sb_w = 0, sb_h = 0;
+  std::vector<float> sb_feat;
+  std::vector<int32_t> queue_sb, queue_sb_static, queue_static, key_off_static, bc_edges;
 };
 
 struct Ctx {
@@ -112,6 +122,7 @@ struct Ctx {
   uint64_t* trace = nullptr;
   size_t trace_cap = 0;
   int trace_n = 0;
+  const void* trace_plan = nullptr; /* the plan of a traced decode of one frame */
   size_t cdef_dir_cap = 0;
   int16_t* grain_tmpl = nullptr;
   uint8_t* grain_offs = nullptr;
@@ -120,6 +131,8 @@ struct Ctx {
   size_t pack_cap = 0;
   void* host_pin = nullptr;
   size_t host_pin_cap = 0;
+  int pool_slots = kSlots;       /* reference slots the pool holds   */
+  int pool_alloc = 0;            /* slots of the allocated pool      */
   bool used[kSlots] = {};
   Layout slot_lay[kSlots];
   int last_slot = -1;
@@ -133,6 +146,14 @@ struct Ctx {
   hipEvent_t ev_side_go = nullptr, ev_side_done = nullptr, ev_itx_done = nullptr;
   bool timing = false;
   hipEvent_t sev[2 * AV1H_STAGES] = {};
+  /* the frame table of a batch this context leads: jobs, prefix table,
+   * merged queue */
+  size_t decode_nv12_pitch = 0; /* > 0: every decode also packs NV12 into `pack` */
+  bool nv12_current = false;     /* `pack` holds the NV12 of the latest frame       */
+  size_t nv12_pitch_done = 0;
+  void* batch_dev = nullptr;
+  void* batch_pin = nullptr;
+  size_t batch_cap = 0;
   bool stage_hit[AV1H_STAGES] = {};
   double stage_ms[AV1H_STAGES] = {};
   int64_t stage_n[AV1H_STAGES] = {};
@@ -306,6 +327,9 @@ int mc_filter_row(int cls, int extent) {
 
 /* ---- flattening ----------------------------------------------------------- */
 
+struct Host;
+void sb_features(const Host& H, const av1h_frame_header& hdr, const Layout& L, Plan* plan);
+
 struct Host {
   std::vector<DevBlock> blocks;
   std::vector<DevTx> tx;
@@ -363,6 +387,89 @@ struct Owner {
     return true;
   }
 };
+
+/* The superblock features the cost model reads (their list is in the
+ * header, av1h_plan_sb_features), the superblock of every queue entry, and
+ * the superblocks intra block copies read. */
+void sb_features(const Host& H, const av1h_frame_header& hdr, const Layout& L, Plan* plan) {
+  const int sw = (L.grid_w[0] + 63) >> 6, sh = (L.grid_h[0] + 63) >> 6;
+  plan->sb_w = sw;
+  plan->sb_h = sh;
+  const int nsb = sw * sh;
+  std::vector<float>& F = plan->sb_feat;
+  F.assign((size_t)nsb * AV1H_CM_FEATURES, 0.f);
+  auto sb_at = [&](int plane, int x, int y) {
+    const int sx = plane ? L.ss_x : 0, sy = plane ? L.ss_y : 0;
+    const int bx = std::min(std::max((x << sx) >> 6, 0), sw - 1), by = std::min(std::max((y << sy) >> 6, 0), sh - 1);
+    return by * sw + bx;
+  };
+  std::vector<char> queued(H.blocks.size(), 0);
+  for (int32_t b : plan->queue_host) queued[(size_t)b] = 1;
+  std::vector<int> n_inter(nsb, 0), n_sub(nsb, 0);
+  for (size_t i = 0; i < H.blocks.size(); ++i) {
+    const DevBlock& d = H.blocks[i];
+    const int sb = sb_at(d.plane, d.x, d.y);
+    float* f = &F[(size_t)sb * AV1H_CM_FEATURES];
+    const float area = (float)d.w * d.h / 4096.f;
+    f[0] += 1.f;
+    for (int t = 0; t < d.tx_count; ++t) {
+      const DevTx& tx = H.tx[(size_t)(d.tx_first + t)];
+      const int side = std::max(tx.w, tx.h);
+      const int cls = side <= 4 ? 0 : side <= 8 ? 1 : side <= 16 ? 2 : side <= 32 ? 3 : 4;
+      f[1 + cls] += 1.f;
+      f[6] += (float)tx.eob / 64.f;
+      f[16] = std::max(f[16], (float)side / 64.f);
+    }
+    if (d.kind == kIntra) {
+      f[7] += 1.f;
+      if (d.mode >= AV1H_PRED_V && d.mode <= AV1H_PRED_D67) f[11] += 1.f;
+      if (d.mode >= AV1H_PRED_SMOOTH && d.mode <= AV1H_PRED_PAETH) f[12] += 1.f;
+      if (d.mode == AV1H_PRED_CFL || (d.flags & (kPalette | kFilterIntra))) f[13] += 1.f;
+    } else {
+      f[8] += 1.f;
+      f[15] += area;
+      ++n_inter[sb];
+      if (d.compound) f[9] += 1.f;
+      bool sub = false;
+      for (int r = 0; r < (d.compound ? 2 : 1); ++r) sub = sub || (d.mv[r][0] & 7) || (d.mv[r][1] & 7);
+      if (sub) ++n_sub[sb];
+      if (d.flags & (kInterIntra | kWarp0 | kWarp1 | kMaskWarp0 | kMaskWarp1) || d.obmc_count > 0 || d.mask_kind)
+        f[20] += 1.f;
+    }
+    if (d.flags & (kLossless | kResidualRaw)) f[17] += 1.f;
+    if (d.flags & kIntraBC) f[18] += 1.f;
+    if (d.plane) f[19] += 1.f;
+    if (queued[i]) {
+      f[14] += area;
+      f[22] += 1.f;
+    }
+  }
+  for (int sb = 0; sb < nsb; ++sb) {
+    float* f = &F[(size_t)sb * AV1H_CM_FEATURES];
+    f[10] = n_inter[sb] ? (float)n_sub[sb] / (float)n_inter[sb] : 0.f;
+    f[21] = hdr.enable_intra_edge_filter ? 1.f : 0.f;
+    f[23] = hdr.bit_depth > 8 ? 1.f : 0.f;
+  }
+  plan->queue_sb.resize(plan->queue_host.size());
+  plan->bc_edges.clear();
+  for (size_t k = 0; k < plan->queue_host.size(); ++k) {
+    const DevBlock& d = H.blocks[(size_t)plan->queue_host[k]];
+    const int sb = sb_at(d.plane, d.x, d.y);
+    plan->queue_sb[k] = sb;
+    if ((d.flags & kIntraBC) && d.dep_rx1 >= d.dep_rx0 && d.dep_ry1 >= d.dep_ry0) {
+      const int sh_ = L.cell_shift[d.plane];
+      const int a = sb_at(d.plane, d.dep_rx0 << sh_, d.dep_ry0 << sh_);
+      const int b = sb_at(d.plane, d.dep_rx1 << sh_, d.dep_ry1 << sh_);
+      for (int y = a / sw; y <= b / sw; ++y)
+        for (int x = a % sw; x <= b % sw; ++x)
+          if (y * sw + x != sb) {
+            plan->bc_edges.push_back(y * sw + x);
+            plan->bc_edges.push_back(sb);
+          }
+    }
+  }
+  plan->queue_sb_static = plan->queue_sb;
+}
 
 int resolve_slot(const av1h_frame_header& h, int ref) {
   if (ref < 0 || ref >= h.num_reference_slots) return -1;
@@ -685,7 +792,17 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
     std::vector<int32_t> q(ord.size());
     for (size_t k = 0; k < ord.size(); ++k) q[k] = H->queue_list[ord[k]];
     H->queue_list.swap(q);
+    plan->queue_host = H->queue_list;
+    plan->key_off.clear();
+    for (size_t k = 0; k < ord.size(); ++k) {
+      const int kk = (int)(key[ord[k]] >> 32);
+      while ((int)plan->key_off.size() <= kk) plan->key_off.push_back((int32_t)k);
+    }
+    plan->key_off.push_back((int32_t)ord.size());
+    plan->queue_static = plan->queue_host;
+    plan->key_off_static = plan->key_off;
   }
+  sb_features(*H, hdr, L, plan);
 
   /* motion compensation tasks: items (whole blocks, or row strips of blocks
    * over kMcSamples samples) in coding order, with their positions, windows
@@ -1144,10 +1261,20 @@ int prepare(Ctx* c, const Layout& L) {
     HIP_OK(c, hipMalloc(&c->lr_curr, bytes));
     HIP_OK(c, hipMalloc(&c->lr_cdef, bytes));
     HIP_OK(c, hipMalloc(&c->lr_prev_up, bytes));
-    HIP_OK(c, hipMalloc(&c->pool, bytes * kSlots));
     HIP_OK(c, hipMemsetAsync(c->frame, 0, bytes, c->stream));
-    HIP_OK(c, hipMemsetAsync(c->pool, 0, bytes * kSlots, c->stream));
     c->cap = (size_t)L.size;
+    c->pool_alloc = 0;
+    std::memset(c->used, 0, sizeof(c->used));
+    c->last_slot = -1;
+  }
+  if (c->pool_alloc != c->pool_slots) {
+    if (c->pool) (void)hipFree(c->pool);
+    c->pool = nullptr;
+    c->pool_alloc = 0;
+    const size_t bytes = c->cap * 2 * (size_t)c->pool_slots;
+    HIP_OK(c, hipMalloc(&c->pool, bytes));
+    HIP_OK(c, hipMemsetAsync(c->pool, 0, bytes, c->stream));
+    c->pool_alloc = c->pool_slots;
     std::memset(c->used, 0, sizeof(c->used));
     c->last_slot = -1;
   }
@@ -1173,238 +1300,411 @@ int prepare(Ctx* c, const Layout& L) {
 }
 
 int alloc_slot(Ctx* c) {
-  for (int i = 0; i < kSlots; ++i)
+  for (int i = 0; i < c->pool_slots; ++i)
     if (!c->used[i]) return i;
   return -1;
 }
 
-int run_plan(Ctx* c, Plan* plan, int keep, int reserved, const av1h_planes* out, av1h_frame_info* info,
-             int* slot_out, double t0) {
-  if (slot_out) *slot_out = -1;
-  const Layout& L = plan->lay;
-  int rc = prepare(c, L);
-  if (rc != AV1H_OK) return rc;
-  for (int s : plan->slots)
-    if (!c->used[s]) return set_err(c, AV1H_ERR_INVALID, "a block names reference slot %d, which holds no picture", s);
-  int slot = -1;
-  if (keep) {
-    if (reserved >= 0) {
-      if (reserved >= kSlots) return set_err(c, AV1H_ERR_INVALID, "invalid reference slot");
-      slot = reserved;
-    } else {
-      slot = alloc_slot(c);
-      if (slot < 0) return set_err(c, AV1H_ERR_SLOTS, "reference pool exhausted");
-    }
+int ensure_pack(Ctx* c, size_t bytes) {
+  if (c->pack_cap < bytes) {
+    if (c->pack) (void)hipFree(c->pack);
+    c->pack = nullptr;
+    c->pack_cap = 0;
+    HIP_OK(c, hipMalloc(&c->pack, bytes));
+    c->pack_cap = bytes;
   }
-  hipStream_t s = c->stream;
+  return AV1H_OK;
+}
+
+/* ---- batches --------------------------------------------------------------
+ * Every decode runs as a batch: the frames of one or more contexts (one per
+ * stream), each kernel launched once for all of them. */
+
+struct JobSpec {
+  Ctx* c;
+  Plan* plan;
+  int keep, reserved;
+  const av1h_planes* out;
+  av1h_frame_info* info;
+  int* slot_out;
+  int slot;
+};
+
+int copy_groups(const Layout& L) { return (int)((L.size * 2 + kCopyChunk - 1) / kCopyChunk); }
+
+/* virtual workgroups of kernel k for one frame */
+int kernel_groups(const FrameJob& fj, int k) {
+  const FramePlanDev& p = fj.r.plan;
+  const FilterArgs& f = fj.f;
+  const Layout& L = fj.r.lay;
+  switch (k) {
+    case kKMc: return p.mc_tasks;
+    case kKSpecial: return p.n_special;
+    case kKItxInter: return itx_groups(p, 0);
+    case kKItxIntra: return itx_groups(p, 1);
+    case kKDeblockV: return p.n_deblock_v > 0 ? (p.lines_v + 255) / 256 : 0;
+    case kKDeblockH: return p.n_deblock_h > 0 ? (p.lines_h + 255) / 256 : 0;
+    case kKCdefDir: return p.n_cdef > 0 ? (p.n_cdef_dir + 3) / 4 : 0;
+    case kKCdefLuma: return p.n_cdef > 0 ? (p.n_cdef_luma + 3) / 4 : 0;
+    case kKCdefChroma: {
+      const int nc = p.n_cdef - p.n_cdef_luma;
+      const int per = 256 / ((8 >> L.ss_x) * (8 >> L.ss_y));
+      return nc > 0 ? (nc + per - 1) / per : 0;
+    }
+    case kKUpscale: return f.upscaled ? upscale_tiles(L, f.uses_lr ? 2 : 1) : 0;
+    case kKLr: return f.uses_lr ? p.n_lr_tiles : 0;
+    case kKGrainTemplate: return f.has_grain ? 1 : 0;
+    case kKGrainApply: return f.has_grain ? grain_tiles(L) : 0;
+    case kKNv12: return f.nv12 ? (L.height + ((L.height + 1) >> 1)) * ((f.nv12_pitch + 1023) / 1024) : 0;
+    case kKCopyLrCurr: return f.uses_lr ? copy_groups(L) : 0;
+    case kKCopyCdefSrc: return p.n_cdef > 0 && !f.uses_lr ? copy_groups(L) : 0;
+    case kKCopyUpSrc: return f.upscaled ? copy_groups(L) : 0;
+    case kKCopyLrCdef: return f.uses_lr ? copy_groups(L) : 0;
+    case kKCopyStore: return f.store ? copy_groups(L) : 0;
+    default: return 0;
+  }
+}
+
+/* The queues of a batch merged into one hand-out order. Inside a frame the
+ * order stays its own (superblock diagonals, a topological order); across
+ * frames a diagonal is ranked by the diagonals still after it in its frame,
+ * so the frame with the longest chain of diagonals starts first and all
+ * frames finish their last diagonals together. */
+void merge_queues(const std::vector<JobSpec>& js, uint32_t* out) {
+  int span = 0;
+  for (const JobSpec& j : js) span = std::max(span, (int)j.plan->key_off.size() - 1);
+  size_t n = 0;
+  for (int lvl = 0; lvl < span; ++lvl)
+    for (size_t i = 0; i < js.size(); ++i) {
+      const std::vector<int32_t>& ko = js[i].plan->key_off;
+      const int keys = (int)ko.size() - 1;
+      const int k = lvl - (span - keys);
+      if (k < 0) continue;
+      const int32_t* q = js[i].plan->queue_host.data();
+      for (int32_t e = ko[k]; e < ko[k + 1]; ++e) out[n++] = (uint32_t)i << kQueueJobShift | (uint32_t)q[e];
+    }
+}
+
+int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
+  const int n = (int)js.size();
+  for (JobSpec& j : js) {
+    j.slot = -1;
+    if (j.slot_out) *j.slot_out = -1;
+  }
+  if (n < 1 || n > kMaxBatch) return set_err(lead, AV1H_ERR_INVALID, "a batch holds 1 to %d frames", kMaxBatch);
+  for (int i = 0; i < n; ++i) {
+    if (js[i].c->device != lead->device)
+      return set_err(lead, AV1H_ERR_INVALID, "the contexts of a batch must share one device");
+    for (int k = 0; k < i; ++k)
+      if (js[k].c == js[i].c) return set_err(lead, AV1H_ERR_INVALID, "a context appears twice in one batch");
+  }
+  /* per frame buffers, slots, epochs */
+  int64_t nq = 0;
+  for (JobSpec& j : js) {
+    Ctx* c = j.c;
+    Plan* plan = j.plan;
+    const Layout& L = plan->lay;
+    int rc = prepare(c, L);
+    if (rc != AV1H_OK) return c == lead ? rc : set_err(lead, rc, "%s", c->err.c_str());
+    for (int sl : plan->slots)
+      if (!c->used[sl])
+        return set_err(lead, AV1H_ERR_INVALID, "a block names reference slot %d, which holds no picture", sl);
+    if (j.keep) {
+      if (j.reserved >= 0) {
+        if (j.reserved >= c->pool_slots) return set_err(lead, AV1H_ERR_INVALID, "invalid reference slot");
+        j.slot = j.reserved;
+      } else {
+        j.slot = alloc_slot(c);
+        if (j.slot < 0) return set_err(lead, AV1H_ERR_SLOTS, "reference pool exhausted");
+      }
+    }
+    if (plan->has_grain) {
+      const size_t need = (size_t)plan->grain.stripes * plan->grain.blocks + 64;
+      if (c->grain_offs_cap < need) {
+        HIP_OK(lead, hipStreamSynchronize(c->stream));
+        if (c->grain_offs) (void)hipFree(c->grain_offs);
+        c->grain_offs = nullptr;
+        HIP_OK(lead, hipMalloc(&c->grain_offs, need));
+        c->grain_offs_cap = need;
+      }
+    }
+    c->nv12_current = false;
+    if (c->decode_nv12_pitch > 0 &&
+        av1h_nv12_size(L.width, L.height, L.ss_x, L.ss_y, c->decode_nv12_pitch) > 0) {
+      rc = ensure_pack(c, av1h_nv12_size(L.width, L.height, L.ss_x, L.ss_y, c->decode_nv12_pitch));
+      if (rc != AV1H_OK) return c == lead ? rc : set_err(lead, rc, "%s", c->err.c_str());
+    }
+    if ((int64_t)plan->queue_host.size() >= (1 << kQueueJobShift))
+      return set_err(lead, AV1H_ERR_INVALID, "too many intra blocks in one frame");
+    nq += plan->dev.n_queue;
+  }
+  hipStream_t s = lead->stream;
+  /* the frame table */
+  std::vector<FrameJob> jobs(n);
+  for (int i = 0; i < n; ++i) {
+    Ctx* c = js[i].c;
+    Plan* plan = js[i].plan;
+    const Layout& L = plan->lay;
+    ++c->epoch;
+    ReconArgs& ra = jobs[i].r;
+    ra.plan = plan->dev;
+    ra.lay = L;
+    ra.frame = c->frame;
+    ra.pool = c->pool;
+    ra.slot_size = L.size;
+    ra.cells = c->cells;
+    ra.epoch = c->epoch;
+    ra.res = reinterpret_cast<int16_t*>(c->scratch); /* free until the CDEF snapshot */
+    ra.maxv = (1 << L.bit_depth) - 1;
+    ra.edge_filter = plan->hdr.enable_intra_edge_filter;
+    FilterArgs& fa = jobs[i].f;
+    fa.plan = plan->dev;
+    fa.lay = L;
+    fa.frame = c->frame;
+    fa.scratch = c->scratch;
+    fa.lr_curr = c->lr_curr;
+    fa.lr_cdef = c->lr_cdef;
+    fa.lr_prev_up = c->lr_prev_up;
+    fa.cdef_src = plan->uses_lr ? c->lr_curr : c->scratch;
+    fa.cdef_dir = c->cdef_dir;
+    fa.cdef_dir_w = L.grid_w[0] >> 3;
+    fa.sharpness = plan->hdr.sharpness;
+    fa.maxv = ra.maxv;
+    for (int p = 0; p < 3; ++p) superres_params(L.plane_w[p], L.out_w[p], &fa.up_step[p], &fa.up_initial[p]);
+    fa.upscaled = plan->upscaled;
+    fa.uses_lr = plan->uses_lr;
+    fa.has_grain = plan->has_grain;
+    fa.grain_tmpl = c->grain_tmpl;
+    fa.grain_offs = c->grain_offs;
+    fa.store = js[i].slot >= 0 ? c->pool + (size_t)js[i].slot * L.size : nullptr;
+    const bool nv12 = c->decode_nv12_pitch > 0 &&
+                      av1h_nv12_size(L.width, L.height, L.ss_x, L.ss_y, c->decode_nv12_pitch) > 0;
+    fa.nv12 = nv12 ? c->pack : nullptr;
+    fa.nv12_pitch = nv12 ? (int32_t)c->decode_nv12_pitch : 0;
+  }
+  std::vector<int32_t> prefix((size_t)kKernels * (n + 1), 0);
+  for (int k = 0; k < kKernels; ++k)
+    for (int i = 0; i < n; ++i)
+      prefix[(size_t)k * (n + 1) + i + 1] = prefix[(size_t)k * (n + 1) + i] + kernel_groups(jobs[i], k);
+  auto total_of = [&](int k) { return prefix[(size_t)k * (n + 1) + n]; };
+  /* upload: [jobs][prefix][merged queue] */
+  const size_t o_prefix = align256(sizeof(FrameJob) * n);
+  const size_t o_queue = o_prefix + align256(prefix.size() * 4);
+  const size_t bytes_tab = o_queue + (n > 1 ? (size_t)nq * 4 : 0);
+  if (lead->batch_cap < bytes_tab) {
+    HIP_OK(lead, hipStreamSynchronize(s));
+    if (lead->batch_dev) (void)hipFree(lead->batch_dev);
+    if (lead->batch_pin) (void)hipHostFree(lead->batch_pin);
+    lead->batch_dev = lead->batch_pin = nullptr;
+    lead->batch_cap = 0;
+    const size_t cap = std::max(bytes_tab + bytes_tab / 4, (size_t)65536);
+    HIP_OK(lead, hipMalloc(&lead->batch_dev, cap));
+    HIP_OK(lead, hipHostMalloc(&lead->batch_pin, cap, hipHostMallocDefault));
+    lead->batch_cap = cap;
+  }
+  uint8_t* pin = static_cast<uint8_t*>(lead->batch_pin);
+  std::memcpy(pin, jobs.data(), sizeof(FrameJob) * n);
+  std::memcpy(pin + o_prefix, prefix.data(), prefix.size() * 4);
+  if (n > 1) merge_queues(js, reinterpret_cast<uint32_t*>(pin + o_queue));
+  /* everything queued on the other contexts' streams (plan uploads, buffer
+   * clears) comes first */
+  for (JobSpec& j : js)
+    if (j.c != lead) {
+      HIP_OK(lead, hipEventRecord(j.c->ev[3], j.c->stream));
+      HIP_OK(lead, hipStreamWaitEvent(s, j.c->ev[3], 0));
+    }
+  uint8_t* tab = static_cast<uint8_t*>(lead->batch_dev);
+  HIP_OK(lead, hipMemcpyAsync(tab, pin, bytes_tab, hipMemcpyHostToDevice, s));
+  BatchDev B;
+  B.jobs = reinterpret_cast<const FrameJob*>(tab);
+  B.prefix = reinterpret_cast<const int32_t*>(tab + o_prefix);
+  B.njobs = n;
+  B.queue = n > 1 ? reinterpret_cast<const uint32_t*>(tab + o_queue) : nullptr;
+  B.nqueue = (int32_t)nq;
+  B.queue_counter = lead->counters;
+  B.error = lead->counters + 1;
+  B.trace = nullptr;
+  grain_jump_matrices(B.jump82, B.jump44);
+  if (lead->trace_on && nq > 0) {
+    const size_t need = (size_t)nq * 4;
+    if (lead->trace_cap < need) {
+      HIP_OK(lead, hipStreamSynchronize(s));
+      if (lead->trace) (void)hipFree(lead->trace);
+      lead->trace = nullptr;
+      lead->trace_cap = 0;
+      HIP_OK(lead, hipMalloc(&lead->trace, need * 8));
+      lead->trace_cap = need;
+    }
+    B.trace = lead->trace;
+  }
+  lead->trace_n = lead->trace_on ? (int)nq : 0;
+  lead->trace_plan = lead->trace_on && n == 1 ? js[0].plan : nullptr;
+
   int launches = 0;
-  ++c->epoch;
-  const size_t bytes = (size_t)L.size * 2;
-  ReconArgs ra;
-  ra.plan = plan->dev;
-  ra.lay = L;
-  ra.frame = c->frame;
-  ra.pool = c->pool;
-  ra.slot_size = L.size;
-  ra.cells = c->cells;
-  ra.epoch = c->epoch;
-  ra.queue_counter = c->counters;
-  ra.error = c->counters + 1;
-  ra.res = reinterpret_cast<int16_t*>(c->scratch); /* free until the CDEF snapshot */
-  ra.trace = nullptr;
-  if (c->trace_on && plan->dev.n_queue > 0) {
-    const size_t need = (size_t)plan->dev.n_queue * 4;
-    if (c->trace_cap < need) {
-      HIP_OK(c, hipStreamSynchronize(s));
-      if (c->trace) (void)hipFree(c->trace);
-      c->trace = nullptr;
-      c->trace_cap = 0;
-      HIP_OK(c, hipMalloc(&c->trace, need * 8));
-      c->trace_cap = need;
+  auto run = [&](int k, hipStream_t st) -> hipError_t {
+    const int total = total_of(k);
+    if (total <= 0) return hipSuccess;
+    ++launches;
+    switch (k) {
+      case kKMc:
+      case kKSpecial:
+      case kKItxInter:
+      case kKItxIntra: return launch_recon_kernel(B, k, total, lead->cus, st);
+      case kKGrainTemplate:
+      case kKGrainApply: return launch_grain_kernel(B, k, total, lead->cus, st);
+      default: return launch_filter_kernel(B, k, total, lead->cus, st);
     }
-    ra.trace = c->trace;
-  }
-  c->trace_n = c->trace_on ? plan->dev.n_queue : 0;
-  ra.maxv = (1 << L.bit_depth) - 1;
-  ra.edge_filter = plan->hdr.enable_intra_edge_filter;
-  if (plan->has_grain) {
-    const size_t need = (size_t)plan->grain.stripes * plan->grain.blocks + 64;
-    if (c->grain_offs_cap < need) {
-      HIP_OK(c, hipStreamSynchronize(s));
-      if (c->grain_offs) (void)hipFree(c->grain_offs);
-      c->grain_offs = nullptr;
-      HIP_OK(c, hipMalloc(&c->grain_offs, need));
-      c->grain_offs_cap = need;
-    }
-  }
+  };
   /* The second stream takes what does not depend on the prediction: the
    * transforms of intra blocks (the queue waits for them) and the grain
-   * templates (the grain blend waits for them). It starts after everything
-   * queued before this frame: the plan upload, the last frame's grain. */
-  const bool side_itx = itx_groups(plan->dev, 1) > 0;
-  if (side_itx || plan->has_grain) {
-    HIP_OK(c, hipEventRecord(c->ev_side_go, s));
-    HIP_OK(c, hipStreamWaitEvent(c->side, c->ev_side_go, 0));
+   * templates (the grain blend waits for them). */
+  const bool side_itx = total_of(kKItxIntra) > 0, side_grain = total_of(kKGrainTemplate) > 0;
+  if (side_itx || side_grain) {
+    HIP_OK(lead, hipEventRecord(lead->ev_side_go, s));
+    HIP_OK(lead, hipStreamWaitEvent(lead->side, lead->ev_side_go, 0));
   }
   if (side_itx) {
-    HIP_OK(c, launch_itx(ra, 1, c->side, &launches));
-    HIP_OK(c, hipEventRecord(c->ev_itx_done, c->side));
+    HIP_OK(lead, run(kKItxIntra, lead->side));
+    HIP_OK(lead, hipEventRecord(lead->ev_itx_done, lead->side));
   }
-  if (plan->has_grain) {
-    HIP_OK(c, launch_grain_template(L, plan->dev.grain, c->grain_tmpl, c->grain_offs, c->side, &launches));
-    HIP_OK(c, hipEventRecord(c->ev_side_done, c->side));
+  if (side_grain) {
+    HIP_OK(lead, run(kKGrainTemplate, lead->side));
+    HIP_OK(lead, hipEventRecord(lead->ev_side_done, lead->side));
   }
-  HIP_OK(c, hipEventRecord(c->ev[0], s));
-  for (bool& h : c->stage_hit) h = false;
-  auto begin = [&](int st) {
-    if (c->timing) (void)hipEventRecord(c->sev[2 * st], s);
-  };
-  auto end = [&](int st) {
-    if (c->timing) {
-      (void)hipEventRecord(c->sev[2 * st + 1], s);
-      c->stage_hit[st] = true;
+  HIP_OK(lead, hipEventRecord(lead->ev[0], s));
+  for (bool& h : lead->stage_hit) h = false;
+  auto stage = [&](int st, std::initializer_list<int> ks) -> hipError_t {
+    bool any = false;
+    for (int k : ks) any = any || total_of(k) > 0;
+    if (!any) return hipSuccess;
+    if (lead->timing) (void)hipEventRecord(lead->sev[2 * st], s);
+    for (int k : ks) {
+      const hipError_t e = run(k, s);
+      if (e != hipSuccess) return e;
     }
+    if (lead->timing) {
+      (void)hipEventRecord(lead->sev[2 * st + 1], s);
+      lead->stage_hit[st] = true;
+    }
+    return hipSuccess;
   };
-  if (plan->dev.mc_tasks > 0 || plan->dev.n_special > 0 || itx_groups(plan->dev, 0) > 0) {
-    begin(AV1H_STAGE_INTER);
-    HIP_OK(c, launch_mc(ra, c->cus, s, &launches));
-    HIP_OK(c, launch_special(ra, s, &launches));
-    HIP_OK(c, launch_itx(ra, 0, s, &launches));
-    end(AV1H_STAGE_INTER);
+  HIP_OK(lead, stage(AV1H_STAGE_INTER, {kKMc, kKSpecial, kKItxInter}));
+  if (side_itx) HIP_OK(lead, hipStreamWaitEvent(s, lead->ev_itx_done, 0));
+  if (nq > 0) {
+    if (lead->timing) (void)hipEventRecord(lead->sev[2 * AV1H_STAGE_QUEUE], s);
+    HIP_OK(lead, launch_queue(B, lead->cus, s, &launches));
+    if (lead->timing) {
+      (void)hipEventRecord(lead->sev[2 * AV1H_STAGE_QUEUE + 1], s);
+      lead->stage_hit[AV1H_STAGE_QUEUE] = true;
+    }
   }
-  if (side_itx) HIP_OK(c, hipStreamWaitEvent(s, c->ev_itx_done, 0));
-  if (plan->dev.n_queue > 0) {
-    begin(AV1H_STAGE_QUEUE);
-    HIP_OK(c, launch_queue(ra, c->cus, s, &launches));
-    end(AV1H_STAGE_QUEUE);
-  }
-  HIP_OK(c, hipEventRecord(c->ev[1], s));
-  FilterArgs fa;
-  fa.plan = plan->dev;
-  fa.lay = L;
-  fa.frame = c->frame;
-  fa.scratch = c->scratch;
-  fa.lr_curr = c->lr_curr;
-  fa.lr_cdef = c->lr_cdef;
-  fa.lr_prev_up = c->lr_prev_up;
-  fa.cdef_dir = c->cdef_dir;
-  fa.cdef_dir_w = L.grid_w[0] >> 3;
-  fa.sharpness = plan->hdr.sharpness;
-  fa.maxv = ra.maxv;
-  if (plan->dev.lines_v + plan->dev.lines_h > 0) {
-    begin(AV1H_STAGE_DEBLOCK);
-    HIP_OK(c, launch_deblock(fa, s, &launches));
-    end(AV1H_STAGE_DEBLOCK);
-  }
-  if (plan->uses_lr) HIP_OK(c, hipMemcpyAsync(c->lr_curr, c->frame, bytes, hipMemcpyDeviceToDevice, s));
-  if (plan->dev.n_cdef > 0) {
-    HIP_OK(c, hipMemcpyAsync(c->scratch, c->frame, bytes, hipMemcpyDeviceToDevice, s));
-    begin(AV1H_STAGE_CDEF);
-    HIP_OK(c, launch_cdef(fa, s, &launches));
-    end(AV1H_STAGE_CDEF);
-  }
-  if (plan->upscaled) {
-    HIP_OK(c, hipMemcpyAsync(c->scratch, c->frame, bytes, hipMemcpyDeviceToDevice, s));
-    begin(AV1H_STAGE_SUPERRES);
-    HIP_OK(c, launch_superres(fa, c->scratch, c->frame, plan->uses_lr ? c->lr_curr : nullptr,
-                              plan->uses_lr ? c->lr_prev_up : nullptr, s, &launches));
-    end(AV1H_STAGE_SUPERRES);
-  }
-  if (plan->uses_lr) {
-    HIP_OK(c, hipMemcpyAsync(c->lr_cdef, c->frame, bytes, hipMemcpyDeviceToDevice, s));
-    begin(AV1H_STAGE_LR);
-    HIP_OK(c, launch_lr(fa, c->lr_cdef, plan->upscaled ? c->lr_prev_up : c->lr_curr, s, &launches));
-    end(AV1H_STAGE_LR);
-  }
-  if (slot >= 0) {
-    begin(AV1H_STAGE_STORE);
-    HIP_OK(c, hipMemcpyAsync(c->pool + (size_t)slot * L.size, c->frame, bytes, hipMemcpyDeviceToDevice, s));
-    end(AV1H_STAGE_STORE);
-  }
-  if (plan->has_grain) {
-    HIP_OK(c, hipStreamWaitEvent(s, c->ev_side_done, 0));
-    begin(AV1H_STAGE_GRAIN);
-    HIP_OK(c, launch_grain_apply(L, plan->dev.grain, c->frame, c->grain_tmpl, c->grain_offs, s, &launches));
-    end(AV1H_STAGE_GRAIN);
-  }
-  HIP_OK(c, hipEventRecord(c->ev[2], s));
+  HIP_OK(lead, hipEventRecord(lead->ev[1], s));
+  HIP_OK(lead, stage(AV1H_STAGE_DEBLOCK, {kKDeblockV, kKDeblockH}));
+  HIP_OK(lead, run(kKCopyLrCurr, s));
+  HIP_OK(lead, run(kKCopyCdefSrc, s));
+  HIP_OK(lead, stage(AV1H_STAGE_CDEF, {kKCdefDir, kKCdefLuma, kKCdefChroma}));
+  HIP_OK(lead, run(kKCopyUpSrc, s));
+  HIP_OK(lead, stage(AV1H_STAGE_SUPERRES, {kKUpscale}));
+  HIP_OK(lead, run(kKCopyLrCdef, s));
+  HIP_OK(lead, stage(AV1H_STAGE_LR, {kKLr}));
+  HIP_OK(lead, stage(AV1H_STAGE_STORE, {kKCopyStore}));
+  if (side_grain) HIP_OK(lead, hipStreamWaitEvent(s, lead->ev_side_done, 0));
+  HIP_OK(lead, stage(AV1H_STAGE_GRAIN, {kKGrainApply}));
+  HIP_OK(lead, run(kKNv12, s));
+  HIP_OK(lead, hipEventRecord(lead->ev[2], s));
   const double t_sub = now_ms();
-  if (slot >= 0) {
-    c->used[slot] = true;
-    c->slot_lay[slot] = L;
-  }
-  c->last_slot = slot;
-  c->have_frame = true;
   /* read back */
-  if (out) {
-    size_t host_need = (size_t)L.size * 2;
+  for (JobSpec& j : js) {
+    if (!j.out) continue;
+    Ctx* c = j.c;
+    const size_t host_need = (size_t)j.plan->lay.size * 2;
     if (c->host_pin_cap < host_need) {
-      HIP_OK(c, hipStreamSynchronize(s));
+      HIP_OK(lead, hipStreamSynchronize(s));
+      HIP_OK(lead, hipStreamSynchronize(c->stream));
       if (c->host_pin) (void)hipHostFree(c->host_pin);
       c->host_pin = nullptr;
       c->host_pin_cap = 0;
-      HIP_OK(c, hipHostMalloc(&c->host_pin, host_need, hipHostMallocDefault));
+      HIP_OK(lead, hipHostMalloc(&c->host_pin, host_need, hipHostMallocDefault));
       c->host_pin_cap = host_need;
     }
-    HIP_OK(c, hipMemcpyAsync(c->host_pin, c->frame, host_need, hipMemcpyDeviceToHost, s));
+    HIP_OK(lead, hipMemcpyAsync(c->host_pin, c->frame, host_need, hipMemcpyDeviceToHost, s));
   }
   uint32_t err_word = 0;
-  HIP_OK(c, hipMemcpyAsync(&err_word, &c->counters[1], 4, hipMemcpyDeviceToHost, s));
-  HIP_OK(c, hipStreamSynchronize(s));
+  HIP_OK(lead, hipMemcpyAsync(&err_word, &lead->counters[1], 4, hipMemcpyDeviceToHost, s));
+  HIP_OK(lead, hipStreamSynchronize(s));
   const double t_done = now_ms();
+  for (JobSpec& j : js) j.c->have_frame = true;
   if (err_word) {
-    HIP_OK(c, hipMemsetAsync(c->counters + 1, 0, 4, s));
-    HIP_OK(c, hipStreamSynchronize(s));
-    if (slot >= 0) c->used[slot] = false;
-    return set_err(c, AV1H_ERR_INVALID,
+    HIP_OK(lead, hipMemsetAsync(lead->counters + 1, 0, 4, s));
+    HIP_OK(lead, hipStreamSynchronize(s));
+    for (JobSpec& j : js) j.c->last_slot = -1;
+    return set_err(lead, AV1H_ERR_INVALID,
                    "a block waited too long for a neighbour: the availability flags name blocks "
                    "that come later in coding order");
   }
-  if (out) {
-    const uint16_t* h = static_cast<const uint16_t*>(c->host_pin);
-    for (int p = 0; p < 3; ++p)
-      for (int y = 0; y < L.out_h[p]; ++y)
-        std::memcpy(out->plane[p] + (size_t)y * out->stride[p], h + L.plane_off[p] + (size_t)y * L.stride[p],
-                    (size_t)L.out_w[p] * 2);
-  }
-  if (info) {
-    info->width = L.width;
-    info->height = L.height;
-    info->bit_depth = L.bit_depth;
-    info->ss_x = L.ss_x;
-    info->ss_y = L.ss_y;
-  }
-  if (c->timing)
+  float recon = 0, filt = 0;
+  (void)hipEventElapsedTime(&recon, lead->ev[0], lead->ev[1]);
+  (void)hipEventElapsedTime(&filt, lead->ev[1], lead->ev[2]);
+  if (lead->timing)
     for (int st = 0; st < AV1H_STAGES; ++st)
-      if (c->stage_hit[st]) {
+      if (lead->stage_hit[st]) {
         float ms = 0;
-        if (hipEventElapsedTime(&ms, c->sev[2 * st], c->sev[2 * st + 1]) == hipSuccess) {
-          c->stage_ms[st] += ms;
-          ++c->stage_n[st];
-          c->stage_bytes[st] += plan->alg_bytes[st];
+        if (hipEventElapsedTime(&ms, lead->sev[2 * st], lead->sev[2 * st + 1]) == hipSuccess) {
+          lead->stage_ms[st] += ms;
+          ++lead->stage_n[st];
+          for (JobSpec& j : js) lead->stage_bytes[st] += j.plan->alg_bytes[st];
         }
       }
-  float recon = 0, filt = 0;
-  (void)hipEventElapsedTime(&recon, c->ev[0], c->ev[1]);
-  (void)hipEventElapsedTime(&filt, c->ev[1], c->ev[2]);
-  c->stats.blocks = plan->blocks;
-  c->stats.waves = 0;
-  c->stats.kernel_launches = launches;
-  c->stats.device_recon_ms = recon;
-  c->stats.device_filter_ms = filt;
-  c->stats.reconstruct_ms = recon;
-  c->stats.filter_ms = filt;
-  c->stats.readback_ms = t_done - t_sub;
-  c->stats.total_ms = t_done - t0;
-  if (slot_out) *slot_out = slot;
+  for (JobSpec& j : js) {
+    Ctx* c = j.c;
+    const Layout& L = j.plan->lay;
+    if (j.slot >= 0) {
+      c->used[j.slot] = true;
+      c->slot_lay[j.slot] = L;
+    }
+    c->last_slot = j.slot;
+    c->nv12_current = jobs[&j - js.data()].f.nv12 != nullptr;
+    c->nv12_pitch_done = c->decode_nv12_pitch;
+    if (j.out) {
+      const uint16_t* h = static_cast<const uint16_t*>(c->host_pin);
+      for (int p = 0; p < 3; ++p)
+        for (int y = 0; y < L.out_h[p]; ++y)
+          std::memcpy(j.out->plane[p] + (size_t)y * j.out->stride[p], h + L.plane_off[p] + (size_t)y * L.stride[p],
+                      (size_t)L.out_w[p] * 2);
+    }
+    if (j.info) {
+      j.info->width = L.width;
+      j.info->height = L.height;
+      j.info->bit_depth = L.bit_depth;
+      j.info->ss_x = L.ss_x;
+      j.info->ss_y = L.ss_y;
+    }
+    c->stats.blocks = j.plan->blocks;
+    c->stats.waves = 0;
+    c->stats.kernel_launches = launches;
+    c->stats.device_recon_ms = recon;
+    c->stats.device_filter_ms = filt;
+    c->stats.reconstruct_ms = recon;
+    c->stats.filter_ms = filt;
+    c->stats.readback_ms = t_done - t_sub;
+    c->stats.total_ms = t_done - t0;
+    if (j.slot_out) *j.slot_out = j.slot;
+  }
   return AV1H_OK;
+}
+
+int run_plan(Ctx* c, Plan* plan, int keep, int reserved, const av1h_planes* out, av1h_frame_info* info,
+             int* slot_out, double t0) {
+  std::vector<JobSpec> js(1);
+  js[0] = JobSpec{c, plan, keep, reserved, out, info, slot_out, -1};
+  return run_batch(c, js, t0);
 }
 
 /* the picture a slot names (the reference's gather rule,
  * cpu_backend.cpp:557-587) */
 const uint16_t* gather(Ctx* c, int slot, Layout* L) {
   if (slot >= 0) {
-    if (slot >= kSlots || !c->used[slot]) {
+    if (slot >= c->pool_slots || !c->used[slot]) {
       set_err(c, AV1H_ERR_INVALID, "invalid reference slot");
       return nullptr;
     }
@@ -1419,17 +1719,6 @@ const uint16_t* gather(Ctx* c, int slot, Layout* L) {
   }
   *L = c->lay;
   return c->frame;
-}
-
-int ensure_pack(Ctx* c, size_t bytes) {
-  if (c->pack_cap < bytes) {
-    if (c->pack) (void)hipFree(c->pack);
-    c->pack = nullptr;
-    c->pack_cap = 0;
-    HIP_OK(c, hipMalloc(&c->pack, bytes));
-    c->pack_cap = bytes;
-  }
-  return AV1H_OK;
 }
 
 }  // namespace
@@ -1623,7 +1912,7 @@ int av1h_reserve_reference(av1h_ctx* h) {
 void av1h_release_reference(av1h_ctx* h, int slot) {
   if (!h) return;
   std::lock_guard<std::mutex> lk(h->c.mu);
-  if (slot >= 0 && slot < kSlots) h->c.used[slot] = false;
+  if (slot >= 0 && slot < h->c.pool_slots) h->c.used[slot] = false;
 }
 
 size_t av1h_nv12_size(int width, int height, int ss_x, int ss_y, size_t pitch) {
@@ -1648,13 +1937,173 @@ int av1h_to_nv12(av1h_ctx* h, int slot, size_t pitch, uint8_t* out, size_t out_b
   if (out && out_bytes < need) return set_err(c, AV1H_ERR_INVALID, "output buffer too small");
   int rc = ensure_pack(c, need);
   if (rc != AV1H_OK) return rc;
-  HIP_OK(c, launch_nv12(L, src, c->pack, (int)pitch, c->stream));
+  /* the decode packed this picture already */
+  const bool packed = c->nv12_current && pitch == c->nv12_pitch_done && src == c->frame;
+  if (!packed) {
+    c->nv12_current = false;
+    HIP_OK(c, launch_nv12(L, src, c->pack, (int)pitch, c->stream));
+  }
   if (out) HIP_OK(c, hipMemcpyAsync(out, c->pack, need, hipMemcpyDeviceToHost, c->stream));
   HIP_OK(c, hipStreamSynchronize(c->stream));
   return AV1H_OK;
 }
 
 const uint8_t* av1h_nv12_device_buffer(const av1h_ctx* h) { return h ? h->c.pack : nullptr; }
+
+int av1h_plan_sb_features(const void* plan, float* features) {
+  if (!plan) return AV1H_ERR_INVALID;
+  const Plan* p = static_cast<const Plan*>(plan);
+  if (features && !p->sb_feat.empty()) std::memcpy(features, p->sb_feat.data(), p->sb_feat.size() * sizeof(float));
+  return p->sb_w * p->sb_h;
+}
+
+int av1h_plan_sb_times(av1h_ctx* h, const void* plan, float* us) {
+  if (!h || !plan || !us) return AV1H_ERR_INVALID;
+  Ctx* c = &h->c;
+  std::lock_guard<std::mutex> lk(c->mu);
+  const Plan* p = static_cast<const Plan*>(plan);
+  const int nsb = p->sb_w * p->sb_h;
+  for (int i = 0; i < nsb; ++i) us[i] = 0.f;
+  if (c->trace_plan != plan || c->trace_n != (int)p->queue_host.size())
+    return set_err(c, AV1H_ERR_INVALID, "the last decode on this context was not a traced decode of this plan alone");
+  if (c->trace_n == 0) return nsb;
+  (void)hipSetDevice(c->device);
+  std::vector<uint64_t> t((size_t)c->trace_n * 4);
+  HIP_OK(c, hipMemcpy(t.data(), c->trace, t.size() * 8, hipMemcpyDeviceToHost));
+  int khz = 100000;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device) != hipSuccess || khz <= 0) khz = 100000;
+  for (int k = 0; k < c->trace_n; ++k) {
+    const double ticks = (double)(t[4 * (size_t)k + 2] - t[4 * (size_t)k + 1]);
+    us[p->queue_sb[(size_t)k]] += (float)(ticks * 1000.0 / khz);
+  }
+  return nsb;
+}
+
+namespace {
+/* Longest remaining path of every superblock through the superblock
+ * dependency graph (edges from left, above, above-left, above-right and
+ * intra block copy sources), each weighted by its cost; then the queue
+ * ordered by that path, longest first, coding order inside a superblock. */
+bool schedule_costs(Plan* p, const float* cost) {
+  const int sw = p->sb_w, sh = p->sb_h, n = sw * sh;
+  std::vector<std::vector<int>> succ((size_t)n);
+  std::vector<int> indeg((size_t)n, 0);
+  auto edge = [&](int a, int b) {
+    succ[(size_t)a].push_back(b);
+    ++indeg[(size_t)b];
+  };
+  for (int y = 0; y < sh; ++y)
+    for (int x = 0; x < sw; ++x) {
+      const int u = y * sw + x;
+      if (x > 0) edge(u - 1, u);
+      if (y > 0) edge(u - sw, u);
+      if (x > 0 && y > 0) edge(u - sw - 1, u);
+      if (x + 1 < sw && y > 0) edge(u - sw + 1, u);
+    }
+  for (size_t k = 0; k + 1 < p->bc_edges.size(); k += 2) edge(p->bc_edges[k], p->bc_edges[k + 1]);
+  std::vector<int> topo;
+  topo.reserve((size_t)n);
+  for (int u = 0; u < n; ++u)
+    if (!indeg[(size_t)u]) topo.push_back(u);
+  for (size_t i = 0; i < topo.size(); ++i)
+    for (int v : succ[(size_t)topo[i]])
+      if (--indeg[(size_t)v] == 0) topo.push_back(v);
+  if ((int)topo.size() != n) return false; /* not a DAG: keep the static order */
+  std::vector<double> path((size_t)n, 0.0);
+  for (size_t i = topo.size(); i-- > 0;) {
+    const int u = topo[i];
+    double best = 0.0;
+    for (int v : succ[(size_t)u]) best = std::max(best, path[(size_t)v]);
+    /* a floor keeps every dependency strictly ahead of what waits for it */
+    path[(size_t)u] = std::max((double)cost[u], 0.01) + best;
+  }
+  /* entries in coding order (block index), then stably by path */
+  std::vector<std::pair<int32_t, int32_t>> e(p->queue_static.size());
+  for (size_t k = 0; k < e.size(); ++k) e[k] = {p->queue_static[k], p->queue_sb_static[k]};
+  std::sort(e.begin(), e.end());
+  std::stable_sort(e.begin(), e.end(), [&](const std::pair<int32_t, int32_t>& a, const std::pair<int32_t, int32_t>& b) {
+    return path[(size_t)a.second] > path[(size_t)b.second];
+  });
+  /* keys for merging with other frames: remaining path in 0.25 us buckets */
+  constexpr double kBucketUs = 0.25;
+  const int top = e.empty() ? 0 : (int)(path[(size_t)e.front().second] / kBucketUs);
+  p->queue_host.resize(e.size());
+  p->key_off.clear();
+  for (size_t k = 0; k < e.size(); ++k) {
+    p->queue_host[k] = e[k].first;
+    const int key = top - (int)(path[(size_t)e[k].second] / kBucketUs);
+    while ((int)p->key_off.size() <= key) p->key_off.push_back((int32_t)k);
+  }
+  p->key_off.push_back((int32_t)e.size());
+  p->queue_sb.resize(e.size());
+  for (size_t k = 0; k < e.size(); ++k) p->queue_sb[k] = e[k].second;
+  return true;
+}
+
+int upload_queue(Ctx* c, Plan* p) {
+  if (p->queue_host.empty()) return AV1H_OK;
+  HIP_OK(c, hipMemcpy(const_cast<int32_t*>(p->dev.queue_list), p->queue_host.data(), p->queue_host.size() * 4,
+                      hipMemcpyHostToDevice));
+  return AV1H_OK;
+}
+}  // namespace
+
+int av1h_plan_schedule(av1h_ctx* h, void* plan, const float* sb_cost_us) {
+  if (!h || !plan) return AV1H_ERR_INVALID;
+  Ctx* c = &h->c;
+  std::lock_guard<std::mutex> lk(c->mu);
+  (void)hipSetDevice(c->device);
+  Plan* p = static_cast<Plan*>(plan);
+  if (!sb_cost_us) {
+    p->queue_host = p->queue_static;
+    p->key_off = p->key_off_static;
+    p->queue_sb = p->queue_sb_static;
+  } else if (!schedule_costs(p, sb_cost_us)) {
+    return set_err(c, AV1H_ERR_INVALID, "the superblock dependencies of this plan are cyclic");
+  }
+  return upload_queue(c, p);
+}
+
+int av1h_plan_schedule_model(av1h_ctx* h, void* plan, av1h_costmodel* model) {
+  if (!h || !plan || !model) return AV1H_ERR_INVALID;
+  Plan* p = static_cast<Plan*>(plan);
+  const int n = p->sb_w * p->sb_h;
+  std::vector<float> cost((size_t)n);
+  const int rc = av1h_cm_predict(model, p->sb_feat.data(), n, cost.data());
+  if (rc != AV1H_OK) return set_err(&h->c, rc, "cost model inference failed");
+  return av1h_plan_schedule(h, plan, cost.data());
+}
+
+int av1h_set_pool_slots(av1h_ctx* h, int slots) {
+  if (!h || slots < 1 || slots > kSlots) return AV1H_ERR_INVALID;
+  Ctx* c = &h->c;
+  std::lock_guard<std::mutex> lk(c->mu);
+  for (int i = 0; i < kSlots; ++i)
+    if (c->used[i]) return set_err(c, AV1H_ERR_INVALID, "the pool can be resized only while no slot is in use");
+  c->pool_slots = slots;
+  return AV1H_OK;
+}
+
+int64_t av1h_stream_bytes(int width, int height, int bit_depth, int ss_x, int ss_y, int pool_slots,
+                          size_t nv12_pitch) {
+  if (width <= 0 || height <= 0 || pool_slots < 1 || pool_slots > kSlots) return -1;
+  Layout L;
+  make_layout(&L, width, height, bit_depth, ss_x, ss_y, width);
+  const int64_t pic = L.size * 2;
+  int64_t b = pic * (5 + pool_slots);                                  /* work pictures + pool */
+  b += L.cells * 4 + (int64_t)(L.grid_w[0] >> 3) * (L.grid_h[0] >> 3) * 8; /* cells, CDEF directions */
+  b += 3 * 82 * 73 * 2 + 65536;                                          /* grain templates, offsets */
+  if (nv12_pitch) b += (int64_t)av1h_nv12_size(width, height, ss_x, ss_y, nv12_pitch);
+  return b;
+}
+
+int av1h_set_decode_nv12(av1h_ctx* h, size_t pitch) {
+  if (!h || (pitch & 1)) return AV1H_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(h->c.mu);
+  h->c.decode_nv12_pitch = pitch;
+  h->c.nv12_current = false;
+  return AV1H_OK;
+}
 
 int av1h_to_rgb(av1h_ctx* h, int slot, int matrix, int range, uint8_t* out, size_t out_bytes) {
   if (!h || !out) return AV1H_ERR_INVALID;
@@ -1727,7 +2176,7 @@ void av1h_reset(av1h_ctx* h) {
   c->last_slot = -1;
   c->have_frame = false;
   if (c->frame) (void)hipMemsetAsync(c->frame, 0, c->cap * 2, c->stream);
-  if (c->pool) (void)hipMemsetAsync(c->pool, 0, c->cap * 2 * kSlots, c->stream);
+  if (c->pool) (void)hipMemsetAsync(c->pool, 0, c->cap * 2 * c->pool_alloc, c->stream);
   (void)hipStreamSynchronize(c->stream);
   std::memset(&c->stats, 0, sizeof(c->stats));
 }
@@ -1784,13 +2233,68 @@ int64_t av1h_get_queue_trace(av1h_ctx* h, uint64_t* out, int64_t cap) {
   return n;
 }
 
-int av1h_decode_batch(av1h_ctx* const* ctxs, const av1h_frame_desc* const* frames, int count, int* slots_out) {
-  if (!ctxs || !frames || count < 0) return AV1H_ERR_INVALID;
-  for (int i = 0; i < count; ++i) {
-    const int rc = av1h_decode(ctxs[i], frames[i], 1, -1, nullptr, nullptr, slots_out ? &slots_out[i] : nullptr);
-    if (rc != AV1H_OK) return rc;
+}  // extern "C"
+
+namespace {
+/* locks the contexts of a batch in address order */
+struct BatchLock {
+  std::vector<std::unique_lock<std::mutex>> locks;
+  explicit BatchLock(std::vector<Ctx*> cs) {
+    std::sort(cs.begin(), cs.end());
+    cs.erase(std::unique(cs.begin(), cs.end()), cs.end());
+    for (Ctx* c : cs) locks.emplace_back(c->mu);
   }
+};
+
+int check_batch_args(av1h_ctx* const* ctxs, int count) {
+  if (!ctxs || count < 1 || count > kMaxBatch) return AV1H_ERR_INVALID;
+  for (int i = 0; i < count; ++i)
+    if (!ctxs[i]) return AV1H_ERR_INVALID;
   return AV1H_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int av1h_decode_batch(av1h_ctx* const* ctxs, const av1h_frame_desc* const* frames, int count, int* slots_out) {
+  int rc = check_batch_args(ctxs, count);
+  if (rc != AV1H_OK || !frames) return AV1H_ERR_INVALID;
+  std::vector<Ctx*> cs(count);
+  for (int i = 0; i < count; ++i) cs[i] = &ctxs[i]->c;
+  BatchLock lk(cs);
+  Ctx* lead = cs[0];
+  (void)hipSetDevice(lead->device);
+  const double t0 = now_ms();
+  std::vector<JobSpec> js(count);
+  for (int i = 0; i < count; ++i) {
+    if (!frames[i]) return set_err(lead, AV1H_ERR_INVALID, "frame %d of the batch is null", i);
+    for (int k = 0; k < i; ++k)
+      if (cs[k] == cs[i]) return set_err(lead, AV1H_ERR_INVALID, "a context appears twice in one batch");
+    Host host;
+    rc = flatten(cs[i], *frames[i], &cs[i]->scratch_plan, &host);
+    if (rc == AV1H_OK) rc = upload(cs[i], &cs[i]->scratch_plan);
+    if (rc != AV1H_OK) return cs[i] == lead ? rc : set_err(lead, rc, "frame %d: %s", i, cs[i]->err.c_str());
+    js[i] = JobSpec{cs[i], &cs[i]->scratch_plan, 1, -1, nullptr, nullptr, slots_out ? &slots_out[i] : nullptr, -1};
+  }
+  return run_batch(lead, js, t0);
+}
+
+int av1h_decode_planned_batch(av1h_ctx* const* ctxs, void* const* plans, const int* keep, const int* reserved,
+                              int count, int* slots_out) {
+  int rc = check_batch_args(ctxs, count);
+  if (rc != AV1H_OK || !plans) return AV1H_ERR_INVALID;
+  std::vector<Ctx*> cs(count);
+  for (int i = 0; i < count; ++i) cs[i] = &ctxs[i]->c;
+  BatchLock lk(cs);
+  Ctx* lead = cs[0];
+  (void)hipSetDevice(lead->device);
+  std::vector<JobSpec> js(count);
+  for (int i = 0; i < count; ++i) {
+    if (!plans[i]) return set_err(lead, AV1H_ERR_INVALID, "plan %d of the batch is null", i);
+    js[i] = JobSpec{cs[i], static_cast<Plan*>(plans[i]), keep ? keep[i] : 1, reserved ? reserved[i] : -1, nullptr,
+                    nullptr, slots_out ? &slots_out[i] : nullptr, -1};
+  }
+  return run_batch(lead, js, now_ms());
 }
 
 }  // extern "C"

@@ -106,12 +106,12 @@ __device__ void deblock_line(uint16_t* b, int s, int level, int sharp, int fs, i
 /* One thread per line; edges of one direction never share samples. The edge
  * owning line g is found from the first edge of the thread's 256 line chunk
  * (a table the host builds) and a short forward scan. */
-__global__ __launch_bounds__(256) void k_deblock(FilterArgs a, int first, int count, int lines,
-                                                 const int32_t* chunk) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ void deblock_chunk(FArgs a, int first, int count, int lines, const int32_t* chunk,
+                              int ci) {
+  const int g = ci * 256 + threadIdx.x;
   if (g >= lines) return;
   const DevDeblock* edges = a.plan.deblock + first;
-  int i = chunk[blockIdx.x];
+  int i = chunk[ci];
   while (i + 1 < count && edges[i + 1].line0 <= g) ++i;
   const DevDeblock& e = edges[i];
   if (e.level == 0) return;
@@ -128,6 +128,21 @@ __global__ __launch_bounds__(256) void k_deblock(FilterArgs a, int first, int co
     s = st;
   }
   deblock_line(b, s, e.level, a.sharpness, e.filter_size, p != 0, a.lay.bit_depth, a.maxv);
+}
+
+/* the vertical edges (pass 0) or the horizontal edges (pass 1) of every frame */
+__global__ __launch_bounds__(256) void k_deblock(BatchDev B, int pass) {
+  const AV1H_CMEM int32_t* pre = kernel_prefix(B, (pass ? kKDeblockH : kKDeblockV));
+  const int total = pre[B.njobs];
+  for (int v = blockIdx.x; v < total; v += gridDim.x) {
+    const int j = find_job(pre, B.njobs, v);
+    FArgs a = filter_args(B, j);
+    PlanRef P = a.plan;
+    if (pass)
+      deblock_chunk(a, P.n_deblock_v, P.n_deblock_h, P.lines_h, P.db_chunk + (P.lines_v + 255) / 256, v - pre[j]);
+    else
+      deblock_chunk(a, 0, P.n_deblock_v, P.lines_v, P.db_chunk, v - pre[j]);
+  }
 }
 
 /* ---- CDEF (av1gpu_kernels.h:2601-2775) ---------------------------------- */
@@ -152,12 +167,10 @@ __constant__ const int kCdefUvDir[2][2][8] = {{{0, 1, 2, 3, 4, 5, 6, 7}, {1, 2, 
  * per block, one lane per sample: the eight directional line sums are
  * accumulated with LDS atomics, lanes 0-7 weigh one direction each. Writes
  * (direction, variance) for the block into `dirmap`. */
-__global__ __launch_bounds__(256) void k_cdef_dir(FilterArgs a) {
-  __shared__ int s_part[4][8][16];
+__device__ void cdef_dir_group(FArgs a, int grp, int* part) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int bi = blockIdx.x * 4 + wave;
+  const int bi = grp * 4 + wave;
   const bool live = bi < a.plan.n_cdef_dir;
-  int* part = &s_part[wave][0][0];
   for (int i = lane; i < 128; i += 64) part[i] = 0;
   __syncthreads();
   int bx = 0, by = 0;
@@ -167,7 +180,7 @@ __global__ __launch_bounds__(256) void k_cdef_dir(FilterArgs a) {
     by = pos >> 16;
     const int i = lane >> 3, k = lane & 7;
     const int bd = a.lay.bit_depth;
-    const int x = (a.scratch[a.lay.plane_off[0] + (size_t)(by * 8 + i) * a.lay.stride[0] + bx * 8 + k] >> (bd - 8)) -
+    const int x = (a.cdef_src[a.lay.plane_off[0] + (size_t)(by * 8 + i) * a.lay.stride[0] + bx * 8 + k] >> (bd - 8)) -
                   128;
     atomicAdd(&part[0 * 16 + i + k], x);
     atomicAdd(&part[1 * 16 + i + k / 2], x);
@@ -219,15 +232,27 @@ __global__ __launch_bounds__(256) void k_cdef_dir(FilterArgs a) {
   }
 }
 
+__global__ __launch_bounds__(256) void k_cdef_dir(BatchDev B) {
+  __shared__ int s_part[4][8][16];
+  const AV1H_CMEM int32_t* pre = kernel_prefix(B, kKCdefDir);
+  const int total = pre[B.njobs];
+  int* part = &s_part[threadIdx.x >> 6][0][0];
+  for (int v = blockIdx.x; v < total; v += gridDim.x) {
+    const int j = find_job(pre, B.njobs, v);
+    cdef_dir_group(filter_args(B, j), v - pre[j], part);
+    __syncthreads();
+  }
+}
+
 /* CDEF of the listed blocks (av1gpu_kernels.h:2662-2775), one thread per
  * sample, reading the deblocked snapshot; blocks of `S` samples, 256 / S per
  * workgroup. Samples of unlisted blocks are already final in the frame. */
 template <int S>
-__global__ __launch_bounds__(256) void k_cdef(FilterArgs a, int first, int count) {
-  const int bi = blockIdx.x * (256 / S) + threadIdx.x / S;
+__device__ __forceinline__ void cdef_group(FArgs a, int first, int count, int grp) {
+  const int bi = grp * (256 / S) + threadIdx.x / S;
   if (bi >= count) return;
   const DevCdef c = a.plan.cdef[first + bi];
-  const Layout& L = a.lay;
+  LayRef L = a.lay;
   const int p = c.plane, uv = p > 0;
   const int bd = L.bit_depth, cs = bd - 8;
   const int st = L.stride[p];
@@ -252,7 +277,7 @@ __global__ __launch_bounds__(256) void k_cdef(FilterArgs a, int first, int count
     }
   }
   if (pri == 0 && sec == 0) return; /* the frame already holds the snapshot's value */
-  const uint16_t* src = a.scratch + L.plane_off[p] + (size_t)(c.y + r) * st + c.x + k;
+  const uint16_t* src = a.cdef_src + L.plane_off[p] + (size_t)(c.y + r) * st + c.x + k;
   const int ts = (pri >> cs) & 1;
   const int pt0 = ts ? 3 : 4, pt1 = ts ? 3 : 2;
   const int x = src[0];
@@ -288,34 +313,53 @@ __global__ __launch_bounds__(256) void k_cdef(FilterArgs a, int first, int count
       (uint16_t)clampi(clampi(x + ((8 + sum - (sum < 0)) >> 4), lo, hi), 0, a.maxv);
 }
 
-/* ---- superres (av1gpu_kernels.h:2387-2402) ------------------------------ */
+/* luma blocks (plane 0) or chroma blocks (plane 1) of every frame; a chroma
+ * block has 16, 32 or 64 samples by the frame's subsampling */
+__global__ __launch_bounds__(256) void k_cdef(BatchDev B, int chroma) {
+  const AV1H_CMEM int32_t* pre = kernel_prefix(B, (chroma ? kKCdefChroma : kKCdefLuma));
+  const int total = pre[B.njobs];
+  for (int v = blockIdx.x; v < total; v += gridDim.x) {
+    const int j = find_job(pre, B.njobs, v);
+    FArgs a = filter_args(B, j);
+    const int nl = a.plan.n_cdef_luma, grp = v - pre[j];
+    if (!chroma) {
+      cdef_group<64>(a, 0, nl, grp);
+      continue;
+    }
+    const int nc = a.plan.n_cdef - nl;
+    switch ((8 >> a.lay.ss_x) * (8 >> a.lay.ss_y)) {
+      case 16: cdef_group<16>(a, nl, nc, grp); break;
+      case 32: cdef_group<32>(a, nl, nc, grp); break;
+      default: cdef_group<64>(a, nl, nc, grp); break;
+    }
+  }
+}
 
-struct UpscalePlane {
-  int step[3], initial[3];
-};
+/* ---- superres (av1gpu_kernels.h:2387-2402) ------------------------------ */
 
 /* 256 x 4 output samples of one plane of one picture per workgroup: the
  * source span of the 256 columns (at most 264 samples when upscaling) is
  * staged in LDS per row, clamped to the coded width as each tap is
- * (av1gpu_kernels.h:2387-2402). blockIdx.z: picture * 3 + plane. */
+ * (av1gpu_kernels.h:2387-2402). Picture 0 is the frame (from its copy in
+ * `scratch`), picture 1 the deblocked picture loop restoration reads
+ * (lr_curr into lr_prev_up). */
 constexpr int kUpW = 256, kUpRows = 4, kUpSpan = 272;
 
-__global__ __launch_bounds__(256) void k_upscale(FilterArgs a, UpscalePlane u, const uint16_t* src0,
-                                                 uint16_t* dst0, const uint16_t* src1, uint16_t* dst1) {
-  __shared__ int s_k[512];
-  __shared__ uint16_t s_row[kUpRows][kUpSpan];
-  const int pic = blockIdx.z / 3, p = blockIdx.z - 3 * pic;
-  const uint16_t* src = pic ? src1 : src0;
-  uint16_t* dst = pic ? dst1 : dst0;
-  const Layout& L = a.lay;
-  const int ow = p == 0 ? L.out_w[0] : L.out_w[1], oh = p == 0 ? L.out_h[0] : L.out_h[1];
-  const int x0 = blockIdx.x * kUpW, y0 = blockIdx.y * kUpRows;
+__device__ void upscale_tile(FArgs a, int t, const int* s_k, uint16_t (*s_row)[kUpSpan]) {
+  LayRef L = a.lay;
+  const int tx = (L.out_w[0] + kUpW - 1) / kUpW, ty = (L.out_h[0] + kUpRows - 1) / kUpRows;
+  const int z = t / (tx * ty), rem = t - z * tx * ty;
+  const int by = rem / tx, bx = rem - by * tx;
+  const int pic = z / 3, p = z - 3 * pic;
+  const uint16_t* src = pic ? a.lr_curr : a.scratch;
+  uint16_t* dst = pic ? a.lr_prev_up : a.frame;
+  const int ow = L.out_w[p], oh = L.out_h[p];
+  const int x0 = bx * kUpW, y0 = by * kUpRows;
   if (x0 >= ow || y0 >= oh) return;
-  for (int i = threadIdx.x; i < 512; i += 256) s_k[i] = av1t_spec_table[AV1T_OFF_UPSCALE + i];
-  const int step = u.step[p], init = u.initial[p];
-  const int maxx = (p == 0 ? L.plane_w[0] : L.plane_w[1]) - 1;
-  const int stride = p == 0 ? L.stride[0] : L.stride[1];
-  const int64_t poff = p == 0 ? L.plane_off[0] : (p == 1 ? L.plane_off[1] : L.plane_off[2]);
+  const int step = a.up_step[p], init = a.up_initial[p];
+  const int maxx = L.plane_w[p] - 1;
+  const int stride = L.stride[p];
+  const int64_t poff = L.plane_off[p];
   const int xl = min(x0 + kUpW, ow) - 1;
   const int ws = ((-(1 << 14) + init + x0 * step) >> 14) - 3;
   const int wn = ((-(1 << 14) + init + xl * step) >> 14) + 4 - ws + 1;
@@ -333,8 +377,22 @@ __global__ __launch_bounds__(256) void k_upscale(FilterArgs a, UpscalePlane u, c
   for (int r = 0; r < rows; ++r) {
     int sum = 0;
 #pragma unroll
-    for (int t = 0; t < 8; ++t) sum += s_row[r][o + t] * k[t];
+    for (int q = 0; q < 8; ++q) sum += s_row[r][o + q] * k[q];
     dst[poff + (size_t)(y0 + r) * stride + x] = (uint16_t)clampi((sum + 64) >> 7, 0, a.maxv);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_upscale(BatchDev B) {
+  __shared__ int s_k[512];
+  __shared__ uint16_t s_row[kUpRows][kUpSpan];
+  for (int i = threadIdx.x; i < 512; i += 256) s_k[i] = av1t_spec_table[AV1T_OFF_UPSCALE + i];
+  __syncthreads();
+  const AV1H_CMEM int32_t* pre = kernel_prefix(B, kKUpscale);
+  const int total = pre[B.njobs];
+  for (int v = blockIdx.x; v < total; v += gridDim.x) {
+    const int j = find_job(pre, B.njobs, v);
+    upscale_tile(filter_args(B, j), v - pre[j], s_k, s_row);
+    __syncthreads();
   }
 }
 
@@ -367,14 +425,14 @@ constexpr int kLrTileW = kLrTileCols, kLrTileH = kLrTileRows;
 constexpr int kLrSrcW = kLrTileW + 6, kLrSrcH = kLrTileH + 6;
 constexpr int kLrAbW = kLrTileW + 2, kLrAbH = kLrTileH + 2;
 
-__global__ __launch_bounds__(256) void k_lr(FilterArgs a, const uint16_t* cdef, const uint16_t* curr) {
-  __shared__ int s_src[kLrSrcH * kLrSrcW];
-  __shared__ int s_mid[4 * kLrAbH * kLrAbW]; /* Wiener rows, or A0 B0 A1 B1 */
-  const int2 tile = a.plan.lr_tiles[blockIdx.x];
+__device__ void lr_tile(FArgs a, int t, int* s_src, int* s_mid) {
+  const uint16_t* cdef = a.lr_cdef;
+  const uint16_t* curr = a.upscaled ? a.lr_prev_up : a.lr_curr;
+  const int2 tile = a.plan.lr_tiles[t];
   const DevLr& j = a.plan.lr[tile.x];
   const int x0 = j.x + (tile.y & 0xffff), y0 = j.y + (tile.y >> 16);
   const int tw = min(kLrTileW, j.x + j.w - x0), th = min(kLrTileH, j.y + j.h - y0);
-  const Layout& L = a.lay;
+  LayRef L = a.lay;
   const int p = j.plane, bd = L.bit_depth;
   LrCtx c;
   c.cdef = cdef + L.plane_off[p];
@@ -493,6 +551,52 @@ __global__ __launch_bounds__(256) void k_lr(FilterArgs a, const uint16_t* cdef, 
   }
 }
 
+__global__ __launch_bounds__(256) void k_lr(BatchDev B) {
+  __shared__ int s_src[kLrSrcH * kLrSrcW];
+  __shared__ int s_mid[4 * kLrAbH * kLrAbW]; /* Wiener rows, or A0 B0 A1 B1 */
+  const AV1H_CMEM int32_t* pre = kernel_prefix(B, kKLr);
+  const int total = pre[B.njobs];
+  for (int v = blockIdx.x; v < total; v += gridDim.x) {
+    const int j = find_job(pre, B.njobs, v);
+    lr_tile(filter_args(B, j), v - pre[j], s_src, s_mid);
+    __syncthreads();
+  }
+}
+
+/* ---- whole picture copies (the snapshots the filters read, the reference
+ * slot store) ---------------------------------------------------------------- */
+
+__device__ __forceinline__ void copy_kind(FArgs a, int k, const uint4** src, uint4** dst) {
+  *src = reinterpret_cast<const uint4*>(a.frame);
+  switch (k) {
+    case kKCopyLrCurr: *dst = reinterpret_cast<uint4*>(a.lr_curr); break;
+    case kKCopyCdefSrc:
+    case kKCopyUpSrc: *dst = reinterpret_cast<uint4*>(a.scratch); break;
+    case kKCopyLrCdef: *dst = reinterpret_cast<uint4*>(a.lr_cdef); break;
+    default: *dst = reinterpret_cast<uint4*>(a.store); break;
+  }
+}
+
+/* kCopyChunk bytes per workgroup, 16 bytes per thread and step */
+__global__ __launch_bounds__(256) void k_copy(BatchDev B, int k) {
+  const AV1H_CMEM int32_t* pre = kernel_prefix(B, k);
+  const int total = pre[B.njobs];
+  for (int v = blockIdx.x; v < total; v += gridDim.x) {
+    const int j = find_job(pre, B.njobs, v);
+    FArgs a = filter_args(B, j);
+    const uint4* src;
+    uint4* dst;
+    copy_kind(a, k, &src, &dst);
+    const int64_t n = a.lay.size * 2 / 16;
+    const int64_t base = (int64_t)(v - pre[j]) * (kCopyChunk / 16);
+#pragma unroll
+    for (int i = 0; i < kCopyChunk / 16 / 256; ++i) {
+      const int64_t e = base + i * 256 + threadIdx.x;
+      if (e < n) dst[e] = src[e];
+    }
+  }
+}
+
 /* ---- NV12 and RGB packs (av1gpu_kernels.h:3379-3472) -------------------- */
 
 /* one thread per output byte pair */
@@ -521,6 +625,49 @@ __global__ void k_nv12(Layout L, const uint16_t* src, uint8_t* out, int pitch) {
   }
 }
 
+/* NV12 of each frame of a batch that asks for it, after film grain: 1024
+ * output bytes of one row per workgroup, 4 per thread */
+__device__ void nv12_chunk(FArgs a, int t) {
+  LayRef L = a.lay;
+  const int pitch = a.nv12_pitch;
+  const int cpr = (pitch + 1023) >> 10;
+  const int row = t / cpr;
+  const int x0 = (t - row * cpr) * 1024 + threadIdx.x * 4;
+  if (x0 >= pitch) return;
+  const int sh = L.bit_depth - 8, w = L.width, h = L.height;
+  uint32_t word = 0;
+  if (row < h) {
+    const uint16_t* src = a.frame + L.plane_off[0] + (size_t)row * L.stride[0];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (x0 + k < w) word |= (uint32_t)(src[x0 + k] >> sh) << (8 * k);
+  } else {
+    const int cr = row - h, cw = (w + (1 << L.ss_x) - 1) >> L.ss_x;
+    const uint16_t* u = a.frame + L.plane_off[1] + (size_t)cr * L.stride[1];
+    const uint16_t* v = a.frame + L.plane_off[2] + (size_t)cr * L.stride[2];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int c = (x0 + k) >> 1;
+      if (c < cw) word |= (uint32_t)(((x0 + k) & 1 ? v[c] : u[c]) >> sh) << (8 * k);
+    }
+  }
+  uint8_t* d = a.nv12 + (size_t)row * pitch + x0;
+  if ((pitch & 3) == 0 && x0 + 4 <= pitch) {
+    *reinterpret_cast<uint32_t*>(d) = word;
+  } else {
+    for (int k = 0; k < 4 && x0 + k < pitch; ++k) d[k] = (uint8_t)(word >> (8 * k));
+  }
+}
+
+__global__ __launch_bounds__(256) void k_nv12_batch(BatchDev B) {
+  const AV1H_CMEM int32_t* pre = kernel_prefix(B, kKNv12);
+  const int total = pre[B.njobs];
+  for (int v = blockIdx.x; v < total; v += gridDim.x) {
+    const int j = find_job(pre, B.njobs, v);
+    nv12_chunk(filter_args(B, j), v - pre[j]);
+  }
+}
+
 struct ColorM {
   int m[8];
   int o[3];
@@ -544,44 +691,6 @@ __global__ void k_rgb(Layout L, const uint16_t* src, uint8_t* out, ColorM cm) {
 
 }  // namespace
 
-hipError_t launch_deblock(const FilterArgs& a, hipStream_t s, int* launches) {
-  if (a.plan.n_deblock_v > 0 && a.plan.lines_v > 0) {
-    hipLaunchKernelGGL(k_deblock, dim3((a.plan.lines_v + 255) / 256), dim3(256), 0, s, a, 0,
-                       a.plan.n_deblock_v, a.plan.lines_v, a.plan.db_chunk);
-    ++*launches;
-  }
-  if (a.plan.n_deblock_h > 0 && a.plan.lines_h > 0) {
-    hipLaunchKernelGGL(k_deblock, dim3((a.plan.lines_h + 255) / 256), dim3(256), 0, s, a,
-                       a.plan.n_deblock_v, a.plan.n_deblock_h, a.plan.lines_h,
-                       a.plan.db_chunk + (a.plan.lines_v + 255) / 256);
-    ++*launches;
-  }
-  return hipGetLastError();
-}
-
-hipError_t launch_cdef(const FilterArgs& a, hipStream_t s, int* launches) {
-  if (a.plan.n_cdef <= 0) return hipSuccess;
-  if (a.plan.n_cdef_dir > 0) {
-    hipLaunchKernelGGL(k_cdef_dir, dim3((a.plan.n_cdef_dir + 3) / 4), dim3(256), 0, s, a);
-    ++*launches;
-  }
-  const int nl = a.plan.n_cdef_luma, nc = a.plan.n_cdef - nl;
-  if (nl > 0) {
-    hipLaunchKernelGGL(k_cdef<64>, dim3((nl + 3) / 4), dim3(256), 0, s, a, 0, nl);
-    ++*launches;
-  }
-  if (nc > 0) {
-    const int cs = (8 >> a.lay.ss_x) * (8 >> a.lay.ss_y);
-    switch (cs) {
-      case 16: hipLaunchKernelGGL(k_cdef<16>, dim3((nc + 15) / 16), dim3(256), 0, s, a, nl, nc); break;
-      case 32: hipLaunchKernelGGL(k_cdef<32>, dim3((nc + 7) / 8), dim3(256), 0, s, a, nl, nc); break;
-      default: hipLaunchKernelGGL(k_cdef<64>, dim3((nc + 3) / 4), dim3(256), 0, s, a, nl, nc); break;
-    }
-    ++*launches;
-  }
-  return hipGetLastError();
-}
-
 void superres_params(int dw, int uw, int* step, int* initial) {
   /* av1gpu_kernels.h:2375-2383 */
   const int st = ((dw << 14) + (uw / 2)) / uw;
@@ -591,23 +700,32 @@ void superres_params(int dw, int uw, int* step, int* initial) {
   *initial = sub & ((1 << 14) - 1);
 }
 
-hipError_t launch_superres(const FilterArgs& a, const uint16_t* src, uint16_t* dst, const uint16_t* src2,
-                           uint16_t* dst2, hipStream_t s, int* launches) {
-  UpscalePlane u;
-  for (int p = 0; p < 3; ++p) superres_params(a.lay.plane_w[p], a.lay.out_w[p], &u.step[p], &u.initial[p]);
-  const int pics = src2 ? 2 : 1;
-  hipLaunchKernelGGL(k_upscale, dim3((a.lay.out_w[0] + kUpW - 1) / kUpW, (a.lay.out_h[0] + kUpRows - 1) / kUpRows,
-                                     3 * pics),
-                     dim3(256), 0, s, a, u, src, dst, src2 ? src2 : src, dst2 ? dst2 : dst);
-  ++*launches;
-  return hipGetLastError();
+int upscale_tiles(const Layout& L, int pictures) {
+  return (L.out_w[0] + kUpW - 1) / kUpW * ((L.out_h[0] + kUpRows - 1) / kUpRows) * 3 * pictures;
 }
 
-hipError_t launch_lr(const FilterArgs& a, const uint16_t* cdef, const uint16_t* curr, hipStream_t s,
-                     int* launches) {
-  if (a.plan.n_lr_tiles <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_lr, dim3(a.plan.n_lr_tiles), dim3(256), 0, s, a, cdef, curr);
-  ++*launches;
+hipError_t launch_filter_kernel(const BatchDev& b, int k, int total, int cus, hipStream_t s) {
+  const dim3 blk(256);
+  switch (k) {
+    case kKDeblockV:
+    case kKDeblockH:
+      hipLaunchKernelGGL(k_deblock, dim3(min(total, cus * 16)), blk, 0, s, b, k == kKDeblockH ? 1 : 0);
+      break;
+    case kKCdefDir: hipLaunchKernelGGL(k_cdef_dir, dim3(min(total, cus * 16)), blk, 0, s, b); break;
+    case kKCdefLuma:
+    case kKCdefChroma:
+      hipLaunchKernelGGL(k_cdef, dim3(min(total, cus * 16)), blk, 0, s, b, k == kKCdefChroma ? 1 : 0);
+      break;
+    case kKUpscale: hipLaunchKernelGGL(k_upscale, dim3(min(total, cus * 8)), blk, 0, s, b); break;
+    case kKLr: hipLaunchKernelGGL(k_lr, dim3(min(total, cus * 8)), blk, 0, s, b); break;
+    case kKNv12: hipLaunchKernelGGL(k_nv12_batch, dim3(min(total, cus * 16)), blk, 0, s, b); break;
+    case kKCopyLrCurr:
+    case kKCopyCdefSrc:
+    case kKCopyUpSrc:
+    case kKCopyLrCdef:
+    case kKCopyStore: hipLaunchKernelGGL(k_copy, dim3(min(total, cus * 8)), blk, 0, s, b, k); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -41,10 +41,6 @@ __device__ __forceinline__ unsigned mat_apply(const uint16_t* cols, unsigned v) 
   return r;
 }
 
-struct Jump {
-  uint16_t m82[16], m44[16];
-};
-
 /* white noise row `y` of a w wide template (av1gpu_kernels.h:3070-3079) */
 __device__ void noise_row(int16_t* t, int w, int y, unsigned seed, const uint16_t* jump, int shift) {
   unsigned st = seed;
@@ -122,11 +118,8 @@ constexpr int kTmplThreads = 192; /* three waves: luma, cb, cr */
  * Phases: white noise (one row per thread); luma filter on wave 0 while
  * waves 1-2 draw the offsets; cb on wave 1 and cr on wave 2 (both read the
  * finished luma template) while wave 0 stores luma. */
-__global__ __launch_bounds__(kTmplThreads) void k_grain_template(const DevGrain* gp, Jump jmp, int ss_x, int ss_y,
-                                                                 int16_t* tmpl, uint8_t* offs) {
-  __shared__ int16_t s_l[kLW * kLH];
-  __shared__ int16_t s_c[2][kLW * kLH];
-  const DevGrain& g = *gp;
+__device__ void grain_template(const DevGrain& g, const uint16_t* jm82, const uint16_t* jm44, int ss_x, int ss_y,
+                               int16_t* tmpl, uint8_t* offs, int16_t* s_l, int16_t (*s_c)[kLW * kLH]) {
   const int tid = threadIdx.x, wave = tid >> 6;
   const int bd = g.bit_depth;
   const int lo = -(128 << (bd - 8)), hi = (256 << (bd - 8)) - 1 - (128 << (bd - 8));
@@ -134,11 +127,11 @@ __global__ __launch_bounds__(kTmplThreads) void k_grain_template(const DevGrain*
   const int chroma = g.cfl || g.num_cb > 0 || g.num_cr > 0;
   const unsigned seed = (unsigned)g.seed;
   /* white noise (av1gpu_kernels.h:3070-3079), one row per thread */
-  const uint16_t* jc = ss_x ? jmp.m44 : jmp.m82;
+  const uint16_t* jc = ss_x ? jm44 : jm82;
   for (int r = tid; r < kLH + 2 * ch; r += kTmplThreads) {
     if (r < kLH) {
       if (g.num_y > 0)
-        noise_row(s_l, kLW, r, seed, jmp.m82, g.scale_shift);
+        noise_row(s_l, kLW, r, seed, jm82, g.scale_shift);
       else
         for (int x = 0; x < kLW; ++x) s_l[r * kLW + x] = 0;
     } else if (chroma) {
@@ -172,6 +165,19 @@ __global__ __launch_bounds__(kTmplThreads) void k_grain_template(const DevGrain*
     ar_dispatch(g.lag, s_c[pl], cw, ch, pl ? g.ar_cr : g.ar_cb, g.num_y > 0 ? s_l : nullptr, ss_x, ss_y, g.shift,
                 lo, hi);
     for (int i = tid & 63; i < cw * ch; i += 64) tmpl[kLW * kLH + pl * cw * ch + i] = s_c[pl][i];
+  }
+}
+
+/* one workgroup per frame with film grain */
+__global__ __launch_bounds__(kTmplThreads) void k_grain_template(BatchDev B) {
+  __shared__ int16_t s_l[kLW * kLH];
+  __shared__ int16_t s_c[2][kLW * kLH];
+  const AV1H_CMEM int32_t* pre = kernel_prefix(B, kKGrainTemplate);
+  const int total = pre[B.njobs];
+  for (int v = blockIdx.x; v < total; v += gridDim.x) {
+    FArgs a = filter_args(B, find_job(pre, B.njobs, v));
+    grain_template(*a.plan.grain, B.jump82, B.jump44, a.lay.ss_x, a.lay.ss_y, a.grain_tmpl, a.grain_offs, s_l, s_c);
+    __syncthreads();
   }
 }
 
@@ -218,13 +224,17 @@ __device__ __forceinline__ int scale_lut(const int* lut, int index, int bd) {
 }
 
 /* 64x32 luma tiles (32x16 chroma at 4:2:0), 256 threads */
-__global__ __launch_bounds__(256) void k_grain_apply(Layout L, const DevGrain* gp, const int16_t* tmpl,
-                                                      const uint8_t* offs, uint16_t* frame) {
-  const DevGrain& g = *gp;
+__device__ void grain_tile(FArgs a, int t) {
+  LayRef L = a.lay;
+  const DevGrain& g = *a.plan.grain;
+  const int16_t* tmpl = a.grain_tmpl;
+  const uint8_t* offs = a.grain_offs;
+  uint16_t* frame = a.frame;
   const int bd = L.bit_depth;
   const int maxv = (1 << bd) - 1;
   const int lo = -(128 << (bd - 8)), hi = (256 << (bd - 8)) - 1 - (128 << (bd - 8));
-  const int tx0 = blockIdx.x * 64, ty0 = blockIdx.y * 32;
+  const int ntx = (L.out_w[0] + 63) / 64;
+  const int tx0 = (t % ntx) * 64, ty0 = (t / ntx) * 32;
   const int cw = L.ss_x ? 44 : kLW, ch = L.ss_y ? 38 : kLH;
   const int chroma = g.cfl || g.num_cb > 0 || g.num_cr > 0;
   const uint16_t* luma = frame + L.plane_off[0];
@@ -277,6 +287,16 @@ __global__ __launch_bounds__(256) void k_grain_apply(Layout L, const DevGrain* g
   }
 }
 
+__global__ __launch_bounds__(256) void k_grain_apply(BatchDev B) {
+  const AV1H_CMEM int32_t* pre = kernel_prefix(B, kKGrainApply);
+  const int total = pre[B.njobs];
+  for (int v = blockIdx.x; v < total; v += gridDim.x) {
+    const int j = find_job(pre, B.njobs, v);
+    grain_tile(filter_args(B, j), v - pre[j]);
+    __syncthreads();
+  }
+}
+
 unsigned host_step(unsigned r) {
   const unsigned bit = (r ^ (r >> 1) ^ (r >> 3) ^ (r >> 12)) & 1u;
   return (r >> 1) | (bit << 15);
@@ -294,21 +314,14 @@ void grain_jump_matrices(uint16_t m82[16], uint16_t m44[16]) {
   }
 }
 
-hipError_t launch_grain_template(const Layout& lay, const DevGrain* g, int16_t* templates, uint8_t* offsets,
-                                 hipStream_t s, int* launches) {
-  Jump j;
-  grain_jump_matrices(j.m82, j.m44);
-  hipLaunchKernelGGL(k_grain_template, dim3(1), dim3(kTmplThreads), 0, s, g, j, lay.ss_x, lay.ss_y, templates,
-                     offsets);
-  ++*launches;
-  return hipGetLastError();
-}
+int grain_tiles(const Layout& lay) { return (lay.out_w[0] + 63) / 64 * ((lay.out_h[0] + 31) / 32); }
 
-hipError_t launch_grain_apply(const Layout& lay, const DevGrain* g, uint16_t* frame, const int16_t* templates,
-                              const uint8_t* offsets, hipStream_t s, int* launches) {
-  hipLaunchKernelGGL(k_grain_apply, dim3((lay.out_w[0] + 63) / 64, (lay.out_h[0] + 31) / 32), dim3(256), 0, s,
-                     lay, g, templates, offsets, frame);
-  ++*launches;
+hipError_t launch_grain_kernel(const BatchDev& b, int k, int total, int cus, hipStream_t s) {
+  switch (k) {
+    case kKGrainTemplate: hipLaunchKernelGGL(k_grain_template, dim3(min(total, cus)), dim3(kTmplThreads), 0, s, b); break;
+    case kKGrainApply: hipLaunchKernelGGL(k_grain_apply, dim3(min(total, cus * 16)), dim3(256), 0, s, b); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -192,7 +192,9 @@ struct FramePlanDev {
   int32_t n_lr_tiles;
 };
 
-/* launchers (one per translation unit, all on the context's stream) */
+/* ---- one decode: the frames of a batch ---------------------------------- */
+
+/* What the reconstruction kernels read of one frame. */
 struct ReconArgs {
   FramePlanDev plan;
   Layout lay;
@@ -201,26 +203,12 @@ struct ReconArgs {
   int64_t slot_size;
   uint32_t* cells;      /* epoch per dependency cell          */
   uint32_t epoch;
-  uint32_t* queue_counter;
-  uint32_t* error;      /* bit 0: a dependency wait timed out  */
   int16_t* res;         /* residual of intra blocks, frame layout */
-  uint64_t* trace;      /* null, or 4 timestamps per queue entry (diagnostics) */
   int maxv;
   int edge_filter;      /* enable_intra_edge_filter           */
 };
 
-/* motion compensation of every inter block into the frame */
-hipError_t launch_mc(const ReconArgs& a, int cus, hipStream_t s, int* launches);
-/* the inter blocks with warp, masks or OBMC, after motion compensation */
-hipError_t launch_special(const ReconArgs& a, hipStream_t s, int* launches);
-/* the transforms of one set: 0 inter blocks (residual added into the frame,
- * after motion compensation), 1 intra blocks (residual into `res`) */
-hipError_t launch_itx(const ReconArgs& a, int set, hipStream_t s, int* launches);
-/* the dependency queue over the intra blocks */
-hipError_t launch_queue(const ReconArgs& a, int cus, hipStream_t s, int* launches);
-/* workgroups a transform set launches (0: nothing to do) */
-int itx_groups(const FramePlanDev& p, int set);
-
+/* What the filter kernels read of one frame. */
 struct FilterArgs {
   FramePlanDev plan;
   Layout lay;
@@ -229,28 +217,110 @@ struct FilterArgs {
   uint16_t* lr_curr;
   uint16_t* lr_cdef;
   uint16_t* lr_prev_up;
+  const uint16_t* cdef_src; /* the deblocked picture CDEF reads: lr_curr or scratch */
   int32_t* cdef_dir;    /* (direction, variance) per luma 8x8 block */
   int cdef_dir_w;       /* luma 8x8 blocks per row                  */
   int sharpness;
   int maxv;
+  int32_t up_step[3], up_initial[3]; /* superres per plane          */
+  int upscaled, uses_lr, has_grain;
+  int16_t* grain_tmpl;
+  uint8_t* grain_offs;
+  uint16_t* store;      /* reference slot to store into, or null  */
+  uint8_t* nv12;        /* NV12 output on the device, or null     */
+  int32_t nv12_pitch;
 };
 
-hipError_t launch_deblock(const FilterArgs& a, hipStream_t s, int* launches);
-hipError_t launch_cdef(const FilterArgs& a, hipStream_t s, int* launches);
-/* upscales src into dst, and src2 into dst2 when src2 is not null, in one launch */
-hipError_t launch_superres(const FilterArgs& a, const uint16_t* src, uint16_t* dst, const uint16_t* src2,
-                           uint16_t* dst2, hipStream_t s, int* launches);
-hipError_t launch_lr(const FilterArgs& a, const uint16_t* cdef, const uint16_t* curr, hipStream_t s,
-                     int* launches);
-/* the three grain templates and the block offsets (one workgroup; depends only
- * on the grain parameters), then the blend into the picture */
-hipError_t launch_grain_template(const Layout& lay, const DevGrain* g, int16_t* templates, uint8_t* offsets,
-                                 hipStream_t s, int* launches);
-hipError_t launch_grain_apply(const Layout& lay, const DevGrain* g, uint16_t* frame, const int16_t* templates,
-                              const uint8_t* offsets, hipStream_t s, int* launches);
+struct FrameJob {
+  ReconArgs r;
+  FilterArgs f;
+};
+
+/* Every kernel runs the work of all frames of a batch in one launch: its
+ * workgroups walk "virtual workgroups", numbered frame after frame, and find
+ * their frame in a prefix table (kernel k, frame j: prefix[k * (jobs + 1) + j]). */
+enum KernelId {
+  kKMc, kKSpecial, kKItxInter, kKItxIntra, kKDeblockV, kKDeblockH, kKCdefDir, kKCdefLuma, kKCdefChroma,
+  kKUpscale, kKLr, kKGrainTemplate, kKGrainApply, kKNv12,
+  kKCopyLrCurr, kKCopyCdefSrc, kKCopyUpSrc, kKCopyLrCdef, kKCopyStore,
+  kKernels
+};
+
+/* The frame table, the prefix table and the plan descriptors are written
+ * before a launch and only read while kernels run: read through the constant
+ * address space, the reads the compiler sees are uniform become scalar loads
+ * that stay valid across the kernels' stores. */
+#define AV1H_CMEM __attribute__((address_space(4)))
+template <class T>
+__device__ __forceinline__ const AV1H_CMEM T* cmem(const T* p) {
+  return (const AV1H_CMEM T*)p;
+}
+
+/* the frame a virtual workgroup belongs to: the last j < n with pre[j] <= v */
+__device__ __forceinline__ int find_job(const AV1H_CMEM int32_t* pre, int n, int v) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int m = (lo + hi + 1) >> 1;
+    if (pre[m] <= v)
+      lo = m;
+    else
+      hi = m - 1;
+  }
+  return lo;
+}
+
+/* Queue entries of a batch: frame << kQueueJobShift | block. A batch of one
+ * frame hands out the frame's own queue list. */
+constexpr int kQueueJobShift = 24;
+constexpr int kMaxBatch = 128;
+
+/* bytes one copy workgroup moves */
+constexpr int kCopyChunk = 16384;
+
+struct BatchDev {
+  const FrameJob* jobs;
+  const int32_t* prefix;
+  int32_t njobs;
+  const uint32_t* queue;     /* intra blocks of all frames in hand-out order, or null */
+  int32_t nqueue;
+  uint32_t* queue_counter;
+  uint32_t* error;           /* bit 0: a dependency wait timed out   */
+  uint64_t* trace;           /* null, or 4 timestamps per queue entry */
+  uint16_t jump82[16], jump44[16]; /* grain LFSR jump matrices        */
+};
+
+typedef const AV1H_CMEM ReconArgs& RArgs;
+typedef const AV1H_CMEM FilterArgs& FArgs;
+typedef const AV1H_CMEM Layout& LayRef;
+typedef const AV1H_CMEM FramePlanDev& PlanRef;
+typedef const AV1H_CMEM DevBlock& BlkRef;
+
+/* kernel k's row of the prefix table */
+__device__ __forceinline__ const AV1H_CMEM int32_t* kernel_prefix(const BatchDev& B, int k) {
+  return cmem(B.prefix) + k * (B.njobs + 1);
+}
+__device__ __forceinline__ RArgs recon_args(const BatchDev& B, int j) { return cmem(B.jobs)[j].r; }
+__device__ __forceinline__ FArgs filter_args(const BatchDev& B, int j) { return cmem(B.jobs)[j].f; }
+
+/* Kernel k over every frame of the batch, `total` virtual workgroups; `cus`
+ * caps the grid, whose workgroups then loop. By file: */
+hipError_t launch_recon_kernel(const BatchDev& b, int k, int total, int cus, hipStream_t s);
+hipError_t launch_filter_kernel(const BatchDev& b, int k, int total, int cus, hipStream_t s);
+hipError_t launch_grain_kernel(const BatchDev& b, int k, int total, int cus, hipStream_t s);
+/* the dependency queue over the intra blocks of the batch */
+hipError_t launch_queue(const BatchDev& b, int cus, hipStream_t s, int* launches);
+/* superres: step and initial subpixel position of a plane */
+void superres_params(int dw, int uw, int* step, int* initial);
+/* upscaling workgroups of one frame (1 or 2 pictures) */
+int upscale_tiles(const Layout& lay, int pictures);
+/* grain blend workgroups of one frame */
+int grain_tiles(const Layout& lay);
+/* picture conversions outside a decode */
 hipError_t launch_nv12(const Layout& lay, const uint16_t* src, uint8_t* out, int pitch, hipStream_t s);
 hipError_t launch_rgb(const Layout& lay, const uint16_t* src, uint8_t* out, const int m[8],
                       const int o[3], hipStream_t s);
+/* transform workgroups of one set of a frame (0 inter, 1 intra) */
+int itx_groups(const FramePlanDev& p, int set);
 
 /* film grain: the LFSR jump matrices the template kernel uses */
 void grain_jump_matrices(uint16_t m82[16], uint16_t m44[16]);

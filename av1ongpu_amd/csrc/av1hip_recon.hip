@@ -153,7 +153,7 @@ __device__ __forceinline__ void col_pass_n(int l2, const int* tmp, int lane, int
  * (its prediction is already there); that of an intra block goes to `res`,
  * where the dependency queue adds it after the prediction. */
 template <int G>
-__device__ __forceinline__ void itx_group(const ReconArgs& a, const int32_t* list, int n, int wg, int* s_tmp) {
+__device__ __forceinline__ void itx_group(RArgs a, const int32_t* list, int n, int wg, int* s_tmp) {
   constexpr int kPer = 64 / G;
   const int grp = threadIdx.x / G, l = threadIdx.x - grp * G;
   const int idx = wg * kPer + grp;
@@ -204,9 +204,7 @@ __device__ __forceinline__ void itx_group(const ReconArgs& a, const int32_t* lis
 
 /* One launch for every class of a set: workgroups of the 64 class first (the
  * longest), then 32, 16, 8, 4. */
-__global__ __launch_bounds__(64) void k_itx(ReconArgs a, const int32_t* offs) {
-  __shared__ int s_tmp[64 * 65];
-  int b = blockIdx.x;
+__device__ __forceinline__ void itx_body(RArgs a, const int32_t* offs, int b, int* s_tmp) {
   for (int c = kTxClasses - 1; c >= 0; --c) {
     const int o0 = offs[c], n = offs[c + 1] - o0;
     const int per = 16 >> c;
@@ -223,6 +221,18 @@ __global__ __launch_bounds__(64) void k_itx(ReconArgs a, const int32_t* offs) {
       return;
     }
     b -= wgs;
+  }
+}
+
+__global__ __launch_bounds__(64) void k_itx(BatchDev B, int set) {
+  __shared__ int s_tmp[64 * 65];
+  const AV1H_CMEM int32_t* pre = kernel_prefix(B, (set ? kKItxIntra : kKItxInter));
+  const int total = pre[B.njobs];
+  for (int v = blockIdx.x; v < total; v += gridDim.x) {
+    const int j = find_job(pre, B.njobs, v);
+    RArgs a = recon_args(B, j);
+    itx_body(a, a.plan.itx_offs + set * (kTxClasses + 1), v - pre[j], s_tmp);
+    __syncthreads();
   }
 }
 
@@ -287,7 +297,7 @@ __device__ McPos mc_pos(int bx, int by, int mvx, int mvy, int sub_x, int sub_y, 
   return p;
 }
 
-__device__ const uint16_t* ref_plane(const ReconArgs& a, int slot, int plane) {
+__device__ const uint16_t* ref_plane(RArgs a, int slot, int plane) {
   const uint16_t* base = slot >= 0 ? a.pool + (size_t)slot * a.slot_size : a.frame;
   return base + a.lay.plane_off[plane];
 }
@@ -332,7 +342,7 @@ __device__ int mc_tap(const uint16_t* ref, bool cur, int stride, int ref_w, int 
 __device__ __forceinline__ int warp_offs(int v) { return clampi(((v + 512) >> 10) + 64, 0, 192); }
 
 /* one sample of a warped reference (av1gpu_kernels.h:1919-1958) */
-__device__ int warp_tap(int sub_x, int sub_y, int ref_w, int ref_h, const int* wp, const int* abcd,
+__device__ int warp_tap(int sub_x, int sub_y, int ref_w, int ref_h, const AV1H_CMEM int* wp, const AV1H_CMEM int* abcd,
                         const uint16_t* ref, bool cur, int stride, int bx, int by, int x, int y, int bd,
                         int compound) {
   const int* wf = SPEC(AV1T_OFF_WARP);
@@ -385,7 +395,7 @@ __device__ int wedge_master(int dir, int i, int j) {
     default: return wedge_ob63(i, j);
   }
 }
-__device__ int wedge_px(const DevBlock& b, int i, int k) {
+__device__ int wedge_px(BlkRef b, int i, int k) {
   const int m = wedge_master(b.wedge_dir, clampi(b.wedge_yoff + i, 0, 63), clampi(b.wedge_xoff + k, 0, 63));
   return b.wedge_flip ? 64 - m : m;
 }
@@ -452,7 +462,7 @@ __device__ int ii_sample(const IiCtx& c, int mode, int xs, int ys) {
   return 1 << (c.bd - 1);
 }
 
-__device__ int ii_mask(const DevBlock& b, int sub_x, int sub_y, int xs, int ys) {
+__device__ int ii_mask(BlkRef b, int sub_x, int sub_y, int xs, int ys) {
   if (b.flags & kIIWedge) {
     const int ny = sub_y + 1, nx = sub_x + 1, n = nx * ny;
     int sum = 0;
@@ -474,8 +484,8 @@ __device__ int ii_mask(const DevBlock& b, int sub_x, int sub_y, int xs, int ys) 
 
 /* Sample (xs, ys) of a block using any inter tool (av1gpu_kernels.h:2207-2326).
  * References of slot -1 (intra block copy) are the frame being decoded. */
-__device__ int inter_sample(const ReconArgs& a, const DevBlock& b, int xs, int ys) {
-  const Layout& L = a.lay;
+__device__ __forceinline__ int inter_sample(RArgs a, BlkRef b, int xs, int ys) {
+  LayRef L = a.lay;
   const int p = b.plane, bd = L.bit_depth;
   const int sub_x = p > 0 ? L.ss_x : 0, sub_y = p > 0 ? L.ss_y : 0;
   const int stride = L.stride[p];
@@ -630,7 +640,7 @@ __device__ void edge_upsample(int* buf, int* tmp, int n, int maxv) {
 /* The whole intra prediction of one block into s_pred (av1gpu_kernels.h:
  * 1578-1677 with the edges of 1108-1151 and the directional preparation of
  * 1452-1481). */
-__device__ void predict_intra(const ReconArgs& a, const DevBlock& b, int* work, uint16_t* pred) {
+__device__ void predict_intra(RArgs a, BlkRef b, int* work, uint16_t* pred) {
   const int lane = threadIdx.x;
   const int w = b.w, h = b.h, lw = ilog2(w), p = b.plane;
   const int stride = a.lay.stride[p];
@@ -856,19 +866,19 @@ __device__ void predict_intra(const ReconArgs& a, const DevBlock& b, int* work, 
 /* block driver                                                              */
 /* ------------------------------------------------------------------------ */
 
-__device__ void store_block(const ReconArgs& a, const DevBlock& b, const uint16_t* pred) {
+__device__ void store_block(RArgs a, BlkRef b, const uint16_t* pred) {
   const int p = b.plane, lw = ilog2(b.w);
   const int stride = a.lay.stride[p];
   uint16_t* dst = a.frame + a.lay.plane_off[p] + (size_t)b.y * stride + b.x;
   for (int s = threadIdx.x; s < b.w * b.h; s += 64) st_dev(dst + (size_t)(s >> lw) * stride + (s & (b.w - 1)), pred[s]);
 }
 
-__device__ __forceinline__ int64_t cell_index(const Layout& L, int p, int x, int y) {
+__device__ __forceinline__ int64_t cell_index(LayRef L, int p, int x, int y) {
   return L.cell_off[p] + (int64_t)(y >> L.cell_shift[p]) * L.cells_w[p] + (x >> L.cell_shift[p]);
 }
 
 /* the cells of a block, flagged with this frame's epoch */
-__device__ void mark_cells(const ReconArgs& a, const DevBlock& b, bool atomic) {
+__device__ void mark_cells(RArgs a, BlkRef b, bool atomic) {
   const int p = b.plane, sh = a.lay.cell_shift[p];
   const int cw = max(b.w >> sh, 1), ch = max(b.h >> sh, 1);
   for (int i = threadIdx.x; i < cw * ch; i += 64) {
@@ -886,8 +896,8 @@ __device__ void mark_cells(const ReconArgs& a, const DevBlock& b, bool atomic) {
  * column, the above row and the corner the reference scheduler waits for
  * (engine.cpp:811-875), and for chroma from luma the co-located luma cells
  * (engine.cpp:908-916). */
-__device__ void wait_cells(const ReconArgs& a, const DevBlock& b) {
-  const Layout& L = a.lay;
+__device__ void wait_cells(RArgs a, BlkRef b, uint32_t* error) {
+  LayRef L = a.lay;
   const int p = b.plane, sh = L.cell_shift[p], cell = 1 << sh;
   const int gw = L.grid_w[p], gh = L.grid_h[p];
   int nl = 0, na = 0, nc = 0, ncfl = 0;
@@ -937,7 +947,7 @@ __device__ void wait_cells(const ReconArgs& a, const DevBlock& b) {
     }
     if (__all(ok)) break;
     if (iter >= kSpinLimit) {
-      if (threadIdx.x == 0) atomicOr(a.error, 1u);
+      if (threadIdx.x == 0) atomicOr(error, 1u);
       break;
     }
     __builtin_amdgcn_s_sleep(1);
@@ -951,7 +961,7 @@ __device__ void wait_cells(const ReconArgs& a, const DevBlock& b) {
 /* A block of the queue: the prediction, then the residuals of its transforms, which
  * k_itx computed before the queue started (av1gpu_kernels.h:933-937 for the
  * add), then one store. */
-__device__ void reconstruct_queued(const ReconArgs& a, const DevBlock& b, int* work, uint16_t* pred) {
+__device__ void reconstruct_queued(RArgs a, BlkRef b, int* work, uint16_t* pred) {
   if (b.kind == kIntra) {
     predict_intra(a, b, work, pred);
   } else {
@@ -1020,16 +1030,23 @@ __device__ __forceinline__ McPos mc_pos_of(const McItemDev& it, const McRefDev& 
   return P;
 }
 
-__global__ __launch_bounds__(256) void k_mc(ReconArgs a) {
-  __shared__ int s_filt[768];
-  __shared__ uint16_t s_win[kMcWin];
-  __shared__ int16_t s_mid[kMcMid];
-  __shared__ McItemDev s_it[kMcItems];
-  __shared__ int s_woff[kMcPairs + 1], s_moff[kMcPairs + 1], s_px[kMcItems + 1];
+struct McLds {
+  uint16_t win[kMcWin];
+  int16_t mid[kMcMid];
+  McItemDev it[kMcItems];
+  int woff[kMcPairs + 1], moff[kMcPairs + 1], px[kMcItems + 1];
+};
+
+__device__ void mc_body(RArgs a, int tn, const int* s_filt, McLds& L_) {
+  uint16_t* s_win = L_.win;
+  int16_t* s_mid = L_.mid;
+  McItemDev* s_it = L_.it;
+  int* s_woff = L_.woff;
+  int* s_moff = L_.moff;
+  int* s_px = L_.px;
   const int tid = threadIdx.x;
-  const McTaskDev task = a.plan.mc_task[blockIdx.x];
+  const AV1H_CMEM McTaskDev& task = cmem(a.plan.mc_task)[tn];
   const int nit = task.count, npairs = 2 * nit;
-  for (int i = tid; i < 768; i += 256) s_filt[i] = av1t_spec_table[AV1T_OFF_SUBPEL + i];
   {
     const int32_t* src = reinterpret_cast<const int32_t*>(a.plan.mc_items + task.first);
     int32_t* dst = reinterpret_cast<int32_t*>(s_it);
@@ -1142,40 +1159,67 @@ __global__ __launch_bounds__(256) void k_mc(ReconArgs a) {
   }
 }
 
-/* Inter blocks with warp, masks or OBMC (they read only references): one
- * workgroup per block, the direct per sample computation. */
-__global__ __launch_bounds__(64) void k_inter_special(ReconArgs a) {
-  const int bi = __builtin_amdgcn_readfirstlane(a.plan.special_list[blockIdx.x]);
-  const DevBlock& b = a.plan.blocks[bi];
-  const int p = b.plane, stride = a.lay.stride[p], lw = ilog2(b.w);
-  uint16_t* dst = a.frame + a.lay.plane_off[p] + (size_t)b.y * stride + b.x;
-  for (int s = threadIdx.x; s < b.w * b.h; s += 64)
-    dst[(size_t)(s >> lw) * stride + (s & (b.w - 1))] = (uint16_t)inter_sample(a, b, s & (b.w - 1), s >> lw);
-  mark_cells(a, b, false);
+__global__ __launch_bounds__(256) void k_mc(BatchDev B) {
+  __shared__ int s_filt[768];
+  __shared__ McLds s_lds;
+  for (int i = threadIdx.x; i < 768; i += 256) s_filt[i] = av1t_spec_table[AV1T_OFF_SUBPEL + i];
+  const AV1H_CMEM int32_t* pre = kernel_prefix(B, kKMc);
+  const int total = pre[B.njobs];
+  for (int v = blockIdx.x; v < total; v += gridDim.x) {
+    const int j = find_job(pre, B.njobs, v);
+    mc_body(recon_args(B, j), v - pre[j], s_filt, s_lds);
+    __syncthreads();
+  }
 }
 
-__global__ __launch_bounds__(64) void k_queue(ReconArgs a) {
+/* Inter blocks with warp, masks or OBMC (they read only references): one
+ * workgroup per block, the direct per sample computation. */
+__global__ __launch_bounds__(64) void k_inter_special(BatchDev B) {
+  const AV1H_CMEM int32_t* pre = kernel_prefix(B, kKSpecial);
+  const int total = pre[B.njobs];
+  for (int v = blockIdx.x; v < total; v += gridDim.x) {
+    const int j = find_job(pre, B.njobs, v);
+    RArgs a = recon_args(B, j);
+    const int bi = cmem(a.plan.special_list)[v - pre[j]];
+    BlkRef b = cmem(a.plan.blocks)[bi];
+    const int p = b.plane, stride = a.lay.stride[p], lw = ilog2(b.w);
+    uint16_t* dst = a.frame + a.lay.plane_off[p] + (size_t)b.y * stride + b.x;
+    for (int s = threadIdx.x; s < b.w * b.h; s += 64)
+      dst[(size_t)(s >> lw) * stride + (s & (b.w - 1))] = (uint16_t)inter_sample(a, b, s & (b.w - 1), s >> lw);
+    mark_cells(a, b, false);
+  }
+}
+
+__global__ __launch_bounds__(64) void k_queue(BatchDev B) {
   __shared__ int s_work[kWork];
   __shared__ uint16_t s_pred[64 * 64];
   for (;;) {
     int idx = 0;
-    if (threadIdx.x == 0) idx = (int)atomicAdd(a.queue_counter, 1u);
+    if (threadIdx.x == 0) idx = (int)atomicAdd(B.queue_counter, 1u);
     idx = __builtin_amdgcn_readfirstlane(__shfl(idx, 0));
-    if (idx >= a.plan.n_queue) return;
-    const uint64_t t0 = a.trace ? (uint64_t)wall_clock64() : 0;
-    const int bi = __builtin_amdgcn_readfirstlane(a.plan.queue_list[idx]);
-    const DevBlock& b = a.plan.blocks[bi];
-    wait_cells(a, b);
-    const uint64_t t1 = a.trace ? (uint64_t)wall_clock64() : 0;
+    if (idx >= B.nqueue) return;
+    const uint64_t t0 = B.trace ? (uint64_t)wall_clock64() : 0;
+    int job = 0, blk;
+    if (B.queue) {
+      const uint32_t e = cmem(B.queue)[idx];
+      job = (int)(e >> kQueueJobShift);
+      blk = (int)(e & ((1u << kQueueJobShift) - 1));
+    } else {
+      blk = cmem(recon_args(B, 0).plan.queue_list)[idx];
+    }
+    RArgs a = recon_args(B, __builtin_amdgcn_readfirstlane(job));
+    BlkRef b = cmem(a.plan.blocks)[__builtin_amdgcn_readfirstlane(blk)];
+    wait_cells(a, b, B.error);
+    const uint64_t t1 = B.trace ? (uint64_t)wall_clock64() : 0;
     reconstruct_queued(a, b, s_work, s_pred);
     /* every sample store of the block has completed before its cells flip */
     __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     mark_cells(a, b, true);
     wsync();
-    if (a.trace && threadIdx.x == 0) {
+    if (B.trace && threadIdx.x == 0) {
       /* diagnostics: when the block was taken, found ready, published */
-      uint64_t* t = a.trace + 4 * (size_t)idx;
+      uint64_t* t = B.trace + 4 * (size_t)idx;
       t[0] = t0;
       t[1] = t1;
       t[2] = (uint64_t)wall_clock64();
@@ -1200,33 +1244,22 @@ int itx_groups(const FramePlanDev& p, int set) {
   return g;
 }
 
-hipError_t launch_mc(const ReconArgs& a, int cus, hipStream_t s, int* launches) {
-  if (a.plan.mc_tasks <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_mc, dim3(a.plan.mc_tasks), dim3(256), 0, s, a);
-  ++*launches;
+hipError_t launch_recon_kernel(const BatchDev& b, int k, int total, int cus, hipStream_t s) {
+  switch (k) {
+    case kKMc: hipLaunchKernelGGL(k_mc, dim3(min(total, cus * 8)), dim3(256), 0, s, b); break;
+    case kKSpecial: hipLaunchKernelGGL(k_inter_special, dim3(min(total, cus * 16)), dim3(64), 0, s, b); break;
+    case kKItxInter: hipLaunchKernelGGL(k_itx, dim3(min(total, cus * 16)), dim3(64), 0, s, b, 0); break;
+    case kKItxIntra: hipLaunchKernelGGL(k_itx, dim3(min(total, cus * 16)), dim3(64), 0, s, b, 1); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
-hipError_t launch_special(const ReconArgs& a, hipStream_t s, int* launches) {
-  if (a.plan.n_special <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_inter_special, dim3(a.plan.n_special), dim3(64), 0, s, a);
-  ++*launches;
-  return hipGetLastError();
-}
-
-hipError_t launch_itx(const ReconArgs& a, int set, hipStream_t s, int* launches) {
-  const int groups = itx_groups(a.plan, set);
-  if (groups <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_itx, dim3(groups), dim3(64), 0, s, a, a.plan.itx_offs + set * (kTxClasses + 1));
-  ++*launches;
-  return hipGetLastError();
-}
-
-hipError_t launch_queue(const ReconArgs& a, int cus, hipStream_t s, int* launches) {
-  if (a.plan.n_queue <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, a.queue_counter, 1);
-  const int grid = min(a.plan.n_queue, cus * 6);
-  hipLaunchKernelGGL(k_queue, dim3(grid), dim3(64), 0, s, a);
+hipError_t launch_queue(const BatchDev& b, int cus, hipStream_t s, int* launches) {
+  if (b.nqueue <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, b.queue_counter, 1);
+  const int grid = min(b.nqueue, cus * 6);
+  hipLaunchKernelGGL(k_queue, dim3(grid), dim3(64), 0, s, b);
   *launches += 2;
   return hipGetLastError();
 }

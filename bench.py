@@ -141,6 +141,119 @@ def cpu_baseline(clip, seconds, gpu_hashes):
                       f"oracle (OpenMP, {cores} threads) with syntax ingest and read back, {dt:.1f} s"}, n, mism
 
 
+# The 16-stream mix: resolutions as named, 8 and 10 bit alternating inside
+# each resolution group (the 3840x2160 stream is 10 bit), config-1 syntax
+# statistics, 32 frames per stream.
+MIX_STREAMS = [(426, 240, 8), (426, 240, 10), (426, 240, 8), (426, 240, 10),
+               (854, 480, 8), (854, 480, 10), (854, 480, 8), (854, 480, 10),
+               (1280, 720, 8), (1280, 720, 10), (1280, 720, 8), (1280, 720, 10),
+               (1920, 1080, 8), (1920, 1080, 10), (1920, 1080, 8),
+               (3840, 2160, 10)]
+MIX_FRAMES = 32
+
+
+def run_mix(args):
+    """The 16-stream mix decoded two ways from the same device-resident plans:
+    plain (one decode call per frame, streams round robin) and batched (one
+    av1h_decode_planned_batch per round, the frame of every stream in it).
+    Returns the report object; outputs of the two are compared frame by frame."""
+    import av1ongpu_amd as pkg
+    from concurrent.futures import ThreadPoolExecutor
+
+    t = time.perf_counter()
+    workers = max(1, min(8, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else 8))
+
+    def make(i):
+        w, h, bd = MIX_STREAMS[i]
+        return pkg.Clip("c1", args.seed * 1000 + i, width=w, height=h, bit_depth=bd, frames=MIX_FRAMES)
+
+    with ThreadPoolExecutor(workers) as ex:
+        clips = list(ex.map(make, range(len(MIX_STREAMS))))
+    ns = len(clips)
+    sched = [slot_schedule(c) for c in clips]
+    px = [w * h for w, h, _ in MIX_STREAMS]
+    log(f"mix: {ns} streams x {MIX_FRAMES} frames, {sum(px) / 1e6:.2f} Mpixel per round, generated in "
+        f"{time.perf_counter() - t:.1f} s")
+    decs = [pkg.HipDecoder(0) for _ in clips]
+
+    def plan_stream(i):
+        c = clips[i]
+        refs = sched[i][2]
+        out = []
+        for f in range(c.frames):
+            out.append(decs[i].plan_wire(pkg.wire_set_slots(c.wire(f), refs[f])))
+            c._wire.pop(f, None)
+        return out
+
+    t = time.perf_counter()
+    with ThreadPoolExecutor(workers) as ex:
+        plans = list(ex.map(plan_stream, range(ns)))
+    del clips
+    log(f"mix: planned {ns * MIX_FRAMES} frames in {time.perf_counter() - t:.1f} s")
+
+    def release(i, f):
+        for sl in sched[i][1][f]:
+            decs[i].release(sl)
+
+    def plain(lat, hashes=None):
+        for f in range(MIX_FRAMES):
+            for i in range(ns):
+                t1 = time.perf_counter()
+                decs[i].decode_planned(plans[i][f], keep=sched[i][0][f] >= 0, reserved=sched[i][0][f])
+                lat.append(time.perf_counter() - t1)
+                if hashes is not None:
+                    w, h, _ = MIX_STREAMS[i]
+                    hashes.append(pkg.fnv1a64(pkg.yuv16_bytes(decs[i].read_slot(-1, w, h))))
+                release(i, f)
+
+    def batched(lat, hashes=None):
+        keeps = [[sched[i][0][f] >= 0 for i in range(ns)] for f in range(MIX_FRAMES)]
+        res = [[sched[i][0][f] for i in range(ns)] for f in range(MIX_FRAMES)]
+        for f in range(MIX_FRAMES):
+            t1 = time.perf_counter()
+            pkg.decode_planned_batch(decs, [plans[i][f] for i in range(ns)], keeps[f], res[f])
+            lat.append(time.perf_counter() - t1)
+            if hashes is not None:
+                for i in range(ns):
+                    w, h, _ = MIX_STREAMS[i]
+                    hashes.append(pkg.fnv1a64(pkg.yuv16_bytes(decs[i].read_slot(-1, w, h))))
+            for i in range(ns):
+                release(i, f)
+
+    # parity pass (untimed): every frame of every stream, plain vs batched
+    hp, hb = [], []
+    plain([], hp)
+    batched([], hb)
+    mism = sum(1 for a, b in zip(hp, hb) if a != b)
+    log(f"mix: batched vs plain on {len(hp)} frames: {mism} mismatched")
+    for _ in range(args.warmup):
+        plain([])
+        batched([])
+    out = {}
+    for name, fn in (("plain", plain), ("batched", batched)):
+        lat = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn(lat)
+        dt = time.perf_counter() - t0
+        out[name] = {"value": round(args.steps * MIX_FRAMES * sum(px) / dt / 1e6, 2), "unit": "Mpixel/s",
+                     "ms_per_step": round(dt * 1e3 / args.steps, 3),
+                     "call_latency_ms": {"p50": round(pct(lat, 50) * 1e3, 3), "p99": round(pct(lat, 99) * 1e3, 3)}}
+        log(f"mix {name}: {out[name]['value']} Mpixel/s")
+    for i in range(ns):
+        for p in plans[i]:
+            decs[i].discard_plan(p)
+        decs[i].close()
+    return {
+        "workload": "16 streams: 4x 426x240, 4x 854x480, 4x 1280x720, 3x 1920x1080, 1x 3840x2160, 8 and 10 bit "
+                    "alternating (2160p 10 bit), 4:2:0, 32 frames each, 1280x720-workload syntax statistics",
+        "plain": out["plain"], "batched": out["batched"],
+        "speedup": round(out["batched"]["value"] / out["plain"]["value"], 3),
+        "bit_exact": {"frames_checked": len(hp), "mismatched_frames": mism,
+                      "method": "FNV-1a-64 of every sample, batched decode vs plain decode of the same plans"},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -150,6 +263,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline and the parity sample")
+    ap.add_argument("--no-mix", action="store_true", help="skip the 16-stream batched vs plain mix")
+    ap.add_argument("--mix-only", action="store_true", help="only the 16-stream mix")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this backend decodes each stream on one device and has no "
@@ -161,6 +276,9 @@ def main():
 
     import av1ongpu_amd as pkg
 
+    if args.mix_only:
+        print(json.dumps({"batched_mix": run_mix(args)}), flush=True)
+        return
     dec = pkg.HipDecoder(0)
     log(f"device: {dec.device_info().name.decode(errors='replace')}")
     t = time.perf_counter()
@@ -172,11 +290,12 @@ def main():
     log(f"clip {args.workload} seed {args.seed}: {n} frames, {px} display px/frame, "
         f"{sum(len(w) for w in wires) / 1e6:.0f} MB of syntax, generated in {time.perf_counter() - t:.1f} s")
 
+    # every decode packs NV12 on the device as its last launch
+    dec.set_decode_nv12(NV12_PITCH)
     # --- ingest-inclusive pass: syntax flattened and uploaded inside the timing
     t0 = time.perf_counter()
     for f in range(n):
         slot, _ = dec.decode_wire(wires[f], keep=keep_slot[f] >= 0, reserved=keep_slot[f], read_back=False)
-        dec.pack_nv12_device(-1, NV12_PITCH)
         for s in releases[f]:
             dec.release(s)
     ingest_s = time.perf_counter() - t0
@@ -191,7 +310,6 @@ def main():
         for f in range(n):
             t1 = time.perf_counter()
             dec.decode_planned(plans[f], keep=keep_slot[f] >= 0, reserved=keep_slot[f])
-            dec.pack_nv12_device(-1, NV12_PITCH)
             lat.append(time.perf_counter() - t1)
             for s in releases[f]:
                 dec.release(s)
@@ -256,6 +374,8 @@ def main():
                                "method": "FNV-1a-64 of every sample of each decoded frame, GPU vs CPU oracle"}
         if mism:
             log(f"MISMATCH on frames {mism[:10]}")
+    if not args.no_mix:
+        result["batched_mix"] = run_mix(args)
     print(json.dumps(result), flush=True)
 
 

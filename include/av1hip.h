@@ -379,6 +379,19 @@ int av1h_decode_planned(av1h_ctx* ctx, void* plan, int keep_reference, int reser
                         const av1h_planes* out, av1h_frame_info* info, int* slot_out);
 void av1h_discard_plan(av1h_ctx* ctx, void* plan);
 
+/* Sizes the context's reference pool to `slots` (1..32, default 32) slots;
+ * allowed while no slot is in use, the pool is re-allocated at the next
+ * decode. Callers that admit streams against a memory budget use it with
+ * av1h_stream_bytes. No counterpart in the reference (a fixed 32 slot pool,
+ * cuda_backend.cpp:242-279). */
+int av1h_set_pool_slots(av1h_ctx* ctx, int slots);
+
+/* Device bytes a context holds for pictures of this geometry with a pool of
+ * `pool_slots` slots (and decode-time NV12 at `nv12_pitch`, 0 for none),
+ * syntax plans excluded; -1 for invalid arguments. */
+int64_t av1h_stream_bytes(int width, int height, int bit_depth, int ss_x, int ss_y, int pool_slots,
+                          size_t nv12_pitch);
+
 /* Reserves a reference slot (reference Decoder::reserve_reference,
  * av1gpu.hpp:661). Returns the slot or a negative status. */
 int av1h_reserve_reference(av1h_ctx* ctx);
@@ -403,6 +416,12 @@ int av1h_to_nv12(av1h_ctx* ctx, int slot, size_t pitch, uint8_t* out, size_t out
  * context), for consumers that take the device picture (no counterpart in the
  * reference, whose to_nv12 always copies to the host). */
 const uint8_t* av1h_nv12_device_buffer(const av1h_ctx* ctx);
+/* With `pitch` > 0 (even), every decode on this context also packs the
+ * finished picture into NV12 at that pitch on the device, in the same launch
+ * sequence (4:2:0 frames only); av1h_to_nv12 of that picture at that pitch
+ * then only copies. 0 turns it off. No counterpart in the reference, whose
+ * decoder packs only when asked (Decoder::to_nv12, av1gpu.hpp:745). */
+int av1h_set_decode_nv12(av1h_ctx* ctx, size_t pitch);
 
 /* YUV -> RGB24 of a slot (reference Decoder::to_rgb, av1gpu.hpp:739): `matrix`
  * 0 BT.601, 1 BT.709, 2 BT.2020, 3 identity; `range` 0 limited, 1 full. */
@@ -463,11 +482,139 @@ int64_t av1h_get_queue_trace(av1h_ctx* ctx, uint64_t* out, int64_t cap);
  * own context) decoded together, one launch per stage group for the whole
  * batch. `frames[i]` is decoded by `ctxs[i]` exactly as av1h_decode would
  * (keep_reference 1, no read back); slots land in `slots_out[i]`. All contexts
- * must live on the same device. Each stream's result is bit identical to
- * decoding it alone.
+ * must live on the same device and be distinct (at most 128). Each stream's
+ * result is bit identical to decoding it alone: every kernel runs the work of
+ * all frames of the batch in one launch, and the dependency queue hands out
+ * the intra blocks of all frames from one merged order.
  */
 int av1h_decode_batch(av1h_ctx* const* ctxs, const av1h_frame_desc* const* frames, int count,
                       int* slots_out);
+
+/* The batch of planned frames: `plans[i]` (from av1h_plan_frame on
+ * `ctxs[i]`) decoded by `ctxs[i]` as av1h_decode_planned would with
+ * keep_reference `keep[i]` and reserved slot `reserved[i]` (null arrays: keep
+ * 1, reserved -1). Contexts must be distinct; the batch's launches go to the
+ * stream of ctxs[0], errors to av1h_last_error(ctxs[0]). */
+int av1h_decode_planned_batch(av1h_ctx* const* ctxs, void* const* plans, const int* keep,
+                              const int* reserved, int count, int* slots_out);
+
+/* ------------------------------------------------------ superblock costs */
+/*
+ * The queue's schedule. By default (the static schedule) the dependency queue
+ * hands intra blocks out by superblock anti-diagonal. A schedule from costs
+ * orders superblocks by their longest remaining path through the superblock
+ * dependency graph (left, above, above-left, above-right and intra block copy
+ * sources), each superblock weighted by its predicted work: blocks of the
+ * superblock with the longest path go first, coding order inside one. Every
+ * dependency still comes earlier than what waits for it, so the decode is
+ * correct under either schedule and the pixels are the same.
+ * No counterpart in the reference (engine.cpp:1080-1302 orders by wave).
+ */
+enum { AV1H_CM_FEATURES = 24, AV1H_CM_PARAMS = 5825 };
+
+/* Superblocks (64x64 luma) of a plan and their 24 syntax features each:
+ * blocks; transforms by larger side 4/8/16/32/64; nonzero coefficients / 64;
+ * intra, inter, compound blocks; fraction of inter blocks with subpel
+ * motion; directional, smooth/Paeth, CFL/palette/filter intra blocks; queued
+ * and inter samples / 4096; largest transform side / 64; lossless or raw
+ * residual blocks; intra block copies; chroma blocks; warp/mask/OBMC/
+ * inter-intra blocks; edge filter on; queued blocks; bit depth above 8.
+ * Returns the superblock count; `features` (count x 24) may be null. */
+int av1h_plan_sb_features(const void* plan, float* features);
+/* After a decode of `plan` alone with the queue trace on: the queue work
+ * (sample time after the wait, per block, summed) of every superblock in
+ * microseconds. Returns the superblock count. */
+int av1h_plan_sb_times(av1h_ctx* ctx, const void* plan, float* us);
+/* Orders the plan's queue by `sb_cost_us` (one per superblock; null restores
+ * the static schedule). */
+int av1h_plan_schedule(av1h_ctx* ctx, void* plan, const float* sb_cost_us);
+
+/* The cost model (a 24-64-64-1 FP32 perceptron trained and run by HIP
+ * kernels on the device). */
+typedef struct av1h_costmodel av1h_costmodel;
+int av1h_cm_create(int device, uint64_t seed, av1h_costmodel** out);
+void av1h_cm_destroy(av1h_costmodel* model);
+/* W1[64][24], b1[64], W2[64][64], b2[64], W3[64], b3 */
+int av1h_cm_set_params(av1h_costmodel* model, const float* params);
+int av1h_cm_get_params(av1h_costmodel* model, float* params);
+/* feature standardisation and the time scale the network predicts in */
+int av1h_cm_set_norm(av1h_costmodel* model, const float* mean, const float* dev, float y_mean, float y_dev);
+int av1h_cm_get_norm(av1h_costmodel* model, float* mean, float* dev, float* y_mean, float* y_dev);
+int av1h_cm_predict(av1h_costmodel* model, const float* features, int64_t n, float* out_us);
+/* Gradient of the mean squared error (standardised time) over the n samples,
+ * and that loss: what training steps on. */
+int av1h_cm_gradients(av1h_costmodel* model, const float* features, const float* y_us, int64_t n, float* grad,
+                      float* loss);
+/* Adam, batches of 64 samples in a seeded shuffle per epoch; `set_norm`
+ * first sets the standardisation from these samples. */
+int av1h_cm_train(av1h_costmodel* model, const float* features, const float* y_us, int64_t n, int epochs,
+                  float lr, uint64_t seed, int set_norm, float* loss_per_epoch);
+/* Predicts the plan's superblock costs and schedules its queue by them. */
+int av1h_plan_schedule_model(av1h_ctx* ctx, void* plan, av1h_costmodel* model);
+
+/* ---------------------------------------------------------------- serving */
+/*
+ * A serving loop over av1h_decode_planned_batch (no counterpart in the
+ * reference; its nearest relative is the single stream pipelined ingest of
+ * dsfilter/decode_core.hpp:164-292). Streams open with their geometry and
+ * pool size and are admitted in order while the device memory of admitted
+ * streams (av1h_stream_bytes) fits `hbm_budget`; the first stream is always
+ * admitted. Frames are submitted as wire bytes (planned by `planner_threads`
+ * host threads once the stream is admitted) or as plans; frames of a stream
+ * decode in submission order, and one scheduler thread forms batches from
+ * the oldest planned head frame of every idle stream, up to `max_batch`
+ * frames, while the previous batch is still returning. A closed stream gives
+ * its memory back after its last frame.
+ */
+typedef struct av1h_server av1h_server;
+
+typedef struct av1h_server_config {
+  int32_t device;
+  int32_t max_batch;        /* frames per batch, 1..128                        */
+  int32_t planner_threads;  /* host threads flattening submitted wire frames   */
+  int32_t capture;          /* 1: every completion carries a host copy         */
+  int64_t hbm_budget;       /* bytes of admitted stream state                  */
+  size_t nv12_pitch;        /* > 0: NV12 packed on the device with each frame   */
+} av1h_server_config;
+
+typedef struct av1h_completion {
+  uint64_t tag;             /* the submitter's                                  */
+  int32_t stream, slot;     /* slot the frame was kept in, or -1                */
+  int32_t status;           /* AV1H_OK or the error                             */
+  int32_t batch_frames;     /* frames of the batch it ran in                    */
+  double arrival_ms, start_ms, done_ms; /* av1h_server_now_ms clock            */
+  uint16_t* picture;        /* capture: Y, U, V display planes back to back     */
+  int64_t picture_samples;  /* free with av1h_server_free_picture               */
+} av1h_completion;
+
+typedef struct av1h_server_stats {
+  int64_t frames, batches, admitted, waiting, active, budget_used;
+  double busy_ms;           /* wall time inside batch decodes                   */
+  int32_t max_batch_frames;
+} av1h_server_stats;
+
+double av1h_server_now_ms(void);
+int av1h_server_create(const av1h_server_config* cfg, av1h_server** out);
+/* Stops the threads; frames not yet decoded are dropped. */
+void av1h_server_destroy(av1h_server* server);
+/* Returns the stream id (>= 0) or a negative status. */
+int av1h_server_open_stream(av1h_server* server, int width, int height, int bit_depth, int ss_x, int ss_y,
+                            int pool_slots);
+/* `reserved_slot`: the slot to keep the frame in (-1: not kept); `release`:
+ * slots given back once the frame is decoded (as av1h_release_reference). */
+int av1h_server_submit_wire(av1h_server* server, int stream, const uint8_t* wire, size_t len,
+                            int reserved_slot, const int32_t* release, int nrelease, double arrival_ms,
+                            uint64_t tag);
+/* A plan from av1h_plan_frame on any context of the server's device; the
+ * caller keeps it and may submit it again (plans are read only). */
+int av1h_server_submit_plan(av1h_server* server, int stream, void* plan, int reserved_slot,
+                            const int32_t* release, int nrelease, double arrival_ms, uint64_t tag);
+int av1h_server_close_stream(av1h_server* server, int stream);
+/* Up to `max` completions, waiting up to `wait_ms` for the first. */
+int av1h_server_poll(av1h_server* server, av1h_completion* out, int max, int wait_ms);
+void av1h_server_free_picture(uint16_t* picture);
+int av1h_server_get_stats(av1h_server* server, av1h_server_stats* out);
+const char* av1h_server_last_error(av1h_server* server);
 
 #ifdef __cplusplus
 } /* extern "C" */

@@ -1,0 +1,101 @@
+"""Multi-stream batches: frames of several streams (one decoder context each)
+decoded together through av1h_decode_planned_batch, one launch per kernel for
+the whole batch and one merged dependency queue. Each stream's output must be
+bit identical to decoding it alone, and to the CPU oracle.
+
+The streams differ in size, bit depth and tools, so one batch mixes frames
+with and without superres, film grain, loop restoration, intra block copy and
+key frames; streams of different lengths drop out of later batches.
+"""
+import numpy as np
+import pytest
+
+import av1ongpu_amd as pkg
+from oracle_lib import OracleDecoder
+
+pytestmark = pytest.mark.gpu
+
+
+def _alone(clip):
+    return [planes for _f, _s, planes in pkg.decode_clip(pkg.HipDecoder(), clip)]
+
+
+def _check(name, f, got, want):
+    for p in range(3):
+        if not np.array_equal(got[p], want[p]):
+            bad = np.argwhere(got[p] != want[p])
+            y, x = bad[0]
+            raise AssertionError(f"{name} frame {f} plane {p}: {len(bad)} mismatched samples, first at ({x},{y}) "
+                                 f"batched={got[p][y, x]} alone={want[p][y, x]}")
+
+
+def _run_batched(clips):
+    decs = [pkg.HipDecoder() for _ in clips]
+    got = [[] for _ in clips]
+    for _f, outs in pkg.decode_clips_batched(decs, clips):
+        for i, _slot, planes in outs:
+            got[i].append(planes)
+    return got
+
+
+MIX = [("c1_mini", 1, {}), ("c2_mini", 2, {}), ("tools_mini", 1, {}), ("contract_mini", 1, {}),
+       ("c1_mini", 3, dict(width=200, height=120, bit_depth=10)),
+       ("c1_mini", 4, dict(key_interval=1, frames=3)),
+       ("c1_mini", 8, dict(width=426, height=240, frames=3))]
+
+
+def test_batch_matches_alone():
+    clips = [pkg.Clip(w, s, **o) for w, s, o in MIX]
+    got = _run_batched(clips)
+    for c, g in zip(clips, got):
+        want = _alone(c)
+        assert len(g) == len(want) == c.frames
+        for f, (a, b) in enumerate(zip(g, want)):
+            _check(f"{c.workload} seed {c.seed}", f, a, b)
+
+
+def test_batch_matches_oracle():
+    clips = [pkg.Clip("c2_mini", 5), pkg.Clip("tools_mini", 2), pkg.Clip("c1_mini", 6)]
+    got = _run_batched(clips)
+    for c, g in zip(clips, got):
+        want = [planes for _f, _s, planes in pkg.decode_clip(OracleDecoder(), c)]
+        for f, (a, b) in enumerate(zip(g, want)):
+            _check(f"{c.workload} seed {c.seed} (oracle)", f, a, b)
+
+
+def test_batch_of_one_is_plain():
+    clip = pkg.Clip("c2_mini", 3)
+    got = _run_batched([clip])[0]
+    for f, (a, b) in enumerate(zip(got, _alone(clip))):
+        _check("c2_mini seed 3", f, a, b)
+
+
+def test_batch_rejects_repeated_context():
+    clip = pkg.Clip("c1_mini", 1)
+    dec = pkg.HipDecoder()
+    plans = [dec.plan_wire(clip.wire(0)), dec.plan_wire(clip.wire(0))]
+    try:
+        with pytest.raises(pkg.HipError, match="twice"):
+            pkg.decode_planned_batch([dec, dec], plans)
+    finally:
+        for p in plans:
+            dec.discard_plan(p)
+    # the context still decodes afterwards
+    slot, _ = dec.decode_wire(clip.wire(0), keep=False, read_back=False)
+    assert slot == -1
+
+
+@pytest.mark.parametrize("extra", [0, 6])
+def test_decode_time_nv12(extra):
+    """NV12 packed inside the decode (set_decode_nv12) equals to_nv12 after a
+    plain decode, at a pitch that is a multiple of 4 and at one that is not."""
+    clip = pkg.Clip("c2_mini", 4)
+    w, _cw, h, _bd, _sx, _sy = pkg.wire_header(clip.wire(0))
+    pitch = 2048 if extra == 0 else 2 * ((w + 1) // 2) + extra
+    a, b = pkg.HipDecoder(), pkg.HipDecoder()
+    b.set_decode_nv12(pitch)
+    for (f, sa, _), (_, sb, _) in zip(pkg.decode_clip(a, clip, read_back=False),
+                                      pkg.decode_clip(b, clip, read_back=False)):
+        na = a.to_nv12(-1, pitch, w, h)
+        nb = b.to_nv12(-1, pitch, w, h)
+        assert bytes(na) == bytes(nb), f"frame {f}"
