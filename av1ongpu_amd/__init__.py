@@ -230,6 +230,7 @@ def hip_lib():
         lib.av1h_get_queue_trace.restype = ctypes.c_int64
         lib.av1h_get_queue_trace.argtypes = [P, ctypes.c_void_p, ctypes.c_int64]
         lib.av1h_set_decode_nv12.argtypes = [P, ctypes.c_size_t]
+        lib.av1h_set_graphs.argtypes = [P, ctypes.c_int]
         lib.av1h_nv12_device_buffer.restype = ctypes.c_void_p
         lib.av1h_nv12_device_buffer.argtypes = [P]
         _bind_server(lib)
@@ -502,6 +503,10 @@ class HipDecoder:
         rc = self._lib.av1h_plan_schedule_model(self._h, plan, model.handle)
         if rc != AV1H_OK:
             raise HipError(f"schedule_model failed ({rc}): {self.error()}")
+
+    def set_graphs(self, enable=True):
+        """Replay captured launch sequences (HIP graphs) for batches this context leads."""
+        self._lib.av1h_set_graphs(self._h, 1 if enable else 0)
 
     def set_decode_nv12(self, pitch):
         """Pack NV12 at `pitch` on the device as part of every decode (0: off)."""

@@ -154,6 +154,11 @@ struct Ctx {
   void* batch_dev = nullptr;
   void* batch_pin = nullptr;
   size_t batch_cap = 0;
+  bool dirty = false;            /* work queued on `stream` nobody waited for */
+  /* captured launch sequences of batches this context leads, by frame count
+   * (valid while batch_dev stays where it is) */
+  bool graphs = true;
+  std::vector<std::pair<int, hipGraphExec_t>> graph_exec;
   bool stage_hit[AV1H_STAGES] = {};
   double stage_ms[AV1H_STAGES] = {};
   int64_t stage_n[AV1H_STAGES] = {};
@@ -1234,6 +1239,7 @@ int upload(Ctx* c, Plan* plan) {
   HIP_OK(c, hipStreamSynchronize(c->stream));
   std::memcpy(c->host_pin, plan->hbuf.data(), n);
   HIP_OK(c, hipMemcpyAsync(plan->dbuf, c->host_pin, n, hipMemcpyHostToDevice, c->stream));
+  c->dirty = true;
   return AV1H_OK;
 }
 
@@ -1262,6 +1268,7 @@ int prepare(Ctx* c, const Layout& L) {
     HIP_OK(c, hipMalloc(&c->lr_cdef, bytes));
     HIP_OK(c, hipMalloc(&c->lr_prev_up, bytes));
     HIP_OK(c, hipMemsetAsync(c->frame, 0, bytes, c->stream));
+    c->dirty = true;
     c->cap = (size_t)L.size;
     c->pool_alloc = 0;
     std::memset(c->used, 0, sizeof(c->used));
@@ -1275,6 +1282,7 @@ int prepare(Ctx* c, const Layout& L) {
     HIP_OK(c, hipMalloc(&c->pool, bytes));
     HIP_OK(c, hipMemsetAsync(c->pool, 0, bytes, c->stream));
     c->pool_alloc = c->pool_slots;
+    c->dirty = true;
     std::memset(c->used, 0, sizeof(c->used));
     c->last_slot = -1;
   }
@@ -1291,12 +1299,18 @@ int prepare(Ctx* c, const Layout& L) {
     c->cells = nullptr;
     HIP_OK(c, hipMalloc(&c->cells, (size_t)L.cells * 4));
     HIP_OK(c, hipMemsetAsync(c->cells, 0, (size_t)L.cells * 4, c->stream));
+    c->dirty = true;
     c->cells_cap = L.cells;
     c->epoch = 0;
   }
   c->lay = L;
   c->have_layout = true;
   return AV1H_OK;
+}
+
+void drop_graphs(Ctx* c) {
+  for (auto& g : c->graph_exec) (void)hipGraphExecDestroy(g.second);
+  c->graph_exec.clear();
 }
 
 int alloc_slot(Ctx* c) {
@@ -1482,10 +1496,11 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
     fa.nv12 = nv12 ? c->pack : nullptr;
     fa.nv12_pitch = nv12 ? (int32_t)c->decode_nv12_pitch : 0;
   }
-  std::vector<int32_t> prefix((size_t)kKernels * (n + 1), 0);
+  std::vector<int32_t> prefix((size_t)kKernels * (n + 1) + 1, 0);
   for (int k = 0; k < kKernels; ++k)
     for (int i = 0; i < n; ++i)
       prefix[(size_t)k * (n + 1) + i + 1] = prefix[(size_t)k * (n + 1) + i] + kernel_groups(jobs[i], k);
+  prefix[(size_t)kKernels * (n + 1)] = (int32_t)nq; /* the queue's entry count */
   auto total_of = [&](int k) { return prefix[(size_t)k * (n + 1) + n]; };
   /* upload: [jobs][prefix][merged queue] */
   const size_t o_prefix = align256(sizeof(FrameJob) * n);
@@ -1493,6 +1508,7 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
   const size_t bytes_tab = o_queue + (n > 1 ? (size_t)nq * 4 : 0);
   if (lead->batch_cap < bytes_tab) {
     HIP_OK(lead, hipStreamSynchronize(s));
+    drop_graphs(lead);
     if (lead->batch_dev) (void)hipFree(lead->batch_dev);
     if (lead->batch_pin) (void)hipHostFree(lead->batch_pin);
     lead->batch_dev = lead->batch_pin = nullptr;
@@ -1509,9 +1525,10 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
   /* everything queued on the other contexts' streams (plan uploads, buffer
    * clears) comes first */
   for (JobSpec& j : js)
-    if (j.c != lead) {
+    if (j.c != lead && j.c->dirty) {
       HIP_OK(lead, hipEventRecord(j.c->ev[3], j.c->stream));
       HIP_OK(lead, hipStreamWaitEvent(s, j.c->ev[3], 0));
+      j.c->dirty = false;
     }
   uint8_t* tab = static_cast<uint8_t*>(lead->batch_dev);
   HIP_OK(lead, hipMemcpyAsync(tab, pin, bytes_tab, hipMemcpyHostToDevice, s));
@@ -1520,7 +1537,6 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
   B.prefix = reinterpret_cast<const int32_t*>(tab + o_prefix);
   B.njobs = n;
   B.queue = n > 1 ? reinterpret_cast<const uint32_t*>(tab + o_queue) : nullptr;
-  B.nqueue = (int32_t)nq;
   B.queue_counter = lead->counters;
   B.error = lead->counters + 1;
   B.trace = nullptr;
@@ -1541,8 +1557,12 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
   lead->trace_plan = lead->trace_on && n == 1 ? js[0].plan : nullptr;
 
   int launches = 0;
+  /* Captured launches: every kernel at its full grid, counts read on the
+   * device, so one graph per frame count serves every batch of that size.
+   * Stage timing and the queue trace take the eager path. */
+  const bool use_graph = lead->graphs && !lead->timing && !lead->trace_on;
   auto run = [&](int k, hipStream_t st) -> hipError_t {
-    const int total = total_of(k);
+    const int total = use_graph ? kAnyTotal : total_of(k);
     if (total <= 0) return hipSuccess;
     ++launches;
     switch (k) {
@@ -1555,26 +1575,8 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
       default: return launch_filter_kernel(B, k, total, lead->cus, st);
     }
   };
-  /* The second stream takes what does not depend on the prediction: the
-   * transforms of intra blocks (the queue waits for them) and the grain
-   * templates (the grain blend waits for them). */
-  const bool side_itx = total_of(kKItxIntra) > 0, side_grain = total_of(kKGrainTemplate) > 0;
-  if (side_itx || side_grain) {
-    HIP_OK(lead, hipEventRecord(lead->ev_side_go, s));
-    HIP_OK(lead, hipStreamWaitEvent(lead->side, lead->ev_side_go, 0));
-  }
-  if (side_itx) {
-    HIP_OK(lead, run(kKItxIntra, lead->side));
-    HIP_OK(lead, hipEventRecord(lead->ev_itx_done, lead->side));
-  }
-  if (side_grain) {
-    HIP_OK(lead, run(kKGrainTemplate, lead->side));
-    HIP_OK(lead, hipEventRecord(lead->ev_side_done, lead->side));
-  }
-  HIP_OK(lead, hipEventRecord(lead->ev[0], s));
-  for (bool& h : lead->stage_hit) h = false;
   auto stage = [&](int st, std::initializer_list<int> ks) -> hipError_t {
-    bool any = false;
+    bool any = use_graph;
     for (int k : ks) any = any || total_of(k) > 0;
     if (!any) return hipSuccess;
     if (lead->timing) (void)hipEventRecord(lead->sev[2 * st], s);
@@ -1588,29 +1590,88 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
     }
     return hipSuccess;
   };
-  HIP_OK(lead, stage(AV1H_STAGE_INTER, {kKMc, kKSpecial, kKItxInter}));
-  if (side_itx) HIP_OK(lead, hipStreamWaitEvent(s, lead->ev_itx_done, 0));
-  if (nq > 0) {
-    if (lead->timing) (void)hipEventRecord(lead->sev[2 * AV1H_STAGE_QUEUE], s);
-    HIP_OK(lead, launch_queue(B, lead->cus, s, &launches));
-    if (lead->timing) {
-      (void)hipEventRecord(lead->sev[2 * AV1H_STAGE_QUEUE + 1], s);
-      lead->stage_hit[AV1H_STAGE_QUEUE] = true;
+  /* The second stream takes what does not depend on the prediction: the
+   * transforms of intra blocks (the queue waits for them) and the grain
+   * templates (the grain blend waits for them). */
+  const bool side_itx = use_graph || total_of(kKItxIntra) > 0;
+  const bool side_grain = use_graph || total_of(kKGrainTemplate) > 0;
+  auto sequence = [&]() -> hipError_t {
+    hipError_t e = hipSuccess;
+#define SEQ(call)        \
+  do {                   \
+    e = (call);          \
+    if (e != hipSuccess) return e; \
+  } while (0)
+    if (side_itx || side_grain) {
+      SEQ(hipEventRecord(lead->ev_side_go, s));
+      SEQ(hipStreamWaitEvent(lead->side, lead->ev_side_go, 0));
     }
+    if (side_itx) {
+      SEQ(run(kKItxIntra, lead->side));
+      SEQ(hipEventRecord(lead->ev_itx_done, lead->side));
+    }
+    if (side_grain) {
+      SEQ(run(kKGrainTemplate, lead->side));
+      SEQ(hipEventRecord(lead->ev_side_done, lead->side));
+    }
+    if (!use_graph) SEQ(hipEventRecord(lead->ev[0], s));
+    SEQ(stage(AV1H_STAGE_INTER, {kKMc, kKSpecial, kKItxInter}));
+    if (side_itx) SEQ(hipStreamWaitEvent(s, lead->ev_itx_done, 0));
+    if (use_graph || nq > 0) {
+      if (lead->timing) (void)hipEventRecord(lead->sev[2 * AV1H_STAGE_QUEUE], s);
+      SEQ(launch_queue(B, use_graph ? -1 : (int)nq, lead->cus, s, &launches));
+      if (lead->timing) {
+        (void)hipEventRecord(lead->sev[2 * AV1H_STAGE_QUEUE + 1], s);
+        lead->stage_hit[AV1H_STAGE_QUEUE] = true;
+      }
+    }
+    if (!use_graph) SEQ(hipEventRecord(lead->ev[1], s));
+    SEQ(stage(AV1H_STAGE_DEBLOCK, {kKDeblockV, kKDeblockH}));
+    SEQ(run(kKCopyLrCurr, s));
+    SEQ(run(kKCopyCdefSrc, s));
+    SEQ(stage(AV1H_STAGE_CDEF, {kKCdefDir, kKCdefLuma, kKCdefChroma}));
+    SEQ(run(kKCopyUpSrc, s));
+    SEQ(stage(AV1H_STAGE_SUPERRES, {kKUpscale}));
+    SEQ(run(kKCopyLrCdef, s));
+    SEQ(stage(AV1H_STAGE_LR, {kKLr}));
+    SEQ(stage(AV1H_STAGE_STORE, {kKCopyStore}));
+    if (side_grain) SEQ(hipStreamWaitEvent(s, lead->ev_side_done, 0));
+    SEQ(stage(AV1H_STAGE_GRAIN, {kKGrainApply}));
+    SEQ(run(kKNv12, s));
+#undef SEQ
+    return e;
+  };
+  for (bool& h : lead->stage_hit) h = false;
+  if (use_graph) {
+    hipGraphExec_t exec = nullptr;
+    for (auto& g : lead->graph_exec)
+      if (g.first == n) exec = g.second;
+    if (!exec) {
+      HIP_OK(lead, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      const hipError_t e = sequence();
+      hipGraph_t graph = nullptr;
+      const hipError_t e2 = hipStreamEndCapture(s, &graph);
+      if (e != hipSuccess || e2 != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return set_err(lead, AV1H_ERR_DEVICE, "capturing the batch launches: %s",
+                       hipGetErrorString(e != hipSuccess ? e : e2));
+      }
+      const hipError_t e3 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (e3 != hipSuccess)
+        return set_err(lead, AV1H_ERR_DEVICE, "instantiating the batch graph: %s", hipGetErrorString(e3));
+      if (lead->graph_exec.size() >= 32) {
+        (void)hipGraphExecDestroy(lead->graph_exec.front().second);
+        lead->graph_exec.erase(lead->graph_exec.begin());
+      }
+      lead->graph_exec.emplace_back(n, exec);
+    }
+    HIP_OK(lead, hipEventRecord(lead->ev[0], s));
+    HIP_OK(lead, hipGraphLaunch(exec, s));
+    HIP_OK(lead, hipEventRecord(lead->ev[1], s));
+  } else {
+    HIP_OK(lead, sequence());
   }
-  HIP_OK(lead, hipEventRecord(lead->ev[1], s));
-  HIP_OK(lead, stage(AV1H_STAGE_DEBLOCK, {kKDeblockV, kKDeblockH}));
-  HIP_OK(lead, run(kKCopyLrCurr, s));
-  HIP_OK(lead, run(kKCopyCdefSrc, s));
-  HIP_OK(lead, stage(AV1H_STAGE_CDEF, {kKCdefDir, kKCdefLuma, kKCdefChroma}));
-  HIP_OK(lead, run(kKCopyUpSrc, s));
-  HIP_OK(lead, stage(AV1H_STAGE_SUPERRES, {kKUpscale}));
-  HIP_OK(lead, run(kKCopyLrCdef, s));
-  HIP_OK(lead, stage(AV1H_STAGE_LR, {kKLr}));
-  HIP_OK(lead, stage(AV1H_STAGE_STORE, {kKCopyStore}));
-  if (side_grain) HIP_OK(lead, hipStreamWaitEvent(s, lead->ev_side_done, 0));
-  HIP_OK(lead, stage(AV1H_STAGE_GRAIN, {kKGrainApply}));
-  HIP_OK(lead, run(kKNv12, s));
   HIP_OK(lead, hipEventRecord(lead->ev[2], s));
   const double t_sub = now_ms();
   /* read back */
@@ -1772,11 +1833,14 @@ void av1h_destroy(av1h_ctx* h) {
   Ctx* c = &h->c;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  drop_graphs(c);
   for (void* p : {(void*)c->frame, (void*)c->scratch, (void*)c->lr_curr, (void*)c->lr_cdef, (void*)c->lr_prev_up,
                   (void*)c->pool, (void*)c->cells, (void*)c->counters, (void*)c->grain_tmpl, (void*)c->grain_offs,
                   (void*)c->pack, (void*)c->cdef_dir, (void*)c->trace})
     if (p) (void)hipFree(p);
   if (c->host_pin) (void)hipHostFree(c->host_pin);
+  if (c->batch_dev) (void)hipFree(c->batch_dev);
+  if (c->batch_pin) (void)hipHostFree(c->batch_pin);
   free_plan(&c->scratch_plan);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -2095,6 +2159,17 @@ int64_t av1h_stream_bytes(int width, int height, int bit_depth, int ss_x, int ss
   b += 3 * 82 * 73 * 2 + 65536;                                          /* grain templates, offsets */
   if (nv12_pitch) b += (int64_t)av1h_nv12_size(width, height, ss_x, ss_y, nv12_pitch);
   return b;
+}
+
+int av1h_set_graphs(av1h_ctx* h, int enable) {
+  if (!h) return AV1H_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(h->c.mu);
+  h->c.graphs = enable != 0;
+  if (!enable) {
+    (void)hipSetDevice(h->c.device);
+    drop_graphs(&h->c);
+  }
+  return AV1H_OK;
 }
 
 int av1h_set_decode_nv12(av1h_ctx* h, size_t pitch) {

@@ -282,7 +282,6 @@ struct BatchDev {
   const int32_t* prefix;
   int32_t njobs;
   const uint32_t* queue;     /* intra blocks of all frames in hand-out order, or null */
-  int32_t nqueue;
   uint32_t* queue_counter;
   uint32_t* error;           /* bit 0: a dependency wait timed out   */
   uint64_t* trace;           /* null, or 4 timestamps per queue entry */
@@ -307,8 +306,13 @@ __device__ __forceinline__ FArgs filter_args(const BatchDev& B, int j) { return 
 hipError_t launch_recon_kernel(const BatchDev& b, int k, int total, int cus, hipStream_t s);
 hipError_t launch_filter_kernel(const BatchDev& b, int k, int total, int cus, hipStream_t s);
 hipError_t launch_grain_kernel(const BatchDev& b, int k, int total, int cus, hipStream_t s);
-/* the dependency queue over the intra blocks of the batch */
-hipError_t launch_queue(const BatchDev& b, int cus, hipStream_t s, int* launches);
+/* the dependency queue over the intra blocks of the batch; the entry count
+ * is read on the device (the word after the prefix table), so a captured
+ * launch serves any batch of the same size. `total` < 0: the grid for any
+ * count. */
+hipError_t launch_queue(const BatchDev& b, int total, int cus, hipStream_t s, int* launches);
+/* the grids every kernel takes in a captured graph (its cap) */
+constexpr int kAnyTotal = 1 << 30;
 /* superres: step and initial subpixel position of a plane */
 void superres_params(int dw, int uw, int* step, int* initial);
 /* upscaling workgroups of one frame (1 or 2 pictures) */

@@ -1193,11 +1193,12 @@ __global__ __launch_bounds__(64) void k_inter_special(BatchDev B) {
 __global__ __launch_bounds__(64) void k_queue(BatchDev B) {
   __shared__ int s_work[kWork];
   __shared__ uint16_t s_pred[64 * 64];
+  const int nqueue = kernel_prefix(B, kKernels)[0];
   for (;;) {
     int idx = 0;
     if (threadIdx.x == 0) idx = (int)atomicAdd(B.queue_counter, 1u);
     idx = __builtin_amdgcn_readfirstlane(__shfl(idx, 0));
-    if (idx >= B.nqueue) return;
+    if (idx >= nqueue) return;
     const uint64_t t0 = B.trace ? (uint64_t)wall_clock64() : 0;
     int job = 0, blk;
     if (B.queue) {
@@ -1255,10 +1256,10 @@ hipError_t launch_recon_kernel(const BatchDev& b, int k, int total, int cus, hip
   return hipGetLastError();
 }
 
-hipError_t launch_queue(const BatchDev& b, int cus, hipStream_t s, int* launches) {
-  if (b.nqueue <= 0) return hipSuccess;
+hipError_t launch_queue(const BatchDev& b, int total, int cus, hipStream_t s, int* launches) {
+  if (total == 0) return hipSuccess;
   hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, b.queue_counter, 1);
-  const int grid = min(b.nqueue, cus * 6);
+  const int grid = total < 0 ? cus * 6 : min(total, cus * 6);
   hipLaunchKernelGGL(k_queue, dim3(grid), dim3(64), 0, s, b);
   *launches += 2;
   return hipGetLastError();

@@ -423,6 +423,15 @@ const uint8_t* av1h_nv12_device_buffer(const av1h_ctx* ctx);
  * decoder packs only when asked (Decoder::to_nv12, av1gpu.hpp:745). */
 int av1h_set_decode_nv12(av1h_ctx* ctx, size_t pitch);
 
+/* Captured launches (default on): the first batch of each frame count this
+ * context leads records its launch sequence into a HIP graph, with every
+ * kernel at its full persistent grid reading its work counts from the
+ * batch's device tables; later batches of that count replay it with one
+ * launch. Decodes with stage timing or the queue trace on launch eagerly.
+ * Off: every decode launches eagerly. With captured launches the stats'
+ * device_recon_ms holds the whole batch and device_filter_ms 0. */
+int av1h_set_graphs(av1h_ctx* ctx, int enable);
+
 /* YUV -> RGB24 of a slot (reference Decoder::to_rgb, av1gpu.hpp:739): `matrix`
  * 0 BT.601, 1 BT.709, 2 BT.2020, 3 identity; `range` 0 limited, 1 full. */
 int av1h_to_rgb(av1h_ctx* ctx, int slot, int matrix, int range, uint8_t* out,

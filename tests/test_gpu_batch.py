@@ -99,3 +99,27 @@ def test_decode_time_nv12(extra):
         na = a.to_nv12(-1, pitch, w, h)
         nb = b.to_nv12(-1, pitch, w, h)
         assert bytes(na) == bytes(nb), f"frame {f}"
+
+
+def test_graph_replays_match_eager():
+    """Captured launch sequences (replayed for every batch of the same size,
+    with eager decodes and synchronisations between replays) give the same
+    pictures as eager launches, for batches of 1 and of several frames."""
+    clips = [pkg.Clip("c2_mini", 8), pkg.Clip("tools_mini", 3), pkg.Clip("c1_mini", 9, bit_depth=10)]
+    eager = [pkg.HipDecoder() for _ in clips]
+    for d in eager:
+        d.set_graphs(False)
+    want = [[] for _ in clips]
+    for _f, outs in pkg.decode_clips_batched(eager, clips):
+        for i, _s, planes in outs:
+            want[i].append(planes)
+    for rep in range(2):
+        got = _run_batched(clips)
+        for c, g, w in zip(clips, got, want):
+            for f, (a, b) in enumerate(zip(g, w)):
+                _check(f"graph rep {rep} {c.workload} seed {c.seed}", f, a, b)
+        # an eager decode in between
+        _alone(clips[0])
+    for c, w in zip(clips, want):
+        for f, (a, b) in enumerate(zip(_alone(c), w)):
+            _check(f"graph alone {c.workload}", f, a, b)
