@@ -844,7 +844,7 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
         it.ref_w = d.ref_w;
         it.ref_h = d.ref_h;
         it.nref = d.compound ? 2 : 1;
-        int wsz = 0, msz = 0;
+        int msz = 0;
         for (int r = 0; r < it.nref; ++r) {
           McRefDev& q = it.r[r];
           q.slot = d.slot[r];
@@ -855,14 +855,11 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
           q.wy0 = c2 - 3;
           q.ww = c1 - c0 + 8;
           q.wh = c3 - c2 + 8;
-          wsz += q.ww * q.wh;
           msz += q.wh * d.w;
         }
         const int sb = it.rows * d.w;
-        if (cur.count + 1 > kMcItems || cur.samples + sb > kMcSamples || cur.wtotal + wsz > kMcWin ||
-            cur.mtotal + msz > kMcMid)
-          close_task();
-        if (wsz > kMcWin || msz > kMcMid) return set_err(c, AV1H_ERR_UNSUPPORTED, "motion compensation window");
+        if (cur.count + 1 > kMcItems || cur.samples + sb > kMcSamples || cur.mtotal + msz > kMcMid) close_task();
+        if (msz > kMcMid) return set_err(c, AV1H_ERR_UNSUPPORTED, "motion compensation window");
         it.px0 = cur.samples;
         for (int r = 0; r < 2; ++r) {
           McRefDev& q = it.r[r];

@@ -88,15 +88,14 @@ struct DevTx {
 /* transform classes: the larger side, 4 8 16 32 64 */
 constexpr int kTxClasses = 5;
 /* Motion compensation tasks: at most kMcSamples samples and kMcItems items
- * (blocks, or row strips of large blocks) whose reference windows fit
- * kMcWin samples and whose horizontal pass fits kMcMid values. The host
- * computes each item's positions and windows (the position arithmetic of the
- * reference, av1gpu_kernels.h:1761-1763 and 1819-1840) and its place in the
- * task's LDS, so a workgroup reads one descriptor array and nothing else. */
-constexpr int kMcSamples = 256;
-constexpr int kMcItems = 16;
-constexpr int kMcWin = 6144;
-constexpr int kMcMid = 3072;
+ * (blocks, or row strips of large blocks) whose horizontal pass fits kMcMid
+ * values. The host computes each item's positions and reference windows (the
+ * position arithmetic of the reference, av1gpu_kernels.h:1761-1763 and
+ * 1819-1840) and its place in the task's LDS, so a workgroup reads one
+ * descriptor array and the reference rows and nothing else. */
+constexpr int kMcSamples = 512;
+constexpr int kMcItems = 32;
+constexpr int kMcMid = 4096;
 
 struct McRefDev {
   int32_t slot;

@@ -985,18 +985,19 @@ __device__ void reconstruct_queued(RArgs a, BlkRef b, int* work, uint16_t* pred)
   store_block(a, b, pred);
 }
 
-/* Motion compensation through LDS, one 256 thread workgroup per task (at
- * most 256 samples of consecutive items in coding order, an item being a
- * block or a row strip of a large one):
+/* Motion compensation, one 256 thread workgroup per task (at most 512
+ * samples of consecutive items in coding order, an item being a block or a
+ * row strip of a large one):
  *   A  the task's item descriptors (positions, reference windows, LDS
  *      offsets, all computed by the host) are copied into LDS;
- *   B  the reference windows are loaded into LDS, clamped to the visible
- *      reference the way each tap is (av1gpu_kernels.h:1704-1711), with the
- *      loads of a thread issued in batches;
- *   C  the horizontal 8 tap pass over every window row, rounded by
- *      InterRound0, into an LDS intermediate (16 bit: at most 200 * 4095 >> 5);
- *   D  one thread per output sample: the vertical pass, InterRound1, the
- *      compound blend (av1gpu_kernels.h:2207-2293), the store. */
+ *   B  the horizontal 8 tap pass over every reference row the item's
+ *      vertical taps need, the taps read from the reference and clamped to
+ *      its visible area the way each tap is (av1gpu_kernels.h:1704-1711),
+ *      rounded by InterRound0, into an LDS intermediate (16 bit: at most
+ *      200 * 4095 >> 5); neighbouring threads read neighbouring columns, so
+ *      the 8 reads of a thread come from the same cache lines;
+ *   C  per output sample: the vertical pass, InterRound1, the compound blend
+ *      (av1gpu_kernels.h:2207-2293), the store. */
 constexpr int kMcPairs = 2 * kMcItems;
 
 __device__ __forceinline__ int find_prefix(const int* pre, int n, int v) {
@@ -1031,17 +1032,14 @@ __device__ __forceinline__ McPos mc_pos_of(const McItemDev& it, const McRefDev& 
 }
 
 struct McLds {
-  uint16_t win[kMcWin];
   int16_t mid[kMcMid];
   McItemDev it[kMcItems];
-  int woff[kMcPairs + 1], moff[kMcPairs + 1], px[kMcItems + 1];
+  int moff[kMcPairs + 1], px[kMcItems + 1];
 };
 
 __device__ void mc_body(RArgs a, int tn, const int* s_filt, McLds& L_) {
-  uint16_t* s_win = L_.win;
   int16_t* s_mid = L_.mid;
   McItemDev* s_it = L_.it;
-  int* s_woff = L_.woff;
   int* s_moff = L_.moff;
   int* s_px = L_.px;
   const int tid = threadIdx.x;
@@ -1054,105 +1052,89 @@ __device__ void mc_body(RArgs a, int tn, const int* s_filt, McLds& L_) {
     for (int i = tid; i < n; i += 256) dst[i] = src[i];
   }
   __syncthreads();
-  if (tid < npairs) {
-    s_woff[tid] = s_it[tid >> 1].r[tid & 1].win_off;
-    s_moff[tid] = s_it[tid >> 1].r[tid & 1].mid_off;
-  }
+  if (tid < npairs) s_moff[tid] = s_it[tid >> 1].r[tid & 1].mid_off;
   if (tid < nit) s_px[tid] = s_it[tid].px0;
   if (tid == 0) {
-    s_woff[npairs] = task.wtotal;
     s_moff[npairs] = task.mtotal;
     s_px[nit] = task.samples;
   }
   __syncthreads();
   const int bd = a.lay.bit_depth;
   const int round0 = bd >= 12 ? 5 : 3;
-  /* B: the windows */
-  constexpr int kBatch = 8;
-  for (int e0 = tid; e0 < task.wtotal; e0 += 256 * kBatch) {
-    uint16_t v[kBatch];
-#pragma unroll
-    for (int u = 0; u < kBatch; ++u) {
-      const int e = e0 + u * 256;
-      if (e < task.wtotal) {
-        const int k = find_prefix(s_woff, npairs, e);
-        const McItemDev& it = s_it[k >> 1];
-        const McRefDev& q = it.r[k & 1];
-        const int o = e - q.win_off;
-        const int rr = o / q.ww, cc = o - rr * q.ww;
-        const int y = clampi(q.wy0 + rr, 0, it.ref_h - 1), x = clampi(q.wx0 + cc, 0, it.ref_w - 1);
-        const uint16_t* ref = a.pool + (size_t)q.slot * a.slot_size + a.lay.plane_off[it.plane];
-        v[u] = ref[(size_t)y * a.lay.stride[it.plane] + x];
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < kBatch; ++u)
-      if (e0 + u * 256 < task.wtotal) s_win[e0 + u * 256] = v[u];
-  }
-  __syncthreads();
-  /* C: horizontal pass over every window row */
+  /* horizontal pass over every reference row an item needs, taps read from
+   * the reference (clamped to it) */
   for (int e = tid; e < task.mtotal; e += 256) {
     const int k = find_prefix(s_moff, npairs, e);
     const McItemDev& it = s_it[k >> 1];
     const McRefDev& q = it.r[k & 1];
     const int o = e - q.mid_off;
-    const int rr = o / it.w, x = o - rr * it.w;
-    const int sub_x = it.plane > 0 ? a.lay.ss_x : 0;
+    const int lw = ilog2(it.w);
+    const int rr = o >> lw, x = o & (it.w - 1);
+    const int p = it.plane;
+    const int sub_x = p > 0 ? a.lay.ss_x : 0;
     const McPos P = mc_pos_of(it, q, sub_x, 0);
     int cx, fx;
     P.col(x, &cx, &fx);
-    const uint16_t* row = s_win + q.win_off + rr * q.ww + (cx - 3 - q.wx0);
+    const int y = clampi(q.wy0 + rr, 0, it.ref_h - 1);
+    const int stride = p == 0 ? a.lay.stride[0] : (p == 1 ? a.lay.stride[1] : a.lay.stride[2]);
+    const int64_t poff = p == 0 ? a.lay.plane_off[0] : (p == 1 ? a.lay.plane_off[1] : a.lay.plane_off[2]);
+    const uint16_t* row = a.pool + (size_t)q.slot * a.slot_size + poff + (size_t)y * stride;
     const int* hk = s_filt + it.hk + fx * 8;
     int sum = 0;
+    if (cx >= 3 && cx + 4 < it.ref_w) {
 #pragma unroll
-    for (int t = 0; t < 8; ++t) sum += (int)row[t] * hk[t];
+      for (int t = 0; t < 8; ++t) sum += (int)row[cx - 3 + t] * hk[t];
+    } else {
+#pragma unroll
+      for (int t = 0; t < 8; ++t) sum += (int)row[clampi(cx - 3 + t, 0, it.ref_w - 1)] * hk[t];
+    }
     s_mid[e] = (int16_t)((sum + (1 << (round0 - 1))) >> round0);
   }
   __syncthreads();
-  /* D: vertical pass and blend, one thread per output sample */
-  if (tid >= task.samples) return;
-  const int item = find_prefix(s_px, nit, tid);
-  const McItemDev& it = s_it[item];
-  const int s = tid - it.px0;
-  const int lw = ilog2(it.w), x = s & (it.w - 1), y = it.y0 + (s >> lw);
-  const int compound = it.compound != 0;
-  int round1 = compound ? 7 : 11;
-  if (bd >= 12 && !compound) round1 -= 2;
-  const int sub_y = it.plane > 0 ? a.lay.ss_y : 0;
-  int v[2] = {0, 0};
+  /* vertical pass and blend, one thread per output sample */
+  for (int si = tid; si < task.samples; si += 256) {
+    const int item = find_prefix(s_px, nit, si);
+    const McItemDev& it = s_it[item];
+    const int s = si - it.px0;
+    const int lw = ilog2(it.w), x = s & (it.w - 1), y = it.y0 + (s >> lw);
+    const int compound = it.compound != 0;
+    int round1 = compound ? 7 : 11;
+    if (bd >= 12 && !compound) round1 -= 2;
+    const int sub_y = it.plane > 0 ? a.lay.ss_y : 0;
+    int v[2] = {0, 0};
 #pragma unroll
-  for (int r = 0; r < 2; ++r) {
-    if (r == 1 && !compound) break;
-    const McRefDev& q = it.r[r];
-    const McPos P = mc_pos_of(it, q, 0, sub_y);
-    int cy, fy;
-    P.row(y, &cy, &fy);
-    const int16_t* m = s_mid + q.mid_off + (cy - 3 - q.wy0) * it.w + x;
-    const int* vk = s_filt + it.vk + fy * 8;
-    int acc = 0;
+    for (int r = 0; r < 2; ++r) {
+      if (r == 1 && !compound) break;
+      const McRefDev& q = it.r[r];
+      const McPos P = mc_pos_of(it, q, 0, sub_y);
+      int cy, fy;
+      P.row(y, &cy, &fy);
+      const int16_t* m = s_mid + q.mid_off + (cy - 3 - q.wy0) * it.w + x;
+      const int* vk = s_filt + it.vk + fy * 8;
+      int acc = 0;
 #pragma unroll
-    for (int t = 0; t < 8; ++t) acc += (int)m[t * it.w] * vk[t];
-    v[r] = (acc + (1 << (round1 - 1))) >> round1;
-  }
-  int out = v[0];
-  if (compound) {
-    const int post = 2 * 7 - (round0 + 7);
-    if (it.compound == AV1H_COMPOUND_AVERAGE)
-      out = (v[0] + v[1] + (1 << post)) >> (1 + post);
-    else
-      out = (it.w0 * v[0] + it.w1 * v[1] + (1 << (3 + post))) >> (4 + post);
-  }
-  const int p = it.plane;
-  const int stride = p == 0 ? a.lay.stride[0] : (p == 1 ? a.lay.stride[1] : a.lay.stride[2]);
-  const int64_t poff = p == 0 ? a.lay.plane_off[0] : (p == 1 ? a.lay.plane_off[1] : a.lay.plane_off[2]);
-  a.frame[poff + (int64_t)(it.y + y) * stride + it.x + x] = (uint16_t)clampi(out, 0, a.maxv);
-  if (s == 0 && it.y0 == 0) {
-    /* the block's dependency cells: final once the inter transforms after
-     * this launch have run, which the queue launch follows */
-    const int sh = a.lay.cell_shift[p];
-    const int cw = max(it.w >> sh, 1), ch = max(it.h >> sh, 1);
-    for (int k = 0; k < cw * ch; ++k) {
-      const int cx = (it.x >> sh) + k % cw, cy = (it.y >> sh) + k / cw;
+      for (int t = 0; t < 8; ++t) acc += (int)m[t * it.w] * vk[t];
+      v[r] = (acc + (1 << (round1 - 1))) >> round1;
+    }
+    int out = v[0];
+    if (compound) {
+      const int post = 2 * 7 - (round0 + 7);
+      if (it.compound == AV1H_COMPOUND_AVERAGE)
+        out = (v[0] + v[1] + (1 << post)) >> (1 + post);
+      else
+        out = (it.w0 * v[0] + it.w1 * v[1] + (1 << (3 + post))) >> (4 + post);
+    }
+    const int p = it.plane;
+    const int stride = p == 0 ? a.lay.stride[0] : (p == 1 ? a.lay.stride[1] : a.lay.stride[2]);
+    const int64_t poff = p == 0 ? a.lay.plane_off[0] : (p == 1 ? a.lay.plane_off[1] : a.lay.plane_off[2]);
+    const int ax = it.x + x, ay = it.y + y;
+    a.frame[poff + (int64_t)ay * stride + ax] = (uint16_t)clampi(out, 0, a.maxv);
+    /* the block's dependency cells, each by the thread of its first sample
+     * in the block: final once the inter transforms after this launch have
+     * run, which the queue launch follows */
+    const int sh = a.lay.cell_shift[p], cm = (1 << sh) - 1;
+    if (((ax & cm) == 0 || x == 0) && ((ay & cm) == 0 || y == 0)) {
+      const int cx = ax >> sh, cy = ay >> sh;
       if (cx < a.lay.cells_w[p] && cy < a.lay.cells_h[p])
         a.cells[a.lay.cell_off[p] + (int64_t)cy * a.lay.cells_w[p] + cx] = a.epoch;
     }

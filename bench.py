@@ -19,8 +19,11 @@ returns once its frame is complete on the device) is reported as p50/p99.
 Also reported: the same clip through av1h_decode_wire, whose time includes
 flattening and the upload of the syntax (`ingest_inclusive`); the dominant
 kernel's roofline from HIP events around each stage, on the decode stream,
-during the timed steps; the CPU oracle on a bounded sample of the same clip,
-timed on this host, and the bit-exactness of the GPU output on that sample.
+in one more pass over the clip after the timed steps (the timed steps replay
+captured launch sequences; the event pass launches eagerly); the CPU oracle
+on a bounded sample of the same clip, timed on this host, and the
+bit-exactness of the GPU output on that sample; and the 16-stream mix,
+batched vs plain (`batched_mix`).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
