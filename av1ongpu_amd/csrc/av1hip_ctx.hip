@@ -98,7 +98,11 @@ struct Plan {
    * (source superblock, superblock) */
   int sb_w = 0, sb_h = 0;
   std::vector<float> sb_feat;
-  std::vector<int32_t> queue_sb, queue_sb_static, queue_static, key_off_static, bc_edges;
+  std::vector<int32_t> queue_sb, queue_sb_static, queue_static, key_off_static;
+  /* the queued blocks in coding order (index qi): block, predecessors
+   * (q_dep[q_dep_off[qi] .. q_dep_off[qi + 1])), and the qi of every entry
+   * of the static order */
+  std::vector<int32_t> q_block, q_dep_off, q_dep, queue_qi, q_area;
 };
 
 struct Ctx {
@@ -456,22 +460,9 @@ void sb_features(const Host& H, const av1h_frame_header& hdr, const Layout& L, P
     f[23] = hdr.bit_depth > 8 ? 1.f : 0.f;
   }
   plan->queue_sb.resize(plan->queue_host.size());
-  plan->bc_edges.clear();
   for (size_t k = 0; k < plan->queue_host.size(); ++k) {
     const DevBlock& d = H.blocks[(size_t)plan->queue_host[k]];
-    const int sb = sb_at(d.plane, d.x, d.y);
-    plan->queue_sb[k] = sb;
-    if ((d.flags & kIntraBC) && d.dep_rx1 >= d.dep_rx0 && d.dep_ry1 >= d.dep_ry0) {
-      const int sh_ = L.cell_shift[d.plane];
-      const int a = sb_at(d.plane, d.dep_rx0 << sh_, d.dep_ry0 << sh_);
-      const int b = sb_at(d.plane, d.dep_rx1 << sh_, d.dep_ry1 << sh_);
-      for (int y = a / sw; y <= b / sw; ++y)
-        for (int x = a % sw; x <= b % sw; ++x)
-          if (y * sw + x != sb) {
-            plan->bc_edges.push_back(y * sw + x);
-            plan->bc_edges.push_back(sb);
-          }
-    }
+    plan->queue_sb[k] = sb_at(d.plane, d.x, d.y);
   }
   plan->queue_sb_static = plan->queue_sb;
 }
@@ -770,40 +761,79 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
   H->palette.assign(f.palette_map, f.palette_map + f.num_palette);
   H->palette.resize(H->palette.size() + 8, 0);
 
-  /* The order the queue hands blocks out: superblock anti-diagonals (a 64x64
-   * luma superblock at (sx, sy) has key sx + 2 sy, the wavefront of the
-   * left, above and above-right dependencies), coding order inside a key.
-   * Every block a block waits for has a key no larger and comes earlier in
-   * coding order, so the order is a topological one and the queue always
-   * progresses; an intra block copy takes the largest key of its source. */
+  /* The order the queue hands blocks out. The queued blocks form a graph:
+   * a block depends on the queued blocks that own the cells it waits for
+   * (left column, above row, corner, co-located luma for chroma from luma,
+   * the source of an intra block copy), all earlier in coding order. The
+   * static order is by dependency level (a block's longest chain of queued
+   * predecessors), coding order inside a level: every block a block waits
+   * for is handed out before it, so the queue always progresses, and the
+   * blocks a level holds can all run at once. */
   {
-    std::vector<int64_t> key(H->queue_list.size());
-    for (size_t k = 0; k < H->queue_list.size(); ++k) {
-      const DevBlock& d = H->blocks[H->queue_list[k]];
-      const int sx = d.plane ? hdr.ss_x : 0, sy = d.plane ? hdr.ss_y : 0;
-      int64_t kk = ((d.x << sx) >> 6) + 2 * ((d.y << sy) >> 6);
-      if (d.flags & kIntraBC) {
-        const int sh = L.cell_shift[d.plane];
-        if (d.dep_rx1 >= d.dep_rx0 && d.dep_ry1 >= d.dep_ry0) {
-          const int64_t rk = ((((d.dep_rx1 << sh) << sx)) >> 6) + 2 * (((d.dep_ry1 << sh) << sy) >> 6);
-          kk = std::max(kk, rk);
-        }
+    const int nq = (int)H->queue_list.size();
+    std::vector<int32_t> owner[3];
+    for (int p = 0; p < 3; ++p) owner[p].assign((size_t)L.cells_w[p] * L.cells_h[p], -1);
+    plan->q_dep_off.assign(1, 0);
+    plan->q_dep.clear();
+    plan->q_area.assign((size_t)nq, 0);
+    std::vector<int32_t> level((size_t)nq, 0);
+    std::vector<int32_t> deps;
+    for (int qi = 0; qi < nq; ++qi) {
+      const DevBlock& d = H->blocks[(size_t)H->queue_list[(size_t)qi]];
+      const int p = d.plane, sh = L.cell_shift[p], cell = 1 << sh;
+      const int gw = L.grid_w[p], gh = L.grid_h[p];
+      deps.clear();
+      auto dep = [&](int pl, int cx, int cy) {
+        if (cx < 0 || cy < 0 || cx >= L.cells_w[pl] || cy >= L.cells_h[pl]) return;
+        const int o = owner[pl][(size_t)cy * L.cells_w[pl] + cx];
+        if (o >= 0 && (deps.empty() || deps.back() != o)) deps.push_back(o);
+      };
+      const int ly0 = d.y & ~(cell - 1), ax0 = d.x & ~(cell - 1);
+      if (d.x > 0 && d.dep_y1 >= ly0)
+        for (int y = ly0; y <= std::min(d.dep_y1, gh - 1); y += cell) dep(p, (d.x - 1) >> sh, y >> sh);
+      if (d.y > 0 && d.dep_x1 >= ax0)
+        for (int x = ax0; x <= std::min(d.dep_x1, gw - 1); x += cell) dep(p, x >> sh, (d.y - 1) >> sh);
+      if (d.dep_corner && d.x > 0 && d.y > 0) dep(p, (d.x - 1) >> sh, (d.y - 1) >> sh);
+      if (d.kind == kIntra && d.mode == AV1H_PRED_CFL && p > 0) {
+        const int s0 = L.cell_shift[0];
+        const int lx0 = (d.x << L.ss_x) >> s0, ly0l = (d.y << L.ss_y) >> s0;
+        const int lx1 = std::min((d.x + d.w - 1) << L.ss_x, L.grid_w[0] - 1) >> s0;
+        const int ly1 = std::min((d.y + d.h - 1) << L.ss_y, L.grid_h[0] - 1) >> s0;
+        for (int cy = ly0l; cy <= ly1; ++cy)
+          for (int cx = lx0; cx <= lx1; ++cx) dep(0, cx, cy);
       }
-      key[k] = (kk << 32) | (int64_t)k;
+      if ((d.flags & kIntraBC) && d.dep_rx1 >= d.dep_rx0 && d.dep_ry1 >= d.dep_ry0)
+        for (int cy = d.dep_ry0; cy <= d.dep_ry1; ++cy)
+          for (int cx = d.dep_rx0; cx <= d.dep_rx1; ++cx) dep(p, cx, cy);
+      std::sort(deps.begin(), deps.end());
+      deps.erase(std::unique(deps.begin(), deps.end()), deps.end());
+      int lv = 0;
+      for (int32_t o : deps) lv = std::max(lv, level[(size_t)o] + 1);
+      level[(size_t)qi] = lv;
+      plan->q_area[(size_t)qi] = d.w * d.h;
+      plan->q_dep.insert(plan->q_dep.end(), deps.begin(), deps.end());
+      plan->q_dep_off.push_back((int32_t)plan->q_dep.size());
+      /* the block's own cells */
+      const int cw = std::max(d.w >> sh, 1), ch = std::max(d.h >> sh, 1);
+      for (int cy = d.y >> sh; cy < std::min((d.y >> sh) + ch, L.cells_h[p]); ++cy)
+        for (int cx = d.x >> sh; cx < std::min((d.x >> sh) + cw, L.cells_w[p]); ++cx)
+          owner[p][(size_t)cy * L.cells_w[p] + cx] = qi;
     }
-    std::vector<size_t> ord(H->queue_list.size());
-    for (size_t k = 0; k < ord.size(); ++k) ord[k] = k;
-    std::sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return key[a] < key[b]; });
-    std::vector<int32_t> q(ord.size());
-    for (size_t k = 0; k < ord.size(); ++k) q[k] = H->queue_list[ord[k]];
+    plan->q_block = H->queue_list; /* coding order */
+    std::vector<int32_t> ord((size_t)nq);
+    for (int k = 0; k < nq; ++k) ord[(size_t)k] = k;
+    std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t b) { return level[(size_t)a] < level[(size_t)b]; });
+    std::vector<int32_t> q((size_t)nq);
+    for (int k = 0; k < nq; ++k) q[(size_t)k] = H->queue_list[(size_t)ord[(size_t)k]];
     H->queue_list.swap(q);
     plan->queue_host = H->queue_list;
+    plan->queue_qi = ord;
     plan->key_off.clear();
-    for (size_t k = 0; k < ord.size(); ++k) {
-      const int kk = (int)(key[ord[k]] >> 32);
+    for (int k = 0; k < nq; ++k) {
+      const int kk = level[(size_t)ord[(size_t)k]];
       while ((int)plan->key_off.size() <= kk) plan->key_off.push_back((int32_t)k);
     }
-    plan->key_off.push_back((int32_t)ord.size());
+    plan->key_off.push_back((int32_t)nq);
     plan->queue_static = plan->queue_host;
     plan->key_off_static = plan->key_off;
   }
@@ -2041,63 +2071,49 @@ int av1h_plan_sb_times(av1h_ctx* h, const void* plan, float* us) {
 }
 
 namespace {
-/* Longest remaining path of every superblock through the superblock
- * dependency graph (edges from left, above, above-left, above-right and
- * intra block copy sources), each weighted by its cost; then the queue
- * ordered by that path, longest first, coding order inside a superblock. */
+/* The cost schedule: every queued block costs its superblock's cost shared
+ * by sample count; a block's priority is its longest remaining path through
+ * the block graph (its cost plus the longest path of what waits for it), the
+ * longest first, coding order between equals. A block's predecessors have
+ * strictly longer paths, so the order stays topological. */
 bool schedule_costs(Plan* p, const float* cost) {
-  const int sw = p->sb_w, sh = p->sb_h, n = sw * sh;
-  std::vector<std::vector<int>> succ((size_t)n);
-  std::vector<int> indeg((size_t)n, 0);
-  auto edge = [&](int a, int b) {
-    succ[(size_t)a].push_back(b);
-    ++indeg[(size_t)b];
-  };
-  for (int y = 0; y < sh; ++y)
-    for (int x = 0; x < sw; ++x) {
-      const int u = y * sw + x;
-      if (x > 0) edge(u - 1, u);
-      if (y > 0) edge(u - sw, u);
-      if (x > 0 && y > 0) edge(u - sw - 1, u);
-      if (x + 1 < sw && y > 0) edge(u - sw + 1, u);
-    }
-  for (size_t k = 0; k + 1 < p->bc_edges.size(); k += 2) edge(p->bc_edges[k], p->bc_edges[k + 1]);
-  std::vector<int> topo;
-  topo.reserve((size_t)n);
-  for (int u = 0; u < n; ++u)
-    if (!indeg[(size_t)u]) topo.push_back(u);
-  for (size_t i = 0; i < topo.size(); ++i)
-    for (int v : succ[(size_t)topo[i]])
-      if (--indeg[(size_t)v] == 0) topo.push_back(v);
-  if ((int)topo.size() != n) return false; /* not a DAG: keep the static order */
-  std::vector<double> path((size_t)n, 0.0);
-  for (size_t i = topo.size(); i-- > 0;) {
-    const int u = topo[i];
-    double best = 0.0;
-    for (int v : succ[(size_t)u]) best = std::max(best, path[(size_t)v]);
-    /* a floor keeps every dependency strictly ahead of what waits for it */
-    path[(size_t)u] = std::max((double)cost[u], 0.01) + best;
+  const int nq = (int)p->q_block.size();
+  const int nsb = p->sb_w * p->sb_h;
+  /* queued samples per superblock (feature 14 holds them / 4096) */
+  std::vector<double> area((size_t)nq, 0.0), sb_area((size_t)nsb, 0.0);
+  std::vector<int32_t> sb_of((size_t)nq, 0);
+  for (size_t k = 0; k < p->queue_static.size(); ++k) sb_of[(size_t)p->queue_qi[k]] = p->queue_sb_static[k];
+  for (int qi = 0; qi < nq; ++qi) {
+    const double a = (double)p->q_area[(size_t)qi];
+    area[(size_t)qi] = a;
+    sb_area[(size_t)sb_of[(size_t)qi]] += a;
   }
-  /* entries in coding order (block index), then stably by path */
-  std::vector<std::pair<int32_t, int32_t>> e(p->queue_static.size());
-  for (size_t k = 0; k < e.size(); ++k) e[k] = {p->queue_static[k], p->queue_sb_static[k]};
-  std::sort(e.begin(), e.end());
-  std::stable_sort(e.begin(), e.end(), [&](const std::pair<int32_t, int32_t>& a, const std::pair<int32_t, int32_t>& b) {
-    return path[(size_t)a.second] > path[(size_t)b.second];
-  });
-  /* keys for merging with other frames: remaining path in 0.25 us buckets */
+  std::vector<double> path((size_t)nq, 0.0), succ((size_t)nq, 0.0);
+  for (int qi = nq - 1; qi >= 0; --qi) {
+    const int sb = sb_of[(size_t)qi];
+    const double c = sb_area[(size_t)sb] > 0 ? (double)cost[sb] * area[(size_t)qi] / sb_area[(size_t)sb] : 0.0;
+    path[(size_t)qi] = std::max(c, 0.01) + succ[(size_t)qi];
+    for (int32_t e = p->q_dep_off[(size_t)qi]; e < p->q_dep_off[(size_t)qi + 1]; ++e) {
+      const int32_t o = p->q_dep[(size_t)e];
+      succ[(size_t)o] = std::max(succ[(size_t)o], path[(size_t)qi]);
+    }
+  }
+  std::vector<int32_t> ord((size_t)nq);
+  for (int k = 0; k < nq; ++k) ord[(size_t)k] = k;
+  std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t b) { return path[(size_t)a] > path[(size_t)b]; });
   constexpr double kBucketUs = 0.25;
-  const int top = e.empty() ? 0 : (int)(path[(size_t)e.front().second] / kBucketUs);
-  p->queue_host.resize(e.size());
+  const int top = nq ? (int)(path[(size_t)ord[0]] / kBucketUs) : 0;
+  p->queue_host.resize((size_t)nq);
+  p->queue_sb.resize((size_t)nq);
   p->key_off.clear();
-  for (size_t k = 0; k < e.size(); ++k) {
-    p->queue_host[k] = e[k].first;
-    const int key = top - (int)(path[(size_t)e[k].second] / kBucketUs);
+  for (int k = 0; k < nq; ++k) {
+    const int32_t qi = ord[(size_t)k];
+    p->queue_host[(size_t)k] = p->q_block[(size_t)qi];
+    p->queue_sb[(size_t)k] = sb_of[(size_t)qi];
+    const int key = top - (int)(path[(size_t)qi] / kBucketUs);
     while ((int)p->key_off.size() <= key) p->key_off.push_back((int32_t)k);
   }
-  p->key_off.push_back((int32_t)e.size());
-  p->queue_sb.resize(e.size());
-  for (size_t k = 0; k < e.size(); ++k) p->queue_sb[k] = e[k].second;
+  p->key_off.push_back((int32_t)nq);
   return true;
 }
 
