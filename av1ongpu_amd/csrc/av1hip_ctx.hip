@@ -1386,10 +1386,10 @@ int kernel_groups(const FrameJob& fj, int k) {
     case kKDeblockV: return p.n_deblock_v > 0 ? (p.lines_v + 255) / 256 : 0;
     case kKDeblockH: return p.n_deblock_h > 0 ? (p.lines_h + 255) / 256 : 0;
     case kKCdefDir: return p.n_cdef > 0 ? (p.n_cdef_dir + 3) / 4 : 0;
-    case kKCdefLuma: return p.n_cdef > 0 ? (p.n_cdef_luma + 3) / 4 : 0;
+    case kKCdefLuma: return p.n_cdef > 0 ? (p.n_cdef_luma + 4 * kCdefBatch - 1) / (4 * kCdefBatch) : 0;
     case kKCdefChroma: {
       const int nc = p.n_cdef - p.n_cdef_luma;
-      const int per = 256 / ((8 >> L.ss_x) * (8 >> L.ss_y));
+      const int per = kCdefBatch * 256 / ((8 >> L.ss_x) * (8 >> L.ss_y));
       return nc > 0 ? (nc + per - 1) / per : 0;
     }
     case kKUpscale: return f.upscaled ? upscale_tiles(L, f.uses_lr ? 2 : 1) : 0;

@@ -107,13 +107,25 @@ __device__ void deblock_line(uint16_t* b, int s, int level, int sharp, int fs, i
  * owning line g is found from the first edge of the thread's 256 line chunk
  * (a table the host builds) and a short forward scan. */
 __device__ void deblock_chunk(FArgs a, int first, int count, int lines, const int32_t* chunk,
-                              int ci) {
+                              int ci, int* s_line0) {
+  /* the chunk's lines lie in the 256 edges from its first one (an edge
+   * covers at least one line): their first lines go to LDS, and every
+   * thread finds its edge by binary search there */
+  const DevDeblock* edges = a.plan.deblock + first;
+  const int e0 = chunk[ci], ne = min(count - e0, 256);
+  if ((int)threadIdx.x < ne) s_line0[threadIdx.x] = edges[e0 + threadIdx.x].line0;
+  __syncthreads();
   const int g = ci * 256 + threadIdx.x;
   if (g >= lines) return;
-  const DevDeblock* edges = a.plan.deblock + first;
-  int i = chunk[ci];
-  while (i + 1 < count && edges[i + 1].line0 <= g) ++i;
-  const DevDeblock& e = edges[i];
+  int lo = 0, hi = ne - 1;
+  while (lo < hi) {
+    const int m = (lo + hi + 1) >> 1;
+    if (s_line0[m] <= g)
+      lo = m;
+    else
+      hi = m - 1;
+  }
+  const DevDeblock& e = edges[e0 + lo];
   if (e.level == 0) return;
   const int k = g - e.line0;
   const int p = e.plane, st = a.lay.stride[p];
@@ -132,6 +144,7 @@ __device__ void deblock_chunk(FArgs a, int first, int count, int lines, const in
 
 /* the vertical edges (pass 0) or the horizontal edges (pass 1) of every frame */
 __global__ __launch_bounds__(256) void k_deblock(BatchDev B, int pass) {
+  __shared__ int s_line0[256];
   const AV1H_CMEM int32_t* pre = kernel_prefix(B, (pass ? kKDeblockH : kKDeblockV));
   const int total = pre[B.njobs];
   for (int v = blockIdx.x; v < total; v += gridDim.x) {
@@ -139,9 +152,11 @@ __global__ __launch_bounds__(256) void k_deblock(BatchDev B, int pass) {
     FArgs a = filter_args(B, j);
     PlanRef P = a.plan;
     if (pass)
-      deblock_chunk(a, P.n_deblock_v, P.n_deblock_h, P.lines_h, P.db_chunk + (P.lines_v + 255) / 256, v - pre[j]);
+      deblock_chunk(a, P.n_deblock_v, P.n_deblock_h, P.lines_h, P.db_chunk + (P.lines_v + 255) / 256, v - pre[j],
+                    s_line0);
     else
-      deblock_chunk(a, 0, P.n_deblock_v, P.lines_v, P.db_chunk, v - pre[j]);
+      deblock_chunk(a, 0, P.n_deblock_v, P.lines_v, P.db_chunk, v - pre[j], s_line0);
+    __syncthreads();
   }
 }
 
@@ -247,19 +262,19 @@ __global__ __launch_bounds__(256) void k_cdef_dir(BatchDev B) {
 /* CDEF of the listed blocks (av1gpu_kernels.h:2662-2775), one thread per
  * sample, reading the deblocked snapshot; blocks of `S` samples, 256 / S per
  * workgroup. Samples of unlisted blocks are already final in the frame. */
+/* One sample of block `bi`: false when it keeps the snapshot's value. */
 template <int S>
-__device__ __forceinline__ void cdef_group(FArgs a, int first, int count, int grp) {
-  const int bi = grp * (256 / S) + threadIdx.x / S;
-  if (bi >= count) return;
+__device__ __forceinline__ bool cdef_sample(FArgs a, int first, int bi, int lane, int64_t* off, int* val,
+                                            const int (*dirs)[2][2]) {
   const DevCdef c = a.plan.cdef[first + bi];
   LayRef L = a.lay;
   const int p = c.plane, uv = p > 0;
   const int bd = L.bit_depth, cs = bd - 8;
   const int st = L.stride[p];
   const int w = uv ? (8 >> L.ss_x) : 8;
-  const int r = (threadIdx.x % S) / w, k = (threadIdx.x % S) % w;
+  const int r = lane / w, k = lane % w;
   const int pw = L.grid_w[p], ph = L.grid_h[p];
-  if (c.x + k >= pw || c.y + r >= ph) return;
+  if (c.x + k >= pw || c.y + r >= ph) return false;
   int pri = c.pri << cs;
   const int sec = c.sec << cs;
   const int damping = c.damping + cs - uv;
@@ -276,7 +291,7 @@ __device__ __forceinline__ void cdef_group(FArgs a, int first, int count, int gr
       pri = var ? ((pri * (4 + vs) + 8) >> 4) : 0;
     }
   }
-  if (pri == 0 && sec == 0) return; /* the frame already holds the snapshot's value */
+  if (pri == 0 && sec == 0) return false; /* the frame already holds the snapshot's value */
   const uint16_t* src = a.cdef_src + L.plane_off[p] + (size_t)(c.y + r) * st + c.x + k;
   const int ts = (pri >> cs) & 1;
   const int pt0 = ts ? 3 : 4, pt1 = ts ? 3 : 2;
@@ -286,7 +301,7 @@ __device__ __forceinline__ void cdef_group(FArgs a, int first, int count, int gr
   for (int t = 0; t < 2; ++t)
 #pragma unroll
     for (int sg = -1; sg <= 1; sg += 2) {
-      int dy = sg * kCdefDirs[dir][t][0], dx = sg * kCdefDirs[dir][t][1];
+      int dy = sg * dirs[dir][t][0], dx = sg * dirs[dir][t][1];
       int px = c.x + k + dx, py = c.y + r + dy;
       if (px >= 0 && px < pw && py >= 0 && py < ph) {
         const int v = src[dy * st + dx];
@@ -297,8 +312,8 @@ __device__ __forceinline__ void cdef_group(FArgs a, int first, int count, int gr
 #pragma unroll
       for (int o = -2; o <= 2; o += 4) {
         const int sd = (dir + o) & 7;
-        dy = sg * kCdefDirs[sd][t][0];
-        dx = sg * kCdefDirs[sd][t][1];
+        dy = sg * dirs[sd][t][0];
+        dx = sg * dirs[sd][t][1];
         px = c.x + k + dx;
         py = c.y + r + dy;
         if (px >= 0 && px < pw && py >= 0 && py < ph) {
@@ -309,13 +324,39 @@ __device__ __forceinline__ void cdef_group(FArgs a, int first, int count, int gr
         }
       }
     }
-  a.frame[L.plane_off[p] + (size_t)(c.y + r) * st + c.x + k] =
-      (uint16_t)clampi(clampi(x + ((8 + sum - (sum < 0)) >> 4), lo, hi), 0, a.maxv);
+  *off = L.plane_off[p] + (int64_t)(c.y + r) * st + c.x + k;
+  *val = clampi(clampi(x + ((8 + sum - (sum < 0)) >> 4), lo, hi), 0, a.maxv);
+  return true;
+}
+
+/* kCdefBatch groups of 256 / S blocks per workgroup: a thread takes the same
+ * sample of one block of each group, computes all of them (their reads
+ * independent of each other) and then stores them. */
+
+template <int S>
+__device__ __forceinline__ void cdef_group(FArgs a, int first, int count, int grp, const int (*dirs)[2][2]) {
+  constexpr int kPer = 256 / S;
+  const int lane = threadIdx.x % S;
+  int64_t off[kCdefBatch];
+  int val[kCdefBatch];
+  bool ok[kCdefBatch];
+#pragma unroll
+  for (int u = 0; u < kCdefBatch; ++u) {
+    const int bi = (grp * kCdefBatch + u) * kPer + threadIdx.x / S;
+    ok[u] = bi < count && cdef_sample<S>(a, first, bi, lane, &off[u], &val[u], dirs);
+  }
+#pragma unroll
+  for (int u = 0; u < kCdefBatch; ++u)
+    if (ok[u]) a.frame[off[u]] = (uint16_t)val[u];
 }
 
 /* luma blocks (plane 0) or chroma blocks (plane 1) of every frame; a chroma
  * block has 16, 32 or 64 samples by the frame's subsampling */
 __global__ __launch_bounds__(256) void k_cdef(BatchDev B, int chroma) {
+  /* the direction table in LDS: lanes index it by their own direction */
+  __shared__ int s_dirs[8][2][2];
+  if (threadIdx.x < 32) (&s_dirs[0][0][0])[threadIdx.x] = (&kCdefDirs[0][0][0])[threadIdx.x];
+  __syncthreads();
   const AV1H_CMEM int32_t* pre = kernel_prefix(B, (chroma ? kKCdefChroma : kKCdefLuma));
   const int total = pre[B.njobs];
   for (int v = blockIdx.x; v < total; v += gridDim.x) {
@@ -323,14 +364,14 @@ __global__ __launch_bounds__(256) void k_cdef(BatchDev B, int chroma) {
     FArgs a = filter_args(B, j);
     const int nl = a.plan.n_cdef_luma, grp = v - pre[j];
     if (!chroma) {
-      cdef_group<64>(a, 0, nl, grp);
+      cdef_group<64>(a, 0, nl, grp, s_dirs);
       continue;
     }
     const int nc = a.plan.n_cdef - nl;
     switch ((8 >> a.lay.ss_x) * (8 >> a.lay.ss_y)) {
-      case 16: cdef_group<16>(a, nl, nc, grp); break;
-      case 32: cdef_group<32>(a, nl, nc, grp); break;
-      default: cdef_group<64>(a, nl, nc, grp); break;
+      case 16: cdef_group<16>(a, nl, nc, grp, s_dirs); break;
+      case 32: cdef_group<32>(a, nl, nc, grp, s_dirs); break;
+      default: cdef_group<64>(a, nl, nc, grp, s_dirs); break;
     }
   }
 }

@@ -273,6 +273,9 @@ __device__ __forceinline__ int find_job(const AV1H_CMEM int32_t* pre, int n, int
 constexpr int kQueueJobShift = 24;
 constexpr int kMaxBatch = 128;
 
+/* CDEF: groups of 256 / (block samples) blocks a workgroup takes per step */
+constexpr int kCdefBatch = 1;
+
 /* bytes one copy workgroup moves */
 constexpr int kCopyChunk = 16384;
 
