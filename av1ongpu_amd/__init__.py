@@ -231,6 +231,8 @@ def hip_lib():
         lib.av1h_get_queue_trace.argtypes = [P, ctypes.c_void_p, ctypes.c_int64]
         lib.av1h_set_decode_nv12.argtypes = [P, ctypes.c_size_t]
         lib.av1h_set_graphs.argtypes = [P, ctypes.c_int]
+        lib.av1h_set_output_hash.argtypes = [P, ctypes.c_int]
+        lib.av1h_last_hash.argtypes = [P, ctypes.POINTER(ctypes.c_uint64)]
         lib.av1h_nv12_device_buffer.restype = ctypes.c_void_p
         lib.av1h_nv12_device_buffer.argtypes = [P]
         _bind_server(lib)
@@ -241,14 +243,16 @@ def hip_lib():
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("max_batch", ctypes.c_int32), ("planner_threads", ctypes.c_int32),
-                ("capture", ctypes.c_int32), ("hbm_budget", ctypes.c_int64), ("nv12_pitch", ctypes.c_size_t)]
+                ("capture", ctypes.c_int32), ("hash_pictures", ctypes.c_int32), ("hbm_budget", ctypes.c_int64),
+                ("nv12_pitch", ctypes.c_size_t)]
 
 
 class Completion(ctypes.Structure):
     _fields_ = [("tag", ctypes.c_uint64), ("stream", ctypes.c_int32), ("slot", ctypes.c_int32),
                 ("status", ctypes.c_int32), ("batch_frames", ctypes.c_int32), ("arrival_ms", ctypes.c_double),
                 ("start_ms", ctypes.c_double), ("done_ms", ctypes.c_double),
-                ("picture", ctypes.POINTER(ctypes.c_uint16)), ("picture_samples", ctypes.c_int64)]
+                ("picture", ctypes.POINTER(ctypes.c_uint16)), ("picture_samples", ctypes.c_int64),
+                ("hash", ctypes.c_uint64)]
 
 
 class ServerStats(ctypes.Structure):
@@ -504,6 +508,20 @@ class HipDecoder:
         if rc != AV1H_OK:
             raise HipError(f"schedule_model failed ({rc}): {self.error()}")
 
+    def set_pool_slots(self, slots):
+        if self._lib.av1h_set_pool_slots(self._h, slots) != AV1H_OK:
+            raise HipError(self.error())
+
+    def set_output_hash(self, enable=True):
+        """Checksum every decoded picture on the device (picture_hash for the host version)."""
+        self._lib.av1h_set_output_hash(self._h, 1 if enable else 0)
+
+    def last_hash(self):
+        v = ctypes.c_uint64()
+        if self._lib.av1h_last_hash(self._h, ctypes.byref(v)) != AV1H_OK:
+            raise HipError(self.error())
+        return v.value
+
     def set_graphs(self, enable=True):
         """Replay captured launch sequences (HIP graphs) for batches this context leads."""
         self._lib.av1h_set_graphs(self._h, 1 if enable else 0)
@@ -618,10 +636,11 @@ class Server:
     formed batches."""
 
     def __init__(self, device=0, max_batch=16, planner_threads=4, capture=False, hbm_budget=200 << 30,
-                 nv12_pitch=0):
+                 nv12_pitch=0, hash_pictures=False):
         lib = hip_lib()
         self._lib = lib
-        cfg = ServerConfig(device, max_batch, planner_threads, 1 if capture else 0, hbm_budget, nv12_pitch)
+        cfg = ServerConfig(device, max_batch, planner_threads, 1 if capture else 0, 1 if hash_pictures else 0,
+                           hbm_budget, nv12_pitch)
         h = ctypes.c_void_p()
         rc = lib.av1h_server_create(ctypes.byref(cfg), ctypes.byref(h))
         if rc != AV1H_OK:
@@ -693,6 +712,21 @@ class Server:
 
 def stream_bytes(width, height, bit_depth, ss_x=1, ss_y=1, pool_slots=8, nv12_pitch=0):
     return hip_lib().av1h_stream_bytes(width, height, bit_depth, ss_x, ss_y, pool_slots, nv12_pitch)
+
+
+def _mix64(i):
+    z = (i + 1) * np.uint64(0x9E3779B97F4A7C15)
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def picture_hash(planes):
+    """The device picture checksum (av1h_set_output_hash) of host planes."""
+    with np.errstate(over="ignore"):
+        v = np.concatenate([np.ascontiguousarray(p).reshape(-1) for p in planes]).astype(np.uint64)
+        i = np.arange(v.size, dtype=np.uint64)
+        return int(np.sum(v * _mix64(i), dtype=np.uint64))
 
 
 def slot_schedule(clip, pool_slots=32):

@@ -152,6 +152,10 @@ struct Ctx {
   hipEvent_t sev[2 * AV1H_STAGES] = {};
   /* the frame table of a batch this context leads: jobs, prefix table,
    * merged queue */
+  bool hash_on = false;          /* every decode also checksums its picture */
+  bool hash_valid = false;
+  uint64_t last_hash = 0;
+  uint64_t* hash_dev = nullptr;  /* checksum accumulators of a batch it leads */
   size_t decode_nv12_pitch = 0; /* > 0: every decode also packs NV12 into `pack` */
   bool nv12_current = false;     /* `pack` holds the NV12 of the latest frame       */
   size_t nv12_pitch_done = 0;
@@ -1397,6 +1401,10 @@ int kernel_groups(const FrameJob& fj, int k) {
     case kKGrainTemplate: return f.has_grain ? 1 : 0;
     case kKGrainApply: return f.has_grain ? grain_tiles(L) : 0;
     case kKNv12: return f.nv12 ? (L.height + ((L.height + 1) >> 1)) * ((f.nv12_pitch + 1023) / 1024) : 0;
+    case kKHash: {
+      const int64_t n = (int64_t)L.out_w[0] * L.out_h[0] + 2 * (int64_t)L.out_w[1] * L.out_h[1];
+      return f.hash ? (int)((n + kHashChunk - 1) / kHashChunk) : 0;
+    }
     case kKCopyLrCurr: return f.uses_lr ? copy_groups(L) : 0;
     case kKCopyCdefSrc: return p.n_cdef > 0 && !f.uses_lr ? copy_groups(L) : 0;
     case kKCopyUpSrc: return f.upscaled ? copy_groups(L) : 0;
@@ -1482,6 +1490,7 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
   hipStream_t s = lead->stream;
   /* the frame table */
   std::vector<FrameJob> jobs(n);
+  bool any_hash = false;
   for (int i = 0; i < n; ++i) {
     Ctx* c = js[i].c;
     Plan* plan = js[i].plan;
@@ -1522,6 +1531,8 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
                       av1h_nv12_size(L.width, L.height, L.ss_x, L.ss_y, c->decode_nv12_pitch) > 0;
     fa.nv12 = nv12 ? c->pack : nullptr;
     fa.nv12_pitch = nv12 ? (int32_t)c->decode_nv12_pitch : 0;
+    fa.hash = c->hash_on ? lead->hash_dev + i : nullptr;
+    any_hash = any_hash || c->hash_on;
   }
   std::vector<int32_t> prefix((size_t)kKernels * (n + 1) + 1, 0);
   for (int k = 0; k < kKernels; ++k)
@@ -1665,6 +1676,10 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
     if (side_grain) SEQ(hipStreamWaitEvent(s, lead->ev_side_done, 0));
     SEQ(stage(AV1H_STAGE_GRAIN, {kKGrainApply}));
     SEQ(run(kKNv12, s));
+    if (use_graph || any_hash) {
+      SEQ(launch_zero64(lead->hash_dev, n, s));
+      SEQ(run(kKHash, s));
+    }
 #undef SEQ
     return e;
   };
@@ -1719,6 +1734,8 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
   }
   uint32_t err_word = 0;
   HIP_OK(lead, hipMemcpyAsync(&err_word, &lead->counters[1], 4, hipMemcpyDeviceToHost, s));
+  uint64_t hashes[kMaxBatch];
+  if (any_hash) HIP_OK(lead, hipMemcpyAsync(hashes, lead->hash_dev, (size_t)n * 8, hipMemcpyDeviceToHost, s));
   HIP_OK(lead, hipStreamSynchronize(s));
   const double t_done = now_ms();
   for (JobSpec& j : js) j.c->have_frame = true;
@@ -1752,6 +1769,8 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
     }
     c->last_slot = j.slot;
     c->nv12_current = jobs[&j - js.data()].f.nv12 != nullptr;
+    c->hash_valid = c->hash_on;
+    if (c->hash_on) c->last_hash = hashes[&j - js.data()];
     c->nv12_pitch_done = c->decode_nv12_pitch;
     if (j.out) {
       const uint16_t* h = static_cast<const uint16_t*>(c->host_pin);
@@ -1835,6 +1854,7 @@ int av1h_create(int device_index, av1h_ctx** out) {
   std::memset(&c->stats, 0, sizeof(c->stats));
   if (hipSetDevice(device_index) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc(&c->counters, 64) != hipSuccess || hipMemset(c->counters, 0, 64) != hipSuccess ||
+      hipMalloc(&c->hash_dev, kMaxBatch * 8) != hipSuccess ||
       hipMalloc(&c->grain_tmpl, 3 * 82 * 73 * 2) != hipSuccess) {
     delete h;
     return AV1H_ERR_DEVICE;
@@ -1861,6 +1881,7 @@ void av1h_destroy(av1h_ctx* h) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   drop_graphs(c);
+  if (c->hash_dev) (void)hipFree(c->hash_dev);
   for (void* p : {(void*)c->frame, (void*)c->scratch, (void*)c->lr_curr, (void*)c->lr_cdef, (void*)c->lr_prev_up,
                   (void*)c->pool, (void*)c->cells, (void*)c->counters, (void*)c->grain_tmpl, (void*)c->grain_offs,
                   (void*)c->pack, (void*)c->cdef_dir, (void*)c->trace})
@@ -2174,6 +2195,21 @@ int64_t av1h_stream_bytes(int width, int height, int bit_depth, int ss_x, int ss
   return b;
 }
 
+int av1h_set_output_hash(av1h_ctx* h, int enable) {
+  if (!h) return AV1H_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(h->c.mu);
+  h->c.hash_on = enable != 0;
+  return AV1H_OK;
+}
+
+int av1h_last_hash(av1h_ctx* h, uint64_t* out) {
+  if (!h || !out) return AV1H_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(h->c.mu);
+  if (!h->c.hash_valid) return set_err(&h->c, AV1H_ERR_INVALID, "the last decode computed no checksum");
+  *out = h->c.last_hash;
+  return AV1H_OK;
+}
+
 int av1h_set_graphs(av1h_ctx* h, int enable) {
   if (!h) return AV1H_ERR_INVALID;
   std::lock_guard<std::mutex> lk(h->c.mu);
@@ -2369,12 +2405,20 @@ int av1h_decode_batch(av1h_ctx* const* ctxs, const av1h_frame_desc* const* frame
 
 int av1h_decode_planned_batch(av1h_ctx* const* ctxs, void* const* plans, const int* keep, const int* reserved,
                               int count, int* slots_out) {
+  if (!ctxs || count < 1) return AV1H_ERR_INVALID;
+  return av1h_decode_planned_batch_on(ctxs[0], ctxs, plans, keep, reserved, count, slots_out);
+}
+
+int av1h_decode_planned_batch_on(av1h_ctx* launcher, av1h_ctx* const* ctxs, void* const* plans, const int* keep,
+                                 const int* reserved, int count, int* slots_out) {
   int rc = check_batch_args(ctxs, count);
-  if (rc != AV1H_OK || !plans) return AV1H_ERR_INVALID;
+  if (rc != AV1H_OK || !plans || !launcher) return AV1H_ERR_INVALID;
   std::vector<Ctx*> cs(count);
   for (int i = 0; i < count; ++i) cs[i] = &ctxs[i]->c;
-  BatchLock lk(cs);
-  Ctx* lead = cs[0];
+  std::vector<Ctx*> locked = cs;
+  locked.push_back(&launcher->c);
+  BatchLock lk(locked);
+  Ctx* lead = &launcher->c;
   (void)hipSetDevice(lead->device);
   std::vector<JobSpec> js(count);
   for (int i = 0; i < count; ++i) {

@@ -709,6 +709,49 @@ __global__ __launch_bounds__(256) void k_nv12_batch(BatchDev B) {
   }
 }
 
+/* Picture checksums (mix64 weights, see the header): kHashChunk samples per
+ * workgroup, 16 per thread, a wave sum added to the frame's accumulator. */
+__global__ __launch_bounds__(256) void k_hash(BatchDev B) {
+  const AV1H_CMEM int32_t* pre = kernel_prefix(B, kKHash);
+  const int total = pre[B.njobs];
+  for (int v = blockIdx.x; v < total; v += gridDim.x) {
+    const int j = find_job(pre, B.njobs, v);
+    FArgs a = filter_args(B, j);
+    LayRef L = a.lay;
+    const int64_t n0 = (int64_t)L.out_w[0] * L.out_h[0], n1 = (int64_t)L.out_w[1] * L.out_h[1];
+    const int64_t n = n0 + 2 * n1;
+    uint64_t acc = 0;
+    const int64_t base = (int64_t)(v - pre[j]) * kHashChunk;
+#pragma unroll 4
+    for (int k = 0; k < kHashChunk / 256; ++k) {
+      const int64_t i = base + k * 256 + threadIdx.x;
+      if (i < n) {
+        int p = 0;
+        int64_t r = i;
+        if (r >= n0) {
+          r -= n0;
+          p = 1;
+          if (r >= n1) {
+            r -= n1;
+            p = 2;
+          }
+        }
+        const int w = L.out_w[p];
+        const int y = (int)(r / w), x = (int)(r - (int64_t)y * w);
+        acc += (uint64_t)a.frame[L.plane_off[p] + (int64_t)y * L.stride[p] + x] * mix64((uint64_t)i);
+      }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if ((threadIdx.x & 63) == 0) atomicAdd((unsigned long long*)a.hash, (unsigned long long)acc);
+  }
+}
+
+__global__ void k_zero64(uint64_t* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
 struct ColorM {
   int m[8];
   int o[3];
@@ -760,6 +803,7 @@ hipError_t launch_filter_kernel(const BatchDev& b, int k, int total, int cus, hi
     case kKUpscale: hipLaunchKernelGGL(k_upscale, dim3(min(total, cus * 8)), blk, 0, s, b); break;
     case kKLr: hipLaunchKernelGGL(k_lr, dim3(min(total, cus * 8)), blk, 0, s, b); break;
     case kKNv12: hipLaunchKernelGGL(k_nv12_batch, dim3(min(total, cus * 16)), blk, 0, s, b); break;
+    case kKHash: hipLaunchKernelGGL(k_hash, dim3(min(total, cus * 8)), blk, 0, s, b); break;
     case kKCopyLrCurr:
     case kKCopyCdefSrc:
     case kKCopyUpSrc:
@@ -767,6 +811,11 @@ hipError_t launch_filter_kernel(const BatchDev& b, int k, int total, int cus, hi
     case kKCopyStore: hipLaunchKernelGGL(k_copy, dim3(min(total, cus * 8)), blk, 0, s, b, k); break;
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_zero64(uint64_t* p, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero64, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
   return hipGetLastError();
 }
 

@@ -228,6 +228,7 @@ struct FilterArgs {
   uint16_t* store;      /* reference slot to store into, or null  */
   uint8_t* nv12;        /* NV12 output on the device, or null     */
   int32_t nv12_pitch;
+  uint64_t* hash;       /* picture checksum accumulator, or null  */
 };
 
 struct FrameJob {
@@ -240,7 +241,7 @@ struct FrameJob {
  * their frame in a prefix table (kernel k, frame j: prefix[k * (jobs + 1) + j]). */
 enum KernelId {
   kKMc, kKSpecial, kKItxInter, kKItxIntra, kKDeblockV, kKDeblockH, kKCdefDir, kKCdefLuma, kKCdefChroma,
-  kKUpscale, kKLr, kKGrainTemplate, kKGrainApply, kKNv12,
+  kKUpscale, kKLr, kKGrainTemplate, kKGrainApply, kKNv12, kKHash,
   kKCopyLrCurr, kKCopyCdefSrc, kKCopyUpSrc, kKCopyLrCdef, kKCopyStore,
   kKernels
 };
@@ -275,6 +276,20 @@ constexpr int kMaxBatch = 128;
 
 /* CDEF: groups of 256 / (block samples) blocks a workgroup takes per step */
 constexpr int kCdefBatch = 1;
+
+/* samples one checksum workgroup reads */
+constexpr int kHashChunk = 4096;
+
+/* The picture checksum: the display samples of Y, U, V in raster order, the
+ * i-th weighted by mix64(i), summed modulo 2^64 (mix64: the SplitMix64
+ * finaliser of i + 1). Order independent to compute, position dependent in
+ * value: any change of a sample or of where it is changes it. */
+__host__ __device__ inline uint64_t mix64(uint64_t i) {
+  uint64_t z = (i + 1) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
 
 /* bytes one copy workgroup moves */
 constexpr int kCopyChunk = 16384;
@@ -321,6 +336,8 @@ void superres_params(int dw, int uw, int* step, int* initial);
 int upscale_tiles(const Layout& lay, int pictures);
 /* grain blend workgroups of one frame */
 int grain_tiles(const Layout& lay);
+/* clears n 64 bit words (a kernel, so captured launches carry no memset) */
+hipError_t launch_zero64(uint64_t* p, int n, hipStream_t s);
 /* picture conversions outside a decode */
 hipError_t launch_nv12(const Layout& lay, const uint16_t* src, uint8_t* out, int pitch, hipStream_t s);
 hipError_t launch_rgb(const Layout& lay, const uint16_t* src, uint8_t* out, const int m[8],

@@ -78,6 +78,7 @@ struct av1h_server {
   std::thread sched;
   std::string err;
   av1h_server_stats stats;
+  av1h_ctx* launcher = nullptr; /* the stream and captured launches every batch uses */
 };
 
 namespace {
@@ -96,6 +97,7 @@ void admit(av1h_server* s) {
     }
     av1h_set_pool_slots(ctx, st->slots);
     if (s->cfg.nv12_pitch && st->ssx == 1 && st->ssy == 1) av1h_set_decode_nv12(ctx, s->cfg.nv12_pitch);
+    if (s->cfg.hash_pictures) av1h_set_output_hash(ctx, 1);
     st->ctx = ctx;
     st->state = StreamState::Admitted;
     s->used += st->bytes;
@@ -122,7 +124,7 @@ void retire(av1h_server* s, SStream* st) {
 }
 
 void complete(av1h_server* s, SFrame* f, int slot, int status, double start, double end, int batch,
-              uint16_t* pic, int64_t samples) {
+              uint16_t* pic, int64_t samples, uint64_t hash) {
   av1h_completion c;
   std::memset(&c, 0, sizeof(c));
   c.tag = f->tag;
@@ -135,6 +137,7 @@ void complete(av1h_server* s, SFrame* f, int slot, int status, double start, dou
   c.done_ms = end;
   c.picture = pic;
   c.picture_samples = samples;
+  c.hash = hash;
   s->done.push_back(c);
 }
 
@@ -205,7 +208,7 @@ void sched_main(av1h_server* s) {
       SFrame* f = st->q.front();
       st->q.pop_front();
       const double t = clock_ms();
-      complete(s, f, -1, f->plan_rc, t, t, 0, nullptr, 0);
+      complete(s, f, -1, f->plan_rc, t, t, 0, nullptr, 0, 0);
       delete f;
       s->cv_done.notify_all();
       retire(s, st);
@@ -234,13 +237,17 @@ void sched_main(av1h_server* s) {
     }
     lk.unlock();
     const double t0 = clock_ms();
-    const int rc = av1h_decode_planned_batch(ctxs.data(), plans.data(), keep.data(), reserved.data(), n, slots.data());
+    const int rc = av1h_decode_planned_batch_on(s->launcher, ctxs.data(), plans.data(), keep.data(), reserved.data(),
+                                                n, slots.data());
     const double t1 = clock_ms();
     std::vector<uint16_t*> pics(n, nullptr);
     std::vector<int64_t> samples(n, 0);
+    std::vector<uint64_t> hashes(n, 0);
+    if (rc == AV1H_OK && s->cfg.hash_pictures)
+      for (int i = 0; i < n; ++i) av1h_last_hash(ctxs[i], &hashes[i]);
     if (rc == AV1H_OK && s->cfg.capture)
       for (int i = 0; i < n; ++i) capture(ctxs[i], &pics[i], &samples[i]);
-    std::string msg = rc == AV1H_OK ? std::string() : std::string(av1h_last_error(ctxs[0]));
+    std::string msg = rc == AV1H_OK ? std::string() : std::string(av1h_last_error(s->launcher));
     for (int i = 0; i < n; ++i) {
       SFrame* f = frames[i];
       for (int r : f->release) av1h_release_reference(ctxs[i], r);
@@ -257,7 +264,7 @@ void sched_main(av1h_server* s) {
       SFrame* f = frames[i];
       st->q.pop_front();
       st->busy = false;
-      complete(s, f, rc == AV1H_OK ? slots[i] : -1, rc, t0, t1, n, pics[i], samples[i]);
+      complete(s, f, rc == AV1H_OK ? slots[i] : -1, rc, t0, t1, n, pics[i], samples[i], hashes[i]);
       delete f;
     }
     for (int i = 0; i < n; ++i) retire(s, ready[i]);
@@ -299,6 +306,10 @@ int av1h_server_create(const av1h_server_config* cfg, av1h_server** out) {
   if (rc != AV1H_OK) return rc;
   av1h_server* s = new av1h_server;
   s->cfg = *cfg;
+  if (av1h_create(cfg->device, &s->launcher) != AV1H_OK) {
+    delete s;
+    return AV1H_ERR_DEVICE;
+  }
   std::memset(&s->stats, 0, sizeof(s->stats));
   for (int i = 0; i < cfg->planner_threads; ++i) s->planners.emplace_back(planner_main, s);
   s->sched = std::thread(sched_main, s);
@@ -326,6 +337,7 @@ void av1h_server_destroy(av1h_server* s) {
     delete st;
   }
   for (av1h_completion& c : s->done) delete[] c.picture;
+  av1h_destroy(s->launcher);
   delete s;
 }
 

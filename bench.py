@@ -257,6 +257,144 @@ def run_mix(args):
     }
 
 
+# The serving workload: 64 streams, Poisson arrivals of 1200 frames/s in all,
+# 1280x720-workload-style classes with the 1920x1080 full-tool statistics.
+SERVE_CLASSES = [  # (name, streams, width, height, bit depth, coded width, grain)
+    ("720p 8-bit", 26, 1280, 720, 8, 0, 0),
+    ("1080p 10-bit + grain", 25, 1920, 1080, 10, 0, 1),
+    ("2160p 10-bit + superres 3/4", 13, 3840, 2160, 10, 2880, 0),
+]
+SERVE_RATE = 1200.0
+SERVE_GOP = 16
+SERVE_PITCH = 4096
+
+
+def run_serve(args):
+    """Frames of 64 streams arrive as Poisson processes and go through the
+    serving loop (av1h_server_*: admission against a 200 GB budget,
+    continuous batching, NV12 packed with every frame). Every completed
+    frame's device checksum is compared with the same frame decoded alone.
+    Latency: arrival to decoded, stored and NV12-packed."""
+    import threading
+    import av1ongpu_amd as pkg
+    import numpy as np
+
+    t = time.perf_counter()
+    gops, plans, scheds, want = [], [], [], []
+    owner = pkg.HipDecoder(0)
+    for ci, (name, _n, w, h, bd, cw, grain) in enumerate(SERVE_CLASSES):
+        clip = pkg.Clip("c2", args.seed * 100 + ci, width=w, height=h, bit_depth=bd, coded_width=cw, grain=grain,
+                        frames=SERVE_GOP)
+        keep, rel, refs = pkg.slot_schedule(clip, 16)
+        ps = [owner.plan_wire(pkg.wire_set_slots(clip.wire(f), refs[f])) for f in range(clip.frames)]
+        # the checksums of the group decoded alone, through the same plans
+        ref = pkg.HipDecoder(0)
+        ref.set_output_hash(True)
+        ref.set_pool_slots(16)
+        hs = []
+        for f in range(clip.frames):
+            ref.decode_planned(ps[f], keep=keep[f] >= 0, reserved=keep[f])
+            hs.append(ref.last_hash())
+            for sl in rel[f]:
+                ref.release(sl)
+        ref.close()
+        del clip
+        plans.append(ps)
+        scheds.append((keep, rel))
+        want.append(hs)
+    log(f"serve: planned {len(SERVE_CLASSES)} groups of {SERVE_GOP} frames in {time.perf_counter() - t:.1f} s")
+
+    srv = pkg.Server(0, max_batch=args.serve_batch, planner_threads=0, hbm_budget=200 << 30,
+                     nv12_pitch=SERVE_PITCH, hash_pictures=True)
+    streams = []  # (stream id, class)
+    for ci, (_name, n, w, h, bd, _cw, _g) in enumerate(SERVE_CLASSES):
+        for _ in range(n):
+            streams.append((srv.open_stream(w, h, bd, 1, 1, 16), ci))
+    rng = np.random.default_rng(args.seed)
+    per = SERVE_RATE / len(streams)
+    warm, dur = args.serve_warmup, args.serve_seconds
+    events = []
+    for si in range(len(streams)):
+        tt = rng.exponential(1.0 / per)
+        k = 0
+        while tt < warm + dur:
+            events.append((tt, si, k))
+            tt += rng.exponential(1.0 / per)
+            k += 1
+    events.sort()
+    done = []
+    stop = threading.Event()
+
+    def poller():
+        deadline = None
+        while len(done) < len(events):
+            done.extend(srv.poll(256, 20))
+            if stop.is_set():
+                deadline = deadline or time.perf_counter() + 60.0
+                if time.perf_counter() > deadline:
+                    break
+
+    th = threading.Thread(target=poller, daemon=True)
+    th.start()
+    t0 = srv.now_ms()
+    for tt, si, k in events:
+        sid, ci = streams[si]
+        target = t0 + tt * 1e3
+        while True:
+            now = srv.now_ms()
+            if now >= target:
+                break
+            time.sleep(min(0.002, (target - now) / 1e3))
+        keep, rel = scheds[ci]
+        g = k % SERVE_GOP
+        srv.submit_plan(sid, plans[ci][g], keep[g], rel[g], arrival_ms=target, tag=(si << 32) | k)
+    stop.set()
+    th.join(timeout=120)
+    st = srv.stats()
+    srv.close()
+    for ps in plans:
+        for p in ps:
+            owner.discard_plan(p)
+    owner.close()
+    px = [w * h for _n, _c, w, h, _b, _cw, _g in SERVE_CLASSES]
+    lat = {ci: [] for ci in range(len(SERVE_CLASSES))}
+    mism = 0
+    t_lo, t_hi = t0 + warm * 1e3, t0 + (warm + dur) * 1e3
+    pix = 0
+    last_done = t_lo
+    for c in done:
+        si, k = c["tag"] >> 32, c["tag"] & 0xffffffff
+        ci = streams[si][1]
+        if c["status"] != 0 or c["hash"] != want[ci][k % SERVE_GOP]:
+            mism += 1
+        if c["arrival_ms"] >= t_lo:
+            lat[ci].append(c["done_ms"] - c["arrival_ms"])
+            pix += px[ci]
+            last_done = max(last_done, c["done_ms"])
+    allv = [v for ci in lat for v in lat[ci]]
+    span = (max(last_done, t_hi) - t_lo) / 1e3
+    rep = {
+        "workload": "64 streams (26x 1280x720 8-bit, 25x 1920x1080 10-bit with film grain, 13x 3840x2160 10-bit "
+                    "with superres 3/4), full-tool syntax statistics, Poisson arrivals of 1200 frames/s in all; "
+                    f"each stream cycles through a {SERVE_GOP}-frame group (a key frame and 15 inter frames, 3 "
+                    "references), syntax planned into HBM beforehand; NV12 packed with every frame",
+        "seconds": dur, "frames": len(allv), "completed": len(done), "submitted": len(events),
+        "offered_mpixel_s": round(sum(px[streams[si][1]] for tt, si, k in events if tt >= warm) / dur / 1e6, 1),
+        "value": round(pix / span / 1e6, 1), "unit": "Mpixel/s",
+        "latency_ms": {"p50": round(pct(allv, 50), 3), "p99": round(pct(allv, 99), 3)},
+        "latency_ms_by_class": {SERVE_CLASSES[ci][0]: {"p50": round(pct(lat[ci], 50), 3),
+                                                       "p99": round(pct(lat[ci], 99), 3)}
+                                for ci in lat if lat[ci]},
+        "batches": st["batches"], "mean_batch_frames": round(st["frames"] / max(st["batches"], 1), 2),
+        "device_busy_frac": round(st["busy_ms"] / (max(c["done_ms"] for c in done) - t0), 3),
+        "bit_exact": {"frames_checked": len(done), "mismatched_frames": mism,
+                      "method": "device checksum of every completed picture vs the same frame decoded alone"},
+    }
+    log(f"serve: {rep['value']} Mpixel/s, p50 {rep['latency_ms']['p50']} ms, p99 {rep['latency_ms']['p99']} ms, "
+        f"{mism} mismatched")
+    return rep
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -268,6 +406,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline and the parity sample")
     ap.add_argument("--no-mix", action="store_true", help="skip the 16-stream batched vs plain mix")
     ap.add_argument("--mix-only", action="store_true", help="only the 16-stream mix")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
+    ap.add_argument("--serve-only", action="store_true", help="only the serving workload")
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--serve-warmup", type=float, default=1.0)
+    ap.add_argument("--serve-batch", type=int, default=32)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this backend decodes each stream on one device and has no "
@@ -281,6 +424,9 @@ def main():
 
     if args.mix_only:
         print(json.dumps({"batched_mix": run_mix(args)}), flush=True)
+        return
+    if args.serve_only:
+        print(json.dumps({"serving": run_serve(args)}), flush=True)
         return
     dec = pkg.HipDecoder(0)
     log(f"device: {dec.device_info().name.decode(errors='replace')}")
@@ -382,6 +528,8 @@ def main():
             log(f"MISMATCH on frames {mism[:10]}")
     if not args.no_mix:
         result["batched_mix"] = run_mix(args)
+    if not args.no_serve:
+        result["serving"] = run_serve(args)
     print(json.dumps(result), flush=True)
 
 

@@ -432,6 +432,15 @@ int av1h_set_decode_nv12(av1h_ctx* ctx, size_t pitch);
  * device_recon_ms holds the whole batch and device_filter_ms 0. */
 int av1h_set_graphs(av1h_ctx* ctx, int enable);
 
+/* With `enable`, every decode on this context also computes a 64 bit
+ * checksum of its output picture on the device (the display samples of Y,
+ * U, V in raster order, the i-th multiplied by SplitMix64's finaliser of
+ * i + 1, summed modulo 2^64), read back with the decode's completion:
+ * callers that keep pictures on the device compare outputs without copying
+ * them. No counterpart in the reference. */
+int av1h_set_output_hash(av1h_ctx* ctx, int enable);
+int av1h_last_hash(av1h_ctx* ctx, uint64_t* out);
+
 /* YUV -> RGB24 of a slot (reference Decoder::to_rgb, av1gpu.hpp:739): `matrix`
  * 0 BT.601, 1 BT.709, 2 BT.2020, 3 identity; `range` 0 limited, 1 full. */
 int av1h_to_rgb(av1h_ctx* ctx, int slot, int matrix, int range, uint8_t* out,
@@ -506,6 +515,12 @@ int av1h_decode_batch(av1h_ctx* const* ctxs, const av1h_frame_desc* const* frame
  * stream of ctxs[0], errors to av1h_last_error(ctxs[0]). */
 int av1h_decode_planned_batch(av1h_ctx* const* ctxs, void* const* plans, const int* keep,
                               const int* reserved, int count, int* slots_out);
+/* The same with the launches on `launcher`'s stream (its captured launch
+ * sequences, its batch tables), which need not be one of the batch's
+ * contexts: a caller that forms batches from changing sets of streams keeps
+ * one launcher, so one captured sequence per batch size serves them all. */
+int av1h_decode_planned_batch_on(av1h_ctx* launcher, av1h_ctx* const* ctxs, void* const* plans,
+                                 const int* keep, const int* reserved, int count, int* slots_out);
 
 /* ------------------------------------------------------ superblock costs */
 /*
@@ -582,6 +597,7 @@ typedef struct av1h_server_config {
   int32_t max_batch;        /* frames per batch, 1..128                        */
   int32_t planner_threads;  /* host threads flattening submitted wire frames   */
   int32_t capture;          /* 1: every completion carries a host copy         */
+  int32_t hash_pictures;    /* 1: every completion carries a picture checksum  */
   int64_t hbm_budget;       /* bytes of admitted stream state                  */
   size_t nv12_pitch;        /* > 0: NV12 packed on the device with each frame   */
 } av1h_server_config;
@@ -594,6 +610,7 @@ typedef struct av1h_completion {
   double arrival_ms, start_ms, done_ms; /* av1h_server_now_ms clock            */
   uint16_t* picture;        /* capture: Y, U, V display planes back to back     */
   int64_t picture_samples;  /* free with av1h_server_free_picture               */
+  uint64_t hash;            /* checksum of the picture (hash_pictures)          */
 } av1h_completion;
 
 typedef struct av1h_server_stats {

@@ -81,3 +81,12 @@ def _run_decode(case, planned):
 @pytest.mark.parametrize("planned", [False, True], ids=["direct", "planned"])
 def test_cpp_api_decodes_golden(case, planned):
     _run_decode(case, planned)
+
+
+def test_picture_hash_is_position_dependent():
+    import numpy as np
+    a = [np.arange(12, dtype=np.uint16).reshape(3, 4), np.ones((2, 2), np.uint16), np.zeros((2, 2), np.uint16)]
+    b = [a[0].copy(), a[1].copy(), a[2].copy()]
+    b[0][0, 1], b[0][0, 2] = b[0][0, 2], b[0][0, 1]
+    assert pkg.picture_hash(a) != pkg.picture_hash(b)
+    assert pkg.picture_hash(a) == pkg.picture_hash([p.copy() for p in a])

@@ -123,3 +123,19 @@ def test_graph_replays_match_eager():
     for c, w in zip(clips, want):
         for f, (a, b) in enumerate(zip(_alone(c), w)):
             _check(f"graph alone {c.workload}", f, a, b)
+
+
+def test_output_hash_matches_host():
+    """The device picture checksum equals the host restatement over the read
+    back planes, alone and in a batch."""
+    clips = [pkg.Clip("c2_mini", 10), pkg.Clip("c1_mini", 11, width=426, height=240, frames=2)]
+    decs = [pkg.HipDecoder() for _ in clips]
+    for d in decs:
+        d.set_output_hash(True)
+    for _f, outs in pkg.decode_clips_batched(decs, clips):
+        for i, _slot, planes in outs:
+            assert decs[i].last_hash() == pkg.picture_hash(planes)
+    d = pkg.HipDecoder()
+    d.set_output_hash(True)
+    for f, _s, planes in pkg.decode_clip(d, clips[0]):
+        assert d.last_hash() == pkg.picture_hash(planes), f
