@@ -102,7 +102,7 @@ struct Plan {
   /* the queued blocks in coding order (index qi): block, predecessors
    * (q_dep[q_dep_off[qi] .. q_dep_off[qi + 1])), and the qi of every entry
    * of the static order */
-  std::vector<int32_t> q_block, q_dep_off, q_dep, queue_qi, q_area;
+  std::vector<int32_t> q_block, q_dep_off, q_dep, queue_qi, q_area, q_level;
 };
 
 struct Ctx {
@@ -824,6 +824,7 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
           owner[p][(size_t)cy * L.cells_w[p] + cx] = qi;
     }
     plan->q_block = H->queue_list; /* coding order */
+    plan->q_level = level;
     std::vector<int32_t> ord((size_t)nq);
     for (int k = 0; k < nq; ++k) ord[(size_t)k] = k;
     std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t b) { return level[(size_t)a] < level[(size_t)b]; });
@@ -2093,10 +2094,11 @@ int av1h_plan_sb_times(av1h_ctx* h, const void* plan, float* us) {
 
 namespace {
 /* The cost schedule: every queued block costs its superblock's cost shared
- * by sample count; a block's priority is its longest remaining path through
- * the block graph (its cost plus the longest path of what waits for it), the
- * longest first, coding order between equals. A block's predecessors have
- * strictly longer paths, so the order stays topological. */
+ * by sample count, and its longest remaining path through the block graph
+ * (its cost plus the longest path of what waits for it) ranks it among the
+ * blocks of its dependency level: levels in order as in the static order
+ * (so blocks are still handed out about when they become ready), the most
+ * critical first inside a level, coding order between equals. */
 bool schedule_costs(Plan* p, const float* cost) {
   const int nq = (int)p->q_block.size();
   const int nsb = p->sb_w * p->sb_h;
@@ -2121,9 +2123,10 @@ bool schedule_costs(Plan* p, const float* cost) {
   }
   std::vector<int32_t> ord((size_t)nq);
   for (int k = 0; k < nq; ++k) ord[(size_t)k] = k;
-  std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t b) { return path[(size_t)a] > path[(size_t)b]; });
-  constexpr double kBucketUs = 0.25;
-  const int top = nq ? (int)(path[(size_t)ord[0]] / kBucketUs) : 0;
+  std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t b) {
+    const int32_t la = p->q_level[(size_t)a], lb = p->q_level[(size_t)b];
+    return la != lb ? la < lb : path[(size_t)a] > path[(size_t)b];
+  });
   p->queue_host.resize((size_t)nq);
   p->queue_sb.resize((size_t)nq);
   p->key_off.clear();
@@ -2131,7 +2134,7 @@ bool schedule_costs(Plan* p, const float* cost) {
     const int32_t qi = ord[(size_t)k];
     p->queue_host[(size_t)k] = p->q_block[(size_t)qi];
     p->queue_sb[(size_t)k] = sb_of[(size_t)qi];
-    const int key = top - (int)(path[(size_t)qi] / kBucketUs);
+    const int key = p->q_level[(size_t)qi];
     while ((int)p->key_off.size() <= key) p->key_off.push_back((int32_t)k);
   }
   p->key_off.push_back((int32_t)nq);

@@ -619,6 +619,78 @@ __device__ void edge_filter(int* buf, int* tmp, int sz, int strength) {
   wsync();
 }
 
+/* The edge filter of both edges at once: the above edge on entries
+ * [0, sza), the left edge on [0, szl), one copy phase and one filter phase
+ * for the two (av1gpu_kernels.h:1157-1173 on each). */
+__device__ void edge_filter2(int* above, int* left, int* tmp, int sza, int stra, int szl, int strl) {
+  const int lane = threadIdx.x;
+  if (stra <= 0) sza = 0;
+  if (strl <= 0) szl = 0;
+  if (sza + szl == 0) return;
+  int* ta = tmp;
+  int* tl = tmp + sza;
+  for (int i = lane; i < sza + szl; i += 64) {
+    if (i < sza)
+      ta[i] = above[i - 1];
+    else
+      tl[i - sza] = left[i - sza - 1];
+  }
+  wsync();
+  const int* ka = SPEC(AV1T_OFF_EDGE_KERNEL) + 5 * (stra > 0 ? stra - 1 : 0);
+  const int* kl = SPEC(AV1T_OFF_EDGE_KERNEL) + 5 * (strl > 0 ? strl - 1 : 0);
+  for (int j = lane; j < sza + szl; j += 64) {
+    const bool is_a = j < sza;
+    const int i = is_a ? j : j - sza, sz = is_a ? sza : szl;
+    if (i == 0) continue;
+    const int* t = is_a ? ta : tl;
+    int s = 0;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) s += (is_a ? ka[q] : kl[q]) * t[clampi(i - 2 + q, 0, sz - 1)];
+    (is_a ? above : left)[i - 1] = (s + 8) >> 4;
+  }
+  wsync();
+}
+
+/* 2x upsampling of both edges at once (av1gpu_kernels.h:1177-1191 on each;
+ * n 0 skips an edge) */
+__device__ void edge_upsample2(int* above, int* left, int* tmp, int na, int nl, int maxv) {
+  const int lane = threadIdx.x;
+  if (na + nl == 0) return;
+  int* ta = tmp;           /* na + 3 entries */
+  int* tl = tmp + na + 3;  /* nl + 3 entries */
+  for (int j = lane; j < na + nl + 2; j += 64) {
+    if (j < na + 1)
+      ta[j + 1] = above[j - 1];
+    else
+      tl[j - na - 1 + 1] = left[j - na - 2];
+  }
+  if (lane == 0) {
+    if (na) {
+      ta[0] = above[-1];
+      ta[na + 2] = above[na - 1];
+    }
+    if (nl) {
+      tl[0] = left[-1];
+      tl[nl + 2] = left[nl - 1];
+    }
+  }
+  wsync();
+  if (lane == 0) {
+    if (na) above[-2] = ta[0];
+    if (nl) left[-2] = tl[0];
+  }
+  for (int j = lane; j < na + nl; j += 64) {
+    const bool is_a = j < na;
+    const int i = is_a ? j : j - na;
+    const int* t = is_a ? ta : tl;
+    int* buf = is_a ? above : left;
+    const int s = -t[i] + 9 * t[i + 1] + 9 * t[i + 2] - t[i + 3];
+    buf[2 * i - 1] = clampi(rnd2(s, 4), 0, maxv);
+    buf[2 * i] = t[i + 2];
+  }
+  wsync();
+}
+
 /* 2x upsampling (av1gpu_kernels.h:1177-1191) */
 __device__ void edge_upsample(int* buf, int* tmp, int n, int maxv) {
   const int lane = threadIdx.x;
@@ -768,16 +840,13 @@ __device__ void predict_intra(RArgs a, BlkRef b, int* work, uint16_t* pred) {
         }
         wsync();
       }
-      if (have_above)
-        edge_filter(above, tmp, min(w, max_x - b.x + 1) + (pa < 90 ? h : 0) + 1,
-                    edge_strength(w, h, b.filter_type, pa - 90));
-      if (have_left)
-        edge_filter(left, tmp, min(h, max_y - b.y + 1) + (pa > 180 ? w : 0) + 1,
-                    edge_strength(w, h, b.filter_type, pa - 180));
+      edge_filter2(above, left, tmp, min(w, max_x - b.x + 1) + (pa < 90 ? h : 0) + 1,
+                   have_above ? edge_strength(w, h, b.filter_type, pa - 90) : 0,
+                   min(h, max_y - b.y + 1) + (pa > 180 ? w : 0) + 1,
+                   have_left ? edge_strength(w, h, b.filter_type, pa - 180) : 0);
       ua = upsample_sel(w, h, b.filter_type, pa - 90);
-      if (ua) edge_upsample(above, tmp, w + (pa < 90 ? h : 0), maxv);
       ul = upsample_sel(w, h, b.filter_type, pa - 180);
-      if (ul) edge_upsample(left, tmp, h + (pa > 180 ? w : 0), maxv);
+      edge_upsample2(above, left, tmp, ua ? w + (pa < 90 ? h : 0) : 0, ul ? h + (pa > 180 ? w : 0) : 0, maxv);
     }
     /* av1gpu_kernels.h:1245-1313 */
     const int* dr = SPEC(AV1T_OFF_DR_DERIV);
@@ -961,7 +1030,32 @@ __device__ void wait_cells(RArgs a, BlkRef b, uint32_t* error) {
 /* A block of the queue: the prediction, then the residuals of its transforms, which
  * k_itx computed before the queue started (av1gpu_kernels.h:933-937 for the
  * add), then one store. */
-__device__ void reconstruct_queued(RArgs a, BlkRef b, int* work, uint16_t* pred) {
+/* Residual samples of a block read before its wait (they do not depend on
+ * the neighbours): one transform covering the block, at most 256 samples,
+ * sample lane + 64 k in rv[k]. */
+struct EarlyRes {
+  bool on;
+  int16_t v[4];
+};
+
+__device__ __forceinline__ EarlyRes early_residual(RArgs a, BlkRef b) {
+  EarlyRes e;
+  e.on = false;
+  if (b.tx_count != 1 || b.w * b.h > 256) return e;
+  const AV1H_CMEM DevTx& tx = cmem(a.plan.tx)[b.tx_first];
+  if (tx.x != b.x || tx.y != b.y || tx.w != b.w || tx.h != b.h) return e;
+  e.on = true;
+  const int p = b.plane, stride = a.lay.stride[p], lw = ilog2(b.w);
+  const int16_t* r = a.res + a.lay.plane_off[p] + (int64_t)b.y * stride + b.x;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int s = threadIdx.x + 64 * k;
+    e.v[k] = s < b.w * b.h ? r[(int64_t)(s >> lw) * stride + (s & (b.w - 1))] : 0;
+  }
+  return e;
+}
+
+__device__ void reconstruct_queued(RArgs a, BlkRef b, int* work, uint16_t* pred, const EarlyRes& er) {
   if (b.kind == kIntra) {
     predict_intra(a, b, work, pred);
   } else {
@@ -971,6 +1065,16 @@ __device__ void reconstruct_queued(RArgs a, BlkRef b, int* work, uint16_t* pred)
     wsync();
   }
   const int p = b.plane, stride = a.lay.stride[p];
+  if (er.on) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int s = threadIdx.x + 64 * k;
+      if (s < b.w * b.h) pred[s] = (uint16_t)clampi((int)pred[s] + er.v[k], 0, a.maxv);
+    }
+    wsync();
+    store_block(a, b, pred);
+    return;
+  }
   for (int t = 0; t < b.tx_count; ++t) {
     const DevTx& tx = a.plan.tx[b.tx_first + t];
     const int ox = tx.x - b.x, oy = tx.y - b.y, ltw = ilog2(tx.w);
@@ -1192,9 +1296,10 @@ __global__ __launch_bounds__(64) void k_queue(BatchDev B) {
     }
     RArgs a = recon_args(B, __builtin_amdgcn_readfirstlane(job));
     BlkRef b = cmem(a.plan.blocks)[__builtin_amdgcn_readfirstlane(blk)];
+    const EarlyRes er = early_residual(a, b);
     wait_cells(a, b, B.error);
     const uint64_t t1 = B.trace ? (uint64_t)wall_clock64() : 0;
-    reconstruct_queued(a, b, s_work, s_pred);
+    reconstruct_queued(a, b, s_work, s_pred, er);
     /* every sample store of the block has completed before its cells flip */
     __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");

@@ -243,17 +243,78 @@ def run_mix(args):
                      "ms_per_step": round(dt * 1e3 / args.steps, 3),
                      "call_latency_ms": {"p50": round(pct(lat, 50) * 1e3, 3), "p99": round(pct(lat, 99) * 1e3, 3)}}
         log(f"mix {name}: {out[name]['value']} Mpixel/s")
+    cost = None if args.no_costmodel else run_costmodel(args, decs, plans, sched)
     for i in range(ns):
         for p in plans[i]:
             decs[i].discard_plan(p)
         decs[i].close()
-    return {
+    rep = {
         "workload": "16 streams: 4x 426x240, 4x 854x480, 4x 1280x720, 3x 1920x1080, 1x 3840x2160, 8 and 10 bit "
                     "alternating (2160p 10 bit), 4:2:0, 32 frames each, 1280x720-workload syntax statistics",
         "plain": out["plain"], "batched": out["batched"],
         "speedup": round(out["batched"]["value"] / out["plain"]["value"], 3),
         "bit_exact": {"frames_checked": len(hp), "mismatched_frames": mism,
                       "method": "FNV-1a-64 of every sample, batched decode vs plain decode of the same plans"},
+    }
+    if cost is not None:
+        rep["cost_model"] = cost
+    return rep
+
+
+def run_costmodel(args, decs, plans, sched):
+    """The learned schedule on the 16-stream mix: superblock timings recorded
+    by the decoder on other seeds of the same stream shapes (key frames every
+    other frame, so that a few clips give the samples), the perceptron
+    trained on the GPU, then the mix decoded in batches under the static
+    dependency-level order and under the model's order: queue device time,
+    wall time and the checksums of every picture compared."""
+    import numpy as np
+    import av1ongpu_amd as pkg
+    from av1ongpu_amd import costmodel as cmod
+
+    t = time.perf_counter()
+    Xs, ys = [], []
+    total, rep = 0, 0
+    while total < args.cm_samples and rep < 64:
+        clips = [pkg.Clip("c1", 50000 + 100 * rep + i, width=w, height=h, bit_depth=bd, frames=6, key_interval=2)
+                 for i, (w, h, bd) in enumerate(MIX_STREAMS)]
+        X, y = cmod.collect_samples(clips)
+        Xs.append(X)
+        ys.append(y)
+        total += len(y)
+        rep += 1
+    X, y = np.concatenate(Xs), np.concatenate(ys)
+    log(f"cost model: {len(y)} superblock timings recorded in {time.perf_counter() - t:.1f} s")
+    model, losses, r2, train_s = cmod.train(X, y, epochs=args.cm_epochs, lr=1e-3, seed=args.seed)
+    log(f"cost model: trained {args.cm_epochs} epochs in {train_s:.1f} s, hold-out R^2 {r2:.3f}")
+    ns = len(decs)
+    for d in decs:
+        d.set_output_hash(True)
+    res = {}
+    hashes = {}
+    for name, m in (("static", None), ("model", model)):
+        cmod.schedule_all(decs, plans, m)
+        cmod.run_rounds(decs, plans, sched)                     # warm
+        h = []
+        wall = min(cmod.run_rounds(decs, plans, sched, h)[0] for _ in range(max(1, args.steps)))
+        _, q = cmod.run_rounds(decs, plans, sched, timing=True)
+        hashes[name] = h
+        res[name] = {"wall_ms": round(wall * 1e3, 2), "queue_device_ms": round(q, 2)}
+        log(f"cost model: {name} schedule: {wall * 1e3:.1f} ms wall, queue {q:.1f} ms")
+    cmod.schedule_all(decs, plans, None)
+    for d in decs:
+        d.set_output_hash(False)
+    mism = sum(1 for a, b in zip(hashes["static"], hashes["model"]) for u, v in zip(a, b) if u != v)
+    model.close()
+    return {
+        "samples": int(len(y)), "epochs": args.cm_epochs, "train_seconds": round(train_s, 2),
+        "loss_first_last": [round(float(losses[0]), 4), round(float(losses[-1]), 4)],
+        "holdout_r2": round(r2, 3),
+        "static": res["static"], "model": res["model"],
+        "queue_speedup": round(res["static"]["queue_device_ms"] / max(res["model"]["queue_device_ms"], 1e-9), 3),
+        "wall_speedup": round(res["static"]["wall_ms"] / max(res["model"]["wall_ms"], 1e-9), 3),
+        "bit_exact": {"frames_checked": sum(len(a) for a in hashes["static"]), "mismatched_frames": mism,
+                      "method": "device checksum of every picture, model schedule vs static schedule"},
     }
 
 
@@ -407,6 +468,9 @@ def main():
     ap.add_argument("--no-mix", action="store_true", help="skip the 16-stream batched vs plain mix")
     ap.add_argument("--mix-only", action="store_true", help="only the 16-stream mix")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
+    ap.add_argument("--no-costmodel", action="store_true", help="skip the learned schedule on the mix")
+    ap.add_argument("--cm-samples", type=int, default=200000)
+    ap.add_argument("--cm-epochs", type=int, default=20)
     ap.add_argument("--serve-only", action="store_true", help="only the serving workload")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-warmup", type=float, default=1.0)
