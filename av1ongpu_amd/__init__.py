@@ -232,6 +232,12 @@ def hip_lib():
         lib.av1h_set_decode_nv12.argtypes = [P, ctypes.c_size_t]
         lib.av1h_set_graphs.argtypes = [P, ctypes.c_int]
         lib.av1h_set_output_hash.argtypes = [P, ctypes.c_int]
+        lib.av1h_tune_params.restype = ctypes.c_int
+        lib.av1h_tune_params.argtypes = []
+        lib.av1h_tune_name.restype = ctypes.c_char_p
+        lib.av1h_tune_name.argtypes = [ctypes.c_int]
+        lib.av1h_tune_get.argtypes = [P, ctypes.POINTER(ctypes.c_int32), ctypes.c_int]
+        lib.av1h_tune_set.argtypes = [P, ctypes.POINTER(ctypes.c_int32), ctypes.c_int]
         lib.av1h_last_hash.argtypes = [P, ctypes.POINTER(ctypes.c_uint64)]
         lib.av1h_nv12_device_buffer.restype = ctypes.c_void_p
         lib.av1h_nv12_device_buffer.argtypes = [P]
@@ -511,6 +517,21 @@ class HipDecoder:
     def set_pool_slots(self, slots):
         if self._lib.av1h_set_pool_slots(self._h, slots) != AV1H_OK:
             raise HipError(self.error())
+
+    def tuning(self):
+        """{parameter name: value} of the launch shapes of batches this context leads."""
+        n = self._lib.av1h_tune_params()
+        buf = (ctypes.c_int32 * n)()
+        self._lib.av1h_tune_get(self._h, buf, n)
+        return {self._lib.av1h_tune_name(i).decode(): buf[i] for i in range(n)}
+
+    def set_tuning(self, values):
+        cur = self.tuning()
+        cur.update(values)
+        names = [self._lib.av1h_tune_name(i).decode() for i in range(len(cur))]
+        buf = (ctypes.c_int32 * len(names))(*[cur[k] for k in names])
+        if self._lib.av1h_tune_set(self._h, buf, len(names)) != AV1H_OK:
+            raise HipError(f"invalid launch shapes {values}")
 
     def set_output_hash(self, enable=True):
         """Checksum every decoded picture on the device (picture_hash for the host version)."""

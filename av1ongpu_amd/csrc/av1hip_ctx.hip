@@ -166,6 +166,7 @@ struct Ctx {
   /* captured launch sequences of batches this context leads, by frame count
    * (valid while batch_dev stays where it is) */
   bool graphs = true;
+  Tuning tune;                   /* launch shapes of the batches it leads */
   std::vector<std::pair<int, hipGraphExec_t>> graph_exec;
   bool stage_hit[AV1H_STAGES] = {};
   double stage_ms[AV1H_STAGES] = {};
@@ -1378,8 +1379,20 @@ struct JobSpec {
 
 int copy_groups(const Layout& L) { return (int)((L.size * 2 + kCopyChunk - 1) / kCopyChunk); }
 
+void default_tuning_impl(Tuning* t) {
+  for (int k = 0; k < kTuneParams; ++k) t->v[k] = 16;
+  t->v[kKMc] = 8;
+  t->v[kKUpscale] = 8;
+  t->v[kKLr] = 8;
+  t->v[kKHash] = 8;
+  t->v[kKGrainTemplate] = 1;
+  for (int k : {kKCopyLrCurr, kKCopyCdefSrc, kKCopyUpSrc, kKCopyLrCdef, kKCopyStore}) t->v[k] = 8;
+  t->v[kTuneQueue] = 6;
+  t->v[kTuneCdefBatch] = 1;
+}
+
 /* virtual workgroups of kernel k for one frame */
-int kernel_groups(const FrameJob& fj, int k) {
+int kernel_groups(const FrameJob& fj, int k, int cdef_batch) {
   const FramePlanDev& p = fj.r.plan;
   const FilterArgs& f = fj.f;
   const Layout& L = fj.r.lay;
@@ -1391,10 +1404,10 @@ int kernel_groups(const FrameJob& fj, int k) {
     case kKDeblockV: return p.n_deblock_v > 0 ? (p.lines_v + 255) / 256 : 0;
     case kKDeblockH: return p.n_deblock_h > 0 ? (p.lines_h + 255) / 256 : 0;
     case kKCdefDir: return p.n_cdef > 0 ? (p.n_cdef_dir + 3) / 4 : 0;
-    case kKCdefLuma: return p.n_cdef > 0 ? (p.n_cdef_luma + 4 * kCdefBatch - 1) / (4 * kCdefBatch) : 0;
+    case kKCdefLuma: return p.n_cdef > 0 ? (p.n_cdef_luma + 4 * cdef_batch - 1) / (4 * cdef_batch) : 0;
     case kKCdefChroma: {
       const int nc = p.n_cdef - p.n_cdef_luma;
-      const int per = kCdefBatch * 256 / ((8 >> L.ss_x) * (8 >> L.ss_y));
+      const int per = cdef_batch * 256 / ((8 >> L.ss_x) * (8 >> L.ss_y));
       return nc > 0 ? (nc + per - 1) / per : 0;
     }
     case kKUpscale: return f.upscaled ? upscale_tiles(L, f.uses_lr ? 2 : 1) : 0;
@@ -1538,7 +1551,8 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
   std::vector<int32_t> prefix((size_t)kKernels * (n + 1) + 1, 0);
   for (int k = 0; k < kKernels; ++k)
     for (int i = 0; i < n; ++i)
-      prefix[(size_t)k * (n + 1) + i + 1] = prefix[(size_t)k * (n + 1) + i] + kernel_groups(jobs[i], k);
+      prefix[(size_t)k * (n + 1) + i + 1] =
+          prefix[(size_t)k * (n + 1) + i] + kernel_groups(jobs[i], k, lead->tune.v[kTuneCdefBatch]);
   prefix[(size_t)kKernels * (n + 1)] = (int32_t)nq; /* the queue's entry count */
   auto total_of = [&](int k) { return prefix[(size_t)k * (n + 1) + n]; };
   /* upload: [jobs][prefix][merged queue] */
@@ -1604,14 +1618,15 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
     const int total = use_graph ? kAnyTotal : total_of(k);
     if (total <= 0) return hipSuccess;
     ++launches;
+    const int cap = lead->cus * lead->tune.v[k];
     switch (k) {
       case kKMc:
       case kKSpecial:
       case kKItxInter:
-      case kKItxIntra: return launch_recon_kernel(B, k, total, lead->cus, st);
+      case kKItxIntra: return launch_recon_kernel(B, k, total, cap, st);
       case kKGrainTemplate:
-      case kKGrainApply: return launch_grain_kernel(B, k, total, lead->cus, st);
-      default: return launch_filter_kernel(B, k, total, lead->cus, st);
+      case kKGrainApply: return launch_grain_kernel(B, k, total, cap, st);
+      default: return launch_filter_kernel(B, k, total, cap, lead->tune.v[kTuneCdefBatch], st);
     }
   };
   auto stage = [&](int st, std::initializer_list<int> ks) -> hipError_t {
@@ -1658,7 +1673,7 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
     if (side_itx) SEQ(hipStreamWaitEvent(s, lead->ev_itx_done, 0));
     if (use_graph || nq > 0) {
       if (lead->timing) (void)hipEventRecord(lead->sev[2 * AV1H_STAGE_QUEUE], s);
-      SEQ(launch_queue(B, use_graph ? -1 : (int)nq, lead->cus, s, &launches));
+      SEQ(launch_queue(B, use_graph ? -1 : (int)nq, lead->cus * lead->tune.v[kTuneQueue], s, &launches));
       if (lead->timing) {
         (void)hipEventRecord(lead->sev[2 * AV1H_STAGE_QUEUE + 1], s);
         lead->stage_hit[AV1H_STAGE_QUEUE] = true;
@@ -1852,6 +1867,7 @@ int av1h_create(int device_index, av1h_ctx** out) {
   c->device = device_index;
   c->prop = prop;
   c->cus = prop.multiProcessorCount;
+  default_tuning_impl(&c->tune);
   std::memset(&c->stats, 0, sizeof(c->stats));
   if (hipSetDevice(device_index) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc(&c->counters, 64) != hipSuccess || hipMemset(c->counters, 0, 64) != hipSuccess ||
@@ -2210,6 +2226,35 @@ int av1h_last_hash(av1h_ctx* h, uint64_t* out) {
   std::lock_guard<std::mutex> lk(h->c.mu);
   if (!h->c.hash_valid) return set_err(&h->c, AV1H_ERR_INVALID, "the last decode computed no checksum");
   *out = h->c.last_hash;
+  return AV1H_OK;
+}
+
+int av1h_tune_params(void) { return kTuneParams; }
+
+const char* av1h_tune_name(int i) {
+  static const char* names[kTuneParams] = {
+      "mc", "inter_special", "itx_inter", "itx_intra", "deblock_v", "deblock_h", "cdef_dir", "cdef_luma",
+      "cdef_chroma", "upscale", "lr", "grain_template", "grain_apply", "nv12", "hash", "copy_lr_curr",
+      "copy_cdef_src", "copy_upscale_src", "copy_lr_cdef", "copy_store", "queue", "cdef_blocks_per_thread"};
+  return i >= 0 && i < kTuneParams ? names[i] : nullptr;
+}
+
+int av1h_tune_get(const av1h_ctx* h, int32_t* params, int n) {
+  if (!h || !params || n < kTuneParams) return AV1H_ERR_INVALID;
+  for (int i = 0; i < kTuneParams; ++i) params[i] = h->c.tune.v[i];
+  return kTuneParams;
+}
+
+int av1h_tune_set(av1h_ctx* h, const int32_t* params, int n) {
+  if (!h || !params || n != kTuneParams) return AV1H_ERR_INVALID;
+  for (int i = 0; i < kTuneParams; ++i) {
+    const int lim = i == kTuneCdefBatch ? kCdefMaxBatch : 64;
+    if (params[i] < 1 || params[i] > lim) return AV1H_ERR_INVALID;
+  }
+  std::lock_guard<std::mutex> lk(h->c.mu);
+  (void)hipSetDevice(h->c.device);
+  for (int i = 0; i < kTuneParams; ++i) h->c.tune.v[i] = params[i];
+  drop_graphs(&h->c);
   return AV1H_OK;
 }
 

@@ -334,25 +334,26 @@ __device__ __forceinline__ bool cdef_sample(FArgs a, int first, int bi, int lane
  * independent of each other) and then stores them. */
 
 template <int S>
-__device__ __forceinline__ void cdef_group(FArgs a, int first, int count, int grp, const int (*dirs)[2][2]) {
+__device__ __forceinline__ void cdef_group(FArgs a, int first, int count, int grp, const int (*dirs)[2][2],
+                                           int batch) {
   constexpr int kPer = 256 / S;
   const int lane = threadIdx.x % S;
-  int64_t off[kCdefBatch];
-  int val[kCdefBatch];
-  bool ok[kCdefBatch];
+  int64_t off[kCdefMaxBatch];
+  int val[kCdefMaxBatch];
+  bool ok[kCdefMaxBatch];
 #pragma unroll
-  for (int u = 0; u < kCdefBatch; ++u) {
-    const int bi = (grp * kCdefBatch + u) * kPer + threadIdx.x / S;
-    ok[u] = bi < count && cdef_sample<S>(a, first, bi, lane, &off[u], &val[u], dirs);
+  for (int u = 0; u < kCdefMaxBatch; ++u) {
+    const int bi = (grp * batch + u) * kPer + threadIdx.x / S;
+    ok[u] = u < batch && bi < count && cdef_sample<S>(a, first, bi, lane, &off[u], &val[u], dirs);
   }
 #pragma unroll
-  for (int u = 0; u < kCdefBatch; ++u)
+  for (int u = 0; u < kCdefMaxBatch; ++u)
     if (ok[u]) a.frame[off[u]] = (uint16_t)val[u];
 }
 
 /* luma blocks (plane 0) or chroma blocks (plane 1) of every frame; a chroma
  * block has 16, 32 or 64 samples by the frame's subsampling */
-__global__ __launch_bounds__(256) void k_cdef(BatchDev B, int chroma) {
+__global__ __launch_bounds__(256) void k_cdef(BatchDev B, int chroma, int batch) {
   /* the direction table in LDS: lanes index it by their own direction */
   __shared__ int s_dirs[8][2][2];
   if (threadIdx.x < 32) (&s_dirs[0][0][0])[threadIdx.x] = (&kCdefDirs[0][0][0])[threadIdx.x];
@@ -364,14 +365,14 @@ __global__ __launch_bounds__(256) void k_cdef(BatchDev B, int chroma) {
     FArgs a = filter_args(B, j);
     const int nl = a.plan.n_cdef_luma, grp = v - pre[j];
     if (!chroma) {
-      cdef_group<64>(a, 0, nl, grp, s_dirs);
+      cdef_group<64>(a, 0, nl, grp, s_dirs, batch);
       continue;
     }
     const int nc = a.plan.n_cdef - nl;
     switch ((8 >> a.lay.ss_x) * (8 >> a.lay.ss_y)) {
-      case 16: cdef_group<16>(a, nl, nc, grp, s_dirs); break;
-      case 32: cdef_group<32>(a, nl, nc, grp, s_dirs); break;
-      default: cdef_group<64>(a, nl, nc, grp, s_dirs); break;
+      case 16: cdef_group<16>(a, nl, nc, grp, s_dirs, batch); break;
+      case 32: cdef_group<32>(a, nl, nc, grp, s_dirs, batch); break;
+      default: cdef_group<64>(a, nl, nc, grp, s_dirs, batch); break;
     }
   }
 }
@@ -788,27 +789,23 @@ int upscale_tiles(const Layout& L, int pictures) {
   return (L.out_w[0] + kUpW - 1) / kUpW * ((L.out_h[0] + kUpRows - 1) / kUpRows) * 3 * pictures;
 }
 
-hipError_t launch_filter_kernel(const BatchDev& b, int k, int total, int cus, hipStream_t s) {
-  const dim3 blk(256);
+hipError_t launch_filter_kernel(const BatchDev& b, int k, int total, int cap, int cdef_batch, hipStream_t s) {
+  const dim3 blk(256), grid(min(total, cap));
   switch (k) {
     case kKDeblockV:
-    case kKDeblockH:
-      hipLaunchKernelGGL(k_deblock, dim3(min(total, cus * 16)), blk, 0, s, b, k == kKDeblockH ? 1 : 0);
-      break;
-    case kKCdefDir: hipLaunchKernelGGL(k_cdef_dir, dim3(min(total, cus * 16)), blk, 0, s, b); break;
+    case kKDeblockH: hipLaunchKernelGGL(k_deblock, grid, blk, 0, s, b, k == kKDeblockH ? 1 : 0); break;
+    case kKCdefDir: hipLaunchKernelGGL(k_cdef_dir, grid, blk, 0, s, b); break;
     case kKCdefLuma:
-    case kKCdefChroma:
-      hipLaunchKernelGGL(k_cdef, dim3(min(total, cus * 16)), blk, 0, s, b, k == kKCdefChroma ? 1 : 0);
-      break;
-    case kKUpscale: hipLaunchKernelGGL(k_upscale, dim3(min(total, cus * 8)), blk, 0, s, b); break;
-    case kKLr: hipLaunchKernelGGL(k_lr, dim3(min(total, cus * 8)), blk, 0, s, b); break;
-    case kKNv12: hipLaunchKernelGGL(k_nv12_batch, dim3(min(total, cus * 16)), blk, 0, s, b); break;
-    case kKHash: hipLaunchKernelGGL(k_hash, dim3(min(total, cus * 8)), blk, 0, s, b); break;
+    case kKCdefChroma: hipLaunchKernelGGL(k_cdef, grid, blk, 0, s, b, k == kKCdefChroma ? 1 : 0, cdef_batch); break;
+    case kKUpscale: hipLaunchKernelGGL(k_upscale, grid, blk, 0, s, b); break;
+    case kKLr: hipLaunchKernelGGL(k_lr, grid, blk, 0, s, b); break;
+    case kKNv12: hipLaunchKernelGGL(k_nv12_batch, grid, blk, 0, s, b); break;
+    case kKHash: hipLaunchKernelGGL(k_hash, grid, blk, 0, s, b); break;
     case kKCopyLrCurr:
     case kKCopyCdefSrc:
     case kKCopyUpSrc:
     case kKCopyLrCdef:
-    case kKCopyStore: hipLaunchKernelGGL(k_copy, dim3(min(total, cus * 8)), blk, 0, s, b, k); break;
+    case kKCopyStore: hipLaunchKernelGGL(k_copy, grid, blk, 0, s, b, k); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

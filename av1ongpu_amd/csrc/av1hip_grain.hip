@@ -316,10 +316,11 @@ void grain_jump_matrices(uint16_t m82[16], uint16_t m44[16]) {
 
 int grain_tiles(const Layout& lay) { return (lay.out_w[0] + 63) / 64 * ((lay.out_h[0] + 31) / 32); }
 
-hipError_t launch_grain_kernel(const BatchDev& b, int k, int total, int cus, hipStream_t s) {
+hipError_t launch_grain_kernel(const BatchDev& b, int k, int total, int cap, hipStream_t s) {
+  const dim3 grid(min(total, cap));
   switch (k) {
-    case kKGrainTemplate: hipLaunchKernelGGL(k_grain_template, dim3(min(total, cus)), dim3(kTmplThreads), 0, s, b); break;
-    case kKGrainApply: hipLaunchKernelGGL(k_grain_apply, dim3(min(total, cus * 16)), dim3(256), 0, s, b); break;
+    case kKGrainTemplate: hipLaunchKernelGGL(k_grain_template, grid, dim3(kTmplThreads), 0, s, b); break;
+    case kKGrainApply: hipLaunchKernelGGL(k_grain_apply, grid, dim3(256), 0, s, b); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

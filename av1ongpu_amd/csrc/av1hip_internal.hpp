@@ -274,8 +274,9 @@ __device__ __forceinline__ int find_job(const AV1H_CMEM int32_t* pre, int n, int
 constexpr int kQueueJobShift = 24;
 constexpr int kMaxBatch = 128;
 
-/* CDEF: groups of 256 / (block samples) blocks a workgroup takes per step */
-constexpr int kCdefBatch = 1;
+/* CDEF: groups of 256 / (block samples) blocks a workgroup takes per step
+ * (a thread's samples of up to kCdefMaxBatch blocks computed together) */
+constexpr int kCdefMaxBatch = 4;
 
 /* samples one checksum workgroup reads */
 constexpr int kHashChunk = 4096;
@@ -318,16 +319,24 @@ __device__ __forceinline__ const AV1H_CMEM int32_t* kernel_prefix(const BatchDev
 __device__ __forceinline__ RArgs recon_args(const BatchDev& B, int j) { return cmem(B.jobs)[j].r; }
 __device__ __forceinline__ FArgs filter_args(const BatchDev& B, int j) { return cmem(B.jobs)[j].f; }
 
-/* Kernel k over every frame of the batch, `total` virtual workgroups; `cus`
+/* Kernel k over every frame of the batch, `total` virtual workgroups; `cap`
  * caps the grid, whose workgroups then loop. By file: */
-hipError_t launch_recon_kernel(const BatchDev& b, int k, int total, int cus, hipStream_t s);
-hipError_t launch_filter_kernel(const BatchDev& b, int k, int total, int cus, hipStream_t s);
-hipError_t launch_grain_kernel(const BatchDev& b, int k, int total, int cus, hipStream_t s);
+hipError_t launch_recon_kernel(const BatchDev& b, int k, int total, int cap, hipStream_t s);
+hipError_t launch_filter_kernel(const BatchDev& b, int k, int total, int cap, int cdef_batch, hipStream_t s);
+hipError_t launch_grain_kernel(const BatchDev& b, int k, int total, int cap, hipStream_t s);
+
+/* Launch shapes the autotuner chooses: persistent workgroups per compute
+ * unit of every kernel (the queue's at kKernels), and CDEF blocks per
+ * thread. */
+constexpr int kTuneQueue = kKernels, kTuneCdefBatch = kKernels + 1, kTuneParams = kKernels + 2;
+struct Tuning {
+  int32_t v[kTuneParams];
+};
 /* the dependency queue over the intra blocks of the batch; the entry count
  * is read on the device (the word after the prefix table), so a captured
  * launch serves any batch of the same size. `total` < 0: the grid for any
  * count. */
-hipError_t launch_queue(const BatchDev& b, int total, int cus, hipStream_t s, int* launches);
+hipError_t launch_queue(const BatchDev& b, int total, int cap, hipStream_t s, int* launches);
 /* the grids every kernel takes in a captured graph (its cap) */
 constexpr int kAnyTotal = 1 << 30;
 /* superres: step and initial subpixel position of a plane */

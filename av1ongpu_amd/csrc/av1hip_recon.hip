@@ -1332,21 +1332,22 @@ int itx_groups(const FramePlanDev& p, int set) {
   return g;
 }
 
-hipError_t launch_recon_kernel(const BatchDev& b, int k, int total, int cus, hipStream_t s) {
+hipError_t launch_recon_kernel(const BatchDev& b, int k, int total, int cap, hipStream_t s) {
+  const dim3 grid(min(total, cap));
   switch (k) {
-    case kKMc: hipLaunchKernelGGL(k_mc, dim3(min(total, cus * 8)), dim3(256), 0, s, b); break;
-    case kKSpecial: hipLaunchKernelGGL(k_inter_special, dim3(min(total, cus * 16)), dim3(64), 0, s, b); break;
-    case kKItxInter: hipLaunchKernelGGL(k_itx, dim3(min(total, cus * 16)), dim3(64), 0, s, b, 0); break;
-    case kKItxIntra: hipLaunchKernelGGL(k_itx, dim3(min(total, cus * 16)), dim3(64), 0, s, b, 1); break;
+    case kKMc: hipLaunchKernelGGL(k_mc, grid, dim3(256), 0, s, b); break;
+    case kKSpecial: hipLaunchKernelGGL(k_inter_special, grid, dim3(64), 0, s, b); break;
+    case kKItxInter: hipLaunchKernelGGL(k_itx, grid, dim3(64), 0, s, b, 0); break;
+    case kKItxIntra: hipLaunchKernelGGL(k_itx, grid, dim3(64), 0, s, b, 1); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-hipError_t launch_queue(const BatchDev& b, int total, int cus, hipStream_t s, int* launches) {
+hipError_t launch_queue(const BatchDev& b, int total, int cap, hipStream_t s, int* launches) {
   if (total == 0) return hipSuccess;
   hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, b.queue_counter, 1);
-  const int grid = total < 0 ? cus * 6 : min(total, cus * 6);
+  const int grid = total < 0 ? cap : min(total, cap);
   hipLaunchKernelGGL(k_queue, dim3(grid), dim3(64), 0, s, b);
   *launches += 2;
   return hipGetLastError();

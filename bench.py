@@ -229,6 +229,17 @@ def run_mix(args):
     batched([], hb)
     mism = sum(1 for a, b in zip(hp, hb) if a != b)
     log(f"mix: batched vs plain on {len(hp)} frames: {mism} mismatched")
+    # launch shapes of the batched path, tuned on its first rounds
+    tuned = None
+    if not args.no_tune:
+        from av1ongpu_amd import autotune
+        t = time.perf_counter()
+        rounds = [([plans[i][f] for i in range(ns)], [sched[i][0][f] >= 0 for i in range(ns)],
+                   [sched[i][0][f] for i in range(ns)], [sched[i][1][f] for i in range(ns)]) for f in range(8)]
+        shapes, t_base, t_tuned = autotune.tune(decs, rounds, reps=1)
+        tuned = {"shapes": shapes, "rounds": len(rounds), "default_ms": round(t_base * 1e3, 2),
+                 "tuned_ms": round(t_tuned * 1e3, 2), "seconds": round(time.perf_counter() - t, 1)}
+        log(f"mix: autotuned launch shapes in {tuned['seconds']} s: {t_base * 1e3:.1f} -> {t_tuned * 1e3:.1f} ms")
     for _ in range(args.warmup):
         plain([])
         batched([])
@@ -258,6 +269,8 @@ def run_mix(args):
     }
     if cost is not None:
         rep["cost_model"] = cost
+    if tuned is not None:
+        rep["autotune"] = tuned
     return rep
 
 
@@ -469,6 +482,7 @@ def main():
     ap.add_argument("--mix-only", action="store_true", help="only the 16-stream mix")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
     ap.add_argument("--no-costmodel", action="store_true", help="skip the learned schedule on the mix")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes of the mix")
     ap.add_argument("--cm-samples", type=int, default=200000)
     ap.add_argument("--cm-epochs", type=int, default=20)
     ap.add_argument("--serve-only", action="store_true", help="only the serving workload")

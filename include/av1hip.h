@@ -432,6 +432,18 @@ int av1h_set_decode_nv12(av1h_ctx* ctx, size_t pitch);
  * device_recon_ms holds the whole batch and device_filter_ms 0. */
 int av1h_set_graphs(av1h_ctx* ctx, int enable);
 
+/* Launch shapes of the batches this context leads, the parameters an
+ * autotuner searches (av1ongpu_amd/autotune.py does): for every kernel the
+ * persistent workgroups per compute unit of its grid, the queue's, and the
+ * CDEF blocks each thread filters together (1..4). av1h_tune_params gives
+ * their count and av1h_tune_name their names; setting them drops the
+ * captured launch sequences. No counterpart in the reference (fixed launch
+ * shapes, av1_cuda.cu). */
+int av1h_tune_params(void);
+const char* av1h_tune_name(int index);
+int av1h_tune_get(const av1h_ctx* ctx, int32_t* params, int n);
+int av1h_tune_set(av1h_ctx* ctx, const int32_t* params, int n);
+
 /* With `enable`, every decode on this context also computes a 64 bit
  * checksum of its output picture on the device (the display samples of Y,
  * U, V in raster order, the i-th multiplied by SplitMix64's finaliser of

@@ -139,3 +139,23 @@ def test_output_hash_matches_host():
     d.set_output_hash(True)
     for f, _s, planes in pkg.decode_clip(d, clips[0]):
         assert d.last_hash() == pkg.picture_hash(planes), f
+
+
+def test_launch_shapes_leave_pixels_unchanged():
+    """Any launch shape (grid sizes, CDEF blocks per thread) decodes the same
+    pictures; invalid shapes are refused."""
+    clips = [pkg.Clip("c2_mini", 12), pkg.Clip("c1_mini", 13)]
+    want = _run_batched(clips)
+    decs = [pkg.HipDecoder() for _ in clips]
+    for shape in ({"cdef_blocks_per_thread": 4, "queue": 1, "mc": 1, "deblock_v": 1},
+                  {"cdef_blocks_per_thread": 2, "queue": 12, "lr": 32, "itx_inter": 2}):
+        decs[0].set_tuning(shape)
+        got = [[] for _ in clips]
+        for _f, outs in pkg.decode_clips_batched(decs, clips):
+            for i, _s, planes in outs:
+                got[i].append(planes)
+        for c, g, w in zip(clips, got, want):
+            for f, (a, b) in enumerate(zip(g, w)):
+                _check(f"shape {shape} {c.workload}", f, a, b)
+    with pytest.raises(pkg.HipError):
+        decs[0].set_tuning({"cdef_blocks_per_thread": 5})
