@@ -249,8 +249,9 @@ def hip_lib():
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("max_batch", ctypes.c_int32), ("planner_threads", ctypes.c_int32),
-                ("capture", ctypes.c_int32), ("hash_pictures", ctypes.c_int32), ("hbm_budget", ctypes.c_int64),
-                ("nv12_pitch", ctypes.c_size_t)]
+                ("capture", ctypes.c_int32), ("hash_pictures", ctypes.c_int32), ("launchers", ctypes.c_int32),
+                ("heavy_levels", ctypes.c_int32), ("heavy_launchers", ctypes.c_int32),
+                ("hbm_budget", ctypes.c_int64), ("nv12_pitch", ctypes.c_size_t)]
 
 
 class Completion(ctypes.Structure):
@@ -288,6 +289,7 @@ def _bind_server(lib):
     lib.av1h_stream_bytes.restype = ctypes.c_int64
     lib.av1h_stream_bytes.argtypes = [ctypes.c_int] * 6 + [ctypes.c_size_t]
     lib.av1h_set_pool_slots.argtypes = [P, ctypes.c_int]
+    lib.av1h_plan_levels.argtypes = [P]
 
 
 def _bind_costmodel(lib):
@@ -657,11 +659,11 @@ class Server:
     formed batches."""
 
     def __init__(self, device=0, max_batch=16, planner_threads=4, capture=False, hbm_budget=200 << 30,
-                 nv12_pitch=0, hash_pictures=False):
+                 nv12_pitch=0, hash_pictures=False, launchers=2, heavy_levels=0, heavy_launchers=1):
         lib = hip_lib()
         self._lib = lib
         cfg = ServerConfig(device, max_batch, planner_threads, 1 if capture else 0, 1 if hash_pictures else 0,
-                           hbm_budget, nv12_pitch)
+                           launchers, heavy_levels, heavy_launchers, hbm_budget, nv12_pitch)
         h = ctypes.c_void_p()
         rc = lib.av1h_server_create(ctypes.byref(cfg), ctypes.byref(h))
         if rc != AV1H_OK:

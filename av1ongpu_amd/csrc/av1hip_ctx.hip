@@ -2079,6 +2079,12 @@ int av1h_to_nv12(av1h_ctx* h, int slot, size_t pitch, uint8_t* out, size_t out_b
 
 const uint8_t* av1h_nv12_device_buffer(const av1h_ctx* h) { return h ? h->c.pack : nullptr; }
 
+int av1h_plan_levels(const void* plan) {
+  if (!plan) return AV1H_ERR_INVALID;
+  const Plan* p = static_cast<const Plan*>(plan);
+  return p->key_off_static.empty() ? 0 : (int)p->key_off_static.size() - 1;
+}
+
 int av1h_plan_sb_features(const void* plan, float* features) {
   if (!plan) return AV1H_ERR_INVALID;
   const Plan* p = static_cast<const Plan*>(plan);

@@ -46,6 +46,7 @@ struct SFrame {
   int plan_rc = AV1H_OK;
   int reserved = -1;
   std::vector<int> release;
+  bool heavy = false; /* its queue's critical path is long (a key frame) */
 };
 
 enum class StreamState { Waiting, Admitted };
@@ -75,10 +76,13 @@ struct av1h_server {
   std::deque<SFrame*> to_plan;
   std::deque<av1h_completion> done;
   std::vector<std::thread> planners;
-  std::thread sched;
   std::string err;
   av1h_server_stats stats;
-  av1h_ctx* launcher = nullptr; /* the stream and captured launches every batch uses */
+  /* one scheduler thread per launcher context: each forms and runs its own
+   * batches, so a long batch (a large key frame) does not hold back the
+   * frames that arrive behind it */
+  std::vector<av1h_ctx*> launchers;
+  std::vector<std::thread> scheds;
 };
 
 namespace {
@@ -159,6 +163,7 @@ void planner_main(av1h_server* s) {
     f->planned = true;
     f->plan_rc = rc;
     f->plan = rc == AV1H_OK ? plan : nullptr;
+    f->heavy = f->plan && s->cfg.heavy_levels > 0 && av1h_plan_levels(f->plan) > s->cfg.heavy_levels;
     std::vector<uint8_t>().swap(f->wire);
     s->cv_work.notify_one();
   }
@@ -186,7 +191,8 @@ void capture(av1h_ctx* ctx, uint16_t** pic, int64_t* samples) {
   *samples = n;
 }
 
-void sched_main(av1h_server* s) {
+void sched_main(av1h_server* s, av1h_ctx* launcher, int role) {
+  /* role 0: any frame; 1: heavy frames only; 2: light frames only */
   std::unique_lock<std::mutex> lk(s->mu);
   for (;;) {
     if (s->stop) return;
@@ -201,7 +207,7 @@ void sched_main(av1h_server* s) {
       if (!f->planned) continue;
       if (f->plan_rc != AV1H_OK)
         failed.push_back(st);
-      else
+      else if (role == 0 || (role == 1) == f->heavy)
         ready.push_back(st);
     }
     for (SStream* st : failed) {
@@ -220,7 +226,10 @@ void sched_main(av1h_server* s) {
     }
     std::sort(ready.begin(), ready.end(),
               [](const SStream* a, const SStream* b) { return a->q.front()->arrival < b->q.front()->arrival; });
-    if ((int)ready.size() > s->cfg.max_batch) ready.resize(s->cfg.max_batch);
+    /* a heavy frame's queue is bound by its own critical path: it runs
+     * alone, next to the other heavy frames' batches */
+    const int cap = role == 1 ? 1 : s->cfg.max_batch;
+    if ((int)ready.size() > cap) ready.resize(cap);
     const int n = (int)ready.size();
     std::vector<av1h_ctx*> ctxs(n);
     std::vector<void*> plans(n);
@@ -237,7 +246,7 @@ void sched_main(av1h_server* s) {
     }
     lk.unlock();
     const double t0 = clock_ms();
-    const int rc = av1h_decode_planned_batch_on(s->launcher, ctxs.data(), plans.data(), keep.data(), reserved.data(),
+    const int rc = av1h_decode_planned_batch_on(launcher, ctxs.data(), plans.data(), keep.data(), reserved.data(),
                                                 n, slots.data());
     const double t1 = clock_ms();
     std::vector<uint16_t*> pics(n, nullptr);
@@ -247,7 +256,7 @@ void sched_main(av1h_server* s) {
       for (int i = 0; i < n; ++i) av1h_last_hash(ctxs[i], &hashes[i]);
     if (rc == AV1H_OK && s->cfg.capture)
       for (int i = 0; i < n; ++i) capture(ctxs[i], &pics[i], &samples[i]);
-    std::string msg = rc == AV1H_OK ? std::string() : std::string(av1h_last_error(s->launcher));
+    std::string msg = rc == AV1H_OK ? std::string() : std::string(av1h_last_error(launcher));
     for (int i = 0; i < n; ++i) {
       SFrame* f = frames[i];
       for (int r : f->release) av1h_release_reference(ctxs[i], r);
@@ -298,6 +307,7 @@ double av1h_server_now_ms(void) { return clock_ms(); }
 
 int av1h_server_create(const av1h_server_config* cfg, av1h_server** out) {
   if (!cfg || !out || cfg->max_batch < 1 || cfg->max_batch > 128 || cfg->planner_threads < 0 ||
+      cfg->launchers < 0 || cfg->launchers > 8 ||
       (cfg->nv12_pitch & 1))
     return AV1H_ERR_INVALID;
   *out = nullptr;
@@ -306,13 +316,21 @@ int av1h_server_create(const av1h_server_config* cfg, av1h_server** out) {
   if (rc != AV1H_OK) return rc;
   av1h_server* s = new av1h_server;
   s->cfg = *cfg;
-  if (av1h_create(cfg->device, &s->launcher) != AV1H_OK) {
-    delete s;
-    return AV1H_ERR_DEVICE;
+  const int nl = cfg->launchers > 0 ? cfg->launchers : 2;
+  for (int i = 0; i < nl; ++i) {
+    av1h_ctx* l = nullptr;
+    if (av1h_create(cfg->device, &l) != AV1H_OK) {
+      for (av1h_ctx* c : s->launchers) av1h_destroy(c);
+      delete s;
+      return AV1H_ERR_DEVICE;
+    }
+    s->launchers.push_back(l);
   }
   std::memset(&s->stats, 0, sizeof(s->stats));
   for (int i = 0; i < cfg->planner_threads; ++i) s->planners.emplace_back(planner_main, s);
-  s->sched = std::thread(sched_main, s);
+  const bool split = cfg->heavy_levels > 0 && nl > 1;
+  const int nh = split ? std::min(std::max(cfg->heavy_launchers, 1), nl - 1) : 0;
+  for (int i = 0; i < nl; ++i) s->scheds.emplace_back(sched_main, s, s->launchers[(size_t)i], split ? (i < nh ? 1 : 2) : 0);
   *out = s;
   return AV1H_OK;
 }
@@ -325,7 +343,7 @@ void av1h_server_destroy(av1h_server* s) {
   }
   s->cv_plan.notify_all();
   s->cv_work.notify_all();
-  s->sched.join();
+  for (std::thread& t : s->scheds) t.join();
   for (std::thread& t : s->planners) t.join();
   for (auto& kv : s->streams) {
     SStream* st = kv.second;
@@ -337,7 +355,7 @@ void av1h_server_destroy(av1h_server* s) {
     delete st;
   }
   for (av1h_completion& c : s->done) delete[] c.picture;
-  av1h_destroy(s->launcher);
+  for (av1h_ctx* l : s->launchers) av1h_destroy(l);
   delete s;
 }
 
@@ -383,6 +401,7 @@ int av1h_server_submit_plan(av1h_server* s, int stream, void* plan, int reserved
   f->plan = plan;
   f->own_plan = false;
   f->planned = true;
+  f->heavy = s->cfg.heavy_levels > 0 && av1h_plan_levels(plan) > s->cfg.heavy_levels;
   f->reserved = reserved_slot;
   f->release.assign(release, release + nrelease);
   f->arrival = arrival_ms;

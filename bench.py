@@ -378,8 +378,15 @@ def run_serve(args):
         want.append(hs)
     log(f"serve: planned {len(SERVE_CLASSES)} groups of {SERVE_GOP} frames in {time.perf_counter() - t:.1f} s")
 
+    # key frames (their queue has far more dependency levels than an inter
+    # frame's) run on a launcher of their own
+    levels = [max(pkg.hip_lib().av1h_plan_levels(p) for p in ps) for ps in plans]
+    inter_levels = [sorted(pkg.hip_lib().av1h_plan_levels(p) for p in ps)[len(ps) // 2] for ps in plans]
+    heavy = args.serve_heavy if args.serve_heavy >= 0 else 4 * max(inter_levels)
+    log(f"serve: queue levels per group: max {levels}, median {inter_levels}; heavy above {heavy}")
     srv = pkg.Server(0, max_batch=args.serve_batch, planner_threads=0, hbm_budget=200 << 30,
-                     nv12_pitch=SERVE_PITCH, hash_pictures=True)
+                     nv12_pitch=SERVE_PITCH, hash_pictures=True, launchers=args.serve_launchers,
+                     heavy_levels=heavy, heavy_launchers=max(1, args.serve_launchers - 1))
     streams = []  # (stream id, class)
     for ci, (_name, n, w, h, bd, _cw, _g) in enumerate(SERVE_CLASSES):
         for _ in range(n):
@@ -489,6 +496,9 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-warmup", type=float, default=1.0)
     ap.add_argument("--serve-batch", type=int, default=32)
+    ap.add_argument("--serve-launchers", type=int, default=2)
+    ap.add_argument("--serve-heavy", type=int, default=-1,
+                    help="dependency levels above which a frame runs on the key-frame launcher (-1: automatic, 0: off)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this backend decodes each stream on one device and has no "

@@ -557,6 +557,9 @@ enum { AV1H_CM_FEATURES = 24, AV1H_CM_PARAMS = 5825 };
  * inter-intra blocks; edge filter on; queued blocks; bit depth above 8.
  * Returns the superblock count; `features` (count x 24) may be null. */
 int av1h_plan_sb_features(const void* plan, float* features);
+/* Dependency levels of the plan's queue (its critical path in blocks; 0
+ * when nothing is queued): what bounds the frame's queue time. */
+int av1h_plan_levels(const void* plan);
 /* After a decode of `plan` alone with the queue trace on: the queue work
  * (sample time after the wait, per block, summed) of every superblock in
  * microseconds. Returns the superblock count. */
@@ -597,10 +600,12 @@ int av1h_plan_schedule_model(av1h_ctx* ctx, void* plan, av1h_costmodel* model);
  * streams (av1h_stream_bytes) fits `hbm_budget`; the first stream is always
  * admitted. Frames are submitted as wire bytes (planned by `planner_threads`
  * host threads once the stream is admitted) or as plans; frames of a stream
- * decode in submission order, and one scheduler thread forms batches from
- * the oldest planned head frame of every idle stream, up to `max_batch`
- * frames, while the previous batch is still returning. A closed stream gives
- * its memory back after its last frame.
+ * decode in submission order. `launchers` scheduler threads, each with its
+ * own launcher context and stream, form batches from the oldest planned
+ * head frames of the idle streams, up to `max_batch` frames, so batches run
+ * on the device at the same time and a long one (a large key frame) does not
+ * hold back the frames arriving behind it. A closed stream gives its memory
+ * back after its last frame.
  */
 typedef struct av1h_server av1h_server;
 
@@ -610,6 +615,12 @@ typedef struct av1h_server_config {
   int32_t planner_threads;  /* host threads flattening submitted wire frames   */
   int32_t capture;          /* 1: every completion carries a host copy         */
   int32_t hash_pictures;    /* 1: every completion carries a picture checksum  */
+  int32_t launchers;        /* batches in flight at once (0: 2)                */
+  int32_t heavy_levels;     /* > 0 and launchers > 1: frames whose queue has
+                               more dependency levels than this (key frames)
+                               run one per batch on the first heavy_launchers
+                               launchers, the others on the rest            */
+  int32_t heavy_launchers;  /* launchers for heavy frames (0: 1)              */
   int64_t hbm_budget;       /* bytes of admitted stream state                  */
   size_t nv12_pitch;        /* > 0: NV12 packed on the device with each frame   */
 } av1h_server_config;
