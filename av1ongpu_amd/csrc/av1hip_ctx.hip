@@ -1399,8 +1399,10 @@ int kernel_groups(const FrameJob& fj, int k, int cdef_batch) {
   switch (k) {
     case kKMc: return p.mc_tasks;
     case kKSpecial: return p.n_special;
-    case kKItxInter: return itx_groups(p, 0);
-    case kKItxIntra: return itx_groups(p, 1);
+    case kKItxInter: return itx_groups(p, 0, 0, 3);
+    case kKItxIntra: return itx_groups(p, 1, 0, 3);
+    case kKItxInterL: return itx_groups(p, 0, 3, kTxClasses);
+    case kKItxIntraL: return itx_groups(p, 1, 3, kTxClasses);
     case kKDeblockV: return p.n_deblock_v > 0 ? (p.lines_v + 255) / 256 : 0;
     case kKDeblockH: return p.n_deblock_h > 0 ? (p.lines_h + 255) / 256 : 0;
     case kKCdefDir: return p.n_cdef > 0 ? (p.n_cdef_dir + 3) / 4 : 0;
@@ -1623,7 +1625,9 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
       case kKMc:
       case kKSpecial:
       case kKItxInter:
-      case kKItxIntra: return launch_recon_kernel(B, k, total, cap, st);
+      case kKItxIntra:
+      case kKItxInterL:
+      case kKItxIntraL: return launch_recon_kernel(B, k, total, cap, st);
       case kKGrainTemplate:
       case kKGrainApply: return launch_grain_kernel(B, k, total, cap, st);
       default: return launch_filter_kernel(B, k, total, cap, lead->tune.v[kTuneCdefBatch], st);
@@ -1647,7 +1651,7 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
   /* The second stream takes what does not depend on the prediction: the
    * transforms of intra blocks (the queue waits for them) and the grain
    * templates (the grain blend waits for them). */
-  const bool side_itx = use_graph || total_of(kKItxIntra) > 0;
+  const bool side_itx = use_graph || total_of(kKItxIntra) > 0 || total_of(kKItxIntraL) > 0;
   const bool side_grain = use_graph || total_of(kKGrainTemplate) > 0;
   auto sequence = [&]() -> hipError_t {
     hipError_t e = hipSuccess;
@@ -1661,6 +1665,7 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
       SEQ(hipStreamWaitEvent(lead->side, lead->ev_side_go, 0));
     }
     if (side_itx) {
+      SEQ(run(kKItxIntraL, lead->side));
       SEQ(run(kKItxIntra, lead->side));
       SEQ(hipEventRecord(lead->ev_itx_done, lead->side));
     }
@@ -1669,7 +1674,7 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
       SEQ(hipEventRecord(lead->ev_side_done, lead->side));
     }
     if (!use_graph) SEQ(hipEventRecord(lead->ev[0], s));
-    SEQ(stage(AV1H_STAGE_INTER, {kKMc, kKSpecial, kKItxInter}));
+    SEQ(stage(AV1H_STAGE_INTER, {kKMc, kKSpecial, kKItxInterL, kKItxInter}));
     if (side_itx) SEQ(hipStreamWaitEvent(s, lead->ev_itx_done, 0));
     if (use_graph || nq > 0) {
       if (lead->timing) (void)hipEventRecord(lead->sev[2 * AV1H_STAGE_QUEUE], s);
@@ -2239,7 +2244,8 @@ int av1h_tune_params(void) { return kTuneParams; }
 
 const char* av1h_tune_name(int i) {
   static const char* names[kTuneParams] = {
-      "mc", "inter_special", "itx_inter", "itx_intra", "deblock_v", "deblock_h", "cdef_dir", "cdef_luma",
+      "mc", "inter_special", "itx_inter", "itx_intra", "itx_inter_large", "itx_intra_large", "deblock_v",
+      "deblock_h", "cdef_dir", "cdef_luma",
       "cdef_chroma", "upscale", "lr", "grain_template", "grain_apply", "nv12", "hash", "copy_lr_curr",
       "copy_cdef_src", "copy_upscale_src", "copy_lr_cdef", "copy_store", "queue", "cdef_blocks_per_thread"};
   return i >= 0 && i < kTuneParams ? names[i] : nullptr;

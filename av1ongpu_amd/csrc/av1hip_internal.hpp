@@ -240,7 +240,8 @@ struct FrameJob {
  * workgroups walk "virtual workgroups", numbered frame after frame, and find
  * their frame in a prefix table (kernel k, frame j: prefix[k * (jobs + 1) + j]). */
 enum KernelId {
-  kKMc, kKSpecial, kKItxInter, kKItxIntra, kKDeblockV, kKDeblockH, kKCdefDir, kKCdefLuma, kKCdefChroma,
+  kKMc, kKSpecial, kKItxInter, kKItxIntra, kKItxInterL, kKItxIntraL, kKDeblockV, kKDeblockH, kKCdefDir,
+  kKCdefLuma, kKCdefChroma,
   kKUpscale, kKLr, kKGrainTemplate, kKGrainApply, kKNv12, kKHash,
   kKCopyLrCurr, kKCopyCdefSrc, kKCopyUpSrc, kKCopyLrCdef, kKCopyStore,
   kKernels
@@ -351,8 +352,9 @@ hipError_t launch_zero64(uint64_t* p, int n, hipStream_t s);
 hipError_t launch_nv12(const Layout& lay, const uint16_t* src, uint8_t* out, int pitch, hipStream_t s);
 hipError_t launch_rgb(const Layout& lay, const uint16_t* src, uint8_t* out, const int m[8],
                       const int o[3], hipStream_t s);
-/* transform workgroups of one set of a frame (0 inter, 1 intra) */
-int itx_groups(const FramePlanDev& p, int set);
+/* transform workgroups of classes [c0, c1) of one set of a frame (0 inter,
+ * 1 intra) */
+int itx_groups(const FramePlanDev& p, int set, int c0, int c1);
 
 /* film grain: the LFSR jump matrices the template kernel uses */
 void grain_jump_matrices(uint16_t m82[16], uint16_t m44[16]);

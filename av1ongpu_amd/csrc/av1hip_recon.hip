@@ -202,10 +202,13 @@ __device__ __forceinline__ void itx_group(RArgs a, const int32_t* list, int n, i
   if (coded) col_pass_n<G>(lh, tmp, l, w, tx.tx_type, lossless, a.lay.bit_depth, out);
 }
 
-/* One launch for every class of a set: workgroups of the 64 class first (the
- * longest), then 32, 16, 8, 4. */
+/* The classes [C0, C1) of a set in one launch, the larger first: two
+ * launches (4-16 and 32-64) so that the small transforms, most of them, run
+ * with the registers and LDS they need rather than the 64-point ones'. */
+template <int C0, int C1>
 __device__ __forceinline__ void itx_body(RArgs a, const int32_t* offs, int b, int* s_tmp) {
-  for (int c = kTxClasses - 1; c >= 0; --c) {
+#pragma unroll
+  for (int c = C1 - 1; c >= C0; --c) {
     const int o0 = offs[c], n = offs[c + 1] - o0;
     const int per = 16 >> c;
     const int wgs = (n + per - 1) / per;
@@ -224,14 +227,17 @@ __device__ __forceinline__ void itx_body(RArgs a, const int32_t* offs, int b, in
   }
 }
 
+template <int C0, int C1>
 __global__ __launch_bounds__(64) void k_itx(BatchDev B, int set) {
-  __shared__ int s_tmp[64 * 65];
-  const AV1H_CMEM int32_t* pre = kernel_prefix(B, (set ? kKItxIntra : kKItxInter));
+  constexpr int kG = 4 << (C1 - 1);
+  __shared__ int s_tmp[64 * (kG + 1)];
+  const int kid = C0 == 0 ? (set ? kKItxIntra : kKItxInter) : (set ? kKItxIntraL : kKItxInterL);
+  const AV1H_CMEM int32_t* pre = kernel_prefix(B, kid);
   const int total = pre[B.njobs];
   for (int v = blockIdx.x; v < total; v += gridDim.x) {
     const int j = find_job(pre, B.njobs, v);
     RArgs a = recon_args(B, j);
-    itx_body(a, a.plan.itx_offs + set * (kTxClasses + 1), v - pre[j], s_tmp);
+    itx_body<C0, C1>(a, a.plan.itx_offs + set * (kTxClasses + 1), v - pre[j], s_tmp);
     __syncthreads();
   }
 }
@@ -1323,9 +1329,9 @@ __global__ void k_zero(uint32_t* p, int n) {
 
 }  // namespace
 
-int itx_groups(const FramePlanDev& p, int set) {
+int itx_groups(const FramePlanDev& p, int set, int c0, int c1) {
   int g = 0;
-  for (int c = 0; c < kTxClasses; ++c) {
+  for (int c = c0; c < c1; ++c) {
     const int n = p.itx_off[set][c + 1] - p.itx_off[set][c], per = 16 >> c;
     g += (n + per - 1) / per;
   }
@@ -1337,8 +1343,10 @@ hipError_t launch_recon_kernel(const BatchDev& b, int k, int total, int cap, hip
   switch (k) {
     case kKMc: hipLaunchKernelGGL(k_mc, grid, dim3(256), 0, s, b); break;
     case kKSpecial: hipLaunchKernelGGL(k_inter_special, grid, dim3(64), 0, s, b); break;
-    case kKItxInter: hipLaunchKernelGGL(k_itx, grid, dim3(64), 0, s, b, 0); break;
-    case kKItxIntra: hipLaunchKernelGGL(k_itx, grid, dim3(64), 0, s, b, 1); break;
+    case kKItxInter: hipLaunchKernelGGL((k_itx<0, 3>), grid, dim3(64), 0, s, b, 0); break;
+    case kKItxIntra: hipLaunchKernelGGL((k_itx<0, 3>), grid, dim3(64), 0, s, b, 1); break;
+    case kKItxInterL: hipLaunchKernelGGL((k_itx<3, 5>), grid, dim3(64), 0, s, b, 0); break;
+    case kKItxIntraL: hipLaunchKernelGGL((k_itx<3, 5>), grid, dim3(64), 0, s, b, 1); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
