@@ -625,10 +625,18 @@ __device__ void edge_filter(int* buf, int* tmp, int sz, int strength) {
   wsync();
 }
 
+/* The tables intra prediction indexes per lane (smooth weights, edge
+ * kernels, filter intra taps), staged in LDS by the queue kernel: one
+ * contiguous range of the specification table. */
+constexpr int kTabLo = AV1T_OFF_SM_WEIGHTS, kTabHi = AV1T_OFF_II_WEIGHTS;
+static_assert(AV1T_OFF_EDGE_KERNEL > kTabLo && AV1T_OFF_FILTER_INTRA > kTabLo && AV1T_OFF_FILTER_INTRA < kTabHi,
+              "staged table range");
+#define TAB(off) (tab + ((off) - kTabLo))
+
 /* The edge filter of both edges at once: the above edge on entries
  * [0, sza), the left edge on [0, szl), one copy phase and one filter phase
  * for the two (av1gpu_kernels.h:1157-1173 on each). */
-__device__ void edge_filter2(int* above, int* left, int* tmp, int sza, int stra, int szl, int strl) {
+__device__ void edge_filter2(int* above, int* left, int* tmp, int sza, int stra, int szl, int strl, const int* tab) {
   const int lane = threadIdx.x;
   if (stra <= 0) sza = 0;
   if (strl <= 0) szl = 0;
@@ -642,8 +650,8 @@ __device__ void edge_filter2(int* above, int* left, int* tmp, int sza, int stra,
       tl[i - sza] = left[i - sza - 1];
   }
   wsync();
-  const int* ka = SPEC(AV1T_OFF_EDGE_KERNEL) + 5 * (stra > 0 ? stra - 1 : 0);
-  const int* kl = SPEC(AV1T_OFF_EDGE_KERNEL) + 5 * (strl > 0 ? strl - 1 : 0);
+  const int* ka = TAB(AV1T_OFF_EDGE_KERNEL) + 5 * (stra > 0 ? stra - 1 : 0);
+  const int* kl = TAB(AV1T_OFF_EDGE_KERNEL) + 5 * (strl > 0 ? strl - 1 : 0);
   for (int j = lane; j < sza + szl; j += 64) {
     const bool is_a = j < sza;
     const int i = is_a ? j : j - sza, sz = is_a ? sza : szl;
@@ -718,7 +726,7 @@ __device__ void edge_upsample(int* buf, int* tmp, int n, int maxv) {
 /* The whole intra prediction of one block into s_pred (av1gpu_kernels.h:
  * 1578-1677 with the edges of 1108-1151 and the directional preparation of
  * 1452-1481). */
-__device__ void predict_intra(RArgs a, BlkRef b, int* work, uint16_t* pred) {
+__device__ void predict_intra(RArgs a, BlkRef b, int* work, uint16_t* pred, const int* tab) {
   const int lane = threadIdx.x;
   const int w = b.w, h = b.h, lw = ilog2(w), p = b.plane;
   const int stride = a.lay.stride[p];
@@ -809,7 +817,7 @@ __device__ void predict_intra(RArgs a, BlkRef b, int* work, uint16_t* pred) {
      * of 4x2 sub-blocks at a time: a sub-block reads only the ones above, to
      * its left and above-left */
     const int w4 = w >> 2, h2 = h >> 1;
-    const int* taps = SPEC(AV1T_OFF_FILTER_INTRA) + 56 * b.fi_mode;
+    const int* taps = TAB(AV1T_OFF_FILTER_INTRA) + 56 * b.fi_mode;
     for (int d = 0; d < w4 + h2 - 1; ++d) {
       const int c_lo = max(0, d - (h2 - 1)), c_hi = min(d, w4 - 1);
       const int nsub = c_hi - c_lo + 1;
@@ -849,7 +857,7 @@ __device__ void predict_intra(RArgs a, BlkRef b, int* work, uint16_t* pred) {
       edge_filter2(above, left, tmp, min(w, max_x - b.x + 1) + (pa < 90 ? h : 0) + 1,
                    have_above ? edge_strength(w, h, b.filter_type, pa - 90) : 0,
                    min(h, max_y - b.y + 1) + (pa > 180 ? w : 0) + 1,
-                   have_left ? edge_strength(w, h, b.filter_type, pa - 180) : 0);
+                   have_left ? edge_strength(w, h, b.filter_type, pa - 180) : 0, tab);
       ua = upsample_sel(w, h, b.filter_type, pa - 90);
       ul = upsample_sel(w, h, b.filter_type, pa - 180);
       edge_upsample2(above, left, tmp, ua ? w + (pa < 90 ? h : 0) : 0, ul ? h + (pa > 180 ? w : 0) : 0, maxv);
@@ -901,7 +909,7 @@ __device__ void predict_intra(RArgs a, BlkRef b, int* work, uint16_t* pred) {
   }
   if (mode >= AV1H_PRED_SMOOTH && mode <= AV1H_PRED_SMOOTH_H) {
     /* av1gpu_kernels.h:1333-1353 */
-    const int* sw = SPEC(AV1T_OFF_SM_WEIGHTS);
+    const int* sw = TAB(AV1T_OFF_SM_WEIGHTS);
     for (int s = lane; s < w * h; s += 64) {
       const int y = s >> lw, x = s & (w - 1);
       int v;
@@ -1061,9 +1069,10 @@ __device__ __forceinline__ EarlyRes early_residual(RArgs a, BlkRef b) {
   return e;
 }
 
-__device__ void reconstruct_queued(RArgs a, BlkRef b, int* work, uint16_t* pred, const EarlyRes& er) {
+__device__ void reconstruct_queued(RArgs a, BlkRef b, int* work, uint16_t* pred, const EarlyRes& er,
+                                   const int* tab) {
   if (b.kind == kIntra) {
-    predict_intra(a, b, work, pred);
+    predict_intra(a, b, work, pred, tab);
   } else {
     /* inter/intra or intra block copy: reads the frame being decoded */
     const int lw = ilog2(b.w);
@@ -1285,6 +1294,9 @@ __global__ __launch_bounds__(64) void k_inter_special(BatchDev B) {
 __global__ __launch_bounds__(64) void k_queue(BatchDev B) {
   __shared__ int s_work[kWork];
   __shared__ uint16_t s_pred[64 * 64];
+  __shared__ int s_tab[kTabHi - kTabLo];
+  for (int i = threadIdx.x; i < kTabHi - kTabLo; i += 64) s_tab[i] = av1t_spec_table[kTabLo + i];
+  __syncthreads();
   const int nqueue = kernel_prefix(B, kKernels)[0];
   for (;;) {
     int idx = 0;
@@ -1305,7 +1317,7 @@ __global__ __launch_bounds__(64) void k_queue(BatchDev B) {
     const EarlyRes er = early_residual(a, b);
     wait_cells(a, b, B.error);
     const uint64_t t1 = B.trace ? (uint64_t)wall_clock64() : 0;
-    reconstruct_queued(a, b, s_work, s_pred, er);
+    reconstruct_queued(a, b, s_work, s_pred, er, s_tab);
     /* every sample store of the block has completed before its cells flip */
     __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");

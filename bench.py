@@ -386,7 +386,7 @@ def run_serve(args):
     log(f"serve: queue levels per group: max {levels}, median {inter_levels}; heavy above {heavy}")
     srv = pkg.Server(0, max_batch=args.serve_batch, planner_threads=0, hbm_budget=200 << 30,
                      nv12_pitch=SERVE_PITCH, hash_pictures=True, launchers=args.serve_launchers,
-                     heavy_levels=heavy, heavy_launchers=max(1, args.serve_launchers - 1))
+                     heavy_levels=heavy, heavy_launchers=args.serve_heavy_launchers)
     streams = []  # (stream id, class)
     for ci, (_name, n, w, h, bd, _cw, _g) in enumerate(SERVE_CLASSES):
         for _ in range(n):
@@ -439,6 +439,7 @@ def run_serve(args):
     owner.close()
     px = [w * h for _n, _c, w, h, _b, _cw, _g in SERVE_CLASSES]
     lat = {ci: [] for ci in range(len(SERVE_CLASSES))}
+    lat_key, lat_inter = [], []
     mism = 0
     t_lo, t_hi = t0 + warm * 1e3, t0 + (warm + dur) * 1e3
     pix = 0
@@ -450,6 +451,7 @@ def run_serve(args):
             mism += 1
         if c["arrival_ms"] >= t_lo:
             lat[ci].append(c["done_ms"] - c["arrival_ms"])
+            (lat_key if k % SERVE_GOP == 0 else lat_inter).append(c["done_ms"] - c["arrival_ms"])
             pix += px[ci]
             last_done = max(last_done, c["done_ms"])
     allv = [v for ci in lat for v in lat[ci]]
@@ -466,6 +468,10 @@ def run_serve(args):
         "latency_ms_by_class": {SERVE_CLASSES[ci][0]: {"p50": round(pct(lat[ci], 50), 3),
                                                        "p99": round(pct(lat[ci], 99), 3)}
                                 for ci in lat if lat[ci]},
+        "latency_ms_key_frames": {"p50": round(pct(lat_key, 50), 3), "p99": round(pct(lat_key, 99), 3),
+                                  "frames": len(lat_key)},
+        "latency_ms_inter_frames": {"p50": round(pct(lat_inter, 50), 3), "p99": round(pct(lat_inter, 99), 3),
+                                    "frames": len(lat_inter)},
         "batches": st["batches"], "mean_batch_frames": round(st["frames"] / max(st["batches"], 1), 2),
         "device_busy_frac": round(st["busy_ms"] / (max(c["done_ms"] for c in done) - t0), 3),
         "bit_exact": {"frames_checked": len(done), "mismatched_frames": mism,
@@ -497,6 +503,7 @@ def main():
     ap.add_argument("--serve-warmup", type=float, default=1.0)
     ap.add_argument("--serve-batch", type=int, default=32)
     ap.add_argument("--serve-launchers", type=int, default=2)
+    ap.add_argument("--serve-heavy-launchers", type=int, default=1)
     ap.add_argument("--serve-heavy", type=int, default=-1,
                     help="dependency levels above which a frame runs on the key-frame launcher (-1: automatic, 0: off)")
     args = ap.parse_args()
