@@ -247,11 +247,27 @@ def hip_lib():
     return _hip
 
 
+def _get_tuning(lib, h):
+    n = lib.av1h_tune_params()
+    buf = (ctypes.c_int32 * n)()
+    lib.av1h_tune_get(h, buf, n)
+    return {lib.av1h_tune_name(i).decode(): buf[i] for i in range(n)}
+
+
+def _set_tuning(lib, h, values):
+    cur = _get_tuning(lib, h)
+    cur.update(values)
+    names = [lib.av1h_tune_name(i).decode() for i in range(len(cur))]
+    buf = (ctypes.c_int32 * len(names))(*[cur[k] for k in names])
+    if lib.av1h_tune_set(h, buf, len(names)) != AV1H_OK:
+        raise HipError(f"invalid launch shapes {values}")
+
+
 class ServerConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("max_batch", ctypes.c_int32), ("planner_threads", ctypes.c_int32),
                 ("capture", ctypes.c_int32), ("hash_pictures", ctypes.c_int32), ("launchers", ctypes.c_int32),
                 ("heavy_levels", ctypes.c_int32), ("heavy_launchers", ctypes.c_int32),
-                ("hbm_budget", ctypes.c_int64), ("nv12_pitch", ctypes.c_size_t)]
+                ("heavy_batch", ctypes.c_int32), ("hbm_budget", ctypes.c_int64), ("nv12_pitch", ctypes.c_size_t)]
 
 
 class Completion(ctypes.Structure):
@@ -284,12 +300,15 @@ def _bind_server(lib):
     lib.av1h_server_poll.argtypes = [P, ctypes.POINTER(Completion), ctypes.c_int, ctypes.c_int]
     lib.av1h_server_free_picture.argtypes = [ctypes.POINTER(ctypes.c_uint16)]
     lib.av1h_server_get_stats.argtypes = [P, ctypes.POINTER(ServerStats)]
+    lib.av1h_server_launcher.restype = P
+    lib.av1h_server_launcher.argtypes = [P, ctypes.c_int]
     lib.av1h_server_last_error.restype = ctypes.c_char_p
     lib.av1h_server_last_error.argtypes = [P]
     lib.av1h_stream_bytes.restype = ctypes.c_int64
     lib.av1h_stream_bytes.argtypes = [ctypes.c_int] * 6 + [ctypes.c_size_t]
     lib.av1h_set_pool_slots.argtypes = [P, ctypes.c_int]
     lib.av1h_plan_levels.argtypes = [P]
+    lib.av1h_plan_queue_counts.argtypes = [P, ctypes.POINTER(ctypes.c_int32)]
 
 
 def _bind_costmodel(lib):
@@ -487,6 +506,13 @@ class HipDecoder:
         if rc != AV1H_OK:
             raise HipError(f"to_nv12 failed ({rc}): {self.error()}")
 
+    def queue_counts(self, plan):
+        """{blocks, packed_readers, publishers, unmarked} of a plan's queue."""
+        c = (ctypes.c_int32 * 4)()
+        if self._lib.av1h_plan_queue_counts(plan, c) != AV1H_OK:
+            raise HipError("invalid plan")
+        return dict(zip(("blocks", "packed_readers", "publishers", "unmarked"), list(c)))
+
     def sb_features(self, plan):
         """(superblocks x 24) features of a plan."""
         n = self._lib.av1h_plan_sb_features(plan, None)
@@ -522,18 +548,10 @@ class HipDecoder:
 
     def tuning(self):
         """{parameter name: value} of the launch shapes of batches this context leads."""
-        n = self._lib.av1h_tune_params()
-        buf = (ctypes.c_int32 * n)()
-        self._lib.av1h_tune_get(self._h, buf, n)
-        return {self._lib.av1h_tune_name(i).decode(): buf[i] for i in range(n)}
+        return _get_tuning(self._lib, self._h)
 
     def set_tuning(self, values):
-        cur = self.tuning()
-        cur.update(values)
-        names = [self._lib.av1h_tune_name(i).decode() for i in range(len(cur))]
-        buf = (ctypes.c_int32 * len(names))(*[cur[k] for k in names])
-        if self._lib.av1h_tune_set(self._h, buf, len(names)) != AV1H_OK:
-            raise HipError(f"invalid launch shapes {values}")
+        _set_tuning(self._lib, self._h, values)
 
     def set_output_hash(self, enable=True):
         """Checksum every decoded picture on the device (picture_hash for the host version)."""
@@ -659,11 +677,12 @@ class Server:
     formed batches."""
 
     def __init__(self, device=0, max_batch=16, planner_threads=4, capture=False, hbm_budget=200 << 30,
-                 nv12_pitch=0, hash_pictures=False, launchers=2, heavy_levels=0, heavy_launchers=1):
+                 nv12_pitch=0, hash_pictures=False, launchers=2, heavy_levels=0, heavy_launchers=1,
+                 heavy_batch=1):
         lib = hip_lib()
         self._lib = lib
         cfg = ServerConfig(device, max_batch, planner_threads, 1 if capture else 0, 1 if hash_pictures else 0,
-                           launchers, heavy_levels, heavy_launchers, hbm_budget, nv12_pitch)
+                           launchers, heavy_levels, heavy_launchers, heavy_batch, hbm_budget, nv12_pitch)
         h = ctypes.c_void_p()
         rc = lib.av1h_server_create(ctypes.byref(cfg), ctypes.byref(h))
         if rc != AV1H_OK:
@@ -715,6 +734,20 @@ class Server:
                 self._lib.av1h_server_free_picture(c.picture)
             out.append(d)
         return out
+
+    def launcher_tuning(self, index):
+        """Launch shapes of launcher `index` (heavy-frame launchers first)."""
+        return _get_tuning(self._lib, self._launcher(index))
+
+    def set_launcher_tuning(self, index, values):
+        """Sets launch shapes of launcher `index`; before frames are submitted."""
+        _set_tuning(self._lib, self._launcher(index), values)
+
+    def _launcher(self, index):
+        h = self._lib.av1h_server_launcher(self._h, index)
+        if not h:
+            raise HipError(f"no launcher {index}")
+        return h
 
     def stats(self):
         st = ServerStats()

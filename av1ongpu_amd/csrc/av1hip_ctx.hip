@@ -44,7 +44,7 @@ void make_layout(Layout* g, int width, int height, int bd, int ss_x, int ss_y, i
   g->plane_h[0] = height;
   g->plane_w[1] = g->plane_w[2] = (coded_width + (1 << ss_x) - 1) >> ss_x;
   g->plane_h[1] = g->plane_h[2] = g->out_h[1];
-  int64_t size = 0, cells = 0;
+  int64_t size = 0, cells = 0, edges = 0;
   for (int p = 0; p < 3; ++p) {
     const int sx = p ? ss_x : 0, sy = p ? ss_y : 0;
     g->grid_w[p] = ((coded_width + 7) & ~7) >> sx;
@@ -60,9 +60,16 @@ void make_layout(Layout* g, int width, int height, int bd, int ss_x, int ss_y, i
     g->cells_h[p] = (g->grid_h[p] + (1 << g->cell_shift[p]) - 1) >> g->cell_shift[p];
     g->cell_off[p] = cells;
     cells += (int64_t)g->cells_w[p] * g->cells_h[p];
+    g->edge_row_words[p] = (g->cells_w[p] << g->cell_shift[p]) >> 1;
+    g->edge_col_words[p] = (g->cells_h[p] << g->cell_shift[p]) >> 1;
+    g->edge_row_off[p] = edges;
+    edges += (int64_t)g->cells_h[p] * g->edge_row_words[p];
+    g->edge_col_off[p] = edges;
+    edges += (int64_t)g->cells_w[p] * g->edge_col_words[p];
   }
   g->size = size;
   g->cells = cells;
+  g->edge_words = edges;
 }
 
 }  // namespace av1hip
@@ -103,6 +110,9 @@ struct Plan {
    * (q_dep[q_dep_off[qi] .. q_dep_off[qi + 1])), and the qi of every entry
    * of the static order */
   std::vector<int32_t> q_block, q_dep_off, q_dep, queue_qi, q_area, q_level;
+  /* queued blocks, packed-edge readers, edge publishers, blocks no wait
+   * reads the cells of (av1h_plan_queue_counts) */
+  int32_t q_counts[4] = {0, 0, 0, 0};
 };
 
 struct Ctx {
@@ -119,6 +129,8 @@ struct Ctx {
   uint16_t* pool = nullptr;
   uint32_t* cells = nullptr;
   int64_t cells_cap = 0;
+  uint64_t* edges = nullptr; /* packed edge words, cleared with the cells */
+  int64_t edges_cap = 0;
   uint32_t epoch = 0;
   uint32_t* counters = nullptr; /* [0] queue counter, [1] error word */
   int32_t* cdef_dir = nullptr;   /* CDEF direction map                */
@@ -477,6 +489,53 @@ int resolve_slot(const av1h_frame_header& h, int ref) {
   return h.reference_slots[ref];
 }
 
+/* Whether intra block d can take its edges from packed words: every sample
+ * of its above row (x .. the last it depends on) is the last row of a queued
+ * block, every sample of its left column the last column of one, and its
+ * corner either. Then the owners are flagged to publish those rows and
+ * columns, and d its corner's side. `owner` maps the plane's cells to the
+ * queued blocks decoded before d (index into queue_list), -1 elsewhere. */
+bool packed_edges(const Layout& L, DevBlock& d, const std::vector<int32_t>& owner, std::vector<DevBlock>& blocks,
+                  const std::vector<int32_t>& queue_list) {
+  if (d.kind != kIntra || (d.flags & (kPalette | kIntraBC)) || (d.mode == AV1H_PRED_CFL && d.plane > 0)) return false;
+  const int p = d.plane, sh = L.cell_shift[p], cell = 1 << sh;
+  const bool ha = (d.flags & kHaveAbove) != 0, hl = (d.flags & kHaveLeft) != 0;
+  const int alast = std::min(L.grid_w[p] - 1, d.x + ((d.flags & kHaveAR) ? 2 * d.w : d.w) - 1);
+  const int llast = std::min(L.grid_h[p] - 1, d.y + ((d.flags & kHaveBL) ? 2 * d.h : d.h) - 1);
+  const int ae = std::min(d.dep_x1, alast), le = std::min(d.dep_y1, llast);
+  if ((ha && ae < d.x) || (hl && le < d.y)) return false;
+  auto own = [&](int px, int py) -> DevBlock* {
+    const int o = owner[(size_t)(py >> sh) * L.cells_w[p] + (px >> sh)];
+    return o < 0 ? nullptr : &blocks[(size_t)queue_list[(size_t)o]];
+  };
+  if (ha)
+    for (int px = d.x; px <= ae; px += cell) {
+      const DevBlock* o = own(px, d.y - 1);
+      if (!o || o->y + o->h != d.y) return false;
+    }
+  if (hl)
+    for (int py = d.y; py <= le; py += cell) {
+      const DevBlock* o = own(d.x - 1, py);
+      if (!o || o->x + o->w != d.x) return false;
+    }
+  int corner = 0; /* 1: row, 2: column */
+  if (ha && hl) {
+    const DevBlock* o = own(d.x - 1, d.y - 1);
+    if (!o) return false;
+    corner = o->y + o->h == d.y ? 1 : (o->x + o->w == d.x ? 2 : 0);
+    if (!corner) return false;
+  }
+  if (ha)
+    for (int px = d.x; px <= ae; px += cell) own(px, d.y - 1)->flags |= kEdgeRow;
+  if (hl)
+    for (int py = d.y; py <= le; py += cell) own(d.x - 1, py)->flags |= kEdgeCol;
+  if (corner) {
+    own(d.x - 1, d.y - 1)->flags |= corner == 1 ? kEdgeRow : kEdgeCol;
+    if (corner == 1) d.flags |= kCornerRow;
+  }
+  return true;
+}
+
 int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
   const av1h_frame_header& hdr = f.hdr;
   if (hdr.width <= 0 || hdr.height <= 0 || hdr.width > 16384 || hdr.height > 16384)
@@ -783,8 +842,16 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
     plan->q_area.assign((size_t)nq, 0);
     std::vector<int32_t> level((size_t)nq, 0);
     std::vector<int32_t> deps;
+    /* packed edges need every queued block on whole cells, so that each
+     * packed word has one writer */
+    bool packed = true;
     for (int qi = 0; qi < nq; ++qi) {
       const DevBlock& d = H->blocks[(size_t)H->queue_list[(size_t)qi]];
+      if ((d.x | d.y | d.w | d.h) & ((1 << L.cell_shift[d.plane]) - 1)) packed = false;
+    }
+    std::vector<char> need_mark((size_t)nq, 0);
+    for (int qi = 0; qi < nq; ++qi) {
+      DevBlock& d = H->blocks[(size_t)H->queue_list[(size_t)qi]];
       const int p = d.plane, sh = L.cell_shift[p], cell = 1 << sh;
       const int gw = L.grid_w[p], gh = L.grid_h[p];
       deps.clear();
@@ -812,6 +879,11 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
           for (int cx = d.dep_rx0; cx <= d.dep_rx1; ++cx) dep(p, cx, cy);
       std::sort(deps.begin(), deps.end());
       deps.erase(std::unique(deps.begin(), deps.end()), deps.end());
+      if (packed && packed_edges(L, d, owner[p], H->blocks, H->queue_list)) {
+        d.flags |= kPackedEdges;
+      } else {
+        for (int32_t o : deps) need_mark[(size_t)o] = 1;
+      }
       int lv = 0;
       for (int32_t o : deps) lv = std::max(lv, level[(size_t)o] + 1);
       level[(size_t)qi] = lv;
@@ -823,6 +895,15 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
       for (int cy = d.y >> sh; cy < std::min((d.y >> sh) + ch, L.cells_h[p]); ++cy)
         for (int cx = d.x >> sh; cx < std::min((d.x >> sh) + cw, L.cells_w[p]); ++cx)
           owner[p][(size_t)cy * L.cells_w[p] + cx] = qi;
+    }
+    for (int k = 0; k < 4; ++k) plan->q_counts[k] = 0;
+    plan->q_counts[0] = nq;
+    for (int qi = 0; qi < nq; ++qi) {
+      DevBlock& d = H->blocks[(size_t)H->queue_list[(size_t)qi]];
+      if (!need_mark[(size_t)qi]) d.flags |= kSkipMark;
+      plan->q_counts[1] += (d.flags & kPackedEdges) != 0;
+      plan->q_counts[2] += (d.flags & (kEdgeRow | kEdgeCol)) != 0;
+      plan->q_counts[3] += (d.flags & kSkipMark) != 0;
     }
     plan->q_block = H->queue_list; /* coding order */
     plan->q_level = level;
@@ -1327,13 +1408,20 @@ int prepare(Ctx* c, const Layout& L) {
     HIP_OK(c, hipMalloc(&c->cdef_dir, ndir * 4));
     c->cdef_dir_cap = ndir;
   }
-  if (c->cells_cap < L.cells) {
+  if (c->cells_cap < L.cells || c->edges_cap < L.edge_words) {
+    /* epochs restart at 1: every cell and edge word starts below */
     if (c->cells) (void)hipFree(c->cells);
+    if (c->edges) (void)hipFree(c->edges);
     c->cells = nullptr;
+    c->edges = nullptr;
+    c->cells_cap = c->edges_cap = 0;
     HIP_OK(c, hipMalloc(&c->cells, (size_t)L.cells * 4));
+    HIP_OK(c, hipMalloc(&c->edges, (size_t)L.edge_words * 8));
     HIP_OK(c, hipMemsetAsync(c->cells, 0, (size_t)L.cells * 4, c->stream));
+    HIP_OK(c, hipMemsetAsync(c->edges, 0, (size_t)L.edge_words * 8, c->stream));
     c->dirty = true;
     c->cells_cap = L.cells;
+    c->edges_cap = L.edge_words;
     c->epoch = 0;
   }
   c->lay = L;
@@ -1519,6 +1607,7 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
     ra.pool = c->pool;
     ra.slot_size = L.size;
     ra.cells = c->cells;
+    ra.edges = c->edges;
     ra.epoch = c->epoch;
     ra.res = reinterpret_cast<int16_t*>(c->scratch); /* free until the CDEF snapshot */
     ra.maxv = (1 << L.bit_depth) - 1;
@@ -1905,7 +1994,7 @@ void av1h_destroy(av1h_ctx* h) {
   drop_graphs(c);
   if (c->hash_dev) (void)hipFree(c->hash_dev);
   for (void* p : {(void*)c->frame, (void*)c->scratch, (void*)c->lr_curr, (void*)c->lr_cdef, (void*)c->lr_prev_up,
-                  (void*)c->pool, (void*)c->cells, (void*)c->counters, (void*)c->grain_tmpl, (void*)c->grain_offs,
+                  (void*)c->pool, (void*)c->cells, (void*)c->edges, (void*)c->counters, (void*)c->grain_tmpl, (void*)c->grain_offs,
                   (void*)c->pack, (void*)c->cdef_dir, (void*)c->trace})
     if (p) (void)hipFree(p);
   if (c->host_pin) (void)hipHostFree(c->host_pin);
@@ -2090,6 +2179,13 @@ int av1h_plan_levels(const void* plan) {
   return p->key_off_static.empty() ? 0 : (int)p->key_off_static.size() - 1;
 }
 
+int av1h_plan_queue_counts(const void* plan, int32_t* counts) {
+  if (!plan || !counts) return AV1H_ERR_INVALID;
+  const Plan* p = static_cast<const Plan*>(plan);
+  for (int k = 0; k < 4; ++k) counts[k] = p->q_counts[k];
+  return AV1H_OK;
+}
+
 int av1h_plan_sb_features(const void* plan, float* features) {
   if (!plan) return AV1H_ERR_INVALID;
   const Plan* p = static_cast<const Plan*>(plan);
@@ -2219,7 +2315,8 @@ int64_t av1h_stream_bytes(int width, int height, int bit_depth, int ss_x, int ss
   make_layout(&L, width, height, bit_depth, ss_x, ss_y, width);
   const int64_t pic = L.size * 2;
   int64_t b = pic * (5 + pool_slots);                                  /* work pictures + pool */
-  b += L.cells * 4 + (int64_t)(L.grid_w[0] >> 3) * (L.grid_h[0] >> 3) * 8; /* cells, CDEF directions */
+  b += L.cells * 4 + L.edge_words * 8;                                  /* cells, packed edges */
+  b += (int64_t)(L.grid_w[0] >> 3) * (L.grid_h[0] >> 3) * 8;             /* CDEF directions */
   b += 3 * 82 * 73 * 2 + 65536;                                          /* grain templates, offsets */
   if (nv12_pitch) b += (int64_t)av1h_nv12_size(width, height, ss_x, ss_y, nv12_pitch);
   return b;

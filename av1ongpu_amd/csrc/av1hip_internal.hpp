@@ -32,6 +32,13 @@ struct Layout {
   int cells_w[3], cells_h[3];
   int64_t cell_off[3];
   int64_t cells;
+  /* packed edges: per plane, the last sample row of every cell row
+   * (edge_row_words 64-bit words each) and the last sample column of every
+   * cell column (edge_col_words each); a word holds two samples and the
+   * epoch of the frame that wrote them */
+  int edge_row_words[3], edge_col_words[3];
+  int64_t edge_row_off[3], edge_col_off[3];
+  int64_t edge_words;
 };
 
 void make_layout(Layout* g, int width, int height, int bit_depth, int ss_x, int ss_y,
@@ -46,7 +53,13 @@ enum {
   kHaveLeft = 1, kHaveAbove = 2, kHaveAR = 4, kHaveBL = 8, kFilterIntra = 16,
   kPalette = 32, kLossless = 64, kResidualRaw = 128, kInterIntra = 256,
   kIIWedge = 512, kWarp0 = 1024, kWarp1 = 2048, kMaskWarp0 = 4096, kMaskWarp1 = 8192,
-  kIntraBC = 16384
+  kIntraBC = 16384,
+  /* queue protocol: kPackedEdges - every edge sample the block reads comes
+   * from a queued block's packed edge words (no cell wait, no frame read);
+   * kCornerRow - its corner sample is in the packed row, else the column;
+   * kEdgeRow / kEdgeCol - the block publishes its last row / column as
+   * packed words; kSkipMark - no block waits on its cells */
+  kPackedEdges = 1 << 15, kCornerRow = 1 << 16, kEdgeRow = 1 << 17, kEdgeCol = 1 << 18, kSkipMark = 1 << 19
 };
 
 struct DevBlock {
@@ -201,6 +214,7 @@ struct ReconArgs {
   const uint16_t* pool;
   int64_t slot_size;
   uint32_t* cells;      /* epoch per dependency cell          */
+  uint64_t* edges;      /* packed edge words (Layout::edge_*)  */
   uint32_t epoch;
   int16_t* res;         /* residual of intra blocks, frame layout */
   int maxv;

@@ -726,7 +726,103 @@ __device__ void edge_upsample(int* buf, int* tmp, int n, int maxv) {
 /* The whole intra prediction of one block into s_pred (av1gpu_kernels.h:
  * 1578-1677 with the edges of 1108-1151 and the directional preparation of
  * 1452-1481). */
-__device__ void predict_intra(RArgs a, BlkRef b, int* work, uint16_t* pred, const int* tab) {
+/* The edges of a block flagged kPackedEdges: its above row (x .. ae), left
+ * column (y .. le) and corner come from the packed words its queued
+ * neighbours published, each 64-bit word two samples and the epoch of the
+ * frame that wrote them, so the one load per word is both the dependency
+ * wait and the read (no cell flag, no second round trip to the frame).
+ * Entries past ae / le repeat the last sample: only modes that read no
+ * further than their dependencies reach use them. */
+__device__ void gather_packed(RArgs a, BlkRef b, int* above, int* left, int alast, int llast, int mid,
+                              uint32_t* error) {
+  LayRef L = a.lay;
+  const int p = b.plane, sh = L.cell_shift[p];
+  const int x = b.x, y = b.y, lane = threadIdx.x;
+  const int ha = (b.flags & kHaveAbove) != 0, hl = (b.flags & kHaveLeft) != 0;
+  const int ae = min(b.dep_x1, alast), le = min(b.dep_y1, llast);
+  const int wa0 = x >> 1, na = ha ? (ae >> 1) - wa0 + 1 : 0;
+  const int wl0 = y >> 1, nl = hl ? (le >> 1) - wl0 + 1 : 0;
+  const int nc = ha && hl ? 1 : 0;
+  const uint64_t* rowp = a.edges + L.edge_row_off[p] + (int64_t)((y >> sh) - 1) * L.edge_row_words[p];
+  const uint64_t* colp = a.edges + L.edge_col_off[p] + (int64_t)((x >> sh) - 1) * L.edge_col_words[p];
+  for (int i = lane; i < na + nl + nc; i += 64) {
+    const uint64_t* q;
+    if (i < na)
+      q = rowp + wa0 + i;
+    else if (i < na + nl)
+      q = colp + wl0 + (i - na);
+    else
+      q = (b.flags & kCornerRow) ? rowp + wa0 - 1 : colp + wl0 - 1;
+    uint64_t v;
+    for (int iter = 0;; ++iter) {
+      v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if ((uint32_t)(v >> 32) == a.epoch) break;
+      if (iter >= kSpinLimit) {
+        atomicOr(error, 1u);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    const int s0 = (int)(v & 0xffff), s1 = (int)((v >> 16) & 0xffff);
+    if (i < na) {
+      const int j = 2 * (wa0 + i) - x;
+      if (j <= ae - x) above[j] = s0;
+      if (j + 1 <= ae - x) above[j + 1] = s1;
+    } else if (i < na + nl) {
+      const int j = 2 * (wl0 + i - na) - y;
+      if (j <= le - y) left[j] = s0;
+      if (j + 1 <= le - y) left[j + 1] = s1;
+    } else {
+      above[-1] = s1; /* sample x - 1 of the row, or y - 1 of the column */
+      left[-1] = s1;
+    }
+  }
+  wsync();
+  const int n = b.w + b.h;
+  for (int i = lane; i < n; i += 64) {
+    if (ha) {
+      if (i > ae - x) above[i] = above[ae - x];
+    } else {
+      above[i] = hl ? left[0] : mid - 1;
+    }
+    if (hl) {
+      if (i > le - y) left[i] = left[le - y];
+    } else {
+      left[i] = ha ? above[0] : mid + 1;
+    }
+  }
+  if (lane == 0 && !nc) {
+    const int c = ha ? above[0] : (hl ? left[0] : mid);
+    above[-1] = c;
+    left[-1] = c;
+  }
+}
+
+/* A queued block's last row and column as packed words (gather_packed). */
+__device__ void write_edges(RArgs a, BlkRef b, const uint16_t* pred) {
+  LayRef L = a.lay;
+  const int p = b.plane, sh = L.cell_shift[p], w = b.w, h = b.h;
+  const int nr = (b.flags & kEdgeRow) ? w >> 1 : 0, nc = (b.flags & kEdgeCol) ? h >> 1 : 0;
+  const uint64_t tag = (uint64_t)a.epoch << 32;
+  uint64_t* rowp = a.edges + L.edge_row_off[p] + (int64_t)(((b.y + h) >> sh) - 1) * L.edge_row_words[p] + (b.x >> 1);
+  uint64_t* colp = a.edges + L.edge_col_off[p] + (int64_t)(((b.x + w) >> sh) - 1) * L.edge_col_words[p] + (b.y >> 1);
+  const uint16_t* last = pred + (h - 1) * w;
+  for (int k = threadIdx.x; k < nr + nc; k += 64) {
+    uint64_t v;
+    uint64_t* q;
+    if (k < nr) {
+      v = tag | last[2 * k] | ((uint64_t)last[2 * k + 1] << 16);
+      q = rowp + k;
+    } else {
+      const int j = k - nr;
+      v = tag | pred[2 * j * w + w - 1] | ((uint64_t)pred[(2 * j + 1) * w + w - 1] << 16);
+      q = colp + j;
+    }
+    __hip_atomic_store(q, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+__device__ void predict_intra(RArgs a, BlkRef b, int* work, uint16_t* pred, const int* tab, uint32_t* error) {
   const int lane = threadIdx.x;
   const int w = b.w, h = b.h, lw = ilog2(w), p = b.plane;
   const int stride = a.lay.stride[p];
@@ -747,7 +843,10 @@ __device__ void predict_intra(RArgs a, BlkRef b, int* work, uint16_t* pred, cons
   const int mid = 1 << (bd - 1);
   const int max_x = a.lay.grid_w[p] - 1, max_y = a.lay.grid_h[p] - 1;
   const int n = w + h;
-  {
+  if (b.flags & kPackedEdges) {
+    gather_packed(a, b, above, left, min(max_x, b.x + ((b.flags & kHaveAR) ? 2 * w : w) - 1),
+                  min(max_y, b.y + ((b.flags & kHaveBL) ? 2 * h : h) - 1), mid, error);
+  } else {
     const int alast = min(max_x, b.x + ((b.flags & kHaveAR) ? 2 * w : w) - 1);
     const int llast = min(max_y, b.y + ((b.flags & kHaveBL) ? 2 * h : h) - 1);
     for (int i = lane; i < n; i += 64) {
@@ -1070,9 +1169,9 @@ __device__ __forceinline__ EarlyRes early_residual(RArgs a, BlkRef b) {
 }
 
 __device__ void reconstruct_queued(RArgs a, BlkRef b, int* work, uint16_t* pred, const EarlyRes& er,
-                                   const int* tab) {
+                                   const int* tab, uint32_t* error) {
   if (b.kind == kIntra) {
-    predict_intra(a, b, work, pred, tab);
+    predict_intra(a, b, work, pred, tab, error);
   } else {
     /* inter/intra or intra block copy: reads the frame being decoded */
     const int lw = ilog2(b.w);
@@ -1315,13 +1414,16 @@ __global__ __launch_bounds__(64) void k_queue(BatchDev B) {
     RArgs a = recon_args(B, __builtin_amdgcn_readfirstlane(job));
     BlkRef b = cmem(a.plan.blocks)[__builtin_amdgcn_readfirstlane(blk)];
     const EarlyRes er = early_residual(a, b);
-    wait_cells(a, b, B.error);
+    if (!(b.flags & kPackedEdges)) wait_cells(a, b, B.error);
     const uint64_t t1 = B.trace ? (uint64_t)wall_clock64() : 0;
-    reconstruct_queued(a, b, s_work, s_pred, er, s_tab);
-    /* every sample store of the block has completed before its cells flip */
-    __builtin_amdgcn_s_waitcnt(0);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    mark_cells(a, b, true);
+    reconstruct_queued(a, b, s_work, s_pred, er, s_tab, B.error);
+    if (b.flags & (kEdgeRow | kEdgeCol)) write_edges(a, b, s_pred);
+    if (!(b.flags & kSkipMark)) {
+      /* every sample store of the block has completed before its cells flip */
+      __builtin_amdgcn_s_waitcnt(0);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      mark_cells(a, b, true);
+    }
     wsync();
     if (B.trace && threadIdx.x == 0) {
       /* diagnostics: when the block was taken, found ready, published */

@@ -226,9 +226,9 @@ void sched_main(av1h_server* s, av1h_ctx* launcher, int role) {
     }
     std::sort(ready.begin(), ready.end(),
               [](const SStream* a, const SStream* b) { return a->q.front()->arrival < b->q.front()->arrival; });
-    /* a heavy frame's queue is bound by its own critical path: it runs
-     * alone, next to the other heavy frames' batches */
-    const int cap = role == 1 ? 1 : s->cfg.max_batch;
+    /* a heavy frame's queue is bound by its own critical path; heavy
+     * frames batch among themselves (heavy_batch), beside the light ones */
+    const int cap = role == 1 ? std::max(s->cfg.heavy_batch, 1) : s->cfg.max_batch;
     if ((int)ready.size() > cap) ready.resize(cap);
     const int n = (int)ready.size();
     std::vector<av1h_ctx*> ctxs(n);
@@ -307,7 +307,7 @@ double av1h_server_now_ms(void) { return clock_ms(); }
 
 int av1h_server_create(const av1h_server_config* cfg, av1h_server** out) {
   if (!cfg || !out || cfg->max_batch < 1 || cfg->max_batch > 128 || cfg->planner_threads < 0 ||
-      cfg->launchers < 0 || cfg->launchers > 8 ||
+      cfg->launchers < 0 || cfg->launchers > 8 || cfg->heavy_batch < 0 || cfg->heavy_batch > 128 ||
       (cfg->nv12_pitch & 1))
     return AV1H_ERR_INVALID;
   *out = nullptr;
@@ -443,6 +443,11 @@ int av1h_server_get_stats(av1h_server* s, av1h_server_stats* out) {
   out->waiting = (int64_t)s->waiting.size();
   out->active = (int64_t)s->streams.size() - out->waiting;
   return AV1H_OK;
+}
+
+av1h_ctx* av1h_server_launcher(av1h_server* s, int index) {
+  if (!s || index < 0 || index >= (int)s->launchers.size()) return nullptr;
+  return s->launchers[index];
 }
 
 const char* av1h_server_last_error(av1h_server* s) { return s ? s->err.c_str() : "no server"; }

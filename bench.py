@@ -386,7 +386,8 @@ def run_serve(args):
     log(f"serve: queue levels per group: max {levels}, median {inter_levels}; heavy above {heavy}")
     srv = pkg.Server(0, max_batch=args.serve_batch, planner_threads=0, hbm_budget=200 << 30,
                      nv12_pitch=SERVE_PITCH, hash_pictures=True, launchers=args.serve_launchers,
-                     heavy_levels=heavy, heavy_launchers=args.serve_heavy_launchers)
+                     heavy_levels=heavy, heavy_launchers=args.serve_heavy_launchers,
+                     heavy_batch=args.serve_heavy_batch)
     streams = []  # (stream id, class)
     for ci, (_name, n, w, h, bd, _cw, _g) in enumerate(SERVE_CLASSES):
         for _ in range(n):
@@ -504,6 +505,7 @@ def main():
     ap.add_argument("--serve-batch", type=int, default=32)
     ap.add_argument("--serve-launchers", type=int, default=2)
     ap.add_argument("--serve-heavy-launchers", type=int, default=1)
+    ap.add_argument("--serve-heavy-batch", type=int, default=1)
     ap.add_argument("--serve-heavy", type=int, default=-1,
                     help="dependency levels above which a frame runs on the key-frame launcher (-1: automatic, 0: off)")
     args = ap.parse_args()

@@ -560,6 +560,10 @@ int av1h_plan_sb_features(const void* plan, float* features);
 /* Dependency levels of the plan's queue (its critical path in blocks; 0
  * when nothing is queued): what bounds the frame's queue time. */
 int av1h_plan_levels(const void* plan);
+/* counts[0..3]: queued blocks; blocks that read their edges from packed
+ * words (one load per word is both wait and read); blocks that publish
+ * packed edge words; blocks whose cells no other block waits on. */
+int av1h_plan_queue_counts(const void* plan, int32_t* counts);
 /* After a decode of `plan` alone with the queue trace on: the queue work
  * (sample time after the wait, per block, summed) of every superblock in
  * microseconds. Returns the superblock count. */
@@ -618,9 +622,12 @@ typedef struct av1h_server_config {
   int32_t launchers;        /* batches in flight at once (0: 2)                */
   int32_t heavy_levels;     /* > 0 and launchers > 1: frames whose queue has
                                more dependency levels than this (key frames)
-                               run one per batch on the first heavy_launchers
-                               launchers, the others on the rest            */
+                               run on the first heavy_launchers launchers,
+                               the others on the rest                       */
   int32_t heavy_launchers;  /* launchers for heavy frames (0: 1)              */
+  int32_t heavy_batch;      /* heavy frames per batch (0: 1); their queues
+                               merge, so a batch of them takes about as long
+                               as its longest critical path                 */
   int64_t hbm_budget;       /* bytes of admitted stream state                  */
   size_t nv12_pitch;        /* > 0: NV12 packed on the device with each frame   */
 } av1h_server_config;
@@ -663,6 +670,10 @@ int av1h_server_close_stream(av1h_server* server, int stream);
 int av1h_server_poll(av1h_server* server, av1h_completion* out, int max, int wait_ms);
 void av1h_server_free_picture(uint16_t* picture);
 int av1h_server_get_stats(av1h_server* server, av1h_server_stats* out);
+/* Launcher context `index` (0 .. launchers-1, the heavy-frame launchers
+ * first), owned by the server: its launch shapes can be set with
+ * av1h_tune_set before frames are submitted. NULL for a bad index. */
+av1h_ctx* av1h_server_launcher(av1h_server* server, int index);
 const char* av1h_server_last_error(av1h_server* server);
 
 #ifdef __cplusplus
