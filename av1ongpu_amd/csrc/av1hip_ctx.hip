@@ -222,6 +222,56 @@ int col_kind(int t) { return (0x99895144u >> (2 * t)) & 3; }
 
 const int kModeAngle[13] = {0, 90, 180, 45, 135, 113, 157, 203, 67, 0, 0, 0, 0};
 
+/* intra edge filter strength and upsampling choice (av1gpu_kernels.h:
+ * 1452-1481; the specification's 7.11.2.9 and 7.11.2.10) */
+int edge_strength(int w, int h, int ft, int delta) {
+  const int d = delta < 0 ? -delta : delta, blk = w + h;
+  if (ft == 0) {
+    if (blk <= 8) return d >= 56 ? 1 : 0;
+    if (blk <= 16) return d >= 40 ? 1 : 0;
+    if (blk <= 24) return d >= 32 ? 3 : (d >= 16 ? 2 : (d >= 8 ? 1 : 0));
+    if (blk <= 32) return d >= 32 ? 3 : (d >= 4 ? 2 : 1);
+    return 3;
+  }
+  if (blk <= 8) return d >= 64 ? 2 : (d >= 40 ? 1 : 0);
+  if (blk <= 16) return d >= 48 ? 2 : (d >= 20 ? 1 : 0);
+  if (blk <= 24) return d >= 4 ? 3 : 0;
+  return 3;
+}
+int upsample_sel(int w, int h, int ft, int delta) {
+  const int d = delta < 0 ? -delta : delta;
+  if (d <= 0 || d >= 40) return 0;
+  return ft == 0 ? (w + h <= 16) : (w + h <= 8);
+}
+
+/* DevBlock::dir_a / dir_b of a directional block with prediction angle pa:
+ * the edge preparation av1gpu_kernels.h:1452-1481 performs for it (the
+ * corner filter, both edges' filter strengths and upsampling) and the
+ * derivative steps of 1245-1313; the block's flags and filter_type are set */
+void directional_params(DevBlock* d, int pa) {
+  d->dir_a = pa;
+  const int w = d->w, h = d->h, ft = d->filter_type;
+  if (ft >= 0 && pa != 90 && pa != 180) {
+    d->dir_a |= kDirFilter;
+    if (pa > 90 && pa < 180 && w + h >= 24) d->dir_a |= kDirCorner;
+    if (d->flags & kHaveAbove) d->dir_a |= edge_strength(w, h, ft, pa - 90) << kDirStrA;
+    if (d->flags & kHaveLeft) d->dir_a |= edge_strength(w, h, ft, pa - 180) << kDirStrL;
+    if (upsample_sel(w, h, ft, pa - 90)) d->dir_a |= kDirUpA;
+    if (upsample_sel(w, h, ft, pa - 180)) d->dir_a |= kDirUpL;
+  }
+  const int* dr = av1t_spec_table + AV1T_OFF_DR_DERIV;
+  int dx = 0, dy = 0;
+  if (pa < 90)
+    dx = dr[pa];
+  else if (pa > 90 && pa < 180)
+    dx = dr[180 - pa];
+  if (pa > 90 && pa < 180)
+    dy = dr[pa - 90];
+  else if (pa > 180)
+    dy = dr[270 - pa];
+  d->dir_b = dx | (dy << 16);
+}
+
 /* scaling_lookup_initialization (reference av1gpu_kernels.h:3157-3188) */
 void scaling_lut(const av1h_grain& p, int plane, int* lut) {
   const int *val, *scl;
@@ -652,6 +702,7 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
         const int pa = kModeAngle[b.mode] + 3 * b.angle_delta;
         if (pa < 90 && ar) ax1 = b.x + 2 * b.w - 1;
         if (pa > 180 && bl) ly1 = b.y + 2 * b.h - 1;
+        directional_params(&d, pa);
       }
       d.dep_x1 = ax1;
       d.dep_y1 = ly1;

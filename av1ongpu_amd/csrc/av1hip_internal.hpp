@@ -83,7 +83,14 @@ struct DevBlock {
    * the corner when dep_corner; dep_x1 < x and dep_y1 < y disable a side */
   int32_t dep_x1, dep_y1, dep_corner;
   int32_t dep_rx0, dep_ry0, dep_rx1, dep_ry1; /* intra block copy: source cells */
+  /* directional intra: dir_a the prediction angle (bits 0-8) and the edge
+   * preparation (kDir*), dir_b the two Dr_Intra_Derivative steps (dx in
+   * bits 0-15, dy in 16-31), all fixed by the syntax */
+  int32_t dir_a, dir_b;
 };
+
+enum { kDirFilter = 1 << 9, kDirCorner = 1 << 10, kDirUpA = 1 << 11, kDirUpL = 1 << 12 };
+constexpr int kDirStrA = 13, kDirStrL = 16; /* 3 bit edge filter strengths */
 
 /* flag bits of DevTx::flags */
 enum { kTxLossless = 1, kTxRaw = 2, kTxToResidual = 4 };

@@ -588,43 +588,6 @@ __device__ __forceinline__ int inter_sample(RArgs a, BlkRef b, int xs, int ys) {
 constexpr int kEdge = 160;   /* edge entries (w + h <= 128, upsampled small) */
 constexpr int kEdgeOff = 8;
 
-__device__ __forceinline__ int edge_strength(int w, int h, int ft, int delta) {
-  const int d = delta < 0 ? -delta : delta, blk = w + h;
-  if (ft == 0) {
-    if (blk <= 8) return d >= 56 ? 1 : 0;
-    if (blk <= 16) return d >= 40 ? 1 : 0;
-    if (blk <= 24) return d >= 32 ? 3 : (d >= 16 ? 2 : (d >= 8 ? 1 : 0));
-    if (blk <= 32) return d >= 32 ? 3 : (d >= 4 ? 2 : 1);
-    return 3;
-  }
-  if (blk <= 8) return d >= 64 ? 2 : (d >= 40 ? 1 : 0);
-  if (blk <= 16) return d >= 48 ? 2 : (d >= 20 ? 1 : 0);
-  if (blk <= 24) return d >= 4 ? 3 : 0;
-  return 3;
-}
-__device__ __forceinline__ int upsample_sel(int w, int h, int ft, int delta) {
-  const int d = delta < 0 ? -delta : delta;
-  if (d <= 0 || d >= 40) return 0;
-  return ft == 0 ? (w + h <= 16) : (w + h <= 8);
-}
-
-/* 5 tap edge filter over `sz` entries (av1gpu_kernels.h:1157-1173); `buf` is
- * block relative (entry i - 1 is edge sample i) */
-__device__ void edge_filter(int* buf, int* tmp, int sz, int strength) {
-  const int lane = threadIdx.x;
-  if (strength <= 0) return;
-  for (int i = lane; i < sz; i += 64) tmp[i] = buf[i - 1];
-  wsync();
-  const int* k = SPEC(AV1T_OFF_EDGE_KERNEL) + 5 * (strength - 1);
-  for (int i = 1 + lane; i < sz; i += 64) {
-    int s = 0;
-#pragma unroll
-    for (int t = 0; t < 5; ++t) s += k[t] * tmp[clampi(i - 2 + t, 0, sz - 1)];
-    buf[i - 1] = (s + 8) >> 4;
-  }
-  wsync();
-}
-
 /* The tables intra prediction indexes per lane (smooth weights, edge
  * kernels, filter intra taps), staged in LDS by the queue kernel: one
  * contiguous range of the specification table. */
@@ -633,99 +596,122 @@ static_assert(AV1T_OFF_EDGE_KERNEL > kTabLo && AV1T_OFF_FILTER_INTRA > kTabLo &&
               "staged table range");
 #define TAB(off) (tab + ((off) - kTabLo))
 
-/* The edge filter of both edges at once: the above edge on entries
- * [0, sza), the left edge on [0, szl), one copy phase and one filter phase
- * for the two (av1gpu_kernels.h:1157-1173 on each). */
-__device__ void edge_filter2(int* above, int* left, int* tmp, int sza, int stra, int szl, int strl, const int* tab) {
-  const int lane = threadIdx.x;
-  if (stra <= 0) sza = 0;
-  if (strl <= 0) szl = 0;
-  if (sza + szl == 0) return;
-  int* ta = tmp;
-  int* tl = tmp + sza;
-  for (int i = lane; i < sza + szl; i += 64) {
-    if (i < sza)
-      ta[i] = above[i - 1];
-    else
-      tl[i - sza] = left[i - sza - 1];
-  }
-  wsync();
-  const int* ka = TAB(AV1T_OFF_EDGE_KERNEL) + 5 * (stra > 0 ? stra - 1 : 0);
-  const int* kl = TAB(AV1T_OFF_EDGE_KERNEL) + 5 * (strl > 0 ? strl - 1 : 0);
-  for (int j = lane; j < sza + szl; j += 64) {
-    const bool is_a = j < sza;
-    const int i = is_a ? j : j - sza, sz = is_a ? sza : szl;
-    if (i == 0) continue;
-    const int* t = is_a ? ta : tl;
-    int s = 0;
+/* The edge preparation of a directional block (av1gpu_kernels.h:1452-1481,
+ * with the filter of 1157-1173 and the upsampling of 1177-1191), its
+ * choices precomputed on the host (DevBlock::dir_a): the corner filter, the
+ * 5 tap filter of each edge into fa / fl and the 2x upsampling into ga / gl,
+ * out of place so that each step is one LDS pass. `above` / `left` hold
+ * entries -1 (the corner) .. w + h - 1; *ea / *el get the edges to predict
+ * from. */
+constexpr int kPrep1 = 2 * (kEdge + 2 * kEdgeOff), kPrep2 = 4 * (kEdge + 2 * kEdgeOff);
+static_assert(kPrep2 + 2 * (kEdge + 2 * kEdgeOff) <= kWork, "edge preparation area");
+
+__device__ void prepare_edges(BlkRef b, const int* above, const int* left, int* work, const int* tab, int maxv,
+                              int max_x, int max_y, const int** ea, const int** el) {
+  const int lane = threadIdx.x, w = b.w, h = b.h, n = w + h;
+  const int da = b.dir_a, pa = da & 511;
+  const int stra = (da >> kDirStrA) & 7, strl = (da >> kDirStrL) & 7;
+  int corner = above[-1];
+  if (da & kDirCorner) corner = rnd2(left[0] * 5 + above[-1] * 6 + above[0] * 5, 4);
+  const int sza = stra ? min(w, max_x - b.x + 1) + (pa < 90 ? h : 0) + 1 : 0;
+  const int szl = strl ? min(h, max_y - b.y + 1) + (pa > 180 ? w : 0) + 1 : 0;
+  int* fa = work + kPrep1 + kEdgeOff;
+  int* fl = fa + kEdge + 2 * kEdgeOff;
+  const int* ka = TAB(AV1T_OFF_EDGE_KERNEL) + 5 * max(stra - 1, 0);
+  const int* kl = TAB(AV1T_OFF_EDGE_KERNEL) + 5 * max(strl - 1, 0);
+  /* entry k of the filtered edge: the filter over entries clamp(k + q - 2,
+   * -1, sz - 2) for 0 <= k <= sz - 2, the entry itself elsewhere */
+  for (int j = lane - 1; j < 2 * n + 1; j += 64) {
+    const bool is_a = j < n;
+    const int k = is_a ? j : j - n - 1;
+    const int* src = is_a ? above : left;
+    const int sz = is_a ? sza : szl;
+    int v;
+    if (k >= 0 && k <= sz - 2) {
+      const int* kk = is_a ? ka : kl;
+      int acc = 0;
 #pragma unroll
-    for (int q = 0; q < 5; ++q) s += (is_a ? ka[q] : kl[q]) * t[clampi(i - 2 + q, 0, sz - 1)];
-    (is_a ? above : left)[i - 1] = (s + 8) >> 4;
-  }
-  wsync();
-}
-
-/* 2x upsampling of both edges at once (av1gpu_kernels.h:1177-1191 on each;
- * n 0 skips an edge) */
-__device__ void edge_upsample2(int* above, int* left, int* tmp, int na, int nl, int maxv) {
-  const int lane = threadIdx.x;
-  if (na + nl == 0) return;
-  int* ta = tmp;           /* na + 3 entries */
-  int* tl = tmp + na + 3;  /* nl + 3 entries */
-  for (int j = lane; j < na + nl + 2; j += 64) {
-    if (j < na + 1)
-      ta[j + 1] = above[j - 1];
-    else
-      tl[j - na - 1 + 1] = left[j - na - 2];
-  }
-  if (lane == 0) {
-    if (na) {
-      ta[0] = above[-1];
-      ta[na + 2] = above[na - 1];
+      for (int q = 0; q < 5; ++q) {
+        const int m = clampi(k + q - 2, -1, sz - 2);
+        acc += kk[q] * (m < 0 ? corner : src[m]);
+      }
+      v = (acc + 8) >> 4;
+    } else {
+      v = k < 0 ? corner : src[k];
     }
-    if (nl) {
-      tl[0] = left[-1];
-      tl[nl + 2] = left[nl - 1];
+    (is_a ? fa : fl)[k] = v;
+  }
+  wsync();
+  *ea = fa;
+  *el = fl;
+  const int ua = (da & kDirUpA) != 0, ul = (da & kDirUpL) != 0;
+  if (!ua && !ul) return;
+  /* upsampled entry k: -2 the first tap, odd k = 2i - 1 the 4 tap
+   * interpolation, even k = 2i the sample, past 2m - 2 (up to w + h - 1)
+   * the entry as it was */
+  const int na = ua ? w + (pa < 90 ? h : 0) : 0, nl = ul ? h + (pa > 180 ? w : 0) : 0;
+  int* ga = work + kPrep2 + kEdgeOff;
+  int* gl = ga + kEdge + 2 * kEdgeOff;
+  const int span = 2 * n + 1; /* entries -2 .. 2n - 2 (upsampling needs w + h <= 16) */
+  for (int j = lane; j < 2 * span; j += 64) {
+    const bool is_a = j < span;
+    const int m = is_a ? na : nl;
+    const int k = (is_a ? j : j - span) - 2;
+    if (m == 0 || (k > 2 * m - 2 && k > n - 1)) continue;
+    const int* f = is_a ? fa : fl;
+    auto t = [&](int i) { return f[clampi(i - 2, -1, m - 1)]; };
+    int v;
+    if (k == -2) {
+      v = t(0);
+    } else if (k <= 2 * m - 2) {
+      if (k & 1) {
+        const int i = (k + 1) >> 1;
+        v = clampi(rnd2(-t(i) + 9 * t(i + 1) + 9 * t(i + 2) - t(i + 3), 4), 0, maxv);
+      } else {
+        v = t((k >> 1) + 2);
+      }
+    } else {
+      v = f[k];
     }
+    (is_a ? ga : gl)[k] = v;
   }
   wsync();
-  if (lane == 0) {
-    if (na) above[-2] = ta[0];
-    if (nl) left[-2] = tl[0];
-  }
-  for (int j = lane; j < na + nl; j += 64) {
-    const bool is_a = j < na;
-    const int i = is_a ? j : j - na;
-    const int* t = is_a ? ta : tl;
-    int* buf = is_a ? above : left;
-    const int s = -t[i] + 9 * t[i + 1] + 9 * t[i + 2] - t[i + 3];
-    buf[2 * i - 1] = clampi(rnd2(s, 4), 0, maxv);
-    buf[2 * i] = t[i + 2];
-  }
-  wsync();
-}
-
-/* 2x upsampling (av1gpu_kernels.h:1177-1191) */
-__device__ void edge_upsample(int* buf, int* tmp, int n, int maxv) {
-  const int lane = threadIdx.x;
-  for (int i = lane - 1; i < n; i += 64) tmp[i + 2] = buf[i];
-  if (lane == 0) {
-    tmp[0] = buf[-1];
-    tmp[n + 2] = buf[n - 1];
-  }
-  wsync();
-  if (lane == 0) buf[-2] = tmp[0];
-  for (int i = lane; i < n; i += 64) {
-    const int s = -tmp[i] + 9 * tmp[i + 1] + 9 * tmp[i + 2] - tmp[i + 3];
-    buf[2 * i - 1] = clampi(rnd2(s, 4), 0, maxv);
-    buf[2 * i] = tmp[i + 2];
-  }
-  wsync();
+  if (ua) *ea = ga;
+  if (ul) *el = gl;
 }
 
 /* The whole intra prediction of one block into s_pred (av1gpu_kernels.h:
  * 1578-1677 with the edges of 1108-1151 and the directional preparation of
  * 1452-1481). */
+/* Polls a packed word until it carries `epoch`, with kPollDepth loads in
+ * flight spaced about a round trip / kPollDepth apart (loads return in
+ * order), so a word that lands is seen that much sooner than by one load
+ * at a time. */
+constexpr int kPollDepth = 1;
+__device__ __forceinline__ uint64_t ld_word(const uint64_t* q) {
+  return __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ uint64_t poll_word(const uint64_t* q, uint32_t epoch, uint32_t* error) {
+  uint64_t v[kPollDepth];
+#pragma unroll
+  for (int k = 0; k < kPollDepth; ++k) {
+    if (k) __builtin_amdgcn_s_sleep(8);
+    v[k] = ld_word(q);
+  }
+  for (int iter = 0;; iter += kPollDepth) {
+#pragma unroll
+    for (int k = 0; k < kPollDepth; ++k) {
+      if ((uint32_t)(v[k] >> 32) == epoch) return v[k];
+      if (kPollDepth == 1) __builtin_amdgcn_s_sleep(1);
+      v[k] = ld_word(q);
+    }
+    if (iter >= kSpinLimit) {
+      atomicOr(error, 1u);
+      return 0;
+    }
+  }
+}
+
 /* The edges of a block flagged kPackedEdges: its above row (x .. ae), left
  * column (y .. le) and corner come from the packed words its queued
  * neighbours published, each 64-bit word two samples and the epoch of the
@@ -753,16 +739,7 @@ __device__ void gather_packed(RArgs a, BlkRef b, int* above, int* left, int alas
       q = colp + wl0 + (i - na);
     else
       q = (b.flags & kCornerRow) ? rowp + wa0 - 1 : colp + wl0 - 1;
-    uint64_t v;
-    for (int iter = 0;; ++iter) {
-      v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if ((uint32_t)(v >> 32) == a.epoch) break;
-      if (iter >= kSpinLimit) {
-        atomicOr(error, 1u);
-        break;
-      }
-      __builtin_amdgcn_s_sleep(1);
-    }
+    const uint64_t v = poll_word(q, a.epoch, error);
     const int s0 = (int)(v & 0xffff), s1 = (int)((v >> 16) & 0xffff);
     if (i < na) {
       const int j = 2 * (wa0 + i) - x;
@@ -942,36 +919,13 @@ __device__ void predict_intra(RArgs a, BlkRef b, int* work, uint16_t* pred, cons
     return;
   }
   if (mode >= 1 && mode <= 8) {
-    const int pa = SPEC(AV1T_OFF_MODE_TO_ANGLE)[mode] + 3 * b.angle;
-    int ua = 0, ul = 0;
-    if (b.filter_type >= 0 && pa != 90 && pa != 180) {
-      if (pa > 90 && pa < 180 && w + h >= 24) {
-        if (lane == 0) {
-          const int c = rnd2(left[0] * 5 + above[-1] * 6 + above[0] * 5, 4);
-          above[-1] = c;
-          left[-1] = c;
-        }
-        wsync();
-      }
-      edge_filter2(above, left, tmp, min(w, max_x - b.x + 1) + (pa < 90 ? h : 0) + 1,
-                   have_above ? edge_strength(w, h, b.filter_type, pa - 90) : 0,
-                   min(h, max_y - b.y + 1) + (pa > 180 ? w : 0) + 1,
-                   have_left ? edge_strength(w, h, b.filter_type, pa - 180) : 0, tab);
-      ua = upsample_sel(w, h, b.filter_type, pa - 90);
-      ul = upsample_sel(w, h, b.filter_type, pa - 180);
-      edge_upsample2(above, left, tmp, ua ? w + (pa < 90 ? h : 0) : 0, ul ? h + (pa > 180 ? w : 0) : 0, maxv);
-    }
+    const int da = b.dir_a, pa = da & 511;
+    const int ua = (da & kDirUpA) ? 1 : 0, ul = (da & kDirUpL) ? 1 : 0;
+    const int* ea = above;
+    const int* el = left;
+    if (da & kDirFilter) prepare_edges(b, above, left, work, tab, maxv, max_x, max_y, &ea, &el);
     /* av1gpu_kernels.h:1245-1313 */
-    const int* dr = SPEC(AV1T_OFF_DR_DERIV);
-    int dx = 0, dy = 0;
-    if (pa < 90)
-      dx = dr[pa];
-    else if (pa > 90 && pa < 180)
-      dx = dr[180 - pa];
-    if (pa > 90 && pa < 180)
-      dy = dr[pa - 90];
-    else if (pa > 180)
-      dy = dr[270 - pa];
+    const int dx = b.dir_b & 0xffff, dy = (int)((uint32_t)b.dir_b >> 16);
     for (int s = lane; s < w * h; s += 64) {
       const int i = s >> lw, c = s & (w - 1);
       int v;
@@ -980,26 +934,26 @@ __device__ void predict_intra(RArgs a, BlkRef b, int* work, uint16_t* pred, cons
         const int idx = (i + 1) * dx;
         const int base = (idx >> (6 - ua)) + (c << ua);
         const int sh = ((idx << ua) >> 1) & 31;
-        v = base < maxb ? rnd2(above[base] * (32 - sh) + above[base + 1] * sh, 5) : above[maxb];
+        v = base < maxb ? rnd2(ea[base] * (32 - sh) + ea[base + 1] * sh, 5) : ea[maxb];
       } else if (pa > 90 && pa < 180) {
         int idx = (c << 6) - (i + 1) * dx;
         int base = idx >> (6 - ua);
         if (base >= -(1 << ua)) {
           const int sh = ((idx * (1 << ua)) >> 1) & 31;
-          v = rnd2(above[base] * (32 - sh) + above[base + 1] * sh, 5);
+          v = rnd2(ea[base] * (32 - sh) + ea[base + 1] * sh, 5);
         } else {
           idx = (i << 6) - (c + 1) * dy;
           base = idx >> (6 - ul);
           const int sh = ((idx * (1 << ul)) >> 1) & 31;
-          v = rnd2(left[base] * (32 - sh) + left[base + 1] * sh, 5);
+          v = rnd2(el[base] * (32 - sh) + el[base + 1] * sh, 5);
         }
       } else if (pa > 180) {
         const int idx = (c + 1) * dy;
         const int base = (idx >> (6 - ul)) + (i << ul);
         const int sh = ((idx << ul) >> 1) & 31;
-        v = rnd2(left[base] * (32 - sh) + left[base + 1] * sh, 5);
+        v = rnd2(el[base] * (32 - sh) + el[base + 1] * sh, 5);
       } else {
-        v = pa == 90 ? above[c] : left[i];
+        v = pa == 90 ? ea[c] : el[i];
       }
       pred[s] = (uint16_t)clampi(v, 0, maxv);
     }
@@ -1142,7 +1096,7 @@ __device__ void wait_cells(RArgs a, BlkRef b, uint32_t* error) {
 
 /* A block of the queue: the prediction, then the residuals of its transforms, which
  * k_itx computed before the queue started (av1gpu_kernels.h:933-937 for the
- * add), then one store. */
+ * add), into `pred`; the caller stores it. */
 /* Residual samples of a block read before its wait (they do not depend on
  * the neighbours): one transform covering the block, at most 256 samples,
  * sample lane + 64 k in rv[k]. */
@@ -1186,7 +1140,6 @@ __device__ void reconstruct_queued(RArgs a, BlkRef b, int* work, uint16_t* pred,
       if (s < b.w * b.h) pred[s] = (uint16_t)clampi((int)pred[s] + er.v[k], 0, a.maxv);
     }
     wsync();
-    store_block(a, b, pred);
     return;
   }
   for (int t = 0; t < b.tx_count; ++t) {
@@ -1200,7 +1153,6 @@ __device__ void reconstruct_queued(RArgs a, BlkRef b, int* work, uint16_t* pred,
     }
   }
   wsync();
-  store_block(a, b, pred);
 }
 
 /* Motion compensation, one 256 thread workgroup per task (at most 512
@@ -1417,7 +1369,9 @@ __global__ __launch_bounds__(64) void k_queue(BatchDev B) {
     if (!(b.flags & kPackedEdges)) wait_cells(a, b, B.error);
     const uint64_t t1 = B.trace ? (uint64_t)wall_clock64() : 0;
     reconstruct_queued(a, b, s_work, s_pred, er, s_tab, B.error);
+    /* the packed edges first: the blocks waiting on this one read only them */
     if (b.flags & (kEdgeRow | kEdgeCol)) write_edges(a, b, s_pred);
+    store_block(a, b, s_pred);
     if (!(b.flags & kSkipMark)) {
       /* every sample store of the block has completed before its cells flip */
       __builtin_amdgcn_s_waitcnt(0);
