@@ -539,14 +539,19 @@ int resolve_slot(const av1h_frame_header& h, int ref) {
   return h.reference_slots[ref];
 }
 
-/* Whether intra block d can take its edges from packed words: every sample
- * of its above row (x .. the last it depends on) is the last row of a queued
- * block, every sample of its left column the last column of one, and its
- * corner either. Then the owners are flagged to publish those rows and
- * columns, and d its corner's side. `owner` maps the plane's cells to the
- * queued blocks decoded before d (index into queue_list), -1 elsewhere. */
-bool packed_edges(const Layout& L, DevBlock& d, const std::vector<int32_t>& owner, std::vector<DevBlock>& blocks,
-                  const std::vector<int32_t>& queue_list) {
+/* Whether intra block d can take its edges without a cell wait: every
+ * sample of its above row (x .. the last it depends on) is either the last
+ * row of a queued block decoded before it (read as a packed word) or
+ * belongs to a block outside the queue (final before the queue starts,
+ * read from the frame); the same for its left column with last columns,
+ * and its corner either way. Then the queued owners are flagged to publish
+ * those rows and columns, and d gets its per-cell sources (edge_mask_a /
+ * edge_mask_l, bit i for the i-th cell of the span: 1 packed, 0 frame) and
+ * its corner's. `owner` maps the plane's cells to the queued blocks decoded
+ * before d (index into queue_list), -1 elsewhere; `queued` flags the cells
+ * of every queued block of the plane. */
+bool packed_edges(const Layout& L, DevBlock& d, const std::vector<int32_t>& owner, const std::vector<char>& queued,
+                  std::vector<DevBlock>& blocks, const std::vector<int32_t>& queue_list) {
   if (d.kind != kIntra || (d.flags & (kPalette | kIntraBC)) || (d.mode == AV1H_PRED_CFL && d.plane > 0)) return false;
   const int p = d.plane, sh = L.cell_shift[p], cell = 1 << sh;
   const bool ha = (d.flags & kHaveAbove) != 0, hl = (d.flags & kHaveLeft) != 0;
@@ -554,35 +559,52 @@ bool packed_edges(const Layout& L, DevBlock& d, const std::vector<int32_t>& owne
   const int llast = std::min(L.grid_h[p] - 1, d.y + ((d.flags & kHaveBL) ? 2 * d.h : d.h) - 1);
   const int ae = std::min(d.dep_x1, alast), le = std::min(d.dep_y1, llast);
   if ((ha && ae < d.x) || (hl && le < d.y)) return false;
-  auto own = [&](int px, int py) -> DevBlock* {
-    const int o = owner[(size_t)(py >> sh) * L.cells_w[p] + (px >> sh)];
-    return o < 0 ? nullptr : &blocks[(size_t)queue_list[(size_t)o]];
+  if ((ha && ((ae - d.x) >> sh) >= 32) || (hl && ((le - d.y) >> sh) >= 32)) return false;
+  /* 1: a queued block decoded before d, 0: outside the queue, -1: neither */
+  auto src = [&](int px, int py, DevBlock** o) -> int {
+    const size_t c = (size_t)(py >> sh) * L.cells_w[p] + (px >> sh);
+    *o = nullptr;
+    if (owner[c] >= 0) {
+      *o = &blocks[(size_t)queue_list[(size_t)owner[c]]];
+      return 1;
+    }
+    return queued[c] ? -1 : 0;
   };
+  uint32_t ma = 0, ml = 0;
+  DevBlock* o;
   if (ha)
     for (int px = d.x; px <= ae; px += cell) {
-      const DevBlock* o = own(px, d.y - 1);
-      if (!o || o->y + o->h != d.y) return false;
+      const int k = src(px, d.y - 1, &o);
+      if (k < 0 || (k && o->y + o->h != d.y)) return false;
+      if (k) ma |= 1u << ((px - d.x) >> sh);
     }
   if (hl)
     for (int py = d.y; py <= le; py += cell) {
-      const DevBlock* o = own(d.x - 1, py);
-      if (!o || o->x + o->w != d.x) return false;
+      const int k = src(d.x - 1, py, &o);
+      if (k < 0 || (k && o->x + o->w != d.x)) return false;
+      if (k) ml |= 1u << ((py - d.y) >> sh);
     }
-  int corner = 0; /* 1: row, 2: column */
+  int corner = 0; /* 1: packed row, 2: packed column, 3: frame */
   if (ha && hl) {
-    const DevBlock* o = own(d.x - 1, d.y - 1);
-    if (!o) return false;
-    corner = o->y + o->h == d.y ? 1 : (o->x + o->w == d.x ? 2 : 0);
+    const int k = src(d.x - 1, d.y - 1, &o);
+    if (k < 0) return false;
+    corner = !k ? 3 : o->y + o->h == d.y ? 1 : (o->x + o->w == d.x ? 2 : 0);
     if (!corner) return false;
   }
   if (ha)
-    for (int px = d.x; px <= ae; px += cell) own(px, d.y - 1)->flags |= kEdgeRow;
+    for (int px = d.x; px <= ae; px += cell)
+      if (src(px, d.y - 1, &o) > 0) o->flags |= kEdgeRow;
   if (hl)
-    for (int py = d.y; py <= le; py += cell) own(d.x - 1, py)->flags |= kEdgeCol;
-  if (corner) {
-    own(d.x - 1, d.y - 1)->flags |= corner == 1 ? kEdgeRow : kEdgeCol;
-    if (corner == 1) d.flags |= kCornerRow;
+    for (int py = d.y; py <= le; py += cell)
+      if (src(d.x - 1, py, &o) > 0) o->flags |= kEdgeCol;
+  if (corner == 1 || corner == 2) {
+    src(d.x - 1, d.y - 1, &o);
+    o->flags |= corner == 1 ? kEdgeRow : kEdgeCol;
   }
+  if (corner == 1) d.flags |= kCornerRow;
+  if (corner == 3) d.flags |= kCornerFrame;
+  d.edge_mask_a = (int32_t)ma;
+  d.edge_mask_l = (int32_t)ml;
   return true;
 }
 
@@ -901,6 +923,15 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
       if ((d.x | d.y | d.w | d.h) & ((1 << L.cell_shift[d.plane]) - 1)) packed = false;
     }
     std::vector<char> need_mark((size_t)nq, 0);
+    std::vector<char> queued[3];
+    for (int p = 0; p < 3; ++p) queued[p].assign((size_t)L.cells_w[p] * L.cells_h[p], 0);
+    for (int qi = 0; qi < nq; ++qi) {
+      const DevBlock& d = H->blocks[(size_t)H->queue_list[(size_t)qi]];
+      const int p = d.plane, sh = L.cell_shift[p];
+      for (int cy = d.y >> sh; cy < std::min((d.y + d.h + (1 << sh) - 1) >> sh, L.cells_h[p]); ++cy)
+        for (int cx = d.x >> sh; cx < std::min((d.x + d.w + (1 << sh) - 1) >> sh, L.cells_w[p]); ++cx)
+          queued[p][(size_t)cy * L.cells_w[p] + cx] = 1;
+    }
     for (int qi = 0; qi < nq; ++qi) {
       DevBlock& d = H->blocks[(size_t)H->queue_list[(size_t)qi]];
       const int p = d.plane, sh = L.cell_shift[p], cell = 1 << sh;
@@ -930,7 +961,7 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
           for (int cx = d.dep_rx0; cx <= d.dep_rx1; ++cx) dep(p, cx, cy);
       std::sort(deps.begin(), deps.end());
       deps.erase(std::unique(deps.begin(), deps.end()), deps.end());
-      if (packed && packed_edges(L, d, owner[p], H->blocks, H->queue_list)) {
+      if (packed && packed_edges(L, d, owner[p], queued[p], H->blocks, H->queue_list)) {
         d.flags |= kPackedEdges;
       } else {
         for (int32_t o : deps) need_mark[(size_t)o] = 1;

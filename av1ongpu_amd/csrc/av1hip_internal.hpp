@@ -55,11 +55,13 @@ enum {
   kIIWedge = 512, kWarp0 = 1024, kWarp1 = 2048, kMaskWarp0 = 4096, kMaskWarp1 = 8192,
   kIntraBC = 16384,
   /* queue protocol: kPackedEdges - every edge sample the block reads comes
-   * from a queued block's packed edge words (no cell wait, no frame read);
-   * kCornerRow - its corner sample is in the packed row, else the column;
+   * from a queued block's packed edge words or from the frame where no
+   * queued block wrote it (no cell wait); kCornerRow / kCornerFrame - its
+   * corner sample is in the packed row / the frame, else the packed column;
    * kEdgeRow / kEdgeCol - the block publishes its last row / column as
    * packed words; kSkipMark - no block waits on its cells */
-  kPackedEdges = 1 << 15, kCornerRow = 1 << 16, kEdgeRow = 1 << 17, kEdgeCol = 1 << 18, kSkipMark = 1 << 19
+  kPackedEdges = 1 << 15, kCornerRow = 1 << 16, kEdgeRow = 1 << 17, kEdgeCol = 1 << 18, kSkipMark = 1 << 19,
+  kCornerFrame = 1 << 20
 };
 
 struct DevBlock {
@@ -87,6 +89,9 @@ struct DevBlock {
    * preparation (kDir*), dir_b the two Dr_Intra_Derivative steps (dx in
    * bits 0-15, dy in 16-31), all fixed by the syntax */
   int32_t dir_a, dir_b;
+  /* kPackedEdges: the source of each cell of the above / left span (bit i:
+   * 1 a packed word of a queued block, 0 the frame, final before the queue) */
+  int32_t edge_mask_a, edge_mask_l;
 };
 
 enum { kDirFilter = 1 << 9, kDirCorner = 1 << 10, kDirUpA = 1 << 11, kDirUpL = 1 << 12 };

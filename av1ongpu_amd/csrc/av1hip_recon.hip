@@ -716,7 +716,8 @@ __device__ uint64_t poll_word(const uint64_t* q, uint32_t epoch, uint32_t* error
  * column (y .. le) and corner come from the packed words its queued
  * neighbours published, each 64-bit word two samples and the epoch of the
  * frame that wrote them, so the one load per word is both the dependency
- * wait and the read (no cell flag, no second round trip to the frame).
+ * wait and the read (no cell flag, no second round trip to the frame), and
+ * where the neighbour is outside the queue, from the frame without a wait.
  * Entries past ae / le repeat the last sample: only modes that read no
  * further than their dependencies reach use them. */
 __device__ void gather_packed(RArgs a, BlkRef b, int* above, int* left, int alast, int llast, int mid,
@@ -731,15 +732,28 @@ __device__ void gather_packed(RArgs a, BlkRef b, int* above, int* left, int alas
   const int nc = ha && hl ? 1 : 0;
   const uint64_t* rowp = a.edges + L.edge_row_off[p] + (int64_t)((y >> sh) - 1) * L.edge_row_words[p];
   const uint64_t* colp = a.edges + L.edge_col_off[p] + (int64_t)((x >> sh) - 1) * L.edge_col_words[p];
+  /* samples of blocks outside the queue: final before it started */
+  const int stride = L.stride[p];
+  const uint16_t* fr = a.frame + L.plane_off[p] + (int64_t)y * stride + x;
+  const uint32_t ma = (uint32_t)b.edge_mask_a, ml = (uint32_t)b.edge_mask_l;
   for (int i = lane; i < na + nl + nc; i += 64) {
-    const uint64_t* q;
-    if (i < na)
-      q = rowp + wa0 + i;
-    else if (i < na + nl)
-      q = colp + wl0 + (i - na);
-    else
-      q = (b.flags & kCornerRow) ? rowp + wa0 - 1 : colp + wl0 - 1;
-    const uint64_t v = poll_word(q, a.epoch, error);
+    uint64_t v;
+    if (i < na) {
+      if ((ma >> ((2 * i) >> sh)) & 1)
+        v = poll_word(rowp + wa0 + i, a.epoch, error);
+      else
+        v = *reinterpret_cast<const uint32_t*>(fr - stride + 2 * i);
+    } else if (i < na + nl) {
+      const int j = i - na;
+      if ((ml >> ((2 * j) >> sh)) & 1)
+        v = poll_word(colp + wl0 + j, a.epoch, error);
+      else
+        v = (uint64_t)fr[(int64_t)(2 * j) * stride - 1] | ((uint64_t)fr[(int64_t)(2 * j + 1) * stride - 1] << 16);
+    } else if (b.flags & kCornerFrame) {
+      v = (uint64_t)fr[-stride - 1] << 16;
+    } else {
+      v = poll_word((b.flags & kCornerRow) ? rowp + wa0 - 1 : colp + wl0 - 1, a.epoch, error);
+    }
     const int s0 = (int)(v & 0xffff), s1 = (int)((v >> 16) & 0xffff);
     if (i < na) {
       const int j = 2 * (wa0 + i) - x;
