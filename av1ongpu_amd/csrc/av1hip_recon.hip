@@ -724,68 +724,55 @@ __device__ void gather_packed(RArgs a, BlkRef b, int* above, int* left, int alas
                               uint32_t* error) {
   LayRef L = a.lay;
   const int p = b.plane, sh = L.cell_shift[p];
-  const int x = b.x, y = b.y, lane = threadIdx.x;
+  const int x = b.x, y = b.y, n = b.w + b.h;
   const int ha = (b.flags & kHaveAbove) != 0, hl = (b.flags & kHaveLeft) != 0;
-  const int ae = min(b.dep_x1, alast), le = min(b.dep_y1, llast);
-  const int wa0 = x >> 1, na = ha ? (ae >> 1) - wa0 + 1 : 0;
-  const int wl0 = y >> 1, nl = hl ? (le >> 1) - wl0 + 1 : 0;
-  const int nc = ha && hl ? 1 : 0;
+  const int ae = min(b.dep_x1, alast) - x, le = min(b.dep_y1, llast) - y;
   const uint64_t* rowp = a.edges + L.edge_row_off[p] + (int64_t)((y >> sh) - 1) * L.edge_row_words[p];
   const uint64_t* colp = a.edges + L.edge_col_off[p] + (int64_t)((x >> sh) - 1) * L.edge_col_words[p];
   /* samples of blocks outside the queue: final before it started */
   const int stride = L.stride[p];
   const uint16_t* fr = a.frame + L.plane_off[p] + (int64_t)y * stride + x;
   const uint32_t ma = (uint32_t)b.edge_mask_a, ml = (uint32_t)b.edge_mask_l;
-  for (int i = lane; i < na + nl + nc; i += 64) {
-    uint64_t v;
-    if (i < na) {
-      if ((ma >> ((2 * i) >> sh)) & 1)
-        v = poll_word(rowp + wa0 + i, a.epoch, error);
-      else
-        v = *reinterpret_cast<const uint32_t*>(fr - stride + 2 * i);
-    } else if (i < na + nl) {
-      const int j = i - na;
-      if ((ml >> ((2 * j) >> sh)) & 1)
-        v = poll_word(colp + wl0 + j, a.epoch, error);
-      else
-        v = (uint64_t)fr[(int64_t)(2 * j) * stride - 1] | ((uint64_t)fr[(int64_t)(2 * j + 1) * stride - 1] << 16);
-    } else if (b.flags & kCornerFrame) {
-      v = (uint64_t)fr[-stride - 1] << 16;
-    } else {
-      v = poll_word((b.flags & kCornerRow) ? rowp + wa0 - 1 : colp + wl0 - 1, a.epoch, error);
+  /* sample i of the above row / left column (block relative) */
+  auto above_at = [&](int i) -> int {
+    if ((ma >> (i >> sh)) & 1) {
+      const uint64_t v = poll_word(rowp + ((x + i) >> 1), a.epoch, error);
+      return (int)((v >> (16 * (i & 1))) & 0xffff);
     }
-    const int s0 = (int)(v & 0xffff), s1 = (int)((v >> 16) & 0xffff);
-    if (i < na) {
-      const int j = 2 * (wa0 + i) - x;
-      if (j <= ae - x) above[j] = s0;
-      if (j + 1 <= ae - x) above[j + 1] = s1;
-    } else if (i < na + nl) {
-      const int j = 2 * (wl0 + i - na) - y;
-      if (j <= le - y) left[j] = s0;
-      if (j + 1 <= le - y) left[j + 1] = s1;
-    } else {
-      above[-1] = s1; /* sample x - 1 of the row, or y - 1 of the column */
-      left[-1] = s1;
+    return fr[i - stride];
+  };
+  auto left_at = [&](int i) -> int {
+    if ((ml >> (i >> sh)) & 1) {
+      const uint64_t v = poll_word(colp + ((y + i) >> 1), a.epoch, error);
+      return (int)((v >> (16 * (i & 1))) & 0xffff);
     }
-  }
-  wsync();
-  const int n = b.w + b.h;
-  for (int i = lane; i < n; i += 64) {
-    if (ha) {
-      if (i > ae - x) above[i] = above[ae - x];
+    return fr[(int64_t)i * stride - 1];
+  };
+  /* one pass over the corner (e = 0), above[0 .. n) and left[0 .. n), the
+   * entries past the last sample a side depends on repeating it */
+  for (int e = threadIdx.x; e < 2 * n + 1; e += 64) {
+    int v;
+    if (e == 0) {
+      if (ha && hl) {
+        if (b.flags & kCornerFrame) {
+          v = fr[-stride - 1];
+        } else {
+          const uint64_t wv =
+              poll_word((b.flags & kCornerRow) ? rowp + (x >> 1) - 1 : colp + (y >> 1) - 1, a.epoch, error);
+          v = (int)((wv >> 16) & 0xffff); /* sample x - 1 of the row, or y - 1 of the column */
+        }
+      } else {
+        v = ha ? above_at(0) : (hl ? left_at(0) : mid);
+      }
+      above[-1] = v;
+      left[-1] = v;
+    } else if (e <= n) {
+      v = ha ? above_at(min(e - 1, ae)) : (hl ? left_at(0) : mid - 1);
+      above[e - 1] = v;
     } else {
-      above[i] = hl ? left[0] : mid - 1;
+      v = hl ? left_at(min(e - n - 1, le)) : (ha ? above_at(0) : mid + 1);
+      left[e - n - 1] = v;
     }
-    if (hl) {
-      if (i > le - y) left[i] = left[le - y];
-    } else {
-      left[i] = ha ? above[0] : mid + 1;
-    }
-  }
-  if (lane == 0 && !nc) {
-    const int c = ha ? above[0] : (hl ? left[0] : mid);
-    above[-1] = c;
-    left[-1] = c;
   }
 }
 
@@ -813,7 +800,34 @@ __device__ void write_edges(RArgs a, BlkRef b, const uint16_t* pred) {
   }
 }
 
-__device__ void predict_intra(RArgs a, BlkRef b, int* work, uint16_t* pred, const int* tab, uint32_t* error) {
+/* Residual samples of a block read before its wait (they do not depend on
+ * the neighbours): one transform covering the block, at most 256 samples,
+ * sample lane + 64 k in v[k]. */
+struct EarlyRes {
+  bool on;
+  int16_t v[4];
+};
+
+/* Writes prediction samples f(s), s = 0 .. n - 1, clamped, into pred; with
+ * an early residual (one transform over at most 256 samples, sample
+ * lane + 64 k in er.v[k]) the residual is added and clamped in the same
+ * pass (av1gpu_kernels.h:933-937). */
+template <class F>
+__device__ __forceinline__ void emit(int n, uint16_t* pred, const EarlyRes& er, int maxv, F f) {
+  if (er.on) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int s = threadIdx.x + 64 * k;
+      if (s < n) pred[s] = (uint16_t)clampi(clampi(f(s), 0, maxv) + er.v[k], 0, maxv);
+    }
+  } else {
+    for (int s = threadIdx.x; s < n; s += 64) pred[s] = (uint16_t)clampi(f(s), 0, maxv);
+  }
+}
+
+/* Returns whether the early residual is already added. */
+__device__ bool predict_intra(RArgs a, BlkRef b, int* work, uint16_t* pred, const int* tab, const EarlyRes& er,
+                              uint32_t* error) {
   const int lane = threadIdx.x;
   const int w = b.w, h = b.h, lw = ilog2(w), p = b.plane;
   const int stride = a.lay.stride[p];
@@ -825,7 +839,7 @@ __device__ void predict_intra(RArgs a, BlkRef b, int* work, uint16_t* pred, cons
       pred[s] = (uint16_t)b.palette[a.plan.palette[b.palette_off + y * b.palette_stride + x]];
     }
     wsync();
-    return;
+    return false;
   }
   int* above = work + kEdgeOff;
   int* left = work + kEdgeOff + kEdge + 2 * kEdgeOff;
@@ -897,10 +911,9 @@ __device__ void predict_intra(RArgs a, BlkRef b, int* work, uint16_t* pred, cons
     const int sum = wave_sum(part);
     const int shift = ilog2(w) + ilog2(h);
     const int avg = (sum + (1 << (shift - 1))) >> shift;
-    for (int s = lane; s < w * h; s += 64)
-      pred[s] = (uint16_t)clampi(dc + rnd2s(b.cfl_alpha * (tmp[s] - avg), 6), 0, maxv);
+    emit(w * h, pred, er, maxv, [&](int s) { return dc + rnd2s(b.cfl_alpha * (tmp[s] - avg), 6); });
     wsync();
-    return;
+    return true;
   }
   if ((b.flags & kFilterIntra) && w >= 4 && h >= 2) {
     /* recursive filter intra (av1gpu_kernels.h:1392-1427), one anti-diagonal
@@ -930,7 +943,7 @@ __device__ void predict_intra(RArgs a, BlkRef b, int* work, uint16_t* pred, cons
       }
       wsync();
     }
-    return;
+    return false;
   }
   if (mode >= 1 && mode <= 8) {
     const int da = b.dir_a, pa = da & 511;
@@ -940,7 +953,7 @@ __device__ void predict_intra(RArgs a, BlkRef b, int* work, uint16_t* pred, cons
     if (da & kDirFilter) prepare_edges(b, above, left, work, tab, maxv, max_x, max_y, &ea, &el);
     /* av1gpu_kernels.h:1245-1313 */
     const int dx = b.dir_b & 0xffff, dy = (int)((uint32_t)b.dir_b >> 16);
-    for (int s = lane; s < w * h; s += 64) {
+    emit(w * h, pred, er, maxv, [&](int s) {
       const int i = s >> lw, c = s & (w - 1);
       int v;
       if (pa < 90) {
@@ -969,15 +982,15 @@ __device__ void predict_intra(RArgs a, BlkRef b, int* work, uint16_t* pred, cons
       } else {
         v = pa == 90 ? ea[c] : el[i];
       }
-      pred[s] = (uint16_t)clampi(v, 0, maxv);
-    }
+      return v;
+    });
     wsync();
-    return;
+    return true;
   }
   if (mode >= AV1H_PRED_SMOOTH && mode <= AV1H_PRED_SMOOTH_H) {
     /* av1gpu_kernels.h:1333-1353 */
     const int* sw = TAB(AV1T_OFF_SM_WEIGHTS);
-    for (int s = lane; s < w * h; s += 64) {
+    emit(w * h, pred, er, maxv, [&](int s) {
       const int y = s >> lw, x = s & (w - 1);
       int v;
       if (mode == AV1H_PRED_SMOOTH) {
@@ -990,26 +1003,26 @@ __device__ void predict_intra(RArgs a, BlkRef b, int* work, uint16_t* pred, cons
         const int wx = sw[w - 4 + x];
         v = rnd2(wx * left[y] + (256 - wx) * above[w - 1], 8);
       }
-      pred[s] = (uint16_t)clampi(v, 0, maxv);
-    }
+      return v;
+    });
     wsync();
-    return;
+    return true;
   }
   if (mode == AV1H_PRED_PAETH) {
     /* av1gpu_kernels.h:1364-1382 */
-    for (int s = lane; s < w * h; s += 64) {
+    emit(w * h, pred, er, maxv, [&](int s) {
       const int y = s >> lw, x = s & (w - 1);
       const int base = above[x] + left[y] - above[-1];
       const int pl = abs(base - left[y]), pt = abs(base - above[x]), ptl = abs(base - above[-1]);
-      const int v = (pl <= pt && pl <= ptl) ? left[y] : (pt <= ptl ? above[x] : above[-1]);
-      pred[s] = (uint16_t)clampi(v, 0, maxv);
-    }
+      return (pl <= pt && pl <= ptl) ? left[y] : (pt <= ptl ? above[x] : above[-1]);
+    });
     wsync();
-    return;
+    return true;
   }
   const int dc = clampi(dc_value(), 0, maxv);
-  for (int s = lane; s < w * h; s += 64) pred[s] = (uint16_t)dc;
+  emit(w * h, pred, er, maxv, [&](int) { return dc; });
   wsync();
+  return true;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1111,13 +1124,7 @@ __device__ void wait_cells(RArgs a, BlkRef b, uint32_t* error) {
 /* A block of the queue: the prediction, then the residuals of its transforms, which
  * k_itx computed before the queue started (av1gpu_kernels.h:933-937 for the
  * add), into `pred`; the caller stores it. */
-/* Residual samples of a block read before its wait (they do not depend on
- * the neighbours): one transform covering the block, at most 256 samples,
- * sample lane + 64 k in rv[k]. */
-struct EarlyRes {
-  bool on;
-  int16_t v[4];
-};
+
 
 __device__ __forceinline__ EarlyRes early_residual(RArgs a, BlkRef b) {
   EarlyRes e;
@@ -1138,8 +1145,9 @@ __device__ __forceinline__ EarlyRes early_residual(RArgs a, BlkRef b) {
 
 __device__ void reconstruct_queued(RArgs a, BlkRef b, int* work, uint16_t* pred, const EarlyRes& er,
                                    const int* tab, uint32_t* error) {
+  bool fused = false;
   if (b.kind == kIntra) {
-    predict_intra(a, b, work, pred, tab, error);
+    fused = predict_intra(a, b, work, pred, tab, er, error);
   } else {
     /* inter/intra or intra block copy: reads the frame being decoded */
     const int lw = ilog2(b.w);
@@ -1148,6 +1156,7 @@ __device__ void reconstruct_queued(RArgs a, BlkRef b, int* work, uint16_t* pred,
   }
   const int p = b.plane, stride = a.lay.stride[p];
   if (er.on) {
+    if (fused) return;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int s = threadIdx.x + 64 * k;
