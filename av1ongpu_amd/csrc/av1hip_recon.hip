@@ -776,6 +776,18 @@ __device__ void gather_packed(RArgs a, BlkRef b, int* above, int* left, int alas
   }
 }
 
+/* The dependency wait of a kPackedEdges intra block: its edges into the
+ * intra work area (entries -1 .. w + h - 1 of above / left, as
+ * predict_intra reads them). */
+__device__ void gather_edges(RArgs a, BlkRef b, int* work, uint32_t* error) {
+  const int p = b.plane, w = b.w, h = b.h;
+  gather_packed(a, b, work + kEdgeOff, work + kEdgeOff + kEdge + 2 * kEdgeOff,
+                min(a.lay.grid_w[p] - 1, b.x + ((b.flags & kHaveAR) ? 2 * w : w) - 1),
+                min(a.lay.grid_h[p] - 1, b.y + ((b.flags & kHaveBL) ? 2 * h : h) - 1), 1 << (a.lay.bit_depth - 1),
+                error);
+  wsync();
+}
+
 /* A queued block's last row and column as packed words (gather_packed). */
 __device__ void write_edges(RArgs a, BlkRef b, const uint16_t* pred) {
   LayRef L = a.lay;
@@ -848,10 +860,7 @@ __device__ bool predict_intra(RArgs a, BlkRef b, int* work, uint16_t* pred, cons
   const int mid = 1 << (bd - 1);
   const int max_x = a.lay.grid_w[p] - 1, max_y = a.lay.grid_h[p] - 1;
   const int n = w + h;
-  if (b.flags & kPackedEdges) {
-    gather_packed(a, b, above, left, min(max_x, b.x + ((b.flags & kHaveAR) ? 2 * w : w) - 1),
-                  min(max_y, b.y + ((b.flags & kHaveBL) ? 2 * h : h) - 1), mid, error);
-  } else {
+  if (!(b.flags & kPackedEdges)) { /* packed edges: gathered by the caller (gather_edges) */
     const int alast = min(max_x, b.x + ((b.flags & kHaveAR) ? 2 * w : w) - 1);
     const int llast = min(max_y, b.y + ((b.flags & kHaveBL) ? 2 * h : h) - 1);
     for (int i = lane; i < n; i += 64) {
@@ -1389,7 +1398,10 @@ __global__ __launch_bounds__(64) void k_queue(BatchDev B) {
     RArgs a = recon_args(B, __builtin_amdgcn_readfirstlane(job));
     BlkRef b = cmem(a.plan.blocks)[__builtin_amdgcn_readfirstlane(blk)];
     const EarlyRes er = early_residual(a, b);
-    if (!(b.flags & kPackedEdges)) wait_cells(a, b, B.error);
+    if (b.flags & kPackedEdges)
+      gather_edges(a, b, s_work, B.error);
+    else
+      wait_cells(a, b, B.error);
     const uint64_t t1 = B.trace ? (uint64_t)wall_clock64() : 0;
     reconstruct_queued(a, b, s_work, s_pred, er, s_tab, B.error);
     /* the packed edges first: the blocks waiting on this one read only them */
