@@ -147,3 +147,27 @@ def test_serve_admission_budget():
         assert srv.stats()["budget_used"] == 0
     finally:
         srv.close()
+
+
+def test_serve_heavy_frames_batched_on_their_launcher():
+    """Frames whose queue has many dependency levels (the key frames here)
+    go to launcher 0, several per batch; each launcher keeps the launch
+    shapes set on it; every frame still equals the stream decoded alone."""
+    clips = [pkg.Clip(w, s, **o) for w, s, o in CLIPS[:4]]
+    alone = [_alone(c) for c in clips]
+    srv = pkg.Server(max_batch=8, planner_threads=2, capture=True, launchers=2, heavy_levels=8,
+                     heavy_launchers=1, heavy_batch=3)
+    try:
+        srv.set_launcher_tuning(0, {"queue": 2, "cdef_blocks_per_thread": 2})
+        srv.set_launcher_tuning(1, {"queue": 12, "mc": 4})
+        assert srv.launcher_tuning(0)["queue"] == 2 and srv.launcher_tuning(0)["cdef_blocks_per_thread"] == 2
+        assert srv.launcher_tuning(1)["queue"] == 12 and srv.launcher_tuning(1)["mc"] == 4
+        with pytest.raises(pkg.HipError):
+            srv.launcher_tuning(2)
+        sids = [srv.open_stream(*_geometry(c), pool_slots=8) for c in clips]
+        n = _submit_all(srv, clips, sids, 4)
+        got = _collect(srv, n)
+        _check(clips, alone, got)
+        assert srv.stats()["frames"] == n
+    finally:
+        srv.close()
