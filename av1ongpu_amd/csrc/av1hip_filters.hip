@@ -271,8 +271,8 @@ __device__ __forceinline__ bool cdef_sample(FArgs a, int first, int bi, int lane
   const int p = c.plane, uv = p > 0;
   const int bd = L.bit_depth, cs = bd - 8;
   const int st = L.stride[p];
-  const int w = uv ? (8 >> L.ss_x) : 8;
-  const int r = lane / w, k = lane % w;
+  const int lw = uv ? 3 - L.ss_x : 3, w = 1 << lw;
+  const int r = lane >> lw, k = lane & (w - 1);
   const int pw = L.grid_w[p], ph = L.grid_h[p];
   if (c.x + k >= pw || c.y + r >= ph) return false;
   int pri = c.pri << cs;
@@ -406,10 +406,9 @@ __device__ void upscale_tile(FArgs a, int t, const int* s_k, uint16_t (*s_row)[k
   const int ws = ((-(1 << 14) + init + x0 * step) >> 14) - 3;
   const int wn = ((-(1 << 14) + init + xl * step) >> 14) + 4 - ws + 1;
   const int rows = min(kUpRows, oh - y0);
-  for (int i = threadIdx.x; i < rows * wn; i += 256) {
-    const int r = i / wn, c = i - r * wn;
-    s_row[r][c] = src[poff + (size_t)(y0 + r) * stride + clampi(ws + c, 0, maxx)];
-  }
+  for (int r = 0; r < rows; ++r)
+    for (int c = threadIdx.x; c < wn; c += 256)
+      s_row[r][c] = src[poff + (size_t)(y0 + r) * stride + clampi(ws + c, 0, maxx)];
   __syncthreads();
   const int x = x0 + threadIdx.x;
   if (x > xl) return;
@@ -440,6 +439,30 @@ __global__ __launch_bounds__(256) void k_upscale(BatchDev B) {
 
 /* ---- loop restoration (av1gpu_kernels.h:2832-2982) ---------------------- */
 
+/* Row / column of element i of a w-wide 2D range walked by 256 threads
+ * (i = threadIdx.x, then + 256 each step), without a division per step. */
+struct Walk256 {
+  int r, q, dr, dq, w;
+  __device__ __forceinline__ explicit Walk256(int width) : w(width) {
+    r = (int)threadIdx.x / w;
+    q = (int)threadIdx.x - r * w;
+    dr = 256 / w;
+    dq = 256 - dr * w;
+  }
+  __device__ __forceinline__ void next() {
+    q += dq;
+    r += dr;
+    if (q >= w) {
+      q -= w;
+      ++r;
+    }
+  }
+};
+
+/* a2 of the self-guided filter for z = 0 .. 255 (the specification's
+ * x_by_xplus1 mapping, av1gpu_kernels.h:2898): z >= 255 gives 256 */
+__device__ __forceinline__ int sgr_a2(int z) { return z == 0 ? 1 : ((z << 8) + z / 2) / (z + 1); }
+
 struct LrCtx {
   const uint16_t* cdef;
   const uint16_t* curr;
@@ -467,7 +490,7 @@ constexpr int kLrTileW = kLrTileCols, kLrTileH = kLrTileRows;
 constexpr int kLrSrcW = kLrTileW + 6, kLrSrcH = kLrTileH + 6;
 constexpr int kLrAbW = kLrTileW + 2, kLrAbH = kLrTileH + 2;
 
-__device__ void lr_tile(FArgs a, int t, int* s_src, int* s_mid) {
+__device__ void lr_tile(FArgs a, int t, int* s_src, int* s_mid, const int* s_a2) {
   const uint16_t* cdef = a.lr_cdef;
   const uint16_t* curr = a.upscaled ? a.lr_prev_up : a.lr_curr;
   const int2 tile = a.plan.lr_tiles[t];
@@ -484,10 +507,7 @@ __device__ void lr_tile(FArgs a, int t, int* s_src, int* s_mid) {
   c.ph = L.out_h[p];
   c.s0 = j.stripe0;
   c.s1 = j.stripe1;
-  for (int i = threadIdx.x; i < (th + 6) * (tw + 6); i += 256) {
-    const int r = i / (tw + 6), q = i - r * (tw + 6);
-    s_src[r * kLrSrcW + q] = c.src(x0 - 3 + q, y0 - 3 + r);
-  }
+  for (Walk256 it(tw + 6); it.r < th + 6; it.next()) s_src[it.r * kLrSrcW + it.q] = c.src(x0 - 3 + it.q, y0 - 3 + it.r);
   __syncthreads();
   uint16_t* dst = a.frame + L.plane_off[p] + (size_t)y0 * c.stride + x0;
   if (j.type == 1) {
@@ -502,8 +522,8 @@ __device__ void lr_tile(FArgs a, int t, int* s_src, int* s_mid) {
     }
     vt[3] = 128 - 2 * (j.vf[0] + j.vf[1] + j.vf[2]);
     ht[3] = 128 - 2 * (j.hf[0] + j.hf[1] + j.hf[2]);
-    for (int i = threadIdx.x; i < (th + 6) * tw; i += 256) {
-      const int r = i / tw, q = i - r * tw;
+    for (Walk256 it(tw); it.r < th + 6; it.next()) {
+      const int r = it.r, q = it.q;
       const int* row = s_src + r * kLrSrcW + q;
       int sum = 0;
 #pragma unroll
@@ -511,8 +531,8 @@ __device__ void lr_tile(FArgs a, int t, int* s_src, int* s_mid) {
       s_mid[r * kLrTileW + q] = clampi(rsr64(sum, r0), -off, lim - off);
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < th * tw; i += 256) {
-      const int r = i / tw, q = i - r * tw;
+    for (Walk256 it(tw); it.r < th; it.next()) {
+      const int r = it.r, q = it.q;
       int sum = 0;
 #pragma unroll
       for (int t = 0; t < 7; ++t) sum += vt[t] * s_mid[(r + t) * kLrTileW + q];
@@ -522,7 +542,6 @@ __device__ void lr_tile(FArgs a, int t, int* s_src, int* s_mid) {
   }
   /* self guided: A and B of each pass on the (th + 2) x (tw + 2) positions
    * around the tile */
-  const int npos = (th + 2) * (tw + 2);
   for (int pass = 0; pass < 2; ++pass) {
     const int rr = j.r[pass];
     if (rr == 0) continue;
@@ -532,19 +551,21 @@ __device__ void lr_tile(FArgs a, int t, int* s_src, int* s_mid) {
     const int inv = ((1 << 12) + n / 2) / n;
     int* A = s_mid + (2 * pass) * kLrAbH * kLrAbW;
     int* B = A + kLrAbH * kLrAbW;
-    for (int i = threadIdx.x; i < npos; i += 256) {
-      const int r = i / (tw + 2), q = i - r * (tw + 2);
+    for (Walk256 it(tw + 2); it.r < th + 2; it.next()) {
+      const int r = it.r, q = it.q;
       /* position (x0 - 1 + q, y0 - 1 + r); its window starts at src row
        * r + 2 - rr, column q + 2 - rr of the staged window */
-      int64_t sa = 0, sb = 0;
+      /* at most 25 samples of 12 bits: the sums fit 32 bits */
+      int sa32 = 0, sb32 = 0;
       for (int dy = 0; dy < d; ++dy) {
         const int* row = s_src + (r + 2 - rr + dy) * kLrSrcW + q + 2 - rr;
         for (int dx = 0; dx < d; ++dx) {
           const int v = row[dx];
-          sa += (int64_t)v * v;
-          sb += v;
+          sa32 += v * v;
+          sb32 += v;
         }
       }
+      int64_t sa = sa32, sb = sb32;
       const int64_t braw = sb;
       if (bd > 8) {
         sa = rsr64(sa, 2 * (bd - 8));
@@ -553,15 +574,15 @@ __device__ void lr_tile(FArgs a, int t, int* s_src, int* s_mid) {
       int64_t pp = sa * n - sb * sb;
       if (pp < 0) pp = 0;
       const int z = rsr64(pp * mt, 20);
-      const int a2 = z >= 255 ? 256 : (z == 0 ? 1 : ((z << 8) + z / 2) / (z + 1));
+      const int a2 = z >= 255 ? 256 : s_a2[z];
       A[r * kLrAbW + q] = a2;
       B[r * kLrAbW + q] = rsr64((int64_t)(256 - a2) * braw * inv, 12);
     }
   }
   __syncthreads();
   const int w0 = j.xqd[0], w1 = j.xqd[1], w2 = (1 << 7) - w0 - w1;
-  for (int i = threadIdx.x; i < th * tw; i += 256) {
-    const int r = i / tw, q = i - r * tw;
+  for (Walk256 it(tw); it.r < th; it.next()) {
+    const int r = it.r, q = it.q;
     const int y = y0 + r;
     const int u = s_src[(r + 3) * kLrSrcW + q + 3];
     int flt[2] = {0, 0};
@@ -596,11 +617,14 @@ __device__ void lr_tile(FArgs a, int t, int* s_src, int* s_mid) {
 __global__ __launch_bounds__(256) void k_lr(BatchDev B) {
   __shared__ int s_src[kLrSrcH * kLrSrcW];
   __shared__ int s_mid[4 * kLrAbH * kLrAbW]; /* Wiener rows, or A0 B0 A1 B1 */
+  __shared__ int s_a2[256];
+  s_a2[threadIdx.x] = sgr_a2(threadIdx.x);
+  __syncthreads();
   const AV1H_CMEM int32_t* pre = kernel_prefix(B, kKLr);
   const int total = pre[B.njobs];
   for (int v = blockIdx.x; v < total; v += gridDim.x) {
     const int j = find_job(pre, B.njobs, v);
-    lr_tile(filter_args(B, j), v - pre[j], s_src, s_mid);
+    lr_tile(filter_args(B, j), v - pre[j], s_src, s_mid, s_a2);
     __syncthreads();
   }
 }

@@ -193,8 +193,8 @@ struct NoiseCtx {
   }
   /* the stripe buffer after the horizontal overlap blends */
   __device__ __forceinline__ int H(int s, int i, int x) const {
-    const int bw = 32 >> sx;
-    const int b = x / bw, j = x - b * bw;
+    const int lbw = 5 - sx, bw = 1 << lbw;
+    const int b = x >> lbw, j = x & (bw - 1);
     int g = G(s, b, i, j);
     if (overlap && b > 0 && j < (sx ? 1 : 2)) {
       const int old = G(s, b - 1, i, j + bw);
@@ -204,8 +204,8 @@ struct NoiseCtx {
   }
   /* the noise image sample after the vertical blends */
   __device__ __forceinline__ int N(int x, int y) const {
-    const int bh = 32 >> sy;
-    const int s = y / bh, row = y - s * bh;
+    const int lbh = 5 - sy, bh = 1 << lbh;
+    const int s = y >> lbh, row = y & (bh - 1);
     int v = H(s, row, x);
     if (overlap && s > 0 && row < (sy ? 1 : 2)) {
       const int old = H(s - 1, row + bh, x);
@@ -252,8 +252,9 @@ __device__ void grain_tile(FArgs a, int t) {
     uint16_t* plane = frame + L.plane_off[pl];
     const int st = L.stride[pl];
     const int lw = L.out_w[0];
+    const int ltw = 6 - L.ss_x;
     for (int i = threadIdx.x; i < tw * th; i += 256) {
-      const int x = x0 + (i % tw), y = y0 + (i / tw);
+      const int x = x0 + (i & (tw - 1)), y = y0 + (i >> ltw);
       if (x >= pw || y >= ph) continue;
       const int orig = plane[(size_t)y * st + x];
       const int lx = x << L.ss_x, ly = y << L.ss_y;
