@@ -329,11 +329,91 @@ __device__ __forceinline__ bool cdef_sample(FArgs a, int first, int bi, int lane
   return true;
 }
 
+/* constrain() with the shift of its threshold precomputed */
+__device__ __forceinline__ int constrain_adj(int diff, int thr, int adj) {
+  const int mag = abs(diff);
+  const int v = clampi(thr - (mag >> adj), 0, mag);
+  return diff < 0 ? -v : v;
+}
+
+/* A luma sample: one 8x8 block per wave, so the block's parameters and tap
+ * offsets are uniform (scalar registers), and blocks whose taps all stay
+ * inside the plane (every block but the outer ring) skip the per-tap
+ * bounds checks. Same arithmetic as cdef_sample. */
+__device__ __forceinline__ bool cdef_luma_sample(FArgs a, int bi, int lane, int64_t* off, int* val) {
+  bi = __builtin_amdgcn_readfirstlane(bi);
+  const AV1H_CMEM DevCdef& c = cmem(a.plan.cdef)[bi];
+  LayRef L = a.lay;
+  const int cs = L.bit_depth - 8, st = L.stride[0], pw = L.grid_w[0], ph = L.grid_h[0];
+  const int cx = c.x, cy = c.y;
+  const int r = lane >> 3, k = lane & 7;
+  int pri = c.pri << cs;
+  const int sec = c.sec << cs;
+  const int damping = c.damping + cs;
+  int dir = c.dir;
+  if (dir < 0) {
+    const int* m = a.cdef_dir + 2 * ((cy >> 3) * a.cdef_dir_w + (cx >> 3));
+    const int yd = __builtin_amdgcn_readfirstlane(m[0]), var = __builtin_amdgcn_readfirstlane(m[1]);
+    const int vs = (var >> 6) ? min(flog2(var >> 6), 12) : 0;
+    dir = pri == 0 ? 0 : yd;
+    pri = var ? ((pri * (4 + vs) + 8) >> 4) : 0;
+  }
+  if (pri == 0 && sec == 0) return false;
+  if (cx + k >= pw || cy + r >= ph) return false;
+  const int adjp = pri ? max(0, damping - flog2(pri)) : 0, adjs = sec ? max(0, damping - flog2(sec)) : 0;
+  const int ts = (pri >> cs) & 1;
+  const int pt0 = ts ? 3 : 4, pt1 = ts ? 3 : 2;
+  const int d1 = (dir + 2) & 7, d2 = (dir + 6) & 7;
+  const uint16_t* src = a.cdef_src + L.plane_off[0] + (size_t)(cy + r) * st + cx + k;
+  const int x = src[0];
+  int sum = 0, lo = x, hi = x;
+  if (cx >= 2 && cy >= 2 && cx + 10 <= pw && cy + 10 <= ph) {
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int op = kCdefDirs[dir][t][0] * st + kCdefDirs[dir][t][1];
+      const int o1 = kCdefDirs[d1][t][0] * st + kCdefDirs[d1][t][1];
+      const int o2 = kCdefDirs[d2][t][0] * st + kCdefDirs[d2][t][1];
+      const int wp = t ? pt1 : pt0, ws = t ? 1 : 2;
+#pragma unroll
+      for (int sg = -1; sg <= 1; sg += 2) {
+        const int vp = src[sg * op], v1 = src[sg * o1], v2 = src[sg * o2];
+        if (pri) sum += wp * constrain_adj(vp - x, pri, adjp);
+        if (sec) sum += ws * (constrain_adj(v1 - x, sec, adjs) + constrain_adj(v2 - x, sec, adjs));
+        lo = min(lo, min(vp, min(v1, v2)));
+        hi = max(hi, max(vp, max(v1, v2)));
+      }
+    }
+  } else {
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int sg = -1; sg <= 1; sg += 2)
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+          const int dd = u == 0 ? dir : (u == 1 ? d1 : d2);
+          const int dy = sg * kCdefDirs[dd][t][0], dx = sg * kCdefDirs[dd][t][1];
+          const int px = cx + k + dx, py = cy + r + dy;
+          if (px >= 0 && px < pw && py >= 0 && py < ph) {
+            const int v = src[dy * st + dx];
+            if (u == 0)
+              sum += (t ? pt1 : pt0) * constrain_adj(v - x, pri, adjp);
+            else if (sec)
+              sum += (t ? 1 : 2) * constrain_adj(v - x, sec, adjs);
+            lo = min(lo, v);
+            hi = max(hi, v);
+          }
+        }
+  }
+  *off = L.plane_off[0] + (int64_t)(cy + r) * st + cx + k;
+  *val = clampi(clampi(x + ((8 + sum - (sum < 0)) >> 4), lo, hi), 0, a.maxv);
+  return true;
+}
+
 /* kCdefBatch groups of 256 / S blocks per workgroup: a thread takes the same
  * sample of one block of each group, computes all of them (their reads
  * independent of each other) and then stores them. */
 
-template <int S>
+template <int S, bool kLuma>
 __device__ __forceinline__ void cdef_group(FArgs a, int first, int count, int grp, const int (*dirs)[2][2],
                                            int batch) {
   constexpr int kPer = 256 / S;
@@ -344,7 +424,10 @@ __device__ __forceinline__ void cdef_group(FArgs a, int first, int count, int gr
 #pragma unroll
   for (int u = 0; u < kCdefMaxBatch; ++u) {
     const int bi = (grp * batch + u) * kPer + threadIdx.x / S;
-    ok[u] = u < batch && bi < count && cdef_sample<S>(a, first, bi, lane, &off[u], &val[u], dirs);
+    if (kLuma)
+      ok[u] = u < batch && bi < count && cdef_luma_sample(a, bi, lane, &off[u], &val[u]);
+    else
+      ok[u] = u < batch && bi < count && cdef_sample<S>(a, first, bi, lane, &off[u], &val[u], dirs);
   }
 #pragma unroll
   for (int u = 0; u < kCdefMaxBatch; ++u)
@@ -365,14 +448,14 @@ __global__ __launch_bounds__(256) void k_cdef(BatchDev B, int chroma, int batch)
     FArgs a = filter_args(B, j);
     const int nl = a.plan.n_cdef_luma, grp = v - pre[j];
     if (!chroma) {
-      cdef_group<64>(a, 0, nl, grp, s_dirs, batch);
+      cdef_group<64, true>(a, 0, nl, grp, s_dirs, batch);
       continue;
     }
     const int nc = a.plan.n_cdef - nl;
     switch ((8 >> a.lay.ss_x) * (8 >> a.lay.ss_y)) {
-      case 16: cdef_group<16>(a, nl, nc, grp, s_dirs, batch); break;
-      case 32: cdef_group<32>(a, nl, nc, grp, s_dirs, batch); break;
-      default: cdef_group<64>(a, nl, nc, grp, s_dirs, batch); break;
+      case 16: cdef_group<16, false>(a, nl, nc, grp, s_dirs, batch); break;
+      case 32: cdef_group<32, false>(a, nl, nc, grp, s_dirs, batch); break;
+      default: cdef_group<64, false>(a, nl, nc, grp, s_dirs, batch); break;
     }
   }
 }
