@@ -172,6 +172,13 @@ __device__ __forceinline__ int constrain(int diff, int thr, int damping) {
   return diff < 0 ? -v : v;
 }
 
+/* constrain() with the shift of its threshold precomputed */
+__device__ __forceinline__ int constrain_adj(int diff, int thr, int adj) {
+  const int mag = abs(diff);
+  const int v = clampi(thr - (mag >> adj), 0, mag);
+  return diff < 0 ? -v : v;
+}
+
 __constant__ const int kCdefDirs[8][2][2] = {{{-1, 1}, {-2, 2}}, {{0, 1}, {-1, 2}}, {{0, 1}, {0, 2}},
                                              {{0, 1}, {1, 2}},   {{1, 1}, {2, 2}},   {{1, 0}, {2, 1}},
                                              {{1, 0}, {2, 0}},   {{1, 0}, {2, -1}}};
@@ -297,6 +304,29 @@ __device__ __forceinline__ bool cdef_sample(FArgs a, int first, int bi, int lane
   const int pt0 = ts ? 3 : 4, pt1 = ts ? 3 : 2;
   const int x = src[0];
   int sum = 0, lo = x, hi = x;
+  if (c.x >= 2 && c.y >= 2 && c.x + w + 2 <= pw && c.y + (S >> lw) + 2 <= ph) {
+    /* every tap inside the plane */
+    const int adjp = pri ? max(0, damping - flog2(pri)) : 0, adjs = sec ? max(0, damping - flog2(sec)) : 0;
+    const int d1 = (dir + 2) & 7, d2 = (dir + 6) & 7;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int op = dirs[dir][t][0] * st + dirs[dir][t][1];
+      const int o1 = dirs[d1][t][0] * st + dirs[d1][t][1];
+      const int o2 = dirs[d2][t][0] * st + dirs[d2][t][1];
+      const int wp = t ? pt1 : pt0, ws = t ? 1 : 2;
+#pragma unroll
+      for (int sg = -1; sg <= 1; sg += 2) {
+        const int vp = src[sg * op], v1 = src[sg * o1], v2 = src[sg * o2];
+        if (pri) sum += wp * constrain_adj(vp - x, pri, adjp);
+        if (sec) sum += ws * (constrain_adj(v1 - x, sec, adjs) + constrain_adj(v2 - x, sec, adjs));
+        lo = min(lo, min(vp, min(v1, v2)));
+        hi = max(hi, max(vp, max(v1, v2)));
+      }
+    }
+    *off = L.plane_off[p] + (int64_t)(c.y + r) * st + c.x + k;
+    *val = clampi(clampi(x + ((8 + sum - (sum < 0)) >> 4), lo, hi), 0, a.maxv);
+    return true;
+  }
 #pragma unroll
   for (int t = 0; t < 2; ++t)
 #pragma unroll
@@ -327,13 +357,6 @@ __device__ __forceinline__ bool cdef_sample(FArgs a, int first, int bi, int lane
   *off = L.plane_off[p] + (int64_t)(c.y + r) * st + c.x + k;
   *val = clampi(clampi(x + ((8 + sum - (sum < 0)) >> 4), lo, hi), 0, a.maxv);
   return true;
-}
-
-/* constrain() with the shift of its threshold precomputed */
-__device__ __forceinline__ int constrain_adj(int diff, int thr, int adj) {
-  const int mag = abs(diff);
-  const int v = clampi(thr - (mag >> adj), 0, mag);
-  return diff < 0 ? -v : v;
 }
 
 /* A luma sample: one 8x8 block per wave, so the block's parameters and tap
