@@ -1202,9 +1202,9 @@ __device__ void reconstruct_queued(RArgs a, BlkRef b, int* work, uint16_t* pred,
  *      (av1gpu_kernels.h:2207-2293), the store. */
 constexpr int kMcPairs = 2 * kMcItems;
 
-__device__ __forceinline__ int find_prefix(const int* pre, int n, int v) {
-  /* the last k < n with pre[k] <= v */
-  int lo = 0, hi = n - 1;
+__device__ __forceinline__ int find_prefix(const int* pre, int n, int v, int lo = 0) {
+  /* the last k < n with pre[k] <= v, known to be >= lo */
+  int hi = n - 1;
   while (lo < hi) {
     const int m = (lo + hi + 1) >> 1;
     if (pre[m] <= v)
@@ -1239,7 +1239,20 @@ struct McLds {
   int moff[kMcPairs + 1], px[kMcItems + 1];
 };
 
-__device__ void mc_body(RArgs a, int tn, const int* s_filt, McLds& L_) {
+/* the 8 taps of a subpel filter row: 8 int16 in one 16-byte LDS read */
+__device__ __forceinline__ void mc_taps(const int16_t* f, int* t) {
+  const int4 v = *reinterpret_cast<const int4*>(f);
+  t[0] = (int)(int16_t)v.x;
+  t[1] = v.x >> 16;
+  t[2] = (int)(int16_t)v.y;
+  t[3] = v.y >> 16;
+  t[4] = (int)(int16_t)v.z;
+  t[5] = v.z >> 16;
+  t[6] = (int)(int16_t)v.w;
+  t[7] = v.w >> 16;
+}
+
+__device__ void mc_body(RArgs a, int tn, const int16_t* s_filt, McLds& L_) {
   int16_t* s_mid = L_.mid;
   McItemDev* s_it = L_.it;
   int* s_moff = L_.moff;
@@ -1265,8 +1278,9 @@ __device__ void mc_body(RArgs a, int tn, const int* s_filt, McLds& L_) {
   const int round0 = bd >= 12 ? 5 : 3;
   /* horizontal pass over every reference row an item needs, taps read from
    * the reference (clamped to it) */
+  int kprev = 0;
   for (int e = tid; e < task.mtotal; e += 256) {
-    const int k = find_prefix(s_moff, npairs, e);
+    const int k = kprev = find_prefix(s_moff, npairs, e, kprev);
     const McItemDev& it = s_it[k >> 1];
     const McRefDev& q = it.r[k & 1];
     const int o = e - q.mid_off;
@@ -1281,7 +1295,8 @@ __device__ void mc_body(RArgs a, int tn, const int* s_filt, McLds& L_) {
     const int stride = p == 0 ? a.lay.stride[0] : (p == 1 ? a.lay.stride[1] : a.lay.stride[2]);
     const int64_t poff = p == 0 ? a.lay.plane_off[0] : (p == 1 ? a.lay.plane_off[1] : a.lay.plane_off[2]);
     const uint16_t* row = a.pool + (size_t)q.slot * a.slot_size + poff + (size_t)y * stride;
-    const int* hk = s_filt + it.hk + fx * 8;
+    int hk[8];
+    mc_taps(s_filt + it.hk + fx * 8, hk);
     int sum = 0;
     if (cx >= 3 && cx + 4 < it.ref_w) {
 #pragma unroll
@@ -1294,8 +1309,9 @@ __device__ void mc_body(RArgs a, int tn, const int* s_filt, McLds& L_) {
   }
   __syncthreads();
   /* vertical pass and blend, one thread per output sample */
+  int iprev = 0;
   for (int si = tid; si < task.samples; si += 256) {
-    const int item = find_prefix(s_px, nit, si);
+    const int item = iprev = find_prefix(s_px, nit, si, iprev);
     const McItemDev& it = s_it[item];
     const int s = si - it.px0;
     const int lw = ilog2(it.w), x = s & (it.w - 1), y = it.y0 + (s >> lw);
@@ -1312,7 +1328,8 @@ __device__ void mc_body(RArgs a, int tn, const int* s_filt, McLds& L_) {
       int cy, fy;
       P.row(y, &cy, &fy);
       const int16_t* m = s_mid + q.mid_off + (cy - 3 - q.wy0) * it.w + x;
-      const int* vk = s_filt + it.vk + fy * 8;
+      int vk[8];
+      mc_taps(s_filt + it.vk + fy * 8, vk);
       int acc = 0;
 #pragma unroll
       for (int t = 0; t < 8; ++t) acc += (int)m[t * it.w] * vk[t];
@@ -1344,9 +1361,9 @@ __device__ void mc_body(RArgs a, int tn, const int* s_filt, McLds& L_) {
 }
 
 __global__ __launch_bounds__(256) void k_mc(BatchDev B) {
-  __shared__ int s_filt[768];
+  __shared__ __align__(16) int16_t s_filt[768];
   __shared__ McLds s_lds;
-  for (int i = threadIdx.x; i < 768; i += 256) s_filt[i] = av1t_spec_table[AV1T_OFF_SUBPEL + i];
+  for (int i = threadIdx.x; i < 768; i += 256) s_filt[i] = (int16_t)av1t_spec_table[AV1T_OFF_SUBPEL + i];
   const AV1H_CMEM int32_t* pre = kernel_prefix(B, kKMc);
   const int total = pre[B.njobs];
   for (int v = blockIdx.x; v < total; v += gridDim.x) {
