@@ -147,7 +147,9 @@ __global__ __launch_bounds__(256) void k_deblock(BatchDev B, int pass) {
   __shared__ int s_line0[256];
   const AV1H_CMEM int32_t* pre = kernel_prefix(B, (pass ? kKDeblockH : kKDeblockV));
   const int total = pre[B.njobs];
-  for (int v = blockIdx.x; v < total; v += gridDim.x) {
+  for (int base = 0; base < total; base += gridDim.x) {
+    const int v = xcd_group(base, total);
+    if (v < 0) continue;
     const int j = find_job(pre, B.njobs, v);
     FArgs a = filter_args(B, j);
     PlanRef P = a.plan;
@@ -259,7 +261,9 @@ __global__ __launch_bounds__(256) void k_cdef_dir(BatchDev B) {
   const AV1H_CMEM int32_t* pre = kernel_prefix(B, kKCdefDir);
   const int total = pre[B.njobs];
   int* part = &s_part[threadIdx.x >> 6][0][0];
-  for (int v = blockIdx.x; v < total; v += gridDim.x) {
+  for (int base = 0; base < total; base += gridDim.x) {
+    const int v = xcd_group(base, total);
+    if (v < 0) continue;
     const int j = find_job(pre, B.njobs, v);
     cdef_dir_group(filter_args(B, j), v - pre[j], part);
     __syncthreads();
@@ -466,7 +470,9 @@ __global__ __launch_bounds__(256) void k_cdef(BatchDev B, int chroma, int batch)
   __syncthreads();
   const AV1H_CMEM int32_t* pre = kernel_prefix(B, (chroma ? kKCdefChroma : kKCdefLuma));
   const int total = pre[B.njobs];
-  for (int v = blockIdx.x; v < total; v += gridDim.x) {
+  for (int base = 0; base < total; base += gridDim.x) {
+    const int v = xcd_group(base, total);
+    if (v < 0) continue;
     const int j = find_job(pre, B.njobs, v);
     FArgs a = filter_args(B, j);
     const int nl = a.plan.n_cdef_luma, grp = v - pre[j];
@@ -728,7 +734,9 @@ __global__ __launch_bounds__(256) void k_lr(BatchDev B) {
   __syncthreads();
   const AV1H_CMEM int32_t* pre = kernel_prefix(B, kKLr);
   const int total = pre[B.njobs];
-  for (int v = blockIdx.x; v < total; v += gridDim.x) {
+  for (int base = 0; base < total; base += gridDim.x) {
+    const int v = xcd_group(base, total);
+    if (v < 0) continue;
     const int j = find_job(pre, B.njobs, v);
     lr_tile(filter_args(B, j), v - pre[j], s_src, s_mid, s_a2);
     __syncthreads();

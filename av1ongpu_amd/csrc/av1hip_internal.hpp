@@ -284,6 +284,20 @@ __device__ __forceinline__ const AV1H_CMEM T* cmem(const T* p) {
 }
 
 /* the frame a virtual workgroup belongs to: the last j < n with pre[j] <= v */
+/* This workgroup's virtual workgroup in the sweep of the grid that starts
+ * at `base` (sweeps of gridDim.x until `total`), or -1: the sweep's r
+ * virtual workgroups are cut into 8 consecutive runs, run x for the
+ * workgroups on XCD x (dispatched round robin: workgroup b on XCD b % 8),
+ * so neighbouring blocks, which read overlapping reference and picture
+ * data, share one XCD's L2, and a sweep shorter than the grid still spreads
+ * over all XCDs. */
+__device__ __forceinline__ int xcd_group(int base, int total) {
+  const int g = (int)gridDim.x, b = (int)blockIdx.x;
+  if (g & 7) return base + b < total ? base + b : -1;
+  const int r = min(g, total - base), c = (r + 7) >> 3, j = b >> 3, v = (b & 7) * c + j;
+  return j < c && v < r ? base + v : -1;
+}
+
 __device__ __forceinline__ int find_job(const AV1H_CMEM int32_t* pre, int n, int v) {
   int lo = 0, hi = n - 1;
   while (lo < hi) {

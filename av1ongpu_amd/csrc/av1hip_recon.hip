@@ -1366,7 +1366,9 @@ __global__ __launch_bounds__(256) void k_mc(BatchDev B) {
   for (int i = threadIdx.x; i < 768; i += 256) s_filt[i] = (int16_t)av1t_spec_table[AV1T_OFF_SUBPEL + i];
   const AV1H_CMEM int32_t* pre = kernel_prefix(B, kKMc);
   const int total = pre[B.njobs];
-  for (int v = blockIdx.x; v < total; v += gridDim.x) {
+  for (int base = 0; base < total; base += gridDim.x) {
+    const int v = xcd_group(base, total);
+    if (v < 0) continue;
     const int j = find_job(pre, B.njobs, v);
     mc_body(recon_args(B, j), v - pre[j], s_filt, s_lds);
     __syncthreads();
