@@ -1175,13 +1175,25 @@ __device__ void reconstruct_queued(RArgs a, BlkRef b, int* work, uint16_t* pred,
     return;
   }
   for (int t = 0; t < b.tx_count; ++t) {
-    const DevTx& tx = a.plan.tx[b.tx_first + t];
-    const int ox = tx.x - b.x, oy = tx.y - b.y, ltw = ilog2(tx.w);
+    const AV1H_CMEM DevTx& tx = cmem(a.plan.tx)[b.tx_first + t];
+    const int ox = tx.x - b.x, oy = tx.y - b.y, ltw = ilog2(tx.w), n = tx.w * tx.h;
     const int16_t* r = a.res + a.lay.plane_off[p] + (int64_t)tx.y * stride + tx.x;
-    for (int s = threadIdx.x; s < tx.w * tx.h; s += 64) {
-      const int yy = s >> ltw, xx = s & (tx.w - 1);
-      uint16_t* q = pred + (oy + yy) * b.w + ox + xx;
-      *q = (uint16_t)clampi((int)*q + r[(int64_t)yy * stride + xx], 0, a.maxv);
+    /* eight residual loads in flight per lane before the first add */
+    for (int s0 = threadIdx.x; s0 < n; s0 += 64 * 8) {
+      int rv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int s = s0 + 64 * u;
+        rv[u] = s < n ? r[(int64_t)(s >> ltw) * stride + (s & (tx.w - 1))] : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int s = s0 + 64 * u;
+        if (s < n) {
+          uint16_t* q = pred + (oy + (s >> ltw)) * b.w + ox + (s & (tx.w - 1));
+          *q = (uint16_t)clampi((int)*q + rv[u], 0, a.maxv);
+        }
+      }
     }
   }
   wsync();
