@@ -1416,7 +1416,14 @@ __global__ __launch_bounds__(64) void k_queue(BatchDev B) {
     int idx = 0;
     if (threadIdx.x == 0) idx = (int)atomicAdd(B.queue_counter, 1u);
     idx = __builtin_amdgcn_readfirstlane(__shfl(idx, 0));
-    if (idx >= nqueue) return;
+    if (idx >= nqueue) {
+      /* every workgroup takes exactly one entry past the end; the last of
+       * those is the counter's last use in this launch, so it resets the
+       * counter for the next one */
+      if (threadIdx.x == 0 && idx == nqueue + (int)gridDim.x - 1)
+        __hip_atomic_store(B.queue_counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return;
+    }
     const uint64_t t0 = B.trace ? (uint64_t)wall_clock64() : 0;
     int job = 0, blk;
     if (B.queue) {
@@ -1456,10 +1463,6 @@ __global__ __launch_bounds__(64) void k_queue(BatchDev B) {
   }
 }
 
-__global__ void k_zero(uint32_t* p, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
-}
 
 }  // namespace
 
@@ -1488,10 +1491,9 @@ hipError_t launch_recon_kernel(const BatchDev& b, int k, int total, int cap, hip
 
 hipError_t launch_queue(const BatchDev& b, int total, int cap, hipStream_t s, int* launches) {
   if (total == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, b.queue_counter, 1);
   const int grid = total < 0 ? cap : min(total, cap);
   hipLaunchKernelGGL(k_queue, dim3(grid), dim3(64), 0, s, b);
-  *launches += 2;
+  *launches += 1;
   return hipGetLastError();
 }
 
