@@ -1575,7 +1575,7 @@ int kernel_groups(const FrameJob& fj, int k, int cdef_batch) {
     case kKItxIntraL: return itx_groups(p, 1, 3, kTxClasses);
     case kKDeblockV: return p.n_deblock_v > 0 ? (p.lines_v + 255) / 256 : 0;
     case kKDeblockH: return p.n_deblock_h > 0 ? (p.lines_h + 255) / 256 : 0;
-    case kKCdefDir: return p.n_cdef > 0 ? (p.n_cdef_dir + 3) / 4 : 0;
+    case kKCdefDir: return p.n_cdef > 0 ? (p.n_cdef_dir + 4 * kCdefDirPerWave - 1) / (4 * kCdefDirPerWave) : 0;
     case kKCdefLuma: return p.n_cdef > 0 ? (p.n_cdef_luma + 4 * cdef_batch - 1) / (4 * cdef_batch) : 0;
     case kKCdefChroma: {
       const int nc = p.n_cdef - p.n_cdef_luma;

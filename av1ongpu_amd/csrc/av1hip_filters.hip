@@ -187,55 +187,72 @@ __constant__ const int kCdefDirs[8][2][2] = {{{-1, 1}, {-2, 2}}, {{0, 1}, {-1, 2
 __constant__ const int kCdefUvDir[2][2][8] = {{{0, 1, 2, 3, 4, 5, 6, 7}, {1, 2, 2, 2, 3, 4, 6, 0}},
                                               {{7, 0, 2, 4, 5, 6, 6, 6}, {0, 1, 2, 3, 4, 5, 6, 7}}};
 
-/* Direction search of luma 8x8 blocks (av1gpu_kernels.h:2601-2660), one wave
- * per block, one lane per sample: the eight directional line sums are
- * accumulated with LDS atomics, lanes 0-7 weigh one direction each. Writes
- * (direction, variance) for the block into `dirmap`. */
-__device__ void cdef_dir_group(FArgs a, int grp, int* part) {
+/* Direction search of luma 8x8 blocks (av1gpu_kernels.h:2601-2660), two
+ * blocks per wave (kCdefDirPerWave), one lane per sample of each: the eight
+ * directional line sums are accumulated with LDS atomics, then lanes 0-7
+ * (first block) and 8-15 (second) weigh one direction each. Writes
+ * (direction, variance) for each block into `dirmap`. */
+__device__ void cdef_dir_group(FArgs a, int grp, int (*part)[8][16]) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int bi = grp * 4 + wave;
-  const bool live = bi < a.plan.n_cdef_dir;
-  for (int i = lane; i < 128; i += 64) part[i] = 0;
+  const int i = lane >> 3, k = lane & 7;
+  const int bd = a.lay.bit_depth;
+  int bx[kCdefDirPerWave], by[kCdefDirPerWave];
+  bool live[kCdefDirPerWave];
+  for (int u = 0; u < kCdefDirPerWave; ++u)
+    for (int e = lane; e < 128; e += 64) (&part[u][0][0])[e] = 0;
   __syncthreads();
-  int bx = 0, by = 0;
-  if (live) {
-    const int pos = a.plan.cdef_dir_list[bi];
-    bx = pos & 0xffff;
-    by = pos >> 16;
-    const int i = lane >> 3, k = lane & 7;
-    const int bd = a.lay.bit_depth;
-    const int x = (a.cdef_src[a.lay.plane_off[0] + (size_t)(by * 8 + i) * a.lay.stride[0] + bx * 8 + k] >> (bd - 8)) -
-                  128;
-    atomicAdd(&part[0 * 16 + i + k], x);
-    atomicAdd(&part[1 * 16 + i + k / 2], x);
-    atomicAdd(&part[2 * 16 + i], x);
-    atomicAdd(&part[3 * 16 + 3 + i - k / 2], x);
-    atomicAdd(&part[4 * 16 + 7 + i - k], x);
-    atomicAdd(&part[5 * 16 + 3 - i / 2 + k], x);
-    atomicAdd(&part[6 * 16 + k], x);
-    atomicAdd(&part[7 * 16 + i / 2 + k], x);
-  }
-  __syncthreads();
-  if (!live) return;
-  const int div[9] = {0, 840, 420, 280, 210, 168, 140, 120, 105};
-  int cost = 0;
-  if (lane < 8) {
-    const int d = lane;
-    const int* pd = part + d * 16;
-    if (d == 2 || d == 6) {
-      for (int i = 0; i < 8; ++i) cost += pd[i] * pd[i];
-      cost *= div[8];
-    } else if (d == 0 || d == 4) {
-      for (int i = 0; i < 7; ++i) cost += (pd[i] * pd[i] + pd[14 - i] * pd[14 - i]) * div[i + 1];
-      cost += pd[7] * pd[7] * div[8];
-    } else {
-      for (int k = 0; k < 5; ++k) cost += pd[3 + k] * pd[3 + k];
-      cost *= div[8];
-      for (int k = 0; k < 3; ++k) cost += (pd[k] * pd[k] + pd[10 - k] * pd[10 - k]) * div[2 * k + 2];
+  int x[kCdefDirPerWave];
+#pragma unroll
+  for (int u = 0; u < kCdefDirPerWave; ++u) {
+    const int bi = (grp * 4 + wave) * kCdefDirPerWave + u;
+    live[u] = bi < a.plan.n_cdef_dir;
+    bx[u] = by[u] = 0;
+    x[u] = 0;
+    if (live[u]) {
+      const int pos = a.plan.cdef_dir_list[bi];
+      bx[u] = pos & 0xffff;
+      by[u] = pos >> 16;
+      x[u] = (a.cdef_src[a.lay.plane_off[0] + (size_t)(by[u] * 8 + i) * a.lay.stride[0] + bx[u] * 8 + k] >> (bd - 8)) -
+             128;
     }
   }
-  /* the first direction with the largest cost (a strict > scan from 0) */
-  int best = lane < 8 ? cost : INT_MIN, bdir = lane;
+#pragma unroll
+  for (int u = 0; u < kCdefDirPerWave; ++u) {
+    if (!live[u]) continue;
+    int* pu = &part[u][0][0];
+    atomicAdd(&pu[0 * 16 + i + k], x[u]);
+    atomicAdd(&pu[1 * 16 + i + k / 2], x[u]);
+    atomicAdd(&pu[2 * 16 + i], x[u]);
+    atomicAdd(&pu[3 * 16 + 3 + i - k / 2], x[u]);
+    atomicAdd(&pu[4 * 16 + 7 + i - k], x[u]);
+    atomicAdd(&pu[5 * 16 + 3 - i / 2 + k], x[u]);
+    atomicAdd(&pu[6 * 16 + k], x[u]);
+    atomicAdd(&pu[7 * 16 + i / 2 + k], x[u]);
+  }
+  __syncthreads();
+  /* lanes 8 u .. 8 u + 7 weigh the directions of block u */
+  const int u = lane >> 3;
+  const bool mine = u < kCdefDirPerWave && live[u < kCdefDirPerWave ? u : 0];
+  const int div[9] = {0, 840, 420, 280, 210, 168, 140, 120, 105};
+  int cost = 0;
+  if (mine) {
+    const int d = lane & 7;
+    const int* pd = &part[u][d][0];
+    if (d == 2 || d == 6) {
+      for (int t = 0; t < 8; ++t) cost += pd[t] * pd[t];
+      cost *= div[8];
+    } else if (d == 0 || d == 4) {
+      for (int t = 0; t < 7; ++t) cost += (pd[t] * pd[t] + pd[14 - t] * pd[14 - t]) * div[t + 1];
+      cost += pd[7] * pd[7] * div[8];
+    } else {
+      for (int t = 0; t < 5; ++t) cost += pd[3 + t] * pd[3 + t];
+      cost *= div[8];
+      for (int t = 0; t < 3; ++t) cost += (pd[t] * pd[t] + pd[10 - t] * pd[10 - t]) * div[2 * t + 2];
+    }
+  }
+  /* the first direction with the largest cost (a strict > scan from 0),
+   * within each group of 8 lanes */
+  int best = mine ? cost : INT_MIN, bdir = lane & 7;
 #pragma unroll
   for (int o = 1; o < 8; o <<= 1) {
     const int oc = __shfl_xor(best, o), od = __shfl_xor(bdir, o);
@@ -244,23 +261,24 @@ __device__ void cdef_dir_group(FArgs a, int grp, int* part) {
       bdir = od;
     }
   }
-  const int bcost = __shfl(best, 0);
+  const int g0 = lane & ~7;
+  const int bcost = __shfl(best, g0);
   /* a block whose costs are all zero keeps direction 0, as the scan does */
-  const int dd = bcost > 0 ? __shfl(bdir, 0) : 0;
-  const int opp = __shfl(cost, (dd + 4) & 7);
-  if (lane == 0) {
+  const int dd = bcost > 0 ? __shfl(bdir, g0) : 0;
+  const int opp = __shfl(cost, g0 + ((dd + 4) & 7));
+  if (mine && (lane & 7) == 0) {
     const int var = ((bcost > 0 ? bcost : 0) - opp) >> 10;
-    int* m = a.cdef_dir + 2 * (by * a.cdef_dir_w + bx);
+    int* m = a.cdef_dir + 2 * (by[u] * a.cdef_dir_w + bx[u]);
     m[0] = dd;
     m[1] = var;
   }
 }
 
 __global__ __launch_bounds__(256) void k_cdef_dir(BatchDev B) {
-  __shared__ int s_part[4][8][16];
+  __shared__ int s_part[4][kCdefDirPerWave][8][16];
   const AV1H_CMEM int32_t* pre = kernel_prefix(B, kKCdefDir);
   const int total = pre[B.njobs];
-  int* part = &s_part[threadIdx.x >> 6][0][0];
+  int(*part)[8][16] = s_part[threadIdx.x >> 6];
   for (int base = 0; base < total; base += gridDim.x) {
     const int v = xcd_group(base, total);
     if (v < 0) continue;
