@@ -319,7 +319,7 @@ constexpr int kMaxBatch = 128;
  * (a thread's samples of up to kCdefMaxBatch blocks computed together) */
 constexpr int kCdefMaxBatch = 4;
 /* CDEF direction search: luma 8x8 blocks per wave (4 waves per workgroup) */
-constexpr int kCdefDirPerWave = 2;
+constexpr int kCdefDirPerWave = 4;
 
 /* samples one checksum workgroup reads */
 constexpr int kHashChunk = 4096;
