@@ -1042,6 +1042,13 @@ __device__ void store_block(RArgs a, BlkRef b, const uint16_t* pred) {
   const int p = b.plane, lw = ilog2(b.w);
   const int stride = a.lay.stride[p];
   uint16_t* dst = a.frame + a.lay.plane_off[p] + (size_t)b.y * stride + b.x;
+  if (b.flags & kSkipMark) {
+    /* no block of this launch reads these samples (its readers take the
+     * packed edge words): ordinary stores, written back through the L2 at
+     * the end of the launch like any kernel's */
+    for (int s = threadIdx.x; s < b.w * b.h; s += 64) dst[(size_t)(s >> lw) * stride + (s & (b.w - 1))] = pred[s];
+    return;
+  }
   for (int s = threadIdx.x; s < b.w * b.h; s += 64) st_dev(dst + (size_t)(s >> lw) * stride + (s & (b.w - 1)), pred[s]);
 }
 
