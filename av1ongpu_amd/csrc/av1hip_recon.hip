@@ -119,8 +119,21 @@ __device__ __forceinline__ void col_pass(const int* tmp, int lane, int w, int tx
   }
   const int ud = itx::flip_ud(tx_type), lr = itx::flip_lr(tx_type);
   const int xx = lr ? w - 1 - x : x;
+  /* eight rows at a time: their current samples (an inter block's
+   * prediction) are all loaded before any is stored, so the adds do not wait
+   * on one load after another */
 #pragma unroll
-  for (int i = 0; i < M; ++i) out(xx, ud ? M - 1 - i : i, lossless ? t[i] : rnd2(t[i], 4));
+  for (int i0 = 0; i0 < M; i0 += 8) {
+    constexpr int kB = M < 8 ? M : 8;
+    int cur[kB];
+#pragma unroll
+    for (int u = 0; u < kB; ++u) cur[u] = out.load(xx, ud ? M - 1 - (i0 + u) : i0 + u);
+#pragma unroll
+    for (int u = 0; u < kB; ++u) {
+      const int i = i0 + u;
+      out.store(xx, ud ? M - 1 - i : i, cur[u], lossless ? t[i] : rnd2(t[i], 4));
+    }
+  }
 }
 
 /* the row / column pass of the size `l2` (log2 points), instantiated only up
@@ -175,7 +188,23 @@ __device__ __forceinline__ void itx_group(RArgs a, const int32_t* list, int n, i
   const int64_t off = poff + (int64_t)tx.y * stride + tx.x;
   const bool to_res = (tx.flags & kTxToResidual) != 0;
   const int maxv = a.maxv;
-  auto out = [&](int xx, int yy, int v) {
+  struct {
+    RArgs a;
+    int64_t off;
+    int stride, maxv;
+    bool to_res;
+    __device__ __forceinline__ int load(int xx, int yy) const {
+      return to_res ? 0 : (int)a.frame[off + (int64_t)yy * stride + xx];
+    }
+    __device__ __forceinline__ void store(int xx, int yy, int cur, int v) const {
+      const int64_t o = off + (int64_t)yy * stride + xx;
+      if (to_res)
+        a.res[o] = (int16_t)clampi(v, -32768, 32767);
+      else
+        a.frame[o] = (uint16_t)clampi(cur + v, 0, maxv);
+    }
+  } out{a, off, stride, maxv, to_res};
+  auto out1 = [&](int xx, int yy, int v) {
     const int64_t o = off + (int64_t)yy * stride + xx;
     if (to_res)
       a.res[o] = (int16_t)clampi(v, -32768, 32767);
@@ -186,7 +215,7 @@ __device__ __forceinline__ void itx_group(RArgs a, const int32_t* list, int n, i
   const int lw = ilog2(w), lh = ilog2(h);
   if (live && !coded) {
     /* raw residual (av1gpu_kernels.h:933-937) */
-    for (int i = l; i < w * h; i += G) out(i & (w - 1), i >> lw, a.plan.residual[tx.coef + i]);
+    for (int i = l; i < w * h; i += G) out1(i & (w - 1), i >> lw, a.plan.residual[tx.coef + i]);
   }
   if (coded) {
     const int cs = w < 32 ? w : 32, cr = h < 32 ? h : 32;
