@@ -24,12 +24,9 @@ __device__ __forceinline__ int f4c(int v, int bd) { return clampi(v, -(1 << (bd 
 
 /* The flat filters: outputs p(N-1)..q(N-1) from taps p(N)..q(N), the centre
  * N2 taps weighted twice, all indices fixed at compile time
- * (av1gpu_kernels.h:2540-2568). */
+ * (av1gpu_kernels.h:2540-2568). v[j] holds the sample at b[(j - 5) s]. */
 template <int N, int N2, int LG>
-__device__ __forceinline__ void wide_filter(uint16_t* b, int s, int maxv) {
-  int v[2 * N + 2];
-#pragma unroll
-  for (int i = -(N + 1); i <= N; ++i) v[i + N + 1] = b[(2 + i) * s];
+__device__ __forceinline__ void wide_filter(const int* v, uint16_t* b, int s, int maxv) {
   int res[2 * N];
 #pragma unroll
   for (int i = -N; i < N; ++i) {
@@ -37,7 +34,7 @@ __device__ __forceinline__ void wide_filter(uint16_t* b, int s, int maxv) {
 #pragma unroll
     for (int k = -N; k <= N; ++k) {
       const int idx = i + k < -(N + 1) ? -(N + 1) : (i + k > N ? N : i + k);
-      t += v[idx + N + 1] * ((k <= N2 && k >= -N2) ? 2 : 1);
+      t += v[idx + 7] * ((k <= N2 && k >= -N2) ? 2 : 1);
     }
     res[i + N] = rnd2(t, LG);
   }
@@ -45,12 +42,20 @@ __device__ __forceinline__ void wide_filter(uint16_t* b, int s, int maxv) {
   for (int i = -N; i < N; ++i) b[(2 + i) * s] = (uint16_t)clampi(res[i + N], 0, maxv);
 }
 
-/* one line across one edge: b points at p1, s is the step across the edge */
+/* one line across one edge: b points at p1, s is the step across the edge.
+ * Every sample the line's filter size can use is loaded first, together, so
+ * the decisions do not wait on one load after another. */
 __device__ void deblock_line(uint16_t* b, int s, int level, int sharp, int fs, int chroma, int bd,
                              int maxv) {
   const int sh = bd - 8, mid = 0x80 << sh;
-  const int p1 = b[0], p0 = b[s], q0 = b[2 * s], q1 = b[3 * s];
   const int flen = fs == 4 ? 4 : (chroma ? 6 : (fs == 8 ? 8 : 16));
+  /* v[j] = b[(j - 5) s]: p6 .. p1 at 0 .. 5, p0 6, q0 7, q1 .. q6 at 8 .. 13 */
+  const int jlo = fs >= 16 ? 0 : (flen >= 8 ? 3 : (flen >= 6 ? 4 : 5));
+  const int jhi = fs >= 16 ? 13 : (flen >= 8 ? 10 : (flen >= 6 ? 9 : 8));
+  int v[14];
+#pragma unroll
+  for (int j = 0; j < 14; ++j) v[j] = (j >= jlo && j <= jhi) ? (int)b[(j - 5) * s] : 0;
+  const int p1 = v[5], p0 = v[6], q0 = v[7], q1 = v[8];
   const int lsh = sharp > 4 ? 2 : (sharp > 0 ? 1 : 0);
   int limit = level >> lsh;
   if (limit < 1) limit = 1;
@@ -59,17 +64,9 @@ __device__ void deblock_line(uint16_t* b, int s, int level, int sharp, int fs, i
   const int one = 1 << sh;
   const int hev = abs(p1 - p0) > thr || abs(q1 - q0) > thr;
   int bad = abs(p1 - p0) > lim || abs(q1 - q0) > lim || abs(p0 - q0) * 2 + abs(p1 - q1) / 2 > blim;
-  int p2 = 0, q2 = 0, p3 = 0, q3 = 0;
-  if (flen >= 6) {
-    p2 = b[-s];
-    q2 = b[4 * s];
-    bad = bad || abs(p2 - p1) > lim || abs(q2 - q1) > lim;
-  }
-  if (flen >= 8) {
-    p3 = b[-2 * s];
-    q3 = b[5 * s];
-    bad = bad || abs(p3 - p2) > lim || abs(q3 - q2) > lim;
-  }
+  const int p2 = v[4], q2 = v[9], p3 = v[3], q3 = v[10];
+  if (flen >= 6) bad = bad || abs(p2 - p1) > lim || abs(q2 - q1) > lim;
+  if (flen >= 8) bad = bad || abs(p3 - p2) > lim || abs(q3 - q2) > lim;
   if (bad) return;
   int flat = 1, flat2 = 1;
   if (fs >= 8) {
@@ -78,8 +75,8 @@ __device__ void deblock_line(uint16_t* b, int s, int level, int sharp, int fs, i
     flat = !f;
   }
   if (fs >= 16) {
-    flat2 = !(abs(b[-5 * s] - p0) > one || abs(b[8 * s] - q0) > one || abs(b[-4 * s] - p0) > one ||
-              abs(b[7 * s] - q0) > one || abs(b[-3 * s] - p0) > one || abs(b[6 * s] - q0) > one);
+    flat2 = !(abs(v[0] - p0) > one || abs(v[13] - q0) > one || abs(v[1] - p0) > one || abs(v[12] - q0) > one ||
+              abs(v[2] - p0) > one || abs(v[11] - q0) > one);
   }
   if (fs == 4 || !flat) {
     const int ps1 = p1 - mid, ps0 = p0 - mid, qs0 = q0 - mid, qs1 = q1 - mid;
@@ -96,11 +93,11 @@ __device__ void deblock_line(uint16_t* b, int s, int level, int sharp, int fs, i
     return;
   }
   if (fs >= 16 && flat2)
-    wide_filter<6, 1, 4>(b, s, maxv);
+    wide_filter<6, 1, 4>(v, b, s, maxv);
   else if (chroma)
-    wide_filter<2, 1, 3>(b, s, maxv);
+    wide_filter<2, 1, 3>(v, b, s, maxv);
   else
-    wide_filter<3, 0, 3>(b, s, maxv);
+    wide_filter<3, 0, 3>(v, b, s, maxv);
 }
 
 /* One thread per line; edges of one direction never share samples. The edge
