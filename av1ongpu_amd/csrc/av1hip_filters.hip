@@ -533,9 +533,25 @@ __device__ void upscale_tile(FArgs a, int t, const int* s_k, uint16_t (*s_row)[k
   const int ws = ((-(1 << 14) + init + x0 * step) >> 14) - 3;
   const int wn = ((-(1 << 14) + init + xl * step) >> 14) + 4 - ws + 1;
   const int rows = min(kUpRows, oh - y0);
-  for (int r = 0; r < rows; ++r)
-    for (int c = threadIdx.x; c < wn; c += 256)
-      s_row[r][c] = src[poff + (size_t)(y0 + r) * stride + clampi(ws + c, 0, maxx)];
+  {
+    /* all of a thread's loads issued before its LDS writes */
+    constexpr int kPer = (kUpSpan + 255) / 256;
+    uint16_t val[kUpRows][kPer];
+#pragma unroll
+    for (int r = 0; r < kUpRows; ++r)
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) {
+        const int c = threadIdx.x + 256 * u;
+        val[r][u] = r < rows && c < wn ? src[poff + (size_t)(y0 + r) * stride + clampi(ws + c, 0, maxx)] : 0;
+      }
+#pragma unroll
+    for (int r = 0; r < kUpRows; ++r)
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) {
+        const int c = threadIdx.x + 256 * u;
+        if (r < rows && c < wn) s_row[r][c] = val[r][u];
+      }
+  }
   __syncthreads();
   const int x = x0 + threadIdx.x;
   if (x > xl) return;
@@ -634,7 +650,21 @@ __device__ void lr_tile(FArgs a, int t, int* s_src, int* s_mid, const int* s_a2)
   c.ph = L.out_h[p];
   c.s0 = j.stripe0;
   c.s1 = j.stripe1;
-  for (Walk256 it(tw + 6); it.r < th + 6; it.next()) s_src[it.r * kLrSrcW + it.q] = c.src(x0 - 3 + it.q, y0 - 3 + it.r);
+  {
+    /* all of a thread's loads issued before its LDS writes */
+    constexpr int kStage = (kLrSrcW * kLrSrcH + 255) / 256;
+    int val[kStage], at[kStage];
+    Walk256 it(tw + 6);
+#pragma unroll
+    for (int u = 0; u < kStage; ++u) {
+      at[u] = it.r < th + 6 ? it.r * kLrSrcW + it.q : -1;
+      val[u] = at[u] >= 0 ? c.src(x0 - 3 + it.q, y0 - 3 + it.r) : 0;
+      it.next();
+    }
+#pragma unroll
+    for (int u = 0; u < kStage; ++u)
+      if (at[u] >= 0) s_src[at[u]] = val[u];
+  }
   __syncthreads();
   uint16_t* dst = a.frame + L.plane_off[p] + (size_t)y0 * c.stride + x0;
   if (j.type == 1) {

@@ -1309,10 +1309,17 @@ __device__ void mc_body(RArgs a, int tn, const int16_t* s_filt, McLds& L_) {
   const AV1H_CMEM McTaskDev& task = cmem(a.plan.mc_task)[tn];
   const int nit = task.count, npairs = 2 * nit;
   {
+    /* all of a thread's loads issued before its LDS writes */
     const int32_t* src = reinterpret_cast<const int32_t*>(a.plan.mc_items + task.first);
     int32_t* dst = reinterpret_cast<int32_t*>(s_it);
     const int n = nit * (int)(sizeof(McItemDev) / 4);
-    for (int i = tid; i < n; i += 256) dst[i] = src[i];
+    constexpr int kPer = (kMcItems * (int)(sizeof(McItemDev) / 4) + 255) / 256;
+    int32_t val[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) val[u] = tid + 256 * u < n ? src[tid + 256 * u] : 0;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u)
+      if (tid + 256 * u < n) dst[tid + 256 * u] = val[u];
   }
   __syncthreads();
   if (tid < npairs) s_moff[tid] = s_it[tid >> 1].r[tid & 1].mid_off;
