@@ -224,7 +224,7 @@ __device__ __forceinline__ int scale_lut(const int* lut, int index, int bd) {
 }
 
 /* 64x32 luma tiles (32x16 chroma at 4:2:0), 256 threads */
-__device__ void grain_tile(FArgs a, int t) {
+__device__ void grain_tile(FArgs a, int t, int* s_lut) {
   LayRef L = a.lay;
   const DevGrain& g = *a.plan.grain;
   const int16_t* tmpl = a.grain_tmpl;
@@ -239,13 +239,21 @@ __device__ void grain_tile(FArgs a, int t) {
   const int chroma = g.cfl || g.num_cb > 0 || g.num_cr > 0;
   const uint16_t* luma = frame + L.plane_off[0];
   const int ls = L.stride[0];
+  /* the three scaling tables in LDS (indexed by each sample's value) */
+  for (int i = threadIdx.x; i < 768; i += 256) s_lut[i] = g.lut[i];
+  __syncthreads();
+  /* Each thread's samples (8 luma, up to 8 chroma per plane) go in three
+   * batches: every load first (samples, luma, noise template), then the
+   * arithmetic, then the stores, so no load waits behind another sample's
+   * store. */
+  constexpr int kPer = 64 * 32 / 256;
   for (int pl = 1; pl < 3 && chroma; ++pl) {
     if (!g.cfl && (pl == 1 ? g.num_cb : g.num_cr) <= 0) continue;
     NoiseCtx nc{tmpl + kLW * kLH + (pl - 1) * cw * ch, offs, cw, g.blocks, L.ss_x, L.ss_y, g.overlap, lo, hi};
     const int pw = L.out_w[pl], ph = L.out_h[pl];
     const int x0 = tx0 >> L.ss_x, y0 = ty0 >> L.ss_y;
     const int tw = 64 >> L.ss_x, th = 32 >> L.ss_y;
-    const int* lut = g.lut + 256 * pl;
+    const int* lut = s_lut + 256 * pl;
     const int mult = pl == 1 ? g.cb_mult : g.cr_mult;
     const int lmult = pl == 1 ? g.cb_luma_mult : g.cr_luma_mult;
     const int offset = pl == 1 ? g.cb_offset : g.cr_offset;
@@ -253,47 +261,78 @@ __device__ void grain_tile(FArgs a, int t) {
     const int st = L.stride[pl];
     const int lw = L.out_w[0];
     const int ltw = 6 - L.ss_x;
-    for (int i = threadIdx.x; i < tw * th; i += 256) {
+    int orig[kPer], avg[kPer], nv[kPer];
+    bool ok[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int i = threadIdx.x + 256 * u;
       const int x = x0 + (i & (tw - 1)), y = y0 + (i >> ltw);
-      if (x >= pw || y >= ph) continue;
-      const int orig = plane[(size_t)y * st + x];
-      const int lx = x << L.ss_x, ly = y << L.ss_y;
-      const int nx = lx + 1 < lw ? lx + 1 : lw - 1;
-      int avg = luma[(size_t)ly * ls + lx];
-      if (L.ss_x) avg = grnd(avg + luma[(size_t)ly * ls + nx], 1);
+      ok[u] = i < tw * th && x < pw && y < ph;
+      orig[u] = avg[u] = 0;
+      if (ok[u]) {
+        orig[u] = plane[(size_t)y * st + x];
+        const int lx = x << L.ss_x, ly = y << L.ss_y;
+        const int nx = lx + 1 < lw ? lx + 1 : lw - 1;
+        const int l0 = luma[(size_t)ly * ls + lx];
+        avg[u] = L.ss_x ? grnd(l0 + luma[(size_t)ly * ls + nx], 1) : l0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int i = threadIdx.x + 256 * u;
+      nv[u] = ok[u] ? nc.N(x0 + (i & (tw - 1)), y0 + (i >> ltw)) : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      if (!ok[u]) continue;
+      const int i = threadIdx.x + 256 * u;
+      const int x = x0 + (i & (tw - 1)), y = y0 + (i >> ltw);
       int index;
       if (g.cfl) {
-        index = avg;
+        index = avg[u];
       } else {
-        const int comb = avg * (lmult - 128) + orig * (mult - 128);
+        const int comb = avg[u] * (lmult - 128) + orig[u] * (mult - 128);
         index = clampi((comb >> 6) + (offset - 256) * (1 << (bd - 8)), 0, maxv);
       }
-      const int n = nc.N(x, y);
-      const int gv = grnd(scale_lut(lut, index, bd) * n, g.scaling_shift);
-      plane[(size_t)y * st + x] = (uint16_t)clampi(orig + gv, g.min_value, g.max_chroma);
+      const int gv = grnd(scale_lut(lut, index, bd) * nv[u], g.scaling_shift);
+      plane[(size_t)y * st + x] = (uint16_t)clampi(orig[u] + gv, g.min_value, g.max_chroma);
     }
   }
   __syncthreads();
   if (g.num_y > 0) {
     NoiseCtx nc{tmpl, offs, kLW, g.blocks, 0, 0, g.overlap, lo, hi};
     uint16_t* plane = frame + L.plane_off[0];
-    for (int i = threadIdx.x; i < 64 * 32; i += 256) {
+    int orig[kPer], nv[kPer];
+    bool ok[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int i = threadIdx.x + 256 * u;
       const int x = tx0 + (i & 63), y = ty0 + (i >> 6);
-      if (x >= L.out_w[0] || y >= L.out_h[0]) continue;
-      const int orig = plane[(size_t)y * ls + x];
-      const int n = nc.N(x, y);
-      const int gv = grnd(scale_lut(g.lut, orig, bd) * n, g.scaling_shift);
-      plane[(size_t)y * ls + x] = (uint16_t)clampi(orig + gv, g.min_value, g.max_luma);
+      ok[u] = x < L.out_w[0] && y < L.out_h[0];
+      orig[u] = ok[u] ? plane[(size_t)y * ls + x] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int i = threadIdx.x + 256 * u;
+      nv[u] = ok[u] ? nc.N(tx0 + (i & 63), ty0 + (i >> 6)) : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      if (!ok[u]) continue;
+      const int i = threadIdx.x + 256 * u;
+      const int gv = grnd(scale_lut(s_lut, orig[u], bd) * nv[u], g.scaling_shift);
+      plane[(size_t)(ty0 + (i >> 6)) * ls + tx0 + (i & 63)] = (uint16_t)clampi(orig[u] + gv, g.min_value, g.max_luma);
     }
   }
 }
 
 __global__ __launch_bounds__(256) void k_grain_apply(BatchDev B) {
+  __shared__ int s_lut[768];
   const AV1H_CMEM int32_t* pre = kernel_prefix(B, kKGrainApply);
   const int total = pre[B.njobs];
   for (int v = blockIdx.x; v < total; v += gridDim.x) {
     const int j = find_job(pre, B.njobs, v);
-    grain_tile(filter_args(B, j), v - pre[j]);
+    grain_tile(filter_args(B, j), v - pre[j], s_lut);
     __syncthreads();
   }
 }
