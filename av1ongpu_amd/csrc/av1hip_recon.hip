@@ -119,12 +119,12 @@ __device__ __forceinline__ void col_pass(const int* tmp, int lane, int w, int tx
   }
   const int ud = itx::flip_ud(tx_type), lr = itx::flip_lr(tx_type);
   const int xx = lr ? w - 1 - x : x;
-  /* eight rows at a time: their current samples (an inter block's
+  /* up to 32 rows at a time: their current samples (an inter block's
    * prediction) are all loaded before any is stored, so the adds do not wait
    * on one load after another */
+  constexpr int kB = M < 32 ? M : 32;
 #pragma unroll
-  for (int i0 = 0; i0 < M; i0 += 8) {
-    constexpr int kB = M < 8 ? M : 8;
+  for (int i0 = 0; i0 < M; i0 += kB) {
     int cur[kB];
 #pragma unroll
     for (int u = 0; u < kB; ++u) cur[u] = out.load(xx, ud ? M - 1 - (i0 + u) : i0 + u);
@@ -204,23 +204,42 @@ __device__ __forceinline__ void itx_group(RArgs a, const int32_t* list, int n, i
         a.frame[o] = (uint16_t)clampi(cur + v, 0, maxv);
     }
   } out{a, off, stride, maxv, to_res};
-  auto out1 = [&](int xx, int yy, int v) {
-    const int64_t o = off + (int64_t)yy * stride + xx;
-    if (to_res)
-      a.res[o] = (int16_t)clampi(v, -32768, 32767);
-    else
-      a.frame[o] = (uint16_t)clampi((int)a.frame[o] + v, 0, maxv);
-  };
   const bool coded = live && !(tx.flags & kTxRaw);
   const int lw = ilog2(w), lh = ilog2(h);
   if (live && !coded) {
-    /* raw residual (av1gpu_kernels.h:933-937) */
-    for (int i = l; i < w * h; i += G) out1(i & (w - 1), i >> lw, a.plan.residual[tx.coef + i]);
+    /* raw residual (av1gpu_kernels.h:933-937), eight samples per lane at a
+     * time: residuals and current samples loaded before any store */
+    const int n = w * h;
+    for (int i0 = l; i0 < n; i0 += 8 * G) {
+      int r[8], cur[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int i = i0 + u * G;
+        r[u] = i < n ? a.plan.residual[tx.coef + i] : 0;
+        cur[u] = i < n ? out.load(i & (w - 1), i >> lw) : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int i = i0 + u * G;
+        if (i < n) out.store(i & (w - 1), i >> lw, cur[u], r[u]);
+      }
+    }
   }
   if (coded) {
+    /* every coefficient a lane stages is loaded before the first LDS write:
+     * one memory round trip instead of one per coefficient row */
+    constexpr int kC = G < 32 ? G : 32, kIt = kC * kC / G;
     const int cs = w < 32 ? w : 32, cr = h < 32 ? h : 32;
-    const int lcs = ilog2(cs), st = w + 1;
-    for (int i = l; i < cs * cr; i += G) tmp[(i >> lcs) * st + (i & (cs - 1))] = a.plan.coef[tx.coef + i];
+    const int lcs = ilog2(cs), st = w + 1, n = cs * cr;
+    const int32_t* src = a.plan.coef + tx.coef;
+    int v[kIt];
+#pragma unroll
+    for (int k = 0; k < kIt; ++k) v[k] = l + k * G < n ? src[l + k * G] : 0;
+#pragma unroll
+    for (int k = 0; k < kIt; ++k) {
+      const int i = l + k * G;
+      if (i < n) tmp[(i >> lcs) * st + (i & (cs - 1))] = v[k];
+    }
   }
   __syncthreads();
   const int lossless = (tx.flags & kTxLossless) ? 1 : 0;
