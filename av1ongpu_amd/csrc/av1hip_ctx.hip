@@ -1556,7 +1556,7 @@ void default_tuning_impl(Tuning* t) {
   t->v[kKLr] = 8;
   t->v[kKHash] = 8;
   t->v[kKGrainTemplate] = 1;
-  for (int k : {kKCopyLrCurr, kKCopyCdefSrc, kKCopyUpSrc, kKCopyLrCdef, kKCopyStore}) t->v[k] = 8;
+  for (int k : {kKCopyLrCurr, kKCopyCdefSrc, kKCopyUpSrc, kKCopyStore}) t->v[k] = 8;
   t->v[kTuneQueue] = 6;
   t->v[kTuneCdefBatch] = 1;
 }
@@ -1591,10 +1591,9 @@ int kernel_groups(const FrameJob& fj, int k, int cdef_batch) {
       const int64_t n = (int64_t)L.out_w[0] * L.out_h[0] + 2 * (int64_t)L.out_w[1] * L.out_h[1];
       return f.hash ? (int)((n + kHashChunk - 1) / kHashChunk) : 0;
     }
-    case kKCopyLrCurr: return f.uses_lr ? copy_groups(L) : 0;
-    case kKCopyCdefSrc: return p.n_cdef > 0 && !f.uses_lr ? copy_groups(L) : 0;
-    case kKCopyUpSrc: return f.upscaled ? copy_groups(L) : 0;
-    case kKCopyLrCdef: return f.uses_lr ? copy_groups(L) : 0;
+    case kKCopyLrCurr: return f.uses_lr || (f.upscaled && p.n_cdef > 0) ? copy_groups(L) : 0;
+    case kKCopyCdefSrc: return p.n_cdef > 0 && !f.uses_lr && !f.upscaled ? copy_groups(L) : 0;
+    case kKCopyUpSrc: return f.upscaled && !f.uses_lr && p.n_cdef == 0 ? copy_groups(L) : 0;
     case kKCopyStore: return f.store ? copy_groups(L) : 0;
     default: return 0;
   }
@@ -1702,7 +1701,21 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
     fa.lr_curr = c->lr_curr;
     fa.lr_cdef = c->lr_cdef;
     fa.lr_prev_up = c->lr_prev_up;
-    fa.cdef_src = plan->uses_lr ? c->lr_curr : c->scratch;
+    /* The pictures between the filters, with no copy that a kernel's second
+     * store can replace: the deblocked snapshot (kKCopyLrCurr) goes to
+     * lr_curr, and also to scratch when superres follows (CDEF writes its
+     * output there, superres reads it) or to lr_cdef when restoration reads
+     * the CDEF output at the coded size (CDEF writes there too); the
+     * upscaled frame also goes to lr_cdef when restoration follows. Without
+     * restoration or superres, CDEF reads a snapshot in scratch
+     * (kKCopyCdefSrc) and writes the frame; superres without CDEF and
+     * restoration reads a copy of the frame in scratch (kKCopyUpSrc). */
+    const bool snap = plan->uses_lr || plan->upscaled;
+    fa.cdef_src = snap ? c->lr_curr : c->scratch;
+    fa.cdef_dst = plan->upscaled ? c->scratch : c->frame;
+    fa.cdef_dst2 = plan->uses_lr && !plan->upscaled ? c->lr_cdef : nullptr;
+    fa.snap2 = plan->upscaled ? c->scratch : (plan->uses_lr ? c->lr_cdef : nullptr);
+    fa.up_dst2 = plan->uses_lr ? c->lr_cdef : nullptr;
     fa.cdef_dir = c->cdef_dir;
     fa.cdef_dir_w = L.grid_w[0] >> 3;
     fa.sharpness = plan->hdr.sharpness;
@@ -1862,7 +1875,6 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
     SEQ(stage(AV1H_STAGE_CDEF, {kKCdefDir, kKCdefLuma, kKCdefChroma}));
     SEQ(run(kKCopyUpSrc, s));
     SEQ(stage(AV1H_STAGE_SUPERRES, {kKUpscale}));
-    SEQ(run(kKCopyLrCdef, s));
     SEQ(stage(AV1H_STAGE_LR, {kKLr}));
     SEQ(stage(AV1H_STAGE_STORE, {kKCopyStore}));
     if (side_grain) SEQ(hipStreamWaitEvent(s, lead->ev_side_done, 0));
@@ -2426,7 +2438,7 @@ const char* av1h_tune_name(int i) {
       "mc", "inter_special", "itx_inter", "itx_intra", "itx_inter_large", "itx_intra_large", "deblock_v",
       "deblock_h", "cdef_dir", "cdef_luma",
       "cdef_chroma", "upscale", "lr", "grain_template", "grain_apply", "nv12", "hash", "copy_lr_curr",
-      "copy_cdef_src", "copy_upscale_src", "copy_lr_cdef", "copy_store", "queue", "cdef_blocks_per_thread"};
+      "copy_cdef_src", "copy_upscale_src", "copy_store", "queue", "cdef_blocks_per_thread"};
   return i >= 0 && i < kTuneParams ? names[i] : nullptr;
 }
 

@@ -317,7 +317,7 @@ __device__ __forceinline__ bool cdef_sample(FArgs a, int first, int bi, int lane
       pri = var ? ((pri * (4 + vs) + 8) >> 4) : 0;
     }
   }
-  if (pri == 0 && sec == 0) return false; /* the frame already holds the snapshot's value */
+  if (pri == 0 && sec == 0) return false; /* the destination already holds the snapshot's value */
   const uint16_t* src = a.cdef_src + L.plane_off[p] + (size_t)(c.y + r) * st + c.x + k;
   const int ts = (pri >> cs) & 1;
   const int pt0 = ts ? 3 : 4, pt1 = ts ? 3 : 2;
@@ -473,7 +473,10 @@ __device__ __forceinline__ void cdef_group(FArgs a, int first, int count, int gr
   }
 #pragma unroll
   for (int u = 0; u < kCdefMaxBatch; ++u)
-    if (ok[u]) a.frame[off[u]] = (uint16_t)val[u];
+    if (ok[u]) {
+      a.cdef_dst[off[u]] = (uint16_t)val[u];
+      if (a.cdef_dst2) a.cdef_dst2[off[u]] = (uint16_t)val[u];
+    }
 }
 
 /* luma blocks (plane 0) or chroma blocks (plane 1) of every frame; a chroma
@@ -509,9 +512,10 @@ __global__ __launch_bounds__(256) void k_cdef(BatchDev B, int chroma, int batch)
 /* 256 x 4 output samples of one plane of one picture per workgroup: the
  * source span of the 256 columns (at most 264 samples when upscaling) is
  * staged in LDS per row, clamped to the coded width as each tap is
- * (av1gpu_kernels.h:2387-2402). Picture 0 is the frame (from its copy in
- * `scratch`), picture 1 the deblocked picture loop restoration reads
- * (lr_curr into lr_prev_up). */
+ * (av1gpu_kernels.h:2387-2402). Picture 0 is the frame (from the CDEF
+ * output in `scratch`; also stored to lr_cdef when loop restoration reads
+ * it), picture 1 the deblocked picture loop restoration reads (lr_curr into
+ * lr_prev_up). */
 constexpr int kUpW = 256, kUpRows = 4, kUpSpan = 272;
 
 __device__ void upscale_tile(FArgs a, int t, const int* s_k, uint16_t (*s_row)[kUpSpan]) {
@@ -558,11 +562,15 @@ __device__ void upscale_tile(FArgs a, int t, const int* s_k, uint16_t (*s_row)[k
   const int sx = -(1 << 14) + init + x * step;
   const int o = (sx >> 14) - 3 - ws;
   const int* k = s_k + ((sx & ((1 << 14) - 1)) >> 8) * 8;
+  uint16_t* dst2 = pic ? nullptr : a.up_dst2;
   for (int r = 0; r < rows; ++r) {
     int sum = 0;
 #pragma unroll
     for (int q = 0; q < 8; ++q) sum += s_row[r][o + q] * k[q];
-    dst[poff + (size_t)(y0 + r) * stride + x] = (uint16_t)clampi((sum + 64) >> 7, 0, a.maxv);
+    const uint16_t v = (uint16_t)clampi((sum + 64) >> 7, 0, a.maxv);
+    const size_t at = poff + (size_t)(y0 + r) * stride + x;
+    dst[at] = v;
+    if (dst2) dst2[at] = v;
   }
 }
 
@@ -797,7 +805,6 @@ __device__ __forceinline__ void copy_kind(FArgs a, int k, const uint4** src, uin
     case kKCopyLrCurr: *dst = reinterpret_cast<uint4*>(a.lr_curr); break;
     case kKCopyCdefSrc:
     case kKCopyUpSrc: *dst = reinterpret_cast<uint4*>(a.scratch); break;
-    case kKCopyLrCdef: *dst = reinterpret_cast<uint4*>(a.lr_cdef); break;
     default: *dst = reinterpret_cast<uint4*>(a.store); break;
   }
 }
@@ -812,12 +819,18 @@ __global__ __launch_bounds__(256) void k_copy(BatchDev B, int k) {
     const uint4* src;
     uint4* dst;
     copy_kind(a, k, &src, &dst);
+    /* the deblocked snapshot may have a second destination */
+    uint4* dst2 = k == kKCopyLrCurr ? reinterpret_cast<uint4*>(a.snap2) : nullptr;
     const int64_t n = a.lay.size * 2 / 16;
     const int64_t base = (int64_t)(v - pre[j]) * (kCopyChunk / 16);
 #pragma unroll
     for (int i = 0; i < kCopyChunk / 16 / 256; ++i) {
       const int64_t e = base + i * 256 + threadIdx.x;
-      if (e < n) dst[e] = src[e];
+      if (e < n) {
+        const uint4 x = src[e];
+        dst[e] = x;
+        if (dst2) dst2[e] = x;
+      }
     }
   }
 }
@@ -987,7 +1000,6 @@ hipError_t launch_filter_kernel(const BatchDev& b, int k, int total, int cap, in
     case kKCopyLrCurr:
     case kKCopyCdefSrc:
     case kKCopyUpSrc:
-    case kKCopyLrCdef:
     case kKCopyStore: hipLaunchKernelGGL(k_copy, grid, blk, 0, s, b, k); break;
     default: return hipErrorInvalidValue;
   }

@@ -243,6 +243,10 @@ struct FilterArgs {
   uint16_t* lr_cdef;
   uint16_t* lr_prev_up;
   const uint16_t* cdef_src; /* the deblocked picture CDEF reads: lr_curr or scratch */
+  uint16_t* cdef_dst;   /* where CDEF writes: the frame, or scratch (superres reads it) */
+  uint16_t* cdef_dst2;  /* a second destination of CDEF's output (lr_cdef), or null */
+  uint16_t* snap2;      /* a second destination of the deblocked snapshot, or null */
+  uint16_t* up_dst2;    /* a second destination of the upscaled frame (lr_cdef), or null */
   int32_t* cdef_dir;    /* (direction, variance) per luma 8x8 block */
   int cdef_dir_w;       /* luma 8x8 blocks per row                  */
   int sharpness;
@@ -269,7 +273,7 @@ enum KernelId {
   kKMc, kKSpecial, kKItxInter, kKItxIntra, kKItxInterL, kKItxIntraL, kKDeblockV, kKDeblockH, kKCdefDir,
   kKCdefLuma, kKCdefChroma,
   kKUpscale, kKLr, kKGrainTemplate, kKGrainApply, kKNv12, kKHash,
-  kKCopyLrCurr, kKCopyCdefSrc, kKCopyUpSrc, kKCopyLrCdef, kKCopyStore,
+  kKCopyLrCurr, kKCopyCdefSrc, kKCopyUpSrc, kKCopyStore,
   kKernels
 };
 

@@ -65,3 +65,12 @@ def test_stage_isolated(stage):
 def test_intra_only_key_frames():
     """Key frames only: the whole frame goes through the dependency queue."""
     _compare(pkg.Clip("c1_mini", 11, key_interval=1, frames=3))
+
+
+@pytest.mark.parametrize("tools", [dict(cdef=1, lr=0), dict(cdef=1, lr=2), dict(cdef=0, lr=2)])
+def test_filters_without_superres(tools):
+    """CDEF and loop restoration at the coded size: the pictures between the
+    filters take another route than with superres."""
+    kw = dict(coded_width=0, deblock_lo=8, deblock_hi=32, grain=0)
+    kw.update(tools)
+    _compare(pkg.Clip("c2_mini", 8, **kw))
