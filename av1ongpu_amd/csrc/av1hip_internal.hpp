@@ -364,6 +364,26 @@ __device__ __forceinline__ const AV1H_CMEM int32_t* kernel_prefix(const BatchDev
   return cmem(B.prefix) + k * (B.njobs + 1);
 }
 __device__ __forceinline__ RArgs recon_args(const BatchDev& B, int j) { return cmem(B.jobs)[j].r; }
+
+/* One layout field of the three planes in scalar registers. Indexed by a
+ * plane that differs between lanes, a field in constant memory becomes a
+ * vector load on the dependent path of every sample; read once here, the
+ * planes' values are selected instead. */
+template <typename T>
+struct PlaneVals {
+  T v0, v1, v2;
+  __device__ __forceinline__ T operator[](int p) const { return p == 0 ? v0 : (p == 1 ? v1 : v2); }
+};
+__device__ __forceinline__ int32_t sreg(int32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int64_t sreg(int64_t v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+template <typename T>
+__device__ __forceinline__ PlaneVals<T> plane_vals(const AV1H_CMEM T* f) {
+  return {sreg(f[0]), sreg(f[1]), sreg(f[2])};
+}
 __device__ __forceinline__ FArgs filter_args(const BatchDev& B, int j) { return cmem(B.jobs)[j].f; }
 
 /* Kernel k over every frame of the batch, `total` virtual workgroups; `cap`

@@ -182,9 +182,8 @@ __device__ __forceinline__ void itx_group(RArgs a, const int32_t* list, int n, i
   }
   int* tmp = s_tmp + grp * G * (G + 1);
   const int w = tx.w, h = tx.h;
-  const int stride = tx.plane == 0 ? a.lay.stride[0] : (tx.plane == 1 ? a.lay.stride[1] : a.lay.stride[2]);
-  const int64_t poff =
-      tx.plane == 0 ? a.lay.plane_off[0] : (tx.plane == 1 ? a.lay.plane_off[1] : a.lay.plane_off[2]);
+  const int stride = plane_vals(a.lay.stride)[tx.plane];
+  const int64_t poff = plane_vals(a.lay.plane_off)[tx.plane];
   const int64_t off = poff + (int64_t)tx.y * stride + tx.x;
   const bool to_res = (tx.flags & kTxToResidual) != 0;
   const int maxv = a.maxv;
@@ -1350,6 +1349,8 @@ __device__ void mc_body(RArgs a, int tn, const int16_t* s_filt, McLds& L_) {
   __syncthreads();
   const int bd = a.lay.bit_depth;
   const int round0 = bd >= 12 ? 5 : 3;
+  const PlaneVals<int> strides = plane_vals(a.lay.stride);
+  const PlaneVals<int64_t> poffs = plane_vals(a.lay.plane_off);
   /* horizontal pass over every reference row an item needs, taps read from
    * the reference (clamped to it) */
   int kprev = 0;
@@ -1366,9 +1367,7 @@ __device__ void mc_body(RArgs a, int tn, const int16_t* s_filt, McLds& L_) {
     int cx, fx;
     P.col(x, &cx, &fx);
     const int y = clampi(q.wy0 + rr, 0, it.ref_h - 1);
-    const int stride = p == 0 ? a.lay.stride[0] : (p == 1 ? a.lay.stride[1] : a.lay.stride[2]);
-    const int64_t poff = p == 0 ? a.lay.plane_off[0] : (p == 1 ? a.lay.plane_off[1] : a.lay.plane_off[2]);
-    const uint16_t* row = a.pool + (size_t)q.slot * a.slot_size + poff + (size_t)y * stride;
+    const uint16_t* row = a.pool + (size_t)q.slot * a.slot_size + poffs[p] + (size_t)y * strides[p];
     int hk[8];
     mc_taps(s_filt + it.hk + fx * 8, hk);
     int sum = 0;
@@ -1383,6 +1382,9 @@ __device__ void mc_body(RArgs a, int tn, const int16_t* s_filt, McLds& L_) {
   }
   __syncthreads();
   /* vertical pass and blend, one thread per output sample */
+  const PlaneVals<int> cshift = plane_vals(a.lay.cell_shift), cwidth = plane_vals(a.lay.cells_w),
+                       cheight = plane_vals(a.lay.cells_h);
+  const PlaneVals<int64_t> coffs = plane_vals(a.lay.cell_off);
   int iprev = 0;
   for (int si = tid; si < task.samples; si += 256) {
     const int item = iprev = find_prefix(s_px, nit, si, iprev);
@@ -1418,18 +1420,15 @@ __device__ void mc_body(RArgs a, int tn, const int16_t* s_filt, McLds& L_) {
         out = (it.w0 * v[0] + it.w1 * v[1] + (1 << (3 + post))) >> (4 + post);
     }
     const int p = it.plane;
-    const int stride = p == 0 ? a.lay.stride[0] : (p == 1 ? a.lay.stride[1] : a.lay.stride[2]);
-    const int64_t poff = p == 0 ? a.lay.plane_off[0] : (p == 1 ? a.lay.plane_off[1] : a.lay.plane_off[2]);
     const int ax = it.x + x, ay = it.y + y;
-    a.frame[poff + (int64_t)ay * stride + ax] = (uint16_t)clampi(out, 0, a.maxv);
+    a.frame[poffs[p] + (int64_t)ay * strides[p] + ax] = (uint16_t)clampi(out, 0, a.maxv);
     /* the block's dependency cells, each by the thread of its first sample
      * in the block: final once the inter transforms after this launch have
      * run, which the queue launch follows */
-    const int sh = a.lay.cell_shift[p], cm = (1 << sh) - 1;
+    const int sh = cshift[p], cm = (1 << sh) - 1;
     if (((ax & cm) == 0 || x == 0) && ((ay & cm) == 0 || y == 0)) {
       const int cx = ax >> sh, cy = ay >> sh;
-      if (cx < a.lay.cells_w[p] && cy < a.lay.cells_h[p])
-        a.cells[a.lay.cell_off[p] + (int64_t)cy * a.lay.cells_w[p] + cx] = a.epoch;
+      if (cx < cwidth[p] && cy < cheight[p]) a.cells[coffs[p] + (int64_t)cy * cwidth[p] + cx] = a.epoch;
     }
   }
 }
