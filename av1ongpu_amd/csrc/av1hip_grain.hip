@@ -41,11 +41,12 @@ __device__ __forceinline__ unsigned mat_apply(const uint16_t* cols, unsigned v) 
   return r;
 }
 
-/* white noise row `y` of a w wide template (av1gpu_kernels.h:3070-3079) */
-__device__ void noise_row(int16_t* t, int w, int y, unsigned seed, const uint16_t* jump, int shift) {
+/* white noise row `y` of a w wide template (av1gpu_kernels.h:3070-3079),
+ * the Gaussian sequence `gs` staged in LDS */
+__device__ void noise_row(int16_t* t, int w, int y, unsigned seed, const uint16_t* jump, int shift,
+                          const int16_t* gs) {
   unsigned st = seed;
   for (int k = 0; k < y; ++k) st = mat_apply(jump, st);
-  const int* gs = SPEC(AV1T_OFF_GAUSSIAN);
   for (int x = 0; x < w; ++x) {
     st = lfsr_step(st);
     t[y * w + x] = (int16_t)grnd(gs[(st >> 5) & 2047], shift);
@@ -62,7 +63,7 @@ __device__ __forceinline__ void wave_lds_sync() { __asm__ volatile("s_waitcnt lg
  * which is exactly the lag-3 filter's reach into the previous row, so no two
  * waves ever synchronise. The coefficients stay in registers. */
 template <int LAG>
-__device__ void ar_plane(int16_t* pl, int w, int h, const int* coeffs, const int16_t* luma, int ss_x, int ss_y,
+__device__ __forceinline__ void ar_plane(int16_t* pl, int w, int h, const int* coeffs, const int16_t* luma, int ss_x, int ss_y,
                          int shift, int lo, int hi) {
   constexpr int kTaps = (2 * LAG + 1) * LAG + LAG;
   int c[kTaps + 1];
@@ -101,7 +102,8 @@ __device__ void ar_plane(int16_t* pl, int w, int h, const int* coeffs, const int
   }
 }
 
-__device__ void ar_dispatch(int lag, int16_t* pl, int w, int h, const int* coeffs, const int16_t* luma, int ss_x,
+/* inlined into its callers so that the template pointers stay LDS accesses */
+__device__ __forceinline__ void ar_dispatch(int lag, int16_t* pl, int w, int h, const int* coeffs, const int16_t* luma, int ss_x,
                             int ss_y, int shift, int lo, int hi) {
   switch (lag) {
     case 0: ar_plane<0>(pl, w, h, coeffs, luma, ss_x, ss_y, shift, lo, hi); break;
@@ -119,7 +121,8 @@ constexpr int kTmplThreads = 192; /* three waves: luma, cb, cr */
  * waves 1-2 draw the offsets; cb on wave 1 and cr on wave 2 (both read the
  * finished luma template) while wave 0 stores luma. */
 __device__ void grain_template(const DevGrain& g, const uint16_t* jm82, const uint16_t* jm44, int ss_x, int ss_y,
-                               int16_t* tmpl, uint8_t* offs, int16_t* s_l, int16_t (*s_c)[kLW * kLH]) {
+                               int16_t* tmpl, uint8_t* offs, int16_t* s_l, int16_t (*s_c)[kLW * kLH],
+                               const int16_t* s_gauss) {
   const int tid = threadIdx.x, wave = tid >> 6;
   const int bd = g.bit_depth;
   const int lo = -(128 << (bd - 8)), hi = (256 << (bd - 8)) - 1 - (128 << (bd - 8));
@@ -131,13 +134,13 @@ __device__ void grain_template(const DevGrain& g, const uint16_t* jm82, const ui
   for (int r = tid; r < kLH + 2 * ch; r += kTmplThreads) {
     if (r < kLH) {
       if (g.num_y > 0)
-        noise_row(s_l, kLW, r, seed, jm82, g.scale_shift);
+        noise_row(s_l, kLW, r, seed, jm82, g.scale_shift, s_gauss);
       else
         for (int x = 0; x < kLW; ++x) s_l[r * kLW + x] = 0;
     } else if (chroma) {
       const int pl = (r - kLH) / ch, y = (r - kLH) - pl * ch;
       if (pl == 0 ? (g.num_cb > 0 || g.cfl) : (g.num_cr > 0 || g.cfl))
-        noise_row(s_c[pl], cw, y, seed ^ (pl == 0 ? 0xb524u : 0x49d8u), jc, g.scale_shift);
+        noise_row(s_c[pl], cw, y, seed ^ (pl == 0 ? 0xb524u : 0x49d8u), jc, g.scale_shift, s_gauss);
       else
         for (int x = 0; x < cw; ++x) s_c[pl][y * cw + x] = 0;
     }
@@ -172,11 +175,31 @@ __device__ void grain_template(const DevGrain& g, const uint16_t* jm82, const ui
 __global__ __launch_bounds__(kTmplThreads) void k_grain_template(BatchDev B) {
   __shared__ int16_t s_l[kLW * kLH];
   __shared__ int16_t s_c[2][kLW * kLH];
+  __shared__ int16_t s_gauss[2048];
+  {
+    /* the Gaussian sequence (12 bit values), every load issued before the
+     * LDS writes */
+    constexpr int kPer = (2048 + kTmplThreads - 1) / kTmplThreads;
+    const int* gs = SPEC(AV1T_OFF_GAUSSIAN);
+    int v[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int i = threadIdx.x + kTmplThreads * u;
+      v[u] = i < 2048 ? gs[i] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int i = threadIdx.x + kTmplThreads * u;
+      if (i < 2048) s_gauss[i] = (int16_t)v[u];
+    }
+  }
+  __syncthreads();
   const AV1H_CMEM int32_t* pre = kernel_prefix(B, kKGrainTemplate);
   const int total = pre[B.njobs];
   for (int v = blockIdx.x; v < total; v += gridDim.x) {
     FArgs a = filter_args(B, find_job(pre, B.njobs, v));
-    grain_template(*a.plan.grain, B.jump82, B.jump44, a.lay.ss_x, a.lay.ss_y, a.grain_tmpl, a.grain_offs, s_l, s_c);
+    grain_template(*a.plan.grain, B.jump82, B.jump44, a.lay.ss_x, a.lay.ss_y, a.grain_tmpl, a.grain_offs, s_l, s_c,
+                   s_gauss);
     __syncthreads();
   }
 }
