@@ -1250,6 +1250,13 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
     if (g.num_y_points < 0 || g.num_y_points > 16 || g.num_cb_points < 0 || g.num_cb_points > 16 ||
         g.num_cr_points < 0 || g.num_cr_points > 16)
       return set_err(c, AV1H_ERR_INVALID, "grain points");
+    /* the auto-regressive coefficients are coded as 8 bit values minus 128
+     * (src/frontend/headers.cpp:716-725); the template filter multiplies
+     * them with 24 bit products */
+    for (int k = 0; k < 25; ++k)
+      if ((k < 24 && (g.ar_coeffs_y[k] < -128 || g.ar_coeffs_y[k] > 127)) || g.ar_coeffs_cb[k] < -128 ||
+          g.ar_coeffs_cb[k] > 127 || g.ar_coeffs_cr[k] < -128 || g.ar_coeffs_cr[k] > 127)
+        return set_err(c, AV1H_ERR_INVALID, "grain auto-regressive coefficient out of range");
     d.seed = g.seed;
     d.bit_depth = hdr.bit_depth;
     d.num_y = g.num_y_points;

@@ -85,7 +85,7 @@ __device__ __forceinline__ void ar_plane(int16_t* pl, int w, int h, const int* c
 #pragma unroll
           for (int dc = -LAG; dc <= LAG; ++dc) {
             if (dr == 0 && dc >= 0) break;
-            sum += pl[(y + dr) * w + x + dc] * c[pos];
+            sum += __mul24(pl[(y + dr) * w + x + dc], c[pos]); /* 16 x 8 bit */
             ++pos;
           }
         if (luma) {
@@ -93,7 +93,7 @@ __device__ __forceinline__ void ar_plane(int16_t* pl, int w, int h, const int* c
           int acc = 0;
           for (int i = 0; i <= ss_y; ++i)
             for (int k = 0; k <= ss_x; ++k) acc += luma[(ly + i) * kLW + lx + k];
-          sum += grnd(acc, ss_x + ss_y) * c[kTaps];
+          sum += __mul24(grnd(acc, ss_x + ss_y), c[kTaps]);
         }
         pl[y * w + x] = (int16_t)clampi(pl[y * w + x] + grnd(sum, shift), lo, hi);
       }
