@@ -516,19 +516,33 @@ __global__ __launch_bounds__(256) void k_cdef(BatchDev B, int chroma, int batch)
  * output in `scratch`; also stored to lr_cdef when loop restoration reads
  * it), picture 1 the deblocked picture loop restoration reads (lr_curr into
  * lr_prev_up). */
-constexpr int kUpW = 256, kUpRows = 4, kUpSpan = 272;
+constexpr int kUpW = 256, kUpRows = 8, kUpSpan = 272;
+
+/* output tiles of a plane of ow x oh display samples */
+__host__ __device__ __forceinline__ int up_tiles(int ow, int oh) {
+  return (ow + kUpW - 1) / kUpW * ((oh + kUpRows - 1) / kUpRows);
+}
 
 __device__ void upscale_tile(FArgs a, int t, const int* s_k, uint16_t (*s_row)[kUpSpan]) {
   LayRef L = a.lay;
-  const int tx = (L.out_w[0] + kUpW - 1) / kUpW, ty = (L.out_h[0] + kUpRows - 1) / kUpRows;
-  const int z = t / (tx * ty), rem = t - z * tx * ty;
+  const int t0 = up_tiles(L.out_w[0], L.out_h[0]), t1 = up_tiles(L.out_w[1], L.out_h[1]);
+  const int tp = t0 + t1 + up_tiles(L.out_w[2], L.out_h[2]);
+  const int pic = t >= tp;
+  int rem = t - pic * tp, p = 0;
+  if (rem >= t0) {
+    rem -= t0;
+    p = 1;
+    if (rem >= t1) {
+      rem -= t1;
+      p = 2;
+    }
+  }
+  const int ow = L.out_w[p], oh = L.out_h[p];
+  const int tx = (ow + kUpW - 1) / kUpW;
   const int by = rem / tx, bx = rem - by * tx;
-  const int pic = z / 3, p = z - 3 * pic;
   const uint16_t* src = pic ? a.lr_curr : a.scratch;
   uint16_t* dst = pic ? a.lr_prev_up : a.frame;
-  const int ow = L.out_w[p], oh = L.out_h[p];
   const int x0 = bx * kUpW, y0 = by * kUpRows;
-  if (x0 >= ow || y0 >= oh) return;
   const int step = a.up_step[p], init = a.up_initial[p];
   const int maxx = L.plane_w[p] - 1;
   const int stride = L.stride[p];
@@ -982,7 +996,8 @@ void superres_params(int dw, int uw, int* step, int* initial) {
 }
 
 int upscale_tiles(const Layout& L, int pictures) {
-  return (L.out_w[0] + kUpW - 1) / kUpW * ((L.out_h[0] + kUpRows - 1) / kUpRows) * 3 * pictures;
+  return (up_tiles(L.out_w[0], L.out_h[0]) + up_tiles(L.out_w[1], L.out_h[1]) + up_tiles(L.out_w[2], L.out_h[2])) *
+         pictures;
 }
 
 hipError_t launch_filter_kernel(const BatchDev& b, int k, int total, int cap, int cdef_batch, hipStream_t s) {
