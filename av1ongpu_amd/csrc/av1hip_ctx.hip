@@ -179,7 +179,7 @@ struct Ctx {
    * (valid while batch_dev stays where it is) */
   bool graphs = true;
   Tuning tune;                   /* launch shapes of the batches it leads */
-  std::vector<std::pair<int, hipGraphExec_t>> graph_exec;
+  std::vector<std::pair<uint64_t, hipGraphExec_t>> graph_exec; /* (kernel set << 8 | frames, graph) */
   bool stage_hit[AV1H_STAGES] = {};
   double stage_ms[AV1H_STAGES] = {};
   int64_t stage_n[AV1H_STAGES] = {};
@@ -1803,12 +1803,21 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
   lead->trace_plan = lead->trace_on && n == 1 ? js[0].plan : nullptr;
 
   int launches = 0;
-  /* Captured launches: every kernel at its full grid, counts read on the
-   * device, so one graph per frame count serves every batch of that size.
-   * Stage timing and the queue trace take the eager path. */
+  /* Captured launches: every kernel with work in the batch at its full grid,
+   * counts read on the device, so one graph per frame count and set of
+   * kernels with work serves every batch of that shape (a kernel without
+   * work would still dispatch its whole grid). Stage timing and the queue
+   * trace take the eager path. */
   const bool use_graph = lead->graphs && !lead->timing && !lead->trace_on;
+  uint64_t live = 0; /* bit k: kernel k has work; bit kKernels: the queue */
+  for (int k = 0; k < kKernels; ++k)
+    if (total_of(k) > 0) live |= 1ull << k;
+  if (nq > 0) live |= 1ull << kKernels;
+  if (any_hash) live |= 1ull << kKHash;
+  static_assert(kKernels < 64, "kernel set mask");
+  auto has = [&](int k) { return ((live >> k) & 1) != 0; };
   auto run = [&](int k, hipStream_t st) -> hipError_t {
-    const int total = use_graph ? kAnyTotal : total_of(k);
+    const int total = use_graph ? (has(k) ? kAnyTotal : 0) : total_of(k);
     if (total <= 0) return hipSuccess;
     ++launches;
     const int cap = lead->cus * lead->tune.v[k];
@@ -1825,8 +1834,8 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
     }
   };
   auto stage = [&](int st, std::initializer_list<int> ks) -> hipError_t {
-    bool any = use_graph;
-    for (int k : ks) any = any || total_of(k) > 0;
+    bool any = false;
+    for (int k : ks) any = any || has(k);
     if (!any) return hipSuccess;
     if (lead->timing) (void)hipEventRecord(lead->sev[2 * st], s);
     for (int k : ks) {
@@ -1842,8 +1851,8 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
   /* The second stream takes what does not depend on the prediction: the
    * transforms of intra blocks (the queue waits for them) and the grain
    * templates (the grain blend waits for them). */
-  const bool side_itx = use_graph || total_of(kKItxIntra) > 0 || total_of(kKItxIntraL) > 0;
-  const bool side_grain = use_graph || total_of(kKGrainTemplate) > 0;
+  const bool side_itx = has(kKItxIntra) || has(kKItxIntraL);
+  const bool side_grain = has(kKGrainTemplate);
   auto sequence = [&]() -> hipError_t {
     hipError_t e = hipSuccess;
 #define SEQ(call)        \
@@ -1867,7 +1876,7 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
     if (!use_graph) SEQ(hipEventRecord(lead->ev[0], s));
     SEQ(stage(AV1H_STAGE_INTER, {kKMc, kKSpecial, kKItxInterL, kKItxInter}));
     if (side_itx) SEQ(hipStreamWaitEvent(s, lead->ev_itx_done, 0));
-    if (use_graph || nq > 0) {
+    if (has(kKernels)) {
       if (lead->timing) (void)hipEventRecord(lead->sev[2 * AV1H_STAGE_QUEUE], s);
       SEQ(launch_queue(B, use_graph ? -1 : (int)nq, lead->cus * lead->tune.v[kTuneQueue], s, &launches));
       if (lead->timing) {
@@ -1887,7 +1896,7 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
     if (side_grain) SEQ(hipStreamWaitEvent(s, lead->ev_side_done, 0));
     SEQ(stage(AV1H_STAGE_GRAIN, {kKGrainApply}));
     SEQ(run(kKNv12, s));
-    if (use_graph || any_hash) {
+    if (any_hash) {
       SEQ(launch_zero64(lead->hash_dev, n, s));
       SEQ(run(kKHash, s));
     }
@@ -1897,8 +1906,9 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
   for (bool& h : lead->stage_hit) h = false;
   if (use_graph) {
     hipGraphExec_t exec = nullptr;
+    const uint64_t key = live << 8 | (uint64_t)n; /* n <= 128 */
     for (auto& g : lead->graph_exec)
-      if (g.first == n) exec = g.second;
+      if (g.first == key) exec = g.second;
     if (!exec) {
       HIP_OK(lead, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       const hipError_t e = sequence();
@@ -1913,11 +1923,11 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
       (void)hipGraphDestroy(graph);
       if (e3 != hipSuccess)
         return set_err(lead, AV1H_ERR_DEVICE, "instantiating the batch graph: %s", hipGetErrorString(e3));
-      if (lead->graph_exec.size() >= 32) {
+      if (lead->graph_exec.size() >= 64) {
         (void)hipGraphExecDestroy(lead->graph_exec.front().second);
         lead->graph_exec.erase(lead->graph_exec.begin());
       }
-      lead->graph_exec.emplace_back(n, exec);
+      lead->graph_exec.emplace_back(key, exec);
     }
     HIP_OK(lead, hipEventRecord(lead->ev[0], s));
     HIP_OK(lead, hipGraphLaunch(exec, s));
