@@ -563,9 +563,11 @@ class HipDecoder:
             raise HipError(self.error())
         return v.value
 
-    def set_graphs(self, enable=True):
-        """Replay captured launch sequences (HIP graphs) for batches this context leads."""
-        self._lib.av1h_set_graphs(self._h, 1 if enable else 0)
+    def set_graphs(self, enable=True, per_kernel_set=True):
+        """Replay captured launch sequences (HIP graphs) for batches this context
+        leads: one per batch size and set of kernels with work, or with
+        per_kernel_set=False one per batch size with every kernel launched."""
+        self._lib.av1h_set_graphs(self._h, 0 if not enable else (1 if per_kernel_set else 2))
 
     def set_decode_nv12(self, pitch):
         """Pack NV12 at `pitch` on the device as part of every decode (0: off)."""

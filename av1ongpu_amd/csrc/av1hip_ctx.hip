@@ -178,6 +178,7 @@ struct Ctx {
   /* captured launch sequences of batches this context leads, by frame count
    * (valid while batch_dev stays where it is) */
   bool graphs = true;
+  bool graph_sets = true; /* graphs keyed by the set of kernels with work too */
   Tuning tune;                   /* launch shapes of the batches it leads */
   std::vector<std::pair<uint64_t, hipGraphExec_t>> graph_exec; /* (kernel set << 8 | frames, graph) */
   bool stage_hit[AV1H_STAGES] = {};
@@ -1814,6 +1815,7 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
     if (total_of(k) > 0) live |= 1ull << k;
   if (nq > 0) live |= 1ull << kKernels;
   if (any_hash) live |= 1ull << kKHash;
+  if (use_graph && !lead->graph_sets) live = ~0ull >> (63 - kKernels); /* every kernel, the queue */
   static_assert(kKernels < 64, "kernel set mask");
   auto has = [&](int k) { return ((live >> k) & 1) != 0; };
   auto run = [&](int k, hipStream_t st) -> hipError_t {
@@ -1896,7 +1898,7 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
     if (side_grain) SEQ(hipStreamWaitEvent(s, lead->ev_side_done, 0));
     SEQ(stage(AV1H_STAGE_GRAIN, {kKGrainApply}));
     SEQ(run(kKNv12, s));
-    if (any_hash) {
+    if (has(kKHash)) {
       SEQ(launch_zero64(lead->hash_dev, n, s));
       SEQ(run(kKHash, s));
     }
@@ -2478,11 +2480,13 @@ int av1h_tune_set(av1h_ctx* h, const int32_t* params, int n) {
   return AV1H_OK;
 }
 
-int av1h_set_graphs(av1h_ctx* h, int enable) {
-  if (!h) return AV1H_ERR_INVALID;
+int av1h_set_graphs(av1h_ctx* h, int mode) {
+  if (!h || mode < 0 || mode > 2) return AV1H_ERR_INVALID;
   std::lock_guard<std::mutex> lk(h->c.mu);
-  h->c.graphs = enable != 0;
-  if (!enable) {
+  const bool changed = (h->c.graphs != (mode != 0)) || (mode != 0 && h->c.graph_sets != (mode == 1));
+  h->c.graphs = mode != 0;
+  h->c.graph_sets = mode == 1;
+  if (changed) {
     (void)hipSetDevice(h->c.device);
     drop_graphs(&h->c);
   }

@@ -324,6 +324,8 @@ int av1h_server_create(const av1h_server_config* cfg, av1h_server** out) {
       delete s;
       return AV1H_ERR_DEVICE;
     }
+    /* batches of changing size and content: one captured graph per size */
+    av1h_set_graphs(l, 2);
     s->launchers.push_back(l);
   }
   std::memset(&s->stats, 0, sizeof(s->stats));

@@ -423,14 +423,18 @@ const uint8_t* av1h_nv12_device_buffer(const av1h_ctx* ctx);
  * decoder packs only when asked (Decoder::to_nv12, av1gpu.hpp:745). */
 int av1h_set_decode_nv12(av1h_ctx* ctx, size_t pitch);
 
-/* Captured launches (default on): the first batch of each frame count this
- * context leads records its launch sequence into a HIP graph, with every
- * kernel at its full persistent grid reading its work counts from the
- * batch's device tables; later batches of that count replay it with one
- * launch. Decodes with stage timing or the queue trace on launch eagerly.
- * Off: every decode launches eagerly. With captured launches the stats'
+/* Captured launches. Mode 1 (the default): the first batch of each frame
+ * count and set of kernels with work this context leads records its launch
+ * sequence into a HIP graph, each of those kernels at its full persistent
+ * grid reading its work counts from the batch's device tables; later
+ * batches of that shape replay it with one launch. Mode 2: one graph per
+ * frame count with every kernel launched (kernels without work exit at
+ * once), for callers whose batches change shape often (the serving loop's
+ * launchers): fewer captures, each replay dispatching every kernel. Mode 0:
+ * every decode launches eagerly. Decodes with stage timing or the queue
+ * trace on launch eagerly. With captured launches the stats'
  * device_recon_ms holds the whole batch and device_filter_ms 0. */
-int av1h_set_graphs(av1h_ctx* ctx, int enable);
+int av1h_set_graphs(av1h_ctx* ctx, int mode);
 
 /* Launch shapes of the batches this context leads, the parameters an
  * autotuner searches (av1ongpu_amd/autotune.py does): for every kernel the

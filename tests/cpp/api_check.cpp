@@ -170,9 +170,43 @@ static int decode(int argc, char** argv) {
   return 0;
 }
 
+/* A key frame with film grain whose first luma auto-regressive coefficient
+ * lies outside the coded range [-128, 127] is refused with an error; the same
+ * frame with the coefficient at the range's ends decodes. */
+static int grain_range() {
+  synth::Config cfg = synth::workload("c2_mini", 1);
+  cfg.frames = 1;
+  synth::Clip clip = synth::generate(cfg);
+  std::string err;
+  BackendOptions o;
+  o.kind = BackendKind::HIP;
+  auto dec = Decoder::create(o, &err);
+  if (!dec) {
+    std::fprintf(stderr, "create: %s\n", err.c_str());
+    return 3;
+  }
+  for (const int v : {-129, 128, -128, 127}) {
+    FrameSyntax s = clip.frames[0];
+    s.reference_slots.clear();
+    s.grain.ar_coeffs_y[0] = v;
+    Frame out;
+    err.clear();
+    const int slot = dec->decode_reference(s, &out, &err);
+    const bool in_range = v >= -128 && v <= 127;
+    if (in_range != (slot >= 0) || (!in_range && err.find("auto-regressive") == std::string::npos)) {
+      std::fprintf(stderr, "coefficient %d: slot %d, error '%s'\n", v, slot, err.c_str());
+      return 4;
+    }
+    if (slot >= 0) dec->release_reference(slot);
+  }
+  std::printf("grain range ok\n");
+  return 0;
+}
+
 int main(int argc, char** argv) {
   if (argc >= 2 && std::string(argv[1]) == "host") return host();
+  if (argc >= 2 && std::string(argv[1]) == "grain_range") return grain_range();
   if (argc >= 5 && std::string(argv[1]) == "decode") return decode(argc, argv);
-  std::fprintf(stderr, "usage: api_check host | decode <workload> <seed> <nv12 pitch> [planned] [key=value...]\n");
+  std::fprintf(stderr, "usage: api_check host | grain_range | decode <workload> <seed> <nv12 pitch> [planned] [key=value...]\n");
   return 2;
 }

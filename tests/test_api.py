@@ -83,6 +83,16 @@ def test_cpp_api_decodes_golden(case, planned):
     _run_decode(case, planned)
 
 
+@pytest.mark.gpu
+def test_cpp_api_rejects_grain_coefficients_out_of_range():
+    """Film grain auto-regressive coefficients are coded as 8 bit values
+    minus 128 (the reference's src/frontend/headers.cpp:716-725); a frame
+    with one outside [-128, 127] is refused with an error, the range's ends
+    decode."""
+    r = subprocess.run([API_CHECK, "grain_range"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+
 def test_picture_hash_is_position_dependent():
     import numpy as np
     a = [np.arange(12, dtype=np.uint16).reshape(3, 4), np.ones((2, 2), np.uint16), np.zeros((2, 2), np.uint16)]

@@ -29,8 +29,10 @@ def _check(name, f, got, want):
                                  f"batched={got[p][y, x]} alone={want[p][y, x]}")
 
 
-def _run_batched(clips):
+def _run_batched(clips, per_kernel_set=True):
     decs = [pkg.HipDecoder() for _ in clips]
+    if not per_kernel_set:
+        decs[0].set_graphs(True, per_kernel_set=False)
     got = [[] for _ in clips]
     for _f, outs in pkg.decode_clips_batched(decs, clips):
         for i, _slot, planes in outs:
@@ -101,10 +103,13 @@ def test_decode_time_nv12(extra):
         assert bytes(na) == bytes(nb), f"frame {f}"
 
 
-def test_graph_replays_match_eager():
-    """Captured launch sequences (replayed for every batch of the same size,
+@pytest.mark.parametrize("per_kernel_set", [True, False], ids=["per_kernel_set", "per_size"])
+def test_graph_replays_match_eager(per_kernel_set):
+    """Captured launch sequences (replayed for every batch of the same shape,
     with eager decodes and synchronisations between replays) give the same
-    pictures as eager launches, for batches of 1 and of several frames."""
+    pictures as eager launches, for batches of 1 and of several frames, with
+    graphs per batch size and set of kernels with work and per batch size
+    with every kernel launched."""
     clips = [pkg.Clip("c2_mini", 8), pkg.Clip("tools_mini", 3), pkg.Clip("c1_mini", 9, bit_depth=10)]
     eager = [pkg.HipDecoder() for _ in clips]
     for d in eager:
@@ -114,7 +119,7 @@ def test_graph_replays_match_eager():
         for i, _s, planes in outs:
             want[i].append(planes)
     for rep in range(2):
-        got = _run_batched(clips)
+        got = _run_batched(clips, per_kernel_set)
         for c, g, w in zip(clips, got, want):
             for f, (a, b) in enumerate(zip(g, w)):
                 _check(f"graph rep {rep} {c.workload} seed {c.seed}", f, a, b)
