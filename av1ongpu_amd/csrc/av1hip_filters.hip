@@ -171,11 +171,20 @@ __device__ __forceinline__ int constrain(int diff, int thr, int damping) {
   return diff < 0 ? -v : v;
 }
 
-/* constrain() with the shift of its threshold precomputed */
-__device__ __forceinline__ int constrain_adj(int diff, int thr, int adj) {
-  const int mag = abs(diff);
-  const int v = clampi(thr - (mag >> adj), 0, mag);
-  return diff < 0 ? -v : v;
+/* A sample at byte offset o of a picture whose base is the same for the
+ * whole wave: one scalar base plus a 32-bit vector offset, no per-lane
+ * 64-bit address arithmetic (a picture buffer is far below 4 GB) */
+__device__ __forceinline__ int tap16(const char* base, unsigned o) {
+  return *reinterpret_cast<const uint16_t*>(base + o);
+}
+
+/* constrain() of tap v around the centre x, the shift of its threshold
+ * precomputed: |v - x| is one sum-of-absolute-differences instruction
+ * (samples are at most 16 bit, the upper halves zero) */
+__device__ __forceinline__ int constrain_vx(int v, int x, int thr, int adj) {
+  const int mag = (int)__builtin_amdgcn_sad_u16((unsigned)v, (unsigned)x, 0u);
+  const int c = max(0, min(thr - (mag >> adj), mag));
+  return v < x ? -c : c;
 }
 
 __constant__ const int kCdefDirs[8][2][2] = {{{-1, 1}, {-2, 2}}, {{0, 1}, {-1, 2}}, {{0, 1}, {0, 2}},
@@ -327,6 +336,8 @@ __device__ __forceinline__ bool cdef_sample(FArgs a, int first, int bi, int lane
     /* every tap inside the plane */
     const int adjp = pri ? max(0, damping - flog2(pri)) : 0, adjs = sec ? max(0, damping - flog2(sec)) : 0;
     const int d1 = (dir + 2) & 7, d2 = (dir + 6) & 7;
+    const char* pb = reinterpret_cast<const char*>(a.cdef_src);
+    const unsigned o0 = 2u * (unsigned)(L.plane_off[p] + (int64_t)(c.y + r) * st + c.x + k);
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
       const int op = dirs[dir][t][0] * st + dirs[dir][t][1];
@@ -335,9 +346,10 @@ __device__ __forceinline__ bool cdef_sample(FArgs a, int first, int bi, int lane
       const int wp = t ? pt1 : pt0, ws = t ? 1 : 2;
 #pragma unroll
       for (int sg = -1; sg <= 1; sg += 2) {
-        const int vp = src[sg * op], v1 = src[sg * o1], v2 = src[sg * o2];
-        if (pri) sum += wp * constrain_adj(vp - x, pri, adjp);
-        if (sec) sum += ws * (constrain_adj(v1 - x, sec, adjs) + constrain_adj(v2 - x, sec, adjs));
+        const int vp = tap16(pb, o0 + 2u * (unsigned)(sg * op)), v1 = tap16(pb, o0 + 2u * (unsigned)(sg * o1)),
+                  v2 = tap16(pb, o0 + 2u * (unsigned)(sg * o2));
+        if (pri) sum += __mul24(wp, constrain_vx(vp, x, pri, adjp));
+        if (sec) sum += __mul24(ws, constrain_vx(v1, x, sec, adjs) + constrain_vx(v2, x, sec, adjs));
         lo = min(lo, min(vp, min(v1, v2)));
         hi = max(hi, max(vp, max(v1, v2)));
       }
@@ -410,6 +422,8 @@ __device__ __forceinline__ bool cdef_luma_sample(FArgs a, int bi, int lane, int6
   const int x = src[0];
   int sum = 0, lo = x, hi = x;
   if (cx >= 2 && cy >= 2 && cx + 10 <= pw && cy + 10 <= ph) {
+    const char* pb = reinterpret_cast<const char*>(a.cdef_src + L.plane_off[0]);
+    const unsigned o0 = 2u * (unsigned)((cy + r) * st + cx + k);
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
       const int op = kCdefDirs[dir][t][0] * st + kCdefDirs[dir][t][1];
@@ -418,9 +432,10 @@ __device__ __forceinline__ bool cdef_luma_sample(FArgs a, int bi, int lane, int6
       const int wp = t ? pt1 : pt0, ws = t ? 1 : 2;
 #pragma unroll
       for (int sg = -1; sg <= 1; sg += 2) {
-        const int vp = src[sg * op], v1 = src[sg * o1], v2 = src[sg * o2];
-        if (pri) sum += wp * constrain_adj(vp - x, pri, adjp);
-        if (sec) sum += ws * (constrain_adj(v1 - x, sec, adjs) + constrain_adj(v2 - x, sec, adjs));
+        const int vp = tap16(pb, o0 + 2u * (unsigned)(sg * op)), v1 = tap16(pb, o0 + 2u * (unsigned)(sg * o1)),
+                  v2 = tap16(pb, o0 + 2u * (unsigned)(sg * o2));
+        if (pri) sum += __mul24(wp, constrain_vx(vp, x, pri, adjp));
+        if (sec) sum += __mul24(ws, constrain_vx(v1, x, sec, adjs) + constrain_vx(v2, x, sec, adjs));
         lo = min(lo, min(vp, min(v1, v2)));
         hi = max(hi, max(vp, max(v1, v2)));
       }
@@ -438,9 +453,9 @@ __device__ __forceinline__ bool cdef_luma_sample(FArgs a, int bi, int lane, int6
           if (px >= 0 && px < pw && py >= 0 && py < ph) {
             const int v = src[dy * st + dx];
             if (u == 0)
-              sum += (t ? pt1 : pt0) * constrain_adj(v - x, pri, adjp);
+              sum += __mul24(t ? pt1 : pt0, constrain_vx(v, x, pri, adjp));
             else if (sec)
-              sum += (t ? 1 : 2) * constrain_adj(v - x, sec, adjs);
+              sum += __mul24(t ? 1 : 2, constrain_vx(v, x, sec, adjs));
             lo = min(lo, v);
             hi = max(hi, v);
           }
