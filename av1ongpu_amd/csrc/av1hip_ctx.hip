@@ -173,6 +173,7 @@ struct Ctx {
   size_t nv12_pitch_done = 0;
   void* batch_dev = nullptr;
   void* batch_pin = nullptr;
+  void* rb_pin = nullptr; /* pinned: the error word and checksums read back after a batch */
   size_t batch_cap = 0;
   bool dirty = false;            /* work queued on `stream` nobody waited for */
   /* captured launch sequences of batches this context leads, by frame count
@@ -1955,11 +1956,16 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
     }
     HIP_OK(lead, hipMemcpyAsync(c->host_pin, c->frame, host_need, hipMemcpyDeviceToHost, s));
   }
-  uint32_t err_word = 0;
-  HIP_OK(lead, hipMemcpyAsync(&err_word, &lead->counters[1], 4, hipMemcpyDeviceToHost, s));
-  uint64_t hashes[kMaxBatch];
-  if (any_hash) HIP_OK(lead, hipMemcpyAsync(hashes, lead->hash_dev, (size_t)n * 8, hipMemcpyDeviceToHost, s));
+  /* into pinned memory: a copy to pageable memory is staged through a
+   * runtime buffer, which costs more than the copy */
+  if (!lead->rb_pin) HIP_OK(lead, hipHostMalloc(&lead->rb_pin, 64 + kMaxBatch * 8, hipHostMallocDefault));
+  uint32_t* err_pin = static_cast<uint32_t*>(lead->rb_pin);
+  const uint64_t* hashes = reinterpret_cast<const uint64_t*>(static_cast<uint8_t*>(lead->rb_pin) + 64);
+  HIP_OK(lead, hipMemcpyAsync(err_pin, &lead->counters[1], 4, hipMemcpyDeviceToHost, s));
+  if (any_hash)
+    HIP_OK(lead, hipMemcpyAsync(const_cast<uint64_t*>(hashes), lead->hash_dev, (size_t)n * 8, hipMemcpyDeviceToHost, s));
   HIP_OK(lead, hipStreamSynchronize(s));
+  const uint32_t err_word = *err_pin;
   const double t_done = now_ms();
   for (JobSpec& j : js) j.c->have_frame = true;
   if (err_word) {
@@ -2113,6 +2119,7 @@ void av1h_destroy(av1h_ctx* h) {
   if (c->host_pin) (void)hipHostFree(c->host_pin);
   if (c->batch_dev) (void)hipFree(c->batch_dev);
   if (c->batch_pin) (void)hipHostFree(c->batch_pin);
+  if (c->rb_pin) (void)hipHostFree(c->rb_pin);
   free_plan(&c->scratch_plan);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
