@@ -534,7 +534,8 @@ int av1h_decode_planned_batch(av1h_ctx* const* ctxs, void* const* plans, const i
 /* The same with the launches on `launcher`'s stream (its captured launch
  * sequences, its batch tables), which need not be one of the batch's
  * contexts: a caller that forms batches from changing sets of streams keeps
- * one launcher, so one captured sequence per batch size serves them all. */
+ * one launcher, whose captured sequences then serve them all (one per batch
+ * size with av1h_set_graphs mode 2, as the serving loop's launchers use). */
 int av1h_decode_planned_batch_on(av1h_ctx* launcher, av1h_ctx* const* ctxs, void* const* plans,
                                  const int* keep, const int* reserved, int count, int* slots_out);
 
