@@ -59,6 +59,9 @@ int av1s_config_set(void* h, const char* key, double v) {
   else if (k == "contract_tools") c->contract_tools = i != 0;
   else if (k == "inter_tools") c->inter_tools = v;
   else if (k == "intrabc") c->intrabc = v;
+  else if (k == "avail_auto") c->avail_auto = v;
+  else if (k == "residual_frac") c->residual_frac = v;
+  else if (k == "lr_log2") c->lr_log2 = i;
   else if (k == "seed") c->seed = static_cast<uint64_t>(v);
   else return -1;
   return 0;

@@ -234,7 +234,7 @@ class FrameBuilder {
     return static_cast<int>(r);
   }
 
-  TransformBlock make_tx(int rx, int ry, int tw, int th, bool lossless) {
+  TransformBlock make_tx(int rx, int ry, int tw, int th, bool lossless, bool raw = false) {
     TransformBlock t;
     t.x = rx;
     t.y = ry;
@@ -246,6 +246,21 @@ class FrameBuilder {
     if (rng_.chance(cfg_.tx_skip)) {
       t.eob = 0;
       t.coef_offset = 0;
+      return t;
+    }
+    if (raw) {
+      /* raw residual samples (residual_provided): the whole tw x th raster,
+       * half of it zero, the rest reaching past the sample range on both
+       * sides so the add's clip is exercised */
+      const int lim = (1 << cfg_.bit_depth) + 64;
+      t.coef_offset = static_cast<int>(fs_.residual.size());
+      int nnz = 0;
+      for (int i = 0; i < tw * th; ++i) {
+        const int v = rng_.chance(0.5) ? 0 : rng_.range(-lim, lim);
+        nnz += v != 0;
+        fs_.residual.push_back(v);
+      }
+      t.eob = std::max(nnz, 1);
       return t;
     }
     const int cw = std::min(32, tw), ch = std::min(32, th);
@@ -295,26 +310,27 @@ class FrameBuilder {
     const bool intra = key_ || nrefs_ == 0 || rng_.chance(cfg_.intra_frac);
     const int level = rng_.range(cfg_.deblock_lo, cfg_.deblock_hi);
     const bool lossless = cfg_.contract_tools && intra && w == 8 && h == 8 && rng_.chance(0.05);
+    const bool raw = cfg_.residual_frac > 0 && !lossless && rng_.chance(cfg_.residual_frac);
     if (intra && key_ && cfg_.intrabc > 0 && y >= 64 && rng_.chance(cfg_.intrabc)) {
-      intrabc_block(x, y, w, h, level);
+      intrabc_block(x, y, w, h, level, raw);
     } else if (intra) {
       const int ymode = rng_.range(0, 12);
       const int yangle = (ymode >= 1 && ymode <= 8) ? rng_.range(-3, 3) : 0;
-      intra_block(0, x, y, w, h, ymode, yangle, level, lossless);
+      intra_block(0, x, y, w, h, ymode, yangle, level, lossless, 0, raw);
       int uvmode = rng_.range(0, 12);
       if (cfg_.contract_tools && rng_.chance(0.2)) uvmode = 13;
       const int uvangle = (uvmode >= 1 && uvmode <= 8) ? rng_.range(-3, 3) : 0;
       const int cfl_alpha = rng_.range(-16, 16);
       for (int p = 1; p < 3; ++p)
         intra_block(p, x >> 1, y >> 1, w >> 1, h >> 1, uvmode, uvangle, level, lossless,
-                    cfl_alpha);
+                    cfl_alpha, raw);
     } else {
       InterParams ip = draw_inter();
       if (cfg_.inter_tools > 0 && w >= 8 && h >= 8 && w <= 32 && h <= 32 && rng_.chance(cfg_.inter_tools))
         draw_tool(&ip, w, h);
       for (int p = 0; p < 3; ++p) {
         const int s = p ? 1 : 0;
-        inter_block(p, x >> s, y >> s, w >> s, h >> s, ip, level);
+        inter_block(p, x >> s, y >> s, w >> s, h >> s, ip, level, raw);
       }
     }
   }
@@ -386,7 +402,7 @@ class FrameBuilder {
   /* Intra block copy from the superblock row above (key frames): every plane
    * copies the co-located block 64 luma rows up, shifted by a whole number of
    * chroma samples sideways. */
-  void intrabc_block(int x, int y, int w, int h, int level) {
+  void intrabc_block(int x, int y, int w, int h, int level, bool raw) {
     const int span = grid_w_ - w;
     const int sx = (rng_.range(0, span) / 16) * 16;
     InterParams ip;
@@ -397,12 +413,12 @@ class FrameBuilder {
     ip.intrabc = true;
     for (int p = 0; p < 3; ++p) {
       const int s = p ? 1 : 0;
-      inter_block(p, x >> s, y >> s, w >> s, h >> s, ip, level);
+      inter_block(p, x >> s, y >> s, w >> s, h >> s, ip, level, raw);
     }
   }
 
   void intra_block(int plane, int x, int y, int w, int h, int mode, int angle, int level,
-                   bool lossless, int cfl_alpha = 0) {
+                   bool lossless, int cfl_alpha, bool raw) {
     PlaneState& ps = planes_[plane];
     int tw, th;
     tx_size_for(plane, w, h, true, &tw, &th);
@@ -446,6 +462,10 @@ class FrameBuilder {
             (b.y > 0 && ps.rect_decoded(b.x + b.w, b.x + 2 * b.w - 1, b.y - 1, b.y - 1)) ? 1 : 0;
         b.have_below_left =
             (b.x > 0 && ps.rect_decoded(b.x - 1, b.x - 1, b.y + b.h, b.y + 2 * b.h - 1)) ? 1 : 0;
+        /* left to the decoder: -1 asks it to derive the flag from coding order */
+        if (cfg_.avail_auto > 0 && rng_.chance(cfg_.avail_auto)) b.have_above_right = -1;
+        if (cfg_.avail_auto > 0 && rng_.chance(cfg_.avail_auto)) b.have_below_left = -1;
+        b.residual_provided = raw;
         b.filter_type = filter_type;
         b.lossless = lossless;
         b.cfl_alpha = cfl_alpha;
@@ -460,7 +480,7 @@ class FrameBuilder {
           b.palette_off = palette_base + ty * w + tx;
           b.palette_stride = w;
         }
-        b.transforms.push_back(make_tx(0, 0, tw, th, lossless));
+        b.transforms.push_back(make_tx(0, 0, tw, th, lossless, raw));
         fs_.blocks.push_back(std::move(b));
         ps.mark(x + tx, y + ty, tw, th, is_smooth_mode(mode), tw, th, level);
         ps.mark_tx(x + tx, y + ty, tw, th);
@@ -509,7 +529,8 @@ class FrameBuilder {
     return ip;
   }
 
-  void inter_block(int plane, int x, int y, int w, int h, const InterParams& ip, int level) {
+  void inter_block(int plane, int x, int y, int w, int h, const InterParams& ip, int level,
+                   bool raw) {
     PlaneState& ps = planes_[plane];
     int tw, th;
     tx_size_for(plane, w, h, false, &tw, &th);
@@ -540,6 +561,7 @@ class FrameBuilder {
     b.interintra_mode = ip.interintra_mode;
     b.interintra_wedge = ip.interintra_wedge;
     b.intrabc = ip.intrabc;
+    b.residual_provided = raw;
     if (ip.warp) {
       b.warp[0] = true;
       for (int k = 0; k < 6; ++k) b.warp_params[0][k] = ip.warp_params[k];
@@ -555,7 +577,7 @@ class FrameBuilder {
     }
     for (int ty = 0; ty < h; ty += th)
       for (int tx = 0; tx < w; tx += tw) {
-        b.transforms.push_back(make_tx(tx, ty, tw, th, false));
+        b.transforms.push_back(make_tx(tx, ty, tw, th, false, raw));
         ps.mark_tx(x + tx, y + ty, tw, th);
       }
     fs_.blocks.push_back(std::move(b));
@@ -663,7 +685,8 @@ class FrameBuilder {
   void build_lr() {
     for (int plane = 0; plane < 3; ++plane) {
       const int s = plane ? 1 : 0;
-      const int log2 = plane ? 5 : 6;
+      const int l0 = cfg_.lr_log2 > 0 ? cfg_.lr_log2 : 6;
+      const int log2 = plane ? std::max(5, l0 - 1) : l0;
       const int unit = 1 << log2;
       const int pw = (cfg_.width + s) >> s;
       const int ph = (cfg_.height + s) >> s;

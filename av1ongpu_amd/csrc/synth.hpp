@@ -60,6 +60,12 @@ struct Config {
                                 row) copied from the superblock row above */
   bool contract_tools = false; /* CFL, filter intra, palette, lossless, OBMC, masks,
                                   inter-intra, global warp, edge filter off        */
+  double avail_auto = 0.0;   /* intra blocks whose have_above_right / have_below_left
+                                are -1 (the decoder derives them from coding order) */
+  double residual_frac = 0.0; /* coding blocks carrying raw residual samples
+                                 (residual_provided) instead of coefficients       */
+  int lr_log2 = 0;           /* luma restoration unit log2 (0: 64); chroma one less,
+                                at least 32                                       */
   uint64_t seed = 1;
 };
 

@@ -809,4 +809,15 @@ int64_t av1r_slot_nv12(void* h, int slot, size_t pitch, uint8_t* out, int64_t ca
   return static_cast<int64_t>(v.size());
 }
 
+/* the backend's own to_rgb of a slot (its gather rule) */
+int64_t av1r_slot_rgb(void* h, int slot, int matrix, int range, uint8_t* out, int64_t cap) {
+  RefDecoder* d = static_cast<RefDecoder*>(h);
+  std::vector<uint8_t> v;
+  std::string error;
+  if (!d->backend->to_rgb(slot, static_cast<ColorMatrix>(matrix), static_cast<ColorRange>(range), &v, &error))
+    return -1;
+  if (out && cap >= static_cast<int64_t>(v.size())) std::memcpy(out, v.data(), v.size());
+  return static_cast<int64_t>(v.size());
+}
+
 }  // extern "C"

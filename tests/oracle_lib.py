@@ -80,6 +80,8 @@ def ref_lib():
         lib.av1r_last_rgb.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int64]
         lib.av1r_slot_nv12.restype = ctypes.c_int64
         lib.av1r_slot_nv12.argtypes = [P, ctypes.c_int, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_int64]
+        lib.av1r_slot_rgb.restype = ctypes.c_int64
+        lib.av1r_slot_rgb.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int64]
         _ref = lib
     return _ref
 
@@ -186,6 +188,14 @@ class RefDecoder:
             return None
         buf = np.zeros(n, dtype=np.uint8)
         self._lib.av1r_slot_nv12(self._h, slot, pitch, buf.ctypes.data, n)
+        return buf
+
+    def slot_rgb(self, slot, matrix, rng):
+        n = self._lib.av1r_slot_rgb(self._h, slot, matrix, rng, None, 0)
+        if n < 0:
+            return None
+        buf = np.zeros(n, dtype=np.uint8)
+        self._lib.av1r_slot_rgb(self._h, slot, matrix, rng, buf.ctypes.data, n)
         return buf
 
     def __del__(self):
