@@ -243,6 +243,7 @@ def hip_lib():
         lib.av1h_nv12_device_buffer.argtypes = [P]
         _bind_server(lib)
         _bind_costmodel(lib)
+        _bind_pipe(lib)
         _hip = lib
     return _hip
 
@@ -309,6 +310,21 @@ def _bind_server(lib):
     lib.av1h_set_pool_slots.argtypes = [P, ctypes.c_int]
     lib.av1h_plan_levels.argtypes = [P]
     lib.av1h_plan_queue_counts.argtypes = [P, ctypes.POINTER(ctypes.c_int32)]
+
+
+def _bind_pipe(lib):
+    P = ctypes.c_void_p
+    I32P = ctypes.POINTER(ctypes.c_int32)
+    lib.av1h_pipe_create.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(P)]
+    lib.av1h_pipe_submit_wire.argtypes = [P, ctypes.c_char_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int, I32P,
+                                          ctypes.c_int]
+    lib.av1h_pipe_submit_wire.restype = ctypes.c_int64
+    lib.av1h_pipe_wait.argtypes = [P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
+    lib.av1h_pipe_result.argtypes = [P, ctypes.c_int64, I32P, I32P, ctypes.POINTER(ctypes.c_double),
+                                     ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64)]
+    lib.av1h_pipe_error.argtypes = [P]
+    lib.av1h_pipe_error.restype = ctypes.c_char_p
+    lib.av1h_pipe_destroy.argtypes = [P]
 
 
 def _bind_costmodel(lib):
@@ -419,6 +435,65 @@ STAGES = ("inter", "queue", "deblock", "cdef", "superres", "lr", "store", "grain
 
 class HipError(RuntimeError):
     pass
+
+
+class Pipe:
+    """One stream's frames through planner threads and an in-order decode
+    worker (av1h_pipe_*): submit wire frames, wait for them. The wire bytes of
+    a frame are held here until it completes."""
+
+    def __init__(self, decoder, planner_threads=8, depth=32):
+        self._lib = decoder._lib
+        self._dec = decoder
+        h = ctypes.c_void_p()
+        rc = self._lib.av1h_pipe_create(decoder.handle, planner_threads, depth, ctypes.byref(h))
+        if rc != AV1H_OK:
+            raise HipError(f"av1h_pipe_create failed ({rc})")
+        self._h = h
+        self._held = {}
+        self._done = 0
+
+    def submit(self, wire, keep=True, reserved=-1, release=()):
+        rel = (ctypes.c_int32 * max(len(release), 1))(*release)
+        seq = self._lib.av1h_pipe_submit_wire(self._h, wire, len(wire), 1 if keep else 0, reserved, rel,
+                                              len(release))
+        if seq < 0:
+            raise HipError(f"pipe submit failed ({seq})")
+        self._held[seq] = wire
+        return seq
+
+    def wait(self, seq=-1):
+        """Waits for frames 0..seq (all when negative); raises on a failed frame."""
+        failed = ctypes.c_int64(-1)
+        rc = self._lib.av1h_pipe_wait(self._h, seq, ctypes.byref(failed))
+        upto = max(self._held, default=-1) if seq < 0 else seq
+        for k in [k for k in self._held if k <= upto]:
+            del self._held[k]
+        if rc != AV1H_OK:
+            raise HipError(f"frame {failed.value} failed ({rc}): "
+                           f"{self._lib.av1h_pipe_error(self._h).decode(errors='replace')}")
+
+    def result(self, seq):
+        """(slot, status, submit ms, done ms, picture checksum or 0) of a completed frame."""
+        slot, st = ctypes.c_int32(), ctypes.c_int32()
+        t0, t1 = ctypes.c_double(), ctypes.c_double()
+        hv = ctypes.c_uint64()
+        if self._lib.av1h_pipe_result(self._h, seq, ctypes.byref(slot), ctypes.byref(st), ctypes.byref(t0),
+                                      ctypes.byref(t1), ctypes.byref(hv)) != AV1H_OK:
+            raise HipError(f"frame {seq} is not complete")
+        return slot.value, st.value, t0.value, t1.value, hv.value
+
+    def close(self):
+        if self._h:
+            self._lib.av1h_pipe_destroy(self._h)
+            self._h = None
+            self._held.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class HipDecoder:

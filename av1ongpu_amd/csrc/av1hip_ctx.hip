@@ -91,7 +91,6 @@ struct Plan {
   DevGrain grain;
   void* dbuf = nullptr;
   size_t dbytes = 0;
-  std::vector<uint8_t> hbuf;
   bool uses_lr = false, upscaled = false, has_grain = false;
   int blocks = 0;
   std::vector<int> slots; /* pool slots the frame reads */
@@ -113,6 +112,13 @@ struct Plan {
   /* queued blocks, packed-edge readers, edge publishers, blocks no wait
    * reads the cells of (av1h_plan_queue_counts) */
   int32_t q_counts[4] = {0, 0, 0, 0};
+};
+
+/* a pinned staging buffer and the stream its uploads run on */
+struct Stager {
+  hipStream_t stream = nullptr;
+  void* pin = nullptr;
+  size_t cap = 0;
 };
 
 struct Ctx {
@@ -186,7 +192,22 @@ struct Ctx {
   double stage_ms[AV1H_STAGES] = {};
   int64_t stage_n[AV1H_STAGES] = {};
   int64_t stage_bytes[AV1H_STAGES] = {};
+  /* pre-planning (any thread): stagers and device buffers for reuse */
+  std::mutex plan_mu;
+  std::vector<Stager*> stagers;
+  std::vector<std::pair<size_t, void*>> plan_bufs;
+  size_t plan_buf_bytes = 0;
 };
+
+int set_err(std::string* e, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  *e = buf;
+  return code;
+}
 
 int set_err(Ctx* c, int code, const char* fmt, ...) {
   char buf[512];
@@ -406,6 +427,12 @@ int mc_filter_row(int cls, int extent) {
 /* ---- flattening ----------------------------------------------------------- */
 
 struct Host;
+/* the arrays of a frame plan's device buffer */
+enum {
+  kPartBlocks, kPartTx, kPartInter, kPartQueue, kPartSpecial, kPartObmc, kPartCoef, kPartResidual, kPartPalette,
+  kPartDeblock, kPartCdef, kPartLr, kPartMcItems, kPartMcTasks, kPartItx, kPartItxOffs, kPartDbChunk, kPartCdefDir,
+  kPartLrTiles, kPartGrain, kParts
+};
 void sb_features(const Host& H, const av1h_frame_header& hdr, const Layout& L, Plan* plan);
 
 struct Host {
@@ -419,12 +446,19 @@ struct Host {
   std::vector<int32_t> db_chunk, cdef_dir_list;
   std::vector<int32_t> lr_tiles; /* pairs */
   std::vector<int32_t> itx[kTxClasses];
-  std::vector<int32_t> coef, residual;
-  std::vector<uint8_t> palette;
   std::vector<DevDeblock> deblock;
   std::vector<DevCdef> cdef;
   std::vector<DevLr> lr;
   DevGrain grain;
+  int32_t itx_off[2][kTxClasses + 1];
+  /* the device buffer's arrays: source, bytes, zero bytes after them, offset */
+  struct Part {
+    const void* src;
+    size_t bytes, fill, off;
+  };
+  std::vector<Part> parts;
+  int part_of[kParts];
+  size_t total = 0;
 };
 
 /* BlockDecoded availability for blocks whose syntax asks to derive it
@@ -610,21 +644,21 @@ bool packed_edges(const Layout& L, DevBlock& d, const std::vector<int32_t>& owne
   return true;
 }
 
-int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
+int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
   const av1h_frame_header& hdr = f.hdr;
   if (hdr.width <= 0 || hdr.height <= 0 || hdr.width > 16384 || hdr.height > 16384)
-    return set_err(c, AV1H_ERR_INVALID, "frame has no size");
+    return set_err(err, AV1H_ERR_INVALID, "frame has no size");
   if (hdr.bit_depth != 8 && hdr.bit_depth != 10 && hdr.bit_depth != 12)
-    return set_err(c, AV1H_ERR_INVALID, "bit depth %d", hdr.bit_depth);
+    return set_err(err, AV1H_ERR_INVALID, "bit depth %d", hdr.bit_depth);
   if (hdr.ss_x < 0 || hdr.ss_x > 1 || hdr.ss_y < 0 || hdr.ss_y > 1 || (hdr.ss_y && !hdr.ss_x))
-    return set_err(c, AV1H_ERR_UNSUPPORTED, "chroma subsampling %d,%d", hdr.ss_x, hdr.ss_y);
+    return set_err(err, AV1H_ERR_UNSUPPORTED, "chroma subsampling %d,%d", hdr.ss_x, hdr.ss_y);
   if (hdr.num_reference_slots < 0 || hdr.num_reference_slots > AV1H_MAX_REFS)
-    return set_err(c, AV1H_ERR_INVALID, "bad reference slot count");
+    return set_err(err, AV1H_ERR_INVALID, "bad reference slot count");
   for (int i = 0; i < 7; ++i) {
     const int rw = hdr.reference_width[i] > 0 ? hdr.reference_width[i] : hdr.width;
     const int rh = hdr.reference_height[i] > 0 ? hdr.reference_height[i] : hdr.height;
     if (rw != hdr.width || rh != hdr.height)
-      return set_err(c, AV1H_ERR_UNSUPPORTED,
+      return set_err(err, AV1H_ERR_UNSUPPORTED,
                      "reference scaling between pictures of different sizes is not supported");
   }
   Layout& L = plan->lay;
@@ -638,7 +672,7 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
     scale_x = ((hdr.width << 14) + fw / 2) / fw;
     scale_y = ((hdr.height << 14) + fh / 2) / fh;
     if (scale_x > 2 << 14 || scale_y > 2 << 14)
-      return set_err(c, AV1H_ERR_UNSUPPORTED, "superres beyond 2:1");
+      return set_err(err, AV1H_ERR_UNSUPPORTED, "superres beyond 2:1");
   }
   const int nb = (int)f.num_blocks;
   H->blocks.clear();
@@ -659,15 +693,15 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
   for (int i = 0; i < nb; ++i) {
     const av1h_block& b = f.blocks[i];
     const int p = b.plane;
-    if (p < 0 || p > 2) return set_err(c, AV1H_ERR_INVALID, "block %d: plane %d", i, p);
+    if (p < 0 || p > 2) return set_err(err, AV1H_ERR_INVALID, "block %d: plane %d", i, p);
     if (!pow2_in(b.w, 4, 64) || !pow2_in(b.h, 4, 64) || b.x < 0 || b.y < 0 || (b.x & 3) || (b.y & 3) ||
         b.x + b.w > L.grid_w[p] || b.y + b.h > L.grid_h[p])
-      return set_err(c, AV1H_ERR_UNSUPPORTED, "block %d: %dx%d at %d,%d outside the coded grid", i, b.w,
+      return set_err(err, AV1H_ERR_UNSUPPORTED, "block %d: %dx%d at %d,%d outside the coded grid", i, b.w,
                      b.h, b.x, b.y);
     if (b.bit_depth != hdr.bit_depth)
-      return set_err(c, AV1H_ERR_UNSUPPORTED, "block %d: bit depth differs from the frame", i);
+      return set_err(err, AV1H_ERR_UNSUPPORTED, "block %d: bit depth differs from the frame", i);
     if (b.tx_first < 0 || b.tx_count < 0 || b.tx_first + (int64_t)b.tx_count > f.num_transforms)
-      return set_err(c, AV1H_ERR_INVALID, "block %d: transform range", i);
+      return set_err(err, AV1H_ERR_INVALID, "block %d: transform range", i);
     DevBlock d;
     std::memset(&d, 0, sizeof(d));
     d.x = b.x;
@@ -683,16 +717,16 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
     if (b.lossless && !b.residual_provided)
       for (int t = 0; t < b.tx_count; ++t) {
         const av1h_transform& tx = f.transforms[b.tx_first + t];
-        if (tx.w != 4 || tx.h != 4) return set_err(c, AV1H_ERR_UNSUPPORTED, "lossless transform %dx%d", tx.w, tx.h);
+        if (tx.w != 4 || tx.h != 4) return set_err(err, AV1H_ERR_UNSUPPORTED, "lossless transform %dx%d", tx.w, tx.h);
       }
     if (b.is_intra) {
       d.kind = kIntra;
       d.mode = b.mode;
-      if (b.mode < 0 || b.mode > 13) return set_err(c, AV1H_ERR_INVALID, "block %d: intra mode %d", i, b.mode);
+      if (b.mode < 0 || b.mode > 13) return set_err(err, AV1H_ERR_INVALID, "block %d: intra mode %d", i, b.mode);
       if (b.mode == AV1H_PRED_CFL && p == 0)
-        return set_err(c, AV1H_ERR_UNSUPPORTED, "chroma from luma on the luma plane");
+        return set_err(err, AV1H_ERR_UNSUPPORTED, "chroma from luma on the luma plane");
       if (b.mode == AV1H_PRED_CFL && (b.w > 32 || b.h > 32))
-        return set_err(c, AV1H_ERR_INVALID, "block %d: chroma from luma on a %dx%d block", i, b.w, b.h);
+        return set_err(err, AV1H_ERR_INVALID, "block %d: chroma from luma on a %dx%d block", i, b.w, b.h);
       d.angle = b.angle_delta;
       d.filter_type = hdr.enable_intra_edge_filter ? b.filter_type : -1;
       d.fi_mode = b.filter_intra_mode;
@@ -708,7 +742,7 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
       if (b.use_filter_intra && p == 0) {
         d.flags |= kFilterIntra;
         if (b.filter_intra_mode < 0 || b.filter_intra_mode > 4)
-          return set_err(c, AV1H_ERR_INVALID, "filter intra mode %d", b.filter_intra_mode);
+          return set_err(err, AV1H_ERR_INVALID, "filter intra mode %d", b.filter_intra_mode);
       }
       /* the read set the reference scheduler waits for (engine.cpp:811-875) */
       int ly1 = b.y + b.h - 1, ax1 = b.x + b.w - 1;
@@ -716,7 +750,7 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
         d.flags |= kPalette;
         if (b.palette_off < 0 || b.palette_stride < b.w ||
             (int64_t)b.palette_off + (int64_t)(b.h - 1) * b.palette_stride + b.w > f.num_palette)
-          return set_err(c, AV1H_ERR_INVALID, "block %d: palette map out of bounds", i);
+          return set_err(err, AV1H_ERR_INVALID, "block %d: palette map out of bounds", i);
         for (int k = 0; k < 8; ++k) d.palette[k] = b.palette[k];
         d.palette_off = b.palette_off;
         d.palette_stride = b.palette_stride;
@@ -735,14 +769,14 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
       H->queue_list.push_back(i);
     } else {
       d.kind = kInter;
-      if (b.compound < 0 || b.compound > 4) return set_err(c, AV1H_ERR_INVALID, "compound %d", b.compound);
+      if (b.compound < 0 || b.compound > 4) return set_err(err, AV1H_ERR_INVALID, "compound %d", b.compound);
       const int nref = b.compound ? 2 : 1;
       const int sx = p > 0 ? hdr.ss_x : 0, sy = p > 0 ? hdr.ss_y : 0;
       for (int r = 0; r < nref; ++r) {
         const int s = b.intrabc ? -1 : resolve_slot(hdr, b.ref[r]);
         if (s < 0 && !b.intrabc)
-          return set_err(c, AV1H_ERR_UNSUPPORTED, "block %d: inter prediction without a reference", i);
-        if (s >= kSlots) return set_err(c, AV1H_ERR_INVALID, "block %d: reference slot %d", i, s);
+          return set_err(err, AV1H_ERR_UNSUPPORTED, "block %d: inter prediction without a reference", i);
+        if (s >= kSlots) return set_err(err, AV1H_ERR_INVALID, "block %d: reference slot %d", i, s);
         d.slot[r] = s;
         if (s >= 0 && std::find(plan->slots.begin(), plan->slots.end(), s) == plan->slots.end())
           plan->slots.push_back(s);
@@ -752,7 +786,7 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
         d.mv[r][1] = b.mv[r][1];
       }
       if (b.filter < 0 || b.filter > 3 || b.filter_v < 0 || b.filter_v > 3)
-        return set_err(c, AV1H_ERR_INVALID, "block %d: filter", i);
+        return set_err(err, AV1H_ERR_INVALID, "block %d: filter", i);
       d.filt = b.filter;
       d.filt_v = b.filter_v;
       d.compound = b.compound;
@@ -766,7 +800,7 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
       if (b.intrabc) {
         /* the frame being decoded, 4-rounded coded size, unscaled
          * (engine.cpp job fields; av1gpu_kernels.h intrabc) */
-        if (b.compound) return set_err(c, AV1H_ERR_INVALID, "block %d: compound intra block copy", i);
+        if (b.compound) return set_err(err, AV1H_ERR_INVALID, "block %d: compound intra block copy", i);
         d.flags |= kIntraBC;
         d.ref_w = ((fw + 3) & ~3) >> sx;
         d.ref_h = ((fh + 3) & ~3) >> sy;
@@ -787,35 +821,35 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
         if (d.mask_kind == 1 &&
             !wedge_entry(b.w << sx, b.h << sy, b.wedge_index, &d.wedge_dir, &d.wedge_xoff, &d.wedge_yoff,
                          &d.wedge_flip))
-          return set_err(c, AV1H_ERR_INVALID, "a wedge on a block size that has none");
+          return set_err(err, AV1H_ERR_INVALID, "a wedge on a block size that has none");
         special = true;
       }
       if (b.interintra) {
         d.flags |= kInterIntra;
         d.ii_mode = b.interintra_mode;
         if (b.interintra_mode < 0 || b.interintra_mode > 3)
-          return set_err(c, AV1H_ERR_INVALID, "block %d: inter/intra mode", i);
+          return set_err(err, AV1H_ERR_INVALID, "block %d: inter/intra mode", i);
         if (b.interintra_wedge) {
           if (!wedge_entry(b.w << sx, b.h << sy, b.wedge_index, &d.wedge_dir, &d.wedge_xoff, &d.wedge_yoff,
                            &d.wedge_flip))
-            return set_err(c, AV1H_ERR_INVALID, "an inter/intra wedge on a block size that has none");
+            return set_err(err, AV1H_ERR_INVALID, "an inter/intra wedge on a block size that has none");
           d.flags |= kIIWedge;
         }
       }
       if (b.obmc_count > 0) {
         if (b.obmc_first < 0 || b.obmc_first + (int64_t)b.obmc_count > f.num_obmc)
-          return set_err(c, AV1H_ERR_INVALID, "block %d: OBMC range", i);
+          return set_err(err, AV1H_ERR_INVALID, "block %d: OBMC range", i);
         d.obmc_first = (int)H->obmc.size();
         for (int k = 0; k < b.obmc_count && k < 8; ++k) {
           const av1h_obmc& o = f.obmc[b.obmc_first + k];
           const int s = resolve_slot(hdr, o.ref);
-          if (s < 0 || s >= kSlots) return set_err(c, AV1H_ERR_UNSUPPORTED, "block %d: OBMC without a reference", i);
+          if (s < 0 || s >= kSlots) return set_err(err, AV1H_ERR_UNSUPPORTED, "block %d: OBMC without a reference", i);
           if (o.w != 2 && o.w != 4 && o.w != 8 && o.w != 16 && o.w != 32 && o.pass != 0)
-            return set_err(c, AV1H_ERR_INVALID, "block %d: OBMC width %d", i, o.w);
+            return set_err(err, AV1H_ERR_INVALID, "block %d: OBMC width %d", i, o.w);
           if (o.h != 2 && o.h != 4 && o.h != 8 && o.h != 16 && o.h != 32 && o.pass == 0)
-            return set_err(c, AV1H_ERR_INVALID, "block %d: OBMC height %d", i, o.h);
+            return set_err(err, AV1H_ERR_INVALID, "block %d: OBMC height %d", i, o.h);
           if (o.filter < 0 || o.filter > 3 || o.filter_v < 0 || o.filter_v > 3)
-            return set_err(c, AV1H_ERR_INVALID, "block %d: OBMC filter", i);
+            return set_err(err, AV1H_ERR_INVALID, "block %d: OBMC filter", i);
           if (std::find(plan->slots.begin(), plan->slots.end(), s) == plan->slots.end()) plan->slots.push_back(s);
           H->obmc.push_back(DevObmc{o.pass, o.x, o.y, o.w, o.h, o.mv_x, o.mv_y, o.filter, o.filter_v, s});
         }
@@ -864,12 +898,12 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
       if (tx.eob <= 0) continue;
       if (!pow2_in(tx.w, 4, 64) || !pow2_in(tx.h, 4, 64) || tx.x < 0 || tx.y < 0 || tx.x + tx.w > b.w ||
           tx.y + tx.h > b.h || tx.coef_offset < 0 || tx.tx_type < 0 || tx.tx_type > 15)
-        return set_err(c, AV1H_ERR_INVALID, "block %d: transform %d out of range", i, t);
+        return set_err(err, AV1H_ERR_INVALID, "block %d: transform %d out of range", i, t);
       const int rs = b.lossless ? 0 : tx_row_shift(tx.w, tx.h);
-      if (rs < 0) return set_err(c, AV1H_ERR_UNSUPPORTED, "transform size %dx%d", tx.w, tx.h);
+      if (rs < 0) return set_err(err, AV1H_ERR_UNSUPPORTED, "transform size %dx%d", tx.w, tx.h);
       if (!b.lossless && !b.residual_provided &&
           (!kind_ok(row_kind(tx.tx_type), tx.w) || !kind_ok(col_kind(tx.tx_type), tx.h)))
-        return set_err(c, AV1H_ERR_UNSUPPORTED, "transform type %d is not defined for %dx%d", tx.tx_type, tx.w, tx.h);
+        return set_err(err, AV1H_ERR_UNSUPPORTED, "transform type %d is not defined for %dx%d", tx.tx_type, tx.w, tx.h);
       DevTx dt;
       dt.x = b.x + tx.x;
       dt.y = b.y + tx.y;
@@ -892,13 +926,10 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
     d.tx_count = (int)H->tx.size() - d.tx_first;
     H->blocks.push_back(d);
   }
-  /* pools padded like the reference planner's (engine.cpp:595-660) */
-  H->coef.assign(std::max((size_t)f.num_coefficients, cend + 64), 0);
-  if (f.num_coefficients) std::memcpy(H->coef.data(), f.coefficients, (size_t)f.num_coefficients * 4);
-  H->residual.assign(std::max((size_t)f.num_residual, rend + 64), 0);
-  if (f.num_residual) std::memcpy(H->residual.data(), f.residual, (size_t)f.num_residual * 4);
-  H->palette.assign(f.palette_map, f.palette_map + f.num_palette);
-  H->palette.resize(H->palette.size() + 8, 0);
+  /* pools padded like the reference planner's (engine.cpp:595-660): the
+   * caller's arrays are staged as they are, zeros after them */
+  const size_t coef_n = std::max((size_t)f.num_coefficients, cend + 64);
+  const size_t res_n = std::max((size_t)f.num_residual, rend + 64);
 
   /* The order the queue hands blocks out. The queued blocks form a graph:
    * a block depends on the queued blocks that own the cells it waits for
@@ -1060,7 +1091,7 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
         }
         const int sb = it.rows * d.w;
         if (cur.count + 1 > kMcItems || cur.samples + sb > kMcSamples || cur.mtotal + msz > kMcMid) close_task();
-        if (msz > kMcMid) return set_err(c, AV1H_ERR_UNSUPPORTED, "motion compensation window");
+        if (msz > kMcMid) return set_err(err, AV1H_ERR_UNSUPPORTED, "motion compensation window");
         it.px0 = cur.samples;
         for (int r = 0; r < 2; ++r) {
           McRefDev& q = it.r[r];
@@ -1109,14 +1140,14 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
       const av1h_deblock_edge& e = f.deblock[i];
       if ((e.vertical != 0) != (pass == 0)) continue;
       if (e.plane < 0 || e.plane > 2 || e.length < 0)
-        return set_err(c, AV1H_ERR_INVALID, "deblock edge %lld", (long long)i);
+        return set_err(err, AV1H_ERR_INVALID, "deblock edge %lld", (long long)i);
       const int reach = e.filter_size >= 16 ? 7 : (e.filter_size >= 8 ? 4 : 2);
       const int rows = L.out_h[e.plane] + (e.plane ? 32 : 64);
       const int across = e.vertical ? e.x : e.y, along = e.vertical ? e.y : e.x;
       const int across_lim = e.vertical ? L.stride[e.plane] : rows;
       const int along_lim = e.vertical ? rows : L.stride[e.plane];
       if (across < reach || across + reach > across_lim || along < 0 || along + e.length > along_lim)
-        return set_err(c, AV1H_ERR_INVALID, "deblock edge %lld out of range", (long long)i);
+        return set_err(err, AV1H_ERR_INVALID, "deblock edge %lld out of range", (long long)i);
       if (e.length == 0) continue;
       DevDeblock d;
       d.plane = e.plane;
@@ -1150,10 +1181,10 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
     const av1h_cdef_block& cb = f.cdef[i];
     if (cb.plane < 0 || cb.plane > 2 || cb.x < 0 || cb.y < 0 || cb.x >= L.grid_w[cb.plane] ||
         cb.y >= L.grid_h[cb.plane] || cb.damping < 0 || cb.damping > 15 || cb.dir > 7)
-      return set_err(c, AV1H_ERR_INVALID, "cdef block %lld", (long long)i);
+      return set_err(err, AV1H_ERR_INVALID, "cdef block %lld", (long long)i);
     const int bw = cb.plane ? 8 >> hdr.ss_x : 8, bh = cb.plane ? 8 >> hdr.ss_y : 8;
     if (cb.x % bw || cb.y % bh)
-      return set_err(c, AV1H_ERR_UNSUPPORTED, "cdef block %lld is not on the %dx%d grid", (long long)i, bw, bh);
+      return set_err(err, AV1H_ERR_UNSUPPORTED, "cdef block %lld is not on the %dx%d grid", (long long)i, bw, bh);
     DevCdef d{cb.plane, cb.x, cb.y, cb.pri_strength, cb.sec_strength, cb.damping, cb.dir, 0};
     H->cdef.push_back(d);
   }
@@ -1183,7 +1214,7 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
     if (hdr.restoration_type[p] == AV1H_RESTORE_NONE) continue;
     const int sy = p > 0 ? hdr.ss_y : 0;
     if (hdr.restoration_unit_log2[p] < 0 || hdr.restoration_unit_log2[p] > 10)
-      return set_err(c, AV1H_ERR_INVALID, "restoration unit size");
+      return set_err(err, AV1H_ERR_INVALID, "restoration unit size");
     const int usz = 1 << hdr.restoration_unit_log2[p];
     const int ucols = std::max(hdr.restoration_unit_cols[p], 1);
     const int pw = L.out_w[p], ph = L.out_h[p];
@@ -1247,18 +1278,18 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
   if (plan->has_grain) {
     const av1h_grain& g = hdr.grain;
     DevGrain& d = H->grain;
-    if (g.seed < 0 || g.seed > 65535) return set_err(c, AV1H_ERR_INVALID, "grain seed");
-    if (g.ar_coeff_lag < 0 || g.ar_coeff_lag > 3) return set_err(c, AV1H_ERR_INVALID, "grain lag");
+    if (g.seed < 0 || g.seed > 65535) return set_err(err, AV1H_ERR_INVALID, "grain seed");
+    if (g.ar_coeff_lag < 0 || g.ar_coeff_lag > 3) return set_err(err, AV1H_ERR_INVALID, "grain lag");
     if (g.num_y_points < 0 || g.num_y_points > 16 || g.num_cb_points < 0 || g.num_cb_points > 16 ||
         g.num_cr_points < 0 || g.num_cr_points > 16)
-      return set_err(c, AV1H_ERR_INVALID, "grain points");
+      return set_err(err, AV1H_ERR_INVALID, "grain points");
     /* the auto-regressive coefficients are coded as 8 bit values minus 128
      * (src/frontend/headers.cpp:716-725); the template filter multiplies
      * them with 24 bit products */
     for (int k = 0; k < 25; ++k)
       if ((k < 24 && (g.ar_coeffs_y[k] < -128 || g.ar_coeffs_y[k] > 127)) || g.ar_coeffs_cb[k] < -128 ||
           g.ar_coeffs_cb[k] > 127 || g.ar_coeffs_cr[k] < -128 || g.ar_coeffs_cr[k] > 127)
-        return set_err(c, AV1H_ERR_INVALID, "grain auto-regressive coefficient out of range");
+        return set_err(err, AV1H_ERR_INVALID, "grain auto-regressive coefficient out of range");
     d.seed = g.seed;
     d.bit_depth = hdr.bit_depth;
     d.num_y = g.num_y_points;
@@ -1309,8 +1340,8 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
   for (int64_t& v : plan->alg_bytes) v = 0;
   {
     /* part 0: inter samples written once and read from each reference, the
-     * coefficients of every transform (int16 each); the queue: intra samples
-     * written once */
+     * coefficients of every transform (int16 each); the queue: per queued
+     * sample its residual read and the picture sample written */
     for (int32_t i : H->inter_list) {
       const DevBlock& d = H->blocks[i];
       plan->alg_bytes[AV1H_STAGE_INTER] += (int64_t)d.w * d.h * (2 + 2 * (d.compound ? 2 : 1));
@@ -1318,7 +1349,7 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
     for (const DevTx& t : H->tx)
       if (!(t.flags & kTxToResidual)) plan->alg_bytes[AV1H_STAGE_INTER] += 2 * (int64_t)t.w * t.h;
     for (int32_t i : H->queue_list)
-      plan->alg_bytes[AV1H_STAGE_QUEUE] += 2 * (int64_t)H->blocks[i].w * H->blocks[i].h;
+      plan->alg_bytes[AV1H_STAGE_QUEUE] += 4 * (int64_t)H->blocks[i].w * H->blocks[i].h;
     bool has_edges[3] = {false, false, false};
     for (const DevDeblock& d : H->deblock) has_edges[d.plane] = true;
     int64_t disp = 0;
@@ -1345,79 +1376,44 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
   }
 
   /* one device buffer: the arrays back to back, 256 byte aligned */
-  struct Part {
-    const void* src;
-    size_t bytes;
-    size_t off;
+  H->parts.clear();
+  auto add = [&](int k, const void* src, size_t bytes, size_t fill = 0) {
+    H->part_of[k] = (int)H->parts.size();
+    H->parts.push_back(Host::Part{src, bytes, fill, 0});
   };
-  std::vector<Part> parts;
-  auto add = [&](const void* src, size_t bytes) {
-    parts.push_back(Part{src, bytes, 0});
-    return parts.size() - 1;
-  };
-  const size_t p_blocks = add(H->blocks.data(), H->blocks.size() * sizeof(DevBlock));
-  const size_t p_tx = add(H->tx.data(), H->tx.size() * sizeof(DevTx));
-  const size_t p_inter = add(H->inter_list.data(), H->inter_list.size() * 4);
-  const size_t p_queue = add(H->queue_list.data(), H->queue_list.size() * 4);
-  const size_t p_spec = add(H->special_list.data(), H->special_list.size() * 4);
-  const size_t p_obmc = add(H->obmc.data(), H->obmc.size() * sizeof(DevObmc));
-  const size_t p_coef = add(H->coef.data(), H->coef.size() * 4);
-  const size_t p_res = add(H->residual.data(), H->residual.size() * 4);
-  const size_t p_pal = add(H->palette.data(), H->palette.size());
-  const size_t p_db = add(H->deblock.data(), H->deblock.size() * sizeof(DevDeblock));
-  const size_t p_cdef = add(H->cdef.data(), H->cdef.size() * sizeof(DevCdef));
-  const size_t p_lr = add(H->lr.data(), H->lr.size() * sizeof(DevLr));
-  const size_t p_mcit = add(H->mc_items.data(), H->mc_items.size() * sizeof(McItemDev));
-  const size_t p_mcta = add(H->mc_task.data(), H->mc_task.size() * sizeof(McTaskDev));
-  const size_t p_itx = add(H->itx_all.data(), H->itx_all.size() * 4);
-  const size_t p_itxo = add(itx_off, sizeof(itx_off));
-  const size_t p_dbc = add(H->db_chunk.data(), H->db_chunk.size() * 4);
-  const size_t p_cdir = add(H->cdef_dir_list.data(), H->cdef_dir_list.size() * 4);
-  const size_t p_lrt = add(H->lr_tiles.data(), H->lr_tiles.size() * 4);
-  const size_t p_grain = add(&H->grain, sizeof(DevGrain));
+  add(kPartBlocks, H->blocks.data(), H->blocks.size() * sizeof(DevBlock));
+  add(kPartTx, H->tx.data(), H->tx.size() * sizeof(DevTx));
+  add(kPartInter, H->inter_list.data(), H->inter_list.size() * 4);
+  add(kPartQueue, H->queue_list.data(), H->queue_list.size() * 4);
+  add(kPartSpecial, H->special_list.data(), H->special_list.size() * 4);
+  add(kPartObmc, H->obmc.data(), H->obmc.size() * sizeof(DevObmc));
+  add(kPartCoef, f.coefficients, (size_t)f.num_coefficients * 4, (coef_n - (size_t)f.num_coefficients) * 4);
+  add(kPartResidual, f.residual, (size_t)f.num_residual * 4, (res_n - (size_t)f.num_residual) * 4);
+  add(kPartPalette, f.palette_map, (size_t)f.num_palette, 8);
+  add(kPartDeblock, H->deblock.data(), H->deblock.size() * sizeof(DevDeblock));
+  add(kPartCdef, H->cdef.data(), H->cdef.size() * sizeof(DevCdef));
+  add(kPartLr, H->lr.data(), H->lr.size() * sizeof(DevLr));
+  add(kPartMcItems, H->mc_items.data(), H->mc_items.size() * sizeof(McItemDev));
+  add(kPartMcTasks, H->mc_task.data(), H->mc_task.size() * sizeof(McTaskDev));
+  add(kPartItx, H->itx_all.data(), H->itx_all.size() * 4);
+  std::memcpy(H->itx_off, itx_off, sizeof(itx_off));
+  add(kPartItxOffs, H->itx_off, sizeof(itx_off));
+  add(kPartDbChunk, H->db_chunk.data(), H->db_chunk.size() * 4);
+  add(kPartCdefDir, H->cdef_dir_list.data(), H->cdef_dir_list.size() * 4);
+  add(kPartLrTiles, H->lr_tiles.data(), H->lr_tiles.size() * 4);
+  add(kPartGrain, &H->grain, sizeof(DevGrain));
   size_t total = 0;
-  for (Part& pt : parts) {
+  for (Host::Part& pt : H->parts) {
     pt.off = total;
-    total += align256(pt.bytes);
+    total += align256(pt.bytes + pt.fill);
   }
-  if (plan->dbytes < total) {
-    if (plan->dbuf) (void)hipFree(plan->dbuf);
-    plan->dbuf = nullptr;
-    plan->dbytes = 0;
-    HIP_OK(c, hipMalloc(&plan->dbuf, total));
-    plan->dbytes = total;
-  }
-  plan->hbuf.resize(total);
-  for (Part& pt : parts)
-    if (pt.bytes) std::memcpy(plan->hbuf.data() + pt.off, pt.src, pt.bytes);
-  uint8_t* db = static_cast<uint8_t*>(plan->dbuf);
-  auto at = [&](size_t k) { return db + parts[k].off; };
+  H->total = total;
   FramePlanDev& P = plan->dev;
-  P.blocks = reinterpret_cast<const DevBlock*>(at(p_blocks));
-  P.tx = reinterpret_cast<const DevTx*>(at(p_tx));
-  P.obmc = reinterpret_cast<const DevObmc*>(at(p_obmc));
-  P.special_list = reinterpret_cast<const int32_t*>(at(p_spec));
-  P.n_special = (int32_t)H->special_list.size();
-  P.inter_list = reinterpret_cast<const int32_t*>(at(p_inter));
-  P.queue_list = reinterpret_cast<const int32_t*>(at(p_queue));
-  P.coef = reinterpret_cast<const int32_t*>(at(p_coef));
-  P.residual = reinterpret_cast<const int32_t*>(at(p_res));
-  P.palette = at(p_pal);
-  P.deblock = reinterpret_cast<const DevDeblock*>(at(p_db));
-  P.cdef = reinterpret_cast<const DevCdef*>(at(p_cdef));
-  P.lr = reinterpret_cast<const DevLr*>(at(p_lr));
-  P.grain = reinterpret_cast<const DevGrain*>(at(p_grain));
-  P.itx_list = reinterpret_cast<const int32_t*>(at(p_itx));
   std::memcpy(P.itx_off, itx_off, sizeof(itx_off));
-  P.itx_offs = reinterpret_cast<const int32_t*>(at(p_itxo));
-  P.mc_items = reinterpret_cast<const McItemDev*>(at(p_mcit));
-  P.mc_task = reinterpret_cast<const McTaskDev*>(at(p_mcta));
+  P.n_special = (int32_t)H->special_list.size();
   P.mc_tasks = mc_tasks;
-  P.db_chunk = reinterpret_cast<const int32_t*>(at(p_dbc));
-  P.cdef_dir_list = reinterpret_cast<const int32_t*>(at(p_cdir));
   P.n_cdef_luma = n_cdef_luma;
   P.n_cdef_dir = (int32_t)H->cdef_dir_list.size();
-  P.lr_tiles = reinterpret_cast<const int2*>(at(p_lrt));
   P.n_lr_tiles = (int32_t)(H->lr_tiles.size() / 2);
   P.n_inter = (int)H->inter_list.size();
   P.n_queue = (int)H->queue_list.size();
@@ -1431,10 +1427,57 @@ int flatten(Ctx* c, const av1h_frame_desc& f, Plan* plan, Host* H) {
   return AV1H_OK;
 }
 
-int upload(Ctx* c, Plan* plan) {
-  /* staged through a pinned buffer so the copy is a DMA */
-  const size_t n = plan->hbuf.size();
+/* Writes the arrays of a flattened frame into `dst` (Host::total bytes, the
+ * staging buffer of its upload), zeros after the pools that are padded. */
+void stage_parts(const Host& H, uint8_t* dst) {
+  for (const Host::Part& pt : H.parts) {
+    if (pt.bytes) std::memcpy(dst + pt.off, pt.src, pt.bytes);
+    if (pt.fill) std::memset(dst + pt.off + pt.bytes, 0, pt.fill);
+  }
+}
+
+/* Points the plan's device descriptor at its arrays in `plan->dbuf`. */
+void bind_dev(Plan* plan, const Host& H) {
+  uint8_t* db = static_cast<uint8_t*>(plan->dbuf);
+  auto at = [&](int k) { return db + H.parts[(size_t)H.part_of[k]].off; };
+  FramePlanDev& P = plan->dev;
+  P.blocks = reinterpret_cast<const DevBlock*>(at(kPartBlocks));
+  P.tx = reinterpret_cast<const DevTx*>(at(kPartTx));
+  P.obmc = reinterpret_cast<const DevObmc*>(at(kPartObmc));
+  P.special_list = reinterpret_cast<const int32_t*>(at(kPartSpecial));
+  P.inter_list = reinterpret_cast<const int32_t*>(at(kPartInter));
+  P.queue_list = reinterpret_cast<const int32_t*>(at(kPartQueue));
+  P.coef = reinterpret_cast<const int32_t*>(at(kPartCoef));
+  P.residual = reinterpret_cast<const int32_t*>(at(kPartResidual));
+  P.palette = at(kPartPalette);
+  P.deblock = reinterpret_cast<const DevDeblock*>(at(kPartDeblock));
+  P.cdef = reinterpret_cast<const DevCdef*>(at(kPartCdef));
+  P.lr = reinterpret_cast<const DevLr*>(at(kPartLr));
+  P.grain = reinterpret_cast<const DevGrain*>(at(kPartGrain));
+  P.itx_list = reinterpret_cast<const int32_t*>(at(kPartItx));
+  P.itx_offs = reinterpret_cast<const int32_t*>(at(kPartItxOffs));
+  P.mc_items = reinterpret_cast<const McItemDev*>(at(kPartMcItems));
+  P.mc_task = reinterpret_cast<const McTaskDev*>(at(kPartMcTasks));
+  P.db_chunk = reinterpret_cast<const int32_t*>(at(kPartDbChunk));
+  P.cdef_dir_list = reinterpret_cast<const int32_t*>(at(kPartCdefDir));
+  P.lr_tiles = reinterpret_cast<const int2*>(at(kPartLrTiles));
+}
+
+/* The synchronous path's upload: the scratch plan's device buffer (grown when
+ * it must), staged through the context's pinned buffer so the copy is a DMA,
+ * on the decode stream. */
+int upload(Ctx* c, Plan* plan, const Host& H) {
+  const size_t n = H.total;
+  if (plan->dbytes < n) {
+    if (plan->dbuf) (void)hipFree(plan->dbuf);
+    plan->dbuf = nullptr;
+    plan->dbytes = 0;
+    HIP_OK(c, hipMalloc(&plan->dbuf, n));
+    plan->dbytes = n;
+  }
+  bind_dev(plan, H);
   if (c->host_pin_cap < n) {
+    HIP_OK(c, hipStreamSynchronize(c->stream));
     if (c->host_pin) (void)hipHostFree(c->host_pin);
     c->host_pin = nullptr;
     c->host_pin_cap = 0;
@@ -1442,9 +1485,99 @@ int upload(Ctx* c, Plan* plan) {
     c->host_pin_cap = n;
   }
   HIP_OK(c, hipStreamSynchronize(c->stream));
-  std::memcpy(c->host_pin, plan->hbuf.data(), n);
+  stage_parts(H, static_cast<uint8_t*>(c->host_pin));
   HIP_OK(c, hipMemcpyAsync(plan->dbuf, c->host_pin, n, hipMemcpyHostToDevice, c->stream));
   c->dirty = true;
+  return AV1H_OK;
+}
+
+/* ---- pre-planning from several threads ------------------------------------
+ * A plan owns a device buffer from the context's recycling pool and is
+ * uploaded through a stager (a pinned buffer and a stream of its own), so
+ * any number of threads plan at once, without the context's lock and without
+ * waiting for the decode stream. */
+constexpr size_t kPlanPoolBytes = (size_t)2 << 30; /* device bytes a context keeps for reuse */
+constexpr size_t kPlanGrain = (size_t)1 << 20;
+
+int take_buffer(Ctx* c, size_t n, void** out, size_t* got, std::string* err) {
+  {
+    std::lock_guard<std::mutex> lk(c->plan_mu);
+    size_t best = c->plan_bufs.size();
+    for (size_t i = 0; i < c->plan_bufs.size(); ++i)
+      if (c->plan_bufs[i].first >= n && c->plan_bufs[i].first <= 2 * n + kPlanGrain &&
+          (best == c->plan_bufs.size() || c->plan_bufs[i].first < c->plan_bufs[best].first))
+        best = i;
+    if (best < c->plan_bufs.size()) {
+      *got = c->plan_bufs[best].first;
+      *out = c->plan_bufs[best].second;
+      c->plan_buf_bytes -= *got;
+      c->plan_bufs.erase(c->plan_bufs.begin() + (std::ptrdiff_t)best);
+      return AV1H_OK;
+    }
+  }
+  const size_t sz = (n + kPlanGrain - 1) / kPlanGrain * kPlanGrain;
+  const hipError_t e = hipMalloc(out, sz);
+  if (e != hipSuccess) {
+    *out = nullptr;
+    return set_err(err, AV1H_ERR_DEVICE, "hipMalloc of a frame plan (%zu bytes): %s", sz, hipGetErrorString(e));
+  }
+  *got = sz;
+  return AV1H_OK;
+}
+
+/* back to the pool, or freed when the pool is full */
+void give_buffer(Ctx* c, void* p, size_t n) {
+  if (!p) return;
+  {
+    std::lock_guard<std::mutex> lk(c->plan_mu);
+    if (c->plan_buf_bytes + n <= kPlanPoolBytes) {
+      c->plan_bufs.emplace_back(n, p);
+      c->plan_buf_bytes += n;
+      return;
+    }
+  }
+  (void)hipFree(p);
+}
+
+int upload_plan(Ctx* c, Plan* plan, const Host& H, std::string* err) {
+  const size_t n = H.total;
+  int rc = take_buffer(c, n, &plan->dbuf, &plan->dbytes, err);
+  if (rc != AV1H_OK) return rc;
+  bind_dev(plan, H);
+  Stager* st = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(c->plan_mu);
+    if (!c->stagers.empty()) {
+      st = c->stagers.back();
+      c->stagers.pop_back();
+    }
+  }
+  if (!st) {
+    st = new Stager;
+    if (hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking) != hipSuccess) {
+      delete st;
+      return set_err(err, AV1H_ERR_DEVICE, "creating an upload stream");
+    }
+  }
+  hipError_t e = hipSuccess;
+  if (st->cap < n) {
+    if (st->pin) (void)hipHostFree(st->pin);
+    st->pin = nullptr;
+    st->cap = 0;
+    const size_t sz = (n + 4 * kPlanGrain - 1) / (4 * kPlanGrain) * (4 * kPlanGrain);
+    e = hipHostMalloc(&st->pin, sz, hipHostMallocDefault);
+    if (e == hipSuccess) st->cap = sz;
+  }
+  if (e == hipSuccess) {
+    stage_parts(H, static_cast<uint8_t*>(st->pin));
+    e = hipMemcpyAsync(plan->dbuf, st->pin, n, hipMemcpyHostToDevice, st->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(st->stream);
+  }
+  {
+    std::lock_guard<std::mutex> lk(c->plan_mu);
+    c->stagers.push_back(st);
+  }
+  if (e != hipSuccess) return set_err(err, AV1H_ERR_DEVICE, "frame plan upload: %s", hipGetErrorString(e));
   return AV1H_OK;
 }
 
@@ -1452,8 +1585,6 @@ void free_plan(Plan* p) {
   if (p->dbuf) (void)hipFree(p->dbuf);
   p->dbuf = nullptr;
   p->dbytes = 0;
-  p->hbuf.clear();
-  p->hbuf.shrink_to_fit();
 }
 
 /* grows the per geometry buffers; the pool is re-created when it must grow
@@ -2121,6 +2252,14 @@ void av1h_destroy(av1h_ctx* h) {
   if (c->batch_pin) (void)hipHostFree(c->batch_pin);
   if (c->rb_pin) (void)hipHostFree(c->rb_pin);
   free_plan(&c->scratch_plan);
+  for (auto& b : c->plan_bufs) (void)hipFree(b.second);
+  c->plan_bufs.clear();
+  for (Stager* st : c->stagers) {
+    if (st->stream) (void)hipStreamDestroy(st->stream);
+    if (st->pin) (void)hipHostFree(st->pin);
+    delete st;
+  }
+  c->stagers.clear();
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto& e : c->sev)
@@ -2174,9 +2313,9 @@ int av1h_decode(av1h_ctx* h, const av1h_frame_desc* f, int keep, int reserved, c
   (void)hipSetDevice(c->device);
   const double t0 = now_ms();
   Host host;
-  int rc = flatten(c, *f, &c->scratch_plan, &host);
+  int rc = flatten(*f, &c->scratch_plan, &host, &c->err);
   if (rc != AV1H_OK) return rc;
-  rc = upload(c, &c->scratch_plan);
+  rc = upload(c, &c->scratch_plan, host);
   if (rc != AV1H_OK) return rc;
   return run_plan(c, &c->scratch_plan, keep, reserved, out, info, slot_out, t0);
 }
@@ -2186,6 +2325,7 @@ int av1h_decode_wire(av1h_ctx* h, const uint8_t* wire, size_t len, int keep, int
   av1h_frame_desc d;
   if (!h) return AV1H_ERR_INVALID;
   if (av1h_wire_parse(wire, len, &d) != AV1H_OK) {
+    std::lock_guard<std::mutex> lk(h->c.mu);
     h->c.err = "not a wire frame";
     return AV1H_ERR_INVALID;
   }
@@ -2194,24 +2334,25 @@ int av1h_decode_wire(av1h_ctx* h, const uint8_t* wire, size_t len, int keep, int
 
 /* Pre-planning (reference Decoder::plan_frame / decode_reference_planned,
  * av1gpu.hpp:670-712): the syntax is flattened and uploaded into device
- * memory owned by the plan, so the decode itself starts from HBM. */
+ * memory owned by the plan, so the decode itself starts from HBM. Safe to
+ * call from several threads at once and while the context decodes: it takes
+ * the context's lock only to report an error. */
 int av1h_plan_frame(av1h_ctx* h, const av1h_frame_desc* f, void** plan_out) {
   if (!h || !f || !plan_out) return AV1H_ERR_INVALID;
   Ctx* c = &h->c;
-  std::lock_guard<std::mutex> lk(c->mu);
   (void)hipSetDevice(c->device);
   Plan* p = new Plan;
   Host host;
-  int rc = flatten(c, *f, p, &host);
-  if (rc == AV1H_OK) rc = upload(c, p);
-  if (rc == AV1H_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = AV1H_ERR_DEVICE;
+  std::string err;
+  int rc = flatten(*f, p, &host, &err);
+  if (rc == AV1H_OK) rc = upload_plan(c, p, host, &err);
   if (rc != AV1H_OK) {
-    free_plan(p);
+    give_buffer(c, p->dbuf, p->dbytes);
     delete p;
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->err = err;
     return rc;
   }
-  p->hbuf.clear();
-  p->hbuf.shrink_to_fit();
   *plan_out = p;
   return AV1H_OK;
 }
@@ -2220,6 +2361,7 @@ int av1h_plan_frame_wire(av1h_ctx* h, const uint8_t* wire, size_t len, void** pl
   av1h_frame_desc d;
   if (!h) return AV1H_ERR_INVALID;
   if (av1h_wire_parse(wire, len, &d) != AV1H_OK) {
+    std::lock_guard<std::mutex> lk(h->c.mu);
     h->c.err = "not a wire frame";
     return AV1H_ERR_INVALID;
   }
@@ -2237,8 +2379,13 @@ int av1h_decode_planned(av1h_ctx* h, void* plan, int keep, int reserved, const a
 
 void av1h_discard_plan(av1h_ctx* h, void* plan) {
   if (!plan) return;
-  if (h) (void)hipSetDevice(h->c.device);
   Plan* p = static_cast<Plan*>(plan);
+  if (h) {
+    (void)hipSetDevice(h->c.device);
+    give_buffer(&h->c, p->dbuf, p->dbytes);
+    p->dbuf = nullptr;
+    p->dbytes = 0;
+  }
   free_plan(p);
   delete p;
 }
@@ -2674,8 +2821,8 @@ int av1h_decode_batch(av1h_ctx* const* ctxs, const av1h_frame_desc* const* frame
     for (int k = 0; k < i; ++k)
       if (cs[k] == cs[i]) return set_err(lead, AV1H_ERR_INVALID, "a context appears twice in one batch");
     Host host;
-    rc = flatten(cs[i], *frames[i], &cs[i]->scratch_plan, &host);
-    if (rc == AV1H_OK) rc = upload(cs[i], &cs[i]->scratch_plan);
+    rc = flatten(*frames[i], &cs[i]->scratch_plan, &host, &cs[i]->err);
+    if (rc == AV1H_OK) rc = upload(cs[i], &cs[i]->scratch_plan, host);
     if (rc != AV1H_OK) return cs[i] == lead ? rc : set_err(lead, rc, "frame %d: %s", i, cs[i]->err.c_str());
     js[i] = JobSpec{cs[i], &cs[i]->scratch_plan, 1, -1, nullptr, nullptr, slots_out ? &slots_out[i] : nullptr, -1};
   }

@@ -9,21 +9,25 @@ self-guided restoration, superres 3/4 (coded width 1440), film grain with 24
 luma AR coefficients, every frame packed to NV12 at a 2048 byte pitch. The
 syntax is synthetic and seeded (av1ongpu_amd/csrc/synth.cpp).
 
-A step decodes the whole clip in order: each frame's syntax has been flattened
-and uploaded to HBM beforehand (av1h_plan_frame, the reference's plan_frame
-hook), the decode runs every stage on the device, keeps the reference slots on
-the device, and packs NV12 on the device. `value` is display luma Mpixel/s over
-the timed steps. Frame latency (submit to NV12 packed, each decode call
-returns once its frame is complete on the device) is reported as p50/p99.
+A step decodes the whole clip in order, starting from each frame's syntax in
+host memory (the FrameSyntax wire bytes): the frames go through the ingest
+pipeline (av1h_pipe_*), where planner threads flatten and upload frame N+1..
+while frame N decodes, the decode runs every stage on the device, keeps the
+reference slots on the device, and packs NV12 on the device. `value` is
+display luma Mpixel/s over the timed steps, from the first submit to the last
+frame complete. Frame latency inside the pipeline (submit to NV12 packed) is
+reported as p50/p99.
 
-Also reported: the same clip through av1h_decode_wire, whose time includes
-flattening and the upload of the syntax (`ingest_inclusive`); the dominant
-kernel's roofline from HIP events around each stage, on the decode stream,
-in one more pass over the clip after the timed steps (the timed steps replay
-captured launch sequences; the event pass launches eagerly); the CPU oracle
-on a bounded sample of the same clip, timed on this host, and the
-bit-exactness of the GPU output on that sample; and the 16-stream mix,
-batched vs plain (`batched_mix`).
+Also reported: `device_only`, the same clip from plans already resident in
+HBM (one decode call per frame, its call latency p50/p99); `ingest_serial`,
+each frame flattened, uploaded and decoded before the next on one thread; the
+dominant kernel's roofline from HIP events around each stage, on the decode
+stream, in one more pass over the clip after the timed steps (the timed steps
+replay captured launch sequences; the event pass launches eagerly); the CPU
+oracle on a bounded sample of the same clip, timed on this host, and the
+bit-exactness of the GPU output on that sample; the 16-stream mix, batched vs
+plain (`batched_mix`), with the learned schedule (`cost_model`); and the
+serving loop (`serving`).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -40,9 +44,8 @@ METRIC = "decoded Mpixel/s on 1 MI355X, batched vs plain, bit-exact; p99 frame l
 HBM_PEAK_GBS = 8000.0
 NV12_PITCH = 2048
 # the kernel that dominates each timed stage (the stage's events bracket it)
-STAGE_KERNEL = {"inter": "k_mc", "queue": "k_queue", "deblock": "k_deblock", "cdef": "k_cdef<64>",
-                "superres": "k_upscale", "lr": "k_lr", "store": "__amd_rocclr_copyBuffer",
-                "grain": "k_grain_apply"}
+STAGE_KERNEL = {"inter": "k_mc", "queue": "k_queue", "deblock": "k_deblock", "cdef": "k_cdef",
+                "superres": "k_upscale", "lr": "k_lr", "store": "k_copy", "grain": "k_grain_apply"}
 WORKLOADS = {
     "c2": "1920x1080 10-bit 4:2:0, 60 frames, hierarchical GOP 16 with 3 references, compound "
           "(distance weighted + average), transforms 4x4-64x64 incl. identity and flip-ADST, "
@@ -89,6 +92,17 @@ def display_pixels(clip):
     return w * h
 
 
+def default_planners():
+    """Host threads for planning: the cores this process may use, less two
+    (the decode worker and the submitting thread), at most 14."""
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
+    return max(1, min(14, cores - 2))
+
+
 def pct(values, q):
     v = sorted(values)
     if not v:
@@ -100,17 +114,23 @@ def pct(values, q):
 def load_traffic(workload, kernel):
     """HBM bytes per launch of a kernel from the committed rocprofv3 counter
     summary (FETCH_SIZE and WRITE_SIZE, scaled by a copy kernel of known size),
-    or None."""
+    or None when the summary does not cover the kernel on this workload (said
+    on stderr)."""
     path = os.path.join(REPO, "profiles", "hbm_traffic.json")
     try:
         with open(path) as fh:
             t = json.load(fh)
-        if t.get("workload") != workload:
-            return None
-        k = t["kernels"].get(kernel)
-        return None if k is None else int(k["bytes_per_launch"])
-    except (OSError, ValueError, KeyError):
+    except (OSError, ValueError) as e:
+        log(f"WARNING: no HBM traffic summary ({e}): roofline traffic left null")
         return None
+    if t.get("workload") != workload:
+        log(f"WARNING: the HBM traffic summary is of workload {t.get('workload')}, not {workload}")
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    if k is None:
+        log(f"WARNING: the HBM traffic summary has no kernel {kernel}: roofline traffic left null")
+        return None
+    return int(k["bytes_per_launch"])
 
 
 def cpu_baseline(clip, seconds, gpu_hashes):
@@ -490,6 +510,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--planners", type=int, default=0, help="planner threads of the ingest pipeline (0: automatic)")
+    ap.add_argument("--depth", type=int, default=32, help="frames in flight in the ingest pipeline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline and the parity sample")
     ap.add_argument("--no-mix", action="store_true", help="skip the 16-stream batched vs plain mix")
@@ -538,37 +560,74 @@ def main():
 
     # every decode packs NV12 on the device as its last launch
     dec.set_decode_nv12(NV12_PITCH)
-    # --- ingest-inclusive pass: syntax flattened and uploaded inside the timing
+    # --- serial ingest: one frame at a time, flattening and upload inside
     t0 = time.perf_counter()
     for f in range(n):
         slot, _ = dec.decode_wire(wires[f], keep=keep_slot[f] >= 0, reserved=keep_slot[f], read_back=False)
         for s in releases[f]:
             dec.release(s)
-    ingest_s = time.perf_counter() - t0
-    log(f"ingest-inclusive pass: {ingest_s * 1e3 / n:.2f} ms/frame")
+    serial_s = time.perf_counter() - t0
+    log(f"serial ingest pass: {serial_s * 1e3 / n:.2f} ms/frame")
 
-    # --- plans resident in HBM
+    # --- the headline: every frame submitted as syntax (wire bytes) to the
+    # ingest pipeline, planned by host threads while earlier frames decode
+    planners = args.planners if args.planners > 0 else default_planners()
+    pipe = pkg.Pipe(dec, planner_threads=planners, depth=args.depth)
+    next_seq = [0]
+
+    def pipe_step():
+        base = next_seq[0]
+        for f in range(n):
+            pipe.submit(wires[f], keep=keep_slot[f] >= 0, reserved=keep_slot[f], release=releases[f])
+        pipe.wait()
+        next_seq[0] = base + n
+        return base
+
+    for i in range(args.warmup):
+        pipe_step()
+        log(f"warmup step {i + 1}/{args.warmup} done")
+    bases = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        bases.append(pipe_step())
+        log(f"step {i + 1}/{args.steps} done")
+    elapsed = time.perf_counter() - t0
+    pipe_lat = [r[3] - r[2] for b in bases for r in (pipe.result(b + f) for f in range(n))]
+    # the same pass with a device checksum of every picture (untimed), for
+    # the comparison with the plain path below
+    dec.set_output_hash(True)
+    b = pipe_step()
+    pipe_hashes = [pipe.result(b + f)[4] for f in range(n)]
+    dec.set_output_hash(False)
+    pipe.close()
+
+    # --- device only: plans resident in HBM, one decode call per frame
     t = time.perf_counter()
     plans = [dec.plan_wire(w) for w in wires]
     log(f"planned {n} frames in {time.perf_counter() - t:.1f} s")
 
-    def step(lat):
+    def step(lat, hashes=None):
         for f in range(n):
             t1 = time.perf_counter()
             dec.decode_planned(plans[f], keep=keep_slot[f] >= 0, reserved=keep_slot[f])
             lat.append(time.perf_counter() - t1)
+            if hashes is not None:
+                hashes.append(dec.last_hash())
             for s in releases[f]:
                 dec.release(s)
 
-    for i in range(args.warmup):
-        step([])
-        log(f"warmup step {i + 1}/{args.warmup} done")
+    step([])
     lat = []
     t0 = time.perf_counter()
     for i in range(args.steps):
         step(lat)
-        log(f"step {i + 1}/{args.steps} done")
-    elapsed = time.perf_counter() - t0
+    dev_elapsed = time.perf_counter() - t0
+    dec.set_output_hash(True)
+    plain_hashes = []
+    step([], plain_hashes)
+    dec.set_output_hash(False)
+    pipe_mism = sum(1 for a, b in zip(pipe_hashes, plain_hashes) if a != b)
+    log(f"pipelined vs plain decode: {pipe_mism} of {n} pictures differ")
     # one more pass with HIP events around every stage (launched eagerly)
     # for the per stage times and the roofline
     dec.set_stage_timing(True)
@@ -580,7 +639,7 @@ def main():
 
     value = args.steps * n * px / elapsed / 1e6
     ms_step = elapsed * 1e3 / max(args.steps, 1)
-    # dominant kernel: the stage with the most device time in the timed steps
+    # dominant kernel: the stage with the most device time
     dom = max((s for s in stages if stages[s][1] > 0), key=lambda s: stages[s][0])
     ms, cnt, byts = stages[dom]
     avg_ms = ms / cnt
@@ -600,9 +659,21 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
         "data": "synthetic (seeded FrameSyntax of the named shape)",
         "config": {"workload": f"{args.workload}: {WORKLOADS[args.workload]}", "frames": n, "seed": args.seed,
-                   "path": "plain per-frame decode, syntax pre-planned into HBM, NV12 packed on the device"},
-        "frame_latency_ms": {"p50": round(pct(lat, 50) * 1e3, 3), "p99": round(pct(lat, 99) * 1e3, 3)},
-        "ingest_inclusive": {"value": round(n * px / ingest_s / 1e6, 2), "unit": "Mpixel/s"},
+                   "path": f"plain per-frame decode from submitted syntax (wire bytes in host memory): "
+                           f"flattening and upload by {planners} planner threads overlapped with the "
+                           f"decode of earlier frames (ingest pipeline, {args.depth} frames in flight), "
+                           f"reference slots on the device, NV12 packed on the device"},
+        "frame_latency_ms": {"p50": round(pct(pipe_lat, 50), 3), "p99": round(pct(pipe_lat, 99), 3),
+                             "note": "submit to complete inside the pipeline, frames submitted as fast as "
+                                     "the pipeline admits them"},
+        "device_only": {"value": round(args.steps * n * px / dev_elapsed / 1e6, 2), "unit": "Mpixel/s",
+                        "path": "the same clip from plans resident in HBM, one decode call per frame",
+                        "call_latency_ms": {"p50": round(pct(lat, 50) * 1e3, 3),
+                                            "p99": round(pct(lat, 99) * 1e3, 3)}},
+        "ingest_serial": {"value": round(n * px / serial_s / 1e6, 2), "unit": "Mpixel/s",
+                          "path": "one thread: each frame flattened, uploaded and decoded before the next"},
+        "pipeline_vs_plain": {"frames_checked": n, "mismatched_frames": pipe_mism,
+                              "method": "device checksum of every picture, pipelined vs device-only decode"},
         "stages": stage_report,
         "roofline": roofline,
         "cpu_baseline": None,

@@ -600,6 +600,42 @@ int av1h_cm_train(av1h_costmodel* model, const float* features, const float* y_u
 /* Predicts the plan's superblock costs and schedules its queue by them. */
 int av1h_plan_schedule_model(av1h_ctx* ctx, void* plan, av1h_costmodel* model);
 
+/* -------------------------------------------------------- ingest pipeline */
+/*
+ * One stream's frames through planner threads and an in-order decode worker
+ * (reference: the planner and worker threads of dsfilter/decode_core.hpp:
+ * 164-292 around Decoder::plan_frame / decode_reference_planned, kPipeDepth
+ * frames in flight; here `planner_threads` planners). Each submitted frame is
+ * flattened and uploaded by av1h_plan_frame on some planner while earlier
+ * frames decode; the worker decodes in submission order with
+ * av1h_decode_planned(keep, reserved) (no read back: the picture stays in its
+ * slot, and in the NV12 buffer when the context packs NV12 with every
+ * decode), then releases the slots the frame lists. At most `depth` frames
+ * are in flight: submit blocks until one completes.
+ */
+typedef struct av1h_pipe av1h_pipe;
+int av1h_pipe_create(av1h_ctx* ctx, int planner_threads, int depth, av1h_pipe** out);
+/* The frame's arrays (desc's pointers, or the wire bytes) are borrowed until
+ * the frame completes (av1h_pipe_wait past its number). Returns the frame's
+ * sequence number (0, 1, ...) or a negative error. */
+int64_t av1h_pipe_submit(av1h_pipe* pipe, const av1h_frame_desc* desc, int keep, int reserved,
+                         const int32_t* release, int nrelease);
+int64_t av1h_pipe_submit_wire(av1h_pipe* pipe, const uint8_t* wire, size_t len, int keep, int reserved,
+                              const int32_t* release, int nrelease);
+/* Waits until frames 0..seq are complete (seq < 0: every frame submitted).
+ * Returns AV1H_OK or the status of the first failed frame among them, whose
+ * number goes to *failed_seq (-1 when none failed). */
+int av1h_pipe_wait(av1h_pipe* pipe, int64_t seq, int64_t* failed_seq);
+/* A completed frame's slot (-1 when not kept), status, times
+ * (av1h_server_now_ms clock) and, when the context checksums every decoded
+ * picture (av1h_set_output_hash), its checksum (else 0). */
+int av1h_pipe_result(av1h_pipe* pipe, int64_t seq, int32_t* slot, int32_t* status, double* submit_ms,
+                     double* done_ms, uint64_t* hash);
+/* The first failure's message ("" when none). */
+const char* av1h_pipe_error(av1h_pipe* pipe);
+/* Waits for every submitted frame, then ends the threads. */
+void av1h_pipe_destroy(av1h_pipe* pipe);
+
 /* ---------------------------------------------------------------- serving */
 /*
  * A serving loop over av1h_decode_planned_batch (no counterpart in the

@@ -1,0 +1,81 @@
+"""The ingest pipeline (av1h_pipe_*): frames planned by several host threads
+while earlier frames decode, decoded in submission order. Every picture must
+carry the same device checksum as the same frame decoded by the plain
+synchronous path (av1h_decode_wire), which the golden tests hold to the
+reference; a frame whose syntax the planner refuses fails alone, with its
+number and message, and the frames after it still decode."""
+import numpy as np
+import pytest
+
+import av1ongpu_amd as pkg
+
+pytestmark = pytest.mark.gpu
+
+CLIPS = [("c2_mini", 1, {}), ("c1_mini", 2, {"frames": 8}), ("tools_mini", 1, {}),
+         ("c1_mini", 41, {"width": 426, "height": 240, "frames": 3}),
+         ("contract_mini", 32, {"avail_auto": 0.5})]
+
+
+def _plain_hashes(clip, keep, rel, refs):
+    dec = pkg.HipDecoder()
+    dec.set_output_hash(True)
+    out = []
+    for f in range(clip.frames):
+        dec.decode_wire(pkg.wire_set_slots(clip.wire(f), refs[f]), keep=keep[f] >= 0, reserved=keep[f],
+                        read_back=False)
+        out.append(dec.last_hash())
+        for s in rel[f]:
+            dec.release(s)
+    dec.close()
+    return out
+
+
+@pytest.mark.parametrize("workload,seed,over", CLIPS, ids=[f"{w}-{s}" for w, s, _ in CLIPS])
+@pytest.mark.parametrize("threads,depth", [(1, 1), (4, 3), (8, 16)])
+def test_pipe_matches_plain(workload, seed, over, threads, depth):
+    clip = pkg.Clip(workload, seed, **over)
+    keep, rel, refs = pkg.slot_schedule(clip)
+    want = _plain_hashes(clip, keep, rel, refs)
+    dec = pkg.HipDecoder()
+    dec.set_output_hash(True)
+    pipe = pkg.Pipe(dec, planner_threads=threads, depth=depth)
+    seqs = [pipe.submit(pkg.wire_set_slots(clip.wire(f), refs[f]), keep=keep[f] >= 0, reserved=keep[f],
+                        release=rel[f]) for f in range(clip.frames)]
+    pipe.wait()
+    assert seqs == list(range(clip.frames))
+    for f in range(clip.frames):
+        slot, status, t_sub, t_done, h = pipe.result(f)
+        assert status == 0
+        assert slot == keep[f]
+        assert t_done >= t_sub
+        assert h == want[f], f"{workload} seed {seed} frame {f}: pipelined picture differs from the plain decode"
+    pipe.close()
+    dec.close()
+
+
+def test_pipe_reports_a_refused_frame_and_goes_on():
+    clip = pkg.Clip("c1_mini", 3, frames=4, key_interval=1)  # every frame a key frame: no references
+    want = _plain_hashes(clip, [-1] * 4, [[]] * 4, [[]] * 4)
+    bad = bytearray(clip.wire(1))
+    # the first block of frame 1 moved outside the coded grid: the planner refuses the frame
+    hsize = int(np.frombuffer(bytes(bad), dtype=np.uint32, count=1, offset=12)[0])
+    pos = (16 + hsize + 7) & ~7
+    boff = pos + 13 * 8
+    blk = np.frombuffer(bytes(bad), dtype=np.int32, count=69, offset=boff).copy()
+    blk[0] = 1 << 20  # x
+    bad[boff:boff + blk.nbytes] = blk.tobytes()
+    dec = pkg.HipDecoder()
+    dec.set_output_hash(True)
+    pipe = pkg.Pipe(dec, planner_threads=3, depth=4)
+    for f in range(4):
+        pipe.submit(bytes(bad) if f == 1 else clip.wire(f), keep=False)
+    with pytest.raises(pkg.HipError, match="frame 1 failed"):
+        pipe.wait()
+    for f in range(4):
+        _slot, status, _t0, _t1, h = pipe.result(f)
+        if f == 1:
+            assert status != 0
+        else:
+            assert status == 0 and h == want[f]
+    pipe.close()
+    dec.close()
