@@ -312,6 +312,13 @@ def _bind_server(lib):
     lib.av1h_plan_queue_counts.argtypes = [P, ctypes.POINTER(ctypes.c_int32)]
 
 
+class PipeFrame(ctypes.Structure):
+    _fields_ = [("slot", ctypes.c_int32), ("status", ctypes.c_int32), ("submit_ms", ctypes.c_double),
+                ("plan_start_ms", ctypes.c_double), ("planned_ms", ctypes.c_double),
+                ("decode_start_ms", ctypes.c_double), ("done_ms", ctypes.c_double),
+                ("flatten_ms", ctypes.c_double), ("upload_ms", ctypes.c_double), ("hash", ctypes.c_uint64)]
+
+
 def _bind_pipe(lib):
     P = ctypes.c_void_p
     I32P = ctypes.POINTER(ctypes.c_int32)
@@ -320,8 +327,8 @@ def _bind_pipe(lib):
                                           ctypes.c_int]
     lib.av1h_pipe_submit_wire.restype = ctypes.c_int64
     lib.av1h_pipe_wait.argtypes = [P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
-    lib.av1h_pipe_result.argtypes = [P, ctypes.c_int64, I32P, I32P, ctypes.POINTER(ctypes.c_double),
-                                     ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64)]
+    lib.av1h_pipe_result.argtypes = [P, ctypes.c_int64, ctypes.POINTER(PipeFrame)]
+    lib.av1h_plan_host_times.argtypes = [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     lib.av1h_pipe_error.argtypes = [P]
     lib.av1h_pipe_error.restype = ctypes.c_char_p
     lib.av1h_pipe_destroy.argtypes = [P]
@@ -474,14 +481,11 @@ class Pipe:
                            f"{self._lib.av1h_pipe_error(self._h).decode(errors='replace')}")
 
     def result(self, seq):
-        """(slot, status, submit ms, done ms, picture checksum or 0) of a completed frame."""
-        slot, st = ctypes.c_int32(), ctypes.c_int32()
-        t0, t1 = ctypes.c_double(), ctypes.c_double()
-        hv = ctypes.c_uint64()
-        if self._lib.av1h_pipe_result(self._h, seq, ctypes.byref(slot), ctypes.byref(st), ctypes.byref(t0),
-                                      ctypes.byref(t1), ctypes.byref(hv)) != AV1H_OK:
+        """The completed frame's PipeFrame: slot, status, times, picture checksum or 0."""
+        r = PipeFrame()
+        if self._lib.av1h_pipe_result(self._h, seq, ctypes.byref(r)) != AV1H_OK:
             raise HipError(f"frame {seq} is not complete")
-        return slot.value, st.value, t0.value, t1.value, hv.value
+        return r
 
     def close(self):
         if self._h:

@@ -93,6 +93,7 @@ struct Plan {
   size_t dbytes = 0;
   bool uses_lr = false, upscaled = false, has_grain = false;
   int blocks = 0;
+  double flatten_ms = 0, upload_ms = 0; /* host time of the plan's making */
   std::vector<int> slots; /* pool slots the frame reads */
   int64_t alg_bytes[AV1H_STAGES] = {}; /* compulsory HBM bytes per stage */
   /* the queue list in hand-out order, and where each superblock diagonal key
@@ -113,6 +114,19 @@ struct Plan {
    * reads the cells of (av1h_plan_queue_counts) */
   int32_t q_counts[4] = {0, 0, 0, 0};
 };
+
+/* one entry of the streamed NV12 pack ring: its own device output buffer,
+ * the pinned buffer the copy lands in, and the event that ends the copy */
+struct PackEntry {
+  uint8_t* dev = nullptr;
+  size_t dev_cap = 0;
+  void* pin = nullptr;
+  size_t pin_cap = 0;
+  hipEvent_t done = nullptr;
+  size_t bytes = 0;
+  bool busy = false;
+};
+constexpr int kPackRing = 3;
 
 /* a pinned staging buffer and the stream its uploads run on */
 struct Stager {
@@ -192,6 +206,7 @@ struct Ctx {
   double stage_ms[AV1H_STAGES] = {};
   int64_t stage_n[AV1H_STAGES] = {};
   int64_t stage_bytes[AV1H_STAGES] = {};
+  PackEntry ring[kPackRing];     /* streamed NV12 packs                   */
   /* pre-planning (any thread): stagers and device buffers for reuse */
   std::mutex plan_mu;
   std::vector<Stager*> stagers;
@@ -437,6 +452,7 @@ void sb_features(const Host& H, const av1h_frame_header& hdr, const Layout& L, P
 
 struct Host {
   std::vector<DevBlock> blocks;
+  std::vector<DevBlock> dev_blocks; /* the ones the device reads */
   std::vector<DevTx> tx;
   std::vector<int32_t> inter_list, queue_list, special_list;
   std::vector<DevObmc> obmc;
@@ -1375,19 +1391,37 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
     }
   }
 
-  /* one device buffer: the arrays back to back, 256 byte aligned */
+  /* The device reads the descriptors of queued and special blocks only
+   * (through the queue and the special list): those are uploaded, in coding
+   * order, and every list that names blocks for the device is renumbered. */
+  {
+    std::vector<int32_t> dev_of(H->blocks.size(), -1);
+    for (int32_t b : H->queue_list) dev_of[(size_t)b] = 0;
+    for (int32_t b : H->special_list) dev_of[(size_t)b] = 0;
+    H->dev_blocks.clear();
+    for (size_t b = 0; b < H->blocks.size(); ++b)
+      if (dev_of[b] == 0) {
+        dev_of[b] = (int32_t)H->dev_blocks.size();
+        H->dev_blocks.push_back(H->blocks[b]);
+      }
+    for (std::vector<int32_t>* v : {&H->queue_list, &H->special_list, &plan->queue_host, &plan->queue_static,
+                                    &plan->q_block})
+      for (int32_t& b : *v) b = dev_of[(size_t)b];
+  }
+
+  /* one device buffer: the arrays back to back, 256 byte aligned; the
+   * coefficients last, room for int32 kept, int16 uploaded when they fit */
   H->parts.clear();
   auto add = [&](int k, const void* src, size_t bytes, size_t fill = 0) {
     H->part_of[k] = (int)H->parts.size();
     H->parts.push_back(Host::Part{src, bytes, fill, 0});
   };
-  add(kPartBlocks, H->blocks.data(), H->blocks.size() * sizeof(DevBlock));
+  add(kPartBlocks, H->dev_blocks.data(), H->dev_blocks.size() * sizeof(DevBlock));
   add(kPartTx, H->tx.data(), H->tx.size() * sizeof(DevTx));
   add(kPartInter, H->inter_list.data(), H->inter_list.size() * 4);
   add(kPartQueue, H->queue_list.data(), H->queue_list.size() * 4);
   add(kPartSpecial, H->special_list.data(), H->special_list.size() * 4);
   add(kPartObmc, H->obmc.data(), H->obmc.size() * sizeof(DevObmc));
-  add(kPartCoef, f.coefficients, (size_t)f.num_coefficients * 4, (coef_n - (size_t)f.num_coefficients) * 4);
   add(kPartResidual, f.residual, (size_t)f.num_residual * 4, (res_n - (size_t)f.num_residual) * 4);
   add(kPartPalette, f.palette_map, (size_t)f.num_palette, 8);
   add(kPartDeblock, H->deblock.data(), H->deblock.size() * sizeof(DevDeblock));
@@ -1402,6 +1436,7 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
   add(kPartCdefDir, H->cdef_dir_list.data(), H->cdef_dir_list.size() * 4);
   add(kPartLrTiles, H->lr_tiles.data(), H->lr_tiles.size() * 4);
   add(kPartGrain, &H->grain, sizeof(DevGrain));
+  add(kPartCoef, f.coefficients, (size_t)f.num_coefficients * 4, (coef_n - (size_t)f.num_coefficients) * 4);
   size_t total = 0;
   for (Host::Part& pt : H->parts) {
     pt.off = total;
@@ -1427,13 +1462,35 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
   return AV1H_OK;
 }
 
-/* Writes the arrays of a flattened frame into `dst` (Host::total bytes, the
- * staging buffer of its upload), zeros after the pools that are padded. */
-void stage_parts(const Host& H, uint8_t* dst) {
+/* Writes the arrays of a flattened frame into `dst` (room for Host::total
+ * bytes: the staging buffer of its upload), zeros after the pools that are
+ * padded. The coefficients, last, go as int16 when every value fits (*coef16
+ * set), else as int32. Returns the bytes to upload. */
+size_t stage_parts(const Host& H, uint8_t* dst, int32_t* coef16) {
+  const Host::Part* coef = &H.parts[(size_t)H.part_of[kPartCoef]];
   for (const Host::Part& pt : H.parts) {
+    if (&pt == coef) continue;
     if (pt.bytes) std::memcpy(dst + pt.off, pt.src, pt.bytes);
     if (pt.fill) std::memset(dst + pt.off + pt.bytes, 0, pt.fill);
   }
+  const size_t n = coef->bytes / 4, pad = coef->fill / 4;
+  const int32_t* src = static_cast<const int32_t*>(coef->src);
+  int16_t* d16 = reinterpret_cast<int16_t*>(dst + coef->off);
+  uint32_t wide = 0;
+  for (size_t i = 0; i < n; ++i) {
+    const int32_t v = src[i];
+    wide |= (uint32_t)(v + 32768) > 65535u;
+    d16[i] = (int16_t)v;
+  }
+  if (!wide) {
+    std::memset(d16 + n, 0, pad * 2);
+    *coef16 = 1;
+    return coef->off + (n + pad) * 2;
+  }
+  if (n) std::memcpy(dst + coef->off, src, n * 4);
+  std::memset(dst + coef->off + n * 4, 0, pad * 4);
+  *coef16 = 0;
+  return coef->off + (n + pad) * 4;
 }
 
 /* Points the plan's device descriptor at its arrays in `plan->dbuf`. */
@@ -1447,7 +1504,7 @@ void bind_dev(Plan* plan, const Host& H) {
   P.special_list = reinterpret_cast<const int32_t*>(at(kPartSpecial));
   P.inter_list = reinterpret_cast<const int32_t*>(at(kPartInter));
   P.queue_list = reinterpret_cast<const int32_t*>(at(kPartQueue));
-  P.coef = reinterpret_cast<const int32_t*>(at(kPartCoef));
+  P.coef = at(kPartCoef);
   P.residual = reinterpret_cast<const int32_t*>(at(kPartResidual));
   P.palette = at(kPartPalette);
   P.deblock = reinterpret_cast<const DevDeblock*>(at(kPartDeblock));
@@ -1485,8 +1542,8 @@ int upload(Ctx* c, Plan* plan, const Host& H) {
     c->host_pin_cap = n;
   }
   HIP_OK(c, hipStreamSynchronize(c->stream));
-  stage_parts(H, static_cast<uint8_t*>(c->host_pin));
-  HIP_OK(c, hipMemcpyAsync(plan->dbuf, c->host_pin, n, hipMemcpyHostToDevice, c->stream));
+  const size_t up = stage_parts(H, static_cast<uint8_t*>(c->host_pin), &plan->dev.coef16);
+  HIP_OK(c, hipMemcpyAsync(plan->dbuf, c->host_pin, up, hipMemcpyHostToDevice, c->stream));
   c->dirty = true;
   return AV1H_OK;
 }
@@ -1569,8 +1626,8 @@ int upload_plan(Ctx* c, Plan* plan, const Host& H, std::string* err) {
     if (e == hipSuccess) st->cap = sz;
   }
   if (e == hipSuccess) {
-    stage_parts(H, static_cast<uint8_t*>(st->pin));
-    e = hipMemcpyAsync(plan->dbuf, st->pin, n, hipMemcpyHostToDevice, st->stream);
+    const size_t up = stage_parts(H, static_cast<uint8_t*>(st->pin), &plan->dev.coef16);
+    e = hipMemcpyAsync(plan->dbuf, st->pin, up, hipMemcpyHostToDevice, st->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(st->stream);
   }
   {
@@ -2147,7 +2204,9 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
       j.info->ss_y = L.ss_y;
     }
     c->stats.blocks = j.plan->blocks;
-    c->stats.waves = 0;
+    /* the queue's dependency levels: the counterpart of the reference
+     * scheduler's wave count */
+    c->stats.waves = (int32_t)j.plan->key_off.size() - 1;
     c->stats.kernel_launches = launches;
     c->stats.device_recon_ms = recon;
     c->stats.device_filter_ms = filt;
@@ -2252,6 +2311,12 @@ void av1h_destroy(av1h_ctx* h) {
   if (c->batch_pin) (void)hipHostFree(c->batch_pin);
   if (c->rb_pin) (void)hipHostFree(c->rb_pin);
   free_plan(&c->scratch_plan);
+  for (PackEntry& r : c->ring) {
+    if (r.done) (void)hipEventSynchronize(r.done);
+    if (r.done) (void)hipEventDestroy(r.done);
+    if (r.dev) (void)hipFree(r.dev);
+    if (r.pin) (void)hipHostFree(r.pin);
+  }
   for (auto& b : c->plan_bufs) (void)hipFree(b.second);
   c->plan_bufs.clear();
   for (Stager* st : c->stagers) {
@@ -2344,8 +2409,12 @@ int av1h_plan_frame(av1h_ctx* h, const av1h_frame_desc* f, void** plan_out) {
   Plan* p = new Plan;
   Host host;
   std::string err;
+  const double t0 = now_ms();
   int rc = flatten(*f, p, &host, &err);
+  const double t1 = now_ms();
   if (rc == AV1H_OK) rc = upload_plan(c, p, host, &err);
+  p->flatten_ms = t1 - t0;
+  p->upload_ms = now_ms() - t1;
   if (rc != AV1H_OK) {
     give_buffer(c, p->dbuf, p->dbytes);
     delete p;
@@ -2439,6 +2508,112 @@ int av1h_to_nv12(av1h_ctx* h, int slot, size_t pitch, uint8_t* out, size_t out_b
 }
 
 const uint8_t* av1h_nv12_device_buffer(const av1h_ctx* h) { return h ? h->c.pack : nullptr; }
+
+/* Streamed NV12 pack (reference Decoder::nv12_pack_begin / wait / release,
+ * av1gpu.hpp:748-773, the CUDA backend's 3-entry ring, cuda_backend.cpp:
+ * 1807-1998). The pack kernel and the copy into the entry's pinned buffer go
+ * on the decode stream, after the decode that made the picture and before any
+ * later one that reuses the slot; the decoding thread returns at once and the
+ * thread that shows the frame waits for the copy. */
+int av1h_nv12_pack_begin(av1h_ctx* h, int slot, size_t pitch) {
+  if (!h) return AV1H_ERR_INVALID;
+  Ctx* c = &h->c;
+  std::lock_guard<std::mutex> lk(c->mu);
+  (void)hipSetDevice(c->device);
+  Layout L;
+  const uint16_t* src = gather(c, slot, &L);
+  if (!src) return AV1H_ERR_INVALID;
+  const size_t need = av1h_nv12_size(L.width, L.height, L.ss_x, L.ss_y, pitch);
+  if (need == 0)
+    return set_err(c, AV1H_ERR_INVALID, "NV12 output needs a 4:2:0 frame and an even pitch at least as wide");
+  PackEntry* r = nullptr;
+  for (PackEntry& e : c->ring)
+    if (!e.busy) {
+      r = &e;
+      break;
+    }
+  if (!r) return set_err(c, AV1H_ERR_UNSUPPORTED, "every streamed NV12 pack is in flight: pack synchronously");
+  if (!r->done) HIP_OK(c, hipEventCreateWithFlags(&r->done, hipEventDisableTiming));
+  if (r->pin_cap < need) {
+    if (r->pin) (void)hipHostFree(r->pin);
+    r->pin = nullptr;
+    r->pin_cap = 0;
+    HIP_OK(c, hipHostMalloc(&r->pin, need, hipHostMallocDefault));
+    r->pin_cap = need;
+  }
+  /* the decode packed this picture at this pitch already: copy that */
+  const bool packed = c->nv12_current && pitch == c->nv12_pitch_done && src == c->frame;
+  const uint8_t* from = c->pack;
+  if (!packed) {
+    if (r->dev_cap < need) {
+      if (r->dev) (void)hipFree(r->dev);
+      r->dev = nullptr;
+      r->dev_cap = 0;
+      HIP_OK(c, hipMalloc(&r->dev, need));
+      r->dev_cap = need;
+    }
+    HIP_OK(c, launch_nv12(L, src, r->dev, (int)pitch, c->stream));
+    from = r->dev;
+  }
+  HIP_OK(c, hipMemcpyAsync(r->pin, from, need, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(c, hipEventRecord(r->done, c->stream));
+  c->dirty = true;
+  r->bytes = need;
+  r->busy = true;
+  return (int)(r - c->ring);
+}
+
+int av1h_nv12_pack_wait(av1h_ctx* h, int handle, uint8_t* out, size_t out_bytes, size_t* bytes) {
+  if (!h || handle < 0 || handle >= kPackRing) return AV1H_ERR_INVALID;
+  Ctx* c = &h->c;
+  PackEntry* r = &c->ring[handle];
+  hipEvent_t ev;
+  size_t n;
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!r->busy) return set_err(c, AV1H_ERR_INVALID, "the streamed NV12 pack is not in flight");
+    ev = r->done;
+    n = r->bytes;
+  }
+  (void)hipSetDevice(c->device);
+  if (hipEventSynchronize(ev) != hipSuccess) {
+    std::lock_guard<std::mutex> lk(c->mu);
+    return set_err(c, AV1H_ERR_DEVICE, "the streamed NV12 pack failed");
+  }
+  if (bytes) *bytes = n;
+  if (out) {
+    if (out_bytes < n) {
+      std::lock_guard<std::mutex> lk(c->mu);
+      return set_err(c, AV1H_ERR_INVALID, "output buffer too small");
+    }
+    std::memcpy(out, r->pin, n);
+  }
+  return AV1H_OK;
+}
+
+void av1h_nv12_pack_release(av1h_ctx* h, int handle) {
+  if (!h || handle < 0 || handle >= kPackRing) return;
+  Ctx* c = &h->c;
+  PackEntry* r = &c->ring[handle];
+  hipEvent_t ev = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!r->busy) return;
+    ev = r->done;
+  }
+  (void)hipSetDevice(c->device);
+  (void)hipEventSynchronize(ev);
+  std::lock_guard<std::mutex> lk(c->mu);
+  r->busy = false;
+}
+
+int av1h_plan_host_times(const void* plan, double* flatten_ms, double* upload_ms) {
+  if (!plan) return AV1H_ERR_INVALID;
+  const Plan* p = static_cast<const Plan*>(plan);
+  if (flatten_ms) *flatten_ms = p->flatten_ms;
+  if (upload_ms) *upload_ms = p->upload_ms;
+  return AV1H_OK;
+}
 
 int av1h_plan_levels(const void* plan) {
   if (!plan) return AV1H_ERR_INVALID;
@@ -2728,6 +2903,8 @@ void av1h_reset(av1h_ctx* h) {
   if (c->frame) (void)hipMemsetAsync(c->frame, 0, c->cap * 2, c->stream);
   if (c->pool) (void)hipMemsetAsync(c->pool, 0, c->cap * 2 * c->pool_alloc, c->stream);
   (void)hipStreamSynchronize(c->stream);
+  /* streamed packs are complete after the sync; their handles end here */
+  for (PackEntry& r : c->ring) r.busy = false;
   std::memset(&c->stats, 0, sizeof(c->stats));
 }
 

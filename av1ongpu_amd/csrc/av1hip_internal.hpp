@@ -184,7 +184,11 @@ struct FramePlanDev {
   const int32_t* inter_list;
   const int32_t* queue_list;
   const int32_t* special_list; /* inter blocks with warp, masks or OBMC */
-  const int32_t* coef;
+  /* the dequantised coefficients: int16 when every value of the frame fits
+   * (coef16), else int32; the lists above index `blocks`, which holds the
+   * descriptors of queued and special blocks only */
+  const void* coef;
+  int32_t coef16;
   const int32_t* residual;
   const uint8_t* palette;
   const DevDeblock* deblock;  /* vertical edges first, then horizontal */

@@ -230,10 +230,16 @@ __device__ __forceinline__ void itx_group(RArgs a, const int32_t* list, int n, i
     constexpr int kC = G < 32 ? G : 32, kIt = kC * kC / G;
     const int cs = w < 32 ? w : 32, cr = h < 32 ? h : 32;
     const int lcs = ilog2(cs), st = w + 1, n = cs * cr;
-    const int32_t* src = a.plan.coef + tx.coef;
     int v[kIt];
+    if (a.plan.coef16) {
+      const int16_t* src = static_cast<const int16_t*>(a.plan.coef) + tx.coef;
 #pragma unroll
-    for (int k = 0; k < kIt; ++k) v[k] = l + k * G < n ? src[l + k * G] : 0;
+      for (int k = 0; k < kIt; ++k) v[k] = l + k * G < n ? (int)src[l + k * G] : 0;
+    } else {
+      const int32_t* src = static_cast<const int32_t*>(a.plan.coef) + tx.coef;
+#pragma unroll
+      for (int k = 0; k < kIt; ++k) v[k] = l + k * G < n ? src[l + k * G] : 0;
+    }
 #pragma unroll
     for (int k = 0; k < kIt; ++k) {
       const int i = l + k * G;

@@ -7,6 +7,7 @@
  * pre-planned frame is consumed by its decode.
  */
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -114,6 +115,34 @@ class HipDecoder final : public Decoder {
   }
 
   bool supports_preplanning() const override { return true; }
+
+  /* every slot packs on the device (av1h_to_nv12), the latest with its film
+   * grain, so a pipeline may decode with read_back false and pack the slot */
+  bool packs_nv12_on_device() const override { return true; }
+
+  int nv12_pack_begin(int slot, size_t pitch, std::string* error) override {
+    const int rc = av1h_nv12_pack_begin(ctx_, slot, pitch);
+    if (rc < 0) return fail_rc(error, rc), -1;
+    return rc;
+  }
+
+  bool nv12_pack_wait(int handle, std::vector<uint8_t>* nv12, std::string* error) override {
+    size_t bytes = 0;
+    int rc = av1h_nv12_pack_wait(ctx_, handle, nullptr, 0, &bytes);
+    if (rc != AV1H_OK) return fail_rc(error, rc), false;
+    if (nv12) {
+      nv12->resize(bytes);
+      rc = av1h_nv12_pack_wait(ctx_, handle, nv12->data(), nv12->size(), &bytes);
+      if (rc != AV1H_OK) return fail_rc(error, rc), false;
+    }
+    return true;
+  }
+
+  void nv12_pack_release(int handle) override { av1h_nv12_pack_release(ctx_, handle); }
+
+  /* the output gate of a pipelined caller: stored, as the reference engine
+   * does (engine.cpp:464-466); decoding itself is unaffected */
+  void hold_output(bool hold) override { hold_.store(hold, std::memory_order_relaxed); }
 
   void release_reference(int slot) override {
     std::lock_guard<std::mutex> lk(mu_);
@@ -255,6 +284,7 @@ class HipDecoder final : public Decoder {
   std::mutex mu_;
   std::vector<int> held_;
   bool last_ok_ = false;
+  std::atomic<bool> hold_{false};
 };
 
 }  // namespace

@@ -34,14 +34,7 @@ struct PFrame {
   void* plan = nullptr;
   int plan_rc = AV1H_OK;
   bool planned = false, planning = false;
-  double submit_ms = 0;
-};
-
-struct PResult {
-  int32_t slot = -1;
-  int32_t status = AV1H_OK;
-  double submit_ms = 0, done_ms = 0;
-  uint64_t hash = 0;
+  double plan_start_ms = 0, planned_ms = 0;
 };
 
 }  // namespace
@@ -54,7 +47,7 @@ struct av1h_pipe {
   std::deque<PFrame*> frames; /* submitted, not yet decoded, in order */
   int64_t next_seq = 0;       /* the next submission's number          */
   int64_t completed = 0;      /* frames [0, completed) are done         */
-  std::vector<PResult> results;
+  std::vector<av1h_pipe_frame> results;
   int64_t first_failed = -1;
   std::string err;
   bool stop = false;
@@ -79,11 +72,14 @@ void planner_main(av1h_pipe* p) {
       continue;
     }
     f->planning = true;
+    f->plan_start_ms = av1h_server_now_ms();
     lk.unlock();
     void* plan = nullptr;
     const int rc = av1h_plan_frame(p->ctx, &f->desc, &plan);
     std::string msg = rc != AV1H_OK ? av1h_last_error(p->ctx) : "";
+    const double t = av1h_server_now_ms();
     lk.lock();
+    f->planned_ms = t;
     f->plan = plan;
     f->plan_rc = rc;
     f->planned = true;
@@ -103,6 +99,9 @@ void worker_main(av1h_pipe* p) {
     }
     PFrame* f = p->frames.front();
     lk.unlock();
+    const double t_start = av1h_server_now_ms();
+    double flatten_ms = 0, upload_ms = 0;
+    if (f->plan) (void)av1h_plan_host_times(f->plan, &flatten_ms, &upload_ms);
     int slot = -1;
     int rc = f->plan_rc;
     std::string msg;
@@ -118,10 +117,15 @@ void worker_main(av1h_pipe* p) {
     for (int32_t s : f->release) av1h_release_reference(p->ctx, s);
     const double t = av1h_server_now_ms();
     lk.lock();
-    PResult& r = p->results[(size_t)f->seq];
+    av1h_pipe_frame& r = p->results[(size_t)f->seq];
     r.slot = slot;
     r.status = rc;
+    r.plan_start_ms = f->plan_start_ms;
+    r.planned_ms = f->planned_ms;
+    r.decode_start_ms = t_start;
     r.done_ms = t;
+    r.flatten_ms = flatten_ms;
+    r.upload_ms = upload_ms;
     r.hash = hash;
     if (rc != AV1H_OK) {
       if (p->first_failed < 0) p->first_failed = f->seq;
@@ -163,9 +167,11 @@ int64_t av1h_pipe_submit(av1h_pipe* p, const av1h_frame_desc* desc, int keep, in
   std::unique_lock<std::mutex> lk(p->mu);
   while (p->next_seq - p->completed >= p->depth) p->cv_space.wait(lk);
   f->seq = p->next_seq++;
-  f->submit_ms = av1h_server_now_ms();
-  p->results.emplace_back();
-  p->results.back().submit_ms = f->submit_ms;
+  av1h_pipe_frame r;
+  std::memset(&r, 0, sizeof(r));
+  r.slot = -1;
+  r.submit_ms = av1h_server_now_ms();
+  p->results.push_back(r);
   p->frames.push_back(f);
   p->cv_plan.notify_one();
   return f->seq;
@@ -188,17 +194,11 @@ int av1h_pipe_wait(av1h_pipe* p, int64_t seq, int64_t* failed_seq) {
   return AV1H_OK;
 }
 
-int av1h_pipe_result(av1h_pipe* p, int64_t seq, int32_t* slot, int32_t* status, double* submit_ms,
-                     double* done_ms, uint64_t* hash) {
-  if (!p) return AV1H_ERR_INVALID;
+int av1h_pipe_result(av1h_pipe* p, int64_t seq, av1h_pipe_frame* out) {
+  if (!p || !out) return AV1H_ERR_INVALID;
   std::lock_guard<std::mutex> lk(p->mu);
   if (seq < 0 || seq >= p->completed) return AV1H_ERR_INVALID;
-  const PResult& r = p->results[(size_t)seq];
-  if (slot) *slot = r.slot;
-  if (status) *status = r.status;
-  if (submit_ms) *submit_ms = r.submit_ms;
-  if (done_ms) *done_ms = r.done_ms;
-  if (hash) *hash = r.hash;
+  *out = p->results[(size_t)seq];
   return AV1H_OK;
 }
 

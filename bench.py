@@ -93,14 +93,15 @@ def display_pixels(clip):
 
 
 def default_planners():
-    """Host threads for planning: the cores this process may use, less two
-    (the decode worker and the submitting thread), at most 14."""
+    """Host threads for planning: half the cores this process may use, at
+    most 12 (on 16 cores 8 planners measured faster than 14: the decode
+    worker's launches and waits need a core of their own)."""
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         cores = os.cpu_count() or 1
     cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
-    return max(1, min(14, cores - 2))
+    return max(1, min(12, cores // 2))
 
 
 def pct(values, q):
@@ -511,7 +512,7 @@ def main():
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--planners", type=int, default=0, help="planner threads of the ingest pipeline (0: automatic)")
-    ap.add_argument("--depth", type=int, default=32, help="frames in flight in the ingest pipeline")
+    ap.add_argument("--depth", type=int, default=24, help="frames in flight in the ingest pipeline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline and the parity sample")
     ap.add_argument("--no-mix", action="store_true", help="skip the 16-stream batched vs plain mix")
@@ -575,29 +576,38 @@ def main():
     pipe = pkg.Pipe(dec, planner_threads=planners, depth=args.depth)
     next_seq = [0]
 
-    def pipe_step():
+    def submit_clip():
         base = next_seq[0]
         for f in range(n):
             pipe.submit(wires[f], keep=keep_slot[f] >= 0, reserved=keep_slot[f], release=releases[f])
-        pipe.wait()
         next_seq[0] = base + n
         return base
 
     for i in range(args.warmup):
-        pipe_step()
+        submit_clip()
+        pipe.wait()
         log(f"warmup step {i + 1}/{args.warmup} done")
-    bases = []
+    # the timed steps: every step's frames submitted as the pipeline admits
+    # them, one wait at the end (the clip restarts with a key frame each step)
     t0 = time.perf_counter()
-    for i in range(args.steps):
-        bases.append(pipe_step())
-        log(f"step {i + 1}/{args.steps} done")
+    bases = [submit_clip() for _ in range(args.steps)]
+    pipe.wait()
     elapsed = time.perf_counter() - t0
-    pipe_lat = [r[3] - r[2] for b in bases for r in (pipe.result(b + f) for f in range(n))]
+    log(f"{args.steps} steps done")
+    recs = [pipe.result(b + f) for b in bases for f in range(n)]
+    pipe_lat = [r.done_ms - r.submit_ms for r in recs]
+    idle = [max(0.0, b.decode_start_ms - a.done_ms) for a, b in zip(recs, recs[1:])]
+    phases = {"flatten_ms": round(sum(r.flatten_ms for r in recs) / len(recs), 3),
+              "upload_ms": round(sum(r.upload_ms for r in recs) / len(recs), 3),
+              "decode_ms": round(sum(r.done_ms - r.decode_start_ms for r in recs) / len(recs), 3),
+              "worker_idle_ms": round(sum(idle) / max(len(idle), 1), 3)}
+    log(f"pipeline per frame: {phases}")
     # the same pass with a device checksum of every picture (untimed), for
     # the comparison with the plain path below
     dec.set_output_hash(True)
-    b = pipe_step()
-    pipe_hashes = [pipe.result(b + f)[4] for f in range(n)]
+    b = submit_clip()
+    pipe.wait()
+    pipe_hashes = [pipe.result(b + f).hash for f in range(n)]
     dec.set_output_hash(False)
     pipe.close()
 
@@ -666,6 +676,7 @@ def main():
         "frame_latency_ms": {"p50": round(pct(pipe_lat, 50), 3), "p99": round(pct(pipe_lat, 99), 3),
                              "note": "submit to complete inside the pipeline, frames submitted as fast as "
                                      "the pipeline admits them"},
+        "pipeline_phases": phases,
         "device_only": {"value": round(args.steps * n * px / dev_elapsed / 1e6, 2), "unit": "Mpixel/s",
                         "path": "the same clip from plans resident in HBM, one decode call per frame",
                         "call_latency_ms": {"p50": round(pct(lat, 50) * 1e3, 3),

@@ -416,6 +416,18 @@ int av1h_to_nv12(av1h_ctx* ctx, int slot, size_t pitch, uint8_t* out, size_t out
  * context), for consumers that take the device picture (no counterpart in the
  * reference, whose to_nv12 always copies to the host). */
 const uint8_t* av1h_nv12_device_buffer(const av1h_ctx* ctx);
+/* Streamed NV12 pack (replaces Decoder::nv12_pack_begin / nv12_pack_wait /
+ * nv12_pack_release, reference include/av1gpu/av1gpu.hpp:748-773): begin
+ * queues the pack of `slot` (the slot rule of av1h_to_nv12) and its copy into
+ * page-locked memory on the decode stream and returns a handle (0-2) at once,
+ * or a negative status (AV1H_ERR_UNSUPPORTED: every handle is in flight, pack
+ * synchronously instead). wait blocks until the copy is done, reports the
+ * byte count and copies the bytes when `out` is given (out_bytes at least
+ * that); the handle stays in flight until release, which waits for the copy
+ * first. */
+int av1h_nv12_pack_begin(av1h_ctx* ctx, int slot, size_t pitch);
+int av1h_nv12_pack_wait(av1h_ctx* ctx, int handle, uint8_t* out, size_t out_bytes, size_t* bytes);
+void av1h_nv12_pack_release(av1h_ctx* ctx, int handle);
 /* With `pitch` > 0 (even), every decode on this context also packs the
  * finished picture into NV12 at that pitch on the device, in the same launch
  * sequence (4:2:0 frames only); av1h_to_nv12 of that picture at that pitch
@@ -562,6 +574,8 @@ enum { AV1H_CM_FEATURES = 24, AV1H_CM_PARAMS = 5825 };
  * inter-intra blocks; edge filter on; queued blocks; bit depth above 8.
  * Returns the superblock count; `features` (count x 24) may be null. */
 int av1h_plan_sb_features(const void* plan, float* features);
+/* Host time of the plan's making: flattening and upload (ms). */
+int av1h_plan_host_times(const void* plan, double* flatten_ms, double* upload_ms);
 /* Dependency levels of the plan's queue (its critical path in blocks; 0
  * when nothing is queued): what bounds the frame's queue time. */
 int av1h_plan_levels(const void* plan);
@@ -626,11 +640,18 @@ int64_t av1h_pipe_submit_wire(av1h_pipe* pipe, const uint8_t* wire, size_t len, 
  * Returns AV1H_OK or the status of the first failed frame among them, whose
  * number goes to *failed_seq (-1 when none failed). */
 int av1h_pipe_wait(av1h_pipe* pipe, int64_t seq, int64_t* failed_seq);
-/* A completed frame's slot (-1 when not kept), status, times
- * (av1h_server_now_ms clock) and, when the context checksums every decoded
- * picture (av1h_set_output_hash), its checksum (else 0). */
-int av1h_pipe_result(av1h_pipe* pipe, int64_t seq, int32_t* slot, int32_t* status, double* submit_ms,
-                     double* done_ms, uint64_t* hash);
+/* A completed frame: its slot (-1 when not kept) and status; its times on the
+ * av1h_server_now_ms clock: submitted, taken by a planner, planned (flattened
+ * and uploaded), taken by the worker, complete on the device; how long its
+ * flattening and its upload took; and, when the context checksums every
+ * decoded picture (av1h_set_output_hash), the picture's checksum (else 0). */
+typedef struct av1h_pipe_frame {
+  int32_t slot, status;
+  double submit_ms, plan_start_ms, planned_ms, decode_start_ms, done_ms;
+  double flatten_ms, upload_ms;
+  uint64_t hash;
+} av1h_pipe_frame;
+int av1h_pipe_result(av1h_pipe* pipe, int64_t seq, av1h_pipe_frame* out);
 /* The first failure's message ("" when none). */
 const char* av1h_pipe_error(av1h_pipe* pipe);
 /* Waits for every submitted frame, then ends the threads. */

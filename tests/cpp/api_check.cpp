@@ -6,10 +6,14 @@
  *   api_check host
  *       host-side contracts: backend names, Frame geometry, frame_to_nv12 /
  *       frame_to_yuv16 / frame_to_yuv8 bytes, Decoder::create refusals;
- *   api_check decode <workload> <seed> <nv12 pitch> [planned] [key=value ...]
+ *   api_check decode <workload> <seed> <nv12 pitch> [planned|streamed] [key=value ...]
  *       decodes a synthetic clip on the GPU and prints one line per frame:
  *       "<frame> <slot> <yuv16 fnv> <nv12 fnv> <rgb fnv>"; with `planned`
- *       every other frame goes through plan_frame + decode_reference_planned.
+ *       every other frame goes through plan_frame + decode_reference_planned;
+ *       with `streamed` every frame is planned, decoded into a reserved slot
+ *       with read_back false (the backend packs on the device) and packed
+ *       through nv12_pack_begin / wait / release, the way the reference's
+ *       pipelined caller does (dsfilter/decode_core.hpp), yuv16 printed as "-".
  */
 #include <cstdio>
 #include <cstdlib>
@@ -90,12 +94,12 @@ static int decode(int argc, char** argv) {
   const std::string wl = argv[2];
   const uint64_t seed = std::strtoull(argv[3], nullptr, 10);
   const size_t pitch = std::strtoull(argv[4], nullptr, 10);
-  bool planned = false;
+  bool planned = false, streamed = false;
   synth::Config cfg = synth::workload(wl, seed);
   for (int i = 5; i < argc; ++i) {
     std::string a = argv[i];
-    if (a == "planned") {
-      planned = true;
+    if (a == "planned" || a == "streamed") {
+      (a == "planned" ? planned : streamed) = true;
       continue;
     }
     const size_t eq = a.find('=');
@@ -129,7 +133,40 @@ static int decode(int argc, char** argv) {
     for (int r : clip.refs[f]) s.reference_slots.push_back(slot_of[r]);
     Frame out;
     int slot;
-    if (planned && (f & 1)) {
+    if (streamed) {
+      if (!dec->packs_nv12_on_device()) {
+        std::fprintf(stderr, "the backend does not pack on the device\n");
+        return 4;
+      }
+      void* plan = nullptr;
+      if (!dec->plan_frame(s, &plan, &err)) {
+        std::fprintf(stderr, "plan_frame %zu: %s\n", f, err.c_str());
+        return 4;
+      }
+      const int want = dec->reserve_reference();
+      slot = dec->decode_reference_planned(plan, want, &out, &err, false);
+      if (slot != want || out.width != s.width || out.height != s.height) {
+        std::fprintf(stderr, "streamed decode %zu: slot %d, reserved %d, %dx%d (%s)\n", f, slot, want, out.width,
+                     out.height, err.c_str());
+        return 4;
+      }
+      if (f == 0 && dec->stats().waves <= 0) {
+        std::fprintf(stderr, "a key frame decoded with %d waves\n", dec->stats().waves);
+        return 4;
+      }
+      const int handle = dec->nv12_pack_begin(slot, pitch, &err);
+      std::vector<uint8_t> nv, rgb;
+      if (handle < 0 || !dec->nv12_pack_wait(handle, &nv, &err)) {
+        std::fprintf(stderr, "streamed pack %zu: handle %d (%s)\n", f, handle, err.c_str());
+        return 5;
+      }
+      dec->nv12_pack_release(handle);
+      if (!dec->to_rgb(slot, ColorMatrix::Bt709, ColorRange::Limited, &rgb, &err)) {
+        std::fprintf(stderr, "rgb %zu: %s\n", f, err.c_str());
+        return 5;
+      }
+      std::printf("%zu %d - %016llx %016llx\n", f, slot, (unsigned long long)fnv(nv), (unsigned long long)fnv(rgb));
+    } else if (planned && (f & 1)) {
       void* plan = nullptr;
       if (!dec->plan_frame(s, &plan, &err)) {
         std::fprintf(stderr, "plan_frame %zu: %s\n", f, err.c_str());
@@ -148,14 +185,17 @@ static int decode(int argc, char** argv) {
       std::fprintf(stderr, "decode %zu: %s\n", f, err.c_str());
       return 4;
     }
-    std::vector<uint8_t> y16, nv, rgb;
-    frame_to_yuv16(out, &y16);
-    if (!dec->to_nv12(slot, pitch, &nv, &err) || !dec->to_rgb(slot, ColorMatrix::Bt709, ColorRange::Limited, &rgb, &err)) {
-      std::fprintf(stderr, "pack %zu: %s\n", f, err.c_str());
-      return 5;
+    if (!streamed) {
+      std::vector<uint8_t> y16, nv, rgb;
+      frame_to_yuv16(out, &y16);
+      if (!dec->to_nv12(slot, pitch, &nv, &err) ||
+          !dec->to_rgb(slot, ColorMatrix::Bt709, ColorRange::Limited, &rgb, &err)) {
+        std::fprintf(stderr, "pack %zu: %s\n", f, err.c_str());
+        return 5;
+      }
+      std::printf("%zu %d %016llx %016llx %016llx\n", f, slot, (unsigned long long)fnv(y16),
+                  (unsigned long long)fnv(nv), (unsigned long long)fnv(rgb));
     }
-    std::printf("%zu %d %016llx %016llx %016llx\n", f, slot, (unsigned long long)fnv(y16),
-                (unsigned long long)fnv(nv), (unsigned long long)fnv(rgb));
     const bool keep = clip.last_use[f] > (int)f;
     if (keep)
       slot_of[(int)f] = slot;

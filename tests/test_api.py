@@ -63,24 +63,28 @@ GPU_CASES = [c for c in load_cases() if (c["workload"], c["seed"]) in
              {("c1_mini", 1), ("c2_mini", 1), ("contract_mini", 2), ("c1_mini", 11)}]
 
 
-def _run_decode(case, planned):
+def _run_decode(case, mode):
     args = [API_CHECK, "decode", case["workload"], str(case["seed"]), str(case["nv12_pitch"])]
-    if planned:
-        args.append("planned")
+    if mode != "direct":
+        args.append(mode)
     args += [f"{k}={v}" for k, v in case["overrides"].items()]
     r = subprocess.run(args, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     rows = [line.split() for line in r.stdout.strip().splitlines()]
     assert len(rows) == len(case["frames"])
     for f, (row, exp) in enumerate(zip(rows, case["frames"])):
-        assert (row[2], row[3], row[4]) == (exp["yuv16"], exp["nv12"], exp["rgb"]), f"{case_id(case)} frame {f}"
+        if mode == "streamed":
+            # read_back false: no host picture, the NV12 bytes come from the streamed pack
+            assert (row[2], row[3], row[4]) == ("-", exp["nv12"], exp["rgb"]), f"{case_id(case)} frame {f}"
+        else:
+            assert (row[2], row[3], row[4]) == (exp["yuv16"], exp["nv12"], exp["rgb"]), f"{case_id(case)} frame {f}"
 
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("case", GPU_CASES, ids=[case_id(c) for c in GPU_CASES])
-@pytest.mark.parametrize("planned", [False, True], ids=["direct", "planned"])
-def test_cpp_api_decodes_golden(case, planned):
-    _run_decode(case, planned)
+@pytest.mark.parametrize("mode", ["direct", "planned", "streamed"])
+def test_cpp_api_decodes_golden(case, mode):
+    _run_decode(case, mode)
 
 
 @pytest.mark.gpu

@@ -44,11 +44,12 @@ def test_pipe_matches_plain(workload, seed, over, threads, depth):
     pipe.wait()
     assert seqs == list(range(clip.frames))
     for f in range(clip.frames):
-        slot, status, t_sub, t_done, h = pipe.result(f)
-        assert status == 0
-        assert slot == keep[f]
-        assert t_done >= t_sub
-        assert h == want[f], f"{workload} seed {seed} frame {f}: pipelined picture differs from the plain decode"
+        r = pipe.result(f)
+        assert r.status == 0
+        assert r.slot == keep[f]
+        assert r.submit_ms <= r.plan_start_ms <= r.planned_ms <= r.decode_start_ms <= r.done_ms
+        assert r.flatten_ms > 0 and r.upload_ms > 0
+        assert r.hash == want[f], f"{workload} seed {seed} frame {f}: pipelined picture differs from the plain decode"
     pipe.close()
     dec.close()
 
@@ -72,10 +73,10 @@ def test_pipe_reports_a_refused_frame_and_goes_on():
     with pytest.raises(pkg.HipError, match="frame 1 failed"):
         pipe.wait()
     for f in range(4):
-        _slot, status, _t0, _t1, h = pipe.result(f)
+        r = pipe.result(f)
         if f == 1:
-            assert status != 0
+            assert r.status != 0
         else:
-            assert status == 0 and h == want[f]
+            assert r.status == 0 and r.hash == want[f]
     pipe.close()
     dec.close()
