@@ -461,7 +461,6 @@ struct Host {
   std::vector<McTaskDev> mc_task;
   std::vector<int32_t> db_chunk, cdef_dir_list;
   std::vector<int32_t> lr_tiles; /* pairs */
-  std::vector<int32_t> itx[kTxClasses];
   std::vector<DevDeblock> deblock;
   std::vector<DevCdef> cdef;
   std::vector<DevLr> lr;
@@ -475,6 +474,12 @@ struct Host {
   std::vector<Part> parts;
   int part_of[kParts];
   size_t total = 0;
+  /* scratch of one flattening: queued block per cell, queued cells, device
+   * index of each block, luma 8x8 blocks whose CDEF direction is needed */
+  std::vector<int32_t> s_owner[3];
+  std::vector<char> s_queued[3];
+  std::vector<int32_t> s_dev_of;
+  std::vector<uint8_t> s_need;
 };
 
 /* BlockDecoded availability for blocks whose syntax asks to derive it
@@ -698,6 +703,7 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
   H->special_list.clear();
   H->obmc.clear();
   H->blocks.reserve(nb);
+  H->tx.reserve((size_t)f.num_transforms);
   plan->slots.clear();
   bool derive = false;
   for (int i = 0; i < nb; ++i)
@@ -957,7 +963,7 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
    * blocks a level holds can all run at once. */
   {
     const int nq = (int)H->queue_list.size();
-    std::vector<int32_t> owner[3];
+    std::vector<int32_t>* owner = H->s_owner;
     for (int p = 0; p < 3; ++p) owner[p].assign((size_t)L.cells_w[p] * L.cells_h[p], -1);
     plan->q_dep_off.assign(1, 0);
     plan->q_dep.clear();
@@ -972,7 +978,7 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
       if ((d.x | d.y | d.w | d.h) & ((1 << L.cell_shift[d.plane]) - 1)) packed = false;
     }
     std::vector<char> need_mark((size_t)nq, 0);
-    std::vector<char> queued[3];
+    std::vector<char>* queued = H->s_queued;
     for (int p = 0; p < 3; ++p) queued[p].assign((size_t)L.cells_w[p] * L.cells_h[p], 0);
     for (int qi = 0; qi < nq; ++qi) {
       const DevBlock& d = H->blocks[(size_t)H->queue_list[(size_t)qi]];
@@ -1061,6 +1067,7 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
    * over kMcSamples samples) in coding order, with their positions, windows
    * and LDS offsets, packed into tasks within the kernel's budgets */
   H->mc_items.clear();
+  H->mc_items.reserve(H->inter_list.size() + H->inter_list.size() / 4);
   H->mc_task.clear();
   {
     McTaskDev cur{};
@@ -1127,29 +1134,33 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
   }
   const int32_t mc_tasks = (int32_t)H->mc_task.size();
   /* transforms: inter set then intra set, each by class (larger side), each
-   * class sorted by size so a wave's transforms mostly share one */
-  std::vector<int32_t> itx_sets[2][kTxClasses];
-  for (int t = 0; t < (int)H->tx.size(); ++t) {
-    const int m = std::max(H->tx[t].w, H->tx[t].h);
-    const int set = (H->tx[t].flags & kTxToResidual) ? 1 : 0;
-    itx_sets[set][m <= 4 ? 0 : m <= 8 ? 1 : m <= 16 ? 2 : m <= 32 ? 3 : 4].push_back(t);
-  }
-  H->itx_all.clear();
+   * class by size (width, then height) so a wave's transforms mostly share
+   * one; coding order between equals. A counting sort over the 2 x 25 (set,
+   * width, height) keys, which are in that order already. */
+  H->itx_all.assign(H->tx.size(), 0);
   int32_t itx_off[2][kTxClasses + 1];
-  for (int set = 0; set < 2; ++set)
-    for (int k = 0; k < kTxClasses; ++k) {
-      auto& v = itx_sets[set][k];
-      std::stable_sort(v.begin(), v.end(), [&](int32_t a, int32_t b) {
-        const DevTx &ta = H->tx[a], &tb = H->tx[b];
-        return ta.w != tb.w ? ta.w < tb.w : ta.h < tb.h;
-      });
-      itx_off[set][k] = (int32_t)H->itx_all.size();
-      H->itx_all.insert(H->itx_all.end(), v.begin(), v.end());
-      itx_off[set][k + 1] = (int32_t)H->itx_all.size();
-    }
+  {
+    constexpr int kKeys = 2 * 25;
+    /* class c holds the 2c + 1 sizes whose larger side is 4 << c, from key
+     * c * c on: widths below 4 << c (height 4 << c) first, then width 4 << c
+     * by height */
+    auto key = [](const DevTx& t) {
+      const int lw = __builtin_ctz((unsigned)t.w) - 2, lh = __builtin_ctz((unsigned)t.h) - 2;
+      const int cls = std::max(lw, lh);
+      const int set = (t.flags & kTxToResidual) ? 1 : 0;
+      return set * 25 + cls * cls + (lw < cls ? lw : cls + lh);
+    };
+    int32_t cnt[kKeys + 1] = {};
+    for (const DevTx& t : H->tx) ++cnt[key(t) + 1];
+    for (int k = 0; k < kKeys; ++k) cnt[k + 1] += cnt[k];
+    for (int set = 0; set < 2; ++set)
+      for (int c = 0; c <= kTxClasses; ++c) itx_off[set][c] = cnt[set * 25 + c * c];
+    for (int t = 0; t < (int)H->tx.size(); ++t) H->itx_all[(size_t)cnt[key(H->tx[(size_t)t])]++] = t;
+  }
 
   /* deblocking: vertical edges then horizontal, with a line prefix each */
   H->deblock.clear();
+  H->deblock.reserve((size_t)f.num_deblock);
   int lines[2] = {0, 0}, nedge[2] = {0, 0};
   for (int pass = 0; pass < 2; ++pass)
     for (int64_t i = 0; i < f.num_deblock; ++i) {
@@ -1191,28 +1202,34 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
       e0 += n;
     }
   }
-  /* CDEF */
-  H->cdef.clear();
-  for (int64_t i = 0; i < f.num_cdef; ++i) {
-    const av1h_cdef_block& cb = f.cdef[i];
-    if (cb.plane < 0 || cb.plane > 2 || cb.x < 0 || cb.y < 0 || cb.x >= L.grid_w[cb.plane] ||
-        cb.y >= L.grid_h[cb.plane] || cb.damping < 0 || cb.damping > 15 || cb.dir > 7)
-      return set_err(err, AV1H_ERR_INVALID, "cdef block %lld", (long long)i);
-    const int bw = cb.plane ? 8 >> hdr.ss_x : 8, bh = cb.plane ? 8 >> hdr.ss_y : 8;
-    if (cb.x % bw || cb.y % bh)
-      return set_err(err, AV1H_ERR_UNSUPPORTED, "cdef block %lld is not on the %dx%d grid", (long long)i, bw, bh);
-    DevCdef d{cb.plane, cb.x, cb.y, cb.pri_strength, cb.sec_strength, cb.damping, cb.dir, 0};
-    H->cdef.push_back(d);
-  }
-  /* luma blocks first (one launch per block size), and the luma 8x8 blocks
-   * whose direction is searched */
-  std::stable_partition(H->cdef.begin(), H->cdef.end(), [](const DevCdef& d) { return d.plane == 0; });
+  /* CDEF: luma blocks first (one launch per block size), coding order
+   * inside each; and the luma 8x8 blocks whose direction is searched */
   int32_t n_cdef_luma = 0;
-  for (const DevCdef& d : H->cdef) n_cdef_luma += d.plane == 0;
+  {
+    const int cbw = 8 >> hdr.ss_x, cbh = 8 >> hdr.ss_y;
+    for (int64_t i = 0; i < f.num_cdef; ++i) {
+      const av1h_cdef_block& cb = f.cdef[i];
+      if (cb.plane < 0 || cb.plane > 2 || cb.x < 0 || cb.y < 0 || cb.x >= L.grid_w[cb.plane] ||
+          cb.y >= L.grid_h[cb.plane] || cb.damping < 0 || cb.damping > 15 || cb.dir > 7)
+        return set_err(err, AV1H_ERR_INVALID, "cdef block %lld", (long long)i);
+      const int bw = cb.plane ? cbw : 8, bh = cb.plane ? cbh : 8;
+      if (cb.x % bw || cb.y % bh)
+        return set_err(err, AV1H_ERR_UNSUPPORTED, "cdef block %lld is not on the %dx%d grid", (long long)i, bw, bh);
+      n_cdef_luma += cb.plane == 0;
+    }
+    H->cdef.resize((size_t)f.num_cdef);
+    size_t nl = 0, nc = (size_t)n_cdef_luma;
+    for (int64_t i = 0; i < f.num_cdef; ++i) {
+      const av1h_cdef_block& cb = f.cdef[i];
+      H->cdef[cb.plane == 0 ? nl++ : nc++] =
+          DevCdef{cb.plane, cb.x, cb.y, cb.pri_strength, cb.sec_strength, cb.damping, cb.dir, 0};
+    }
+  }
   H->cdef_dir_list.clear();
   {
     const int bw8 = L.grid_w[0] >> 3, bh8 = L.grid_h[0] >> 3;
-    std::vector<uint8_t> need((size_t)bw8 * bh8, 0);
+    std::vector<uint8_t>& need = H->s_need;
+    need.assign((size_t)bw8 * bh8, 0);
     for (const DevCdef& d : H->cdef) {
       if (d.dir >= 0) continue;
       const int lx = d.plane ? d.x << hdr.ss_x : d.x, ly = d.plane ? d.y << hdr.ss_y : d.y;
@@ -1367,7 +1384,10 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
     for (int32_t i : H->queue_list)
       plan->alg_bytes[AV1H_STAGE_QUEUE] += 4 * (int64_t)H->blocks[i].w * H->blocks[i].h;
     bool has_edges[3] = {false, false, false};
-    for (const DevDeblock& d : H->deblock) has_edges[d.plane] = true;
+    for (const DevDeblock& d : H->deblock) {
+      has_edges[d.plane] = true;
+      if (has_edges[0] && has_edges[1] && has_edges[2]) break;
+    }
     int64_t disp = 0;
     for (int p = 0; p < 3; ++p) {
       if (has_edges[p]) plan->alg_bytes[AV1H_STAGE_DEBLOCK] += 4 * (int64_t)L.plane_w[p] * L.plane_h[p];
@@ -1376,10 +1396,9 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
             (plan->uses_lr ? 2 : 1) * 2 * (int64_t)(L.plane_w[p] + L.out_w[p]) * L.out_h[p];
       disp += (int64_t)L.out_w[p] * L.out_h[p];
     }
-    for (const DevCdef& d : H->cdef) {
-      const int sx = d.plane ? hdr.ss_x : 0, sy = d.plane ? hdr.ss_y : 0;
-      plan->alg_bytes[AV1H_STAGE_CDEF] += 4 * (int64_t)(8 >> sx) * (8 >> sy);
-    }
+    plan->alg_bytes[AV1H_STAGE_CDEF] = 4 * 64 * (int64_t)n_cdef_luma +
+                                       4 * (int64_t)(8 >> hdr.ss_x) * (8 >> hdr.ss_y) *
+                                           (int64_t)(H->cdef.size() - (size_t)n_cdef_luma);
     for (const DevLr& d : H->lr) plan->alg_bytes[AV1H_STAGE_LR] += 4 * (int64_t)d.w * d.h + 8 * (int64_t)d.w;
     plan->alg_bytes[AV1H_STAGE_STORE] = 4 * disp;
     if (plan->has_grain) {
@@ -1395,7 +1414,8 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
    * (through the queue and the special list): those are uploaded, in coding
    * order, and every list that names blocks for the device is renumbered. */
   {
-    std::vector<int32_t> dev_of(H->blocks.size(), -1);
+    std::vector<int32_t>& dev_of = H->s_dev_of;
+    dev_of.assign(H->blocks.size(), -1);
     for (int32_t b : H->queue_list) dev_of[(size_t)b] = 0;
     for (int32_t b : H->special_list) dev_of[(size_t)b] = 0;
     H->dev_blocks.clear();
@@ -2407,7 +2427,9 @@ int av1h_plan_frame(av1h_ctx* h, const av1h_frame_desc* f, void** plan_out) {
   Ctx* c = &h->c;
   (void)hipSetDevice(c->device);
   Plan* p = new Plan;
-  Host host;
+  /* one per planning thread: its arrays keep their capacity from frame to
+   * frame */
+  static thread_local Host host;
   std::string err;
   const double t0 = now_ms();
   int rc = flatten(*f, p, &host, &err);

@@ -93,15 +93,16 @@ def display_pixels(clip):
 
 
 def default_planners():
-    """Host threads for planning: half the cores this process may use, at
-    most 12 (on 16 cores 8 planners measured faster than 14: the decode
+    """Host threads for planning: 3/8 of the cores this process may use, at
+    most 12 (on 16 cores: 6 planners measured 2995 Mpixel/s, 8 2875, 10
+    2719, 14 2050; the planners contend for memory bandwidth and the decode
     worker's launches and waits need a core of their own)."""
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         cores = os.cpu_count() or 1
     cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
-    return max(1, min(12, cores // 2))
+    return max(1, min(12, cores * 3 // 8))
 
 
 def pct(values, q):
