@@ -348,6 +348,7 @@ def _bind_costmodel(lib):
     lib.av1h_cm_set_norm.argtypes = [P, F, F, ctypes.c_float, ctypes.c_float]
     lib.av1h_cm_get_norm.argtypes = [P, F, F, F, F]
     lib.av1h_cm_predict.argtypes = [P, F, ctypes.c_int64, F]
+    lib.av1h_cm_predict_noalloc.argtypes = [P, F, ctypes.c_int64, F]
     lib.av1h_cm_gradients.argtypes = [P, F, F, ctypes.c_int64, F, F]
     lib.av1h_cm_train.argtypes = [P, F, F, ctypes.c_int64, ctypes.c_int, ctypes.c_float, ctypes.c_uint64,
                                   ctypes.c_int, F]

@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <vector>
 
@@ -165,6 +166,36 @@ __global__ __launch_bounds__(kThreads) void k_mlp_infer(const float* params, con
 
 }  // namespace
 
+/* The allocation-free prediction's staging (av1h_cm_predict_noalloc): the
+ * features of up to AV1H_CM_NOALLOC_MAX superblocks, then their predictions.
+ * Each device holds its own copy from library load to process end; host code
+ * reaches it only through the address hipGetSymbolAddress gives per device. */
+static __device__ __align__(256) float scratch[AV1H_CM_NOALLOC_MAX * (AV1H_CM_FEATURES + 1)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch[kMaxDevices] = {};
+
+/* the scratch array's address on `device`, looked up once per device with
+ * that device current (at model creation) */
+int scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return AV1H_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[device]) return AV1H_OK;
+  void* ptr = nullptr;
+  if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return AV1H_ERR_DEVICE;
+  g_scratch[device] = static_cast<float*>(ptr);
+  return AV1H_OK;
+}
+
+float* scratch_of(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch[device];
+}
+}  // namespace
+
 struct av1h_costmodel {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -200,7 +231,8 @@ int av1h_cm_create(int device, uint64_t seed, av1h_costmodel** out) {
   if (rc != AV1H_OK) return rc;
   av1h_costmodel* m = new av1h_costmodel;
   m->device = device;
-  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess ||
+  if (hipSetDevice(device) != hipSuccess || scratch_setup(device) != AV1H_OK ||
+      hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc(&m->params, kParams * sizeof(float)) != hipSuccess ||
       hipMalloc(&m->adam, 2 * kParams * sizeof(float)) != hipSuccess ||
       hipMalloc(&m->norm, (2 * kIn + 2) * sizeof(float)) != hipSuccess) {
@@ -294,6 +326,26 @@ int av1h_cm_predict(av1h_costmodel* m, const float* features, int64_t n, float* 
   if (dx) (void)hipFree(dx);
   if (dy) (void)hipFree(dy);
   return rc;
+}
+
+/* av1h_cm_predict without an allocation: the features and predictions are
+ * staged in the device's scratch array, set up when the model was created.
+ * Calls on one device share the array, so calls that overlap there race. */
+int av1h_cm_predict_noalloc(av1h_costmodel* m, const float* features, int64_t n, float* out_us) {
+  if (!m || n < 0 || (n && (!features || !out_us))) return AV1H_ERR_INVALID;
+  if (n > AV1H_CM_NOALLOC_MAX) return AV1H_ERR_UNSUPPORTED;
+  if (n == 0) return AV1H_OK;
+  (void)hipSetDevice(m->device);
+  float* dx = scratch_of(m->device);
+  if (!dx) return AV1H_ERR_DEVICE;
+  float* dy = dx + (size_t)AV1H_CM_NOALLOC_MAX * kIn;
+  CM_OK(hipMemcpyAsync(dx, features, (size_t)n * kIn * sizeof(float), hipMemcpyHostToDevice, m->stream));
+  hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)((n + kThreads - 1) / kThreads)), dim3(kThreads), 0, m->stream,
+                     m->params, m->norm, dx, (int)n, dy);
+  CM_OK(hipGetLastError());
+  CM_OK(hipMemcpyAsync(out_us, dy, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, m->stream));
+  CM_OK(hipStreamSynchronize(m->stream));
+  return AV1H_OK;
 }
 
 /* Standardised copies of the samples, with the model's normalisation. */

@@ -603,6 +603,16 @@ int av1h_cm_get_params(av1h_costmodel* model, float* params);
 int av1h_cm_set_norm(av1h_costmodel* model, const float* mean, const float* dev, float y_mean, float y_dev);
 int av1h_cm_get_norm(av1h_costmodel* model, float* mean, float* dev, float* y_mean, float* y_dev);
 int av1h_cm_predict(av1h_costmodel* model, const float* features, int64_t n, float* out_us);
+/* av1h_cm_predict that allocates nothing (no counterpart in the reference:
+ * for callers that cannot allocate device memory, such as code under HIP
+ * graph capture or a serving loop that forbids it). The features and
+ * predictions are staged in a static device array of AV1H_CM_NOALLOC_MAX
+ * superblocks (25 floats each, 400 KiB), whose address each device looks up
+ * when a model is created on it; more superblocks than that are refused with
+ * AV1H_ERR_UNSUPPORTED. Calls that overlap on one device share that array
+ * and race. */
+#define AV1H_CM_NOALLOC_MAX 4096
+int av1h_cm_predict_noalloc(av1h_costmodel* model, const float* features, int64_t n, float* out_us);
 /* Gradient of the mean squared error (standardised time) over the n samples,
  * and that loss: what training steps on. */
 int av1h_cm_gradients(av1h_costmodel* model, const float* features, const float* y_us, int64_t n, float* grad,
