@@ -244,6 +244,12 @@ def hip_lib():
         _bind_server(lib)
         _bind_costmodel(lib)
         _bind_pipe(lib)
+        P = ctypes.c_void_p
+        lib.av1h_decode_planned_batch_async.argtypes = [P, ctypes.POINTER(P), ctypes.POINTER(P),
+                                                        ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                                        ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        lib.av1h_batch_finish.argtypes = [P]
+        lib.av1h_batch_pending.argtypes = [P]
         _hip = lib
     return _hip
 
@@ -716,6 +722,36 @@ def decode_planned_batch(decoders, plans, keep=None, reserved=None):
     if rc != AV1H_OK:
         raise HipError(f"decode_planned_batch failed ({rc}): {decoders[0].error()}")
     return list(slots)
+
+
+def decode_planned_batch_async(decoders, plans, keep=None, reserved=None, launcher=None):
+    """Launch one planned frame per decoder as one batch without waiting
+    (av1h_decode_planned_batch_async); returns the slots. The launches go to
+    `launcher`'s stream (the first decoder's by default); batch_finish takes
+    the results."""
+    n = len(decoders)
+    if n != len(plans):
+        raise ValueError("one plan per decoder")
+    lib = hip_lib()
+    P = ctypes.c_void_p
+    lead = (launcher or decoders[0]).handle
+    hs = (P * n)(*[d.handle for d in decoders])
+    ps = (P * n)(*plans)
+    ks = (ctypes.c_int * n)(*[1 if k else 0 for k in (keep if keep is not None else [True] * n)])
+    rs = (ctypes.c_int * n)(*(reserved if reserved is not None else [-1] * n))
+    slots = (ctypes.c_int * n)()
+    rc = lib.av1h_decode_planned_batch_async(lead, hs, ps, ks, rs, n, slots)
+    if rc != AV1H_OK:
+        raise HipError(f"decode_planned_batch_async failed ({rc}): {(launcher or decoders[0]).error()}")
+    return list(slots)
+
+
+def batch_finish(launcher):
+    """Wait for the oldest launched batch led by `launcher`; raises on its failure."""
+    lib = hip_lib()
+    rc = lib.av1h_batch_finish(launcher.handle)
+    if rc != AV1H_OK:
+        raise HipError(f"batch failed ({rc}): {launcher.error()}")
 
 
 def decode_clips_batched(decoders, clips, frames=None, read_back=True):

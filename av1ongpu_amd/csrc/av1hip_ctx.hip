@@ -18,6 +18,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <deque>
 #include <vector>
 
 #define AV1T_TABLE_QUAL static const
@@ -115,6 +116,26 @@ struct Plan {
   int32_t q_counts[4] = {0, 0, 0, 0};
 };
 
+struct Ctx;
+struct Plan;
+/* a launched batch, until its completion is taken */
+struct JobSpec {
+  Ctx* c;
+  Plan* plan;
+  int keep, reserved;
+  const av1h_planes* out;
+  av1h_frame_info* info;
+  int* slot_out;
+  int slot;
+};
+struct BatchTicket {
+  std::vector<JobSpec> js;
+  std::vector<char> nv12; /* per frame: NV12 packed */
+  int launches = 0;
+  bool any_hash = false, live = false;
+  double t0 = 0, t_sub = 0;
+};
+
 /* one entry of the streamed NV12 pack ring: its own device output buffer,
  * the pinned buffer the copy lands in, and the event that ends the copy */
 struct PackEntry {
@@ -170,6 +191,7 @@ struct Ctx {
   int pool_slots = kSlots;       /* reference slots the pool holds   */
   int pool_alloc = 0;            /* slots of the allocated pool      */
   bool used[kSlots] = {};
+  bool has_pic[kSlots] = {};     /* the slot holds a decoded picture */
   Layout slot_lay[kSlots];
   int last_slot = -1;
   bool have_frame = false;
@@ -191,10 +213,21 @@ struct Ctx {
   size_t decode_nv12_pitch = 0; /* > 0: every decode also packs NV12 into `pack` */
   bool nv12_current = false;     /* `pack` holds the NV12 of the latest frame       */
   size_t nv12_pitch_done = 0;
-  void* batch_dev = nullptr;
-  void* batch_pin = nullptr;
-  void* rb_pin = nullptr; /* pinned: the error word and checksums read back after a batch */
-  size_t batch_cap = 0;
+  /* two sets, so a batch can be prepared and launched while the one before
+   * it runs: the table (device and pinned), the pinned error word and
+   * checksums read back after it, its events (start, queue end, end, done) */
+  void* batch_dev[2] = {};
+  void* batch_pin[2] = {};
+  size_t batch_cap[2] = {};
+  void* rb_pin[2] = {};
+  hipEvent_t bev[2][4] = {};
+  BatchTicket ticket[2];
+  std::deque<int> pending; /* launched batches this context leads, oldest first */
+  int next_buf = 0;
+  /* launched batches holding a frame of this context, and their lead: the
+   * context's pictures are final only once they are finished */
+  int inflight = 0;
+  Ctx* inflight_lead = nullptr;
   bool dirty = false;            /* work queued on `stream` nobody waited for */
   /* captured launch sequences of batches this context leads, by frame count
    * (valid while batch_dev stays where it is) */
@@ -1685,6 +1718,7 @@ int prepare(Ctx* c, const Layout& L) {
     c->cap = (size_t)L.size;
     c->pool_alloc = 0;
     std::memset(c->used, 0, sizeof(c->used));
+    std::memset(c->has_pic, 0, sizeof(c->has_pic));
     c->last_slot = -1;
   }
   if (c->pool_alloc != c->pool_slots) {
@@ -1697,6 +1731,7 @@ int prepare(Ctx* c, const Layout& L) {
     c->pool_alloc = c->pool_slots;
     c->dirty = true;
     std::memset(c->used, 0, sizeof(c->used));
+    std::memset(c->has_pic, 0, sizeof(c->has_pic));
     c->last_slot = -1;
   }
   const size_t ndir = (size_t)(L.grid_w[0] >> 3) * (L.grid_h[0] >> 3) * 2;
@@ -1754,15 +1789,23 @@ int ensure_pack(Ctx* c, size_t bytes) {
  * Every decode runs as a batch: the frames of one or more contexts (one per
  * stream), each kernel launched once for all of them. */
 
-struct JobSpec {
-  Ctx* c;
-  Plan* plan;
-  int keep, reserved;
-  const av1h_planes* out;
-  av1h_frame_info* info;
-  int* slot_out;
-  int slot;
-};
+int batch_finish(Ctx* lead);
+
+/* whether a decode of `plan` on `c` must replace a buffer of the context
+ * (what prepare and the grain and NV12 buffers grow) */
+bool must_grow(const Ctx* c, const Plan* plan) {
+  const Layout& L = plan->lay;
+  const size_t ndir = (size_t)(L.grid_w[0] >> 3) * (L.grid_h[0] >> 3) * 2;
+  if (c->cap < (size_t)L.size || c->pool_alloc != c->pool_slots || c->cdef_dir_cap < ndir ||
+      c->cells_cap < L.cells || c->edges_cap < L.edge_words)
+    return true;
+  if (plan->has_grain && c->grain_offs_cap < (size_t)plan->grain.stripes * plan->grain.blocks + 64) return true;
+  if (c->decode_nv12_pitch > 0) {
+    const size_t need = av1h_nv12_size(L.width, L.height, L.ss_x, L.ss_y, c->decode_nv12_pitch);
+    if (need > 0 && c->pack_cap < need) return true;
+  }
+  return false;
+}
 
 int copy_groups(const Layout& L) { return (int)((L.size * 2 + kCopyChunk - 1) / kCopyChunk); }
 
@@ -1836,7 +1879,11 @@ void merge_queues(const std::vector<JobSpec>& js, uint32_t* out) {
     }
 }
 
-int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
+/* Prepares a batch and queues its launches, its read backs and its
+ * completion event on the lead's stream; the slots it keeps are taken at
+ * once, so the next batch may already name them. At most two batches are in
+ * flight per lead (double-buffered tables): a third waits for the oldest. */
+int batch_launch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
   const int n = (int)js.size();
   for (JobSpec& j : js) {
     j.slot = -1;
@@ -1849,6 +1896,22 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
     for (int k = 0; k < i; ++k)
       if (js[k].c == js[i].c) return set_err(lead, AV1H_ERR_INVALID, "a context appears twice in one batch");
   }
+  for (JobSpec& j : js)
+    if (j.c->inflight && j.c->inflight_lead != lead)
+      return set_err(lead, AV1H_ERR_INVALID, "a frame of context %d's stream is in flight in another launcher's batch",
+                     (int)(&j - js.data()));
+  /* buffers that must grow are replaced only with nothing in flight */
+  bool grow = false;
+  for (JobSpec& j : js) grow = grow || must_grow(j.c, j.plan);
+  while (grow && !lead->pending.empty()) {
+    const int rc = batch_finish(lead);
+    if (rc != AV1H_OK) return rc;
+  }
+  if (lead->pending.size() >= 2) {
+    const int rc = batch_finish(lead);
+    if (rc != AV1H_OK) return rc;
+  }
+  const int b = lead->next_buf;
   /* per frame buffers, slots, epochs */
   int64_t nq = 0;
   for (JobSpec& j : js) {
@@ -1868,6 +1931,7 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
         j.slot = alloc_slot(c);
         if (j.slot < 0) return set_err(lead, AV1H_ERR_SLOTS, "reference pool exhausted");
       }
+      if (j.slot_out) *j.slot_out = j.slot;
     }
     if (plan->has_grain) {
       const size_t need = (size_t)plan->grain.stripes * plan->grain.blocks + 64;
@@ -1962,19 +2026,24 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
   const size_t o_prefix = align256(sizeof(FrameJob) * n);
   const size_t o_queue = o_prefix + align256(prefix.size() * 4);
   const size_t bytes_tab = o_queue + (n > 1 ? (size_t)nq * 4 : 0);
-  if (lead->batch_cap < bytes_tab) {
+  if (lead->batch_cap[b] < bytes_tab) {
+    /* the other set may be in flight: wait for it too */
     HIP_OK(lead, hipStreamSynchronize(s));
     drop_graphs(lead);
-    if (lead->batch_dev) (void)hipFree(lead->batch_dev);
-    if (lead->batch_pin) (void)hipHostFree(lead->batch_pin);
-    lead->batch_dev = lead->batch_pin = nullptr;
-    lead->batch_cap = 0;
+    if (lead->batch_dev[b]) (void)hipFree(lead->batch_dev[b]);
+    if (lead->batch_pin[b]) (void)hipHostFree(lead->batch_pin[b]);
+    lead->batch_dev[b] = lead->batch_pin[b] = nullptr;
+    lead->batch_cap[b] = 0;
     const size_t cap = std::max(bytes_tab + bytes_tab / 4, (size_t)65536);
-    HIP_OK(lead, hipMalloc(&lead->batch_dev, cap));
-    HIP_OK(lead, hipHostMalloc(&lead->batch_pin, cap, hipHostMallocDefault));
-    lead->batch_cap = cap;
+    HIP_OK(lead, hipMalloc(&lead->batch_dev[b], cap));
+    HIP_OK(lead, hipHostMalloc(&lead->batch_pin[b], cap, hipHostMallocDefault));
+    lead->batch_cap[b] = cap;
   }
-  uint8_t* pin = static_cast<uint8_t*>(lead->batch_pin);
+  for (int k = 0; k < 4; ++k)
+    if (!lead->bev[b][k])
+      HIP_OK(lead, hipEventCreateWithFlags(&lead->bev[b][k], k == 3 ? hipEventDisableTiming : hipEventDefault));
+  if (!lead->rb_pin[b]) HIP_OK(lead, hipHostMalloc(&lead->rb_pin[b], 64 + kMaxBatch * 8, hipHostMallocDefault));
+  uint8_t* pin = static_cast<uint8_t*>(lead->batch_pin[b]);
   std::memcpy(pin, jobs.data(), sizeof(FrameJob) * n);
   std::memcpy(pin + o_prefix, prefix.data(), prefix.size() * 4);
   if (n > 1) merge_queues(js, reinterpret_cast<uint32_t*>(pin + o_queue));
@@ -1986,7 +2055,7 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
       HIP_OK(lead, hipStreamWaitEvent(s, j.c->ev[3], 0));
       j.c->dirty = false;
     }
-  uint8_t* tab = static_cast<uint8_t*>(lead->batch_dev);
+  uint8_t* tab = static_cast<uint8_t*>(lead->batch_dev[b]);
   HIP_OK(lead, hipMemcpyAsync(tab, pin, bytes_tab, hipMemcpyHostToDevice, s));
   BatchDev B;
   B.jobs = reinterpret_cast<const FrameJob*>(tab);
@@ -2084,7 +2153,7 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
       SEQ(run(kKGrainTemplate, lead->side));
       SEQ(hipEventRecord(lead->ev_side_done, lead->side));
     }
-    if (!use_graph) SEQ(hipEventRecord(lead->ev[0], s));
+    if (!use_graph) SEQ(hipEventRecord(lead->bev[b][0], s));
     SEQ(stage(AV1H_STAGE_INTER, {kKMc, kKSpecial, kKItxInterL, kKItxInter}));
     if (side_itx) SEQ(hipStreamWaitEvent(s, lead->ev_itx_done, 0));
     if (has(kKernels)) {
@@ -2095,7 +2164,7 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
         lead->stage_hit[AV1H_STAGE_QUEUE] = true;
       }
     }
-    if (!use_graph) SEQ(hipEventRecord(lead->ev[1], s));
+    if (!use_graph) SEQ(hipEventRecord(lead->bev[b][1], s));
     SEQ(stage(AV1H_STAGE_DEBLOCK, {kKDeblockV, kKDeblockH}));
     SEQ(run(kKCopyLrCurr, s));
     SEQ(run(kKCopyCdefSrc, s));
@@ -2117,7 +2186,7 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
   for (bool& h : lead->stage_hit) h = false;
   if (use_graph) {
     hipGraphExec_t exec = nullptr;
-    const uint64_t key = live << 8 | (uint64_t)n; /* n <= 128 */
+    const uint64_t key = (uint64_t)b << 62 | live << 8 | (uint64_t)n; /* n <= 128 */
     for (auto& g : lead->graph_exec)
       if (g.first == key) exec = g.second;
     if (!exec) {
@@ -2140,13 +2209,13 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
       }
       lead->graph_exec.emplace_back(key, exec);
     }
-    HIP_OK(lead, hipEventRecord(lead->ev[0], s));
+    HIP_OK(lead, hipEventRecord(lead->bev[b][0], s));
     HIP_OK(lead, hipGraphLaunch(exec, s));
-    HIP_OK(lead, hipEventRecord(lead->ev[1], s));
+    HIP_OK(lead, hipEventRecord(lead->bev[b][1], s));
   } else {
     HIP_OK(lead, sequence());
   }
-  HIP_OK(lead, hipEventRecord(lead->ev[2], s));
+  HIP_OK(lead, hipEventRecord(lead->bev[b][2], s));
   const double t_sub = now_ms();
   /* read back */
   for (JobSpec& j : js) {
@@ -2166,27 +2235,75 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
   }
   /* into pinned memory: a copy to pageable memory is staged through a
    * runtime buffer, which costs more than the copy */
-  if (!lead->rb_pin) HIP_OK(lead, hipHostMalloc(&lead->rb_pin, 64 + kMaxBatch * 8, hipHostMallocDefault));
-  uint32_t* err_pin = static_cast<uint32_t*>(lead->rb_pin);
-  const uint64_t* hashes = reinterpret_cast<const uint64_t*>(static_cast<uint8_t*>(lead->rb_pin) + 64);
+  uint32_t* err_pin = static_cast<uint32_t*>(lead->rb_pin[b]);
+  uint64_t* hashes = reinterpret_cast<uint64_t*>(static_cast<uint8_t*>(lead->rb_pin[b]) + 64);
   HIP_OK(lead, hipMemcpyAsync(err_pin, &lead->counters[1], 4, hipMemcpyDeviceToHost, s));
-  if (any_hash)
-    HIP_OK(lead, hipMemcpyAsync(const_cast<uint64_t*>(hashes), lead->hash_dev, (size_t)n * 8, hipMemcpyDeviceToHost, s));
-  HIP_OK(lead, hipStreamSynchronize(s));
+  if (any_hash) HIP_OK(lead, hipMemcpyAsync(hashes, lead->hash_dev, (size_t)n * 8, hipMemcpyDeviceToHost, s));
+  HIP_OK(lead, hipEventRecord(lead->bev[b][3], s));
+  /* the kept slots are taken now; their pictures count once the batch is done */
+  for (JobSpec& j : js)
+    if (j.slot >= 0) {
+      j.c->used[j.slot] = true;
+      j.c->has_pic[j.slot] = false;
+      if (j.c->last_slot == j.slot) j.c->last_slot = -1;
+    }
+  for (JobSpec& j : js) {
+    ++j.c->inflight;
+    j.c->inflight_lead = lead;
+  }
+  BatchTicket& tk = lead->ticket[b];
+  tk.js = js;
+  tk.nv12.assign((size_t)n, 0);
+  for (int i = 0; i < n; ++i) tk.nv12[(size_t)i] = jobs[(size_t)i].f.nv12 != nullptr;
+  tk.launches = launches;
+  tk.any_hash = any_hash;
+  tk.t0 = t0;
+  tk.t_sub = t_sub;
+  tk.live = true;
+  lead->pending.push_back(b);
+  lead->next_buf = 1 - b;
+  return AV1H_OK;
+}
+
+/* Waits for the oldest launched batch of this lead and takes its results:
+ * the error word, checksums, stage times, slot bookkeeping, read backs. */
+int batch_finish(Ctx* lead) {
+  if (lead->pending.empty()) return AV1H_OK;
+  const int b = lead->pending.front();
+  lead->pending.pop_front();
+  BatchTicket& tk = lead->ticket[b];
+  tk.live = false;
+  std::vector<JobSpec>& js = tk.js;
+  for (JobSpec& j : js)
+    if (--j.c->inflight == 0) j.c->inflight_lead = nullptr;
+  hipStream_t s = lead->stream;
+  const uint32_t* err_pin = static_cast<const uint32_t*>(lead->rb_pin[b]);
+  const uint64_t* hashes = reinterpret_cast<const uint64_t*>(static_cast<uint8_t*>(lead->rb_pin[b]) + 64);
+  HIP_OK(lead, hipEventSynchronize(lead->bev[b][3]));
   const uint32_t err_word = *err_pin;
   const double t_done = now_ms();
-  for (JobSpec& j : js) j.c->have_frame = true;
   if (err_word) {
     HIP_OK(lead, hipMemsetAsync(lead->counters + 1, 0, 4, s));
     HIP_OK(lead, hipStreamSynchronize(s));
-    for (JobSpec& j : js) j.c->last_slot = -1;
+    for (JobSpec& j : js) {
+      j.c->last_slot = -1;
+      j.c->have_frame = false;
+      if (j.slot >= 0) {
+        /* a slot this batch took is given back; a reserved one stays the
+         * caller's, holding no picture */
+        if (j.reserved < 0) j.c->used[j.slot] = false;
+        j.c->has_pic[j.slot] = false;
+      }
+    }
     return set_err(lead, AV1H_ERR_INVALID,
                    "a block waited too long for a neighbour: the availability flags name blocks "
                    "that come later in coding order");
   }
+  const int launches = tk.launches;
+  const double t0 = tk.t0, t_sub = tk.t_sub;
   float recon = 0, filt = 0;
-  (void)hipEventElapsedTime(&recon, lead->ev[0], lead->ev[1]);
-  (void)hipEventElapsedTime(&filt, lead->ev[1], lead->ev[2]);
+  (void)hipEventElapsedTime(&recon, lead->bev[b][0], lead->bev[b][1]);
+  (void)hipEventElapsedTime(&filt, lead->bev[b][1], lead->bev[b][2]);
   if (lead->timing)
     for (int st = 0; st < AV1H_STAGES; ++st)
       if (lead->stage_hit[st]) {
@@ -2200,12 +2317,13 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
   for (JobSpec& j : js) {
     Ctx* c = j.c;
     const Layout& L = j.plan->lay;
+    c->have_frame = true;
     if (j.slot >= 0) {
-      c->used[j.slot] = true;
+      c->has_pic[j.slot] = true;
       c->slot_lay[j.slot] = L;
     }
     c->last_slot = j.slot;
-    c->nv12_current = jobs[&j - js.data()].f.nv12 != nullptr;
+    c->nv12_current = tk.nv12[(size_t)(&j - js.data())] != 0;
     c->hash_valid = c->hash_on;
     if (c->hash_on) c->last_hash = hashes[&j - js.data()];
     c->nv12_pitch_done = c->decode_nv12_pitch;
@@ -2239,6 +2357,13 @@ int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
   return AV1H_OK;
 }
 
+/* A batch launched and waited for (with any launched before it). */
+int run_batch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
+  int rc = batch_launch(lead, js, t0);
+  while (rc == AV1H_OK && !lead->pending.empty()) rc = batch_finish(lead);
+  return rc;
+}
+
 int run_plan(Ctx* c, Plan* plan, int keep, int reserved, const av1h_planes* out, av1h_frame_info* info,
              int* slot_out, double t0) {
   std::vector<JobSpec> js(1);
@@ -2249,8 +2374,12 @@ int run_plan(Ctx* c, Plan* plan, int keep, int reserved, const av1h_planes* out,
 /* the picture a slot names (the reference's gather rule,
  * cpu_backend.cpp:557-587) */
 const uint16_t* gather(Ctx* c, int slot, Layout* L) {
+  if (c->inflight) {
+    set_err(c, AV1H_ERR_INVALID, "a launched batch holds this context's frame: finish it first (av1h_batch_finish)");
+    return nullptr;
+  }
   if (slot >= 0) {
-    if (slot >= c->pool_slots || !c->used[slot]) {
+    if (slot >= c->pool_slots || !c->used[slot] || !c->has_pic[slot]) {
       set_err(c, AV1H_ERR_INVALID, "invalid reference slot");
       return nullptr;
     }
@@ -2327,9 +2456,14 @@ void av1h_destroy(av1h_ctx* h) {
                   (void*)c->pack, (void*)c->cdef_dir, (void*)c->trace})
     if (p) (void)hipFree(p);
   if (c->host_pin) (void)hipHostFree(c->host_pin);
-  if (c->batch_dev) (void)hipFree(c->batch_dev);
-  if (c->batch_pin) (void)hipHostFree(c->batch_pin);
-  if (c->rb_pin) (void)hipHostFree(c->rb_pin);
+  while (!c->pending.empty()) (void)batch_finish(c);
+  for (int b = 0; b < 2; ++b) {
+    if (c->batch_dev[b]) (void)hipFree(c->batch_dev[b]);
+    if (c->batch_pin[b]) (void)hipHostFree(c->batch_pin[b]);
+    if (c->rb_pin[b]) (void)hipHostFree(c->rb_pin[b]);
+    for (hipEvent_t e : c->bev[b])
+      if (e) (void)hipEventDestroy(e);
+  }
   free_plan(&c->scratch_plan);
   for (PackEntry& r : c->ring) {
     if (r.done) (void)hipEventSynchronize(r.done);
@@ -2493,7 +2627,7 @@ int av1h_reserve_reference(av1h_ctx* h) {
 void av1h_release_reference(av1h_ctx* h, int slot) {
   if (!h) return;
   std::lock_guard<std::mutex> lk(h->c.mu);
-  if (slot >= 0 && slot < h->c.pool_slots) h->c.used[slot] = false;
+  if (slot >= 0 && slot < h->c.pool_slots) h->c.used[slot] = h->c.has_pic[slot] = false;
 }
 
 size_t av1h_nv12_size(int width, int height, int ss_x, int ss_y, size_t pitch) {
@@ -2919,7 +3053,9 @@ void av1h_reset(av1h_ctx* h) {
   Ctx* c = &h->c;
   std::lock_guard<std::mutex> lk(c->mu);
   (void)hipSetDevice(c->device);
+  while (!c->pending.empty()) (void)batch_finish(c);
   std::memset(c->used, 0, sizeof(c->used));
+  std::memset(c->has_pic, 0, sizeof(c->has_pic));
   c->last_slot = -1;
   c->have_frame = false;
   if (c->frame) (void)hipMemsetAsync(c->frame, 0, c->cap * 2, c->stream);
@@ -3052,6 +3188,55 @@ int av1h_decode_planned_batch_on(av1h_ctx* launcher, av1h_ctx* const* ctxs, void
                     nullptr, slots_out ? &slots_out[i] : nullptr, -1};
   }
   return run_batch(lead, js, now_ms());
+}
+
+/* The batch launched without waiting for it (av1h_decode_planned_batch_on
+ * split in two): the slots are known at once, the results come with
+ * av1h_batch_finish. Stage timing and the queue trace take the waiting
+ * path. */
+int av1h_decode_planned_batch_async(av1h_ctx* launcher, av1h_ctx* const* ctxs, void* const* plans, const int* keep,
+                                    const int* reserved, int count, int* slots_out) {
+  int rc = check_batch_args(ctxs, count);
+  if (rc != AV1H_OK || !plans || !launcher) return AV1H_ERR_INVALID;
+  std::vector<Ctx*> cs(count);
+  for (int i = 0; i < count; ++i) cs[i] = &ctxs[i]->c;
+  std::vector<Ctx*> locked = cs;
+  locked.push_back(&launcher->c);
+  BatchLock lk(locked);
+  Ctx* lead = &launcher->c;
+  (void)hipSetDevice(lead->device);
+  std::vector<JobSpec> js(count);
+  for (int i = 0; i < count; ++i) {
+    if (!plans[i]) return set_err(lead, AV1H_ERR_INVALID, "plan %d of the batch is null", i);
+    js[i] = JobSpec{cs[i], static_cast<Plan*>(plans[i]), keep ? keep[i] : 1, reserved ? reserved[i] : -1, nullptr,
+                    nullptr, slots_out ? &slots_out[i] : nullptr, -1};
+  }
+  rc = batch_launch(lead, js, now_ms());
+  if (rc == AV1H_OK && (lead->timing || lead->trace_on))
+    while (rc == AV1H_OK && !lead->pending.empty()) rc = batch_finish(lead);
+  for (JobSpec& j : lead->ticket[0].js) j.slot_out = nullptr;
+  for (JobSpec& j : lead->ticket[1].js) j.slot_out = nullptr;
+  return rc;
+}
+
+int av1h_batch_finish(av1h_ctx* launcher) {
+  if (!launcher) return AV1H_ERR_INVALID;
+  Ctx* lead = &launcher->c;
+  std::vector<Ctx*> locked{lead};
+  {
+    std::lock_guard<std::mutex> lk(lead->mu);
+    if (lead->pending.empty()) return AV1H_OK;
+    for (JobSpec& j : lead->ticket[lead->pending.front()].js) locked.push_back(j.c);
+  }
+  BatchLock lk(locked);
+  (void)hipSetDevice(lead->device);
+  return batch_finish(lead);
+}
+
+int av1h_batch_pending(av1h_ctx* launcher) {
+  if (!launcher) return AV1H_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(launcher->c.mu);
+  return (int)launcher->c.pending.size();
 }
 
 }  // extern "C"

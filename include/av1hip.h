@@ -550,6 +550,19 @@ int av1h_decode_planned_batch(av1h_ctx* const* ctxs, void* const* plans, const i
  * size with av1h_set_graphs mode 2, as the serving loop's launchers use). */
 int av1h_decode_planned_batch_on(av1h_ctx* launcher, av1h_ctx* const* ctxs, void* const* plans,
                                  const int* keep, const int* reserved, int count, int* slots_out);
+/* The same batch launched without waiting for it (no counterpart in the
+ * reference): returns once the batch is queued on the launcher's stream,
+ * with its slots in slots_out; av1h_batch_finish(launcher) waits for the
+ * oldest launched batch and returns its status. At most two batches per
+ * launcher are in flight (a third launch first finishes the oldest), so the
+ * host prepares batch r + 1 while the device runs batch r. Until its batch
+ * is finished a context's pictures cannot be read or packed, and only the
+ * same launcher may put its next frame in a batch. */
+int av1h_decode_planned_batch_async(av1h_ctx* launcher, av1h_ctx* const* ctxs, void* const* plans, const int* keep,
+                                    const int* reserved, int count, int* slots_out);
+int av1h_batch_finish(av1h_ctx* launcher);
+/* Launched batches not yet finished. */
+int av1h_batch_pending(av1h_ctx* launcher);
 
 /* ------------------------------------------------------ superblock costs */
 /*

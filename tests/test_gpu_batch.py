@@ -164,3 +164,74 @@ def test_launch_shapes_leave_pixels_unchanged():
                 _check(f"shape {shape} {c.workload}", f, a, b)
     with pytest.raises(pkg.HipError):
         decs[0].set_tuning({"cdef_blocks_per_thread": 5})
+
+
+def _async_rounds(clips, launcher=None):
+    """The clips decoded in lockstep with av1h_decode_planned_batch_async:
+    round r is launched before round r - 1 is finished, releases of round
+    r - 1 taken before round r is launched. Returns each clip's per frame
+    device checksums."""
+    decs = [pkg.HipDecoder() for _ in clips]
+    for d in decs:
+        d.set_output_hash(True)
+    scheds = [pkg.slot_schedule(c) for c in clips]
+    plans = [[decs[i].plan_wire(pkg.wire_set_slots(c.wire(f), scheds[i][2][f])) for f in range(c.frames)]
+             for i, c in enumerate(clips)]
+    lead = launcher or decs[0]
+    got = [[] for _ in clips]
+    n = max(c.frames for c in clips)
+    live_prev = []
+    for f in range(n):
+        live = [i for i, c in enumerate(clips) if f < c.frames]
+        for i in live_prev:
+            for s in scheds[i][1][f - 1]:
+                decs[i].release(s)
+        slots = pkg.decode_planned_batch_async([decs[i] for i in live], [plans[i][f] for i in live],
+                                               [scheds[i][0][f] >= 0 for i in live],
+                                               [scheds[i][0][f] for i in live], launcher=lead)
+        assert slots == [scheds[i][0][f] for i in live]
+        if f > 0:
+            pkg.batch_finish(lead)
+            for i in live_prev:
+                got[i].append(decs[i].last_hash())
+        live_prev = live
+    pkg.batch_finish(lead)
+    for i in live_prev:
+        got[i].append(decs[i].last_hash())
+    assert pkg.hip_lib().av1h_batch_pending(lead.handle) == 0
+    for i, c in enumerate(clips):
+        for p in plans[i]:
+            decs[i].discard_plan(p)
+    return got
+
+
+@pytest.mark.parametrize("own_launcher", [False, True], ids=["lead", "launcher"])
+def test_async_batches_match_alone(own_launcher):
+    clips = [pkg.Clip(w, s, **o) for w, s, o in MIX]
+    launcher = pkg.HipDecoder() if own_launcher else None
+    got = _async_rounds(clips, launcher)
+    for c, g in zip(clips, got):
+        want = [pkg.picture_hash(p) for p in _alone(c)]
+        assert len(g) == len(want) == c.frames
+        for f, (a, b) in enumerate(zip(g, want)):
+            assert a == b, f"{c.workload} seed {c.seed} frame {f}: asynchronous batch differs from the plain decode"
+
+
+def test_async_batch_holds_its_contexts():
+    """While a launched batch holds a context's frame, its pictures cannot be
+    read and another launcher cannot batch it; after the finish they can."""
+    clip = pkg.Clip("c1_mini", 2, frames=1)
+    dec, other = pkg.HipDecoder(), pkg.HipDecoder()
+    plan = dec.plan_wire(clip.wire(0))
+    pkg.decode_planned_batch_async([dec], [plan], [True], [-1])
+    w, _cw, h, _bd, _sx, _sy = pkg.wire_header(clip.wire(0))
+    with pytest.raises(pkg.HipError, match="launched batch"):
+        dec.read_slot(-1, w, h)
+    with pytest.raises(pkg.HipError):
+        pkg.decode_planned_batch_async([dec], [plan], [False], [-1], launcher=other)
+    pkg.batch_finish(dec)
+    planes = dec.read_slot(-1, w, h)
+    want = _alone(clip)[0]
+    for p in range(3):
+        assert np.array_equal(planes[p], want[p])
+    dec.discard_plan(plan)
