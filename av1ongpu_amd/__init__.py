@@ -282,7 +282,7 @@ class Completion(ctypes.Structure):
                 ("status", ctypes.c_int32), ("batch_frames", ctypes.c_int32), ("arrival_ms", ctypes.c_double),
                 ("start_ms", ctypes.c_double), ("done_ms", ctypes.c_double),
                 ("picture", ctypes.POINTER(ctypes.c_uint16)), ("picture_samples", ctypes.c_int64),
-                ("hash", ctypes.c_uint64)]
+                ("hash", ctypes.c_uint64), ("plan_start_ms", ctypes.c_double), ("planned_ms", ctypes.c_double)]
 
 
 class ServerStats(ctypes.Structure):
@@ -301,6 +301,7 @@ def _bind_server(lib):
     lib.av1h_server_submit_wire.argtypes = [P, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t, ctypes.c_int,
                                             ctypes.POINTER(ctypes.c_int32), ctypes.c_int, ctypes.c_double,
                                             ctypes.c_uint64]
+    lib.av1h_server_submit_wire_ref.argtypes = lib.av1h_server_submit_wire.argtypes
     lib.av1h_server_submit_plan.argtypes = [P, ctypes.c_int, P, ctypes.c_int, ctypes.POINTER(ctypes.c_int32),
                                             ctypes.c_int, ctypes.c_double, ctypes.c_uint64]
     lib.av1h_server_close_stream.argtypes = [P, ctypes.c_int]
@@ -346,6 +347,7 @@ def _bind_costmodel(lib):
     lib.av1h_plan_sb_features.argtypes = [P, F]
     lib.av1h_plan_sb_times.argtypes = [P, P, F]
     lib.av1h_plan_schedule.argtypes = [P, P, F]
+    lib.av1h_plan_schedule_order.argtypes = [P, P, ctypes.c_int, F]
     lib.av1h_plan_schedule_model.argtypes = [P, P, P]
     lib.av1h_cm_create.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(P)]
     lib.av1h_cm_destroy.argtypes = [P]
@@ -365,6 +367,7 @@ def _fptr(a):
 
 
 CM_FEATURES = 24
+ORDER_LEVEL, ORDER_CODING, ORDER_PATH = 0, 1, 2
 CM_PARAMS = 5825
 
 
@@ -623,6 +626,14 @@ class HipDecoder:
         if rc != AV1H_OK:
             raise HipError(f"schedule failed ({rc}): {self.error()}")
 
+    def schedule_order(self, plan, order, costs=None):
+        """Order the plan's queue: ORDER_LEVEL, ORDER_CODING or ORDER_PATH, by
+        per-superblock costs (None: sample counts; for ORDER_LEVEL the static order)."""
+        c = None if costs is None else np.ascontiguousarray(costs, np.float32)
+        rc = self._lib.av1h_plan_schedule_order(self._h, plan, order, None if c is None else _fptr(c))
+        if rc != AV1H_OK:
+            raise HipError(f"schedule_order failed ({rc}): {self.error()}")
+
     def schedule_model(self, plan, model):
         rc = self._lib.av1h_plan_schedule_model(self._h, plan, model.handle)
         if rc != AV1H_OK:
@@ -827,6 +838,15 @@ class Server:
         rc = self._lib.av1h_server_submit_wire(self._h, stream, wire, len(wire), reserved, arr, n, t, tag)
         if rc != AV1H_OK:
             raise HipError(f"submit_wire failed ({rc})")
+
+    def submit_wire_ref(self, stream, wire, reserved=-1, release=(), arrival_ms=None, tag=0):
+        """submit_wire without the copy: `wire` (bytes) must stay alive until the
+        frame's completion has been polled."""
+        arr, n = self._rel(release)
+        t = self.now_ms() if arrival_ms is None else arrival_ms
+        rc = self._lib.av1h_server_submit_wire_ref(self._h, stream, wire, len(wire), reserved, arr, n, t, tag)
+        if rc != AV1H_OK:
+            raise HipError(f"submit_wire_ref failed ({rc})")
 
     def submit_plan(self, stream, plan, reserved=-1, release=(), arrival_ms=None, tag=0):
         arr, n = self._rel(release)

@@ -63,14 +63,28 @@ def train(X, y, epochs=20, lr=1e-3, seed=1, holdout=0.1):
     return cm, losses, r2, dt
 
 
-def schedule_all(decoders, plans, model=None):
-    """Orders every plan's queue by the model (None: the static order)."""
+def schedule_all(decoders, plans, model=None, order=None):
+    """Orders every plan's queue: by dependency level with the model's costs
+    breaking ties inside a level (order None; model None: the static level
+    order), or in `order` (ORDER_CODING, ORDER_LEVEL, ORDER_PATH) with the
+    model's predicted superblock costs (model None: sample counts)."""
+    from . import ORDER_LEVEL
     for dec, ps in zip(decoders, plans):
         for p in ps:
-            if model is None:
+            if order is None:
+                if model is None:
+                    dec.schedule(p, None)
+                else:
+                    dec.schedule_model(p, model)
+                continue
+            costs = None
+            if model is not None:
+                x = dec.sb_features(p)
+                costs = model.predict(x) if len(x) else None
+            if order == ORDER_LEVEL and costs is None:
                 dec.schedule(p, None)
             else:
-                dec.schedule_model(p, model)
+                dec.schedule_order(p, order, costs)
 
 
 def run_rounds(decoders, plans, scheds, hashes=None, timing=False):

@@ -92,6 +92,7 @@ struct Plan {
   DevGrain grain;
   void* dbuf = nullptr;
   size_t dbytes = 0;
+  hipEvent_t ready = nullptr; /* its upload has ended (plans of av1h_plan_frame) */
   bool uses_lr = false, upscaled = false, has_grain = false;
   int blocks = 0;
   double flatten_ms = 0, upload_ms = 0; /* host time of the plan's making */
@@ -107,6 +108,7 @@ struct Plan {
   int sb_w = 0, sb_h = 0;
   std::vector<float> sb_feat;
   std::vector<int32_t> queue_sb, queue_sb_static, queue_static, key_off_static;
+
   /* the queued blocks in coding order (index qi): block, predecessors
    * (q_dep[q_dep_off[qi] .. q_dep_off[qi + 1])), and the qi of every entry
    * of the static order */
@@ -152,6 +154,7 @@ constexpr int kPackRing = 3;
 /* a pinned staging buffer and the stream its uploads run on */
 struct Stager {
   hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr; /* its last copy has ended: the buffer is free */
   void* pin = nullptr;
   size_t cap = 0;
 };
@@ -245,6 +248,7 @@ struct Ctx {
   std::vector<Stager*> stagers;
   std::vector<std::pair<size_t, void*>> plan_bufs;
   size_t plan_buf_bytes = 0;
+  std::vector<hipEvent_t> free_events; /* for the plans' upload events */
 };
 
 int set_err(std::string* e, int code, const char* fmt, ...) {
@@ -1099,6 +1103,7 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
   /* motion compensation tasks: items (whole blocks, or row strips of blocks
    * over kMcSamples samples) in coding order, with their positions, windows
    * and LDS offsets, packed into tasks within the kernel's budgets */
+  int64_t inter_bytes = 0; /* compulsory bytes of the inter blocks: prediction written, references read */
   H->mc_items.clear();
   H->mc_items.reserve(H->inter_list.size() + H->inter_list.size() / 4);
   H->mc_task.clear();
@@ -1112,6 +1117,7 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
     close_task();
     for (int32_t bi : H->inter_list) {
       const DevBlock& d = H->blocks[bi];
+      inter_bytes += (int64_t)d.w * d.h * (2 + 2 * (d.compound ? 2 : 1));
       const int sx = d.plane ? hdr.ss_x : 0, sy = d.plane ? hdr.ss_y : 0;
       const int per = d.w * d.h <= kMcSamples ? d.h : kMcSamples / d.w;
       for (int y0 = 0; y0 < d.h; y0 += per) {
@@ -1408,14 +1414,10 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
     /* part 0: inter samples written once and read from each reference, the
      * coefficients of every transform (int16 each); the queue: per queued
      * sample its residual read and the picture sample written */
-    for (int32_t i : H->inter_list) {
-      const DevBlock& d = H->blocks[i];
-      plan->alg_bytes[AV1H_STAGE_INTER] += (int64_t)d.w * d.h * (2 + 2 * (d.compound ? 2 : 1));
-    }
+    plan->alg_bytes[AV1H_STAGE_INTER] = inter_bytes;
     for (const DevTx& t : H->tx)
       if (!(t.flags & kTxToResidual)) plan->alg_bytes[AV1H_STAGE_INTER] += 2 * (int64_t)t.w * t.h;
-    for (int32_t i : H->queue_list)
-      plan->alg_bytes[AV1H_STAGE_QUEUE] += 4 * (int64_t)H->blocks[i].w * H->blocks[i].h;
+    for (const int32_t a : plan->q_area) plan->alg_bytes[AV1H_STAGE_QUEUE] += 4 * (int64_t)a;
     bool has_edges[3] = {false, false, false};
     for (const DevDeblock& d : H->deblock) {
       has_edges[d.plane] = true;
@@ -1649,6 +1651,9 @@ void give_buffer(Ctx* c, void* p, size_t n) {
   (void)hipFree(p);
 }
 
+/* The plan's upload is queued on a stager's stream and the planning thread
+ * goes on: the plan's `ready` event ends the copy, every decode waits for it
+ * on the device, and the stager is taken again once its copy has ended. */
 int upload_plan(Ctx* c, Plan* plan, const Host& H, std::string* err) {
   const size_t n = H.total;
   int rc = take_buffer(c, n, &plan->dbuf, &plan->dbytes, err);
@@ -1657,20 +1662,31 @@ int upload_plan(Ctx* c, Plan* plan, const Host& H, std::string* err) {
   Stager* st = nullptr;
   {
     std::lock_guard<std::mutex> lk(c->plan_mu);
-    if (!c->stagers.empty()) {
-      st = c->stagers.back();
-      c->stagers.pop_back();
+    for (size_t i = 0; i < c->stagers.size(); ++i)
+      if (!c->stagers[i]->done || hipEventQuery(c->stagers[i]->done) == hipSuccess) {
+        st = c->stagers[i];
+        c->stagers.erase(c->stagers.begin() + (std::ptrdiff_t)i);
+        break;
+      }
+    if (!st && c->stagers.size() >= 64) {
+      st = c->stagers.front();
+      c->stagers.erase(c->stagers.begin());
     }
-  }
-  if (!st) {
-    st = new Stager;
-    if (hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking) != hipSuccess) {
-      delete st;
-      return set_err(err, AV1H_ERR_DEVICE, "creating an upload stream");
+    if (!c->free_events.empty()) {
+      plan->ready = c->free_events.back();
+      c->free_events.pop_back();
     }
   }
   hipError_t e = hipSuccess;
-  if (st->cap < n) {
+  if (!plan->ready) e = hipEventCreateWithFlags(&plan->ready, hipEventDisableTiming);
+  if (!st) {
+    st = new Stager;
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&st->done, hipEventDisableTiming);
+  } else if (st->done) {
+    e = hipEventSynchronize(st->done);
+  }
+  if (e == hipSuccess && st->cap < n) {
     if (st->pin) (void)hipHostFree(st->pin);
     st->pin = nullptr;
     st->cap = 0;
@@ -1681,14 +1697,34 @@ int upload_plan(Ctx* c, Plan* plan, const Host& H, std::string* err) {
   if (e == hipSuccess) {
     const size_t up = stage_parts(H, static_cast<uint8_t*>(st->pin), &plan->dev.coef16);
     e = hipMemcpyAsync(plan->dbuf, st->pin, up, hipMemcpyHostToDevice, st->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(st->stream);
+    if (e == hipSuccess) e = hipEventRecord(plan->ready, st->stream);
+    if (e == hipSuccess) e = hipEventRecord(st->done, st->stream);
   }
-  {
+  if (st->stream && st->done) {
     std::lock_guard<std::mutex> lk(c->plan_mu);
     c->stagers.push_back(st);
+  } else {
+    if (st->stream) (void)hipStreamDestroy(st->stream);
+    delete st;
   }
-  if (e != hipSuccess) return set_err(err, AV1H_ERR_DEVICE, "frame plan upload: %s", hipGetErrorString(e));
+  if (e != hipSuccess) {
+    if (plan->ready) (void)hipEventSynchronize(plan->ready);
+    return set_err(err, AV1H_ERR_DEVICE, "frame plan upload: %s", hipGetErrorString(e));
+  }
   return AV1H_OK;
+}
+
+/* waits for the plan's upload and gives its event back */
+void settle_plan(Ctx* c, Plan* p) {
+  if (!p->ready) return;
+  (void)hipEventSynchronize(p->ready);
+  if (c) {
+    std::lock_guard<std::mutex> lk(c->plan_mu);
+    c->free_events.push_back(p->ready);
+  } else {
+    (void)hipEventDestroy(p->ready);
+  }
+  p->ready = nullptr;
 }
 
 void free_plan(Plan* p) {
@@ -2047,8 +2083,10 @@ int batch_launch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
   std::memcpy(pin, jobs.data(), sizeof(FrameJob) * n);
   std::memcpy(pin + o_prefix, prefix.data(), prefix.size() * 4);
   if (n > 1) merge_queues(js, reinterpret_cast<uint32_t*>(pin + o_queue));
-  /* everything queued on the other contexts' streams (plan uploads, buffer
-   * clears) comes first */
+  /* the plans' uploads, and everything queued on the other contexts'
+   * streams (buffer clears), come first */
+  for (JobSpec& j : js)
+    if (j.plan->ready) HIP_OK(lead, hipStreamWaitEvent(s, j.plan->ready, 0));
   for (JobSpec& j : js)
     if (j.c != lead && j.c->dirty) {
       HIP_OK(lead, hipEventRecord(j.c->ev[3], j.c->stream));
@@ -2344,7 +2382,7 @@ int batch_finish(Ctx* lead) {
     c->stats.blocks = j.plan->blocks;
     /* the queue's dependency levels: the counterpart of the reference
      * scheduler's wave count */
-    c->stats.waves = (int32_t)j.plan->key_off.size() - 1;
+    c->stats.waves = (int32_t)j.plan->key_off_static.size() - 1;
     c->stats.kernel_launches = launches;
     c->stats.device_recon_ms = recon;
     c->stats.device_filter_ms = filt;
@@ -2473,7 +2511,11 @@ void av1h_destroy(av1h_ctx* h) {
   }
   for (auto& b : c->plan_bufs) (void)hipFree(b.second);
   c->plan_bufs.clear();
+  for (hipEvent_t e : c->free_events) (void)hipEventDestroy(e);
+  c->free_events.clear();
   for (Stager* st : c->stagers) {
+    if (st->done) (void)hipEventSynchronize(st->done);
+    if (st->done) (void)hipEventDestroy(st->done);
     if (st->stream) (void)hipStreamDestroy(st->stream);
     if (st->pin) (void)hipHostFree(st->pin);
     delete st;
@@ -2572,6 +2614,7 @@ int av1h_plan_frame(av1h_ctx* h, const av1h_frame_desc* f, void** plan_out) {
   p->flatten_ms = t1 - t0;
   p->upload_ms = now_ms() - t1;
   if (rc != AV1H_OK) {
+    settle_plan(c, p);
     give_buffer(c, p->dbuf, p->dbytes);
     delete p;
     std::lock_guard<std::mutex> lk(c->mu);
@@ -2605,8 +2648,9 @@ int av1h_decode_planned(av1h_ctx* h, void* plan, int keep, int reserved, const a
 void av1h_discard_plan(av1h_ctx* h, void* plan) {
   if (!plan) return;
   Plan* p = static_cast<Plan*>(plan);
+  if (h) (void)hipSetDevice(h->c.device);
+  settle_plan(h ? &h->c : nullptr, p);
   if (h) {
-    (void)hipSetDevice(h->c.device);
     give_buffer(&h->c, p->dbuf, p->dbytes);
     p->dbuf = nullptr;
     p->dbytes = 0;
@@ -2814,16 +2858,21 @@ int av1h_plan_sb_times(av1h_ctx* h, const void* plan, float* us) {
 }
 
 namespace {
-/* The cost schedule: every queued block costs its superblock's cost shared
- * by sample count, and its longest remaining path through the block graph
- * (its cost plus the longest path of what waits for it) ranks it among the
- * blocks of its dependency level: levels in order as in the static order
- * (so blocks are still handed out about when they become ready), the most
- * critical first inside a level, coding order between equals. */
-bool schedule_costs(Plan* p, const float* cost) {
+/* Queue orders beyond the static one. Every queued block costs its
+ * superblock's cost shared by sample count (its sample count when no costs
+ * are given), and its remaining path is its cost plus the longest remaining
+ * path of what waits for it.
+ *  - AV1H_ORDER_LEVEL: by dependency level, the longest remaining path first
+ *    inside a level, coding order between equals;
+ *  - AV1H_ORDER_CODING: coding order (the row order of the superblocks);
+ *  - AV1H_ORDER_PATH: the longest remaining path first (list scheduling by
+ *    critical path), coding order between equals.
+ * Each is a topological order of the block graph (a block's remaining path
+ * exceeds that of every block waiting for it), so a queue handed out in it
+ * always progresses. */
+bool schedule_order(Plan* p, int order, const float* cost) {
   const int nq = (int)p->q_block.size();
   const int nsb = p->sb_w * p->sb_h;
-  /* queued samples per superblock (feature 14 holds them / 4096) */
   std::vector<double> area((size_t)nq, 0.0), sb_area((size_t)nsb, 0.0);
   std::vector<int32_t> sb_of((size_t)nq, 0);
   for (size_t k = 0; k < p->queue_static.size(); ++k) sb_of[(size_t)p->queue_qi[k]] = p->queue_sb_static[k];
@@ -2835,19 +2884,31 @@ bool schedule_costs(Plan* p, const float* cost) {
   std::vector<double> path((size_t)nq, 0.0), succ((size_t)nq, 0.0);
   for (int qi = nq - 1; qi >= 0; --qi) {
     const int sb = sb_of[(size_t)qi];
-    const double c = sb_area[(size_t)sb] > 0 ? (double)cost[sb] * area[(size_t)qi] / sb_area[(size_t)sb] : 0.0;
+    double c = area[(size_t)qi] / 64.0;
+    if (cost) c = sb_area[(size_t)sb] > 0 ? (double)cost[sb] * area[(size_t)qi] / sb_area[(size_t)sb] : 0.0;
     path[(size_t)qi] = std::max(c, 0.01) + succ[(size_t)qi];
     for (int32_t e = p->q_dep_off[(size_t)qi]; e < p->q_dep_off[(size_t)qi + 1]; ++e) {
       const int32_t o = p->q_dep[(size_t)e];
+      if (o >= qi) return false; /* a predecessor later in coding order */
       succ[(size_t)o] = std::max(succ[(size_t)o], path[(size_t)qi]);
     }
   }
   std::vector<int32_t> ord((size_t)nq);
   for (int k = 0; k < nq; ++k) ord[(size_t)k] = k;
-  std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t b) {
-    const int32_t la = p->q_level[(size_t)a], lb = p->q_level[(size_t)b];
-    return la != lb ? la < lb : path[(size_t)a] > path[(size_t)b];
-  });
+  if (order == AV1H_ORDER_LEVEL)
+    std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t b) {
+      const int32_t la = p->q_level[(size_t)a], lb = p->q_level[(size_t)b];
+      return la != lb ? la < lb : path[(size_t)a] > path[(size_t)b];
+    });
+  else if (order == AV1H_ORDER_PATH)
+    std::stable_sort(ord.begin(), ord.end(),
+                     [&](int32_t a, int32_t b) { return path[(size_t)a] > path[(size_t)b]; });
+  /* the keys a batch merges its frames' queues by (key_off: where each key
+   * starts; keys non-decreasing along a queue): the level; for the path
+   * order the remaining path in quarter units counted down from the frame's
+   * longest, so that merged frames line up by remaining path; for coding
+   * order runs of 64 entries */
+  const double top = nq ? path[(size_t)ord[0]] : 0.0;
   p->queue_host.resize((size_t)nq);
   p->queue_sb.resize((size_t)nq);
   p->key_off.clear();
@@ -2855,7 +2916,9 @@ bool schedule_costs(Plan* p, const float* cost) {
     const int32_t qi = ord[(size_t)k];
     p->queue_host[(size_t)k] = p->q_block[(size_t)qi];
     p->queue_sb[(size_t)k] = sb_of[(size_t)qi];
-    const int key = p->q_level[(size_t)qi];
+    const int key = order == AV1H_ORDER_LEVEL ? p->q_level[(size_t)qi]
+                    : order == AV1H_ORDER_PATH ? (int)((top - path[(size_t)qi]) * 4.0)
+                                               : k >> 6;
     while ((int)p->key_off.size() <= key) p->key_off.push_back((int32_t)k);
   }
   p->key_off.push_back((int32_t)nq);
@@ -2864,6 +2927,7 @@ bool schedule_costs(Plan* p, const float* cost) {
 
 int upload_queue(Ctx* c, Plan* p) {
   if (p->queue_host.empty()) return AV1H_OK;
+  if (p->ready) HIP_OK(c, hipEventSynchronize(p->ready));
   HIP_OK(c, hipMemcpy(const_cast<int32_t*>(p->dev.queue_list), p->queue_host.data(), p->queue_host.size() * 4,
                       hipMemcpyHostToDevice));
   return AV1H_OK;
@@ -2880,9 +2944,21 @@ int av1h_plan_schedule(av1h_ctx* h, void* plan, const float* sb_cost_us) {
     p->queue_host = p->queue_static;
     p->key_off = p->key_off_static;
     p->queue_sb = p->queue_sb_static;
-  } else if (!schedule_costs(p, sb_cost_us)) {
+  } else if (!schedule_order(p, AV1H_ORDER_LEVEL, sb_cost_us)) {
     return set_err(c, AV1H_ERR_INVALID, "the superblock dependencies of this plan are cyclic");
   }
+  return upload_queue(c, p);
+}
+
+int av1h_plan_schedule_order(av1h_ctx* h, void* plan, int order, const float* sb_cost_us) {
+  if (!h || !plan || order < AV1H_ORDER_LEVEL || order > AV1H_ORDER_PATH) return AV1H_ERR_INVALID;
+  if (order == AV1H_ORDER_LEVEL && !sb_cost_us) return av1h_plan_schedule(h, plan, nullptr);
+  Ctx* c = &h->c;
+  std::lock_guard<std::mutex> lk(c->mu);
+  (void)hipSetDevice(c->device);
+  Plan* p = static_cast<Plan*>(plan);
+  if (!schedule_order(p, order, sb_cost_us))
+    return set_err(c, AV1H_ERR_INVALID, "the superblock dependencies of this plan are cyclic");
   return upload_queue(c, p);
 }
 

@@ -39,6 +39,9 @@ struct SFrame {
   int stream = -1;
   double arrival = 0;
   std::vector<uint8_t> wire; /* to be planned, or empty */
+  const uint8_t* wire_ref = nullptr; /* borrowed wire bytes to be planned */
+  size_t wire_ref_len = 0;
+  double plan_start = 0, planned_at = 0; /* planning of a wire frame */
   void* plan = nullptr;
   bool own_plan = true;
   bool planned = false;
@@ -78,6 +81,8 @@ struct av1h_server {
   std::vector<std::thread> planners;
   std::string err;
   av1h_server_stats stats;
+  int running = 0;         /* batches on the device, over every launcher */
+  double busy_since = 0;   /* when `running` last left 0               */
   /* one scheduler thread per launcher context: each forms and runs its own
    * batches, so a long batch (a large key frame) does not hold back the
    * frames that arrive behind it */
@@ -142,6 +147,8 @@ void complete(av1h_server* s, SFrame* f, int slot, int status, double start, dou
   c.picture = pic;
   c.picture_samples = samples;
   c.hash = hash;
+  c.plan_start_ms = f->plan_start > 0 ? f->plan_start : f->arrival;
+  c.planned_ms = f->planned_at > 0 ? f->planned_at : f->arrival;
   s->done.push_back(c);
 }
 
@@ -155,16 +162,21 @@ void planner_main(av1h_server* s) {
     SStream* st = s->streams[f->stream];
     av1h_ctx* ctx = st->ctx;
     f->planning = true;
+    f->plan_start = clock_ms();
     lk.unlock();
     void* plan = nullptr;
-    const int rc = av1h_plan_frame_wire(ctx, f->wire.data(), f->wire.size(), &plan);
+    const int rc = f->wire_ref ? av1h_plan_frame_wire(ctx, f->wire_ref, f->wire_ref_len, &plan)
+                               : av1h_plan_frame_wire(ctx, f->wire.data(), f->wire.size(), &plan);
+    const double t_planned = clock_ms();
     lk.lock();
+    f->planned_at = t_planned;
     f->planning = false;
     f->planned = true;
     f->plan_rc = rc;
     f->plan = rc == AV1H_OK ? plan : nullptr;
     f->heavy = f->plan && s->cfg.heavy_levels > 0 && av1h_plan_levels(f->plan) > s->cfg.heavy_levels;
     std::vector<uint8_t>().swap(f->wire);
+    f->wire_ref = nullptr;
     s->cv_work.notify_one();
   }
 }
@@ -244,8 +256,10 @@ void sched_main(av1h_server* s, av1h_ctx* launcher, int role) {
       keep[i] = frames[i]->reserved >= 0;
       reserved[i] = frames[i]->reserved;
     }
-    lk.unlock();
     const double t0 = clock_ms();
+    /* busy time is the union of the launchers' batch spans */
+    if (s->running++ == 0) s->busy_since = t0;
+    lk.unlock();
     const int rc = av1h_decode_planned_batch_on(launcher, ctxs.data(), plans.data(), keep.data(), reserved.data(),
                                                 n, slots.data());
     const double t1 = clock_ms();
@@ -264,7 +278,7 @@ void sched_main(av1h_server* s, av1h_ctx* launcher, int role) {
     }
     lk.lock();
     if (rc != AV1H_OK) s->err = msg;
-    s->stats.busy_ms += t1 - t0;
+    if (--s->running == 0) s->stats.busy_ms += clock_ms() - s->busy_since;
     ++s->stats.batches;
     s->stats.frames += n;
     s->stats.max_batch_frames = std::max(s->stats.max_batch_frames, n);
@@ -389,6 +403,20 @@ int av1h_server_submit_wire(av1h_server* s, int stream, const uint8_t* wire, siz
   if (s->cfg.planner_threads < 1) return AV1H_ERR_UNSUPPORTED;
   SFrame* f = new SFrame;
   f->wire.assign(wire, wire + len);
+  f->reserved = reserved_slot;
+  f->release.assign(release, release + nrelease);
+  f->arrival = arrival_ms;
+  f->tag = tag;
+  return queue_frame(s, stream, f);
+}
+
+int av1h_server_submit_wire_ref(av1h_server* s, int stream, const uint8_t* wire, size_t len, int reserved_slot,
+                                const int32_t* release, int nrelease, double arrival_ms, uint64_t tag) {
+  if (!s || !wire || nrelease < 0 || (nrelease && !release)) return AV1H_ERR_INVALID;
+  if (s->cfg.planner_threads < 1) return AV1H_ERR_UNSUPPORTED;
+  SFrame* f = new SFrame;
+  f->wire_ref = wire;
+  f->wire_ref_len = len;
   f->reserved = reserved_slot;
   f->release.assign(release, release + nrelease);
   f->arrival = arrival_ms;

@@ -261,10 +261,13 @@ def run_mix(args):
     for d in decs:
         d.set_output_hash(True)
     hp, hb = [], []
-    plain([], hp)
+    if not args.mix_batched_only:
+        plain([], hp)
     batched([], hb)
     for d in decs:
         d.set_output_hash(False)
+    if args.mix_batched_only:
+        hp = list(hb)
     hp = [hp[f * ns + i] for i in range(ns) for f in range(MIX_FRAMES)]
     hb = [hb[f * ns + i] for i in range(ns) for f in range(MIX_FRAMES)]
     mism = sum(1 for a, b in zip(hp, hb) if a != b)
@@ -281,10 +284,14 @@ def run_mix(args):
                  "tuned_ms": round(t_tuned * 1e3, 2), "seconds": round(time.perf_counter() - t, 1)}
         log(f"mix: autotuned launch shapes in {tuned['seconds']} s: {t_base * 1e3:.1f} -> {t_tuned * 1e3:.1f} ms")
     for _ in range(args.warmup):
-        plain([])
+        if not args.mix_batched_only:
+            plain([])
         batched([])
     out = {}
     for name, fn in (("plain", plain), ("batched", batched)):
+        if name == "plain" and args.mix_batched_only:
+            out[name] = {"value": None}
+            continue
         lat = []
         t0 = time.perf_counter()
         for _ in range(args.steps):
@@ -303,7 +310,7 @@ def run_mix(args):
         "workload": "16 streams: 4x 426x240, 4x 854x480, 4x 1280x720, 3x 1920x1080, 1x 3840x2160, 8 and 10 bit "
                     "alternating (2160p 10 bit), 4:2:0, 32 frames each, 1280x720-workload syntax statistics",
         "plain": out["plain"], "batched": out["batched"],
-        "speedup": round(out["batched"]["value"] / out["plain"]["value"], 3),
+        "speedup": round(out["batched"]["value"] / out["plain"]["value"], 3) if out["plain"]["value"] else None,
         "bit_exact": {"frames_checked": len(hp), "mismatched_frames": mism,
                       "method": "device checksum of every picture, batched decode (rounds launched ahead of "
                                 "the previous round's completion) vs plain decode of the same plans"},
@@ -341,13 +348,19 @@ def run_costmodel(args, decs, plans, sched):
     log(f"cost model: {len(y)} superblock timings recorded in {time.perf_counter() - t:.1f} s")
     model, losses, r2, train_s = cmod.train(X, y, epochs=args.cm_epochs, lr=1e-3, seed=args.seed)
     log(f"cost model: trained {args.cm_epochs} epochs in {train_s:.1f} s, hold-out R^2 {r2:.3f}")
-    ns = len(decs)
     for d in decs:
         d.set_output_hash(True)
     res = {}
     hashes = {}
-    for name, m in (("static", None), ("model", model)):
-        cmod.schedule_all(decs, plans, m)
+    # (name, model, order): the static row-order schedule (coding order),
+    # the static dependency-level order, critical path list scheduling with
+    # sample counts as costs (no model), and the model's costs in the level
+    # order and in critical path order
+    schedules = [("row_order", None, pkg.ORDER_CODING), ("level", None, pkg.ORDER_LEVEL),
+                 ("path_samples", None, pkg.ORDER_PATH), ("level_model", model, None),
+                 ("path_model", model, pkg.ORDER_PATH)]
+    for name, m, order in schedules:
+        cmod.schedule_all(decs, plans, m, order)
         cmod.run_rounds(decs, plans, sched)                     # warm
         h = []
         wall = min(cmod.run_rounds(decs, plans, sched, h)[0] for _ in range(max(1, args.steps)))
@@ -358,17 +371,27 @@ def run_costmodel(args, decs, plans, sched):
     cmod.schedule_all(decs, plans, None)
     for d in decs:
         d.set_output_hash(False)
-    mism = sum(1 for a, b in zip(hashes["static"], hashes["model"]) for u, v in zip(a, b) if u != v)
+    mism = sum(1 for name in hashes for a, b in zip(hashes["level"], hashes[name]) for u, v in zip(a, b) if u != v)
     model.close()
+    best_static = min(("row_order", "level", "path_samples"), key=lambda k: res[k]["wall_ms"])
     return {
         "samples": int(len(y)), "epochs": args.cm_epochs, "train_seconds": round(train_s, 2),
         "loss_first_last": [round(float(losses[0]), 4), round(float(losses[-1]), 4)],
         "holdout_r2": round(r2, 3),
-        "static": res["static"], "model": res["model"],
-        "queue_speedup": round(res["static"]["queue_device_ms"] / max(res["model"]["queue_device_ms"], 1e-9), 3),
-        "wall_speedup": round(res["static"]["wall_ms"] / max(res["model"]["wall_ms"], 1e-9), 3),
-        "bit_exact": {"frames_checked": sum(len(a) for a in hashes["static"]), "mismatched_frames": mism,
-                      "method": "device checksum of every picture, model schedule vs static schedule"},
+        "schedules": res,
+        "static": res["row_order"], "model": res["path_model"],
+        "wall_speedup": round(res["row_order"]["wall_ms"] / max(res["path_model"]["wall_ms"], 1e-9), 3),
+        "queue_speedup": round(res["row_order"]["queue_device_ms"] / max(res["path_model"]["queue_device_ms"], 1e-9),
+                               3),
+        "vs_best_static": {"schedule": best_static,
+                           "wall_speedup": round(res[best_static]["wall_ms"] / max(res["path_model"]["wall_ms"], 1e-9),
+                                                 3)},
+        "note": "static = the static row-order schedule (coding order); model = the queue ordered by the critical "
+                "path of the model's predicted superblock costs; vs_best_static compares with the fastest order "
+                "that uses no model",
+        "bit_exact": {"frames_checked": sum(len(a) for a in hashes["level"]) * (len(hashes) - 1),
+                      "mismatched_frames": mism,
+                      "method": "device checksum of every picture, every schedule vs the static level order"},
     }
 
 
@@ -395,13 +418,14 @@ def run_serve(args):
     import numpy as np
 
     t = time.perf_counter()
-    gops, plans, scheds, want = [], [], [], []
+    gops, plans, scheds, want, wires = [], [], [], [], []
     owner = pkg.HipDecoder(0)
     for ci, (name, _n, w, h, bd, cw, grain) in enumerate(SERVE_CLASSES):
         clip = pkg.Clip("c2", args.seed * 100 + ci, width=w, height=h, bit_depth=bd, coded_width=cw, grain=grain,
                         frames=SERVE_GOP)
         keep, rel, refs = pkg.slot_schedule(clip, 16)
-        ps = [owner.plan_wire(pkg.wire_set_slots(clip.wire(f), refs[f])) for f in range(clip.frames)]
+        ws = [pkg.wire_set_slots(clip.wire(f), refs[f]) for f in range(clip.frames)]
+        ps = [owner.plan_wire(w) for w in ws]
         # the checksums of the group decoded alone, through the same plans
         ref = pkg.HipDecoder(0)
         ref.set_output_hash(True)
@@ -414,6 +438,7 @@ def run_serve(args):
                 ref.release(sl)
         ref.close()
         del clip
+        wires.append(ws)
         plans.append(ps)
         scheds.append((keep, rel))
         want.append(hs)
@@ -425,7 +450,10 @@ def run_serve(args):
     inter_levels = [sorted(pkg.hip_lib().av1h_plan_levels(p) for p in ps)[len(ps) // 2] for ps in plans]
     heavy = args.serve_heavy if args.serve_heavy >= 0 else 4 * max(inter_levels)
     log(f"serve: queue levels per group: max {levels}, median {inter_levels}; heavy above {heavy}")
-    srv = pkg.Server(0, max_batch=args.serve_batch, planner_threads=0, hbm_budget=200 << 30,
+    for ps in plans:
+        for p in ps:
+            owner.discard_plan(p)
+    srv = pkg.Server(0, max_batch=args.serve_batch, planner_threads=args.serve_planners, hbm_budget=200 << 30,
                      nv12_pitch=SERVE_PITCH, hash_pictures=True, launchers=args.serve_launchers,
                      heavy_levels=heavy, heavy_launchers=args.serve_heavy_launchers,
                      heavy_batch=args.serve_heavy_batch)
@@ -470,14 +498,11 @@ def run_serve(args):
             time.sleep(min(0.002, (target - now) / 1e3))
         keep, rel = scheds[ci]
         g = k % SERVE_GOP
-        srv.submit_plan(sid, plans[ci][g], keep[g], rel[g], arrival_ms=target, tag=(si << 32) | k)
+        srv.submit_wire_ref(sid, wires[ci][g], keep[g], rel[g], arrival_ms=target, tag=(si << 32) | k)
     stop.set()
     th.join(timeout=120)
     st = srv.stats()
     srv.close()
-    for ps in plans:
-        for p in ps:
-            owner.discard_plan(p)
     owner.close()
     px = [w * h for _n, _c, w, h, _b, _cw, _g in SERVE_CLASSES]
     lat = {ci: [] for ci in range(len(SERVE_CLASSES))}
@@ -497,12 +522,14 @@ def run_serve(args):
             pix += px[ci]
             last_done = max(last_done, c["done_ms"])
     allv = [v for ci in lat for v in lat[ci]]
+    plan_ms = [c["planned_ms"] - c["plan_start_ms"] for c in done if c["arrival_ms"] >= t_lo]
     span = (max(last_done, t_hi) - t_lo) / 1e3
     rep = {
         "workload": "64 streams (26x 1280x720 8-bit, 25x 1920x1080 10-bit with film grain, 13x 3840x2160 10-bit "
                     "with superres 3/4), full-tool syntax statistics, Poisson arrivals of 1200 frames/s in all; "
                     f"each stream cycles through a {SERVE_GOP}-frame group (a key frame and 15 inter frames, 3 "
-                    "references), syntax planned into HBM beforehand; NV12 packed with every frame",
+                    f"references); each frame submitted as syntax (wire bytes) at its arrival and planned by "
+                    f"{args.serve_planners} planner threads inside its latency; NV12 packed with every frame",
         "seconds": dur, "frames": len(allv), "completed": len(done), "submitted": len(events),
         "offered_mpixel_s": round(sum(px[streams[si][1]] for tt, si, k in events if tt >= warm) / dur / 1e6, 1),
         "value": round(pix / span / 1e6, 1), "unit": "Mpixel/s",
@@ -516,6 +543,7 @@ def run_serve(args):
                                     "frames": len(lat_inter)},
         "batches": st["batches"], "mean_batch_frames": round(st["frames"] / max(st["batches"], 1), 2),
         "device_busy_frac": round(st["busy_ms"] / (max(c["done_ms"] for c in done) - t0), 3),
+        "planning_ms": {"p50": round(pct(plan_ms, 50), 3), "p99": round(pct(plan_ms, 99), 3)} if plan_ms else None,
         "bit_exact": {"frames_checked": len(done), "mismatched_frames": mism,
                       "method": "device checksum of every completed picture vs the same frame decoded alone"},
     }
@@ -537,6 +565,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline and the parity sample")
     ap.add_argument("--no-mix", action="store_true", help="skip the 16-stream batched vs plain mix")
     ap.add_argument("--mix-only", action="store_true", help="only the 16-stream mix")
+    ap.add_argument("--mix-batched-only", action="store_true",
+                    help="the mix's batched path only (no plain pass, no comparison): for profiling")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
     ap.add_argument("--no-costmodel", action="store_true", help="skip the learned schedule on the mix")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes of the mix")
@@ -549,6 +579,7 @@ def main():
     ap.add_argument("--serve-launchers", type=int, default=2)
     ap.add_argument("--serve-heavy-launchers", type=int, default=1)
     ap.add_argument("--serve-heavy-batch", type=int, default=1)
+    ap.add_argument("--serve-planners", type=int, default=8, help="planner threads of the serving loop")
     ap.add_argument("--serve-heavy", type=int, default=-1,
                     help="dependency levels above which a frame runs on the key-frame launcher (-1: automatic, 0: off)")
     args = ap.parse_args()

@@ -603,6 +603,18 @@ int av1h_plan_sb_times(av1h_ctx* ctx, const void* plan, float* us);
 /* Orders the plan's queue by `sb_cost_us` (one per superblock; null restores
  * the static schedule). */
 int av1h_plan_schedule(av1h_ctx* ctx, void* plan, const float* sb_cost_us);
+/* The queue in another order (no counterpart in the reference, whose
+ * scheduler hands blocks out by wave): AV1H_ORDER_LEVEL by dependency level
+ * (with costs: the longest remaining path first inside a level; without:
+ * the static order), AV1H_ORDER_CODING in coding order, superblock row after
+ * row (the static row-order schedule), AV1H_ORDER_PATH the longest remaining
+ * path first, paths summed from `sb_cost_us` (per superblock; null: each
+ * block costs its sample count). A batch merges its frames' queues by key,
+ * the frames' last keys aligned: the level, the remaining path in quarter
+ * units, or runs of 64 entries in coding order. Pixels do not depend on the
+ * order. */
+enum { AV1H_ORDER_LEVEL = 0, AV1H_ORDER_CODING = 1, AV1H_ORDER_PATH = 2 };
+int av1h_plan_schedule_order(av1h_ctx* ctx, void* plan, int order, const float* sb_cost_us);
 
 /* The cost model (a 24-64-64-1 FP32 perceptron trained and run by HIP
  * kernels on the device). */
@@ -726,11 +738,12 @@ typedef struct av1h_completion {
   uint16_t* picture;        /* capture: Y, U, V display planes back to back     */
   int64_t picture_samples;  /* free with av1h_server_free_picture               */
   uint64_t hash;            /* checksum of the picture (hash_pictures)          */
+  double plan_start_ms, planned_ms; /* planning of a wire frame (else arrival)  */
 } av1h_completion;
 
 typedef struct av1h_server_stats {
   int64_t frames, batches, admitted, waiting, active, budget_used;
-  double busy_ms;           /* wall time inside batch decodes                   */
+  double busy_ms;           /* wall time with at least one batch decoding       */
   int32_t max_batch_frames;
 } av1h_server_stats;
 
@@ -746,6 +759,11 @@ int av1h_server_open_stream(av1h_server* server, int width, int height, int bit_
 int av1h_server_submit_wire(av1h_server* server, int stream, const uint8_t* wire, size_t len,
                             int reserved_slot, const int32_t* release, int nrelease, double arrival_ms,
                             uint64_t tag);
+/* The same with the wire bytes borrowed, not copied: they must stay valid
+ * until the frame's completion is polled. */
+int av1h_server_submit_wire_ref(av1h_server* server, int stream, const uint8_t* wire, size_t len,
+                                int reserved_slot, const int32_t* release, int nrelease, double arrival_ms,
+                                uint64_t tag);
 /* A plan from av1h_plan_frame on any context of the server's device; the
  * caller keeps it and may submit it again (plans are read only). */
 int av1h_server_submit_plan(av1h_server* server, int stream, void* plan, int reserved_slot,

@@ -75,8 +75,9 @@ def _decode(dec, clip, plans, keep, rel):
 @pytest.mark.parametrize("workload,seed,over", [("tools_mini", 1, {}), ("c1_mini", 2, dict(key_interval=1, frames=3)),
                                                 ("contract_mini", 1, {})])
 def test_schedules_leave_pixels_unchanged(workload, seed, over):
-    """Static, random-cost and model-cost schedules decode identically
-    (intra block copy edges included)."""
+    """Static, random-cost and model-cost schedules, coding order and
+    critical path order decode identically (intra block copy edges
+    included)."""
     clip = pkg.Clip(workload, seed, **over)
     keep, rel, refs = pkg.slot_schedule(clip)
     dec = pkg.HipDecoder()
@@ -84,12 +85,19 @@ def test_schedules_leave_pixels_unchanged(workload, seed, over):
     base = _decode(dec, clip, plans, keep, rel)
     rng = np.random.default_rng(seed)
     cm = pkg.CostModel(seed=seed)
-    for mode in ("random", "model"):
+    for mode in ("random", "model", "coding", "path", "path_random"):
         for p in plans:
+            nsb = dec.sb_features(p).shape[0]
             if mode == "random":
-                dec.schedule(p, rng.uniform(0, 50, dec.sb_features(p).shape[0]))
-            else:
+                dec.schedule(p, rng.uniform(0, 50, nsb))
+            elif mode == "model":
                 dec.schedule_model(p, cm)
+            elif mode == "coding":
+                dec.schedule_order(p, pkg.ORDER_CODING)
+            elif mode == "path":
+                dec.schedule_order(p, pkg.ORDER_PATH)
+            else:
+                dec.schedule_order(p, pkg.ORDER_PATH, rng.uniform(0, 50, nsb))
         got = _decode(dec, clip, plans, keep, rel)
         for f in range(clip.frames):
             for k in range(3):
@@ -113,3 +121,40 @@ def test_superblock_times_recorded():
     # superblocks with queued samples have recorded work
     assert (t[feats[:, 14] > 0] > 0).all()
     dec.discard_plan(plan)
+
+
+def test_orders_in_a_batch_leave_pixels_unchanged():
+    """Frames of one batch in different queue orders (merged by remaining
+    path instead of by level) decode as they do alone."""
+    clips = [pkg.Clip("c1_mini", 21, key_interval=1, frames=2), pkg.Clip("tools_mini", 2),
+             pkg.Clip("c2_mini", 5, frames=3)]
+    decs = [pkg.HipDecoder() for _ in clips]
+    for d in decs:
+        d.set_output_hash(True)
+    scheds = [pkg.slot_schedule(c) for c in clips]
+    plans = [[decs[i].plan_wire(pkg.wire_set_slots(c.wire(f), scheds[i][2][f])) for f in range(c.frames)]
+             for i, c in enumerate(clips)]
+    from av1ongpu_amd import costmodel as cmod
+    got = {}
+    for name, order in (("level", None), ("coding", pkg.ORDER_CODING), ("path", pkg.ORDER_PATH)):
+        for i, d in enumerate(decs):
+            for p in plans[i]:
+                if order is None:
+                    d.schedule(p, None)
+                else:
+                    d.schedule_order(p, order)
+        h = []
+        n = max(c.frames for c in clips)
+        for f in range(n):
+            live = [i for i, c in enumerate(clips) if f < c.frames]
+            pkg.decode_planned_batch([decs[i] for i in live], [plans[i][f] for i in live],
+                                     [scheds[i][0][f] >= 0 for i in live], [scheds[i][0][f] for i in live])
+            h.append([decs[i].last_hash() for i in live])
+            for i in live:
+                for s in scheds[i][1][f]:
+                    decs[i].release(s)
+        got[name] = h
+    assert got["coding"] == got["level"] and got["path"] == got["level"]
+    for i, d in enumerate(decs):
+        for p in plans[i]:
+            d.discard_plan(p)
