@@ -74,6 +74,8 @@ def syn_lib():
         lib.av1s_clip_pixels.restype = ctypes.c_int64
         lib.av1s_clip_pixels.argtypes = [ctypes.c_void_p, ctypes.c_int]
         lib.av1s_wire_roundtrip.argtypes = [ctypes.c_char_p, ctypes.c_int64]
+        lib.av1s_wire_without_levels.restype = ctypes.c_int64
+        lib.av1s_wire_without_levels.argtypes = [ctypes.c_char_p, ctypes.c_int64, ctypes.c_char_p, ctypes.c_int64]
         lib.av1s_fnv1a64.restype = ctypes.c_uint64
         lib.av1s_fnv1a64.argtypes = [ctypes.c_char_p, ctypes.c_int64]
         _syn = lib
@@ -137,6 +139,17 @@ def wire_header(wire):
     """(width, coded_width, height, bit_depth, ss_x, ss_y) of a wire frame."""
     v = np.frombuffer(wire, dtype=np.int32, count=6, offset=_HDR_OFF)
     return tuple(int(x) for x in v)
+
+
+def wire_without_levels(wire):
+    """The wire frame with its coded levels and quantiser matrices dropped."""
+    lib = syn_lib()
+    n = lib.av1s_wire_without_levels(wire, len(wire), None, 0)
+    buf = ctypes.create_string_buffer(max(n, 1))
+    m = lib.av1s_wire_without_levels(wire, len(wire), buf, n)
+    if m < 0:
+        raise ValueError("not a wire frame")
+    return buf.raw[:m]
 
 
 def wire_set_slots(wire, slots):

@@ -23,6 +23,7 @@
 
 #define AV1T_TABLE_QUAL static const
 #include "av1hip_internal.hpp"
+#include "scan_order.hpp"
 #include "spec_tables.h"
 
 using namespace av1hip;
@@ -483,11 +484,12 @@ struct Host;
 enum {
   kPartBlocks, kPartTx, kPartInter, kPartQueue, kPartSpecial, kPartObmc, kPartCoef, kPartResidual, kPartPalette,
   kPartDeblock, kPartCdef, kPartLr, kPartMcItems, kPartMcTasks, kPartItx, kPartItxOffs, kPartDbChunk, kPartCdefDir,
-  kPartLrTiles, kPartGrain, kParts
+  kPartLrTiles, kPartGrain, kPartLevels, kPartQm, kPartDq, kParts
 };
 void sb_features(const Host& H, const av1h_frame_header& hdr, const Layout& L, Plan* plan);
 
 struct Host {
+  std::vector<DevDq> dq; /* a frame with coded levels: one per transform */
   std::vector<DevBlock> blocks;
   std::vector<DevBlock> dev_blocks; /* the ones the device reads */
   std::vector<DevTx> tx;
@@ -733,6 +735,11 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
       return set_err(err, AV1H_ERR_UNSUPPORTED, "superres beyond 2:1");
   }
   const int nb = (int)f.num_blocks;
+  /* coded levels: the device dequantises them; the raster pool is not read */
+  const bool levels = f.num_levels > 0;
+  if (levels && !f.levels) return set_err(err, AV1H_ERR_INVALID, "levels without an array");
+  H->dq.clear();
+  if (levels) H->dq.reserve((size_t)f.num_transforms);
   H->blocks.clear();
   H->tx.clear();
   H->inter_list.clear();
@@ -978,8 +985,30 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
       const size_t end = (size_t)tx.coef_offset + (size_t)tx.w * tx.h;
       if (b.residual_provided)
         rend = std::max(rend, end);
-      else
+      else if (!levels)
         cend = std::max(cend, end);
+      if (levels) {
+        DevDq q{0, -1, 0, 0, 0};
+        if (!b.residual_provided) {
+          const int64_t area = (int64_t)std::min(tx.w, 32) * std::min(tx.h, 32);
+          const int64_t count = std::min<int64_t>(tx.eob, area);
+          const int den = tx.denominator;
+          if (tx.level_offset < 0 || tx.level_offset + count > f.num_levels)
+            return set_err(err, AV1H_ERR_INVALID, "block %d: transform %d levels out of range", i, t);
+          if (tx.dc_step < 1 || tx.dc_step > 65535 || tx.ac_step < 1 || tx.ac_step > 65535 ||
+              (den != 1 && den != 2 && den != 4))
+            return set_err(err, AV1H_ERR_INVALID, "block %d: transform %d dequantiser", i, t);
+          if (av1scan::tx_class(tx.tx_type) != 0 && (tx.w > 16 || tx.h > 16))
+            return set_err(err, AV1H_ERR_UNSUPPORTED,
+                           "block %d: transform type %d on a %dx%d transform with coded levels: AV1's transform "
+                           "sets have the 1D types only up to 16x16", i, tx.tx_type, tx.w, tx.h);
+          if (tx.qm_off >= 0 && (!f.qm_pool || tx.qm_off + area > f.num_qm))
+            return set_err(err, AV1H_ERR_INVALID, "block %d: transform %d quantiser matrix out of range", i, t);
+          q = DevDq{tx.level_offset, tx.qm_off >= 0 ? tx.qm_off : -1, (uint16_t)tx.dc_step, (uint16_t)tx.ac_step,
+                    den == 1 ? 0 : (den == 2 ? 1 : 2)};
+        }
+        H->dq.push_back(q);
+      }
       H->tx.push_back(dt);
     }
     d.tx_count = (int)H->tx.size() - d.tx_first;
@@ -1491,7 +1520,13 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
   add(kPartCdefDir, H->cdef_dir_list.data(), H->cdef_dir_list.size() * 4);
   add(kPartLrTiles, H->lr_tiles.data(), H->lr_tiles.size() * 4);
   add(kPartGrain, &H->grain, sizeof(DevGrain));
-  add(kPartCoef, f.coefficients, (size_t)f.num_coefficients * 4, (coef_n - (size_t)f.num_coefficients) * 4);
+  add(kPartLevels, levels ? f.levels : nullptr, levels ? (size_t)f.num_levels * 2 : 0);
+  add(kPartQm, levels ? f.qm_pool : nullptr, levels ? (size_t)f.num_qm : 0);
+  add(kPartDq, H->dq.data(), H->dq.size() * sizeof(DevDq));
+  if (levels)
+    add(kPartCoef, nullptr, 0);
+  else
+    add(kPartCoef, f.coefficients, (size_t)f.num_coefficients * 4, (coef_n - (size_t)f.num_coefficients) * 4);
   size_t total = 0;
   for (Host::Part& pt : H->parts) {
     pt.off = total;
@@ -1560,6 +1595,10 @@ void bind_dev(Plan* plan, const Host& H) {
   P.inter_list = reinterpret_cast<const int32_t*>(at(kPartInter));
   P.queue_list = reinterpret_cast<const int32_t*>(at(kPartQueue));
   P.coef = at(kPartCoef);
+  const bool lv = H.dq.size() > 0;
+  P.levels = lv ? reinterpret_cast<const int16_t*>(at(kPartLevels)) : nullptr;
+  P.qm = lv ? at(kPartQm) : nullptr;
+  P.dq = lv ? reinterpret_cast<const DevDq*>(at(kPartDq)) : nullptr;
   P.residual = reinterpret_cast<const int32_t*>(at(kPartResidual));
   P.palette = at(kPartPalette);
   P.deblock = reinterpret_cast<const DevDeblock*>(at(kPartDeblock));

@@ -110,6 +110,16 @@ struct DevTx {
   int32_t flags;        /* kTx*: lossless (WHT), raw residual, intra block  */
 };
 
+/* the dequantiser of a transform whose frame carries coded levels (one per
+ * DevTx, indexed alike): where its levels start, its quantiser matrix (-1:
+ * none), its DC and AC steps and its denominator as a shift */
+struct DevDq {
+  int32_t level_off, qm_off;
+  uint16_t dc_step, ac_step;
+  int32_t denom_shift;
+};
+
+
 /* transform classes: the larger side, 4 8 16 32 64 */
 constexpr int kTxClasses = 5;
 /* Motion compensation tasks: at most kMcSamples samples and kMcItems items
@@ -189,6 +199,12 @@ struct FramePlanDev {
    * descriptors of queued and special blocks only */
   const void* coef;
   int32_t coef16;
+  /* a frame with coded levels: the transforms dequantise while they stage
+   * their coefficients (levels, matrices, one DevDq per transform); coef is
+   * then unused. Null otherwise. */
+  const int16_t* levels;
+  const uint8_t* qm;
+  const DevDq* dq;
   const int32_t* residual;
   const uint8_t* palette;
   const DevDeblock* deblock;  /* vertical edges first, then horizontal */

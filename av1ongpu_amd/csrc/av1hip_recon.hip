@@ -25,8 +25,12 @@
 #define AV1T_TABLE_QUAL static __constant__ const
 #include "av1hip_internal.hpp"
 #include "av1hip_itx.hpp"
+#include "scan_order.hpp"
 
 namespace av1hip {
+/* the diagonal scans (scan_order.hpp), in device memory */
+__device__ const av1scan::DiagTables kScanTab = av1scan::make_diag();
+
 namespace {
 
 #define SPEC(off) (av1t_spec_table + (off))
@@ -231,7 +235,36 @@ __device__ __forceinline__ void itx_group(RArgs a, const int32_t* list, int n, i
     const int cs = w < 32 ? w : 32, cr = h < 32 ? h : 32;
     const int lcs = ilog2(cs), st = w + 1, n = cs * cr;
     int v[kIt];
-    if (a.plan.coef16) {
+    if (a.plan.levels) {
+      /* dequantised here from the coded levels: raster position i holds
+       * scan index si, coded when below the transform's count */
+      const DevDq q = a.plan.dq[list[idx]];
+      const int cls = av1scan::tx_class(tx.tx_type), lcw = lcs - 2, lch = ilog2(cr) - 2;
+      const int count = tx.eob < n ? tx.eob : n;
+      const int16_t* lv = a.plan.levels + q.level_off;
+      int si[kIt];
+#pragma unroll
+      for (int k = 0; k < kIt; ++k) {
+        const int i = l + k * G;
+        si[k] = i < n ? av1scan::scan_index(kScanTab.iscan, cls, lcw, lch, i) : n;
+      }
+#pragma unroll
+      for (int k = 0; k < kIt; ++k) v[k] = si[k] < count ? (int)lv[si[k]] : 0;
+      if (q.qm_off >= 0) {
+        const uint8_t* qm = a.plan.qm + q.qm_off;
+#pragma unroll
+        for (int k = 0; k < kIt; ++k) {
+          const int i = l + k * G;
+          if (v[k]) v[k] = av1scan::dequant(v[k], i ? q.ac_step : q.dc_step, qm[i], q.denom_shift, a.lay.bit_depth);
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < kIt; ++k) {
+          const int i = l + k * G;
+          if (v[k]) v[k] = av1scan::dequant(v[k], i ? q.ac_step : q.dc_step, -1, q.denom_shift, a.lay.bit_depth);
+        }
+      }
+    } else if (a.plan.coef16) {
       const int16_t* src = static_cast<const int16_t*>(a.plan.coef) + tx.coef;
 #pragma unroll
       for (int k = 0; k < kIt; ++k) v[k] = l + k * G < n ? (int)src[l + k * G] : 0;

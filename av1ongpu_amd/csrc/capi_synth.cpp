@@ -62,6 +62,7 @@ int av1s_config_set(void* h, const char* key, double v) {
   else if (k == "avail_auto") c->avail_auto = v;
   else if (k == "residual_frac") c->residual_frac = v;
   else if (k == "lr_log2") c->lr_log2 = i;
+  else if (k == "levels") c->levels = v != 0;
   else if (k == "seed") c->seed = static_cast<uint64_t>(v);
   else return -1;
   return 0;
@@ -117,6 +118,23 @@ int64_t av1s_clip_wire(void* h, int f, const uint8_t** out) {
 int64_t av1s_clip_pixels(const void* h, int f) {
   const av1gpu::FrameSyntax& fs = static_cast<const SynClip*>(h)->clip.frames[static_cast<size_t>(f)];
   return static_cast<int64_t>(fs.width) * fs.height;
+}
+
+/* A wire frame with its coded levels and quantiser matrices dropped (the
+ * coefficient raster left as the only residual): the form a decoder that
+ * does not dequantise on the device reads. Returns the byte size written,
+ * or the size needed when `out` is null or short, negative on a malformed
+ * frame. */
+int64_t av1s_wire_without_levels(const uint8_t* wire, int64_t len, uint8_t* out, int64_t cap) {
+  av1h_frame_desc d;
+  if (av1h_wire_parse(wire, static_cast<size_t>(len), &d) != AV1H_OK) return -1;
+  d.num_levels = 0;
+  d.levels = nullptr;
+  d.num_qm = 0;
+  d.qm_pool = nullptr;
+  const size_t n = av1h_wire_size(&d);
+  if (!out || cap < static_cast<int64_t>(n)) return static_cast<int64_t>(n);
+  return static_cast<int64_t>(av1h_wire_write(&d, out));
 }
 
 /* Round trip check of the wire format: wire -> FrameSyntax -> wire must give

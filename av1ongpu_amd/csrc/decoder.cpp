@@ -120,6 +120,11 @@ class HipDecoder final : public Decoder {
    * grain, so a pipeline may decode with read_back false and pack the slot */
   bool packs_nv12_on_device() const override { return true; }
 
+  /* a frame whose syntax carries coded levels is dequantised on the device
+   * (inside the transforms' coefficient staging): the raster pool may be
+   * left empty */
+  bool dequantises_on_device() const override { return true; }
+
   int nv12_pack_begin(int slot, size_t pitch, std::string* error) override {
     const int rc = av1h_nv12_pack_begin(ctx_, slot, pitch);
     if (rc < 0) return fail_rc(error, rc), -1;

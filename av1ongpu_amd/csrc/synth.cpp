@@ -5,6 +5,7 @@
  * pure function of (config, seed) on every platform.
  */
 #include "synth.hpp"
+#include "scan_order.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -14,6 +15,9 @@
 namespace av1gpu {
 namespace synth {
 namespace {
+
+/* the diagonal scans (scan_order.hpp) */
+constexpr av1scan::DiagTables kDiag = av1scan::make_diag();
 
 struct Rng {
   uint64_t s;
@@ -172,6 +176,8 @@ class FrameBuilder {
   int nrefs_;
   FrameSyntax fs_;
   int coded_w_ = 0, grid_w_ = 0, grid_h_ = 0;
+  bool steps_ready_ = false; /* levels: the frame's steps and matrices are drawn */
+  int dc_step_ = 0, ac_step_ = 0;
   PlaneState planes_[3];
 
   void partition(int x, int y, int size) {
@@ -216,7 +222,12 @@ class FrameBuilder {
     int cand[16];
     int n = 0;
     for (int t = 0; t < 16; ++t)
-      if (kind_fits(row_kind(t), tw) && kind_fits(col_kind(t), th)) cand[n++] = t;
+      if (kind_fits(row_kind(t), tw) && kind_fits(col_kind(t), th)) {
+        /* with coded levels, the types AV1's transform sets allow: the 1D
+         * types (V_*, H_*, scanned row or column wise) only up to 16x16 */
+        if (cfg_.levels && t >= 10 && (tw > 16 || th > 16)) continue;
+        cand[n++] = t;
+      }
     return cand[rng_.range(0, n - 1)];
   }
 
@@ -265,6 +276,7 @@ class FrameBuilder {
     }
     const int cw = std::min(32, tw), ch = std::min(32, th);
     t.coef_offset = static_cast<int>(fs_.coefficients.size());
+    if (cfg_.levels) return levels_tx(t, tw, th, cw, ch, lossless);
     int nnz = 0;
     for (int i = 0; i < cw * ch; ++i) {
       int v = 0;
@@ -276,6 +288,54 @@ class FrameBuilder {
       fs_.coefficients.push_back(v);
     }
     t.eob = nnz;
+    return t;
+  }
+
+  /* The coded levels of a transform, in scan order, and the dequantiser a
+   * frontend would derive for it (the frame's steps, the transform's
+   * denominator, a quantiser matrix for some); the coefficient raster is
+   * their dequantisation, so the frame carries both, as a parsed stream
+   * does. */
+  TransformBlock levels_tx(TransformBlock t, int tw, int th, int cw, int ch, bool lossless) {
+    if (!steps_ready_) {
+      /* the frame's quantiser and matrices, from a stream of their own */
+      Rng q(cfg_.seed * 0x9E3779B97F4A7C15ull + static_cast<uint64_t>(fs_.base_q_idx) * 131u + 7u);
+      const int sh = cfg_.bit_depth - 8;
+      dc_step_ = q.range(24, 80) << sh;
+      ac_step_ = q.range(32, 96) << sh;
+      for (int m = 0; m < 4; ++m)
+        for (int i = 0; i < 1024; ++i) fs_.qm_pool.push_back(static_cast<uint8_t>(q.range(20, 64)));
+      steps_ready_ = true;
+    }
+    const int area = cw * ch, lw = __builtin_ctz(static_cast<unsigned>(cw)) - 2,
+              lh = __builtin_ctz(static_cast<unsigned>(ch)) - 2;
+    const int cls = av1scan::tx_class(static_cast<int>(t.tx_type));
+    t.dc_step = lossless ? 4 : dc_step_;
+    t.ac_step = lossless ? 4 : ac_step_;
+    t.denominator = 1 << ((tw * th > 256) + (tw * th > 1024));
+    t.qm_off = !lossless && rng_.chance(0.2) ? 1024 * rng_.range(0, 3) : -1;
+    std::vector<int> lv(static_cast<size_t>(area), 0);
+    int eob = 0;
+    for (int i = 0; i < area; ++i) {
+      if (rng_.chance(cfg_.coef_zero)) continue;
+      const int step = i == 0 ? t.dc_step : t.ac_step;
+      long v = std::lround(static_cast<double>(coef_value()) / step);
+      if (v == 0) v = rng_.chance(0.5) ? -1 : 1;
+      const long cap = lossless ? 500 : 32767;
+      lv[static_cast<size_t>(i)] = static_cast<int>(std::max(-cap, std::min(cap, v)));
+      eob = i + 1;
+    }
+    t.eob = eob;
+    t.level_offset = static_cast<int>(fs_.levels.size());
+    for (int i = 0; i < eob; ++i) fs_.levels.push_back(static_cast<int16_t>(lv[static_cast<size_t>(i)]));
+    fs_.coefficients.resize(fs_.coefficients.size() + static_cast<size_t>(area), 0);
+    int32_t* raster = fs_.coefficients.data() + t.coef_offset;
+    for (int i = 0; i < eob; ++i) {
+      const int pos = av1scan::scan_pos(kDiag.scan, cls, lw, lh, i);
+      const int qm = t.qm_off >= 0 ? fs_.qm_pool[static_cast<size_t>(t.qm_off + pos)] : -1;
+      raster[pos] = av1scan::dequant(lv[static_cast<size_t>(i)], pos == 0 ? t.dc_step : t.ac_step, qm,
+                                     __builtin_ctz(static_cast<unsigned>(t.denominator)), cfg_.bit_depth);
+    }
     return t;
   }
 

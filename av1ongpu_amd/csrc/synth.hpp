@@ -66,6 +66,9 @@ struct Config {
                                  (residual_provided) instead of coefficients       */
   int lr_log2 = 0;           /* luma restoration unit log2 (0: 64); chroma one less,
                                 at least 32                                       */
+  bool levels = false;       /* coded levels in scan order with their dequantiser
+                                (steps, denominator, quantiser matrices); the raster
+                                is their dequantisation                            */
   uint64_t seed = 1;
 };
 

@@ -78,6 +78,14 @@ int av1o_setup_shear(const int32_t* params, int* alpha, int* beta, int* gamma, i
 void av1o_color_matrix(int matrix, int range, int bit_depth, int m[8], int o[3]);
 /* the filter table the predictions index: subpel, then warp, then upscale */
 const int* av1o_spec_table_ptr(int* size);
+/* The coefficient raster of every coded transform of a frame that carries
+ * coded levels, dequantised on the host (into `raster`, num_coefficients
+ * entries, zeros elsewhere); returns the transforms dequantised, or -1 when
+ * the frame has no levels. */
+int64_t av1o_dequant_frame(const av1h_frame_desc* frame, int32_t* raster);
+/* The same for a wire frame; `cap` entries of room (at least the frame's
+ * num_coefficients, else -2). */
+int64_t av1o_dequant_wire(const uint8_t* wire, size_t len, int32_t* raster, int64_t cap);
 /* FNV-1a 64 over bytes (the reference's --hash-check, tools/av1gpu_decode.cpp:133-139) */
 uint64_t av1o_fnv1a64(const uint8_t* p, size_t n);
 

@@ -1024,3 +1024,78 @@ void av1o_itx_add(const int32_t* coef, int w, int h, int tx_type, int lossless, 
   for (int y = 0; y < h; ++y) o_itx_row(&j, coef, tmp, y);
   for (int x = 0; x < w; ++x) o_itx_col(&j, tmp, dst, x, (1 << bd) - 1);
 }
+
+/* ---- dequantisation of coded levels ---------------------------------------
+ * Restates the reference's av1g_dequant_one (kernels/av1gpu_kernels.h:
+ * 1024-1083): the raster of a transform is min(32, w) x min(32, h), its
+ * levels come in scan order (a diagonal scan for 2D types, row by row for
+ * V_* types, column by column for H_*: av1gpu_kernels.h:1036-1056 with the
+ * scans of src/frontend/coef_tables.hpp), each level times its step (the DC
+ * step at position 0, weighted by the quantiser matrix entry when there is
+ * one), the magnitude cut to 24 bits, divided by the denominator, clipped to
+ * 7 + bit depth bits (1065-1081). */
+
+/* raster position of scan index i of a w x h raster (sides 4..32) of class
+ * cls (0 2D, 1 horizontal, 2 vertical) */
+static int scan_position(int cls, int w, int h, int i) {
+  if (cls == 2) return i;
+  if (cls == 1) return (i % h) * w + i / h;
+  /* the diagonal scan: anti-diagonals in order; on a square raster the
+   * even ones run from the left column up to the top row, the odd ones back;
+   * on a wider raster every one runs upwards, on a taller one downwards */
+  int n = 0;
+  for (int d = 0; d < w + h - 1; ++d) {
+    const int r0 = d - (w - 1) > 0 ? d - (w - 1) : 0, r1 = d < h - 1 ? d : h - 1;
+    const int len = r1 - r0 + 1;
+    if (i < n + len) {
+      const int k = i - n;
+      const int up = w == h ? (d % 2 == 0) : (w > h);
+      const int r = up ? r1 - k : r0 + k;
+      return r * w + (d - r);
+    }
+    n += len;
+  }
+  return -1;
+}
+
+int64_t av1o_dequant_frame(const av1h_frame_desc* f, int32_t* raster) {
+  if (f->num_levels <= 0) return -1;
+  for (int64_t i = 0; i < f->num_coefficients; ++i) raster[i] = 0;
+  int64_t n = 0;
+  for (int64_t bi = 0; bi < f->num_blocks; ++bi) {
+    const av1h_block* b = &f->blocks[bi];
+    if (b->residual_provided) continue;
+    for (int k = 0; k < b->tx_count; ++k) {
+      const av1h_transform* t = &f->transforms[b->tx_first + k];
+      if (t->eob <= 0) continue;
+      const int w = t->w < 32 ? t->w : 32, h = t->h < 32 ? t->h : 32, area = w * h;
+      const int type = t->tx_type;
+      const int cls = (type == 10 || type == 12 || type == 14) ? 2 : (type == 11 || type == 13 || type == 15) ? 1 : 0;
+      const int count = t->eob < area ? t->eob : area;
+      for (int i = 0; i < count; ++i) {
+        const int level = f->levels[t->level_offset + i];
+        if (!level) continue;
+        const int pos = scan_position(cls, w, h, i);
+        int step = pos == 0 ? t->dc_step : t->ac_step;
+        if (t->qm_off >= 0) step = (step * f->qm_pool[t->qm_off + pos] + 16) >> 5;
+        const long long product = (long long)level * step;
+        long long mag = product < 0 ? -product : product;
+        mag &= 0xFFFFFF;
+        const long long scaled = mag / t->denominator;
+        long long v = product < 0 ? -scaled : scaled;
+        const long long lim = 1LL << (7 + b->bit_depth);
+        if (v > lim - 1) v = lim - 1;
+        if (v < -lim) v = -lim;
+        raster[t->coef_offset + pos] = (int32_t)v;
+      }
+      ++n;
+    }
+  }
+  return n;
+}
+
+int64_t av1o_dequant_wire(const uint8_t* wire, size_t len, int32_t* raster, int64_t cap) {
+  av1h_frame_desc d;
+  if (av1h_wire_parse(wire, len, &d) != AV1H_OK || cap < d.num_coefficients) return -2;
+  return av1o_dequant_frame(&d, raster);
+}

@@ -20,6 +20,7 @@
 
 #include "av1hip.h"
 #include "core/backend.hpp"
+#include "frontend/coef_tables.hpp"
 
 using namespace av1gpu;
 using namespace av1gpu::internal;
@@ -818,6 +819,51 @@ int64_t av1r_slot_rgb(void* h, int slot, int matrix, int range, uint8_t* out, in
     return -1;
   if (out && cap >= static_cast<int64_t>(v.size())) std::memcpy(out, v.data(), v.size());
   return static_cast<int64_t>(v.size());
+}
+
+/* The reference's own dequantisation (av1g_dequant_one, kernels/
+ * av1gpu_kernels.h:1024-1083, with the scan tables of src/frontend/
+ * coef_tables.hpp) of every coded transform of a wire frame that carries
+ * levels, compared with the frame's coefficient raster: returns the count of
+ * raster entries that differ (0: the raster is the levels' dequantisation),
+ * -1 when the frame has no levels, -2 for a malformed frame; *coded gets the
+ * number of transforms dequantised. */
+int64_t av1r_dequant_check(const uint8_t* wire, size_t len, int64_t* coded) {
+  av1h_frame_desc d;
+  if (av1h_wire_parse(wire, len, &d) != AV1H_OK) return -2;
+  if (d.num_levels <= 0) return -1;
+  std::vector<int> pool(static_cast<size_t>(d.num_coefficients), 0);
+  int64_t n = 0;
+  for (int64_t bi = 0; bi < d.num_blocks; ++bi) {
+    const av1h_block& b = d.blocks[bi];
+    if (b.residual_provided) continue;
+    for (int k = 0; k < b.tx_count; ++k) {
+      const av1h_transform& t = d.transforms[b.tx_first + k];
+      if (t.eob <= 0) continue;
+      Av1gItxJob job;
+      std::memset(&job, 0, sizeof(job));
+      job.coef_off = t.coef_offset;
+      job.w = t.w;
+      job.h = t.h;
+      job.tx_type = t.tx_type;
+      job.eob = t.eob;
+      job.bitdepth = b.bit_depth;
+      job.level_offset = t.level_offset;
+      job.plane = b.plane;
+      job.dc_step = t.dc_step;
+      job.ac_step = t.ac_step;
+      job.denominator = t.denominator;
+      job.qm_off = t.qm_off;
+      av1g_dequant_one(&job, d.levels, d.num_qm > 0 ? d.qm_pool : nullptr, av1gpu::frontend::coef_tables::kDefaultScan,
+                       av1gpu::frontend::coef_tables::kMrowScan, av1gpu::frontend::coef_tables::kMcolScan,
+                       pool.data());
+      ++n;
+    }
+  }
+  if (coded) *coded = n;
+  int64_t diff = 0;
+  for (size_t i = 0; i < pool.size(); ++i) diff += pool[i] != d.coefficients[i];
+  return diff;
 }
 
 }  // extern "C"

@@ -6,14 +6,16 @@
  *   api_check host
  *       host-side contracts: backend names, Frame geometry, frame_to_nv12 /
  *       frame_to_yuv16 / frame_to_yuv8 bytes, Decoder::create refusals;
- *   api_check decode <workload> <seed> <nv12 pitch> [planned|streamed] [key=value ...]
+ *   api_check decode <workload> <seed> <nv12 pitch> [planned|streamed] [nocoef] [key=value ...]
  *       decodes a synthetic clip on the GPU and prints one line per frame:
  *       "<frame> <slot> <yuv16 fnv> <nv12 fnv> <rgb fnv>"; with `planned`
  *       every other frame goes through plan_frame + decode_reference_planned;
  *       with `streamed` every frame is planned, decoded into a reserved slot
  *       with read_back false (the backend packs on the device) and packed
  *       through nv12_pack_begin / wait / release, the way the reference's
- *       pipelined caller does (dsfilter/decode_core.hpp), yuv16 printed as "-".
+ *       pipelined caller does (dsfilter/decode_core.hpp), yuv16 printed as "-";
+ *       with `nocoef` every frame's coefficient raster is dropped, so a clip
+ *       with coded levels decodes from its levels only.
  */
 #include <cstdio>
 #include <cstdlib>
@@ -94,12 +96,16 @@ static int decode(int argc, char** argv) {
   const std::string wl = argv[2];
   const uint64_t seed = std::strtoull(argv[3], nullptr, 10);
   const size_t pitch = std::strtoull(argv[4], nullptr, 10);
-  bool planned = false, streamed = false;
+  bool planned = false, streamed = false, nocoef = false;
   synth::Config cfg = synth::workload(wl, seed);
   for (int i = 5; i < argc; ++i) {
     std::string a = argv[i];
     if (a == "planned" || a == "streamed") {
       (a == "planned" ? planned : streamed) = true;
+      continue;
+    }
+    if (a == "nocoef") {
+      nocoef = true;
       continue;
     }
     const size_t eq = a.find('=');
@@ -112,6 +118,9 @@ static int decode(int argc, char** argv) {
     else if (k == "deblock_lo") cfg.deblock_lo = (int)v;
     else if (k == "deblock_hi") cfg.deblock_hi = (int)v;
     else if (k == "grain") cfg.grain = v != 0;
+    else if (k == "levels") cfg.levels = v != 0;
+    else if (k == "residual_frac") cfg.residual_frac = v;
+    else if (k == "avail_auto") cfg.avail_auto = v;
     else {
       std::fprintf(stderr, "unknown override %s\n", k.c_str());
       return 2;
@@ -129,6 +138,14 @@ static int decode(int argc, char** argv) {
   std::map<int, int> slot_of;
   for (size_t f = 0; f < clip.frames.size(); ++f) {
     FrameSyntax s = clip.frames[f];
+    if (nocoef) {
+      /* a backend that dequantises on the device needs no raster pool */
+      if (!dec->dequantises_on_device() || s.levels.empty()) {
+        std::fprintf(stderr, "no device dequantisation or no levels\n");
+        return 4;
+      }
+      s.coefficients.clear();
+    }
     s.reference_slots.clear();
     for (int r : clip.refs[f]) s.reference_slots.push_back(slot_of[r]);
     Frame out;

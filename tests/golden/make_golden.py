@@ -36,7 +36,7 @@ sys.path.insert(0, REPO)
 sys.path.insert(0, os.path.join(REPO, "tests"))
 
 import av1ongpu_amd as pkg  # noqa: E402
-from oracle_lib import RefDecoder, ref_available  # noqa: E402
+from oracle_lib import RefDecoder, ref_available, ref_dequant_check  # noqa: E402
 from golden_cases import slot_view_nv12, slot_view_rgb  # noqa: E402
 
 CASES = [
@@ -70,6 +70,14 @@ CASES = [
     ("c1_mini", 56, {"width": 96, "height": 72, "frames": 2, "lr": 2}),
     ("c1_mini", 57, {"width": 32, "height": 24, "frames": 2}),
     ("c1_mini", 58, {"width": 16, "height": 16, "frames": 2, "lr": 2}),
+    # coded levels with their dequantisers (FrameSyntax::levels / qm_pool,
+    # av1gpu.hpp:509-517): the coefficient raster the frames also carry is
+    # checked against the reference's own dequantisation of the levels
+    # (av1gpu_kernels.h:1024-1083) when the fixture is written
+    ("c1_mini", 71, {"levels": 1}),
+    ("c2_mini", 72, {"levels": 1}),
+    ("contract_mini", 73, {"levels": 1}),
+    ("tools_mini", 74, {"levels": 1, "residual_frac": 0.2}),
 ]
 
 MATRICES = [(m, r) for m in range(4) for r in range(2)]
@@ -98,6 +106,12 @@ def run_case(workload, seed, overrides, serial):
                "rgb": f"{pkg.fnv1a64(dec.last_rgb(1, 0).tobytes()):016x}",
                "rgb_all": [f"{pkg.fnv1a64(dec.last_rgb(m, r).tobytes()):016x}" for m, r in MATRICES],
                "waves": dec.waves(), "slot": slot}
+        if overrides.get("levels"):
+            diff, coded = ref_dequant_check(wire)
+            if diff != 0:
+                sys.exit(f"{workload} seed {seed} frame {f}: the raster differs from the reference's "
+                         f"dequantisation of the levels in {diff} entries")
+            rec["dequantised"] = coded
         held = sorted(slot_of.values())
         pitch = nv12_pitch(w)
         rec["slots"] = [[s, f"{pkg.fnv1a64(slot_view_nv12(dec.slot_nv12(s, pitch), h, cw, pitch)):016x}",

@@ -27,6 +27,8 @@ def oracle_lib():
         P = ctypes.c_void_p
         lib.av1o_create.restype = P
         lib.av1o_destroy.argtypes = [P]
+        lib.av1o_dequant_wire.restype = ctypes.c_int64
+        lib.av1o_dequant_wire.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_int64]
         lib.av1o_last_error.restype = ctypes.c_char_p
         lib.av1o_last_error.argtypes = [P]
         lib.av1o_decode_wire.argtypes = [P, ctypes.c_char_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int,
@@ -82,8 +84,41 @@ def ref_lib():
         lib.av1r_slot_nv12.argtypes = [P, ctypes.c_int, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_int64]
         lib.av1r_slot_rgb.restype = ctypes.c_int64
         lib.av1r_slot_rgb.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int64]
+        lib.av1r_dequant_check.restype = ctypes.c_int64
+        lib.av1r_dequant_check.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_int64)]
         _ref = lib
     return _ref
+
+
+def oracle_dequant(wire):
+    """The oracle's dequantisation of a wire frame's levels: (raster int32,
+    transforms dequantised), or (None, -1) without levels."""
+    hsize = int(np.frombuffer(wire, dtype=np.uint32, count=1, offset=12)[0])
+    pos = (16 + hsize + 7) & ~7
+    ncoef = int(np.frombuffer(wire, dtype=np.int64, count=13, offset=pos)[3])
+    out = np.zeros(max(ncoef, 1), np.int32)
+    n = oracle_lib().av1o_dequant_wire(wire, len(wire), out.ctypes.data, out.size)
+    return (out[:ncoef], int(n)) if n >= 0 else (None, int(n))
+
+
+def wire_coefficients(wire):
+    """The coefficient raster a wire frame carries."""
+    hsize = int(np.frombuffer(wire, dtype=np.uint32, count=1, offset=12)[0])
+    pos = (16 + hsize + 7) & ~7
+    counts = np.frombuffer(wire, dtype=np.int64, count=13, offset=pos)
+    pos += 13 * 8
+    sizes = [69 * 4, 13 * 4, 10 * 4]
+    for i in range(3):
+        pos += (int(counts[i]) * sizes[i] + 7) & ~7
+    return np.frombuffer(wire, dtype=np.int32, count=int(counts[3]), offset=pos)
+
+
+def ref_dequant_check(wire):
+    """(raster entries differing from the reference's dequantisation of the
+    frame's levels, transforms dequantised); (-1, 0) without levels."""
+    n = ctypes.c_int64(0)
+    d = ref_lib().av1r_dequant_check(wire, len(wire), ctypes.byref(n))
+    return int(d), int(n.value)
 
 
 class OracleDecoder:

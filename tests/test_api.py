@@ -61,12 +61,14 @@ def test_wire_roundtrip():
 
 GPU_CASES = [c for c in load_cases() if (c["workload"], c["seed"]) in
              {("c1_mini", 1), ("c2_mini", 1), ("contract_mini", 2), ("c1_mini", 11)}]
+LEVEL_CASES = [c for c in load_cases() if c["overrides"].get("levels")]
 
 
-def _run_decode(case, mode):
+def _run_decode(case, mode, extra=()):
     args = [API_CHECK, "decode", case["workload"], str(case["seed"]), str(case["nv12_pitch"])]
     if mode != "direct":
         args.append(mode)
+    args += list(extra)
     args += [f"{k}={v}" for k, v in case["overrides"].items()]
     r = subprocess.run(args, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
@@ -85,6 +87,14 @@ def _run_decode(case, mode):
 @pytest.mark.parametrize("mode", ["direct", "planned", "streamed"])
 def test_cpp_api_decodes_golden(case, mode):
     _run_decode(case, mode)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", LEVEL_CASES, ids=[case_id(c) for c in LEVEL_CASES])
+def test_cpp_api_decodes_levels_without_a_raster(case):
+    """dequantises_on_device(): a caller may leave the raster pool empty when
+    the syntax carries coded levels (reference engine.cpp:375-389)."""
+    _run_decode(case, "direct", ("nocoef",))
 
 
 @pytest.mark.gpu
