@@ -560,6 +560,8 @@ def main():
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--planners", type=int, default=0, help="planner threads of the ingest pipeline (0: automatic)")
+    ap.add_argument("--levels", type=int, default=0,
+                    help="1: the clip's syntax carries coded levels (dequantised on the device)")
     ap.add_argument("--depth", type=int, default=24, help="frames in flight in the ingest pipeline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline and the parity sample")
@@ -602,7 +604,7 @@ def main():
     dec = pkg.HipDecoder(0)
     log(f"device: {dec.device_info().name.decode(errors='replace')}")
     t = time.perf_counter()
-    clip = pkg.Clip(args.workload, args.seed)
+    clip = pkg.Clip(args.workload, args.seed, **({"levels": 1} if args.levels else {}))
     n = clip.frames
     keep_slot, releases, ref_slots = slot_schedule(clip)
     wires = [pkg.wire_set_slots(clip.wire(f), ref_slots[f]) for f in range(n)]
