@@ -263,6 +263,7 @@ def hip_lib():
                                                         ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         lib.av1h_batch_finish.argtypes = [P]
         lib.av1h_batch_pending.argtypes = [P]
+        lib.av1h_plan_needs_idle.argtypes = [P, P]
         _hip = lib
     return _hip
 
@@ -336,7 +337,8 @@ class PipeFrame(ctypes.Structure):
     _fields_ = [("slot", ctypes.c_int32), ("status", ctypes.c_int32), ("submit_ms", ctypes.c_double),
                 ("plan_start_ms", ctypes.c_double), ("planned_ms", ctypes.c_double),
                 ("decode_start_ms", ctypes.c_double), ("done_ms", ctypes.c_double),
-                ("flatten_ms", ctypes.c_double), ("upload_ms", ctypes.c_double), ("hash", ctypes.c_uint64)]
+                ("flatten_ms", ctypes.c_double), ("upload_ms", ctypes.c_double), ("hash", ctypes.c_uint64),
+                ("launch_ms", ctypes.c_double), ("wait_ms", ctypes.c_double), ("device_ms", ctypes.c_double)]
 
 
 def _bind_pipe(lib):
@@ -560,6 +562,11 @@ class HipDecoder:
             raise HipError(f"decode failed ({rc}): {self.error()}")
         return slot.value, arrs
 
+    def plan_needs_idle(self, plan):
+        """True when launching `plan` would wait for every batch in flight on
+        this decoder (buffers that grow, stage timing, the queue trace)."""
+        return self._lib.av1h_plan_needs_idle(self._h, plan) == 1
+
     def plan_wire(self, wire):
         """Flatten and upload a wire frame into a device-resident plan."""
         p = ctypes.c_void_p()
@@ -776,6 +783,11 @@ def batch_finish(launcher):
     rc = lib.av1h_batch_finish(launcher.handle)
     if rc != AV1H_OK:
         raise HipError(f"batch failed ({rc}): {launcher.error()}")
+
+
+def batch_pending(launcher):
+    """Batches launched on `launcher` and not yet finished."""
+    return int(hip_lib().av1h_batch_pending(launcher.handle))
 
 
 def decode_clips_batched(decoders, clips, frames=None, read_back=True):

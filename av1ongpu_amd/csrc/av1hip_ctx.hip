@@ -154,7 +154,6 @@ constexpr int kPackRing = 3;
 
 /* a pinned staging buffer and the stream its uploads run on */
 struct Stager {
-  hipStream_t stream = nullptr;
   hipEvent_t done = nullptr; /* its last copy has ended: the buffer is free */
   void* pin = nullptr;
   size_t cap = 0;
@@ -244,7 +243,11 @@ struct Ctx {
   int64_t stage_n[AV1H_STAGES] = {};
   int64_t stage_bytes[AV1H_STAGES] = {};
   PackEntry ring[kPackRing];     /* streamed NV12 packs                   */
-  /* pre-planning (any thread): stagers and device buffers for reuse */
+  /* pre-planning (any thread): stagers and device buffers for reuse; every
+   * plan's upload goes on the one upload stream. A stream per stager would
+   * share hardware queues with the decode stream (a process gets few), and
+   * a decode queued behind an upload's wait in a shared queue stalls. */
+  hipStream_t upload = nullptr;
   std::mutex plan_mu;
   std::vector<Stager*> stagers;
   std::vector<std::pair<size_t, void*>> plan_bufs;
@@ -1720,7 +1723,6 @@ int upload_plan(Ctx* c, Plan* plan, const Host& H, std::string* err) {
   if (!plan->ready) e = hipEventCreateWithFlags(&plan->ready, hipEventDisableTiming);
   if (!st) {
     st = new Stager;
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&st->done, hipEventDisableTiming);
   } else if (st->done) {
     e = hipEventSynchronize(st->done);
@@ -1735,15 +1737,15 @@ int upload_plan(Ctx* c, Plan* plan, const Host& H, std::string* err) {
   }
   if (e == hipSuccess) {
     const size_t up = stage_parts(H, static_cast<uint8_t*>(st->pin), &plan->dev.coef16);
-    e = hipMemcpyAsync(plan->dbuf, st->pin, up, hipMemcpyHostToDevice, st->stream);
-    if (e == hipSuccess) e = hipEventRecord(plan->ready, st->stream);
-    if (e == hipSuccess) e = hipEventRecord(st->done, st->stream);
+    e = hipMemcpyAsync(plan->dbuf, st->pin, up, hipMemcpyHostToDevice, c->upload);
+    if (e == hipSuccess) e = hipEventRecord(plan->ready, c->upload);
+    if (e == hipSuccess) e = hipEventRecord(st->done, c->upload);
   }
-  if (st->stream && st->done) {
+  if (st->done) {
     std::lock_guard<std::mutex> lk(c->plan_mu);
     c->stagers.push_back(st);
   } else {
-    if (st->stream) (void)hipStreamDestroy(st->stream);
+    if (st->pin) (void)hipHostFree(st->pin);
     delete st;
   }
   if (e != hipSuccess) {
@@ -1910,7 +1912,7 @@ int kernel_groups(const FrameJob& fj, int k, int cdef_batch) {
     case kKItxIntraL: return itx_groups(p, 1, 3, kTxClasses);
     case kKDeblockV: return p.n_deblock_v > 0 ? (p.lines_v + 255) / 256 : 0;
     case kKDeblockH: return p.n_deblock_h > 0 ? (p.lines_h + 255) / 256 : 0;
-    case kKCdefDir: return p.n_cdef > 0 ? (p.n_cdef_dir + 4 * kCdefDirPerWave - 1) / (4 * kCdefDirPerWave) : 0;
+    case kKCdefDir: return p.n_cdef > 0 ? (p.n_cdef_dir + kCdefDirPerGroup - 1) / kCdefDirPerGroup : 0;
     case kKCdefLuma: return p.n_cdef > 0 ? (p.n_cdef_luma + 4 * cdef_batch - 1) / (4 * cdef_batch) : 0;
     case kKCdefChroma: {
       const int nc = p.n_cdef - p.n_cdef_luma;
@@ -2511,6 +2513,7 @@ int av1h_create(int device_index, av1h_ctx** out) {
       return AV1H_ERR_DEVICE;
     }
   if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->upload, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_side_go, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_side_done, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_itx_done, hipEventDisableTiming) != hipSuccess) {
@@ -2555,7 +2558,6 @@ void av1h_destroy(av1h_ctx* h) {
   for (Stager* st : c->stagers) {
     if (st->done) (void)hipEventSynchronize(st->done);
     if (st->done) (void)hipEventDestroy(st->done);
-    if (st->stream) (void)hipStreamDestroy(st->stream);
     if (st->pin) (void)hipHostFree(st->pin);
     delete st;
   }
@@ -2569,6 +2571,7 @@ void av1h_destroy(av1h_ctx* h) {
   if (c->ev_side_done) (void)hipEventDestroy(c->ev_side_done);
   if (c->ev_itx_done) (void)hipEventDestroy(c->ev_itx_done);
   if (c->side) (void)hipStreamDestroy(c->side);
+  if (c->upload) (void)hipStreamDestroy(c->upload);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete h;
 }
@@ -3352,6 +3355,14 @@ int av1h_batch_pending(av1h_ctx* launcher) {
   if (!launcher) return AV1H_ERR_INVALID;
   std::lock_guard<std::mutex> lk(launcher->c.mu);
   return (int)launcher->c.pending.size();
+}
+
+int av1h_plan_needs_idle(av1h_ctx* ctx, void* plan) {
+  if (!ctx || !plan) return AV1H_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(ctx->c.mu);
+  /* stage timing and the queue trace take the waiting path */
+  const Ctx* c = &ctx->c;
+  return c->timing || c->trace_on || must_grow(c, static_cast<const Plan*>(plan)) ? 1 : 0;
 }
 
 }  // extern "C"

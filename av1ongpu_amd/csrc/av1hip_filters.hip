@@ -193,132 +193,141 @@ __constant__ const int kCdefDirs[8][2][2] = {{{-1, 1}, {-2, 2}}, {{0, 1}, {-1, 2
 __constant__ const int kCdefUvDir[2][2][8] = {{{0, 1, 2, 3, 4, 5, 6, 7}, {1, 2, 2, 2, 3, 4, 6, 0}},
                                               {{7, 0, 2, 4, 5, 6, 6, 6}, {0, 1, 2, 3, 4, 5, 6, 7}}};
 
-/* Direction search of luma 8x8 blocks (av1gpu_kernels.h:2601-2660), two
- * blocks per wave (kCdefDirPerWave), one lane per sample of each: the eight
- * directional line sums are accumulated with LDS atomics, then lanes 0-7
- * (first block) and 8-15 (second) weigh one direction each. Writes
- * (direction, variance) for each block into `dirmap`. */
-__device__ void cdef_dir_group(FArgs a, int grp, int (*part)[8][16]) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int i = lane >> 3, k = lane & 7;
-  const int bd = a.lay.bit_depth;
-  int bx[kCdefDirPerWave], by[kCdefDirPerWave];
-  bool live[kCdefDirPerWave];
-  for (int u = 0; u < kCdefDirPerWave; ++u)
-    for (int e = lane; e < 128; e += 64) (&part[u][0][0])[e] = 0;
-  __syncthreads();
-  int x[kCdefDirPerWave];
+/* Direction search of luma 8x8 blocks (av1gpu_kernels.h:2601-2660), one
+ * block per lane: a lane reads its block's eight rows (one 16-byte load
+ * each: rows start on 8-sample boundaries of 128-sample aligned strides),
+ * keeps the eight directions' line sums in registers (every index a
+ * compile-time constant), weighs them and picks the first direction of
+ * largest cost. No LDS and no cross-lane traffic: consecutive lanes take
+ * consecutive listed blocks, so a wave's row loads cover whole 128-byte
+ * spans of a block row. Writes (direction, variance) for each block into
+ * `dirmap`. */
+__device__ __forceinline__ void cdef_dir_block(FArgs a, int bi) {
+  const int pos = a.plan.cdef_dir_list[bi];
+  const int bx = pos & 0xffff, by = pos >> 16;
+  const int sh = a.lay.bit_depth - 8;
+  const uint16_t* src = a.cdef_src + a.lay.plane_off[0] + (size_t)(by * 8) * a.lay.stride[0] + bx * 8;
+  int p0[15] = {}, p1[11] = {}, p2[8] = {}, p3[11] = {}, p4[15] = {}, p5[11] = {}, p6[8] = {}, p7[11] = {};
 #pragma unroll
-  for (int u = 0; u < kCdefDirPerWave; ++u) {
-    const int bi = (grp * 4 + wave) * kCdefDirPerWave + u;
-    live[u] = bi < a.plan.n_cdef_dir;
-    bx[u] = by[u] = 0;
-    x[u] = 0;
-    if (live[u]) {
-      const int pos = a.plan.cdef_dir_list[bi];
-      bx[u] = pos & 0xffff;
-      by[u] = pos >> 16;
-      x[u] = (a.cdef_src[a.lay.plane_off[0] + (size_t)(by[u] * 8 + i) * a.lay.stride[0] + bx[u] * 8 + k] >> (bd - 8)) -
-             128;
+  for (int i = 0; i < 8; ++i) {
+    const uint4 r = *reinterpret_cast<const uint4*>(src + (size_t)i * a.lay.stride[0]);
+    const uint32_t w[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int x = (int)(((w[k >> 1] >> (16 * (k & 1))) & 0xffffu) >> sh) - 128;
+      p0[i + k] += x;
+      p1[i + k / 2] += x;
+      p2[i] += x;
+      p3[3 + i - k / 2] += x;
+      p4[7 + i - k] += x;
+      p5[3 - i / 2 + k] += x;
+      p6[k] += x;
+      p7[i / 2 + k] += x;
     }
   }
+  /* 840 / n for a line of n samples */
+  constexpr int div[9] = {0, 840, 420, 280, 210, 168, 140, 120, 105};
+  auto full = [&](const int* q) {
+    int c = 0;
 #pragma unroll
-  for (int u = 0; u < kCdefDirPerWave; ++u) {
-    if (!live[u]) continue;
-    int* pu = &part[u][0][0];
-    atomicAdd(&pu[0 * 16 + i + k], x[u]);
-    atomicAdd(&pu[1 * 16 + i + k / 2], x[u]);
-    atomicAdd(&pu[2 * 16 + i], x[u]);
-    atomicAdd(&pu[3 * 16 + 3 + i - k / 2], x[u]);
-    atomicAdd(&pu[4 * 16 + 7 + i - k], x[u]);
-    atomicAdd(&pu[5 * 16 + 3 - i / 2 + k], x[u]);
-    atomicAdd(&pu[6 * 16 + k], x[u]);
-    atomicAdd(&pu[7 * 16 + i / 2 + k], x[u]);
-  }
-  __syncthreads();
-  /* lanes 8 u .. 8 u + 7 weigh the directions of block u */
-  const int u = lane >> 3;
-  const bool mine = u < kCdefDirPerWave && live[u < kCdefDirPerWave ? u : 0];
-  const int div[9] = {0, 840, 420, 280, 210, 168, 140, 120, 105};
-  int cost = 0;
-  if (mine) {
-    const int d = lane & 7;
-    const int* pd = &part[u][d][0];
-    if (d == 2 || d == 6) {
-      for (int t = 0; t < 8; ++t) cost += pd[t] * pd[t];
-      cost *= div[8];
-    } else if (d == 0 || d == 4) {
-      for (int t = 0; t < 7; ++t) cost += (pd[t] * pd[t] + pd[14 - t] * pd[14 - t]) * div[t + 1];
-      cost += pd[7] * pd[7] * div[8];
-    } else {
-      for (int t = 0; t < 5; ++t) cost += pd[3 + t] * pd[3 + t];
-      cost *= div[8];
-      for (int t = 0; t < 3; ++t) cost += (pd[t] * pd[t] + pd[10 - t] * pd[10 - t]) * div[2 * t + 2];
-    }
-  }
-  /* the first direction with the largest cost (a strict > scan from 0),
-   * within each group of 8 lanes */
-  int best = mine ? cost : INT_MIN, bdir = lane & 7;
+    for (int t = 0; t < 8; ++t) c += q[t] * q[t];
+    return c * div[8];
+  };
+  auto diag = [&](const int* q) {
+    int c = 0;
 #pragma unroll
-  for (int o = 1; o < 8; o <<= 1) {
-    const int oc = __shfl_xor(best, o), od = __shfl_xor(bdir, o);
-    if (oc > best || (oc == best && od < bdir)) {
-      best = oc;
-      bdir = od;
+    for (int t = 0; t < 7; ++t) c += (q[t] * q[t] + q[14 - t] * q[14 - t]) * div[t + 1];
+    return c + q[7] * q[7] * div[8];
+  };
+  auto half = [&](const int* q) {
+    int c = 0;
+#pragma unroll
+    for (int t = 0; t < 5; ++t) c += q[3 + t] * q[3 + t];
+    c *= div[8];
+#pragma unroll
+    for (int t = 0; t < 3; ++t) c += (q[t] * q[t] + q[10 - t] * q[10 - t]) * div[2 * t + 2];
+    return c;
+  };
+  const int cost[8] = {diag(p0), half(p1), full(p2), half(p3), diag(p4), half(p5), full(p6), half(p7)};
+  /* the first direction with the largest cost (a strict > scan from 0); a
+   * block whose costs are all zero keeps direction 0 */
+  int best = 0, dd = 0;
+#pragma unroll
+  for (int d = 0; d < 8; ++d)
+    if (cost[d] > best) {
+      best = cost[d];
+      dd = d;
     }
-  }
-  const int g0 = lane & ~7;
-  const int bcost = __shfl(best, g0);
-  /* a block whose costs are all zero keeps direction 0, as the scan does */
-  const int dd = bcost > 0 ? __shfl(bdir, g0) : 0;
-  const int opp = __shfl(cost, g0 + ((dd + 4) & 7));
-  if (mine && (lane & 7) == 0) {
-    const int var = ((bcost > 0 ? bcost : 0) - opp) >> 10;
-    int* m = a.cdef_dir + 2 * (by[u] * a.cdef_dir_w + bx[u]);
-    m[0] = dd;
-    m[1] = var;
-  }
+  int opp = cost[0];
+#pragma unroll
+  for (int d = 1; d < 8; ++d)
+    if (d == ((dd + 4) & 7)) opp = cost[d];
+  int* m = a.cdef_dir + 2 * (by * a.cdef_dir_w + bx);
+  m[0] = dd;
+  m[1] = (best - opp) >> 10;
 }
 
 __global__ __launch_bounds__(256) void k_cdef_dir(BatchDev B) {
-  __shared__ int s_part[4][kCdefDirPerWave][8][16];
   const AV1H_CMEM int32_t* pre = kernel_prefix(B, kKCdefDir);
   const int total = pre[B.njobs];
-  int(*part)[8][16] = s_part[threadIdx.x >> 6];
   for (int base = 0; base < total; base += gridDim.x) {
     const int v = xcd_group(base, total);
     if (v < 0) continue;
     const int j = find_job(pre, B.njobs, v);
-    cdef_dir_group(filter_args(B, j), v - pre[j], part);
-    __syncthreads();
+    FArgs a = filter_args(B, j);
+    const int bi = (v - pre[j]) * kCdefDirPerGroup + (int)threadIdx.x;
+    if (bi < a.plan.n_cdef_dir) cdef_dir_block(a, bi);
   }
 }
 
-/* CDEF of the listed blocks (av1gpu_kernels.h:2662-2775), one thread per
- * sample, reading the deblocked snapshot; blocks of `S` samples, 256 / S per
- * workgroup. Samples of unlisted blocks are already final in the frame. */
-/* One sample of block `bi`: false when it keeps the snapshot's value. */
-template <int S>
-__device__ __forceinline__ bool cdef_sample(FArgs a, int first, int bi, int lane, int64_t* off, int* val,
-                                            const int (*dirs)[2][2]) {
-  const DevCdef c = a.plan.cdef[first + bi];
+/* CDEF of the listed blocks (av1gpu_kernels.h:2662-2775). A wave takes
+ * 64 / (W x H) blocks of one plane kind (W x H samples each: 8x8 luma, the
+ * chroma size by subsampling) and first stages each block with a 2-sample
+ * border from the deblocked snapshot into LDS, one 32-bit load per sample
+ * pair (rows start on even samples of 128-sample aligned strides), marking
+ * samples outside the plane; then one lane per sample reads its twelve taps
+ * from LDS. Samples of unlisted blocks are already final in the frame. */
+constexpr uint16_t kCdefOutside = 0xffff; /* no sample reaches this value */
+constexpr int kCdefTileWave = 256;         /* LDS samples per wave         */
+
+/* the tap of direction table entry e = (dy, dx) as an offset in a tile of
+ * row pitch tw */
+__device__ __forceinline__ int cdef_tap(const int* e, int tw) { return e[0] * tw + e[1]; }
+
+/* One sample (r, k) of block `bi`, its tile at `t` (row pitch TW, the
+ * sample at t[(r + 2) * TW + k + 2]). False when it keeps the snapshot's
+ * value. A luma wave holds one block: its parameters, direction and tap
+ * offsets are then wave-uniform and live in scalar registers (the
+ * direction table read from constant memory); chroma waves hold several
+ * blocks and read the table from LDS. */
+template <int W, int H, bool kLuma>
+__device__ __forceinline__ bool cdef_tile_sample(FArgs a, int first, int bi, int r, int k, const uint16_t* t,
+                                                 const int (*sdirs)[2][2], int64_t* off, int* val) {
+  constexpr int TW = W + 4;
+  if (kLuma) bi = __builtin_amdgcn_readfirstlane(bi);
+  DevCdef c;
+  if (kLuma) {
+    const AV1H_CMEM DevCdef& cu = cmem(a.plan.cdef)[first + bi];
+    c = DevCdef{cu.plane, cu.x, cu.y, cu.pri, cu.sec, cu.damping, cu.dir, 0};
+  } else {
+    c = a.plan.cdef[first + bi];
+  }
+  const int (*dirs)[2][2] = kLuma ? kCdefDirs : sdirs;
   LayRef L = a.lay;
-  const int p = c.plane, uv = p > 0;
-  const int bd = L.bit_depth, cs = bd - 8;
-  const int st = L.stride[p];
-  const int lw = uv ? 3 - L.ss_x : 3, w = 1 << lw;
-  const int r = lane >> lw, k = lane & (w - 1);
+  const int p = kLuma ? 0 : c.plane;
+  const int cs = L.bit_depth - 8;
   const int pw = L.grid_w[p], ph = L.grid_h[p];
   if (c.x + k >= pw || c.y + r >= ph) return false;
   int pri = c.pri << cs;
   const int sec = c.sec << cs;
-  const int damping = c.damping + cs - uv;
+  const int damping = c.damping + cs - (kLuma ? 0 : 1);
   int dir = c.dir;
   if (dir < 0) {
-    const int lx = uv ? c.x << L.ss_x : c.x, ly = uv ? c.y << L.ss_y : c.y;
+    const int lx = kLuma ? c.x : c.x << L.ss_x, ly = kLuma ? c.y : c.y << L.ss_y;
     const int* m = a.cdef_dir + 2 * ((ly >> 3) * a.cdef_dir_w + (lx >> 3));
-    const int yd = m[0], var = m[1];
-    if (uv) {
+    const int yd = kLuma ? __builtin_amdgcn_readfirstlane(m[0]) : m[0];
+    const int var = kLuma ? __builtin_amdgcn_readfirstlane(m[1]) : m[1];
+    if (!kLuma) {
       dir = pri == 0 ? 0 : kCdefUvDir[L.ss_x][L.ss_y][yd];
     } else {
       const int vs = (var >> 6) ? min(flog2(var >> 6), 12) : 0;
@@ -327,178 +336,96 @@ __device__ __forceinline__ bool cdef_sample(FArgs a, int first, int bi, int lane
     }
   }
   if (pri == 0 && sec == 0) return false; /* the destination already holds the snapshot's value */
-  const uint16_t* src = a.cdef_src + L.plane_off[p] + (size_t)(c.y + r) * st + c.x + k;
-  const int ts = (pri >> cs) & 1;
-  const int pt0 = ts ? 3 : 4, pt1 = ts ? 3 : 2;
-  const int x = src[0];
-  int sum = 0, lo = x, hi = x;
-  if (c.x >= 2 && c.y >= 2 && c.x + w + 2 <= pw && c.y + (S >> lw) + 2 <= ph) {
-    /* every tap inside the plane */
-    const int adjp = pri ? max(0, damping - flog2(pri)) : 0, adjs = sec ? max(0, damping - flog2(sec)) : 0;
-    const int d1 = (dir + 2) & 7, d2 = (dir + 6) & 7;
-    const char* pb = reinterpret_cast<const char*>(a.cdef_src);
-    const unsigned o0 = 2u * (unsigned)(L.plane_off[p] + (int64_t)(c.y + r) * st + c.x + k);
-#pragma unroll
-    for (int t = 0; t < 2; ++t) {
-      const int op = dirs[dir][t][0] * st + dirs[dir][t][1];
-      const int o1 = dirs[d1][t][0] * st + dirs[d1][t][1];
-      const int o2 = dirs[d2][t][0] * st + dirs[d2][t][1];
-      const int wp = t ? pt1 : pt0, ws = t ? 1 : 2;
-#pragma unroll
-      for (int sg = -1; sg <= 1; sg += 2) {
-        const int vp = tap16(pb, o0 + 2u * (unsigned)(sg * op)), v1 = tap16(pb, o0 + 2u * (unsigned)(sg * o1)),
-                  v2 = tap16(pb, o0 + 2u * (unsigned)(sg * o2));
-        if (pri) sum += __mul24(wp, constrain_vx(vp, x, pri, adjp));
-        if (sec) sum += __mul24(ws, constrain_vx(v1, x, sec, adjs) + constrain_vx(v2, x, sec, adjs));
-        lo = min(lo, min(vp, min(v1, v2)));
-        hi = max(hi, max(vp, max(v1, v2)));
-      }
-    }
-    *off = L.plane_off[p] + (int64_t)(c.y + r) * st + c.x + k;
-    *val = clampi(clampi(x + ((8 + sum - (sum < 0)) >> 4), lo, hi), 0, a.maxv);
-    return true;
-  }
-#pragma unroll
-  for (int t = 0; t < 2; ++t)
-#pragma unroll
-    for (int sg = -1; sg <= 1; sg += 2) {
-      int dy = sg * dirs[dir][t][0], dx = sg * dirs[dir][t][1];
-      int px = c.x + k + dx, py = c.y + r + dy;
-      if (px >= 0 && px < pw && py >= 0 && py < ph) {
-        const int v = src[dy * st + dx];
-        sum += (t ? pt1 : pt0) * constrain(v - x, pri, damping);
-        lo = min(lo, v);
-        hi = max(hi, v);
-      }
-#pragma unroll
-      for (int o = -2; o <= 2; o += 4) {
-        const int sd = (dir + o) & 7;
-        dy = sg * dirs[sd][t][0];
-        dx = sg * dirs[sd][t][1];
-        px = c.x + k + dx;
-        py = c.y + r + dy;
-        if (px >= 0 && px < pw && py >= 0 && py < ph) {
-          const int v = src[dy * st + dx];
-          sum += (t ? 1 : 2) * constrain(v - x, sec, damping);
-          lo = min(lo, v);
-          hi = max(hi, v);
-        }
-      }
-    }
-  *off = L.plane_off[p] + (int64_t)(c.y + r) * st + c.x + k;
-  *val = clampi(clampi(x + ((8 + sum - (sum < 0)) >> 4), lo, hi), 0, a.maxv);
-  return true;
-}
-
-/* A luma sample: one 8x8 block per wave, so the block's parameters and tap
- * offsets are uniform (scalar registers), and blocks whose taps all stay
- * inside the plane (every block but the outer ring) skip the per-tap
- * bounds checks. Same arithmetic as cdef_sample. */
-__device__ __forceinline__ bool cdef_luma_sample(FArgs a, int bi, int lane, int64_t* off, int* val) {
-  bi = __builtin_amdgcn_readfirstlane(bi);
-  const AV1H_CMEM DevCdef& c = cmem(a.plan.cdef)[bi];
-  LayRef L = a.lay;
-  const int cs = L.bit_depth - 8, st = L.stride[0], pw = L.grid_w[0], ph = L.grid_h[0];
-  const int cx = c.x, cy = c.y;
-  const int r = lane >> 3, k = lane & 7;
-  int pri = c.pri << cs;
-  const int sec = c.sec << cs;
-  const int damping = c.damping + cs;
-  int dir = c.dir;
-  if (dir < 0) {
-    const int* m = a.cdef_dir + 2 * ((cy >> 3) * a.cdef_dir_w + (cx >> 3));
-    const int yd = __builtin_amdgcn_readfirstlane(m[0]), var = __builtin_amdgcn_readfirstlane(m[1]);
-    const int vs = (var >> 6) ? min(flog2(var >> 6), 12) : 0;
-    dir = pri == 0 ? 0 : yd;
-    pri = var ? ((pri * (4 + vs) + 8) >> 4) : 0;
-  }
-  if (pri == 0 && sec == 0) return false;
-  if (cx + k >= pw || cy + r >= ph) return false;
   const int adjp = pri ? max(0, damping - flog2(pri)) : 0, adjs = sec ? max(0, damping - flog2(sec)) : 0;
   const int ts = (pri >> cs) & 1;
   const int pt0 = ts ? 3 : 4, pt1 = ts ? 3 : 2;
   const int d1 = (dir + 2) & 7, d2 = (dir + 6) & 7;
-  const uint16_t* src = a.cdef_src + L.plane_off[0] + (size_t)(cy + r) * st + cx + k;
-  const int x = src[0];
+  const uint16_t* s0 = t + (r + 2) * TW + k + 2;
+  const int x = s0[0];
   int sum = 0, lo = x, hi = x;
-  if (cx >= 2 && cy >= 2 && cx + 10 <= pw && cy + 10 <= ph) {
-    const char* pb = reinterpret_cast<const char*>(a.cdef_src + L.plane_off[0]);
-    const unsigned o0 = 2u * (unsigned)((cy + r) * st + cx + k);
+  /* every tap inside the plane: no marks to test */
+  const bool inner = c.x >= 2 && c.y >= 2 && c.x + W + 2 <= pw && c.y + H + 2 <= ph;
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
-      const int op = kCdefDirs[dir][t][0] * st + kCdefDirs[dir][t][1];
-      const int o1 = kCdefDirs[d1][t][0] * st + kCdefDirs[d1][t][1];
-      const int o2 = kCdefDirs[d2][t][0] * st + kCdefDirs[d2][t][1];
-      const int wp = t ? pt1 : pt0, ws = t ? 1 : 2;
+  for (int tt = 0; tt < 2; ++tt) {
+    const int op = cdef_tap(dirs[dir][tt], TW), o1 = cdef_tap(dirs[d1][tt], TW), o2 = cdef_tap(dirs[d2][tt], TW);
+    const int wp = tt ? pt1 : pt0, ws = tt ? 1 : 2;
 #pragma unroll
-      for (int sg = -1; sg <= 1; sg += 2) {
-        const int vp = tap16(pb, o0 + 2u * (unsigned)(sg * op)), v1 = tap16(pb, o0 + 2u * (unsigned)(sg * o1)),
-                  v2 = tap16(pb, o0 + 2u * (unsigned)(sg * o2));
+    for (int sg = -1; sg <= 1; sg += 2) {
+      const int vp = s0[sg * op], v1 = s0[sg * o1], v2 = s0[sg * o2];
+      if (inner) {
         if (pri) sum += __mul24(wp, constrain_vx(vp, x, pri, adjp));
         if (sec) sum += __mul24(ws, constrain_vx(v1, x, sec, adjs) + constrain_vx(v2, x, sec, adjs));
         lo = min(lo, min(vp, min(v1, v2)));
         hi = max(hi, max(vp, max(v1, v2)));
+      } else {
+        if (vp != kCdefOutside) {
+          if (pri) sum += __mul24(wp, constrain_vx(vp, x, pri, adjp));
+          lo = min(lo, vp);
+          hi = max(hi, vp);
+        }
+        if (v1 != kCdefOutside) {
+          if (sec) sum += __mul24(ws, constrain_vx(v1, x, sec, adjs));
+          lo = min(lo, v1);
+          hi = max(hi, v1);
+        }
+        if (v2 != kCdefOutside) {
+          if (sec) sum += __mul24(ws, constrain_vx(v2, x, sec, adjs));
+          lo = min(lo, v2);
+          hi = max(hi, v2);
+        }
       }
     }
-  } else {
-#pragma unroll
-    for (int t = 0; t < 2; ++t)
-#pragma unroll
-      for (int sg = -1; sg <= 1; sg += 2)
-#pragma unroll
-        for (int u = 0; u < 3; ++u) {
-          const int dd = u == 0 ? dir : (u == 1 ? d1 : d2);
-          const int dy = sg * kCdefDirs[dd][t][0], dx = sg * kCdefDirs[dd][t][1];
-          const int px = cx + k + dx, py = cy + r + dy;
-          if (px >= 0 && px < pw && py >= 0 && py < ph) {
-            const int v = src[dy * st + dx];
-            if (u == 0)
-              sum += __mul24(t ? pt1 : pt0, constrain_vx(v, x, pri, adjp));
-            else if (sec)
-              sum += __mul24(t ? 1 : 2, constrain_vx(v, x, sec, adjs));
-            lo = min(lo, v);
-            hi = max(hi, v);
-          }
-        }
   }
-  *off = L.plane_off[0] + (int64_t)(cy + r) * st + cx + k;
+  *off = L.plane_off[p] + (int64_t)(c.y + r) * L.stride[p] + c.x + k;
   *val = clampi(clampi(x + ((8 + sum - (sum < 0)) >> 4), lo, hi), 0, a.maxv);
   return true;
 }
 
-/* kCdefBatch groups of 256 / S blocks per workgroup: a thread takes the same
- * sample of one block of each group, computes all of them (their reads
- * independent of each other) and then stores them. */
-
-template <int S, bool kLuma>
-__device__ __forceinline__ void cdef_group(FArgs a, int first, int count, int grp, const int (*dirs)[2][2],
-                                           int batch) {
-  constexpr int kPer = 256 / S;
-  const int lane = threadIdx.x % S;
-  int64_t off[kCdefMaxBatch];
-  int val[kCdefMaxBatch];
-  bool ok[kCdefMaxBatch];
-#pragma unroll
-  for (int u = 0; u < kCdefMaxBatch; ++u) {
-    const int bi = (grp * batch + u) * kPer + threadIdx.x / S;
-    if (kLuma)
-      ok[u] = u < batch && bi < count && cdef_luma_sample(a, bi, lane, &off[u], &val[u]);
-    else
-      ok[u] = u < batch && bi < count && cdef_sample<S>(a, first, bi, lane, &off[u], &val[u], dirs);
-  }
-#pragma unroll
-  for (int u = 0; u < kCdefMaxBatch; ++u)
-    if (ok[u]) {
-      a.cdef_dst[off[u]] = (uint16_t)val[u];
-      if (a.cdef_dst2) a.cdef_dst2[off[u]] = (uint16_t)val[u];
+/* `batch` rounds of 256 / (W x H) blocks per workgroup: each round stages
+ * the wave's blocks, then filters them. */
+template <int W, int H, bool kLuma>
+__device__ __forceinline__ void cdef_tiles(FArgs a, int first, int count, int grp, int batch, uint16_t* tile,
+                                           const int (*dirs)[2][2]) {
+  constexpr int S = W * H, TW = W + 4, TH = H + 4, kWave = 64 / S, kPer = 256 / S;
+  constexpr int kPairs = TW / 2, kDw = kPairs * TH; /* 32-bit words per block tile */
+  static_assert(kWave * TW * TH <= kCdefTileWave, "tiles of a wave exceed its LDS");
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  uint16_t* wt = tile + wave * kCdefTileWave;
+  LayRef L = a.lay;
+  for (int u = 0; u < batch; ++u) {
+    const int b0 = (grp * batch + u) * kPer + wave * kWave;
+    for (int e = lane; e < kWave * kDw; e += 64) {
+      const int sb = e / kDw, rem = e - sb * kDw, row = rem / kPairs, cp = rem - row * kPairs;
+      const int bi = b0 + sb;
+      uint32_t v = 0xffffffffu;
+      if (bi < count) {
+        const DevCdef& c = a.plan.cdef[first + bi];
+        const int p = kLuma ? 0 : c.plane;
+        const int px = c.x - 2 + 2 * cp, py = c.y - 2 + row;
+        /* grid widths are even: a pair is inside or outside as a whole */
+        if (px >= 0 && px < L.grid_w[p] && py >= 0 && py < L.grid_h[p])
+          v = *reinterpret_cast<const uint32_t*>(a.cdef_src + L.plane_off[p] + (int64_t)py * L.stride[p] + px);
+      }
+      *reinterpret_cast<uint32_t*>(wt + sb * TW * TH + row * TW + 2 * cp) = v;
     }
+    __syncthreads();
+    const int sb = lane / S, sm = lane - sb * S, r = sm / W, k = sm - r * W;
+    const int bi = b0 + sb;
+    int64_t off = 0;
+    int val = 0;
+    if (bi < count && cdef_tile_sample<W, H, kLuma>(a, first, bi, r, k, wt + sb * TW * TH, dirs, &off, &val)) {
+      a.cdef_dst[off] = (uint16_t)val;
+      if (a.cdef_dst2) a.cdef_dst2[off] = (uint16_t)val;
+    }
+    __syncthreads();
+  }
 }
 
-/* luma blocks (plane 0) or chroma blocks (plane 1) of every frame; a chroma
- * block has 16, 32 or 64 samples by the frame's subsampling */
+/* luma blocks (plane 0) or chroma blocks (planes 1 and 2) of every frame */
 __global__ __launch_bounds__(256) void k_cdef(BatchDev B, int chroma, int batch) {
   /* the direction table in LDS: lanes index it by their own direction */
   __shared__ int s_dirs[8][2][2];
+  __shared__ __align__(16) uint16_t s_tile[4 * kCdefTileWave];
   if (threadIdx.x < 32) (&s_dirs[0][0][0])[threadIdx.x] = (&kCdefDirs[0][0][0])[threadIdx.x];
   __syncthreads();
   const AV1H_CMEM int32_t* pre = kernel_prefix(B, (chroma ? kKCdefChroma : kKCdefLuma));
@@ -510,14 +437,15 @@ __global__ __launch_bounds__(256) void k_cdef(BatchDev B, int chroma, int batch)
     FArgs a = filter_args(B, j);
     const int nl = a.plan.n_cdef_luma, grp = v - pre[j];
     if (!chroma) {
-      cdef_group<64, true>(a, 0, nl, grp, s_dirs, batch);
+      cdef_tiles<8, 8, true>(a, 0, nl, grp, batch, s_tile, s_dirs);
       continue;
     }
     const int nc = a.plan.n_cdef - nl;
-    switch ((8 >> a.lay.ss_x) * (8 >> a.lay.ss_y)) {
-      case 16: cdef_group<16, false>(a, nl, nc, grp, s_dirs, batch); break;
-      case 32: cdef_group<32, false>(a, nl, nc, grp, s_dirs, batch); break;
-      default: cdef_group<64, false>(a, nl, nc, grp, s_dirs, batch); break;
+    switch (a.lay.ss_x * 2 + a.lay.ss_y) {
+      case 3: cdef_tiles<4, 4, false>(a, nl, nc, grp, batch, s_tile, s_dirs); break;
+      case 2: cdef_tiles<4, 8, false>(a, nl, nc, grp, batch, s_tile, s_dirs); break;
+      case 1: cdef_tiles<8, 4, false>(a, nl, nc, grp, batch, s_tile, s_dirs); break;
+      default: cdef_tiles<8, 8, false>(a, nl, nc, grp, batch, s_tile, s_dirs); break;
     }
   }
 }

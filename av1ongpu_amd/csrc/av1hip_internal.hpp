@@ -342,8 +342,8 @@ constexpr int kMaxBatch = 128;
 /* CDEF: groups of 256 / (block samples) blocks a workgroup takes per step
  * (a thread's samples of up to kCdefMaxBatch blocks computed together) */
 constexpr int kCdefMaxBatch = 4;
-/* CDEF direction search: luma 8x8 blocks per wave (4 waves per workgroup) */
-constexpr int kCdefDirPerWave = 4;
+/* CDEF direction search: luma 8x8 blocks per workgroup (one per lane) */
+constexpr int kCdefDirPerGroup = 256;
 
 /* samples one checksum workgroup reads */
 constexpr int kHashChunk = 4096;

@@ -1,9 +1,10 @@
 /*
  * Ingest pipeline for one stream: submitted frames are flattened and uploaded
  * by a pool of planner threads (av1h_plan_frame, which runs concurrently with
- * decodes and with itself), while one worker thread decodes them in
- * submission order (av1h_decode_planned), releases the slots each frame lets
- * go and recycles its plan. Frame N+1.. are planned while frame N is on the
+ * decodes and with itself), while one worker thread launches them in
+ * submission order (av1h_decode_planned_batch_async, two frames in flight),
+ * releases the slots each frame lets go, takes the results in order and
+ * recycles the plans. Frame N+1.. are planned while frame N is on the
  * device, so a stream's throughput is set by the slower of the two: the
  * planners together, or the device.
  *
@@ -89,54 +90,127 @@ void planner_main(av1h_pipe* p) {
   }
 }
 
+/* A frame the worker has launched (or failed) and not yet completed. */
+struct Flight {
+  PFrame* f = nullptr;
+  bool pending = false; /* its batch is in flight on the device */
+  int rc = AV1H_OK;
+  std::string msg;
+  int slot = -1;
+  uint64_t hash = 0;
+  double t_start = 0, flatten_ms = 0, upload_ms = 0, launch_ms = 0, wait_ms = 0, device_ms = 0;
+};
+
+/* the device time of the context's last finished batch */
+double last_device_ms(av1h_ctx* c) {
+  av1h_stats st;
+  if (av1h_get_stats(c, &st) != AV1H_OK) return 0;
+  return st.device_recon_ms + st.device_filter_ms;
+}
+
+/* Completes the oldest frame in flight: waits for its batch, recycles its
+ * plan and records its result. Called without the pipeline's lock. */
+void complete(av1h_pipe* p, std::deque<Flight>* flight) {
+  Flight x = std::move(flight->front());
+  flight->pop_front();
+  if (x.pending) {
+    const double t0 = av1h_server_now_ms();
+    x.rc = av1h_batch_finish(p->ctx);
+    x.wait_ms = av1h_server_now_ms() - t0;
+    if (x.rc != AV1H_OK) {
+      x.msg = av1h_last_error(p->ctx);
+    } else {
+      (void)av1h_last_hash(p->ctx, &x.hash);
+      x.device_ms = last_device_ms(p->ctx);
+    }
+  }
+  if (x.f->plan) av1h_discard_plan(p->ctx, x.f->plan);
+  const double t = av1h_server_now_ms();
+  std::lock_guard<std::mutex> lk(p->mu);
+  av1h_pipe_frame& r = p->results[(size_t)x.f->seq];
+  r.slot = x.rc == AV1H_OK ? x.slot : -1;
+  r.status = x.rc;
+  r.plan_start_ms = x.f->plan_start_ms;
+  r.planned_ms = x.f->planned_ms;
+  r.decode_start_ms = x.t_start;
+  r.done_ms = t;
+  r.flatten_ms = x.flatten_ms;
+  r.upload_ms = x.upload_ms;
+  r.hash = x.hash;
+  r.launch_ms = x.launch_ms;
+  r.wait_ms = x.wait_ms;
+  r.device_ms = x.device_ms;
+  if (x.rc != AV1H_OK) {
+    if (p->first_failed < 0) p->first_failed = x.f->seq;
+    if (p->err.empty()) p->err = x.msg;
+  }
+  p->frames.pop_front();
+  p->completed = x.f->seq + 1;
+  delete x.f;
+  p->cv_done.notify_all();
+  p->cv_space.notify_all();
+  p->cv_plan.notify_all();
+}
+
+/* Launches frame f as a batch of its own without waiting for it
+ * (av1h_decode_planned_batch_async), then lets go of the slots it names. A
+ * launch that would wait for the device anyway (buffers that grow, stage
+ * timing) first completes every frame in flight, so that results are taken
+ * in order. Called without the pipeline's lock. */
+void launch(av1h_pipe* p, PFrame* f, std::deque<Flight>* flight) {
+  Flight x;
+  x.f = f;
+  x.t_start = av1h_server_now_ms();
+  x.rc = f->plan_rc;
+  if (f->plan) (void)av1h_plan_host_times(f->plan, &x.flatten_ms, &x.upload_ms);
+  if (x.rc == AV1H_OK) {
+    av1h_ctx* c = p->ctx;
+    if (av1h_plan_needs_idle(c, f->plan) != 0)
+      while (!flight->empty()) complete(p, flight);
+    void* plan = f->plan;
+    int keep = f->keep, reserved = f->reserved;
+    const double t0 = av1h_server_now_ms();
+    x.rc = av1h_decode_planned_batch_async(c, &c, &plan, &keep, &reserved, 1, &x.slot);
+    x.launch_ms = av1h_server_now_ms() - t0;
+    if (x.rc != AV1H_OK) {
+      x.msg = av1h_last_error(c);
+    } else if (av1h_batch_pending(c) == 0) {
+      (void)av1h_last_hash(c, &x.hash); /* the launch waited for it */
+      x.device_ms = last_device_ms(c);
+    } else {
+      x.pending = true;
+    }
+  } else {
+    std::lock_guard<std::mutex> lk(p->mu);
+    x.msg = p->err;
+  }
+  for (int32_t s : f->release) av1h_release_reference(p->ctx, s);
+  flight->push_back(std::move(x));
+}
+
+/* Decodes the planned frames in submission order with up to two in flight
+ * on the device: frame N + 1 is queued behind frame N before frame N is
+ * waited for, so the device does not idle while the host takes a result and
+ * launches the next frame. */
 void worker_main(av1h_pipe* p) {
+  std::deque<Flight> flight;
   std::unique_lock<std::mutex> lk(p->mu);
   for (;;) {
-    if (p->frames.empty() || !p->frames.front()->planned) {
-      if (p->stop && p->frames.empty()) return;
-      p->cv_work.wait(lk);
+    PFrame* next = flight.size() < p->frames.size() ? p->frames[flight.size()] : nullptr;
+    if (next && next->planned && flight.size() < 2) {
+      lk.unlock();
+      launch(p, next, &flight);
+      lk.lock();
       continue;
     }
-    PFrame* f = p->frames.front();
-    lk.unlock();
-    const double t_start = av1h_server_now_ms();
-    double flatten_ms = 0, upload_ms = 0;
-    if (f->plan) (void)av1h_plan_host_times(f->plan, &flatten_ms, &upload_ms);
-    int slot = -1;
-    int rc = f->plan_rc;
-    std::string msg;
-    uint64_t hash = 0;
-    if (rc == AV1H_OK) {
-      rc = av1h_decode_planned(p->ctx, f->plan, f->keep, f->reserved, nullptr, nullptr, &slot);
-      if (rc != AV1H_OK)
-        msg = av1h_last_error(p->ctx);
-      else
-        (void)av1h_last_hash(p->ctx, &hash);
+    if (!flight.empty()) {
+      lk.unlock();
+      complete(p, &flight);
+      lk.lock();
+      continue;
     }
-    if (f->plan) av1h_discard_plan(p->ctx, f->plan);
-    for (int32_t s : f->release) av1h_release_reference(p->ctx, s);
-    const double t = av1h_server_now_ms();
-    lk.lock();
-    av1h_pipe_frame& r = p->results[(size_t)f->seq];
-    r.slot = slot;
-    r.status = rc;
-    r.plan_start_ms = f->plan_start_ms;
-    r.planned_ms = f->planned_ms;
-    r.decode_start_ms = t_start;
-    r.done_ms = t;
-    r.flatten_ms = flatten_ms;
-    r.upload_ms = upload_ms;
-    r.hash = hash;
-    if (rc != AV1H_OK) {
-      if (p->first_failed < 0) p->first_failed = f->seq;
-      if (p->err.empty()) p->err = msg;
-    }
-    p->frames.pop_front();
-    p->completed = f->seq + 1;
-    delete f;
-    p->cv_done.notify_all();
-    p->cv_space.notify_all();
-    p->cv_plan.notify_all();
+    if (p->stop && p->frames.empty()) return;
+    p->cv_work.wait(lk);
   }
 }
 

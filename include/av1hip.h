@@ -563,6 +563,13 @@ int av1h_decode_planned_batch_async(av1h_ctx* launcher, av1h_ctx* const* ctxs, v
 int av1h_batch_finish(av1h_ctx* launcher);
 /* Launched batches not yet finished. */
 int av1h_batch_pending(av1h_ctx* launcher);
+/* 1 when a launch of `plan` on ctx would wait for every batch in flight on
+ * it: a device buffer of ctx has to grow for the plan's picture (the first
+ * frame, a larger resolution, a wider NV12 pack), or stage timing or the
+ * queue trace is on (they take the waiting path); 0 when not. A caller that
+ * finishes its own batches in order (the ingest pipeline) finishes them
+ * before such a launch. */
+int av1h_plan_needs_idle(av1h_ctx* ctx, void* plan);
 
 /* ------------------------------------------------------ superblock costs */
 /*
@@ -685,6 +692,10 @@ typedef struct av1h_pipe_frame {
   double submit_ms, plan_start_ms, planned_ms, decode_start_ms, done_ms;
   double flatten_ms, upload_ms;
   uint64_t hash;
+  /* the worker's host time inside the frame's launch call and blocked
+   * waiting for its result; the device time of its launches (first to last
+   * event of its batch) */
+  double launch_ms, wait_ms, device_ms;
 } av1h_pipe_frame;
 int av1h_pipe_result(av1h_pipe* pipe, int64_t seq, av1h_pipe_frame* out);
 /* The first failure's message ("" when none). */
