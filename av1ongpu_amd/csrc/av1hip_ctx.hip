@@ -489,12 +489,16 @@ enum {
   kPartDeblock, kPartCdef, kPartLr, kPartMcItems, kPartMcTasks, kPartItx, kPartItxOffs, kPartDbChunk, kPartCdefDir,
   kPartLrTiles, kPartGrain, kPartLevels, kPartQm, kPartDq, kParts
 };
-void sb_features(const Host& H, const av1h_frame_header& hdr, const Layout& L, Plan* plan);
+void sb_begin(const Layout& L, Plan* plan, Host* H);
+void sb_block(const Layout& L, const DevBlock& d, const DevTx* tx, Plan* plan, Host* H);
+void sb_queued(const Layout& L, const DevBlock& d, int qi, Plan* plan, Host* H);
+void sb_finish(const Host& H, const av1h_frame_header& hdr, const Layout& L, Plan* plan);
 
 struct Host {
   std::vector<DevDq> dq; /* a frame with coded levels: one per transform */
   std::vector<DevBlock> blocks;
   std::vector<DevBlock> dev_blocks; /* the ones the device reads */
+  bool dev_all = false;             /* ... which are all of them: `blocks` goes */
   std::vector<DevTx> tx;
   std::vector<int32_t> inter_list, queue_list, special_list;
   std::vector<DevObmc> obmc;
@@ -521,6 +525,8 @@ struct Host {
   std::vector<int32_t> s_owner[3];
   std::vector<char> s_queued[3];
   std::vector<int32_t> s_dev_of;
+  std::vector<int32_t> s_sb_inter, s_sb_sub; /* inter blocks, those with subpel motion, per superblock */
+  std::vector<int32_t> s_sb_q;               /* superblock of each queued block              */
   std::vector<uint8_t> s_need;
 };
 
@@ -564,73 +570,82 @@ struct Owner {
 };
 
 /* The superblock features the cost model reads (their list is in the
- * header, av1h_plan_sb_features), the superblock of every queue entry, and
- * the superblocks intra block copies read. */
-void sb_features(const Host& H, const av1h_frame_header& hdr, const Layout& L, Plan* plan) {
+ * header, av1h_plan_sb_features) and the superblock of every queue entry:
+ * sb_begin before the blocks, sb_block for each block as it is flattened
+ * (its descriptor and transforms still in cache), sb_finish once the queue
+ * is known. */
+int sb_of(const Layout& L, int sw, int sh, int plane, int x, int y) {
+  const int sx = plane ? L.ss_x : 0, sy = plane ? L.ss_y : 0;
+  const int bx = std::min(std::max((x << sx) >> 6, 0), sw - 1), by = std::min(std::max((y << sy) >> 6, 0), sh - 1);
+  return by * sw + bx;
+}
+
+void sb_begin(const Layout& L, Plan* plan, Host* H) {
   const int sw = (L.grid_w[0] + 63) >> 6, sh = (L.grid_h[0] + 63) >> 6;
   plan->sb_w = sw;
   plan->sb_h = sh;
-  const int nsb = sw * sh;
-  std::vector<float>& F = plan->sb_feat;
-  F.assign((size_t)nsb * AV1H_CM_FEATURES, 0.f);
-  auto sb_at = [&](int plane, int x, int y) {
-    const int sx = plane ? L.ss_x : 0, sy = plane ? L.ss_y : 0;
-    const int bx = std::min(std::max((x << sx) >> 6, 0), sw - 1), by = std::min(std::max((y << sy) >> 6, 0), sh - 1);
-    return by * sw + bx;
-  };
-  std::vector<char> queued(H.blocks.size(), 0);
-  for (int32_t b : plan->queue_host) queued[(size_t)b] = 1;
-  std::vector<int> n_inter(nsb, 0), n_sub(nsb, 0);
-  for (size_t i = 0; i < H.blocks.size(); ++i) {
-    const DevBlock& d = H.blocks[i];
-    const int sb = sb_at(d.plane, d.x, d.y);
-    float* f = &F[(size_t)sb * AV1H_CM_FEATURES];
-    const float area = (float)d.w * d.h / 4096.f;
-    f[0] += 1.f;
-    for (int t = 0; t < d.tx_count; ++t) {
-      const DevTx& tx = H.tx[(size_t)(d.tx_first + t)];
-      const int side = std::max(tx.w, tx.h);
-      const int cls = side <= 4 ? 0 : side <= 8 ? 1 : side <= 16 ? 2 : side <= 32 ? 3 : 4;
-      f[1 + cls] += 1.f;
-      f[6] += (float)tx.eob / 64.f;
-      f[16] = std::max(f[16], (float)side / 64.f);
-    }
-    if (d.kind == kIntra) {
-      f[7] += 1.f;
-      if (d.mode >= AV1H_PRED_V && d.mode <= AV1H_PRED_D67) f[11] += 1.f;
-      if (d.mode >= AV1H_PRED_SMOOTH && d.mode <= AV1H_PRED_PAETH) f[12] += 1.f;
-      if (d.mode == AV1H_PRED_CFL || (d.flags & (kPalette | kFilterIntra))) f[13] += 1.f;
-    } else {
-      f[8] += 1.f;
-      f[15] += area;
-      ++n_inter[sb];
-      if (d.compound) f[9] += 1.f;
-      bool sub = false;
-      for (int r = 0; r < (d.compound ? 2 : 1); ++r) sub = sub || (d.mv[r][0] & 7) || (d.mv[r][1] & 7);
-      if (sub) ++n_sub[sb];
-      if (d.flags & (kInterIntra | kWarp0 | kWarp1 | kMaskWarp0 | kMaskWarp1) || d.obmc_count > 0 || d.mask_kind)
-        f[20] += 1.f;
-    }
-    if (d.flags & (kLossless | kResidualRaw)) f[17] += 1.f;
-    if (d.flags & kIntraBC) f[18] += 1.f;
-    if (d.plane) f[19] += 1.f;
-    if (queued[i]) {
-      f[14] += area;
-      f[22] += 1.f;
-    }
+  const size_t nsb = (size_t)sw * sh;
+  plan->sb_feat.assign(nsb * AV1H_CM_FEATURES, 0.f);
+  H->s_sb_inter.assign(nsb, 0);
+  H->s_sb_sub.assign(nsb, 0);
+}
+
+void sb_block(const Layout& L, const DevBlock& d, const DevTx* tx, Plan* plan, Host* H) {
+  const int sb = sb_of(L, plan->sb_w, plan->sb_h, d.plane, d.x, d.y);
+  float* f = &plan->sb_feat[(size_t)sb * AV1H_CM_FEATURES];
+  const float area = (float)d.w * d.h / 4096.f;
+  f[0] += 1.f;
+  for (int t = 0; t < d.tx_count; ++t) {
+    const int side = std::max(tx[t].w, tx[t].h);
+    const int cls = side <= 4 ? 0 : side <= 8 ? 1 : side <= 16 ? 2 : side <= 32 ? 3 : 4;
+    f[1 + cls] += 1.f;
+    f[6] += (float)tx[t].eob / 64.f;
+    f[16] = std::max(f[16], (float)side / 64.f);
   }
+  if (d.kind == kIntra) {
+    f[7] += 1.f;
+    if (d.mode >= AV1H_PRED_V && d.mode <= AV1H_PRED_D67) f[11] += 1.f;
+    if (d.mode >= AV1H_PRED_SMOOTH && d.mode <= AV1H_PRED_PAETH) f[12] += 1.f;
+    if (d.mode == AV1H_PRED_CFL || (d.flags & (kPalette | kFilterIntra))) f[13] += 1.f;
+  } else {
+    f[8] += 1.f;
+    f[15] += area;
+    ++H->s_sb_inter[(size_t)sb];
+    if (d.compound) f[9] += 1.f;
+    bool sub = false;
+    for (int r = 0; r < (d.compound ? 2 : 1); ++r) sub = sub || (d.mv[r][0] & 7) || (d.mv[r][1] & 7);
+    if (sub) ++H->s_sb_sub[(size_t)sb];
+    if (d.flags & (kInterIntra | kWarp0 | kWarp1 | kMaskWarp0 | kMaskWarp1) || d.obmc_count > 0 || d.mask_kind)
+      f[20] += 1.f;
+  }
+  if (d.flags & (kLossless | kResidualRaw)) f[17] += 1.f;
+  if (d.flags & kIntraBC) f[18] += 1.f;
+  if (d.plane) f[19] += 1.f;
+}
+
+/* a queued block (queue graph order qi) */
+void sb_queued(const Layout& L, const DevBlock& d, int qi, Plan* plan, Host* H) {
+  const int sb = sb_of(L, plan->sb_w, plan->sb_h, d.plane, d.x, d.y);
+  H->s_sb_q[(size_t)qi] = sb;
+  float* f = &plan->sb_feat[(size_t)sb * AV1H_CM_FEATURES];
+  f[14] += (float)d.w * d.h / 4096.f;
+  f[22] += 1.f;
+}
+
+void sb_finish(const Host& H, const av1h_frame_header& hdr, const Layout& L, Plan* plan) {
+  const int sw = plan->sb_w, sh = plan->sb_h, nsb = sw * sh;
+  std::vector<float>& F = plan->sb_feat;
+  /* queue entry k is queued block queue_qi[k] */
+  plan->queue_sb.resize(plan->queue_host.size());
+  for (size_t k = 0; k < plan->queue_host.size(); ++k) plan->queue_sb[k] = H.s_sb_q[(size_t)plan->queue_qi[k]];
+  plan->queue_sb_static = plan->queue_sb;
   for (int sb = 0; sb < nsb; ++sb) {
     float* f = &F[(size_t)sb * AV1H_CM_FEATURES];
-    f[10] = n_inter[sb] ? (float)n_sub[sb] / (float)n_inter[sb] : 0.f;
+    const int ni = H.s_sb_inter[(size_t)sb];
+    f[10] = ni ? (float)H.s_sb_sub[(size_t)sb] / (float)ni : 0.f;
     f[21] = hdr.enable_intra_edge_filter ? 1.f : 0.f;
     f[23] = hdr.bit_depth > 8 ? 1.f : 0.f;
   }
-  plan->queue_sb.resize(plan->queue_host.size());
-  for (size_t k = 0; k < plan->queue_host.size(); ++k) {
-    const DevBlock& d = H.blocks[(size_t)plan->queue_host[k]];
-    plan->queue_sb[k] = sb_at(d.plane, d.x, d.y);
-  }
-  plan->queue_sb_static = plan->queue_sb;
 }
 
 int resolve_slot(const av1h_frame_header& h, int ref) {
@@ -758,6 +773,7 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
       derive = true;
   Owner own;
   if (derive) own.build(L, f);
+  sb_begin(L, plan, H);
   size_t cend = 0, rend = 0;
   for (int i = 0; i < nb; ++i) {
     const av1h_block& b = f.blocks[i];
@@ -1015,6 +1031,7 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
       H->tx.push_back(dt);
     }
     d.tx_count = (int)H->tx.size() - d.tx_first;
+    sb_block(L, d, H->tx.data() + d.tx_first, plan, H);
     H->blocks.push_back(d);
   }
   /* pools padded like the reference planner's (engine.cpp:595-660): the
@@ -1037,6 +1054,7 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
     plan->q_dep_off.assign(1, 0);
     plan->q_dep.clear();
     plan->q_area.assign((size_t)nq, 0);
+    H->s_sb_q.resize((size_t)nq);
     std::vector<int32_t> level((size_t)nq, 0);
     std::vector<int32_t> deps;
     /* packed edges need every queued block on whole cells, so that each
@@ -1083,8 +1101,20 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
       if ((d.flags & kIntraBC) && d.dep_rx1 >= d.dep_rx0 && d.dep_ry1 >= d.dep_ry0)
         for (int cy = d.dep_ry0; cy <= d.dep_ry1; ++cy)
           for (int cx = d.dep_rx0; cx <= d.dep_rx1; ++cx) dep(p, cx, cy);
-      std::sort(deps.begin(), deps.end());
-      deps.erase(std::unique(deps.begin(), deps.end()), deps.end());
+      /* a handful of entries: insertion sort, duplicates dropped */
+      {
+        size_t m = 0;
+        for (size_t i = 0; i < deps.size(); ++i) {
+          const int32_t v = deps[i];
+          size_t j = m;
+          while (j > 0 && deps[j - 1] > v) --j;
+          if (j > 0 && deps[j - 1] == v) continue;
+          for (size_t k = m; k > j; --k) deps[k] = deps[k - 1];
+          deps[j] = v;
+          ++m;
+        }
+        deps.resize(m);
+      }
       if (packed && packed_edges(L, d, owner[p], queued[p], H->blocks, H->queue_list)) {
         d.flags |= kPackedEdges;
       } else {
@@ -1094,6 +1124,7 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
       for (int32_t o : deps) lv = std::max(lv, level[(size_t)o] + 1);
       level[(size_t)qi] = lv;
       plan->q_area[(size_t)qi] = d.w * d.h;
+      sb_queued(L, d, qi, plan, H);
       plan->q_dep.insert(plan->q_dep.end(), deps.begin(), deps.end());
       plan->q_dep_off.push_back((int32_t)plan->q_dep.size());
       /* the block's own cells */
@@ -1113,9 +1144,16 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
     }
     plan->q_block = H->queue_list; /* coding order */
     plan->q_level = level;
+    /* by level, coding order inside a level: a counting sort */
     std::vector<int32_t> ord((size_t)nq);
-    for (int k = 0; k < nq; ++k) ord[(size_t)k] = k;
-    std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t b) { return level[(size_t)a] < level[(size_t)b]; });
+    {
+      int top = 0;
+      for (int k = 0; k < nq; ++k) top = std::max(top, level[(size_t)k]);
+      std::vector<int32_t> start((size_t)top + 2, 0);
+      for (int k = 0; k < nq; ++k) ++start[(size_t)level[(size_t)k] + 1];
+      for (int v = 0; v <= top; ++v) start[(size_t)v + 1] += start[(size_t)v];
+      for (int k = 0; k < nq; ++k) ord[(size_t)start[(size_t)level[(size_t)k]]++] = k;
+    }
     std::vector<int32_t> q((size_t)nq);
     for (int k = 0; k < nq; ++k) q[(size_t)k] = H->queue_list[(size_t)ord[(size_t)k]];
     H->queue_list.swap(q);
@@ -1130,7 +1168,7 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
     plan->queue_static = plan->queue_host;
     plan->key_off_static = plan->key_off;
   }
-  sb_features(*H, hdr, L, plan);
+  sb_finish(*H, hdr, L, plan);
 
   /* motion compensation tasks: items (whole blocks, or row strips of blocks
    * over kMcSamples samples) in coding order, with their positions, windows
@@ -1483,17 +1521,22 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
   {
     std::vector<int32_t>& dev_of = H->s_dev_of;
     dev_of.assign(H->blocks.size(), -1);
-    for (int32_t b : H->queue_list) dev_of[(size_t)b] = 0;
-    for (int32_t b : H->special_list) dev_of[(size_t)b] = 0;
+    size_t ndev = 0;
+    for (int32_t b : H->queue_list) ndev += dev_of[(size_t)b] < 0 ? (dev_of[(size_t)b] = 0, 1) : 0;
+    for (int32_t b : H->special_list) ndev += dev_of[(size_t)b] < 0 ? (dev_of[(size_t)b] = 0, 1) : 0;
     H->dev_blocks.clear();
-    for (size_t b = 0; b < H->blocks.size(); ++b)
-      if (dev_of[b] == 0) {
-        dev_of[b] = (int32_t)H->dev_blocks.size();
-        H->dev_blocks.push_back(H->blocks[b]);
-      }
-    for (std::vector<int32_t>* v : {&H->queue_list, &H->special_list, &plan->queue_host, &plan->queue_static,
-                                    &plan->q_block})
-      for (int32_t& b : *v) b = dev_of[(size_t)b];
+    /* every block read by the device (an intra frame): the numbering stays */
+    H->dev_all = ndev == H->blocks.size();
+    if (!H->dev_all) {
+      for (size_t b = 0; b < H->blocks.size(); ++b)
+        if (dev_of[b] == 0) {
+          dev_of[b] = (int32_t)H->dev_blocks.size();
+          H->dev_blocks.push_back(H->blocks[b]);
+        }
+      for (std::vector<int32_t>* v : {&H->queue_list, &H->special_list, &plan->queue_host, &plan->queue_static,
+                                      &plan->q_block})
+        for (int32_t& b : *v) b = dev_of[(size_t)b];
+    }
   }
 
   /* one device buffer: the arrays back to back, 256 byte aligned; the
@@ -1503,7 +1546,8 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
     H->part_of[k] = (int)H->parts.size();
     H->parts.push_back(Host::Part{src, bytes, fill, 0});
   };
-  add(kPartBlocks, H->dev_blocks.data(), H->dev_blocks.size() * sizeof(DevBlock));
+  const std::vector<DevBlock>& dblk = H->dev_all ? H->blocks : H->dev_blocks;
+  add(kPartBlocks, dblk.data(), dblk.size() * sizeof(DevBlock));
   add(kPartTx, H->tx.data(), H->tx.size() * sizeof(DevTx));
   add(kPartInter, H->inter_list.data(), H->inter_list.size() * 4);
   add(kPartQueue, H->queue_list.data(), H->queue_list.size() * 4);
