@@ -249,6 +249,7 @@ struct Ctx {
    * a decode queued behind an upload's wait in a shared queue stalls. */
   hipStream_t upload = nullptr;
   std::mutex plan_mu;
+  size_t stage_hint = 0; /* the largest plan staged so far */
   std::vector<Stager*> stagers;
   std::vector<std::pair<size_t, void*>> plan_bufs;
   size_t plan_buf_bytes = 0;
@@ -1746,14 +1747,29 @@ int upload_plan(Ctx* c, Plan* plan, const Host& H, std::string* err) {
   if (rc != AV1H_OK) return rc;
   bind_dev(plan, H);
   Stager* st = nullptr;
+  size_t hint = n;
   {
     std::lock_guard<std::mutex> lk(c->plan_mu);
+    /* a free stager, one large enough if there is one: growing a pinned
+     * buffer (free, then allocate) costs milliseconds and can wait for the
+     * device */
+    size_t pick = c->stagers.size();
     for (size_t i = 0; i < c->stagers.size(); ++i)
       if (!c->stagers[i]->done || hipEventQuery(c->stagers[i]->done) == hipSuccess) {
-        st = c->stagers[i];
-        c->stagers.erase(c->stagers.begin() + (std::ptrdiff_t)i);
-        break;
+        if (pick == c->stagers.size()) pick = i;
+        if (c->stagers[i]->cap >= n) {
+          pick = i;
+          break;
+        }
       }
+    /* none large enough: a new one rather than a reallocation, while the
+     * stagers are few */
+    if (pick < c->stagers.size() && (c->stagers[pick]->cap >= n || c->stagers.size() >= 32)) {
+      st = c->stagers[pick];
+      c->stagers.erase(c->stagers.begin() + (std::ptrdiff_t)pick);
+    }
+    c->stage_hint = std::max(c->stage_hint, n);
+    hint = c->stage_hint;
     if (!st && c->stagers.size() >= 64) {
       st = c->stagers.front();
       c->stagers.erase(c->stagers.begin());
@@ -1775,7 +1791,10 @@ int upload_plan(Ctx* c, Plan* plan, const Host& H, std::string* err) {
     if (st->pin) (void)hipHostFree(st->pin);
     st->pin = nullptr;
     st->cap = 0;
-    const size_t sz = (n + 4 * kPlanGrain - 1) / (4 * kPlanGrain) * (4 * kPlanGrain);
+    /* sized for the largest plan staged so far, so that the stagers stop
+     * growing once every kind of frame of the stream has passed */
+    const size_t want = std::max(n, hint) + std::max(n, hint) / 4;
+    const size_t sz = (want + 4 * kPlanGrain - 1) / (4 * kPlanGrain) * (4 * kPlanGrain);
     e = hipHostMalloc(&st->pin, sz, hipHostMallocDefault);
     if (e == hipSuccess) st->cap = sz;
   }

@@ -93,16 +93,16 @@ def display_pixels(clip):
 
 
 def default_planners():
-    """Host threads for planning: 3/8 of the cores this process may use, at
-    most 12 (on 16 cores: 6 planners measured 2995 Mpixel/s, 8 2875, 10
-    2719, 14 2050; the planners contend for memory bandwidth and the decode
-    worker's launches and waits need a core of their own)."""
+    """Host threads for planning: half the cores this process may use, at
+    most 12 (on 16 cores with 64 frames admitted: 8 planners measured 1933
+    frames/s, 10 1911, 12 at 48 admitted 1884; the planners contend for
+    memory bandwidth and the decode worker needs a core of its own)."""
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         cores = os.cpu_count() or 1
     cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
-    return max(1, min(12, cores * 3 // 8))
+    return max(1, min(12, cores // 2))
 
 
 def pct(values, q):
@@ -407,12 +407,14 @@ SERVE_GOP = 16
 SERVE_PITCH = 4096
 
 
-def run_serve(args):
+def run_serve(args, planned=False):
     """Frames of 64 streams arrive as Poisson processes and go through the
     serving loop (av1h_server_*: admission against a 200 GB budget,
     continuous batching, NV12 packed with every frame). Every completed
     frame's device checksum is compared with the same frame decoded alone.
-    Latency: arrival to decoded, stored and NV12-packed."""
+    Latency: arrival to decoded, stored and NV12-packed. Each frame arrives
+    as syntax (wire bytes), planned by the loop's planner threads inside its
+    latency, or with `planned` as a plan already resident in HBM."""
     import threading
     import av1ongpu_amd as pkg
     import numpy as np
@@ -450,9 +452,10 @@ def run_serve(args):
     inter_levels = [sorted(pkg.hip_lib().av1h_plan_levels(p) for p in ps)[len(ps) // 2] for ps in plans]
     heavy = args.serve_heavy if args.serve_heavy >= 0 else 4 * max(inter_levels)
     log(f"serve: queue levels per group: max {levels}, median {inter_levels}; heavy above {heavy}")
-    for ps in plans:
-        for p in ps:
-            owner.discard_plan(p)
+    if not planned:
+        for ps in plans:
+            for p in ps:
+                owner.discard_plan(p)
     srv = pkg.Server(0, max_batch=args.serve_batch, planner_threads=args.serve_planners, hbm_budget=200 << 30,
                      nv12_pitch=SERVE_PITCH, hash_pictures=True, launchers=args.serve_launchers,
                      heavy_levels=heavy, heavy_launchers=args.serve_heavy_launchers,
@@ -498,11 +501,18 @@ def run_serve(args):
             time.sleep(min(0.002, (target - now) / 1e3))
         keep, rel = scheds[ci]
         g = k % SERVE_GOP
-        srv.submit_wire_ref(sid, wires[ci][g], keep[g], rel[g], arrival_ms=target, tag=(si << 32) | k)
+        if planned:
+            srv.submit_plan(sid, plans[ci][g], keep[g], rel[g], arrival_ms=target, tag=(si << 32) | k)
+        else:
+            srv.submit_wire_ref(sid, wires[ci][g], keep[g], rel[g], arrival_ms=target, tag=(si << 32) | k)
     stop.set()
     th.join(timeout=120)
     st = srv.stats()
     srv.close()
+    if planned:
+        for ps in plans:
+            for p in ps:
+                owner.discard_plan(p)
     owner.close()
     px = [w * h for _n, _c, w, h, _b, _cw, _g in SERVE_CLASSES]
     lat = {ci: [] for ci in range(len(SERVE_CLASSES))}
@@ -528,8 +538,11 @@ def run_serve(args):
         "workload": "64 streams (26x 1280x720 8-bit, 25x 1920x1080 10-bit with film grain, 13x 3840x2160 10-bit "
                     "with superres 3/4), full-tool syntax statistics, Poisson arrivals of 1200 frames/s in all; "
                     f"each stream cycles through a {SERVE_GOP}-frame group (a key frame and 15 inter frames, 3 "
-                    f"references); each frame submitted as syntax (wire bytes) at its arrival and planned by "
-                    f"{args.serve_planners} planner threads inside its latency; NV12 packed with every frame",
+                    f"references); " + ("each frame submitted at its arrival as a plan resident in HBM "
+                                        "(planned beforehand)" if planned else
+                                        f"each frame submitted as syntax (wire bytes) at its arrival and planned "
+                                        f"by {args.serve_planners} planner threads inside its latency") +
+                    "; NV12 packed with every frame",
         "seconds": dur, "frames": len(allv), "completed": len(done), "submitted": len(events),
         "offered_mpixel_s": round(sum(px[streams[si][1]] for tt, si, k in events if tt >= warm) / dur / 1e6, 1),
         "value": round(pix / span / 1e6, 1), "unit": "Mpixel/s",
@@ -562,7 +575,9 @@ def main():
     ap.add_argument("--planners", type=int, default=0, help="planner threads of the ingest pipeline (0: automatic)")
     ap.add_argument("--levels", type=int, default=0,
                     help="1: the clip's syntax carries coded levels (dequantised on the device)")
-    ap.add_argument("--depth", type=int, default=24, help="frames in flight in the ingest pipeline")
+    ap.add_argument("--depth", type=int, default=64,
+                    help="frames admitted to the ingest pipeline (a key frame takes about three inter frames' "
+                         "planning time, so it must start planning well before its turn)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline and the parity sample")
     ap.add_argument("--no-mix", action="store_true", help="skip the 16-stream batched vs plain mix")
@@ -581,7 +596,7 @@ def main():
     ap.add_argument("--serve-launchers", type=int, default=2)
     ap.add_argument("--serve-heavy-launchers", type=int, default=1)
     ap.add_argument("--serve-heavy-batch", type=int, default=1)
-    ap.add_argument("--serve-planners", type=int, default=8, help="planner threads of the serving loop")
+    ap.add_argument("--serve-planners", type=int, default=12, help="planner threads of the serving loop")
     ap.add_argument("--serve-heavy", type=int, default=-1,
                     help="dependency levels above which a frame runs on the key-frame launcher (-1: automatic, 0: off)")
     args = ap.parse_args()
@@ -599,7 +614,7 @@ def main():
         print(json.dumps({"batched_mix": run_mix(args)}), flush=True)
         return
     if args.serve_only:
-        print(json.dumps({"serving": run_serve(args)}), flush=True)
+        print(json.dumps({"serving": run_serve(args), "serving_planned": run_serve(args, planned=True)}), flush=True)
         return
     dec = pkg.HipDecoder(0)
     log(f"device: {dec.device_info().name.decode(errors='replace')}")
@@ -786,6 +801,7 @@ def main():
         result["batched_mix"] = run_mix(args)
     if not args.no_serve:
         result["serving"] = run_serve(args)
+        result["serving_planned"] = run_serve(args, planned=True)
     print(json.dumps(result), flush=True)
 
 
