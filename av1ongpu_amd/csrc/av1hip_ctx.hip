@@ -788,7 +788,8 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
       return set_err(err, AV1H_ERR_UNSUPPORTED, "block %d: bit depth differs from the frame", i);
     if (b.tx_first < 0 || b.tx_count < 0 || b.tx_first + (int64_t)b.tx_count > f.num_transforms)
       return set_err(err, AV1H_ERR_INVALID, "block %d: transform range", i);
-    DevBlock d;
+    /* built in place (room for every block is reserved) */
+    DevBlock& d = H->blocks.emplace_back();
     std::memset(&d, 0, sizeof(d));
     d.x = b.x;
     d.y = b.y;
@@ -1033,7 +1034,6 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
     }
     d.tx_count = (int)H->tx.size() - d.tx_first;
     sb_block(L, d, H->tx.data() + d.tx_first, plan, H);
-    H->blocks.push_back(d);
   }
   /* pools padded like the reference planner's (engine.cpp:595-660): the
    * caller's arrays are staged as they are, zeros after them */
@@ -1192,7 +1192,8 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
       const int sx = d.plane ? hdr.ss_x : 0, sy = d.plane ? hdr.ss_y : 0;
       const int per = d.w * d.h <= kMcSamples ? d.h : kMcSamples / d.w;
       for (int y0 = 0; y0 < d.h; y0 += per) {
-        McItemDev it;
+        /* built in place; a task that closes before it starts at it */
+        McItemDev& it = H->mc_items.emplace_back();
         std::memset(&it, 0, sizeof(it));
         it.x = d.x;
         it.y = d.y;
@@ -1223,7 +1224,10 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
           msz += q.wh * d.w;
         }
         const int sb = it.rows * d.w;
-        if (cur.count + 1 > kMcItems || cur.samples + sb > kMcSamples || cur.mtotal + msz > kMcMid) close_task();
+        if (cur.count + 1 > kMcItems || cur.samples + sb > kMcSamples || cur.mtotal + msz > kMcMid) {
+          close_task();
+          cur.first = (int32_t)H->mc_items.size() - 1;
+        }
         if (msz > kMcMid) return set_err(err, AV1H_ERR_UNSUPPORTED, "motion compensation window");
         it.px0 = cur.samples;
         for (int r = 0; r < 2; ++r) {
@@ -1237,7 +1241,6 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
         }
         cur.samples += sb;
         ++cur.count;
-        H->mc_items.push_back(it);
       }
     }
     close_task();
