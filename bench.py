@@ -606,6 +606,22 @@ def run_serve(args, planned=False):
             last_done = max(last_done, c["done_ms"])
     allv = [v for ci in lat for v in lat[ci]]
     plan_ms = [c["planned_ms"] - c["plan_start_ms"] for c in done if c["arrival_ms"] >= t_lo]
+    # where the slowest 1 % of frames spent their time: waiting for a planner,
+    # planning, waiting for their batch to start, in their batch
+    timed = sorted((c for c in done if c["arrival_ms"] >= t_lo), key=lambda c: c["done_ms"] - c["arrival_ms"])
+    tail = timed[int(len(timed) * 0.99):] or timed[-1:]
+
+    def phases(cs):
+        if not cs:
+            return None
+        n = len(cs)
+        if planned:
+            return {"to_start": round(sum(c["start_ms"] - c["arrival_ms"] for c in cs) / n, 3),
+                    "in_batch": round(sum(c["done_ms"] - c["start_ms"] for c in cs) / n, 3)}
+        return {"to_planner": round(sum(c["plan_start_ms"] - c["arrival_ms"] for c in cs) / n, 3),
+                "planning": round(sum(c["planned_ms"] - c["plan_start_ms"] for c in cs) / n, 3),
+                "to_start": round(sum(c["start_ms"] - c["planned_ms"] for c in cs) / n, 3),
+                "in_batch": round(sum(c["done_ms"] - c["start_ms"] for c in cs) / n, 3)}
     span = (max(last_done, t_hi) - t_lo) / 1e3
     rep = {
         "workload": "64 streams (26x 1280x720 8-bit, 25x 1920x1080 10-bit with film grain, 13x 3840x2160 10-bit "
@@ -630,6 +646,7 @@ def run_serve(args, planned=False):
         "batches": st["batches"], "mean_batch_frames": round(st["frames"] / max(st["batches"], 1), 2),
         "device_busy_frac": round(st["busy_ms"] / (max(c["done_ms"] for c in done) - t0), 3),
         "planning_ms": {"p50": round(pct(plan_ms, 50), 3), "p99": round(pct(plan_ms, 99), 3)} if plan_ms else None,
+        "latency_phases_ms": {"all_frames_mean": phases(timed), "slowest_1pct_mean": phases(tail)},
         "bit_exact": {"frames_checked": len(done), "mismatched_frames": mism,
                       "method": "device checksum of every completed picture vs the same frame decoded alone"},
     }
