@@ -294,24 +294,47 @@ constexpr int kCdefTileWave = 256;         /* LDS samples per wave         */
  * row pitch tw */
 __device__ __forceinline__ int cdef_tap(const int* e, int tw) { return e[0] * tw + e[1]; }
 
-/* One sample (r, k) of block `bi`, its tile at `t` (row pitch TW, the
- * sample at t[(r + 2) * TW + k + 2]). False when it keeps the snapshot's
- * value. A luma wave holds one block: its parameters, direction and tap
- * offsets are then wave-uniform and live in scalar registers (the
- * direction table read from constant memory); chroma waves hold several
- * blocks and read the table from LDS. */
+/* A block's CDEF record and, when its direction is searched, its entry of
+ * the direction map: loaded before its tile is staged, so that the two
+ * loads' latencies overlap. A luma wave holds one block: its values are
+ * wave-uniform and come through the scalar cache (the map was written by an
+ * earlier kernel); chroma waves hold several blocks. */
+struct CdefBlockIn {
+  int plane, x, y, pri, sec, damping, dir, yd, var;
+};
+
+template <bool kLuma>
+__device__ __forceinline__ CdefBlockIn cdef_block_in(FArgs a, int first, int bi) {
+  CdefBlockIn q;
+  if (kLuma) {
+    bi = __builtin_amdgcn_readfirstlane(bi);
+    const AV1H_CMEM DevCdef& cu = cmem(a.plan.cdef)[first + bi];
+    q = CdefBlockIn{cu.plane, cu.x, cu.y, cu.pri, cu.sec, cu.damping, cu.dir, 0, 0};
+    if (q.dir < 0) {
+      const AV1H_CMEM int* m = cmem(a.cdef_dir) + 2 * ((q.y >> 3) * a.cdef_dir_w + (q.x >> 3));
+      q.yd = m[0];
+      q.var = m[1];
+    }
+  } else {
+    const DevCdef c = a.plan.cdef[first + bi];
+    q = CdefBlockIn{c.plane, c.x, c.y, c.pri, c.sec, c.damping, c.dir, 0, 0};
+    if (q.dir < 0) {
+      LayRef L = a.lay;
+      const int lx = c.x << L.ss_x, ly = c.y << L.ss_y;
+      q.yd = a.cdef_dir[2 * ((ly >> 3) * a.cdef_dir_w + (lx >> 3))];
+    }
+  }
+  return q;
+}
+
+/* One sample (r, k) of block `c`, its tile at `t` (row pitch TW, the sample
+ * at t[(r + 2) * TW + k + 2]). False when it keeps the snapshot's value.
+ * Luma reads the direction table from constant memory (its direction is
+ * wave-uniform), chroma from LDS. */
 template <int W, int H, bool kLuma>
-__device__ __forceinline__ bool cdef_tile_sample(FArgs a, int first, int bi, int r, int k, const uint16_t* t,
+__device__ __forceinline__ bool cdef_tile_sample(FArgs a, const CdefBlockIn& c, int r, int k, const uint16_t* t,
                                                  const int (*sdirs)[2][2], int64_t* off, int* val) {
   constexpr int TW = W + 4;
-  if (kLuma) bi = __builtin_amdgcn_readfirstlane(bi);
-  DevCdef c;
-  if (kLuma) {
-    const AV1H_CMEM DevCdef& cu = cmem(a.plan.cdef)[first + bi];
-    c = DevCdef{cu.plane, cu.x, cu.y, cu.pri, cu.sec, cu.damping, cu.dir, 0};
-  } else {
-    c = a.plan.cdef[first + bi];
-  }
   const int (*dirs)[2][2] = kLuma ? kCdefDirs : sdirs;
   LayRef L = a.lay;
   const int p = kLuma ? 0 : c.plane;
@@ -323,16 +346,12 @@ __device__ __forceinline__ bool cdef_tile_sample(FArgs a, int first, int bi, int
   const int damping = c.damping + cs - (kLuma ? 0 : 1);
   int dir = c.dir;
   if (dir < 0) {
-    const int lx = kLuma ? c.x : c.x << L.ss_x, ly = kLuma ? c.y : c.y << L.ss_y;
-    const int* m = a.cdef_dir + 2 * ((ly >> 3) * a.cdef_dir_w + (lx >> 3));
-    const int yd = kLuma ? __builtin_amdgcn_readfirstlane(m[0]) : m[0];
-    const int var = kLuma ? __builtin_amdgcn_readfirstlane(m[1]) : m[1];
     if (!kLuma) {
-      dir = pri == 0 ? 0 : kCdefUvDir[L.ss_x][L.ss_y][yd];
+      dir = pri == 0 ? 0 : kCdefUvDir[L.ss_x][L.ss_y][c.yd];
     } else {
-      const int vs = (var >> 6) ? min(flog2(var >> 6), 12) : 0;
-      dir = pri == 0 ? 0 : yd;
-      pri = var ? ((pri * (4 + vs) + 8) >> 4) : 0;
+      const int vs = (c.var >> 6) ? min(flog2(c.var >> 6), 12) : 0;
+      dir = pri == 0 ? 0 : c.yd;
+      pri = c.var ? ((pri * (4 + vs) + 8) >> 4) : 0;
     }
   }
   if (pri == 0 && sec == 0) return false; /* the destination already holds the snapshot's value */
@@ -381,8 +400,10 @@ __device__ __forceinline__ bool cdef_tile_sample(FArgs a, int first, int bi, int
   return true;
 }
 
-/* `batch` rounds of 256 / (W x H) blocks per workgroup: each round stages
- * the wave's blocks, then filters them. */
+/* `batch` (1 to kCdefMaxBatch) sets of 256 / (W x H) blocks per
+ * workgroup: every lane loads its block record (and direction) of each set,
+ * the wave stages all its sets' tiles, and only then filters them, so that
+ * a wave has the loads of all its sets in flight at once. */
 template <int W, int H, bool kLuma>
 __device__ __forceinline__ void cdef_tiles(FArgs a, int first, int count, int grp, int batch, uint16_t* tile,
                                            const int (*dirs)[2][2]) {
@@ -390,42 +411,62 @@ __device__ __forceinline__ void cdef_tiles(FArgs a, int first, int count, int gr
   constexpr int kPairs = TW / 2, kDw = kPairs * TH; /* 32-bit words per block tile */
   static_assert(kWave * TW * TH <= kCdefTileWave, "tiles of a wave exceed its LDS");
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  uint16_t* wt = tile + wave * kCdefTileWave;
+  uint16_t* wt = tile + wave * kCdefTileWave * kCdefMaxBatch;
   LayRef L = a.lay;
-  for (int u = 0; u < batch; ++u) {
+  const int sb = lane / S, sm = lane - sb * S, r = sm / W, k = sm - r * W;
+  CdefBlockIn q[kCdefMaxBatch];
+  bool live[kCdefMaxBatch];
+#pragma unroll
+  for (int u = 0; u < kCdefMaxBatch; ++u) {
     const int b0 = (grp * batch + u) * kPer + wave * kWave;
+    live[u] = u < batch && b0 + sb < count;
+    q[u] = CdefBlockIn{};
+    if (live[u]) q[u] = cdef_block_in<kLuma>(a, first, b0 + sb);
+  }
+#pragma unroll
+  for (int u = 0; u < kCdefMaxBatch; ++u) {
+    if (u >= batch) break;
+    const int b0 = (grp * batch + u) * kPer + wave * kWave;
+    uint16_t* ut = wt + u * kCdefTileWave;
     for (int e = lane; e < kWave * kDw; e += 64) {
-      const int sb = e / kDw, rem = e - sb * kDw, row = rem / kPairs, cp = rem - row * kPairs;
-      const int bi = b0 + sb;
+      const int eb = e / kDw, rem = e - eb * kDw, row = rem / kPairs, cp = rem - row * kPairs;
       uint32_t v = 0xffffffffu;
-      if (bi < count) {
-        const DevCdef& c = a.plan.cdef[first + bi];
-        const int p = kLuma ? 0 : c.plane;
-        const int px = c.x - 2 + 2 * cp, py = c.y - 2 + row;
+      if (b0 + eb < count) {
+        /* a luma wave stages its own block; a chroma lane may stage another */
+        int p = 0, cx = q[u].x, cy = q[u].y;
+        if (!kLuma) {
+          const DevCdef& c = a.plan.cdef[first + b0 + eb];
+          p = c.plane;
+          cx = c.x;
+          cy = c.y;
+        }
+        const int px = cx - 2 + 2 * cp, py = cy - 2 + row;
         /* grid widths are even: a pair is inside or outside as a whole */
         if (px >= 0 && px < L.grid_w[p] && py >= 0 && py < L.grid_h[p])
           v = *reinterpret_cast<const uint32_t*>(a.cdef_src + L.plane_off[p] + (int64_t)py * L.stride[p] + px);
       }
-      *reinterpret_cast<uint32_t*>(wt + sb * TW * TH + row * TW + 2 * cp) = v;
+      *reinterpret_cast<uint32_t*>(ut + eb * TW * TH + row * TW + 2 * cp) = v;
     }
-    __syncthreads();
-    const int sb = lane / S, sm = lane - sb * S, r = sm / W, k = sm - r * W;
-    const int bi = b0 + sb;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < kCdefMaxBatch; ++u) {
     int64_t off = 0;
     int val = 0;
-    if (bi < count && cdef_tile_sample<W, H, kLuma>(a, first, bi, r, k, wt + sb * TW * TH, dirs, &off, &val)) {
+    if (live[u] &&
+        cdef_tile_sample<W, H, kLuma>(a, q[u], r, k, wt + u * kCdefTileWave + sb * TW * TH, dirs, &off, &val)) {
       a.cdef_dst[off] = (uint16_t)val;
       if (a.cdef_dst2) a.cdef_dst2[off] = (uint16_t)val;
     }
-    __syncthreads();
   }
+  __syncthreads();
 }
 
 /* luma blocks (plane 0) or chroma blocks (planes 1 and 2) of every frame */
 __global__ __launch_bounds__(256) void k_cdef(BatchDev B, int chroma, int batch) {
   /* the direction table in LDS: lanes index it by their own direction */
   __shared__ int s_dirs[8][2][2];
-  __shared__ __align__(16) uint16_t s_tile[4 * kCdefTileWave];
+  __shared__ __align__(16) uint16_t s_tile[4 * kCdefTileWave * kCdefMaxBatch];
   if (threadIdx.x < 32) (&s_dirs[0][0][0])[threadIdx.x] = (&kCdefDirs[0][0][0])[threadIdx.x];
   __syncthreads();
   const AV1H_CMEM int32_t* pre = kernel_prefix(B, (chroma ? kKCdefChroma : kKCdefLuma));
