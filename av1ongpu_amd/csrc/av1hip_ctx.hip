@@ -284,14 +284,12 @@ int set_err(Ctx* c, int code, const char* fmt, ...) {
   } while (0)
 
 int tx_row_shift(int w, int h) {
-  /* Transform_Row_Shift (reference src/core/tables.cpp:357-369) */
-  static const int t[19][3] = {{4, 4, 0},   {8, 8, 1},   {16, 16, 2}, {32, 32, 2}, {64, 64, 2},
-                               {4, 8, 0},   {8, 4, 0},   {8, 16, 1},  {16, 8, 1},  {16, 32, 1},
-                               {32, 16, 1}, {32, 64, 1}, {64, 32, 1}, {4, 16, 1},  {16, 4, 1},
-                               {8, 32, 2},  {32, 8, 2},  {16, 64, 2}, {64, 16, 2}};
-  for (auto& e : t)
-    if (e[0] == w && e[1] == h) return e[2];
-  return -1;
+  /* Transform_Row_Shift (reference src/core/tables.cpp:357-369) by
+   * [log2(w) - 2][log2(h) - 2]; -1: not an AV1 transform size */
+  static constexpr int8_t t[5][5] = {{0, 0, 1, -1, -1}, {0, 1, 1, 2, -1}, {1, 1, 2, 1, 2},
+                                     {-1, 2, 1, 2, 1},  {-1, -1, 2, 1, 2}};
+  if (w < 4 || w > 64 || h < 4 || h > 64 || (w & (w - 1)) || (h & (h - 1))) return -1;
+  return t[__builtin_ctz((unsigned)w) - 2][__builtin_ctz((unsigned)h) - 2];
 }
 
 bool pow2_in(int v, int lo, int hi) { return v >= lo && v <= hi && (v & (v - 1)) == 0; }
@@ -1279,7 +1277,8 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
     for (int64_t i = 0; i < f.num_deblock; ++i) {
       const av1h_deblock_edge& e = f.deblock[i];
       if ((e.vertical != 0) != (pass == 0)) continue;
-      if (e.plane < 0 || e.plane > 2 || e.length < 0)
+      if (e.plane < 0 || e.plane > 2 || e.length < 0 || e.level < 0 || e.level > 255 || e.filter_size < 0 ||
+          e.filter_size > 255)
         return set_err(err, AV1H_ERR_INVALID, "deblock edge %lld", (long long)i);
       const int reach = e.filter_size >= 16 ? 7 : (e.filter_size >= 8 ? 4 : 2);
       const int rows = L.out_h[e.plane] + (e.plane ? 32 : 64);
@@ -1289,14 +1288,13 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
       if (across < reach || across + reach > across_lim || along < 0 || along + e.length > along_lim)
         return set_err(err, AV1H_ERR_INVALID, "deblock edge %lld out of range", (long long)i);
       if (e.length == 0) continue;
-      DevDeblock d;
-      d.plane = e.plane;
-      d.x = e.x;
-      d.y = e.y;
-      d.vertical = e.vertical ? 1 : 0;
-      d.len = e.length;
-      d.level = e.level;
-      d.filter_size = e.filter_size;
+      DevDeblock d{};
+      d.plane_vertical = (uint8_t)(e.plane | (e.vertical ? 4 : 0));
+      d.x = (uint16_t)e.x;
+      d.y = (uint16_t)e.y;
+      d.len = (uint16_t)e.length;
+      d.level = (uint8_t)e.level;
+      d.filter_size = (uint8_t)e.filter_size;
       d.line0 = lines[pass];
       lines[pass] += e.length;
       ++nedge[pass];
@@ -1323,7 +1321,8 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
     for (int64_t i = 0; i < f.num_cdef; ++i) {
       const av1h_cdef_block& cb = f.cdef[i];
       if (cb.plane < 0 || cb.plane > 2 || cb.x < 0 || cb.y < 0 || cb.x >= L.grid_w[cb.plane] ||
-          cb.y >= L.grid_h[cb.plane] || cb.damping < 0 || cb.damping > 15 || cb.dir > 7)
+          cb.y >= L.grid_h[cb.plane] || cb.damping < 0 || cb.damping > 15 || cb.dir > 7 ||
+          cb.pri_strength < 0 || cb.pri_strength > 255 || cb.sec_strength < 0 || cb.sec_strength > 255)
         return set_err(err, AV1H_ERR_INVALID, "cdef block %lld", (long long)i);
       const int bw = cb.plane ? cbw : 8, bh = cb.plane ? cbh : 8;
       if (cb.x % bw || cb.y % bh)
@@ -1335,7 +1334,8 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
     for (int64_t i = 0; i < f.num_cdef; ++i) {
       const av1h_cdef_block& cb = f.cdef[i];
       H->cdef[cb.plane == 0 ? nl++ : nc++] =
-          DevCdef{cb.plane, cb.x, cb.y, cb.pri_strength, cb.sec_strength, cb.damping, cb.dir, 0};
+          DevCdef{(uint16_t)cb.x, (uint16_t)cb.y, (uint8_t)cb.pri_strength, (uint8_t)cb.sec_strength,
+                  (uint8_t)(cb.plane | cb.damping << 2), (int8_t)(cb.dir < 0 ? -1 : cb.dir)};
     }
   }
   H->cdef_dir_list.clear();
@@ -1345,7 +1345,8 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
     need.assign((size_t)bw8 * bh8, 0);
     for (const DevCdef& d : H->cdef) {
       if (d.dir >= 0) continue;
-      const int lx = d.plane ? d.x << hdr.ss_x : d.x, ly = d.plane ? d.y << hdr.ss_y : d.y;
+      const int pl = d.plane_damping & 3;
+      const int lx = pl ? d.x << hdr.ss_x : d.x, ly = pl ? d.y << hdr.ss_y : d.y;
       const int bx = lx >> 3, by = ly >> 3;
       if (bx < bw8 && by < bh8 && !need[(size_t)by * bw8 + bx]) {
         need[(size_t)by * bw8 + bx] = 1;
@@ -1494,7 +1495,7 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
     for (const int32_t a : plan->q_area) plan->alg_bytes[AV1H_STAGE_QUEUE] += 4 * (int64_t)a;
     bool has_edges[3] = {false, false, false};
     for (const DevDeblock& d : H->deblock) {
-      has_edges[d.plane] = true;
+      has_edges[d.plane_vertical & 3] = true;
       if (has_edges[0] && has_edges[1] && has_edges[2]) break;
     }
     int64_t disp = 0;

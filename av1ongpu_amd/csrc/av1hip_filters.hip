@@ -125,11 +125,11 @@ __device__ void deblock_chunk(FArgs a, int first, int count, int lines, const in
   const DevDeblock& e = edges[e0 + lo];
   if (e.level == 0) return;
   const int k = g - e.line0;
-  const int p = e.plane, st = a.lay.stride[p];
+  const int p = e.plane_vertical & 3, st = a.lay.stride[p];
   uint16_t* plane = a.frame + a.lay.plane_off[p];
   uint16_t* b;
   int s;
-  if (e.vertical) {
+  if (e.plane_vertical & 4) {
     b = plane + (size_t)(e.y + k) * st + (e.x - 2);
     s = 1;
   } else {
@@ -309,7 +309,7 @@ __device__ __forceinline__ CdefBlockIn cdef_block_in(FArgs a, int first, int bi)
   if (kLuma) {
     bi = __builtin_amdgcn_readfirstlane(bi);
     const AV1H_CMEM DevCdef& cu = cmem(a.plan.cdef)[first + bi];
-    q = CdefBlockIn{cu.plane, cu.x, cu.y, cu.pri, cu.sec, cu.damping, cu.dir, 0, 0};
+    q = CdefBlockIn{cu.plane_damping & 3, cu.x, cu.y, cu.pri, cu.sec, cu.plane_damping >> 2, cu.dir, 0, 0};
     if (q.dir < 0) {
       const AV1H_CMEM int* m = cmem(a.cdef_dir) + 2 * ((q.y >> 3) * a.cdef_dir_w + (q.x >> 3));
       q.yd = m[0];
@@ -317,7 +317,7 @@ __device__ __forceinline__ CdefBlockIn cdef_block_in(FArgs a, int first, int bi)
     }
   } else {
     const DevCdef c = a.plan.cdef[first + bi];
-    q = CdefBlockIn{c.plane, c.x, c.y, c.pri, c.sec, c.damping, c.dir, 0, 0};
+    q = CdefBlockIn{c.plane_damping & 3, c.x, c.y, c.pri, c.sec, c.plane_damping >> 2, c.dir, 0, 0};
     if (q.dir < 0) {
       LayRef L = a.lay;
       const int lx = c.x << L.ss_x, ly = c.y << L.ss_y;
@@ -436,7 +436,7 @@ __device__ __forceinline__ void cdef_tiles(FArgs a, int first, int count, int gr
         int p = 0, cx = q[u].x, cy = q[u].y;
         if (!kLuma) {
           const DevCdef& c = a.plan.cdef[first + b0 + eb];
-          p = c.plane;
+          p = c.plane_damping & 3;
           cx = c.x;
           cy = c.y;
         }

@@ -155,17 +155,28 @@ struct DevObmc {
   int32_t pass, rx, ry, w, h, mvx, mvy, filt, filt_v, slot;
 };
 
-/* one line of one deblocking edge: the four samples p1..q1 plus the taps the
- * filter may read, at `p1` + k * cross */
+/* one deblocking edge in 16 bytes: its first global line index (prefix),
+ * q0 of its first line, its length in lines, level, filter size, plane
+ * (bits 0-1) and direction (bit 2: vertical) */
 struct DevDeblock {
-  int32_t plane, x, y;   /* q0 of the first line                */
-  int32_t vertical, len, level, filter_size;
-  int32_t line0;         /* prefix: first global line index   */
+  int32_t line0;
+  uint16_t x, y;
+  uint16_t len;
+  uint8_t level, filter_size;
+  uint8_t plane_vertical;
+  uint8_t pad[3];
 };
+static_assert(sizeof(DevDeblock) == 16, "deblocking record layout");
 
+/* one CDEF block in 8 bytes: its position in the plane, strengths, plane
+ * (bits 0-1) and damping (above), direction (-1: searched on the device) */
 struct DevCdef {
-  int32_t plane, x, y, pri, sec, damping, dir, pad;
+  uint16_t x, y;
+  uint8_t pri, sec;
+  uint8_t plane_damping;
+  int8_t dir;
 };
+static_assert(sizeof(DevCdef) == 8, "CDEF record layout");
 
 struct DevLr {
   int32_t plane, x, y, w, h;
