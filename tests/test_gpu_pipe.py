@@ -80,3 +80,59 @@ def test_pipe_reports_a_refused_frame_and_goes_on():
             assert r.status == 0 and r.hash == want[f]
     pipe.close()
     dec.close()
+
+
+def _submit_clip(pipe, clip):
+    keep, rel, refs = pkg.slot_schedule(clip)
+    return [pipe.submit(pkg.wire_set_slots(clip.wire(f), refs[f]), keep=keep[f] >= 0, reserved=keep[f],
+                        release=rel[f]) for f in range(clip.frames)]
+
+
+def test_pipe_across_a_resolution_change():
+    """A stream whose frames grow from 426x240 to 854x480 and back: the
+    launches that must grow the context's buffers wait for the frames in
+    flight (av1h_plan_needs_idle), and every picture still matches."""
+    clips = [pkg.Clip("c1_mini", 7, width=426, height=240, frames=4),
+             pkg.Clip("c1_mini", 8, width=854, height=480, frames=4),
+             pkg.Clip("c1_mini", 9, width=426, height=240, frames=3)]
+    want = []
+    for c in clips:
+        keep, rel, refs = pkg.slot_schedule(c)
+        want += _plain_hashes(c, keep, rel, refs)
+    dec = pkg.HipDecoder()
+    dec.set_output_hash(True)
+    pipe = pkg.Pipe(dec, planner_threads=4, depth=8)
+    seqs = []
+    for c in clips:
+        seqs += _submit_clip(pipe, c)
+    pipe.wait()
+    for i, s in enumerate(seqs):
+        r = pipe.result(s)
+        assert r.status == 0 and r.hash == want[i], f"frame {i}: differs after a resolution change"
+    pipe.close()
+    dec.close()
+
+
+def test_pipe_with_stage_timing_takes_the_waiting_path():
+    """With stage timing on, every launch waits for its frame (the launch
+    reports that it would wait); the worker takes each result then, and the
+    pictures and per-stage times still come out."""
+    clip = pkg.Clip("c2_mini", 3)
+    keep, rel, refs = pkg.slot_schedule(clip)
+    want = _plain_hashes(clip, keep, rel, refs)
+    dec = pkg.HipDecoder()
+    dec.set_output_hash(True)
+    dec.set_stage_timing(True)
+    plan = dec.plan_wire(pkg.wire_set_slots(clip.wire(0), refs[0]))
+    assert dec.plan_needs_idle(plan)
+    dec.discard_plan(plan)
+    pipe = pkg.Pipe(dec, planner_threads=3, depth=6)
+    seqs = _submit_clip(pipe, clip)
+    pipe.wait()
+    for f, s in enumerate(seqs):
+        r = pipe.result(s)
+        assert r.status == 0 and r.hash == want[f]
+        assert r.wait_ms == 0.0  # nothing left to wait for after the launch
+    pipe.close()
+    assert dec.stage_times()["deblock"][1] >= clip.frames
+    dec.close()
