@@ -197,6 +197,7 @@ struct Ctx {
   bool has_pic[kSlots] = {};     /* the slot holds a decoded picture */
   Layout slot_lay[kSlots];
   int last_slot = -1;
+  bool last_grain = false;       /* the most recent frame had film grain */
   bool have_frame = false;
   Plan scratch_plan;
   av1h_stats stats;
@@ -1920,6 +1921,8 @@ int alloc_slot(Ctx* c) {
 
 int ensure_pack(Ctx* c, size_t bytes) {
   if (c->pack_cap < bytes) {
+    /* frames in flight may pack into the old buffer: they end first */
+    if (c->pack && c->inflight) HIP_OK(c, hipStreamSynchronize(c->stream));
     if (c->pack) (void)hipFree(c->pack);
     c->pack = nullptr;
     c->pack_cap = 0;
@@ -2469,6 +2472,7 @@ int batch_finish(Ctx* lead) {
       c->slot_lay[j.slot] = L;
     }
     c->last_slot = j.slot;
+    c->last_grain = j.plan->has_grain;
     c->nv12_current = tk.nv12[(size_t)(&j - js.data())] != 0;
     c->hash_valid = c->hash_on;
     if (c->hash_on) c->last_hash = hashes[&j - js.data()];
@@ -2520,19 +2524,33 @@ int run_plan(Ctx* c, Plan* plan, int keep, int reserved, const av1h_planes* out,
 /* the picture a slot names (the reference's gather rule,
  * cpu_backend.cpp:557-587) */
 const uint16_t* gather(Ctx* c, int slot, Layout* L) {
-  if (c->inflight) {
-    set_err(c, AV1H_ERR_INVALID, "a launched batch holds this context's frame: finish it first (av1h_batch_finish)");
+  /* frames in flight on another launcher's stream: nothing this context
+   * queues is ordered behind them */
+  if (c->inflight && c->inflight_lead != c) {
+    set_err(c, AV1H_ERR_INVALID, "a batch launched by another context holds this context's frames: finish it first");
     return nullptr;
   }
   if (slot >= 0) {
     if (slot >= c->pool_slots || !c->used[slot] || !c->has_pic[slot]) {
-      set_err(c, AV1H_ERR_INVALID, "invalid reference slot");
+      set_err(c, AV1H_ERR_INVALID, c->inflight && slot < c->pool_slots && c->used[slot]
+                                       ? "the frame of this slot is still in flight: finish it first"
+                                       : "invalid reference slot");
       return nullptr;
     }
-    if (slot != c->last_slot) {
+    /* an older slot: its picture is final and no frame in flight writes it
+     * (a frame in flight took its own slot at launch), so a pack or copy
+     * queued on this context's stream behind the frames in flight reads it
+     * as it is. So is the most
+     * recent one's when its frame had no film grain (the slot then holds
+     * the output picture too) */
+    if (slot != c->last_slot || (c->inflight && !c->last_grain)) {
       *L = c->slot_lay[slot];
       return c->pool + (size_t)slot * c->lay.size;
     }
+  }
+  if (c->inflight) {
+    set_err(c, AV1H_ERR_INVALID, "a launched batch holds this context's frame: finish it first (av1h_batch_finish)");
+    return nullptr;
   }
   if (!c->have_frame) {
     set_err(c, AV1H_ERR_INVALID, "no frame to convert");

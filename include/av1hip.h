@@ -556,8 +556,11 @@ int av1h_decode_planned_batch_on(av1h_ctx* launcher, av1h_ctx* const* ctxs, void
  * oldest launched batch and returns its status. At most two batches per
  * launcher are in flight (a third launch first finishes the oldest), so the
  * host prepares batch r + 1 while the device runs batch r. Until its batch
- * is finished a context's pictures cannot be read or packed, and only the
- * same launcher may put its next frame in a batch. */
+ * is finished, only the same launcher may put a context's next frame in a
+ * batch; the context's finished pictures can be read and packed (to_nv12,
+ * to_rgb, read_slot, the NV12 ring) while its frames are in flight only
+ * when it launched them itself (they are then queued behind them on its
+ * stream): an older slot, or the latest when that frame had no film grain. */
 int av1h_decode_planned_batch_async(av1h_ctx* launcher, av1h_ctx* const* ctxs, void* const* plans, const int* keep,
                                     const int* reserved, int count, int* slots_out);
 int av1h_batch_finish(av1h_ctx* launcher);

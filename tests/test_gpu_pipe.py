@@ -136,3 +136,33 @@ def test_pipe_with_stage_timing_takes_the_waiting_path():
     pipe.close()
     assert dec.stage_times()["deblock"][1] >= clip.frames
     dec.close()
+
+
+def test_older_slots_read_while_frames_are_in_flight():
+    """A player's display thread packs finished frames while the pipeline
+    still has later frames on the device: to_nv12 and read_slot of a
+    finished frame's slot (older ones, and the latest when it has no film
+    grain) are queued behind the frames in flight and give the same bytes as
+    the plain decode."""
+    clip = pkg.Clip("c1_mini", 12, frames=8, key_interval=1)  # independent frames, every one kept
+    w, _cw, h, _bd, _sx, _sy = pkg.wire_header(clip.wire(0))
+    ref = pkg.HipDecoder()
+    want_nv12, want_y = [], []
+    for f in range(clip.frames):
+        slot, planes = ref.decode_wire(clip.wire(f), keep=True, reserved=f)
+        want_nv12.append(bytes(ref.to_nv12(slot, 1024, w, h)))
+        want_y.append(planes[0].copy())
+    ref.close()
+    dec = pkg.HipDecoder()
+    pipe = pkg.Pipe(dec, planner_threads=2, depth=8)
+    for f in range(clip.frames):
+        pipe.submit(clip.wire(f), keep=True, reserved=f)
+    for f in range(clip.frames):
+        pipe.wait(f)
+        r = pipe.result(f)
+        assert r.status == 0 and r.slot == f
+        assert bytes(dec.to_nv12(f, 1024, w, h)) == want_nv12[f], f"frame {f}: NV12 differs"
+        assert np.array_equal(dec.read_slot(f, w, h)[0], want_y[f]), f"frame {f}: luma differs"
+    pipe.wait()
+    pipe.close()
+    dec.close()
