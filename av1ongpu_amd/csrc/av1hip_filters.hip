@@ -400,31 +400,32 @@ __device__ __forceinline__ bool cdef_tile_sample(FArgs a, const CdefBlockIn& c, 
   return true;
 }
 
-/* `batch` (1 to kCdefMaxBatch) sets of 256 / (W x H) blocks per
- * workgroup: every lane loads its block record (and direction) of each set,
- * the wave stages all its sets' tiles, and only then filters them, so that
- * a wave has the loads of all its sets in flight at once. */
-template <int W, int H, bool kLuma>
+/* `batch` (1 to kSets) sets of 256 / (W x H) blocks per workgroup: every
+ * lane loads its block record (and direction) of each set, the wave stages
+ * all its sets' tiles, and only then filters them, so that a wave has the
+ * loads of all its sets in flight at once. kSets bounds the registers the
+ * sets take (one set keeps the kernel at 8 waves per SIMD). */
+template <int W, int H, bool kLuma, int kSets>
 __device__ __forceinline__ void cdef_tiles(FArgs a, int first, int count, int grp, int batch, uint16_t* tile,
                                            const int (*dirs)[2][2]) {
   constexpr int S = W * H, TW = W + 4, TH = H + 4, kWave = 64 / S, kPer = 256 / S;
   constexpr int kPairs = TW / 2, kDw = kPairs * TH; /* 32-bit words per block tile */
   static_assert(kWave * TW * TH <= kCdefTileWave, "tiles of a wave exceed its LDS");
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  uint16_t* wt = tile + wave * kCdefTileWave * kCdefMaxBatch;
+  uint16_t* wt = tile + wave * kCdefTileWave * kSets;
   LayRef L = a.lay;
   const int sb = lane / S, sm = lane - sb * S, r = sm / W, k = sm - r * W;
-  CdefBlockIn q[kCdefMaxBatch];
-  bool live[kCdefMaxBatch];
+  CdefBlockIn q[kSets];
+  bool live[kSets];
 #pragma unroll
-  for (int u = 0; u < kCdefMaxBatch; ++u) {
+  for (int u = 0; u < kSets; ++u) {
     const int b0 = (grp * batch + u) * kPer + wave * kWave;
     live[u] = u < batch && b0 + sb < count;
     q[u] = CdefBlockIn{};
     if (live[u]) q[u] = cdef_block_in<kLuma>(a, first, b0 + sb);
   }
 #pragma unroll
-  for (int u = 0; u < kCdefMaxBatch; ++u) {
+  for (int u = 0; u < kSets; ++u) {
     if (u >= batch) break;
     const int b0 = (grp * batch + u) * kPer + wave * kWave;
     uint16_t* ut = wt + u * kCdefTileWave;
@@ -450,7 +451,7 @@ __device__ __forceinline__ void cdef_tiles(FArgs a, int first, int count, int gr
   }
   __syncthreads();
 #pragma unroll
-  for (int u = 0; u < kCdefMaxBatch; ++u) {
+  for (int u = 0; u < kSets; ++u) {
     int64_t off = 0;
     int val = 0;
     if (live[u] &&
@@ -462,11 +463,14 @@ __device__ __forceinline__ void cdef_tiles(FArgs a, int first, int count, int gr
   __syncthreads();
 }
 
-/* luma blocks (plane 0) or chroma blocks (planes 1 and 2) of every frame */
+/* luma blocks (plane 0) or chroma blocks (planes 1 and 2) of every frame,
+ * `batch` sets per workgroup, at most kSets (a kernel per bound: the
+ * registers of more sets would lower the occupancy of one) */
+template <int kSets>
 __global__ __launch_bounds__(256) void k_cdef(BatchDev B, int chroma, int batch) {
   /* the direction table in LDS: lanes index it by their own direction */
   __shared__ int s_dirs[8][2][2];
-  __shared__ __align__(16) uint16_t s_tile[4 * kCdefTileWave * kCdefMaxBatch];
+  __shared__ __align__(16) uint16_t s_tile[4 * kCdefTileWave * kSets];
   if (threadIdx.x < 32) (&s_dirs[0][0][0])[threadIdx.x] = (&kCdefDirs[0][0][0])[threadIdx.x];
   __syncthreads();
   const AV1H_CMEM int32_t* pre = kernel_prefix(B, (chroma ? kKCdefChroma : kKCdefLuma));
@@ -478,15 +482,15 @@ __global__ __launch_bounds__(256) void k_cdef(BatchDev B, int chroma, int batch)
     FArgs a = filter_args(B, j);
     const int nl = a.plan.n_cdef_luma, grp = v - pre[j];
     if (!chroma) {
-      cdef_tiles<8, 8, true>(a, 0, nl, grp, batch, s_tile, s_dirs);
+      cdef_tiles<8, 8, true, kSets>(a, 0, nl, grp, batch, s_tile, s_dirs);
       continue;
     }
     const int nc = a.plan.n_cdef - nl;
     switch (a.lay.ss_x * 2 + a.lay.ss_y) {
-      case 3: cdef_tiles<4, 4, false>(a, nl, nc, grp, batch, s_tile, s_dirs); break;
-      case 2: cdef_tiles<4, 8, false>(a, nl, nc, grp, batch, s_tile, s_dirs); break;
-      case 1: cdef_tiles<8, 4, false>(a, nl, nc, grp, batch, s_tile, s_dirs); break;
-      default: cdef_tiles<8, 8, false>(a, nl, nc, grp, batch, s_tile, s_dirs); break;
+      case 3: cdef_tiles<4, 4, false, kSets>(a, nl, nc, grp, batch, s_tile, s_dirs); break;
+      case 2: cdef_tiles<4, 8, false, kSets>(a, nl, nc, grp, batch, s_tile, s_dirs); break;
+      case 1: cdef_tiles<8, 4, false, kSets>(a, nl, nc, grp, batch, s_tile, s_dirs); break;
+      default: cdef_tiles<8, 8, false, kSets>(a, nl, nc, grp, batch, s_tile, s_dirs); break;
     }
   }
 }
@@ -991,7 +995,14 @@ hipError_t launch_filter_kernel(const BatchDev& b, int k, int total, int cap, in
     case kKDeblockH: hipLaunchKernelGGL(k_deblock, grid, blk, 0, s, b, k == kKDeblockH ? 1 : 0); break;
     case kKCdefDir: hipLaunchKernelGGL(k_cdef_dir, grid, blk, 0, s, b); break;
     case kKCdefLuma:
-    case kKCdefChroma: hipLaunchKernelGGL(k_cdef, grid, blk, 0, s, b, k == kKCdefChroma ? 1 : 0, cdef_batch); break;
+    case kKCdefChroma:
+      if (cdef_batch <= 1)
+        hipLaunchKernelGGL(k_cdef<1>, grid, blk, 0, s, b, k == kKCdefChroma ? 1 : 0, 1);
+      else if (cdef_batch == 2)
+        hipLaunchKernelGGL(k_cdef<2>, grid, blk, 0, s, b, k == kKCdefChroma ? 1 : 0, 2);
+      else
+        hipLaunchKernelGGL(k_cdef<kCdefMaxBatch>, grid, blk, 0, s, b, k == kKCdefChroma ? 1 : 0, cdef_batch);
+      break;
     case kKUpscale: hipLaunchKernelGGL(k_upscale, grid, blk, 0, s, b); break;
     case kKLr: hipLaunchKernelGGL(k_lr, grid, blk, 0, s, b); break;
     case kKNv12: hipLaunchKernelGGL(k_nv12_batch, grid, blk, 0, s, b); break;

@@ -44,7 +44,7 @@ METRIC = "decoded Mpixel/s on 1 MI355X, batched vs plain, bit-exact; p99 frame l
 HBM_PEAK_GBS = 8000.0
 NV12_PITCH = 2048
 # the kernel that dominates each timed stage (the stage's events bracket it)
-STAGE_KERNEL = {"inter": "k_mc", "queue": "k_queue", "deblock": "k_deblock", "cdef": "k_cdef",
+STAGE_KERNEL = {"inter": "k_mc", "queue": "k_queue", "deblock": "k_deblock", "cdef": "k_cdef<1>",
                 "superres": "k_upscale", "lr": "k_lr", "store": "k_copy", "grain": "k_grain_apply"}
 WORKLOADS = {
     "c2": "1920x1080 10-bit 4:2:0, 60 frames, hierarchical GOP 16 with 3 references, compound "
