@@ -2,8 +2,8 @@
 
 Every kernel of a batch runs as a persistent grid of (workgroups per compute
 unit x compute units) workgroups that walk the batch's virtual workgroups;
-the dependency queue has its own grid; CDEF can filter 1 to 4 blocks per
-thread. The best values depend on the batch (frame sizes, tools) and the
+the dependency queue has its own grid; CDEF can stage 1 to 4 sets of blocks per
+workgroup. The best values depend on the batch (frame sizes, tools) and the
 device, so they are measured: coordinate descent over the parameters, each
 candidate timed on the caller's own batches (replayed, captured launches
 included), keeping a change only when it is faster by more than `tol`.
@@ -39,7 +39,8 @@ def tune(decoders, batches, params=None, reps=2, tol=0.01, log=None):
     seconds)."""
     lead = decoders[0]
     cur = lead.tuning()
-    names = params or list(cur)
+    # CDEF's chroma blocks run in the luma launch: its own shape does nothing
+    names = params or [n for n in cur if n != "cdef_chroma"]
     base = _time_batches(decoders, batches, reps)
     best_t = base
     for name in names:

@@ -2246,7 +2246,11 @@ int batch_launch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
   static_assert(kKernels < 64, "kernel set mask");
   auto has = [&](int k) { return ((live >> k) & 1) != 0; };
   auto run = [&](int k, hipStream_t st) -> hipError_t {
-    const int total = use_graph ? (has(k) ? kAnyTotal : 0) : total_of(k);
+    /* CDEF's luma launch also filters the chroma blocks */
+    if (k == kKCdefChroma) return hipSuccess;
+    const bool work = k == kKCdefLuma ? has(kKCdefLuma) || has(kKCdefChroma) : has(k);
+    const int total = use_graph ? (work ? kAnyTotal : 0)
+                                : total_of(k) + (k == kKCdefLuma ? total_of(kKCdefChroma) : 0);
     if (total <= 0) return hipSuccess;
     ++launches;
     const int cap = lead->cus * lead->tune.v[k];

@@ -463,29 +463,35 @@ __device__ __forceinline__ void cdef_tiles(FArgs a, int first, int count, int gr
   __syncthreads();
 }
 
-/* luma blocks (plane 0) or chroma blocks (planes 1 and 2) of every frame,
- * `batch` sets per workgroup, at most kSets (a kernel per bound: the
- * registers of more sets would lower the occupancy of one) */
+/* The luma blocks (plane 0) and then the chroma blocks (planes 1 and 2) of
+ * every frame in one launch: the luma groups of every frame come first, then
+ * the chroma groups (both read the deblocked snapshot and the direction
+ * map, and write different planes, so nothing orders them). `batch` sets
+ * per workgroup, at most kSets (a kernel per bound: the registers of more
+ * sets would lower the occupancy of one). */
 template <int kSets>
-__global__ __launch_bounds__(256) void k_cdef(BatchDev B, int chroma, int batch) {
+__global__ __launch_bounds__(256) void k_cdef(BatchDev B, int batch) {
   /* the direction table in LDS: lanes index it by their own direction */
   __shared__ int s_dirs[8][2][2];
   __shared__ __align__(16) uint16_t s_tile[4 * kCdefTileWave * kSets];
   if (threadIdx.x < 32) (&s_dirs[0][0][0])[threadIdx.x] = (&kCdefDirs[0][0][0])[threadIdx.x];
   __syncthreads();
-  const AV1H_CMEM int32_t* pre = kernel_prefix(B, (chroma ? kKCdefChroma : kKCdefLuma));
-  const int total = pre[B.njobs];
+  const AV1H_CMEM int32_t* pre_l = kernel_prefix(B, kKCdefLuma);
+  const AV1H_CMEM int32_t* pre_c = kernel_prefix(B, kKCdefChroma);
+  const int tl = pre_l[B.njobs], total = tl + pre_c[B.njobs];
   for (int base = 0; base < total; base += gridDim.x) {
     const int v = xcd_group(base, total);
     if (v < 0) continue;
-    const int j = find_job(pre, B.njobs, v);
-    FArgs a = filter_args(B, j);
-    const int nl = a.plan.n_cdef_luma, grp = v - pre[j];
-    if (!chroma) {
-      cdef_tiles<8, 8, true, kSets>(a, 0, nl, grp, batch, s_tile, s_dirs);
+    if (v < tl) {
+      const int j = find_job(pre_l, B.njobs, v);
+      FArgs a = filter_args(B, j);
+      cdef_tiles<8, 8, true, kSets>(a, 0, a.plan.n_cdef_luma, v - pre_l[j], batch, s_tile, s_dirs);
       continue;
     }
-    const int nc = a.plan.n_cdef - nl;
+    const int vc = v - tl;
+    const int j = find_job(pre_c, B.njobs, vc);
+    FArgs a = filter_args(B, j);
+    const int nl = a.plan.n_cdef_luma, nc = a.plan.n_cdef - nl, grp = vc - pre_c[j];
     switch (a.lay.ss_x * 2 + a.lay.ss_y) {
       case 3: cdef_tiles<4, 4, false, kSets>(a, nl, nc, grp, batch, s_tile, s_dirs); break;
       case 2: cdef_tiles<4, 8, false, kSets>(a, nl, nc, grp, batch, s_tile, s_dirs); break;
@@ -995,13 +1001,13 @@ hipError_t launch_filter_kernel(const BatchDev& b, int k, int total, int cap, in
     case kKDeblockH: hipLaunchKernelGGL(k_deblock, grid, blk, 0, s, b, k == kKDeblockH ? 1 : 0); break;
     case kKCdefDir: hipLaunchKernelGGL(k_cdef_dir, grid, blk, 0, s, b); break;
     case kKCdefLuma:
-    case kKCdefChroma:
+      /* luma and chroma in one launch (kKCdefLuma's): `total` counts both */
       if (cdef_batch <= 1)
-        hipLaunchKernelGGL(k_cdef<1>, grid, blk, 0, s, b, k == kKCdefChroma ? 1 : 0, 1);
+        hipLaunchKernelGGL(k_cdef<1>, grid, blk, 0, s, b, 1);
       else if (cdef_batch == 2)
-        hipLaunchKernelGGL(k_cdef<2>, grid, blk, 0, s, b, k == kKCdefChroma ? 1 : 0, 2);
+        hipLaunchKernelGGL(k_cdef<2>, grid, blk, 0, s, b, 2);
       else
-        hipLaunchKernelGGL(k_cdef<kCdefMaxBatch>, grid, blk, 0, s, b, k == kKCdefChroma ? 1 : 0, cdef_batch);
+        hipLaunchKernelGGL(k_cdef<kCdefMaxBatch>, grid, blk, 0, s, b, cdef_batch);
       break;
     case kKUpscale: hipLaunchKernelGGL(k_upscale, grid, blk, 0, s, b); break;
     case kKLr: hipLaunchKernelGGL(k_lr, grid, blk, 0, s, b); break;

@@ -451,7 +451,9 @@ int av1h_set_graphs(av1h_ctx* ctx, int mode);
 /* Launch shapes of the batches this context leads, the parameters an
  * autotuner searches (av1ongpu_amd/autotune.py does): for every kernel the
  * persistent workgroups per compute unit of its grid, the queue's, and the
- * CDEF blocks each thread filters together (1..4). av1h_tune_params gives
+ * sets of CDEF blocks a workgroup stages together (1..4); CDEF's chroma
+ * blocks run in the luma launch, so "cdef_chroma" has no effect.
+ * av1h_tune_params gives
  * their count and av1h_tune_name their names; setting them drops the
  * captured launch sequences. No counterpart in the reference (fixed launch
  * shapes, av1_cuda.cu). */
