@@ -77,7 +77,10 @@ __device__ __forceinline__ void row_pass(int* tmp, int lane, int w, int h, int t
                          int rect) {
   const int y = lane;
   if (y >= h) return;
-  const int cs = w < 32 ? w : 32;
+  /* a row holds min(N, 32) coefficients (AV1 codes none beyond 32): the
+   * other inputs are constant zeros, which folds the first stages of the
+   * 64-point transform */
+  constexpr int kCs = N < 32 ? N : 32;
   const int st = w + 1;
   const int cr = (1 << (max(bd + 6, 16) - 1));
   if (y >= 32) {
@@ -87,7 +90,7 @@ __device__ __forceinline__ void row_pass(int* tmp, int lane, int w, int h, int t
   }
   int t[N];
 #pragma unroll
-  for (int x = 0; x < N; ++x) t[x] = x < cs ? tmp[y * st + x] : 0;
+  for (int x = 0; x < N; ++x) t[x] = x < kCs ? tmp[y * st + x] : 0;
   if (rect) {
 #pragma unroll
     for (int x = 0; x < N; ++x) t[x] = itx::r12((int64_t)t[x] * 2896);
@@ -113,8 +116,9 @@ __device__ __forceinline__ void col_pass(const int* tmp, int lane, int w, int tx
   if (x >= w) return;
   const int st = w + 1;
   int t[M];
+  /* rows from 32 on are zero after the row pass (no coefficients there) */
 #pragma unroll
-  for (int i = 0; i < M; ++i) t[i] = tmp[i * st + x];
+  for (int i = 0; i < M; ++i) t[i] = i < 32 ? tmp[i * st + x] : 0;
   if (lossless) {
     if constexpr (M == 4) itx::iwht(t, 0);
   } else {
