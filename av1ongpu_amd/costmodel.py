@@ -7,7 +7,7 @@ superblock. Those times and the superblocks' syntax features
 (av1h_plan_sb_features) are the samples the network (CostModel, trained by
 the HIP kernels) learns from. A trained model orders the queue of a plan
 through av1h_plan_schedule_model; the static dependency-level order stays the
-default and the reference the model is judged against.
+default and the baseline the model is compared with.
 """
 import time
 

@@ -39,7 +39,7 @@ extern "C" {
 
 #define AV1H_OK 0
 #define AV1H_ERR_INVALID -1      /* bad argument or malformed syntax          */
-#define AV1H_ERR_UNSUPPORTED -2  /* a tool this build does not reconstruct    */
+#define AV1H_ERR_UNSUPPORTED -2  /* a tool this library does not reconstruct  */
 #define AV1H_ERR_NO_DEVICE -3    /* no usable gfx950 device                    */
 #define AV1H_ERR_DEVICE -4       /* a HIP call failed                          */
 #define AV1H_ERR_SLOTS -5        /* the reference pool is exhausted            */
