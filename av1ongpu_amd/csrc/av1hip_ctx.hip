@@ -19,6 +19,9 @@
 #include <mutex>
 #include <string>
 #include <deque>
+#include <memory>
+#include <new>
+#include <type_traits>
 #include <vector>
 
 #define AV1T_TABLE_QUAL static const
@@ -494,10 +497,35 @@ void sb_block(const Layout& L, const DevBlock& d, const DevTx* tx, Plan* plan, H
 void sb_queued(const Layout& L, const DevBlock& d, int qi, Plan* plan, Host* H);
 void sb_finish(const Host& H, const av1h_frame_header& hdr, const Layout& L, Plan* plan);
 
+/* An allocator whose value-initialising construct (emplace_back() with no
+ * arguments) leaves a trivially constructible element uninitialised: the
+ * planner writes each block descriptor's fields itself, and only the ones
+ * its readers use. */
+template <typename T>
+struct NoInitAlloc : std::allocator<T> {
+  template <typename U>
+  struct rebind {
+    using other = NoInitAlloc<U>;
+  };
+  NoInitAlloc() = default;
+  template <typename U>
+  NoInitAlloc(const NoInitAlloc<U>&) {}
+  template <typename U>
+  void construct(U* p) {
+    static_assert(std::is_trivially_default_constructible<U>::value, "default-initialised element");
+    ::new (static_cast<void*>(p)) U;
+  }
+  template <typename U, typename... Args>
+  void construct(U* p, Args&&... args) {
+    ::new (static_cast<void*>(p)) U(std::forward<Args>(args)...);
+  }
+};
+using BlockVec = std::vector<DevBlock, NoInitAlloc<DevBlock>>;
+
 struct Host {
   std::vector<DevDq> dq; /* a frame with coded levels: one per transform */
-  std::vector<DevBlock> blocks;
-  std::vector<DevBlock> dev_blocks; /* the ones the device reads */
+  BlockVec blocks;
+  BlockVec dev_blocks; /* the ones the device reads */
   bool dev_all = false;             /* ... which are all of them: `blocks` goes */
   std::vector<DevTx> tx;
   std::vector<int32_t> inter_list, queue_list, special_list;
@@ -665,7 +693,7 @@ int resolve_slot(const av1h_frame_header& h, int ref) {
  * before d (index into queue_list), -1 elsewhere; `queued` flags the cells
  * of every queued block of the plane. */
 bool packed_edges(const Layout& L, DevBlock& d, const std::vector<int32_t>& owner, const std::vector<char>& queued,
-                  std::vector<DevBlock>& blocks, const std::vector<int32_t>& queue_list) {
+                  BlockVec& blocks, const std::vector<int32_t>& queue_list) {
   if (d.kind != kIntra || (d.flags & (kPalette | kIntraBC)) || (d.mode == AV1H_PRED_CFL && d.plane > 0)) return false;
   const int p = d.plane, sh = L.cell_shift[p], cell = 1 << sh;
   const bool ha = (d.flags & kHaveAbove) != 0, hl = (d.flags & kHaveLeft) != 0;
@@ -787,9 +815,16 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
       return set_err(err, AV1H_ERR_UNSUPPORTED, "block %d: bit depth differs from the frame", i);
     if (b.tx_first < 0 || b.tx_count < 0 || b.tx_first + (int64_t)b.tx_count > f.num_transforms)
       return set_err(err, AV1H_ERR_INVALID, "block %d: transform range", i);
-    /* built in place (room for every block is reserved) */
+    /* built in place (room for every block is reserved). A plain inter
+     * block (its prediction done by the motion compensation tasks, which read
+     * its descriptor up to the mask kind) has only that head cleared; every
+     * other block's descriptor reaches the device and is cleared whole. */
     DevBlock& d = H->blocks.emplace_back();
-    std::memset(&d, 0, sizeof(d));
+    const bool plain_inter = !b.is_intra && !b.intrabc && !b.interintra && b.compound != AV1H_COMPOUND_WEDGE &&
+                             b.compound != AV1H_COMPOUND_DIFFWTD && b.obmc_count <= 0 && !b.warp[0] && !b.warp[1] &&
+                             !b.mask_warp[0] && !b.mask_warp[1];
+    static_assert(offsetof(DevBlock, mask_kind) + sizeof(int32_t) <= 128, "the head of a plain inter block");
+    std::memset(&d, 0, plain_inter ? 128 : sizeof(d));
     d.x = b.x;
     d.y = b.y;
     d.w = b.w;
@@ -1552,7 +1587,7 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
     H->part_of[k] = (int)H->parts.size();
     H->parts.push_back(Host::Part{src, bytes, fill, 0});
   };
-  const std::vector<DevBlock>& dblk = H->dev_all ? H->blocks : H->dev_blocks;
+  const BlockVec& dblk = H->dev_all ? H->blocks : H->dev_blocks;
   add(kPartBlocks, dblk.data(), dblk.size() * sizeof(DevBlock));
   add(kPartTx, H->tx.data(), H->tx.size() * sizeof(DevTx));
   add(kPartInter, H->inter_list.data(), H->inter_list.size() * 4);
