@@ -138,7 +138,7 @@ struct BatchTicket {
   std::vector<JobSpec> js;
   std::vector<char> nv12; /* per frame: NV12 packed */
   int launches = 0;
-  bool any_hash = false, live = false;
+  bool any_hash = false, live = false, graph = false;
   double t0 = 0, t_sub = 0;
 };
 
@@ -2218,21 +2218,26 @@ int batch_launch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
     lead->batch_cap[b] = 0;
     const size_t cap = std::max(bytes_tab + bytes_tab / 4, (size_t)65536);
     HIP_OK(lead, hipMalloc(&lead->batch_dev[b], cap));
-    HIP_OK(lead, hipHostMalloc(&lead->batch_pin[b], cap, hipHostMallocDefault));
+    HIP_OK(lead, hipHostMalloc(&lead->batch_pin[b], kTableHead + cap, hipHostMallocDefault));
     lead->batch_cap[b] = cap;
   }
   for (int k = 0; k < 4; ++k)
     if (!lead->bev[b][k])
-      HIP_OK(lead, hipEventCreateWithFlags(&lead->bev[b][k], k == 3 ? hipEventDisableTiming : hipEventDefault));
+      HIP_OK(lead, hipEventCreateWithFlags(&lead->bev[b][k], hipEventDefault));
   if (!lead->rb_pin[b]) HIP_OK(lead, hipHostMalloc(&lead->rb_pin[b], 64 + kMaxBatch * 8, hipHostMallocDefault));
-  uint8_t* pin = static_cast<uint8_t*>(lead->batch_pin[b]);
+  uint32_t* rb_err = static_cast<uint32_t*>(lead->rb_pin[b]);
+  uint64_t* rb_hash = reinterpret_cast<uint64_t*>(static_cast<uint8_t*>(lead->rb_pin[b]) + 64);
+  /* [byte count][..][the table at kTableHead], fetched by the batch's
+   * first launch */
+  *static_cast<uint64_t*>(lead->batch_pin[b]) = (bytes_tab + 7) & ~(size_t)7;
+  uint8_t* pin = static_cast<uint8_t*>(lead->batch_pin[b]) + kTableHead;
   std::memcpy(pin, jobs.data(), sizeof(FrameJob) * n);
   std::memcpy(pin + o_prefix, prefix.data(), prefix.size() * 4);
   if (n > 1) merge_queues(js, reinterpret_cast<uint32_t*>(pin + o_queue));
   /* the plans' uploads, and everything queued on the other contexts'
    * streams (buffer clears), come first */
   for (JobSpec& j : js)
-    if (j.plan->ready) HIP_OK(lead, hipStreamWaitEvent(s, j.plan->ready, 0));
+    if (j.plan->ready && hipEventQuery(j.plan->ready) != hipSuccess) HIP_OK(lead, hipStreamWaitEvent(s, j.plan->ready, 0));
   for (JobSpec& j : js)
     if (j.c != lead && j.c->dirty) {
       HIP_OK(lead, hipEventRecord(j.c->ev[3], j.c->stream));
@@ -2240,7 +2245,6 @@ int batch_launch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
       j.c->dirty = false;
     }
   uint8_t* tab = static_cast<uint8_t*>(lead->batch_dev[b]);
-  HIP_OK(lead, hipMemcpyAsync(tab, pin, bytes_tab, hipMemcpyHostToDevice, s));
   BatchDev B;
   B.jobs = reinterpret_cast<const FrameJob*>(tab);
   B.prefix = reinterpret_cast<const int32_t*>(tab + o_prefix);
@@ -2328,6 +2332,7 @@ int batch_launch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
     e = (call);          \
     if (e != hipSuccess) return e; \
   } while (0)
+    SEQ(launch_fetch_table(lead->batch_pin[b], tab, n, s));
     if (side_itx || side_grain) {
       SEQ(hipEventRecord(lead->ev_side_go, s));
       SEQ(hipStreamWaitEvent(lead->side, lead->ev_side_go, 0));
@@ -2368,6 +2373,7 @@ int batch_launch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
       SEQ(launch_zero64(lead->hash_dev, n, s));
       SEQ(run(kKHash, s));
     }
+    SEQ(launch_readback(lead->counters + 1, has(kKHash) ? lead->hash_dev : nullptr, n, rb_err, rb_hash, s));
 #undef SEQ
     return e;
   };
@@ -2399,11 +2405,10 @@ int batch_launch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
     }
     HIP_OK(lead, hipEventRecord(lead->bev[b][0], s));
     HIP_OK(lead, hipGraphLaunch(exec, s));
-    HIP_OK(lead, hipEventRecord(lead->bev[b][1], s));
   } else {
     HIP_OK(lead, sequence());
+    HIP_OK(lead, hipEventRecord(lead->bev[b][2], s));
   }
-  HIP_OK(lead, hipEventRecord(lead->bev[b][2], s));
   const double t_sub = now_ms();
   /* read back */
   for (JobSpec& j : js) {
@@ -2421,12 +2426,6 @@ int batch_launch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
     }
     HIP_OK(lead, hipMemcpyAsync(c->host_pin, c->frame, host_need, hipMemcpyDeviceToHost, s));
   }
-  /* into pinned memory: a copy to pageable memory is staged through a
-   * runtime buffer, which costs more than the copy */
-  uint32_t* err_pin = static_cast<uint32_t*>(lead->rb_pin[b]);
-  uint64_t* hashes = reinterpret_cast<uint64_t*>(static_cast<uint8_t*>(lead->rb_pin[b]) + 64);
-  HIP_OK(lead, hipMemcpyAsync(err_pin, &lead->counters[1], 4, hipMemcpyDeviceToHost, s));
-  if (any_hash) HIP_OK(lead, hipMemcpyAsync(hashes, lead->hash_dev, (size_t)n * 8, hipMemcpyDeviceToHost, s));
   HIP_OK(lead, hipEventRecord(lead->bev[b][3], s));
   /* the kept slots are taken now; their pictures count once the batch is done */
   for (JobSpec& j : js)
@@ -2445,6 +2444,7 @@ int batch_launch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
   for (int i = 0; i < n; ++i) tk.nv12[(size_t)i] = jobs[(size_t)i].f.nv12 != nullptr;
   tk.launches = launches;
   tk.any_hash = any_hash;
+  tk.graph = use_graph;
   tk.t0 = t0;
   tk.t_sub = t_sub;
   tk.live = true;
@@ -2490,8 +2490,13 @@ int batch_finish(Ctx* lead) {
   const int launches = tk.launches;
   const double t0 = tk.t0, t_sub = tk.t_sub;
   float recon = 0, filt = 0;
-  (void)hipEventElapsedTime(&recon, lead->bev[b][0], lead->bev[b][1]);
-  (void)hipEventElapsedTime(&filt, lead->bev[b][1], lead->bev[b][2]);
+  if (tk.graph) {
+    /* one captured launch: its whole time counts as reconstruction */
+    (void)hipEventElapsedTime(&recon, lead->bev[b][0], lead->bev[b][3]);
+  } else {
+    (void)hipEventElapsedTime(&recon, lead->bev[b][0], lead->bev[b][1]);
+    (void)hipEventElapsedTime(&filt, lead->bev[b][1], lead->bev[b][2]);
+  }
   if (lead->timing)
     for (int st = 0; st < AV1H_STAGES; ++st)
       if (lead->stage_hit[st]) {

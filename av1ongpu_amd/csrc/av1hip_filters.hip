@@ -957,6 +957,44 @@ __global__ void k_zero64(uint64_t* p, int n) {
   if (i < n) p[i] = 0;
 }
 
+/* The batch table (jobs, kernel prefixes, merged queue) copied from its
+ * pinned staging buffer into device memory as the batch's first launch: the
+ * byte count is the staging buffer's first word and the table follows at
+ * kTableHead, so one captured launch serves every table size, and no copy
+ * command of its own sits between two batches on the stream. The host
+ * rewrites the staging buffer between the batches that use it: the loads
+ * are system scope, past the device's caches. */
+__global__ void k_fetch_table(const uint64_t* src, uint64_t* dst) {
+  const uint64_t words = __hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) >> 3;
+  const uint64_t* from = src + kTableHead / 8;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += 4 * stride) {
+    uint64_t v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      v[k] = i + k * stride < words ? __hip_atomic_load(from + i + k * stride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)
+                                    : 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (i + k * stride < words) dst[i + k * stride] = v[k];
+  }
+}
+
+/* The batch's results written into its pinned read-back buffer as its last
+ * launch, instead of copies queued behind the batch: the error word (then
+ * cleared for the next batch on the stream) and, with `hashes`, the n
+ * pictures' checksums; system-scope stores, which the host reads once the
+ * batch's completion event has fired. */
+__global__ void k_readback(uint32_t* err, const uint64_t* hashes, int n, uint32_t* out_err, uint64_t* out_hash) {
+  if (threadIdx.x == 0) {
+    __hip_atomic_store(out_err, *err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    *err = 0;
+  }
+  if (hashes)
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+      __hip_atomic_store(out_hash + i, hashes[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 struct ColorM {
   int m[8];
   int o[3];
@@ -1024,6 +1062,19 @@ hipError_t launch_filter_kernel(const BatchDev& b, int k, int total, int cap, in
 
 hipError_t launch_zero64(uint64_t* p, int n, hipStream_t s) {
   hipLaunchKernelGGL(k_zero64, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_fetch_table(const void* pin, void* dev, int frames, hipStream_t s) {
+  /* one frame's table is a few KB; a merged queue of a large batch some MB */
+  hipLaunchKernelGGL(k_fetch_table, dim3(frames > 1 ? 256 : 1), dim3(256), 0, s, static_cast<const uint64_t*>(pin),
+                     static_cast<uint64_t*>(dev));
+  return hipGetLastError();
+}
+
+hipError_t launch_readback(uint32_t* err, const uint64_t* hashes, int n, uint32_t* out_err, uint64_t* out_hash,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(k_readback, dim3(1), dim3(64), 0, s, err, hashes, n, out_err, out_hash);
   return hipGetLastError();
 }
 

@@ -445,6 +445,13 @@ int upscale_tiles(const Layout& lay, int pictures);
 int grain_tiles(const Layout& lay);
 /* clears n 64 bit words (a kernel, so captured launches carry no memset) */
 hipError_t launch_zero64(uint64_t* p, int n, hipStream_t s);
+/* The batch table from its pinned staging buffer ([byte count][..][table at
+ * kTableHead]) into device memory; the error word and checksums back into
+ * the pinned read-back buffer. */
+constexpr size_t kTableHead = 256;
+hipError_t launch_fetch_table(const void* pin, void* dev, int frames, hipStream_t s);
+hipError_t launch_readback(uint32_t* err, const uint64_t* hashes, int n, uint32_t* out_err, uint64_t* out_hash,
+                           hipStream_t s);
 /* picture conversions outside a decode */
 hipError_t launch_nv12(const Layout& lay, const uint16_t* src, uint8_t* out, int pitch, hipStream_t s);
 hipError_t launch_rgb(const Layout& lay, const uint16_t* src, uint8_t* out, const int m[8],
