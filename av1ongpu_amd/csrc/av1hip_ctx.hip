@@ -2737,6 +2737,36 @@ int av1h_describe_device(int device_index, av1h_device_info* out) {
   return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0 ? AV1H_OK : AV1H_ERR_UNSUPPORTED;
 }
 
+namespace {
+/* Planning state, whose arrays keep their capacity from frame to frame,
+ * pooled across the planning threads: a frame takes the pooled state sized
+ * for the fewest blocks that still holds its own, else the largest, so a
+ * key frame finds the arrays an earlier key frame grew (and whose pages it
+ * touched) on whichever thread planned it. Grown on first touch, a 1080p
+ * key frame's arrays cost about 3.5 ms more planning. */
+std::mutex s_host_mu;
+std::vector<Host*> s_host_free;
+
+Host* take_host(int64_t blocks) {
+  std::lock_guard<std::mutex> lk(s_host_mu);
+  if (s_host_free.empty()) return new Host;
+  size_t best = 0;
+  for (size_t i = 1; i < s_host_free.size(); ++i) {
+    const int64_t cb = (int64_t)s_host_free[best]->blocks.capacity(), ci = (int64_t)s_host_free[i]->blocks.capacity();
+    const bool fits_i = ci >= blocks, fits_b = cb >= blocks;
+    if ((fits_i && (!fits_b || ci < cb)) || (!fits_i && !fits_b && ci > cb)) best = i;
+  }
+  Host* h = s_host_free[best];
+  s_host_free.erase(s_host_free.begin() + (ptrdiff_t)best);
+  return h;
+}
+
+void give_host(Host* h) {
+  std::lock_guard<std::mutex> lk(s_host_mu);
+  s_host_free.push_back(h);
+}
+}  // namespace
+
 int av1h_decode(av1h_ctx* h, const av1h_frame_desc* f, int keep, int reserved, const av1h_planes* out,
                 av1h_frame_info* info, int* slot_out) {
   if (!h || !f) return AV1H_ERR_INVALID;
@@ -2744,10 +2774,10 @@ int av1h_decode(av1h_ctx* h, const av1h_frame_desc* f, int keep, int reserved, c
   std::lock_guard<std::mutex> lk(c->mu);
   (void)hipSetDevice(c->device);
   const double t0 = now_ms();
-  Host host;
-  int rc = flatten(*f, &c->scratch_plan, &host, &c->err);
-  if (rc != AV1H_OK) return rc;
-  rc = upload(c, &c->scratch_plan, host);
+  Host* host = take_host(f->num_blocks);
+  int rc = flatten(*f, &c->scratch_plan, host, &c->err);
+  if (rc == AV1H_OK) rc = upload(c, &c->scratch_plan, *host);
+  give_host(host);
   if (rc != AV1H_OK) return rc;
   return run_plan(c, &c->scratch_plan, keep, reserved, out, info, slot_out, t0);
 }
@@ -2774,14 +2804,13 @@ int av1h_plan_frame(av1h_ctx* h, const av1h_frame_desc* f, void** plan_out) {
   Ctx* c = &h->c;
   (void)hipSetDevice(c->device);
   Plan* p = new Plan;
-  /* one per planning thread: its arrays keep their capacity from frame to
-   * frame */
-  static thread_local Host host;
+  Host* host = take_host(f->num_blocks);
   std::string err;
   const double t0 = now_ms();
-  int rc = flatten(*f, p, &host, &err);
+  int rc = flatten(*f, p, host, &err);
   const double t1 = now_ms();
-  if (rc == AV1H_OK) rc = upload_plan(c, p, host, &err);
+  if (rc == AV1H_OK) rc = upload_plan(c, p, *host, &err);
+  give_host(host);
   p->flatten_ms = t1 - t0;
   p->upload_ms = now_ms() - t1;
   if (rc != AV1H_OK) {
