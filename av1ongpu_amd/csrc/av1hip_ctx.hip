@@ -96,7 +96,6 @@ struct Plan {
   DevGrain grain;
   void* dbuf = nullptr;
   size_t dbytes = 0;
-  hipEvent_t ready = nullptr; /* its upload has ended (plans of av1h_plan_frame) */
   bool uses_lr = false, upscaled = false, has_grain = false;
   int blocks = 0;
   double flatten_ms = 0, upload_ms = 0; /* host time of the plan's making */
@@ -157,7 +156,6 @@ constexpr int kPackRing = 3;
 
 /* a pinned staging buffer and the stream its uploads run on */
 struct Stager {
-  hipEvent_t done = nullptr; /* its last copy has ended: the buffer is free */
   void* pin = nullptr;
   size_t cap = 0;
 };
@@ -247,17 +245,14 @@ struct Ctx {
   int64_t stage_n[AV1H_STAGES] = {};
   int64_t stage_bytes[AV1H_STAGES] = {};
   PackEntry ring[kPackRing];     /* streamed NV12 packs                   */
-  /* pre-planning (any thread): stagers and device buffers for reuse; every
-   * plan's upload goes on the one upload stream. A stream per stager would
-   * share hardware queues with the decode stream (a process gets few), and
-   * a decode queued behind an upload's wait in a shared queue stalls. */
-  hipStream_t upload = nullptr;
+  /* pre-planning (any thread): stagers, upload streams and device buffers
+   * for reuse */
   std::mutex plan_mu;
   size_t stage_hint = 0; /* the largest plan staged so far */
   std::vector<Stager*> stagers;
+  std::vector<hipStream_t> uploads; /* copy-only streams, one per planning thread at a time */
   std::vector<std::pair<size_t, void*>> plan_bufs;
   size_t plan_buf_bytes = 0;
-  std::vector<hipEvent_t> free_events; /* for the plans' upload events */
 };
 
 int set_err(std::string* e, int code, const char* fmt, ...) {
@@ -1778,55 +1773,51 @@ void give_buffer(Ctx* c, void* p, size_t n) {
   (void)hipFree(p);
 }
 
-/* The plan's upload is queued on a stager's stream and the planning thread
- * goes on: the plan's `ready` event ends the copy, every decode waits for it
- * on the device, and the stager is taken again once its copy has ended. */
+/* The plan's upload: staged into a pinned buffer, copied on a stream that
+ * carries nothing but such copies, and waited for by the planning thread, so
+ * the plan is complete when av1h_plan_frame returns and no decode waits for
+ * it on the device. An event recorded behind an upload instead (the decode's
+ * wait for it) is a barrier in a hardware queue until the copy ends, and
+ * the decode work that shares that queue (a process gets few) waited behind
+ * it: 25 us per 1080p frame with eight threads planning. A stream per
+ * planning thread keeps their copies apart. */
 int upload_plan(Ctx* c, Plan* plan, const Host& H, std::string* err) {
   const size_t n = H.total;
   int rc = take_buffer(c, n, &plan->dbuf, &plan->dbytes, err);
   if (rc != AV1H_OK) return rc;
   bind_dev(plan, H);
   Stager* st = nullptr;
+  hipStream_t us = nullptr;
   size_t hint = n;
   {
     std::lock_guard<std::mutex> lk(c->plan_mu);
-    /* a free stager, one large enough if there is one: growing a pinned
-     * buffer (free, then allocate) costs milliseconds and can wait for the
-     * device */
+    /* the smallest free stager that holds the plan, else the largest (grown
+     * below: growing a pinned buffer, free then allocate, costs
+     * milliseconds), else a new one */
     size_t pick = c->stagers.size();
-    for (size_t i = 0; i < c->stagers.size(); ++i)
-      if (!c->stagers[i]->done || hipEventQuery(c->stagers[i]->done) == hipSuccess) {
-        if (pick == c->stagers.size()) pick = i;
-        if (c->stagers[i]->cap >= n) {
-          pick = i;
-          break;
-        }
+    for (size_t i = 0; i < c->stagers.size(); ++i) {
+      const size_t ci = c->stagers[i]->cap;
+      if (pick == c->stagers.size()) {
+        pick = i;
+        continue;
       }
-    /* none large enough: a new one rather than a reallocation, while the
-     * stagers are few */
-    if (pick < c->stagers.size() && (c->stagers[pick]->cap >= n || c->stagers.size() >= 32)) {
+      const size_t cp = c->stagers[pick]->cap;
+      if ((ci >= n && (cp < n || ci < cp)) || (ci < n && cp < n && ci > cp)) pick = i;
+    }
+    if (pick < c->stagers.size()) {
       st = c->stagers[pick];
       c->stagers.erase(c->stagers.begin() + (std::ptrdiff_t)pick);
     }
     c->stage_hint = std::max(c->stage_hint, n);
     hint = c->stage_hint;
-    if (!st && c->stagers.size() >= 64) {
-      st = c->stagers.front();
-      c->stagers.erase(c->stagers.begin());
-    }
-    if (!c->free_events.empty()) {
-      plan->ready = c->free_events.back();
-      c->free_events.pop_back();
+    if (!c->uploads.empty()) {
+      us = c->uploads.back();
+      c->uploads.pop_back();
     }
   }
   hipError_t e = hipSuccess;
-  if (!plan->ready) e = hipEventCreateWithFlags(&plan->ready, hipEventDisableTiming);
-  if (!st) {
-    st = new Stager;
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&st->done, hipEventDisableTiming);
-  } else if (st->done) {
-    e = hipEventSynchronize(st->done);
-  }
+  if (!us) e = hipStreamCreateWithFlags(&us, hipStreamNonBlocking);
+  if (!st) st = new Stager;
   if (e == hipSuccess && st->cap < n) {
     if (st->pin) (void)hipHostFree(st->pin);
     st->pin = nullptr;
@@ -1840,35 +1831,16 @@ int upload_plan(Ctx* c, Plan* plan, const Host& H, std::string* err) {
   }
   if (e == hipSuccess) {
     const size_t up = stage_parts(H, static_cast<uint8_t*>(st->pin), &plan->dev.coef16);
-    e = hipMemcpyAsync(plan->dbuf, st->pin, up, hipMemcpyHostToDevice, c->upload);
-    if (e == hipSuccess) e = hipEventRecord(plan->ready, c->upload);
-    if (e == hipSuccess) e = hipEventRecord(st->done, c->upload);
+    e = hipMemcpyAsync(plan->dbuf, st->pin, up, hipMemcpyHostToDevice, us);
+    if (e == hipSuccess) e = hipStreamSynchronize(us);
   }
-  if (st->done) {
+  {
     std::lock_guard<std::mutex> lk(c->plan_mu);
     c->stagers.push_back(st);
-  } else {
-    if (st->pin) (void)hipHostFree(st->pin);
-    delete st;
+    if (us) c->uploads.push_back(us);
   }
-  if (e != hipSuccess) {
-    if (plan->ready) (void)hipEventSynchronize(plan->ready);
-    return set_err(err, AV1H_ERR_DEVICE, "frame plan upload: %s", hipGetErrorString(e));
-  }
+  if (e != hipSuccess) return set_err(err, AV1H_ERR_DEVICE, "frame plan upload: %s", hipGetErrorString(e));
   return AV1H_OK;
-}
-
-/* waits for the plan's upload and gives its event back */
-void settle_plan(Ctx* c, Plan* p) {
-  if (!p->ready) return;
-  (void)hipEventSynchronize(p->ready);
-  if (c) {
-    std::lock_guard<std::mutex> lk(c->plan_mu);
-    c->free_events.push_back(p->ready);
-  } else {
-    (void)hipEventDestroy(p->ready);
-  }
-  p->ready = nullptr;
 }
 
 void free_plan(Plan* p) {
@@ -2234,10 +2206,8 @@ int batch_launch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
   std::memcpy(pin, jobs.data(), sizeof(FrameJob) * n);
   std::memcpy(pin + o_prefix, prefix.data(), prefix.size() * 4);
   if (n > 1) merge_queues(js, reinterpret_cast<uint32_t*>(pin + o_queue));
-  /* the plans' uploads, and everything queued on the other contexts'
-   * streams (buffer clears), come first */
-  for (JobSpec& j : js)
-    if (j.plan->ready && hipEventQuery(j.plan->ready) != hipSuccess) HIP_OK(lead, hipStreamWaitEvent(s, j.plan->ready, 0));
+  /* everything queued on the other contexts' streams (buffer clears) comes
+   * first; the plans' uploads ended before av1h_plan_frame returned */
   for (JobSpec& j : js)
     if (j.c != lead && j.c->dirty) {
       HIP_OK(lead, hipEventRecord(j.c->ev[3], j.c->stream));
@@ -2642,7 +2612,6 @@ int av1h_create(int device_index, av1h_ctx** out) {
       return AV1H_ERR_DEVICE;
     }
   if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
-      hipStreamCreateWithFlags(&c->upload, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_side_go, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_side_done, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_itx_done, hipEventDisableTiming) != hipSuccess) {
@@ -2682,15 +2651,13 @@ void av1h_destroy(av1h_ctx* h) {
   }
   for (auto& b : c->plan_bufs) (void)hipFree(b.second);
   c->plan_bufs.clear();
-  for (hipEvent_t e : c->free_events) (void)hipEventDestroy(e);
-  c->free_events.clear();
   for (Stager* st : c->stagers) {
-    if (st->done) (void)hipEventSynchronize(st->done);
-    if (st->done) (void)hipEventDestroy(st->done);
     if (st->pin) (void)hipHostFree(st->pin);
     delete st;
   }
   c->stagers.clear();
+  for (hipStream_t u : c->uploads) (void)hipStreamDestroy(u);
+  c->uploads.clear();
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto& e : c->sev)
@@ -2700,7 +2667,6 @@ void av1h_destroy(av1h_ctx* h) {
   if (c->ev_side_done) (void)hipEventDestroy(c->ev_side_done);
   if (c->ev_itx_done) (void)hipEventDestroy(c->ev_itx_done);
   if (c->side) (void)hipStreamDestroy(c->side);
-  if (c->upload) (void)hipStreamDestroy(c->upload);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete h;
 }
@@ -2814,7 +2780,6 @@ int av1h_plan_frame(av1h_ctx* h, const av1h_frame_desc* f, void** plan_out) {
   p->flatten_ms = t1 - t0;
   p->upload_ms = now_ms() - t1;
   if (rc != AV1H_OK) {
-    settle_plan(c, p);
     give_buffer(c, p->dbuf, p->dbytes);
     delete p;
     std::lock_guard<std::mutex> lk(c->mu);
@@ -2849,7 +2814,6 @@ void av1h_discard_plan(av1h_ctx* h, void* plan) {
   if (!plan) return;
   Plan* p = static_cast<Plan*>(plan);
   if (h) (void)hipSetDevice(h->c.device);
-  settle_plan(h ? &h->c : nullptr, p);
   if (h) {
     give_buffer(&h->c, p->dbuf, p->dbytes);
     p->dbuf = nullptr;
@@ -3127,7 +3091,6 @@ bool schedule_order(Plan* p, int order, const float* cost) {
 
 int upload_queue(Ctx* c, Plan* p) {
   if (p->queue_host.empty()) return AV1H_OK;
-  if (p->ready) HIP_OK(c, hipEventSynchronize(p->ready));
   HIP_OK(c, hipMemcpy(const_cast<int32_t*>(p->dev.queue_list), p->queue_host.data(), p->queue_host.size() * 4,
                       hipMemcpyHostToDevice));
   return AV1H_OK;
