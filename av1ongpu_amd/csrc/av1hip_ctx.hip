@@ -2199,11 +2199,12 @@ int batch_launch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
   if (!lead->rb_pin[b]) HIP_OK(lead, hipHostMalloc(&lead->rb_pin[b], 64 + kMaxBatch * 8, hipHostMallocDefault));
   uint32_t* rb_err = static_cast<uint32_t*>(lead->rb_pin[b]);
   uint64_t* rb_hash = reinterpret_cast<uint64_t*>(static_cast<uint8_t*>(lead->rb_pin[b]) + 64);
-  /* [byte count][..][the table at kTableHead]: the jobs and prefixes, a few
-   * KB, fetched by the batch's first launch; a merged queue (up to MBs) is
-   * copied ahead of it, which moves it in larger transfers than the fetch's
-   * system-scope 8 byte loads (90 us per round of the 16-stream mix) */
-  *static_cast<uint64_t*>(lead->batch_pin[b]) = o_queue;
+  /* [byte count][..][the table at kTableHead]: one frame's table, a few KB,
+   * is fetched by the batch's first launch; a larger one (several frames'
+   * jobs, a merged queue of up to MBs) is copied ahead of the batch, which
+   * moves it in larger transfers than the fetch's system-scope 8 byte loads
+   * (in the fetch: 90 us per round of the 16-stream mix) */
+  *static_cast<uint64_t*>(lead->batch_pin[b]) = bytes_tab;
   uint8_t* pin = static_cast<uint8_t*>(lead->batch_pin[b]) + kTableHead;
   std::memcpy(pin, jobs.data(), sizeof(FrameJob) * n);
   std::memcpy(pin + o_prefix, prefix.data(), prefix.size() * 4);
@@ -2217,7 +2218,7 @@ int batch_launch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
       j.c->dirty = false;
     }
   uint8_t* tab = static_cast<uint8_t*>(lead->batch_dev[b]);
-  if (n > 1 && nq > 0) HIP_OK(lead, hipMemcpyAsync(tab + o_queue, pin + o_queue, (size_t)nq * 4, hipMemcpyHostToDevice, s));
+  if (n > 1) HIP_OK(lead, hipMemcpyAsync(tab, pin, bytes_tab, hipMemcpyHostToDevice, s));
   BatchDev B;
   B.jobs = reinterpret_cast<const FrameJob*>(tab);
   B.prefix = reinterpret_cast<const int32_t*>(tab + o_prefix);
@@ -2305,7 +2306,7 @@ int batch_launch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
     e = (call);          \
     if (e != hipSuccess) return e; \
   } while (0)
-    SEQ(launch_fetch_table(lead->batch_pin[b], tab, n, s));
+    if (n == 1) SEQ(launch_fetch_table(lead->batch_pin[b], tab, n, s));
     if (side_itx || side_grain) {
       SEQ(hipEventRecord(lead->ev_side_go, s));
       SEQ(hipStreamWaitEvent(lead->side, lead->ev_side_go, 0));

@@ -1066,8 +1066,9 @@ hipError_t launch_zero64(uint64_t* p, int n, hipStream_t s) {
 }
 
 hipError_t launch_fetch_table(const void* pin, void* dev, int frames, hipStream_t s) {
-  /* one frame's table is a few KB; a merged queue of a large batch some MB */
-  hipLaunchKernelGGL(k_fetch_table, dim3(frames > 1 ? 256 : 1), dim3(256), 0, s, static_cast<const uint64_t*>(pin),
+  /* a table the launch fetches is one frame's, a few KB */
+  (void)frames;
+  hipLaunchKernelGGL(k_fetch_table, dim3(1), dim3(256), 0, s, static_cast<const uint64_t*>(pin),
                      static_cast<uint64_t*>(dev));
   return hipGetLastError();
 }
