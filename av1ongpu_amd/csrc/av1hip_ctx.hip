@@ -19,6 +19,7 @@
 #include <mutex>
 #include <string>
 #include <deque>
+#include <functional>
 #include <memory>
 #include <new>
 #include <type_traits>
@@ -1639,13 +1640,15 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
  * bytes: the staging buffer of its upload), zeros after the pools that are
  * padded. The coefficients, last, go as int16 when every value fits (*coef16
  * set), else as int32. Returns the bytes to upload. */
-size_t stage_parts(const Host& H, uint8_t* dst, int32_t* coef16) {
+size_t stage_parts(const Host& H, uint8_t* dst, int32_t* coef16, const std::function<void(size_t)>& staged) {
   const Host::Part* coef = &H.parts[(size_t)H.part_of[kPartCoef]];
   for (const Host::Part& pt : H.parts) {
     if (&pt == coef) continue;
     if (pt.bytes) std::memcpy(dst + pt.off, pt.src, pt.bytes);
     if (pt.fill) std::memset(dst + pt.off + pt.bytes, 0, pt.fill);
+    if (staged) staged(pt.off + pt.bytes + pt.fill);
   }
+  if (staged) staged(coef->off);
   const size_t n = coef->bytes / 4, pad = coef->fill / 4;
   const int32_t* src = static_cast<const int32_t*>(coef->src);
   int16_t* d16 = reinterpret_cast<int16_t*>(dst + coef->off);
@@ -1719,7 +1722,7 @@ int upload(Ctx* c, Plan* plan, const Host& H) {
     c->host_pin_cap = n;
   }
   HIP_OK(c, hipStreamSynchronize(c->stream));
-  const size_t up = stage_parts(H, static_cast<uint8_t*>(c->host_pin), &plan->dev.coef16);
+  const size_t up = stage_parts(H, static_cast<uint8_t*>(c->host_pin), &plan->dev.coef16, nullptr);
   HIP_OK(c, hipMemcpyAsync(plan->dbuf, c->host_pin, up, hipMemcpyHostToDevice, c->stream));
   c->dirty = true;
   return AV1H_OK;
@@ -1781,6 +1784,8 @@ void give_buffer(Ctx* c, void* p, size_t n) {
  * the decode work that shares that queue (a process gets few) waited behind
  * it: 25 us per 1080p frame with eight threads planning. A stream per
  * planning thread keeps their copies apart. */
+constexpr size_t kUploadChunk = (size_t)2 << 20;
+
 int upload_plan(Ctx* c, Plan* plan, const Host& H, std::string* err) {
   const size_t n = H.total;
   int rc = take_buffer(c, n, &plan->dbuf, &plan->dbytes, err);
@@ -1830,8 +1835,21 @@ int upload_plan(Ctx* c, Plan* plan, const Host& H, std::string* err) {
     if (e == hipSuccess) st->cap = sz;
   }
   if (e == hipSuccess) {
-    const size_t up = stage_parts(H, static_cast<uint8_t*>(st->pin), &plan->dev.coef16);
-    e = hipMemcpyAsync(plan->dbuf, st->pin, up, hipMemcpyHostToDevice, us);
+    /* copied in chunks as they are staged: the copy of the earlier arrays
+     * overlaps the staging of the later ones (the coefficients' int16
+     * conversion last) */
+    uint8_t* pin = static_cast<uint8_t*>(st->pin);
+    uint8_t* dev = static_cast<uint8_t*>(plan->dbuf);
+    size_t issued = 0;
+    auto staged = [&](size_t end) {
+      if (e == hipSuccess && end >= issued + kUploadChunk) {
+        e = hipMemcpyAsync(dev + issued, pin + issued, end - issued, hipMemcpyHostToDevice, us);
+        issued = end;
+      }
+    };
+    const size_t up = stage_parts(H, pin, &plan->dev.coef16, staged);
+    if (e == hipSuccess && up > issued)
+      e = hipMemcpyAsync(dev + issued, pin + issued, up - issued, hipMemcpyHostToDevice, us);
     if (e == hipSuccess) e = hipStreamSynchronize(us);
   }
   {
