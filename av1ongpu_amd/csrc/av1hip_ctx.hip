@@ -1850,7 +1850,10 @@ int upload_plan(Ctx* c, Plan* plan, const Host& H, std::string* err) {
     const size_t up = stage_parts(H, pin, &plan->dev.coef16, staged);
     if (e == hipSuccess && up > issued)
       e = hipMemcpyAsync(dev + issued, pin + issued, up - issued, hipMemcpyHostToDevice, us);
-    if (e == hipSuccess) e = hipStreamSynchronize(us);
+    /* waited for on failure too: the stager and the device buffer go back
+     * to their pools only once no copy reads or writes them */
+    const hipError_t es = hipStreamSynchronize(us);
+    if (e == hipSuccess) e = es;
   }
   {
     std::lock_guard<std::mutex> lk(c->plan_mu);
