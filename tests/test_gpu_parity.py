@@ -74,3 +74,26 @@ def test_filters_without_superres(tools):
     kw = dict(coded_width=0, deblock_lo=8, deblock_hi=32, grain=0)
     kw.update(tools)
     _compare(pkg.Clip("c2_mini", 8, **kw))
+
+
+def test_planned_full_size_frames_match_the_oracle():
+    """Full-size frames (the 1920x1080 10-bit full-tool clip's key frame and
+    the two after it) through av1h_plan_frame, whose upload at this size goes
+    in several chunks copied while the rest is staged, against the oracle:
+    the checksum of every picture."""
+    clip = pkg.Clip("c2", 1)
+    n = 3
+    keep, rel, refs = pkg.slot_schedule(clip)
+    dec = pkg.HipDecoder()
+    dec.set_output_hash(True)
+    got = []
+    for f in range(n):
+        plan = dec.plan_wire(pkg.wire_set_slots(clip.wire(f), refs[f]))
+        dec.decode_planned(plan, keep=keep[f] >= 0, reserved=keep[f])
+        got.append(dec.last_hash())
+        dec.discard_plan(plan)
+        for s in rel[f]:
+            dec.release(s)
+    dec.close()
+    want = [pkg.picture_hash(planes) for _, _, planes in pkg.decode_clip(OracleDecoder(), clip, n)]
+    assert got == want, [f for f in range(n) if got[f] != want[f]]
