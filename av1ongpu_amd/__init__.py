@@ -240,6 +240,7 @@ def hip_lib():
         lib.av1h_get_stage_times.argtypes = [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64),
                                              ctypes.POINTER(ctypes.c_int64)]
         lib.av1h_set_queue_trace.argtypes = [P, ctypes.c_int]
+        lib.av1h_set_wait_limit.argtypes = [P, ctypes.c_int]
         lib.av1h_get_queue_trace.restype = ctypes.c_int64
         lib.av1h_get_queue_trace.argtypes = [P, ctypes.c_void_p, ctypes.c_int64]
         lib.av1h_set_decode_nv12.argtypes = [P, ctypes.c_size_t]
@@ -600,6 +601,11 @@ class HipDecoder:
 
     def set_queue_trace(self, enable=True):
         self._lib.av1h_set_queue_trace(self._h, 1 if enable else 0)
+
+    def set_wait_limit(self, polls):
+        """Polls of a queued block's dependency wait before the decode fails with
+        a time-out (0: the default)."""
+        self._lib.av1h_set_wait_limit(self._h, int(polls))
 
     def queue_trace(self):
         """(n, 4) uint64: taken, ready, published clock, workgroup per intra block."""

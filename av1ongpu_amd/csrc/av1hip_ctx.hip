@@ -181,6 +181,7 @@ struct Ctx {
   uint32_t* counters = nullptr; /* [0] queue counter, [1] error word */
   int32_t* cdef_dir = nullptr;   /* CDEF direction map                */
   bool trace_on = false;         /* queue timestamps (diagnostics)    */
+  int spin_limit = kSpinLimitDefault; /* polls of a dependency wait before a time-out */
   uint64_t* trace = nullptr;
   size_t trace_cap = 0;
   int trace_n = 0;
@@ -2149,6 +2150,7 @@ int batch_launch(Ctx* lead, std::vector<JobSpec>& js, double t0) {
     ra.res = reinterpret_cast<int16_t*>(c->scratch); /* free until the CDEF snapshot */
     ra.maxv = (1 << L.bit_depth) - 1;
     ra.edge_filter = plan->hdr.enable_intra_edge_filter;
+    ra.spin_limit = c->spin_limit;
     FilterArgs& fa = jobs[i].f;
     fa.plan = plan->dev;
     fa.lay = L;
@@ -3368,6 +3370,10 @@ int av1h_get_stage_times(const av1h_ctx* h, double ms[AV1H_STAGES], int64_t coun
 
 void av1h_set_queue_trace(av1h_ctx* h, int enable) {
   if (h) h->c.trace_on = enable != 0;
+}
+
+void av1h_set_wait_limit(av1h_ctx* h, int polls) {
+  if (h) h->c.spin_limit = polls > 0 ? polls : kSpinLimitDefault;
 }
 
 int64_t av1h_get_queue_trace(av1h_ctx* h, uint64_t* out, int64_t cap) {

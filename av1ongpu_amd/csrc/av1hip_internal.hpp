@@ -250,6 +250,10 @@ struct FramePlanDev {
 /* ---- one decode: the frames of a batch ---------------------------------- */
 
 /* What the reconstruction kernels read of one frame. */
+/* Polls a queued block's dependency wait makes before it gives up and the
+ * decode reports a time-out: about a second of polling. */
+constexpr int kSpinLimitDefault = 1 << 22;
+
 struct ReconArgs {
   FramePlanDev plan;
   Layout lay;
@@ -262,6 +266,7 @@ struct ReconArgs {
   int16_t* res;         /* residual of intra blocks, frame layout */
   int maxv;
   int edge_filter;      /* enable_intra_edge_filter           */
+  int spin_limit;       /* polls a dependency wait makes before it reports a time-out */
 };
 
 /* What the filter kernels read of one frame. */

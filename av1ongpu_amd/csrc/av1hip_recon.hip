@@ -36,7 +36,6 @@ namespace {
 #define SPEC(off) (av1t_spec_table + (off))
 
 constexpr int kWork = 2048;            /* int32 intra work area (edges, CFL)    */
-constexpr int kSpinLimit = 1 << 22;    /* dependency wait bound (then an error) */
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 __device__ __forceinline__ int rnd2(int v, int n) { return n <= 0 ? v : (v + (1 << (n - 1))) >> n; }
@@ -832,7 +831,7 @@ constexpr int kPollDepth = 1;
 __device__ __forceinline__ uint64_t ld_word(const uint64_t* q) {
   return __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ uint64_t poll_word(const uint64_t* q, uint32_t epoch, uint32_t* error) {
+__device__ uint64_t poll_word(const uint64_t* q, uint32_t epoch, int limit, uint32_t* error) {
   uint64_t v[kPollDepth];
 #pragma unroll
   for (int k = 0; k < kPollDepth; ++k) {
@@ -846,7 +845,7 @@ __device__ uint64_t poll_word(const uint64_t* q, uint32_t epoch, uint32_t* error
       if (kPollDepth == 1) __builtin_amdgcn_s_sleep(1);
       v[k] = ld_word(q);
     }
-    if (iter >= kSpinLimit) {
+    if (iter >= limit) {
       atomicOr(error, 1u);
       return 0;
     }
@@ -877,14 +876,14 @@ __device__ void gather_packed(RArgs a, BlkRef b, int* above, int* left, int alas
   /* sample i of the above row / left column (block relative) */
   auto above_at = [&](int i) -> int {
     if ((ma >> (i >> sh)) & 1) {
-      const uint64_t v = poll_word(rowp + ((x + i) >> 1), a.epoch, error);
+      const uint64_t v = poll_word(rowp + ((x + i) >> 1), a.epoch, a.spin_limit, error);
       return (int)((v >> (16 * (i & 1))) & 0xffff);
     }
     return fr[i - stride];
   };
   auto left_at = [&](int i) -> int {
     if ((ml >> (i >> sh)) & 1) {
-      const uint64_t v = poll_word(colp + ((y + i) >> 1), a.epoch, error);
+      const uint64_t v = poll_word(colp + ((y + i) >> 1), a.epoch, a.spin_limit, error);
       return (int)((v >> (16 * (i & 1))) & 0xffff);
     }
     return fr[(int64_t)i * stride - 1];
@@ -898,8 +897,8 @@ __device__ void gather_packed(RArgs a, BlkRef b, int* above, int* left, int alas
         if (b.flags & kCornerFrame) {
           v = fr[-stride - 1];
         } else {
-          const uint64_t wv =
-              poll_word((b.flags & kCornerRow) ? rowp + (x >> 1) - 1 : colp + (y >> 1) - 1, a.epoch, error);
+          const uint64_t wv = poll_word((b.flags & kCornerRow) ? rowp + (x >> 1) - 1 : colp + (y >> 1) - 1, a.epoch,
+                                        a.spin_limit, error);
           v = (int)((wv >> 16) & 0xffff); /* sample x - 1 of the row, or y - 1 of the column */
         }
       } else {
@@ -1267,7 +1266,7 @@ __device__ void wait_cells(RArgs a, BlkRef b, uint32_t* error) {
       if (__hip_atomic_load(a.cells + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != a.epoch) ok = 0;
     }
     if (__all(ok)) break;
-    if (iter >= kSpinLimit) {
+    if (iter >= a.spin_limit) {
       if (threadIdx.x == 0) atomicOr(error, 1u);
       break;
     }

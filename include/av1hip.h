@@ -524,6 +524,14 @@ int av1h_get_stage_times(const av1h_ctx* ctx, double ms[AV1H_STAGES], int64_t co
 void av1h_set_queue_trace(av1h_ctx* ctx, int enable);
 int64_t av1h_get_queue_trace(av1h_ctx* ctx, uint64_t* out, int64_t cap);
 
+/* Diagnostics: how many polls a queued block's dependency wait makes before
+ * it gives up, for the frames of this context decoded from now on (0 or
+ * less: the default, 4,194,304, about a second). A wait that gives up fails
+ * the decode with AV1H_ERR_INVALID and the context decodes on. A small limit
+ * forces that path from the host for tests of it. No counterpart in the
+ * reference (its schedule is on the host). */
+void av1h_set_wait_limit(av1h_ctx* ctx, int polls);
+
 /* --------------------------------------------------------------- batching */
 /*
  * Multi-stream batching: frames of several independent streams (each with its

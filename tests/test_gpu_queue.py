@@ -69,3 +69,22 @@ def test_cfl_and_palette_blocks_keep_the_cell_wait():
         for p in plans:
             dec.discard_plan(p)
         dec.close()
+
+
+def test_a_wait_that_gives_up_fails_the_decode_and_the_next_is_clean():
+    """The queue's time-out path, forced from the host with a one-poll wait
+    limit on a key frame (whose blocks wait on one another): the decode fails
+    with the queue's error and gives its slot back; the batch's last launch
+    clears the error word, so with the limit restored the clip decodes bit
+    exact against the oracle on the same context."""
+    from oracle_lib import OracleDecoder
+    clip = pkg.Clip("c2_mini", 1)
+    dec = pkg.HipDecoder()
+    dec.set_wait_limit(1)
+    with pytest.raises(pkg.HipError, match="waited too long"):
+        dec.decode_wire(pkg.wire_set_slots(clip.wire(0), []), keep=True)
+    dec.set_wait_limit(0)
+    got = [pkg.picture_hash(planes) for _, _, planes in pkg.decode_clip(dec, clip)]
+    want = [pkg.picture_hash(planes) for _, _, planes in pkg.decode_clip(OracleDecoder(), clip)]
+    assert got == want
+    dec.close()
