@@ -526,6 +526,7 @@ struct Host {
   bool dev_all = false;             /* ... which are all of them: `blocks` goes */
   std::vector<DevTx> tx;
   std::vector<int32_t> inter_list, queue_list, special_list;
+  std::vector<DevDeblock> s_deblock_h; /* horizontal deblocking edges, before they are appended */
   std::vector<DevObmc> obmc;
   std::vector<int32_t> itx_all;
   std::vector<McItemDev> mc_items;
@@ -1302,14 +1303,18 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
     for (int t = 0; t < (int)H->tx.size(); ++t) H->itx_all[(size_t)cnt[key(H->tx[(size_t)t])]++] = t;
   }
 
-  /* deblocking: vertical edges then horizontal, with a line prefix each */
+  /* deblocking: vertical edges then horizontal, with a line prefix each;
+   * one pass over the caller's edges, the horizontal ones gathered apart
+   * and appended */
   H->deblock.clear();
   H->deblock.reserve((size_t)f.num_deblock);
+  std::vector<DevDeblock>& horiz = H->s_deblock_h;
+  horiz.clear();
   int lines[2] = {0, 0}, nedge[2] = {0, 0};
-  for (int pass = 0; pass < 2; ++pass)
+  {
     for (int64_t i = 0; i < f.num_deblock; ++i) {
       const av1h_deblock_edge& e = f.deblock[i];
-      if ((e.vertical != 0) != (pass == 0)) continue;
+      const int pass = e.vertical != 0 ? 0 : 1;
       if (e.plane < 0 || e.plane > 2 || e.length < 0 || e.level < 0 || e.level > 255 || e.filter_size < 0 ||
           e.filter_size > 255)
         return set_err(err, AV1H_ERR_INVALID, "deblock edge %lld", (long long)i);
@@ -1331,8 +1336,10 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
       d.line0 = lines[pass];
       lines[pass] += e.length;
       ++nedge[pass];
-      H->deblock.push_back(d);
+      (pass ? horiz : H->deblock).push_back(d);
     }
+    H->deblock.insert(H->deblock.end(), horiz.begin(), horiz.end());
+  }
   /* the first edge of every 256 line chunk of each pass */
   H->db_chunk.clear();
   {
