@@ -1024,17 +1024,17 @@ int flatten(const av1h_frame_desc& f, Plan* plan, Host* H, std::string* err) {
           (!kind_ok(row_kind(tx.tx_type), tx.w) || !kind_ok(col_kind(tx.tx_type), tx.h)))
         return set_err(err, AV1H_ERR_UNSUPPORTED, "transform type %d is not defined for %dx%d", tx.tx_type, tx.w, tx.h);
       DevTx dt;
-      dt.x = b.x + tx.x;
-      dt.y = b.y + tx.y;
-      dt.w = tx.w;
-      dt.h = tx.h;
+      dt.x = (uint16_t)(b.x + tx.x);
+      dt.y = (uint16_t)(b.y + tx.y);
+      dt.w = (uint8_t)tx.w;
+      dt.h = (uint8_t)tx.h;
       dt.coef = tx.coef_offset;
-      dt.tx_type = tx.tx_type;
-      dt.row_shift = rs;
-      dt.eob = tx.eob;
-      dt.plane = p;
-      dt.flags = (b.lossless ? kTxLossless : 0) | (b.residual_provided ? kTxRaw : 0) |
-                 ((b.is_intra || b.intrabc || b.interintra) ? kTxToResidual : 0);
+      dt.tx_type = (uint8_t)tx.tx_type;
+      dt.row_shift = (int8_t)rs;
+      dt.eob = (uint16_t)std::min(tx.eob, 65535);
+      dt.plane = (uint8_t)p;
+      dt.flags = (uint8_t)((b.lossless ? kTxLossless : 0) | (b.residual_provided ? kTxRaw : 0) |
+                           ((b.is_intra || b.intrabc || b.interintra) ? kTxToResidual : 0));
       const size_t end = (size_t)tx.coef_offset + (size_t)tx.w * tx.h;
       if (b.residual_provided)
         rend = std::max(rend, end);

@@ -100,15 +100,17 @@ constexpr int kDirStrA = 13, kDirStrL = 16; /* 3 bit edge filter strengths */
 /* flag bits of DevTx::flags */
 enum { kTxLossless = 1, kTxRaw = 2, kTxToResidual = 4 };
 
+/* one transform block in 16 bytes */
 struct DevTx {
-  int32_t x, y, w, h;   /* in the plane                      */
+  uint16_t x, y;        /* in the plane                      */
   int32_t coef;         /* raster offset in the coefficient (or residual) pool */
-  int32_t tx_type;
-  int32_t row_shift;
-  int32_t eob;
-  int32_t plane;
-  int32_t flags;        /* kTx*: lossless (WHT), raw residual, intra block  */
+  uint16_t eob;         /* end of block, at most 65535 (the device reads min(eob, w * h)) */
+  uint8_t w, h;
+  uint8_t tx_type, plane;
+  uint8_t flags;        /* kTx*: lossless (WHT), raw residual, intra block  */
+  int8_t row_shift;
 };
+static_assert(sizeof(DevTx) == 16, "transform record layout");
 
 /* the dequantiser of a transform whose frame carries coded levels (one per
  * DevTx, indexed alike): where its levels start, its quantiser matrix (-1:
